@@ -1,0 +1,184 @@
+#!/usr/bin/env python3
+"""Benchmark of the mutual-interaction step on one MI355X.
+
+Workload: brute-force all-pairs gravity of a thin disk of N = 262,144 bodies
+around a central mass 1 (G = 1; semi-major axes uniform in [0.9, 1.1],
+Rayleigh eccentricity 0.01 and inclination 0.005 rad; masses log-uniform in
+[1e-11, 1e-9], radius 1e-4 (m/1e-10)^(1/3); relative softening 1e-6),
+integrated with the device RK4 driver (4 force evaluations per step,
+dt = 2 pi / 256, hipGraph-captured stages).  Synthetic data, seeded.
+
+A step is one RK4 step.  value = pair-interactions/s of the mixed form,
+counting N(N-1) ordered interactions per force evaluation.  Printed as one
+JSON line; extra fields give the FP64 form, the accuracy of the mixed form
+against it, the roofline of the pair kernel and the reference CPU baseline.
+"""
+import argparse
+import json
+import math
+import os
+import struct
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "pair-interactions/s & RK4 steps/s at N=262k (mixed vs FP64); adjoint/fwd time"
+FLOP_PER_INTERACTION = 20.0        # convention for all-pairs gravity
+PEAK_FP32_VECTOR_TFLOPS = 157.3    # MI355X published FP32 vector peak
+PEAK_FP64_VECTOR_TFLOPS = 78.6
+
+
+def make_workload(n, seed=0):
+    from ncorpion_amd import ic
+    m, R = ic.disk_c3_masses(n, seed)
+    return ic.disk(n, seed=seed, masses=m, radii=R)
+
+
+def cpu_baseline(bodies, n_sample, timeout=300):
+    """NcorpiON's own vector_field (compiled from source, 1 thread) on the
+    first n_sample bodies of the workload, through the function-level probe."""
+    import numpy as np
+    probe = os.path.join(ROOT, "oracle", "_ref", "probe_c3", "ref_probe")
+    if not os.path.exists(probe):
+        return None
+    b = np.ascontiguousarray(bodies[:n_sample], dtype=np.float64)
+    n = b.shape[0]
+    with tempfile.TemporaryDirectory() as td:
+        fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        with open(fin, "wb") as f:
+            f.write(struct.pack("<4i", 0x4E434750, 0, n, 0))
+            f.write(np.zeros(8).tobytes())
+            f.write(np.array([0, 0, 0, 0, 0, 0, 1.0, 0.05]).tobytes())
+            f.write(b.tobytes())
+            f.write(np.ones(n, np.int32).tobytes())
+            f.write(np.zeros(n, np.int32).tobytes())
+        r = subprocess.run([probe, fin, fout], stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                           text=True, timeout=timeout)
+        if r.returncode != 0:
+            return None
+        secs = float(r.stderr.strip().split("op_seconds=")[-1].split()[0])
+    return {"value": n * (n - 1) / secs, "unit": "pair-interactions/s", "cores": 1,
+            "kind": "reference",
+            "sample": "NcorpiON vector_field (brute force, gcc -O2, 1 thread) on the first %d bodies "
+                      "of the workload, one evaluation of %.1f s" % (n, secs)}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=262144)
+    ap.add_argument("--cpu-sample", type=int, default=45000)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-graph", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only (the path is measured at one GPU's "
+              "share); --gpus %d is not supported" % args.gpus, file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+    from ncorpion_amd import Context
+
+    n = args.n
+    dt = 2.0 * math.pi / 256.0
+    soft = 1e-6
+    b = make_workload(n)
+    ctx = Context(0)
+    dev = torch.device("cuda:0")
+    interactions = float(n) * (n - 1)
+
+    def rk4_run(precision, steps, warmup):
+        bt = torch.from_numpy(b).to(dev)
+        ct = torch.tensor([0, 0, 0, 0, 0, 0, 1.0, 1.0], dtype=torch.float64, device=dev)
+        torch.cuda.synchronize()
+        ctx.rk4_steps_dev(bt, ct, warmup, dt, 1.0, soft, precision, graph=not args.no_graph)
+        ctx.synchronize()
+        kname = "pairs_" + precision
+        ctx.profile(True)
+        ctx.profile_read(kname, reset=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ctx.rk4_steps_dev(bt, ct, steps, dt, 1.0, soft, precision, graph=not args.no_graph)
+        ctx.synchronize()
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0
+        kms, launches = ctx.profile_read(kname, reset=True)
+        ctx.profile(False)
+        return t, kms, launches, bt, ct
+
+    t_mix, kms_mix, nl_mix, bt_mix, _ = rk4_run("mixed", args.steps, args.warmup)
+    evals = 4 * args.steps
+    value = interactions * evals / t_mix
+    kavg_ms = kms_mix / max(nl_mix, 1)
+    achieved_tflops = FLOP_PER_INTERACTION * interactions / (kavg_ms * 1e-3) / 1e12
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "pair-interactions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * t_mix / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 positions/accumulation, f32 far-field forces",
+        "data": "synthetic (seeded disk)",
+        "config": {"workload": "brute-force all-pairs gravity, RK4 (4 evaluations/step), disk of "
+                               "N=262144 bodies around a central mass, softening 1e-6, dt=2pi/256",
+                   "n": n, "precision": "mixed", "graph": not args.no_graph,
+                   "parallelism": "single GPU"},
+        "rk4_steps_per_s": args.steps / t_mix,
+        "roofline": {"bound": "valu", "achieved": achieved_tflops,
+                     "peak": PEAK_FP32_VECTOR_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved_tflops / PEAK_FP32_VECTOR_TFLOPS, "traffic": None,
+                     "kernel": "k_pairs<mixed>", "kernel_avg_ms": kavg_ms,
+                     "kernel_launches": nl_mix,
+                     "flop_per_interaction": FLOP_PER_INTERACTION},
+    }
+
+    if not args.no_fp64:
+        fsteps = max(1, args.steps // 4)
+        t64, kms64, nl64, bt64, _ = rk4_run("fast64", fsteps, 1)
+        v64 = interactions * 4 * fsteps / t64
+        k64 = kms64 / max(nl64, 1)
+        out["fp64"] = {"pair_interactions_per_s": v64, "rk4_steps_per_s": fsteps / t64,
+                       "kernel_avg_ms": k64,
+                       "achieved_tflops": FLOP_PER_INTERACTION * interactions / (k64 * 1e-3) / 1e12,
+                       "frac_of_fp64_peak": FLOP_PER_INTERACTION * interactions / (k64 * 1e-3)
+                       / 1e12 / PEAK_FP64_VECTOR_TFLOPS}
+        out["mixed_over_fp64"] = value / v64
+        # Accuracy of one mixed evaluation against the FP64 form, mutual part only.
+        bt = torch.from_numpy(b).to(dev)
+        am = torch.zeros((n, 3), dtype=torch.float64, device=dev)
+        af = torch.zeros((n, 3), dtype=torch.float64, device=dev)
+        ctx.gravity_allpairs_dev(bt, am, None, None, 1.0, soft, "mixed")
+        ctx.gravity_allpairs_dev(bt, af, None, None, 1.0, soft, "fast64")
+        ctx.synchronize()
+        d = torch.linalg.norm(am - af, dim=1)
+        r = torch.linalg.norm(af, dim=1)
+        rel = (d / r).cpu().numpy()
+        out["accuracy"] = {"max_rel_acc_err": float(rel.max()),
+                           "p999_rel_acc_err": float(np.quantile(rel, 0.999)),
+                           "normwise_max_err": float(d.max() / torch.sqrt(torch.mean(r * r))),
+                           "reference": "FP64-fast form (mutual acceleration only)"}
+
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(b, args.cpu_sample)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,228 @@
+/*
+ * ncg.h -- C ABI of the MI355X mutual-interaction library (libncg.so).
+ *
+ * The library replaces NcorpiON's mutual-interaction step on one GPU:
+ * brute-force O(N^2) gravity with pairwise collision detection, and the mesh
+ * module's cell binning with neighbour-cell gravity and collision search.
+ * Reference entry points it stands behind (NcorpiON src/):
+ *
+ *   void vector_field(struct moonlet *)          physics.h:55, physics.c:63-441
+ *   void brute_force(struct moonlet *)           collision.h:70, collision.c:425-456
+ *   void mesh(struct moonlet *)                  collision.h:67, collision.c:324-417
+ *   void get_neighbours_mesh(struct moonlet *)   physics.h:76, physics.c:1094-1118
+ *
+ * The reference functions work on globals; this ABI takes everything as
+ * arguments (plain pointers and sizes).  The host glue in
+ * ncorpion_amd/glue/ncg_dropin.c marshals the reference globals into these
+ * calls.  Collision outcomes stay on the host: the library finds candidates
+ * on the GPU and drives the caller's own closest_approach/collision_treatment
+ * through ncg_replay_ops, in the reference's visiting order.
+ *
+ * Conventions
+ *   - Every function returns ncg_status; NCG_OK == 0.  On failure a message
+ *     is available from ncg_last_error(ctx) and outputs are left untouched.
+ *   - Host-pointer entry points copy inputs to the device and results back;
+ *     the *_dev entry points take device pointers and a hipStream_t passed as
+ *     void* (NULL = the context's own stream) and do not synchronise.
+ *   - A context is not re-entrant: one context per host thread.
+ *   - Indices are int32 like the reference (int); n is largest_id + 1.
+ */
+#ifndef NCG_H
+#define NCG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NCG_ABI_VERSION 1
+
+typedef struct ncg_ctx ncg_ctx;
+
+typedef enum {
+    NCG_OK = 0,
+    NCG_ERR_INVALID = 1,     /* bad argument (null pointer, negative size, ...) */
+    NCG_ERR_DEVICE = 2,      /* a HIP runtime call failed */
+    NCG_ERR_NOMEM = 3,       /* device or host allocation failed */
+    NCG_ERR_UNSUPPORTED = 4, /* a configuration the GPU path does not cover */
+    NCG_ERR_CAPACITY = 5     /* an output buffer is too small; *count says how many */
+} ncg_status;
+
+/* Arithmetic forms of the gravity kernels.
+ *   NCG_F64_EXACT  FP64, the reference's expression tree and summation order:
+ *                  bit-identical to vector_field (physics.c:120-269).
+ *   NCG_F64_FAST   FP64, hardware reciprocal square root refined by Newton
+ *                  steps, FMA, tile order (not the reference's order).
+ *   NCG_MIXED      FP64 positions and tile anchors; far tiles evaluate
+ *                  1/r^3 and pair forces in FP32 with FP32 tile partials
+ *                  added into FP64 accumulators; near tiles take the FP64
+ *                  form.  */
+typedef enum { NCG_F64_EXACT = 0, NCG_F64_FAST = 1, NCG_MIXED = 2 } ncg_precision;
+
+/* Same 64-byte layout as the reference's struct moonlet (structure.h:74-84). */
+typedef struct {
+    double x, y, z;
+    double vx, vy, vz;
+    double mass, radius;
+} ncg_body;
+
+/* The central body of central_mass_bool (CM, structure.c:82; physics.c:77-80). */
+typedef struct {
+    double x, y, z;
+    double mass;
+    int32_t present;
+    int32_t pad_;
+} ncg_central;
+
+/* Same layout as the reference's struct pair (structure.h:138-143). */
+typedef struct {
+    int32_t fst, snd;
+} ncg_pair;
+
+/* Host callbacks that make the collision decisions and outcomes, in the
+ * reference's order.  `test` must return nonzero exactly when the reference
+ * would call collision_treatment for the pair at the current host state
+ * (closest_approach(a, b) != NULL and neither did_collide flag set,
+ * collision.c:436-437 / 351-352).  `resolve` applies the outcome
+ * (collision_treatment, physics.c:1043).  Both are called with the argument
+ * order of the reference: (i, j) for brute_force, (p, k) for mesh. */
+typedef struct {
+    void *user;
+    int (*test)(void *user, int32_t a, int32_t b);
+    void (*resolve)(void *user, int32_t a, int32_t b);
+} ncg_replay_ops;
+
+/* Per-pass counters of the mesh module (structure.h:130-137). */
+typedef struct {
+    int64_t total_neighbours;
+    int64_t how_many_small;
+    int64_t how_many_big;
+    int64_t collisions;
+    int64_t candidates;     /* GPU candidate pairs handed to the host */
+    int64_t host_tests;     /* pairs the host re-tested after a collision */
+} ncg_mesh_stats;
+
+int ncg_abi_version(void);
+
+/* Context: one GPU (device ordinal), its stream and grow-on-demand buffers. */
+ncg_status ncg_create(ncg_ctx **ctx, int device);
+void ncg_destroy(ncg_ctx *ctx);
+const char *ncg_last_error(const ncg_ctx *ctx);
+ncg_status ncg_synchronize(ncg_ctx *ctx);
+/* Device stream of the context, as hipStream_t. */
+void *ncg_stream(ncg_ctx *ctx);
+
+/* ---------------------------------------------------------------------------
+ * Brute-force gravity: replaces the brute-force and central branches of
+ * vector_field (physics.c:63-82, 120-202, 263-266).
+ *   acc[3k..3k+2] receives the acceleration of every body with exists[k]
+ *   (what the reference writes into .vx/.vy/.vz); entries of holes are left
+ *   untouched.  cm_acc (may be NULL) receives CM_acc (physics.c:80, 140-142),
+ *   the central body's acceleration, summed in the reference's order for
+ *   NCG_F64_EXACT.  exists may be NULL (all present).
+ */
+ncg_status ncg_gravity_allpairs(ncg_ctx *ctx, const ncg_body *bodies, int32_t n,
+                                const int32_t *exists, const ncg_central *central,
+                                double grav_const, double softening, ncg_precision prec,
+                                double *acc, double *cm_acc);
+
+/* Same on device buffers: bodies (n records), exists (n or NULL), acc (3n),
+ * cm_acc (3 doubles or NULL). */
+ncg_status ncg_gravity_allpairs_dev(ncg_ctx *ctx, const ncg_body *d_bodies, int32_t n,
+                                    const int32_t *d_exists, const ncg_central *central,
+                                    double grav_const, double softening, ncg_precision prec,
+                                    double *d_acc, double *d_cm_acc, void *stream);
+
+/* ---------------------------------------------------------------------------
+ * Brute-force collision search: replaces brute_force (collision.c:425-456).
+ * The GPU tests every pair i<j of existing bodies at the given state with the
+ * closest_approach predicate (collision.c:63-172) and hands the candidates to
+ * the host, which replays the reference's lexicographic (i, j) order through
+ * ops: candidate pairs are confirmed with ops->test and resolved with
+ * ops->resolve; after a collision every later pair that involves a modified
+ * body is re-tested on the host through ops->test.  The GPU reads `bodies`
+ * once, at the start of the pass; the callbacks see and mutate the caller's
+ * live array.
+ * *n_collisions (may be NULL) receives the number of resolved collisions.
+ */
+ncg_status ncg_brute_force_pass(ncg_ctx *ctx, const ncg_body *bodies, int32_t n,
+                                const int32_t *exists, double timestep,
+                                const ncg_replay_ops *ops, int64_t *n_collisions);
+
+/* Candidate pairs only (the GPU half of ncg_brute_force_pass), sorted
+ * lexicographically.  Returns NCG_ERR_CAPACITY when more than `capacity`
+ * exist, with *count set to the number found. */
+ncg_status ncg_collide_candidates_allpairs(ncg_ctx *ctx, const ncg_body *bodies, int32_t n,
+                                           const int32_t *exists, double timestep,
+                                           ncg_pair *out, int64_t capacity, int64_t *count);
+
+/* ---------------------------------------------------------------------------
+ * Mesh module.
+ *
+ * ncg_mesh_pass replaces mesh() (mode 0, collision.c:324-417) and
+ * get_neighbours_mesh() (mode 1, physics.c:1094-1118).  It bins the bodies in
+ * the reference's hash-table cells (collision.c:175-261: cell side gam,
+ * collision_cube_cells cells per axis plus one guard layer, key i + j*d + k*d^2)
+ * by a counting sort on the GPU, counts total_neighbours exactly, finds the
+ * collision candidates among neighbour pairs (mode 0) and replays them on the
+ * host through ops in the reference's visiting order (bodies by ascending id;
+ * for each, neighbour cells 26..0 and ascending id within a cell).  Bodies
+ * with radius > gam (mode 0) are visited on the host against every body, as
+ * in collision.c:381-414.  The binning is kept in the context for the next
+ * ncg_mesh_gravity call, which evaluates the pairs registered by the pass at
+ * the positions it is given (physics.c:296-378).
+ *   ops may be NULL in mode 1.
+ */
+ncg_status ncg_mesh_pass(ncg_ctx *ctx, const ncg_body *bodies, int32_t n,
+                         const int32_t *exists, double timestep, double gam,
+                         double collision_cube, int32_t cells, int32_t mode,
+                         const ncg_replay_ops *ops, ncg_mesh_stats *stats);
+
+/* Mesh branch of vector_field (physics.c:63-109, 120-146, 204-261, 296-380):
+ * central term, bodies 0..2 against every body, and the neighbour pairs of
+ * the last ncg_mesh_pass (bodies > 2 only).  Outputs as ncg_gravity_allpairs.
+ * NCG_F64_EXACT reproduces the reference's pair-list summation order. */
+ncg_status ncg_mesh_gravity(ncg_ctx *ctx, const ncg_body *bodies, int32_t n,
+                            const int32_t *exists, const ncg_central *central, double grav_const,
+                            double softening, ncg_precision prec, double *acc,
+                            double *cm_acc);
+
+/* Neighbour pairs of the last ncg_mesh_pass in the reference's registration
+ * order, (k, p) with k the visited body (what mesh()/get_neighbours_mesh()
+ * append to `pairs`, collision.c:353-358, physics.c:1108-1111).  Returns
+ * NCG_ERR_CAPACITY with *count when the buffer is too small. */
+ncg_status ncg_mesh_pairs(ncg_ctx *ctx, ncg_pair *out, int64_t capacity, int64_t *count);
+
+/* ---------------------------------------------------------------------------
+ * Classic RK4 on device buffers (NcorpiON integrates with a leapfrog,
+ * rk4.c:68-127; this driver is an addition).  Advances d_bodies (n records,
+ * x/v updated in place, mass and radius kept) and the central body
+ * d_central = {x, y, z, vx, vy, vz, M, present} by nsteps steps of dt; the
+ * central body moves under CM_acc as kick() applies it (rk4.c:103-105).
+ * prec: NCG_F64_FAST or NCG_MIXED.  flags bit 0: capture the four stages of
+ * a step in a hipGraph once and replay it every step (the spatial sort is
+ * refreshed eagerly before each step).  Asynchronous on `stream`.
+ */
+ncg_status ncg_rk4_steps_dev(ncg_ctx *ctx, ncg_body *d_bodies, int32_t n, const int32_t *d_exists,
+                             double *d_central, double grav_const, double softening,
+                             ncg_precision prec, double dt, int32_t nsteps, int32_t flags,
+                             void *stream);
+
+/* ---------------------------------------------------------------------------
+ * Kernel timing with HIP events recorded on the stream each kernel is
+ * launched on.  Names: "exact", "pairs_fast64", "pairs_mixed",
+ * "candidates", "mesh_bin", "mesh_count", "mesh_gravity".
+ * ncg_profile_read waits for the recorded events and returns the summed
+ * duration and launch count (reset != 0 clears them).
+ */
+ncg_status ncg_profile_enable(ncg_ctx *ctx, int enable);
+ncg_status ncg_profile_read(ncg_ctx *ctx, const char *kernel, double *total_ms, int64_t *launches,
+                            int reset);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NCG_H */

@@ -1,0 +1,277 @@
+"""ctypes binding of libncg.so (the C ABI declared in include/ncg.h).
+
+The library is the product; this module only marshals numpy arrays and torch
+device tensors into it.  Loading fails loudly when libncg.so has not been
+built: there is no fallback implementation.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libncg.so")
+
+NCG_OK, NCG_ERR_INVALID, NCG_ERR_DEVICE, NCG_ERR_NOMEM, NCG_ERR_UNSUPPORTED, NCG_ERR_CAPACITY = range(6)
+PRECISIONS = {"exact": 0, "fast64": 1, "mixed": 2}
+
+
+class Body(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in ("x", "y", "z", "vx", "vy", "vz", "mass", "radius")]
+
+
+class Central(ctypes.Structure):
+    _fields_ = [("x", ctypes.c_double), ("y", ctypes.c_double), ("z", ctypes.c_double),
+                ("mass", ctypes.c_double), ("present", ctypes.c_int32), ("pad_", ctypes.c_int32)]
+
+
+class Pair(ctypes.Structure):
+    _fields_ = [("fst", ctypes.c_int32), ("snd", ctypes.c_int32)]
+
+
+TEST_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32)
+RESOLVE_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32)
+
+
+class ReplayOps(ctypes.Structure):
+    _fields_ = [("user", ctypes.c_void_p), ("test", TEST_FN), ("resolve", RESOLVE_FN)]
+
+
+class MeshStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int64) for n in ("total_neighbours", "how_many_small", "how_many_big",
+                                             "collisions", "candidates", "host_tests")]
+
+
+_lib = None
+
+P = ctypes.c_void_p
+I32 = ctypes.c_int32
+I64 = ctypes.c_int64
+D = ctypes.c_double
+S = ctypes.c_int
+
+_SIGNATURES = {
+    "ncg_abi_version": (S, []),
+    "ncg_create": (S, [ctypes.POINTER(P), S]),
+    "ncg_destroy": (None, [P]),
+    "ncg_last_error": (ctypes.c_char_p, [P]),
+    "ncg_synchronize": (S, [P]),
+    "ncg_stream": (P, [P]),
+    "ncg_gravity_allpairs": (S, [P, P, I32, P, ctypes.POINTER(Central), D, D, S, P, P]),
+    "ncg_gravity_allpairs_dev": (S, [P, P, I32, P, ctypes.POINTER(Central), D, D, S, P, P, P]),
+    "ncg_brute_force_pass": (S, [P, P, I32, P, D, ctypes.POINTER(ReplayOps), ctypes.POINTER(I64)]),
+    "ncg_collide_candidates_allpairs": (S, [P, P, I32, P, D, P, I64, ctypes.POINTER(I64)]),
+    "ncg_mesh_pass": (S, [P, P, I32, P, D, D, D, I32, I32, ctypes.POINTER(ReplayOps),
+                          ctypes.POINTER(MeshStats)]),
+    "ncg_mesh_gravity": (S, [P, P, I32, P, ctypes.POINTER(Central), D, D, S, P, P]),
+    "ncg_mesh_pairs": (S, [P, P, I64, ctypes.POINTER(I64)]),
+    "ncg_rk4_steps_dev": (S, [P, P, I32, P, P, D, D, S, D, I32, I32, P]),
+    "ncg_profile_enable": (S, [P, S]),
+    "ncg_profile_read": (S, [P, ctypes.c_char_p, ctypes.POINTER(D), ctypes.POINTER(I64), S]),
+}
+
+
+def load():
+    """Load libncg.so and declare the C signatures (raises if it is missing)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libncg.so is not built (%s); run __graft_entry__.build()" % LIB_PATH)
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in _SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+class NcgError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__("ncg status %d: %s" % (status, msg))
+        self.status = status
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def as_bodies(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if a.ndim != 2 or a.shape[1] != 8:
+        raise ValueError("bodies must have shape (n, 8)")
+    return a
+
+
+def as_exists(e, n):
+    if e is None:
+        return None
+    e = np.ascontiguousarray(e, dtype=np.int32)
+    if e.shape != (n,):
+        raise ValueError("exists must have shape (n,)")
+    return e
+
+
+def make_central(central):
+    """central: None, or (x, y, z, M)."""
+    c = Central()
+    if central is not None:
+        c.x, c.y, c.z, c.mass = [float(v) for v in central]
+        c.present = 1
+    return c
+
+
+class Context:
+    """One GPU context of libncg (ncg_create / ncg_destroy)."""
+
+    def __init__(self, device=0):
+        self.lib = load()
+        h = ctypes.c_void_p()
+        st = self.lib.ncg_create(ctypes.byref(h), int(device))
+        if st != NCG_OK:
+            raise NcgError(st, "ncg_create(device=%d) failed" % device)
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.ncg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st):
+        if st != NCG_OK:
+            raise NcgError(st, self.lib.ncg_last_error(self.h).decode(errors="replace"))
+
+    @property
+    def stream(self):
+        return self.lib.ncg_stream(self.h)
+
+    def synchronize(self):
+        self._check(self.lib.ncg_synchronize(self.h))
+
+    # -- brute-force gravity (vector_field) ------------------------------------
+    def gravity_allpairs(self, bodies, exists=None, central=None, G=1.0, softening=0.0,
+                         precision="exact", acc=None):
+        b = as_bodies(bodies)
+        n = b.shape[0]
+        e = as_exists(exists, n)
+        out = np.zeros((n, 3)) if acc is None else acc
+        cm = np.zeros(3)
+        c = make_central(central)
+        self._check(self.lib.ncg_gravity_allpairs(self.h, _ptr(b), n, _ptr(e), ctypes.byref(c),
+                                                  float(G), float(softening),
+                                                  PRECISIONS[precision], _ptr(out), _ptr(cm)))
+        return out, cm
+
+    def gravity_allpairs_dev(self, bodies_t, acc_t, exists_t=None, central=None, G=1.0,
+                             softening=0.0, precision="mixed", cm_t=None, stream=None):
+        """bodies_t: (n, 8) float64 device tensor; acc_t: (n, 3) float64 device tensor."""
+        n = int(bodies_t.shape[0])
+        c = make_central(central)
+        self._check(self.lib.ncg_gravity_allpairs_dev(
+            self.h, bodies_t.data_ptr(), n, None if exists_t is None else exists_t.data_ptr(),
+            ctypes.byref(c), float(G), float(softening), PRECISIONS[precision], acc_t.data_ptr(),
+            None if cm_t is None else cm_t.data_ptr(), stream))
+
+    def rk4_steps_dev(self, bodies_t, central_t, nsteps, dt, G=1.0, softening=0.0,
+                      precision="mixed", exists_t=None, graph=True, stream=None):
+        """Advance bodies_t ((n, 8) float64, device) and central_t (8 float64,
+        device: x, y, z, vx, vy, vz, M, present) by nsteps RK4 steps."""
+        n = int(bodies_t.shape[0])
+        self._check(self.lib.ncg_rk4_steps_dev(
+            self.h, bodies_t.data_ptr(), n, None if exists_t is None else exists_t.data_ptr(),
+            central_t.data_ptr(), float(G), float(softening), PRECISIONS[precision], float(dt),
+            int(nsteps), 1 if graph else 0, stream))
+
+    def profile(self, enable=True):
+        self._check(self.lib.ncg_profile_enable(self.h, 1 if enable else 0))
+
+    def profile_read(self, kernel, reset=True):
+        ms = D()
+        cnt = I64()
+        self._check(self.lib.ncg_profile_read(self.h, kernel.encode(), ctypes.byref(ms),
+                                              ctypes.byref(cnt), 1 if reset else 0))
+        return float(ms.value), int(cnt.value)
+
+    # -- brute-force collisions (brute_force) ----------------------------------
+    def collide_candidates(self, bodies, exists=None, dt=0.0):
+        b = as_bodies(bodies)
+        n = b.shape[0]
+        e = as_exists(exists, n)
+        cap = 1 << 16
+        while True:
+            out = np.zeros((cap, 2), dtype=np.int32)
+            cnt = I64()
+            st = self.lib.ncg_collide_candidates_allpairs(self.h, _ptr(b), n, _ptr(e), float(dt),
+                                                          _ptr(out), cap, ctypes.byref(cnt))
+            if st == NCG_ERR_CAPACITY:
+                cap = int(cnt.value)
+                continue
+            self._check(st)
+            return out[: cnt.value].copy()
+
+    def brute_force_pass(self, bodies, exists, dt, test, resolve):
+        """Replay of brute_force with Python callbacks test(a, b) -> bool and
+        resolve(a, b).  `bodies` must be the array the callbacks mutate."""
+        b = bodies
+        if not (isinstance(b, np.ndarray) and b.dtype == np.float64 and b.flags.c_contiguous):
+            raise ValueError("bodies must be a C-contiguous float64 array (mutated in place)")
+        n = b.shape[0]
+        e = as_exists(exists, n)
+        tf = TEST_FN(lambda u, a, c: 1 if test(int(a), int(c)) else 0)
+        rf = RESOLVE_FN(lambda u, a, c: resolve(int(a), int(c)))
+        ops = ReplayOps(None, tf, rf)
+        nc = I64()
+        self._check(self.lib.ncg_brute_force_pass(self.h, _ptr(b), n, _ptr(e), float(dt),
+                                                  ctypes.byref(ops), ctypes.byref(nc)))
+        return int(nc.value)
+
+    # -- mesh module -------------------------------------------------------------
+    def mesh_pass(self, bodies, exists, dt, gam, cube, cells, mode, test=None, resolve=None):
+        b = bodies
+        if not (isinstance(b, np.ndarray) and b.dtype == np.float64 and b.flags.c_contiguous):
+            raise ValueError("bodies must be a C-contiguous float64 array (mutated in place)")
+        n = b.shape[0]
+        e = as_exists(exists, n)
+        stats = MeshStats()
+        ops_p = None
+        if test is not None:
+            tf = TEST_FN(lambda u, a, c: 1 if test(int(a), int(c)) else 0)
+            rf = RESOLVE_FN(lambda u, a, c: resolve(int(a), int(c)))
+            ops = ReplayOps(None, tf, rf)
+            ops_p = ctypes.byref(ops)
+        self._check(self.lib.ncg_mesh_pass(self.h, _ptr(b), n, _ptr(e), float(dt), float(gam),
+                                           float(cube), int(cells), int(mode), ops_p,
+                                           ctypes.byref(stats)))
+        return {f: int(getattr(stats, f)) for f, _ in MeshStats._fields_}
+
+    def mesh_gravity(self, bodies, exists=None, central=None, G=1.0, softening=0.0,
+                     precision="exact"):
+        b = as_bodies(bodies)
+        n = b.shape[0]
+        e = as_exists(exists, n)
+        out = np.zeros((n, 3))
+        cm = np.zeros(3)
+        c = make_central(central)
+        self._check(self.lib.ncg_mesh_gravity(self.h, _ptr(b), n, _ptr(e), ctypes.byref(c),
+                                              float(G), float(softening), PRECISIONS[precision],
+                                              _ptr(out), _ptr(cm)))
+        return out, cm
+
+    def mesh_pairs(self):
+        cap = 1 << 20
+        while True:
+            out = np.zeros((cap, 2), dtype=np.int32)
+            cnt = I64()
+            st = self.lib.ncg_mesh_pairs(self.h, _ptr(out), cap, ctypes.byref(cnt))
+            if st == NCG_ERR_CAPACITY:
+                cap = int(cnt.value)
+                continue
+            self._check(st)
+            return out[: cnt.value].copy()

@@ -1,0 +1,688 @@
+// All-pairs gravity, FP64-fast and mixed FP32/FP64 forms.
+//
+// Computes the same field as the brute-force branch of NcorpiON's
+// vector_field (physics.c:120-202): a_i = sum_{j != i} G m_j (x_j - x_i) / D^3
+// with D^2 = |x_j - x_i|^2 + (s (R_i + R_j))^2, plus the central term
+// (physics.c:127-146), but in tile order on the vector ALUs.
+//
+// Pipeline of one evaluation (all on one stream, no host synchronisation):
+//   1. k_bbox_*      bounding box, largest radius and count of existing bodies
+//   2. k_morton      30-bit Morton key per body (holes sort last)
+//   3. radix sort    (key, index) pairs -> spatially coherent order
+//   4. k_gather      sorted FP64 records {x, y, z, G m} and s R
+//   5. k_tiles       per tile of TJ bodies: FP64 anchor (box centre), FP32
+//                    half extent, FP32 offsets from the anchor and FP32 G m;
+//                    the same for i-blocks of 256*IPL bodies
+//   6. k_pairs<MODE> one workgroup per (i-block, j-split); every lane owns IPL
+//                    bodies and walks the j tiles of its split.  Tile data is
+//                    read with wave-uniform addresses (scalar loads).  For each
+//                    (i-block, tile) pair a uniform test picks the form:
+//                      near: FP64 separations, FP64 rsqrt refined by one Newton
+//                            step, FP64 accumulation (also every tile in
+//                            NCG_F64_FAST);
+//                      far:  separations C + u_j - u_i from the FP64 anchor
+//                            difference C and FP32 offsets, FP32 1/r^3 and pair
+//                            force, FP32 tile partial added into FP64.
+//                    A tile pair is far when its box distance is at least
+//                    theta times (anchor distance + half extents), which bounds
+//                    the FP32 separation error, and when the largest softening
+//                    length is below FP32 resolution of that distance, so the
+//                    far form can omit it without changing an FP32 result.
+//   7. k_finish      sums the j-split partials in fixed order, adds the FP64
+//                    central term and scatters back to body order.
+#include <hipcub/hipcub.hpp>
+
+#include "ncg_internal.hpp"
+
+namespace {
+
+constexpr int TJ = 128;     // bodies per j tile
+constexpr int NT = 256;     // threads per workgroup
+constexpr double TINY2 = 1e-200;  // floor of r^2 in the FP64 form (self pairs give 0 * finite)
+
+struct TileHdr {
+    double ax, ay, az;  // anchor (box centre)
+    float hx, hy, hz;   // half extents, rounded up
+    int count;          // valid bodies in the tile
+};
+
+struct Scal {           // per-evaluation scalars produced on the device
+    double bmin[3], bmax[3];
+    double rmax;
+    int nvalid;
+    int pad;
+};
+
+__device__ inline unsigned expand10(unsigned v)
+{
+    v &= 0x3ffu;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+template <class T>
+__device__ inline T block_reduce(T v, T *sh, int op /*0 min 1 max 2 sum*/)
+{
+    const int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (t < w) {
+            T a = sh[t], c = sh[t + w];
+            sh[t] = op == 0 ? (a < c ? a : c) : op == 1 ? (a > c ? a : c) : a + c;
+        }
+        __syncthreads();
+    }
+    T r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+// ---- 1. bounding box / count ---------------------------------------------
+__global__ __launch_bounds__(NT) void k_bbox_partial(const DBody *__restrict__ b, int n,
+                                                     const int32_t *__restrict__ ex,
+                                                     double *__restrict__ part)
+{
+    __shared__ double sh[NT];
+    double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300}, rm = 0.0, cnt = 0.0;
+    for (int k = blockIdx.x * NT + threadIdx.x; k < n; k += gridDim.x * NT) {
+        if (ex && !ex[k]) continue;
+        const double p[3] = {b[k].x, b[k].y, b[k].z};
+        for (int c = 0; c < 3; ++c) {
+            mn[c] = fmin(mn[c], p[c]);
+            mx[c] = fmax(mx[c], p[c]);
+        }
+        rm = fmax(rm, b[k].radius);
+        cnt += 1.0;
+    }
+    double *o = part + 8 * blockIdx.x;
+    for (int c = 0; c < 3; ++c) {
+        double v = block_reduce(mn[c], sh, 0);
+        if (threadIdx.x == 0) o[c] = v;
+        v = block_reduce(mx[c], sh, 1);
+        if (threadIdx.x == 0) o[3 + c] = v;
+    }
+    double v = block_reduce(rm, sh, 1);
+    if (threadIdx.x == 0) o[6] = v;
+    v = block_reduce(cnt, sh, 2);
+    if (threadIdx.x == 0) o[7] = v;
+}
+
+__global__ __launch_bounds__(NT) void k_bbox_final(const double *__restrict__ part, int nparts,
+                                                   Scal *__restrict__ sc)
+{
+    __shared__ double sh[NT];
+    double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300}, rm = 0.0, cnt = 0.0;
+    for (int q = threadIdx.x; q < nparts; q += NT) {
+        for (int c = 0; c < 3; ++c) {
+            mn[c] = fmin(mn[c], part[8 * q + c]);
+            mx[c] = fmax(mx[c], part[8 * q + 3 + c]);
+        }
+        rm = fmax(rm, part[8 * q + 6]);
+        cnt += part[8 * q + 7];
+    }
+    for (int c = 0; c < 3; ++c) {
+        double v = block_reduce(mn[c], sh, 0);
+        if (threadIdx.x == 0) sc->bmin[c] = v;
+        v = block_reduce(mx[c], sh, 1);
+        if (threadIdx.x == 0) sc->bmax[c] = v;
+    }
+    double v = block_reduce(rm, sh, 1);
+    if (threadIdx.x == 0) sc->rmax = v;
+    v = block_reduce(cnt, sh, 2);
+    if (threadIdx.x == 0) sc->nvalid = (int)v;
+}
+
+// ---- 2. Morton keys -------------------------------------------------------
+__global__ __launch_bounds__(NT) void k_morton(const DBody *__restrict__ b, int n,
+                                               const int32_t *__restrict__ ex,
+                                               const Scal *__restrict__ sc,
+                                               unsigned *__restrict__ key, int *__restrict__ idx)
+{
+    const int k = blockIdx.x * NT + threadIdx.x;
+    if (k >= n) return;
+    idx[k] = k;
+    if (ex && !ex[k]) {
+        key[k] = 0xffffffffu;
+        return;
+    }
+    unsigned q[3];
+    const double p[3] = {b[k].x, b[k].y, b[k].z};
+    for (int c = 0; c < 3; ++c) {
+        const double w = sc->bmax[c] - sc->bmin[c];
+        double f = w > 0.0 ? (p[c] - sc->bmin[c]) / w : 0.0;
+        int v = (int)(f * 1023.999);
+        v = v < 0 ? 0 : (v > 1023 ? 1023 : v);
+        q[c] = (unsigned)v;
+    }
+    key[k] = (expand10(q[0]) << 2) | (expand10(q[1]) << 1) | expand10(q[2]);
+}
+
+// ---- 4. sorted records ----------------------------------------------------
+__global__ __launch_bounds__(NT) void k_gather(const DBody *__restrict__ b, int n,
+                                               const int *__restrict__ perm,
+                                               const Scal *__restrict__ sc, double G, double soft,
+                                               double4 *__restrict__ jd, double *__restrict__ jsig)
+{
+    const int s = blockIdx.x * NT + threadIdx.x;
+    if (s >= n) return;
+    const int nv = sc->nvalid;
+    if (nv <= 0) return;
+    // Slots past the last valid body repeat it with zero mass, so tile boxes
+    // are unchanged and their terms vanish.
+    const int k = perm[s < nv ? s : nv - 1];
+    const DBody q = b[k];
+    jd[s] = make_double4(q.x, q.y, q.z, s < nv ? G * q.mass : 0.0);
+    jsig[s] = soft * q.radius;
+}
+
+// ---- 5. tile anchors and FP32 offsets -------------------------------------
+__global__ __launch_bounds__(NT) void k_tiles(const double4 *__restrict__ jd,
+                                              const Scal *__restrict__ sc, int tsize,
+                                              TileHdr *__restrict__ hdr,
+                                              float4 *__restrict__ ju /* null: header only */)
+{
+    __shared__ double sh[NT];
+    const int nv = sc->nvalid;
+    const int t = blockIdx.x;
+    const int b0 = t * tsize;
+    if (b0 >= nv) return;
+    const int cnt = min(tsize, nv - b0);
+    double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
+    for (int q = threadIdx.x; q < cnt; q += NT) {
+        const double4 p = jd[b0 + q];
+        mn[0] = fmin(mn[0], p.x);
+        mn[1] = fmin(mn[1], p.y);
+        mn[2] = fmin(mn[2], p.z);
+        mx[0] = fmax(mx[0], p.x);
+        mx[1] = fmax(mx[1], p.y);
+        mx[2] = fmax(mx[2], p.z);
+    }
+    double A[3];
+    float H[3];
+    for (int c = 0; c < 3; ++c) {
+        const double lo = block_reduce(mn[c], sh, 0);
+        const double hi = block_reduce(mx[c], sh, 1);
+        A[c] = 0.5 * (lo + hi);
+        const double h = fmax(hi - A[c], A[c] - lo);
+        float hf = (float)h;
+        if ((double)hf < h) hf = nextafterf(hf, 3.0e38f);
+        H[c] = hf;
+    }
+    if (threadIdx.x == 0) {
+        TileHdr th;
+        th.ax = A[0];
+        th.ay = A[1];
+        th.az = A[2];
+        th.hx = H[0];
+        th.hy = H[1];
+        th.hz = H[2];
+        th.count = cnt;
+        hdr[t] = th;
+    }
+    if (ju) {
+        for (int q = threadIdx.x; q < tsize; q += NT) {
+            const int s = b0 + q;
+            if (q < cnt) {
+                const double4 p = jd[s];
+                ju[s] = make_float4((float)(p.x - A[0]), (float)(p.y - A[1]), (float)(p.z - A[2]),
+                                    (float)p.w);
+            } else {
+                ju[s] = make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+        }
+    }
+}
+
+// ---- 6. pair kernel -------------------------------------------------------
+// MODE 0: FP64-fast on every tile; MODE 1: mixed (near FP64, far FP32).
+template <int MODE, int IPL>
+__global__ __launch_bounds__(NT) void k_pairs(const double4 *__restrict__ jd,
+                                              const double *__restrict__ jsig,
+                                              const float4 *__restrict__ ju,
+                                              const TileHdr *__restrict__ thdr,
+                                              const TileHdr *__restrict__ ihdr,
+                                              const Scal *__restrict__ sc, int nsplit,
+                                              double theta, double soft, int nslots,
+                                              double *__restrict__ part)
+{
+    const int nv = sc->nvalid;
+    const int iblk = blockIdx.x;
+    const int i0 = iblk * NT * IPL;
+    if (i0 >= nv) return;
+    const int ntiles = (nv + TJ - 1) / TJ;
+    const int per = (ntiles + nsplit - 1) / nsplit;
+    const int t0 = blockIdx.y * per;
+    const int t1 = min(ntiles, t0 + per);
+
+    const TileHdr ih = ihdr[iblk];
+    double xi[IPL], yi[IPL], zi[IPL], si[IPL];
+    float ux[IPL], uy[IPL], uz[IPL];
+    double ax[IPL], ay[IPL], az[IPL];
+#pragma unroll
+    for (int q = 0; q < IPL; ++q) {
+        const int i = min(i0 + q * NT + (int)threadIdx.x, nv - 1);
+        const double4 p = jd[i];
+        xi[q] = p.x;
+        yi[q] = p.y;
+        zi[q] = p.z;
+        si[q] = jsig[i];
+        ux[q] = (float)(p.x - ih.ax);
+        uy[q] = (float)(p.y - ih.ay);
+        uz[q] = (float)(p.z - ih.az);
+        ax[q] = ay[q] = az[q] = 0.0;
+    }
+    // Largest softening length squared and the FP32 resolution test.
+    const double eps = 2.0 * soft * sc->rmax;
+    const double eps2x = eps * eps * 67108864.0;  // 2^26
+    const double hi2 = (double)ih.hx * ih.hx + (double)ih.hy * ih.hy + (double)ih.hz * ih.hz;
+
+    for (int t = t0; t < t1; ++t) {
+        const TileHdr th = thdr[t];
+        const int jb = t * TJ;
+        const int cnt = th.count;
+        bool near = true;
+        double Cx = th.ax - ih.ax, Cy = th.ay - ih.ay, Cz = th.az - ih.az;
+        if (MODE == 1) {
+            const double gx = fmax(0.0, fabs(Cx) - ((double)ih.hx + th.hx));
+            const double gy = fmax(0.0, fabs(Cy) - ((double)ih.hy + th.hy));
+            const double gz = fmax(0.0, fabs(Cz) - ((double)ih.hz + th.hz));
+            const double dmin2 = gx * gx + gy * gy + gz * gz;
+            const double hj2 = (double)th.hx * th.hx + (double)th.hy * th.hy + (double)th.hz * th.hz;
+            const double size = sqrt(Cx * Cx + Cy * Cy + Cz * Cz) + sqrt(hi2) + sqrt(hj2);
+            near = (dmin2 < theta * theta * size * size) || (dmin2 <= eps2x);
+        }
+        if (near) {
+            for (int jj = 0; jj < cnt; ++jj) {
+                const double4 p = jd[jb + jj];
+                const double sj = jsig[jb + jj];
+#pragma unroll
+                for (int q = 0; q < IPL; ++q) {
+                    const double dx = p.x - xi[q], dy = p.y - yi[q], dz = p.z - zi[q];
+                    const double e = si[q] + sj;
+                    double r2 = fma(dx, dx, fma(dy, dy, fma(dz, dz, e * e)));
+                    r2 = fmax(r2, TINY2);
+                    double y = __builtin_amdgcn_rsq(r2);
+                    const double h = 0.5 * r2;
+                    y = y * fma(-h, y * y, 1.5);
+                    const double sc3 = p.w * y * (y * y);
+                    ax[q] = fma(sc3, dx, ax[q]);
+                    ay[q] = fma(sc3, dy, ay[q]);
+                    az[q] = fma(sc3, dz, az[q]);
+                }
+            }
+        } else {
+            const float cx = (float)Cx, cy = (float)Cy, cz = (float)Cz;
+            float wx[IPL], wy[IPL], wz[IPL], fx[IPL], fy[IPL], fz[IPL];
+#pragma unroll
+            for (int q = 0; q < IPL; ++q) {
+                wx[q] = cx - ux[q];
+                wy[q] = cy - uy[q];
+                wz[q] = cz - uz[q];
+                fx[q] = fy[q] = fz[q] = 0.f;
+            }
+            for (int jj = 0; jj < cnt; ++jj) {
+                const float4 u = ju[jb + jj];
+#pragma unroll
+                for (int q = 0; q < IPL; ++q) {
+                    const float dx = wx[q] + u.x, dy = wy[q] + u.y, dz = wz[q] + u.z;
+                    const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                    const float y = __builtin_amdgcn_rsqf(r2);
+                    const float s3 = (u.w * y) * (y * y);
+                    fx[q] = fmaf(s3, dx, fx[q]);
+                    fy[q] = fmaf(s3, dy, fy[q]);
+                    fz[q] = fmaf(s3, dz, fz[q]);
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < IPL; ++q) {
+                ax[q] += (double)fx[q];
+                ay[q] += (double)fy[q];
+                az[q] += (double)fz[q];
+            }
+        }
+    }
+    double *o = part + (size_t)blockIdx.y * 3 * (size_t)nslots;
+#pragma unroll
+    for (int q = 0; q < IPL; ++q) {
+        const int i = i0 + q * NT + (int)threadIdx.x;
+        if (i < nv) {
+            o[3 * (size_t)i + 0] = ax[q];
+            o[3 * (size_t)i + 1] = ay[q];
+            o[3 * (size_t)i + 2] = az[q];
+        }
+    }
+}
+
+// ---- 7. finish ------------------------------------------------------------
+__global__ __launch_bounds__(NT) void k_finish(const DBody *__restrict__ b,
+                                               const int *__restrict__ perm,
+                                               const Scal *__restrict__ sc,
+                                               const double *__restrict__ part, int nsplit,
+                                               int nslots, int cp, double cx, double cy, double cz,
+                                               double mu, const double *__restrict__ dcen,
+                                               double G, double *__restrict__ acc,
+                                               double *__restrict__ cterm)
+{
+    const int s = blockIdx.x * NT + threadIdx.x;
+    if (s >= sc->nvalid) return;
+    if (dcen) {  // central body state {x, y, z, vx, vy, vz, M, present} on the device
+        cx = dcen[0];
+        cy = dcen[1];
+        cz = dcen[2];
+        mu = G * dcen[6];
+        cp = dcen[7] != 0.0;
+    }
+    const int k = perm[s];
+    double a[3] = {0.0, 0.0, 0.0};
+    for (int p = 0; p < nsplit; ++p) {
+        const double *o = part + (size_t)p * 3 * (size_t)nslots + 3 * (size_t)s;
+        a[0] += o[0];
+        a[1] += o[1];
+        a[2] += o[2];
+    }
+    if (cp) {
+        const DBody q = b[k];
+        const double DX = q.x - cx, DY = q.y - cy, DZ = q.z - cz;
+        const double rk = sqrt(DX * DX + DY * DY + DZ * DZ);
+        const double r3 = rk * rk * rk;
+        a[0] += -mu * DX / r3;
+        a[1] += -mu * DY / r3;
+        a[2] += -mu * DZ / r3;
+        if (cterm) {
+            cterm[3 * (size_t)k + 0] = G * q.mass * DX / r3;
+            cterm[3 * (size_t)k + 1] = G * q.mass * DY / r3;
+            cterm[3 * (size_t)k + 2] = G * q.mass * DZ / r3;
+        }
+    }
+    acc[3 * (size_t)k + 0] = a[0];
+    acc[3 * (size_t)k + 1] = a[1];
+    acc[3 * (size_t)k + 2] = a[2];
+}
+
+// Deterministic sum of the central reaction terms of existing bodies.
+__global__ __launch_bounds__(NT) void k_sum3(const double *__restrict__ v, int n,
+                                             const int32_t *__restrict__ ex,
+                                             double *__restrict__ part)
+{
+    __shared__ double sh[NT];
+    double a[3] = {0.0, 0.0, 0.0};
+    for (int k = blockIdx.x * NT + threadIdx.x; k < n; k += gridDim.x * NT) {
+        if (ex && !ex[k]) continue;
+        a[0] += v[3 * (size_t)k];
+        a[1] += v[3 * (size_t)k + 1];
+        a[2] += v[3 * (size_t)k + 2];
+    }
+    for (int c = 0; c < 3; ++c) {
+        const double r = block_reduce(a[c], sh, 2);
+        if (threadIdx.x == 0) part[3 * blockIdx.x + c] = r;
+    }
+}
+
+__global__ __launch_bounds__(NT) void k_sum3_final(const double *__restrict__ part, int np,
+                                                   double *__restrict__ out)
+{
+    __shared__ double sh[NT];
+    double a[3] = {0.0, 0.0, 0.0};
+    for (int q = threadIdx.x; q < np; q += NT) {
+        a[0] += part[3 * q];
+        a[1] += part[3 * q + 1];
+        a[2] += part[3 * q + 2];
+    }
+    for (int c = 0; c < 3; ++c) {
+        const double r = block_reduce(a[c], sh, 2);
+        if (threadIdx.x == 0) out[c] = r;
+    }
+}
+
+}  // namespace
+
+// Tuning knobs of the tiled pipeline (chosen by the autotuner; defaults here).
+struct TiledCfg {
+    int ipl = 2;
+    int nsplit = 0;  // 0: pick from the CU count
+    double theta = 0.5;
+};
+static TiledCfg g_tiled_cfg;
+
+extern "C" void ncg_tiled_set_config(int ipl, int nsplit, double theta)
+{
+    if (ipl == 1 || ipl == 2 || ipl == 4) g_tiled_cfg.ipl = ipl;
+    if (nsplit >= 0) g_tiled_cfg.nsplit = nsplit;
+    if (theta > 0.0) g_tiled_cfg.theta = theta;
+}
+
+ncg_status ncg_grav_exact_launch(ncg_ctx *ctx, const DBody *d_b, int32_t n, const int32_t *d_ex,
+                                 const ncg_central *c, double G, double soft, double *d_acc,
+                                 double *d_cterm, hipStream_t st);
+
+template <int MODE, int IPL>
+static void launch_pairs(dim3 grid, hipStream_t st, const double4 *jd, const double *jsig,
+                         const float4 *ju, const TileHdr *th, const TileHdr *ih, const Scal *sc,
+                         int nsplit, double theta, double soft, int n, double *part)
+{
+    hipLaunchKernelGGL((k_pairs<MODE, IPL>), grid, dim3(NT), 0, st, jd, jsig, ju, th, ih, sc,
+                       nsplit, theta, soft, n, part);
+}
+
+// Grid shape of the tiled forms for n slots (same in order and eval).
+struct TiledShape {
+    int ipl, iblock, nib, ntiles, nsplit, nbb;
+};
+
+static TiledShape tiled_shape(ncg_ctx *ctx, int32_t n)
+{
+    TiledShape t;
+    t.ipl = g_tiled_cfg.ipl;
+    t.iblock = NT * t.ipl;
+    t.nib = (int)cdiv(n, t.iblock);
+    t.ntiles = (int)cdiv(n, TJ);
+    int nsplit = g_tiled_cfg.nsplit;
+    if (nsplit <= 0) {
+        const int cus = ctx->num_cus > 0 ? ctx->num_cus : 256;
+        nsplit = (int)cdiv(4L * cus, t.nib);
+    }
+    t.nsplit = std::max(1, std::min(nsplit, t.ntiles));
+    t.nbb = std::min(1024, (int)cdiv(n, NT));
+    return t;
+}
+
+ncg_status ncg_tiled_order(ncg_ctx *ctx, const DBody *d_b, int32_t n, const int32_t *d_ex,
+                           hipStream_t st)
+{
+    const TiledShape t = tiled_shape(ctx, n);
+    Scal *sc = dbuf<Scal>(ctx, "t_scal", 1);
+    double *bpart = dbuf<double>(ctx, "t_bpart", 8 * (size_t)t.nbb);
+    unsigned *key = dbuf<unsigned>(ctx, "t_key", n);
+    unsigned *key2 = dbuf<unsigned>(ctx, "t_key2", n);
+    int *idx = dbuf<int>(ctx, "t_idx", n);
+    int *perm = dbuf<int>(ctx, "t_perm", n);
+    if (!sc || !bpart || !key || !key2 || !idx || !perm) return NCG_ERR_NOMEM;
+    size_t tmp_bytes = 0;
+    NCG_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, key, key2, idx, perm, n, 0, 32,
+                                               st));
+    void *tmp = ncg_dbuf(ctx, "t_sorttmp", tmp_bytes);
+    if (!tmp) return NCG_ERR_NOMEM;
+    hipLaunchKernelGGL(k_bbox_partial, dim3(t.nbb), dim3(NT), 0, st, d_b, n, d_ex, bpart);
+    hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(NT), 0, st, bpart, t.nbb, sc);
+    hipLaunchKernelGGL(k_morton, dim3(cdiv(n, NT)), dim3(NT), 0, st, d_b, n, d_ex, sc, key, idx);
+    NCG_CHECK_LAUNCH();
+    NCG_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, key, key2, idx, perm, n, 0, 32, st));
+    return NCG_OK;
+}
+
+ncg_status ncg_tiled_eval(ncg_ctx *ctx, const DBody *d_b, int32_t n, const int32_t *d_ex,
+                          const ncg_central *c, const double *d_central, double G, double soft,
+                          ncg_precision prec, double *d_acc, double *d_cterm, double *d_cm_acc,
+                          hipStream_t st)
+{
+    const TiledShape t = tiled_shape(ctx, n);
+    Scal *sc = dbuf<Scal>(ctx, "t_scal", 1);
+    int *perm = dbuf<int>(ctx, "t_perm", n);
+    double4 *jd = dbuf<double4>(ctx, "t_jd", (size_t)t.ntiles * TJ + t.iblock);
+    double *jsig = dbuf<double>(ctx, "t_jsig", (size_t)t.ntiles * TJ + t.iblock);
+    float4 *ju = dbuf<float4>(ctx, "t_ju", (size_t)t.ntiles * TJ);
+    TileHdr *th = dbuf<TileHdr>(ctx, "t_th", t.ntiles);
+    TileHdr *ih = dbuf<TileHdr>(ctx, "t_ih", t.nib);
+    double *part = dbuf<double>(ctx, "t_part", (size_t)t.nsplit * 3 * n);
+    const bool want_cm = d_cm_acc != nullptr;
+    double *cterm = d_cterm ? d_cterm : (want_cm ? dbuf<double>(ctx, "cterm", 3 * (size_t)n) : nullptr);
+    double *p3 = want_cm ? dbuf<double>(ctx, "sum3", 3 * (size_t)t.nbb) : nullptr;
+    if (!sc || !perm || !jd || !jsig || !ju || !th || !ih || !part || (want_cm && (!cterm || !p3)))
+        return NCG_ERR_NOMEM;
+    hipLaunchKernelGGL(k_gather, dim3(cdiv(n, NT)), dim3(NT), 0, st, d_b, n, perm, sc, G, soft, jd,
+                       jsig);
+    hipLaunchKernelGGL(k_tiles, dim3(t.ntiles), dim3(NT), 0, st, jd, sc, TJ, th, ju);
+    hipLaunchKernelGGL(k_tiles, dim3(t.nib), dim3(NT), 0, st, jd, sc, t.iblock, ih,
+                       (float4 *)nullptr);
+    NCG_CHECK_LAUNCH();
+    const dim3 grid(t.nib, t.nsplit);
+    const double theta = g_tiled_cfg.theta;
+    {
+        ProfScope ps(ctx, st, prec == NCG_F64_FAST ? "pairs_fast64" : "pairs_mixed");
+        if (prec == NCG_F64_FAST) {
+            if (t.ipl == 1) launch_pairs<0, 1>(grid, st, jd, jsig, ju, th, ih, sc, t.nsplit, theta, soft, n, part);
+            else if (t.ipl == 2) launch_pairs<0, 2>(grid, st, jd, jsig, ju, th, ih, sc, t.nsplit, theta, soft, n, part);
+            else launch_pairs<0, 4>(grid, st, jd, jsig, ju, th, ih, sc, t.nsplit, theta, soft, n, part);
+        } else {
+            if (t.ipl == 1) launch_pairs<1, 1>(grid, st, jd, jsig, ju, th, ih, sc, t.nsplit, theta, soft, n, part);
+            else if (t.ipl == 2) launch_pairs<1, 2>(grid, st, jd, jsig, ju, th, ih, sc, t.nsplit, theta, soft, n, part);
+            else launch_pairs<1, 4>(grid, st, jd, jsig, ju, th, ih, sc, t.nsplit, theta, soft, n, part);
+        }
+        NCG_CHECK_LAUNCH();
+    }
+    const int cp = (c && c->present) ? 1 : 0;
+    const double mu = cp ? G * c->mass : 0.0;
+    hipLaunchKernelGGL(k_finish, dim3(cdiv(n, NT)), dim3(NT), 0, st, d_b, perm, sc, part, t.nsplit,
+                       n, cp, cp ? c->x : 0.0, cp ? c->y : 0.0, cp ? c->z : 0.0, mu, d_central, G,
+                       d_acc, cterm);
+    NCG_CHECK_LAUNCH();
+    if (want_cm) {
+        hipLaunchKernelGGL(k_sum3, dim3(t.nbb), dim3(NT), 0, st, cterm, n, d_ex, p3);
+        hipLaunchKernelGGL(k_sum3_final, dim3(1), dim3(NT), 0, st, p3, t.nbb, d_cm_acc);
+        NCG_CHECK_LAUNCH();
+    }
+    return NCG_OK;
+}
+
+ncg_status ncg_gravity_dev_impl(ncg_ctx *ctx, const DBody *d_b, int32_t n, const int32_t *d_ex,
+                                const ncg_central *c, double G, double soft, ncg_precision prec,
+                                double *d_acc, double *d_cterm, double *d_cm_acc, hipStream_t st)
+{
+    NCG_REQUIRE(n >= 0, "n must be >= 0");
+    if (n == 0) return NCG_OK;
+    const int cp = (c && c->present) ? 1 : 0;
+    if (prec == NCG_F64_EXACT) {
+        double *cterm = d_cterm;
+        if (cp && !cterm) {
+            cterm = dbuf<double>(ctx, "cterm", 3 * (size_t)n);
+            if (!cterm) return NCG_ERR_NOMEM;
+        }
+        ncg_status r;
+        {
+            ProfScope ps(ctx, st, "exact");
+            r = ncg_grav_exact_launch(ctx, d_b, n, d_ex, c, G, soft, d_acc, cterm, st);
+        }
+        if (r != NCG_OK) return r;
+        if (cp && d_cm_acc) {
+            // Device tree sum; the host entry point redoes this sum in the
+            // reference's sequential order from the per-body terms.
+            const int nb = std::min(1024, (int)cdiv(n, NT));
+            double *p3 = dbuf<double>(ctx, "sum3", 3 * (size_t)nb);
+            if (!p3) return NCG_ERR_NOMEM;
+            hipLaunchKernelGGL(k_sum3, dim3(nb), dim3(NT), 0, st, cterm, n, d_ex, p3);
+            hipLaunchKernelGGL(k_sum3_final, dim3(1), dim3(NT), 0, st, p3, nb, d_cm_acc);
+            NCG_CHECK_LAUNCH();
+        }
+        return NCG_OK;
+    }
+    NCG_REQUIRE(prec == NCG_F64_FAST || prec == NCG_MIXED, "unknown precision %d", (int)prec);
+    ncg_status r = ncg_tiled_order(ctx, d_b, n, d_ex, st);
+    if (r != NCG_OK) return r;
+    return ncg_tiled_eval(ctx, d_b, n, d_ex, cp ? c : nullptr, nullptr, G, soft, prec, d_acc,
+                          d_cterm, cp ? d_cm_acc : nullptr, st);
+}
+
+// ---- C ABI ------------------------------------------------------------------
+extern "C" ncg_status ncg_gravity_allpairs_dev(ncg_ctx *ctx, const ncg_body *d_bodies, int32_t n,
+                                               const int32_t *d_exists, const ncg_central *central,
+                                               double G, double softening_parameter,
+                                               ncg_precision prec, double *d_acc,
+                                               double *d_cm_acc, void *stream)
+{
+    if (!ctx) return NCG_ERR_INVALID;
+    NCG_REQUIRE(n >= 0 && (n == 0 || (d_bodies && d_acc)), "invalid arguments");
+    NCG_HIP(hipSetDevice(ctx->device));
+    return ncg_gravity_dev_impl(ctx, reinterpret_cast<const DBody *>(d_bodies), n, d_exists,
+                                central, G, softening_parameter, prec, d_acc, nullptr, d_cm_acc,
+                                pick_stream(ctx, stream));
+}
+
+extern "C" ncg_status ncg_gravity_allpairs(ncg_ctx *ctx, const ncg_body *bodies, int32_t n,
+                                           const int32_t *exists, const ncg_central *central,
+                                           double G, double softening_parameter,
+                                           ncg_precision prec, double *acc, double *cm_acc)
+{
+    if (!ctx) return NCG_ERR_INVALID;
+    NCG_REQUIRE(n >= 0 && (n == 0 || (bodies && acc)), "invalid arguments");
+    if (n == 0) {
+        if (cm_acc && central && central->present) cm_acc[0] = cm_acc[1] = cm_acc[2] = 0.0;
+        return NCG_OK;
+    }
+    NCG_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    const int cp = (central && central->present) ? 1 : 0;
+    DBody *db = dbuf<DBody>(ctx, "h_bodies", n);
+    int32_t *dex = exists ? dbuf<int32_t>(ctx, "h_exists", n) : nullptr;
+    double *dacc = dbuf<double>(ctx, "h_acc", 3 * (size_t)n);
+    double *dct = cp ? dbuf<double>(ctx, "h_cterm", 3 * (size_t)n) : nullptr;
+    double *dcm = dbuf<double>(ctx, "h_cm", 4);
+    if (!db || (exists && !dex) || !dacc || (cp && !dct) || !dcm) return NCG_ERR_NOMEM;
+    NCG_HIP(hipMemcpyAsync(db, bodies, sizeof(DBody) * (size_t)n, hipMemcpyHostToDevice, st));
+    if (exists)
+        NCG_HIP(hipMemcpyAsync(dex, exists, sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, st));
+    const bool exact = prec == NCG_F64_EXACT;
+    ncg_status r = ncg_gravity_dev_impl(ctx, db, n, dex, central, G, softening_parameter, prec,
+                                        dacc, dct, (cp && !exact) ? dcm : nullptr, st);
+    if (r != NCG_OK) return r;
+    // Holes keep the caller's values: copy back per existing body.
+    double *hacc = hbuf<double>(ctx, "acc", 3 * (size_t)n);
+    NCG_HIP(hipMemcpyAsync(hacc, dacc, sizeof(double) * 3 * (size_t)n, hipMemcpyDeviceToHost, st));
+    double *hct = nullptr;
+    if (cp && exact) {
+        hct = hbuf<double>(ctx, "cterm", 3 * (size_t)n);
+        NCG_HIP(hipMemcpyAsync(hct, dct, sizeof(double) * 3 * (size_t)n, hipMemcpyDeviceToHost, st));
+    }
+    double hcm[4] = {0, 0, 0, 0};
+    if (cp && !exact) NCG_HIP(hipMemcpyAsync(hcm, dcm, sizeof(double) * 3, hipMemcpyDeviceToHost, st));
+    NCG_HIP(hipStreamSynchronize(st));
+    for (int32_t k = 0; k < n; ++k) {
+        if (exists && !exists[k]) continue;
+        acc[3 * (size_t)k] = hacc[3 * (size_t)k];
+        acc[3 * (size_t)k + 1] = hacc[3 * (size_t)k + 1];
+        acc[3 * (size_t)k + 2] = hacc[3 * (size_t)k + 2];
+    }
+    if (cp && cm_acc) {
+        if (exact) {
+            // CM_acc accumulates body by body in index order (physics.c:80, 140-142).
+            double s0 = 0., s1 = 0., s2 = 0.;
+            for (int32_t k = 0; k < n; ++k) {
+                if (exists && !exists[k]) continue;
+                s0 += hct[3 * (size_t)k];
+                s1 += hct[3 * (size_t)k + 1];
+                s2 += hct[3 * (size_t)k + 2];
+            }
+            cm_acc[0] = s0;
+            cm_acc[1] = s1;
+            cm_acc[2] = s2;
+        } else {
+            cm_acc[0] = hcm[0];
+            cm_acc[1] = hcm[1];
+            cm_acc[2] = hcm[2];
+        }
+    }
+    return NCG_OK;
+}

@@ -1,0 +1,155 @@
+// Internal declarations shared by the libncg translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/ncg.h"
+
+// Device-side view of the reference's struct moonlet (structure.h:74-84).
+struct alignas(64) DBody {
+    double x, y, z, vx, vy, vz, mass, radius;
+};
+static_assert(sizeof(DBody) == sizeof(ncg_body), "body layout");
+
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+};
+
+// Mesh binning kept between ncg_mesh_pass and ncg_mesh_gravity.
+struct MeshState {
+    bool valid = false;
+    int32_t n = 0;           // body slots of the pass
+    int32_t cells = 0;       // collision_cube_cells
+    int64_t nkeys = 0;       // dense key range of the start/end tables
+    int32_t mode = 0;        // 0 = mesh(), 1 = get_neighbours_mesh()
+    int32_t inserted = 0;    // bodies inserted in the hash
+    double gam = 0.0, cube = 0.0;
+    bool big_pairs = false;  // pass registered pairs of bodies larger than gam
+    std::vector<ncg_pair> extra_pairs;  // big-body pairs (host visit order)
+};
+
+struct ProfRec {
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+    double ms = 0.0;
+    int64_t launches = 0;
+};
+
+struct Rk4Graph;  // defined in rk4.hip
+
+struct ncg_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    std::unordered_map<std::string, DevBuf> bufs;
+    std::unordered_map<std::string, std::vector<unsigned char>> hbufs;
+    MeshState mesh;
+    int num_cus = 0;
+    bool profiling = false;
+    std::unordered_map<std::string, ProfRec> prof;
+    Rk4Graph *rk4 = nullptr;
+    uint64_t buf_gen = 0;  // bumped whenever a device buffer is (re)allocated
+    // While a graph is captured with profiling on, the timing events become
+    // graph nodes; they are collected here and re-read after every replay.
+    bool capturing = false;
+    std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> cap_events;
+};
+
+void ncg_rk4_release(ncg_ctx *ctx);
+
+// Brackets one kernel launch with HIP events on its stream when profiling.
+struct ProfScope {
+    ncg_ctx *c;
+    hipStream_t s;
+    const char *name;
+    hipEvent_t a = nullptr, b = nullptr;
+    ProfScope(ncg_ctx *c_, hipStream_t s_, const char *n_) : c(c_), s(s_), name(n_)
+    {
+        if (c->profiling && hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess)
+            (void)hipEventRecord(a, s);
+        else
+            a = b = nullptr;
+    }
+    ~ProfScope()
+    {
+        if (a && b) {
+            (void)hipEventRecord(b, s);
+            if (c->capturing)
+                c->cap_events.emplace_back(name, std::make_pair(a, b));
+            else
+                c->prof[name].pending.emplace_back(a, b);
+        }
+    }
+};
+
+ncg_status ncg_fail(ncg_ctx *ctx, ncg_status code, const char *fmt, ...);
+
+// Device scratch buffer `name`, grown to at least `bytes` (contents are not
+// preserved across growth).  Returns nullptr and records an error on failure.
+void *ncg_dbuf(ncg_ctx *ctx, const char *name, size_t bytes);
+
+template <class T>
+static inline T *dbuf(ncg_ctx *ctx, const char *name, size_t count)
+{
+    return static_cast<T *>(ncg_dbuf(ctx, name, count * sizeof(T) + 16));
+}
+
+// Host staging vector `name` of at least `count` elements.
+template <class T>
+static inline T *hbuf(ncg_ctx *ctx, const char *name, size_t count)
+{
+    auto &v = ctx->hbufs[name];
+    if (v.size() < count * sizeof(T)) v.resize(count * sizeof(T) + 64);
+    return reinterpret_cast<T *>(v.data());
+}
+
+#define NCG_HIP(call)                                                                   \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return ncg_fail(ctx, NCG_ERR_DEVICE, "%s failed: %s (%s:%d)", #call,        \
+                            hipGetErrorString(e_), __FILE__, __LINE__);                 \
+    } while (0)
+
+#define NCG_CHECK_LAUNCH()                                                              \
+    do {                                                                                \
+        hipError_t e_ = hipGetLastError();                                              \
+        if (e_ != hipSuccess)                                                           \
+            return ncg_fail(ctx, NCG_ERR_DEVICE, "kernel launch failed: %s (%s:%d)",     \
+                            hipGetErrorString(e_), __FILE__, __LINE__);                 \
+    } while (0)
+
+#define NCG_REQUIRE(cond, ...)                                                          \
+    do {                                                                                \
+        if (!(cond)) return ncg_fail(ctx, NCG_ERR_INVALID, __VA_ARGS__);                \
+    } while (0)
+
+static inline hipStream_t pick_stream(ncg_ctx *ctx, void *stream)
+{
+    return stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+}
+
+static inline unsigned cdiv(long a, long b) { return (unsigned)((a + b - 1) / b); }
+
+// Internal launchers used across translation units.
+// Spatial order of the tiled forms: bounding box, Morton keys, radix sort.
+// Leaves the permutation and the count of existing bodies on the device.
+ncg_status ncg_tiled_order(ncg_ctx *ctx, const DBody *d_b, int32_t n, const int32_t *d_ex,
+                           hipStream_t s);
+// One evaluation with the order of the last ncg_tiled_order (no allocation,
+// no host synchronisation: safe under stream capture once buffers exist).
+ncg_status ncg_tiled_eval(ncg_ctx *ctx, const DBody *d_b, int32_t n, const int32_t *d_ex,
+                          const ncg_central *central, const double *d_central, double G,
+                          double soft, ncg_precision prec, double *d_acc, double *d_cterm,
+                          double *d_cm_acc, hipStream_t s);
+ncg_status ncg_gravity_dev_impl(ncg_ctx *ctx, const DBody *d_bodies, int32_t n,
+                                const int32_t *d_exists, const ncg_central *central, double G,
+                                double soft, ncg_precision prec, double *d_acc,
+                                double *d_cterm, double *d_cm_acc, hipStream_t s);

@@ -1,0 +1,91 @@
+"""GPU brute-force collision search + host replay against the reference.
+
+The GPU candidate set must contain every pair closest_approach reports at the
+start-of-pass state; the replay through the library must reproduce the
+reference's collision sequence and final state bit for bit.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as orc
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def load(name):
+    return dict(np.load(os.path.join(GOLD, name + ".npz")))
+
+
+def bits(a):
+    return np.ascontiguousarray(a, dtype=np.float64).view(np.uint64)
+
+
+def cluster(n, seed, box=0.05, radius=2e-3, vscale=1.0):
+    rng = np.random.default_rng(seed)
+    b = np.zeros((n, 8))
+    b[:, 0:3] = rng.uniform(-box / 2, box / 2, (n, 3)) + np.array([1.0, 0.0, 0.0])
+    b[:, 3:6] = rng.normal(0.0, vscale, (n, 3))
+    b[:, 6] = rng.uniform(1e-9, 1e-8, n)
+    b[:, 7] = radius * rng.uniform(0.5, 1.5, n)
+    return b
+
+
+def test_candidates_equal_reference_hits(ctx):
+    g = load("ca_n48")
+    cand = ctx.collide_candidates(g["bodies"], None, float(g["dt"]))
+    exact = g["ab"][g["hit"].astype(bool)]
+    assert np.array_equal(cand, exact)
+
+
+@pytest.mark.parametrize("seed,dt", [(1, 0.01), (2, -0.01), (3, 0.005)])
+def test_candidates_superset_random(ctx, seed, dt):
+    b = cluster(700, seed, box=0.08)
+    ex = (np.random.default_rng(seed).uniform(size=700) > 0.1).astype(np.int32)
+    cand = {tuple(p) for p in ctx.collide_candidates(b, ex, dt)}
+    hits = set()
+    for i in range(700):
+        for j in range(i + 1, 700):
+            if orc.closest_approach(b, i, j, dt, ex)[0]:
+                hits.add((i, j))
+    assert hits <= cand
+    assert len(cand - hits) <= 2  # only last-bit boundary cases may be extra
+    assert len(hits) > 10
+
+
+@pytest.mark.parametrize("name", ["bf_inelastic_n300", "bf_elastic_once_n200",
+                                  "bf_inelastic125_n250"])
+def test_replay_matches_reference(ctx, name):
+    g = load(name)
+    r = orc.Replay(g["bodies"], float(g["dt"]), int(g["ctype"]), float(g["f"]), int(g["one_coll"]),
+                   g.get("exists"))
+    n = ctx.brute_force_pass(r.b, r.e, float(g["dt"]), r.test, r.resolve)
+    assert n == len(g["coll"])
+    assert np.array_equal(np.array(r.log, dtype=np.int32).reshape(-1, 2), g["coll"])
+    assert np.array_equal(bits(r.b), bits(g["out_bodies"]))
+
+
+@pytest.mark.parametrize("seed", [21, 22])
+def test_replay_matches_oracle_dense(ctx, seed):
+    b = cluster(1200, seed, box=0.06, radius=2.5e-3)
+    out, _, pairs = orc.brute_force(b, 0.01, 1, 1.5, 0)
+    r = orc.Replay(b, 0.01, 1, 1.5, 0)
+    ctx.brute_force_pass(r.b, r.e, 0.01, r.test, r.resolve)
+    assert np.array_equal(np.array(r.log, dtype=np.int32).reshape(-1, 2), pairs)
+    assert np.array_equal(bits(r.b), bits(out))
+    assert len(pairs) > 100
+
+
+def test_replay_sparse_collisions_few_host_tests(ctx):
+    # A handful of collisions: the host only re-tests rows/columns of the
+    # bodies involved, not all n(n-1)/2 pairs.
+    b = cluster(3000, 31, box=0.5, radius=2e-3)
+    out, _, pairs = orc.brute_force(b, 0.01, 1, 1.5, 0)
+    r = orc.Replay(b, 0.01, 1, 1.5, 0)
+    ctx.brute_force_pass(r.b, r.e, 0.01, r.test, r.resolve)
+    assert np.array_equal(np.array(r.log, dtype=np.int32).reshape(-1, 2), pairs)
+    assert np.array_equal(bits(r.b), bits(out))
+    assert 0 < len(pairs) < 50
+    assert r.tests < 4 * 3000 * (len(pairs) + 1)

@@ -1,0 +1,57 @@
+"""Whole NcorpiON runs: the reference vs the reference with the GPU drop-in.
+
+Both executables are NcorpiON's own main and integrator, compiled from its
+sources with the tracer of oracle/ref_trace.c; the drop-in one replaces
+vector_field / brute_force / mesh / get_neighbours_mesh with the glue of
+ncorpion_amd/glue at link time.  In exact mode every traced call (state hashes
+before and after, accelerations, CM_acc, collision pairs, neighbour counts)
+must be identical.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from ncorpion_amd import ic
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+REF = os.path.join(ROOT, "oracle", "_ref")
+
+
+def run(binary, workdir, env_extra=None, timeout=600):
+    env = dict(os.environ)
+    env["NCG_TRACE"] = os.path.join(workdir, os.path.basename(binary) + ".trace")
+    env.update(env_extra or {})
+    r = subprocess.run([binary], cwd=workdir, env=env, stdout=subprocess.PIPE,
+                       stderr=subprocess.STDOUT, text=True, timeout=timeout)
+    assert r.returncode == 0, r.stdout[-3000:]
+    lines = open(env["NCG_TRACE"]).read().splitlines()
+    return [l for l in lines if not l.startswith("TIME")], r.stdout
+
+
+def compare(cfg, bodies, tmp_path, steps_expected):
+    if not os.path.exists(os.path.join(REF, cfg, "ncorpion_dropin_trace")):
+        pytest.skip("reference builds missing (oracle/build_ref.py)")
+    ic.write_init_txt(str(tmp_path / "init.txt"), bodies)
+    ref, out_ref = run(os.path.join(REF, cfg, "ncorpion_trace"), str(tmp_path))
+    gpu, out_gpu = run(os.path.join(REF, cfg, "ncorpion_dropin_trace"), str(tmp_path))
+    assert "total number of timestep performed : %d" % steps_expected in out_ref
+    assert len(ref) == len(gpu)
+    for a, b in zip(ref, gpu):
+        assert a == b, (a, b)
+    return ref
+
+
+def test_c1_full_run_identical(tmp_path):
+    # 2000 bodies of mass 1e-8, radius 2e-4, 200 steps of 2 pi / 256, inelastic.
+    b = ic.disk(2000, seed=0, mass=1e-8, radius=2e-4)
+    compare("run_c1", b, tmp_path, 200)
+
+
+def test_dense_run_identical_with_collisions(tmp_path):
+    b = ic.disk(2000, seed=1, mass=1e-8, radius=4e-3)
+    tr = compare("run_c1_dense", b, tmp_path, 40)
+    ncoll = sum(1 for l in tr if l.startswith("C "))
+    assert ncoll > 20
