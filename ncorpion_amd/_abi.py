@@ -78,6 +78,13 @@ def load():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise RuntimeError("libncg.so is not built (%s); run __graft_entry__.build()" % LIB_PATH)
+    # PyTorch-ROCm ships its own HIP runtime with the same SONAME
+    # (libamdhip64.so.7).  Loading torch first makes libncg bind to that one,
+    # so torch tensors and libncg share a single runtime in this process.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(LIB_PATH)
     for name, (res, args) in _SIGNATURES.items():
         fn = getattr(lib, name)

@@ -1,24 +1,936 @@
-// Mesh module (placeholder until the binning kernels land).
+// Mesh module: cell binning, neighbour counts, collision candidates, host
+// replay and neighbour-cell gravity.
+//
+// Restates the cell semantics of NcorpiON's mesh algorithm
+// (collision.c:175-417, physics.c:1094-1118):
+//   - a body is inserted in the hash table when it exists, lies inside the
+//     collision cube (|X|, |Y|, |Z| <= collision_cube/2) and, in mesh(), has
+//     radius <= gam (get_neighbours_mesh() inserts every existing body);
+//   - its cell is (i, j, k) = cells/2 + 1 + floor(X/gam) per axis and its key
+//     i + j*d + k*d^2 with d = collision_cube_cells (collision.c:203-207), so
+//     cells near the +x/+y faces alias into the next row exactly as in the
+//     reference;
+//   - body k's neighbours are the bodies inserted before it (p < k) whose key
+//     is one of k's 27 keys, visited for cell l = 26..0 and ascending id within
+//     a cell (the LIFO of chains, collision.c:285-302 and 349-377).
+// The reference walks linked lists; here the bodies are sorted by (key, id)
+// with a radix sort, a dense table gives each key its [start, end) range, and
+// one GPU thread per inserted body counts its neighbours (total_neighbours)
+// and runs the closest_approach predicate on each neighbour pair.  The host
+// then replays the reference's visiting order over the candidates; every pair
+// touching a body modified earlier in the pass is re-tested on the host.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <map>
+#include <set>
+
 #include "ncg_internal.hpp"
 
-extern "C" ncg_status ncg_mesh_pass(ncg_ctx *ctx, const ncg_body *, int32_t, const int32_t *,
-                                    double, double, double, int32_t, int32_t,
-                                    const ncg_replay_ops *, ncg_mesh_stats *)
+#pragma clang fp contract(off)
+
+namespace {
+
+constexpr int MT = 256;
+constexpr unsigned NOKEY = 0xffffffffu;
+
+__device__ __host__ inline long stencil_offset(int l, long d)
 {
-    if (!ctx) return NCG_ERR_INVALID;
-    return ncg_fail(ctx, NCG_ERR_UNSUPPORTED, "mesh pass not built yet");
+    const int di = l % 3 - 1, dj = (l / 3) % 3 - 1, dk = l / 9 - 1;
+    return di + dj * d + dk * d * d;
 }
 
-extern "C" ncg_status ncg_mesh_gravity(ncg_ctx *ctx, const ncg_body *, int32_t, const int32_t *,
-                                       const ncg_central *, double, double, ncg_precision,
-                                       double *, double *)
+template <class T>
+__device__ inline T bsum(T v, T *sh)
 {
-    if (!ctx) return NCG_ERR_INVALID;
-    return ncg_fail(ctx, NCG_ERR_UNSUPPORTED, "mesh gravity not built yet");
+    const int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int w = MT / 2; w > 0; w >>= 1) {
+        if (t < w) sh[t] += sh[t + w];
+        __syncthreads();
+    }
+    T r = sh[0];
+    __syncthreads();
+    return r;
 }
 
-extern "C" ncg_status ncg_mesh_pairs(ncg_ctx *ctx, ncg_pair *, int64_t, int64_t *)
+// Key of every body (NOKEY when not inserted) and the pass counters:
+// cnt[0] inserted, cnt[1] small (R <= gam), cnt[2] big, cnt[3] big-list length.
+__global__ __launch_bounds__(MT) void k_mkey(const DBody *__restrict__ b, int n,
+                                             const int32_t *__restrict__ ex, double gam,
+                                             double cube, int d, int mode,
+                                             unsigned *__restrict__ key, int *__restrict__ idx,
+                                             int *__restrict__ cnt, int *__restrict__ big,
+                                             int big_cap)
+{
+    __shared__ int sh[MT];
+    const int k = blockIdx.x * MT + threadIdx.x;
+    int ins = 0, sm = 0, bg = 0;
+    if (k < n) {
+        idx[k] = k;
+        unsigned kk = NOKEY;
+        if (ex == nullptr || ex[k] != 0) {
+            const DBody q = b[k];
+            const bool small = q.radius <= gam;  // collision.c:344
+            if (mode == 0) {
+                sm = small;
+                bg = !small;
+                if (!small) {
+                    const int s = atomicAdd(&cnt[3], 1);
+                    if (s < big_cap) big[s] = k;
+                }
+            }
+            if (mode == 1 || small) {
+                const double half = cube / 2.0;  // collision.c:197
+                if (!(fabs(q.x) > half || fabs(q.y) > half || fabs(q.z) > half)) {
+                    const long i = d / 2 + 1 + (int)floor(q.x / gam);
+                    const long j = d / 2 + 1 + (int)floor(q.y / gam);
+                    const long l = d / 2 + 1 + (int)floor(q.z / gam);
+                    kk = (unsigned)(i + j * (long)d + l * (long)d * d);
+                    ins = 1;
+                }
+            }
+        }
+        key[k] = kk;
+    }
+    const int a = bsum(ins, sh), s2 = bsum(sm, sh), b2 = bsum(bg, sh);
+    if (threadIdx.x == 0) {
+        if (a) atomicAdd(&cnt[0], a);
+        if (s2) atomicAdd(&cnt[1], s2);
+        if (b2) atomicAdd(&cnt[2], b2);
+    }
+}
+
+__global__ __launch_bounds__(MT) void k_clear_cells(const unsigned *__restrict__ keys, int cnt,
+                                                    unsigned *__restrict__ cs,
+                                                    unsigned *__restrict__ ce)
+{
+    const int s = blockIdx.x * MT + threadIdx.x;
+    if (s < cnt) {
+        cs[keys[s]] = 0u;
+        ce[keys[s]] = 0u;
+    }
+}
+
+__global__ __launch_bounds__(MT) void k_bounds(const unsigned *__restrict__ skey, int ins,
+                                               unsigned *__restrict__ cs, unsigned *__restrict__ ce)
+{
+    const int s = blockIdx.x * MT + threadIdx.x;
+    if (s >= ins) return;
+    const unsigned k = skey[s];
+    if (s == 0 || skey[s - 1] != k) cs[k] = (unsigned)s;
+    if (s == ins - 1 || skey[s + 1] != k) ce[k] = (unsigned)(s + 1);
+}
+
+// Sorted copies {x, y, z, R} and {vx, vy, vz, m} of the inserted bodies.
+__global__ __launch_bounds__(MT) void k_msorted(const DBody *__restrict__ b,
+                                                const int *__restrict__ sid, int ins,
+                                                double4 *__restrict__ pr, double4 *__restrict__ vm)
+{
+    const int s = blockIdx.x * MT + threadIdx.x;
+    if (s >= ins) return;
+    const DBody q = b[sid[s]];
+    pr[s] = make_double4(q.x, q.y, q.z, q.radius);
+    vm[s] = make_double4(q.vx, q.vy, q.vz, q.mass);
+}
+
+// closest_approach's predicate (collision.c:112-137) with a 2^-40 relative
+// slack on the time test; the host decides every candidate exactly.
+__device__ inline bool approach_hit(double4 pa, double4 va, double4 pb, double4 vb, double dt)
+{
+    const double dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
+    const double dvx = va.x - vb.x, dvy = va.y - vb.y, dvz = va.z - vb.z;
+    const double bb = dx * dvx + dy * dvy + dz * dvz;
+    if (!(bb < 0.0)) return false;
+    const double dr2 = dx * dx + dy * dy + dz * dz;
+    const double dv2 = dvx * dvx + dvy * dvy + dvz * dvz;
+    const double R = pa.w + pb.w;
+    const double disc = bb * bb + dv2 * (R * R - dr2);
+    if (!(disc > 0.0)) return false;
+    const double t = -(bb + sqrt(disc)) / dv2;
+    return t <= dt + fabs(dt) * 9.094947017729282e-13;
+}
+
+// One thread per inserted body (sorted position s): neighbour count (p < k in
+// the 27 cells) and, for mesh(), the collision candidates (k, l, p).
+template <int MODE>
+__global__ __launch_bounds__(MT) void k_mcount(const unsigned *__restrict__ skey,
+                                               const int *__restrict__ sid,
+                                               const double4 *__restrict__ pr,
+                                               const double4 *__restrict__ vm,
+                                               const unsigned *__restrict__ cs,
+                                               const unsigned *__restrict__ ce, int ins, int d,
+                                               double dt, int *__restrict__ count_by_id,
+                                               int4 *__restrict__ cand, unsigned long long cap,
+                                               unsigned long long *__restrict__ ccnt,
+                                               unsigned long long *__restrict__ total)
+{
+    __shared__ unsigned long long sh[MT];
+    const int s = blockIdx.x * MT + threadIdx.x;
+    unsigned long long c = 0;
+    if (s < ins) {
+        const int k = sid[s];
+        const long key = skey[s];
+        double4 pk = make_double4(0, 0, 0, 0), vk = pk;
+        if (MODE == 0) {
+            pk = pr[s];
+            vk = vm[s];
+        }
+        for (int l = 0; l < 27; ++l) {
+            const unsigned ck = (unsigned)(key + stencil_offset(l, d));
+            const unsigned a = cs[ck], e = ce[ck];
+            if (a == e) continue;
+            unsigned lo = a, hi = e;  // first position with id >= k
+            while (lo < hi) {
+                const unsigned mid = (lo + hi) >> 1;
+                if (sid[mid] < k) lo = mid + 1;
+                else hi = mid;
+            }
+            c += lo - a;
+            if (MODE == 0) {
+                for (unsigned q = a; q < lo; ++q) {
+                    if (approach_hit(pr[q], vm[q], pk, vk, dt)) {
+                        const unsigned long long slot = atomicAdd(ccnt, 1ull);
+                        if (slot < cap) cand[slot] = make_int4(k, l, sid[q], 0);
+                    }
+                }
+            }
+        }
+        count_by_id[k] = (int)c;
+    }
+    const unsigned long long t = bsum(c, sh);
+    if (threadIdx.x == 0 && t) atomicAdd(total, t);
+}
+
+// ---- neighbour-cell gravity ---------------------------------------------------
+// Term G m_p (X_p - X_k) / D^3 of physics.c:186-200 (reference expression).
+__device__ inline void term_exact(double X, double Y, double Z, double Rk, double4 p, double Rp,
+                                  double G, double s, double &ax, double &ay, double &az)
+{
+    const double dx = X - p.x, dy = Y - p.y, dz = Z - p.z;
+    const double soft = s * (Rk + Rp);
+    const double D = sqrt(dx * dx + dy * dy + dz * dz + soft * soft);
+    const double D3 = D * D * D;
+    const double gm = G * p.w;
+    ax -= gm * dx / D3;
+    ay -= gm * dy / D3;
+    az -= gm * dz / D3;
+}
+
+__device__ inline void term_fast(double X, double Y, double Z, double Rk, double4 p, double Rp,
+                                 double G, double s, double &ax, double &ay, double &az)
+{
+    const double dx = p.x - X, dy = p.y - Y, dz = p.z - Z;
+    const double e = s * (Rk + Rp);
+    double r2 = __builtin_fma(dx, dx, __builtin_fma(dy, dy, __builtin_fma(dz, dz, e * e)));
+    r2 = fmax(r2, 1e-200);
+    double y = __builtin_amdgcn_rsq(r2);
+    y = y * __builtin_fma(-0.5 * r2, y * y, 1.5);
+    const double f = (G * p.w) * y * (y * y);
+    ax = __builtin_fma(f, dx, ax);
+    ay = __builtin_fma(f, dy, ay);
+    az = __builtin_fma(f, dz, az);
+}
+
+// Sorted current positions {x, y, z, m}, radius and existence of the binned bodies.
+__global__ __launch_bounds__(MT) void k_gsorted(const DBody *__restrict__ b,
+                                                const int32_t *__restrict__ ex,
+                                                const int *__restrict__ sid, int ins,
+                                                double4 *__restrict__ pm, double *__restrict__ rr,
+                                                int *__restrict__ sx)
+{
+    const int s = blockIdx.x * MT + threadIdx.x;
+    if (s >= ins) return;
+    const int p = sid[s];
+    const DBody q = b[p];
+    pm[s] = make_double4(q.x, q.y, q.z, q.mass);
+    rr[s] = q.radius;
+    sx[s] = (ex == nullptr || ex[p] != 0) && p > 2;
+}
+
+// Per body k: central term, bodies 0..2 (k > 2), neighbour pairs (k > 2,
+// both inserted).  cterm receives the per-body CM_acc terms.
+template <int FAST>
+__global__ __launch_bounds__(MT) void k_mgrav(const DBody *__restrict__ b, int n,
+                                              const int32_t *__restrict__ ex,
+                                              const unsigned *__restrict__ key,
+                                              const unsigned *__restrict__ cs,
+                                              const unsigned *__restrict__ ce,
+                                              const double4 *__restrict__ pm,
+                                              const double *__restrict__ rr,
+                                              const int *__restrict__ sx,
+                                              const int *__restrict__ sid, int d, int cp,
+                                              double cx, double cy, double cz, double mu,
+                                              double G, double s, double *__restrict__ acc,
+                                              double *__restrict__ cterm)
+{
+    const int k = blockIdx.x * MT + threadIdx.x;
+    if (k >= n || (ex && !ex[k])) return;
+    const DBody q = b[k];
+    double ax = 0., ay = 0., az = 0.;
+    if (cp) {
+        const double DX = q.x - cx, DY = q.y - cy, DZ = q.z - cz;
+        const double rk = sqrt(DX * DX + DY * DY + DZ * DZ);
+        const double r3 = rk * rk * rk;
+        ax = -mu * DX / r3;
+        ay = -mu * DY / r3;
+        az = -mu * DZ / r3;
+        cterm[3 * (size_t)k + 0] = G * q.mass * DX / r3;
+        cterm[3 * (size_t)k + 1] = G * q.mass * DY / r3;
+        cterm[3 * (size_t)k + 2] = G * q.mass * DZ / r3;
+    }
+    if (k > 2) {
+        // physics.c:209-260: pairs (0,k), (1,k), (2,k)
+        for (int big = 0; big < 3 && big < n; ++big) {
+            if (ex && !ex[big]) continue;
+            const DBody g = b[big];
+            term_exact(q.x, q.y, q.z, q.radius, make_double4(g.x, g.y, g.z, g.mass), g.radius, G,
+                       s, ax, ay, az);
+        }
+        const unsigned kk = key[k];
+        if (kk != NOKEY) {
+            for (int l = 0; l < 27; ++l) {
+                const unsigned ck = (unsigned)((long)kk + stencil_offset(l, d));
+                const unsigned a = cs[ck], e = ce[ck];
+                for (unsigned t = a; t < e; ++t) {
+                    if (!sx[t] || sid[t] == k) continue;
+                    if (FAST)
+                        term_fast(q.x, q.y, q.z, q.radius, pm[t], rr[t], G, s, ax, ay, az);
+                    else
+                        term_exact(q.x, q.y, q.z, q.radius, pm[t], rr[t], G, s, ax, ay, az);
+                }
+            }
+        }
+    }
+    acc[3 * (size_t)k + 0] = ax;
+    acc[3 * (size_t)k + 1] = ay;
+    acc[3 * (size_t)k + 2] = az;
+}
+
+// Bodies 0..2 receive += G m_k (X_k - X_big)/D^3 from every k > 2
+// (physics.c:221-224, 237-240, 253-256): per-block partial sums.
+__global__ __launch_bounds__(MT) void k_big_partial(const DBody *__restrict__ b, int n,
+                                                    const int32_t *__restrict__ ex, double G,
+                                                    double s, double *__restrict__ part)
+{
+    __shared__ double sh[MT];
+    double a[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int k = 3 + blockIdx.x * MT + threadIdx.x; k < n; k += gridDim.x * MT) {
+        if (ex && !ex[k]) continue;
+        const DBody q = b[k];
+        for (int big = 0; big < 3; ++big) {
+            if (ex && !ex[big]) continue;
+            const DBody g = b[big];
+            const double dx = q.x - g.x, dy = q.y - g.y, dz = q.z - g.z;
+            const double soft = s * (q.radius + g.radius);
+            const double D = sqrt(dx * dx + dy * dy + dz * dz + soft * soft);
+            const double D3 = D * D * D;
+            const double gm = G * q.mass;
+            a[3 * big + 0] += gm * dx / D3;
+            a[3 * big + 1] += gm * dy / D3;
+            a[3 * big + 2] += gm * dz / D3;
+        }
+    }
+    for (int c = 0; c < 9; ++c) {
+        const double r = bsum(a[c], sh);
+        if (threadIdx.x == 0) part[9 * blockIdx.x + c] = r;
+    }
+}
+
+// Bodies 0..2: add the partials, then the pairs (0,1), (0,2), (1,2)
+// (physics.c:300-341); also sums CM_acc.  One block.
+__global__ __launch_bounds__(MT) void k_big_final(const DBody *__restrict__ b, int n,
+                                                  const int32_t *__restrict__ ex,
+                                                  const double *__restrict__ part, int np,
+                                                  double G, double s, double *__restrict__ acc,
+                                                  const double *__restrict__ cterm, int cp,
+                                                  double *__restrict__ cm)
+{
+    __shared__ double sh[MT];
+    double a[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int q = threadIdx.x; q < np; q += MT)
+        for (int c = 0; c < 9; ++c) a[c] += part[9 * q + c];
+    double tot[9];
+    for (int c = 0; c < 9; ++c) tot[c] = bsum(a[c], sh);
+    double cma[3] = {0, 0, 0};
+    if (cp)
+        for (int k = threadIdx.x; k < n; k += MT) {
+            if (ex && !ex[k]) continue;
+            for (int c = 0; c < 3; ++c) cma[c] += cterm[3 * (size_t)k + c];
+        }
+    double cmt[3];
+    for (int c = 0; c < 3; ++c) cmt[c] = bsum(cma[c], sh);
+    if (threadIdx.x != 0) return;
+    if (cp && cm)
+        for (int c = 0; c < 3; ++c) cm[c] = cmt[c];
+    for (int big = 0; big < 3 && big < n; ++big) {
+        if (ex && !ex[big]) continue;
+        for (int c = 0; c < 3; ++c) acc[3 * big + c] += tot[3 * big + c];
+    }
+    const int pa[3] = {0, 0, 1}, pb[3] = {1, 2, 2};
+    for (int q = 0; q < 3; ++q) {
+        const int i = pa[q], j = pb[q];
+        if (j >= n || (ex && (!ex[i] || !ex[j]))) continue;
+        const DBody A = b[i], B = b[j];
+        const double dx = A.x - B.x, dy = A.y - B.y, dz = A.z - B.z;
+        const double soft = s * (A.radius + B.radius);
+        const double D = sqrt(dx * dx + dy * dy + dz * dz + soft * soft);
+        const double D3 = D * D * D;
+        acc[3 * i + 0] -= G * B.mass * dx / D3;
+        acc[3 * i + 1] -= G * B.mass * dy / D3;
+        acc[3 * i + 2] -= G * B.mass * dz / D3;
+        acc[3 * j + 0] += G * A.mass * dx / D3;
+        acc[3 * j + 1] += G * A.mass * dy / D3;
+        acc[3 * j + 2] += G * A.mass * dz / D3;
+    }
+}
+
+// Pairs registered by visits of bodies larger than the mesh size (both > 2),
+// in registration order; one thread.
+__global__ void k_extra_pairs(const DBody *__restrict__ b, const int32_t *__restrict__ ex,
+                              const int2 *__restrict__ pr, int np, double G, double s,
+                              double *__restrict__ acc)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    for (int q = 0; q < np; ++q) {
+        const int k = pr[q].x, p = pr[q].y;
+        if (ex && (!ex[k] || !ex[p])) continue;
+        const DBody A = b[k], B = b[p];
+        const double dx = A.x - B.x, dy = A.y - B.y, dz = A.z - B.z;
+        const double soft = s * (A.radius + B.radius);
+        const double D = sqrt(dx * dx + dy * dy + dz * dz + soft * soft);
+        const double D3 = D * D * D;
+        acc[3 * k + 0] -= G * B.mass * dx / D3;
+        acc[3 * k + 1] -= G * B.mass * dy / D3;
+        acc[3 * k + 2] -= G * B.mass * dz / D3;
+        acc[3 * p + 0] += G * A.mass * dx / D3;
+        acc[3 * p + 1] += G * A.mass * dy / D3;
+        acc[3 * p + 2] += G * A.mass * dz / D3;
+    }
+}
+
+// Registered pairs of the small bodies, written at their offsets in the
+// reference's order (k ascending; cells 26..0; p ascending).
+__global__ __launch_bounds__(MT) void k_mpairs(const unsigned *__restrict__ skey,
+                                               const int *__restrict__ sid,
+                                               const unsigned *__restrict__ cs,
+                                               const unsigned *__restrict__ ce, int ins, int d,
+                                               const long long *__restrict__ off,
+                                               int2 *__restrict__ out)
+{
+    const int s = blockIdx.x * MT + threadIdx.x;
+    if (s >= ins) return;
+    const int k = sid[s];
+    const long key = skey[s];
+    long long o = off[k];
+    for (int l = 26; l >= 0; --l) {
+        const unsigned ck = (unsigned)(key + stencil_offset(l, d));
+        for (unsigned t = cs[ck], e = ce[ck]; t < e; ++t) {
+            const int p = sid[t];
+            if (p >= k) break;
+            out[o++] = make_int2(k, p);
+        }
+    }
+}
+
+}  // namespace
+
+// ---- host side --------------------------------------------------------------
+
+namespace {
+
+struct HostCells {
+    std::vector<unsigned> key;   // per body
+    std::vector<unsigned> skey;  // per sorted position
+    std::vector<int> sid;
+    bool loaded = false;
+    long d = 0;
+    // Ids of cell `ck`, ascending.
+    std::pair<size_t, size_t> range(unsigned ck) const
+    {
+        auto lo = std::lower_bound(skey.begin(), skey.end(), ck);
+        auto hi = std::upper_bound(lo, skey.end(), ck);
+        return {(size_t)(lo - skey.begin()), (size_t)(hi - skey.begin())};
+    }
+    int stencil_index(long delta) const
+    {
+        for (int l = 0; l < 27; ++l)
+            if (stencil_offset(l, d) == delta) return l;
+        return -1;
+    }
+};
+
+ncg_status load_cells(ncg_ctx *ctx, HostCells &hc, int32_t n, int ins, const unsigned *dkey,
+                      const unsigned *dskey, const int *dsid)
+{
+    if (hc.loaded) return NCG_OK;
+    hc.key.resize(n);
+    hc.skey.resize(ins);
+    hc.sid.resize(ins);
+    hipStream_t st = ctx->stream;
+    NCG_HIP(hipMemcpyAsync(hc.key.data(), dkey, sizeof(unsigned) * n, hipMemcpyDeviceToHost, st));
+    if (ins) {
+        NCG_HIP(hipMemcpyAsync(hc.skey.data(), dskey, sizeof(unsigned) * ins, hipMemcpyDeviceToHost, st));
+        NCG_HIP(hipMemcpyAsync(hc.sid.data(), dsid, sizeof(int) * ins, hipMemcpyDeviceToHost, st));
+    }
+    NCG_HIP(hipStreamSynchronize(st));
+    hc.loaded = true;
+    return NCG_OK;
+}
+
+}  // namespace
+
+extern "C" ncg_status ncg_mesh_pass(ncg_ctx *ctx, const ncg_body *bodies, int32_t n,
+                                    const int32_t *exists, double timestep, double gam,
+                                    double collision_cube, int32_t cells, int32_t mode,
+                                    const ncg_replay_ops *ops, ncg_mesh_stats *stats)
 {
     if (!ctx) return NCG_ERR_INVALID;
-    return ncg_fail(ctx, NCG_ERR_UNSUPPORTED, "mesh pairs not built yet");
+    NCG_REQUIRE(n >= 0 && (n == 0 || bodies) && (mode == 0 || mode == 1) && stats,
+                "invalid arguments");
+    NCG_REQUIRE(mode == 1 || (ops && ops->test && ops->resolve), "mesh() needs replay ops");
+    NCG_REQUIRE(cells >= 2 && gam > 0.0 && collision_cube > 0.0, "invalid mesh geometry");
+    const long d = cells;
+    const long long nkeys = (long long)(d + 2) * (1 + d + d * d) + 1;
+    NCG_REQUIRE(nkeys < 0xffffffffLL, "collision_cube_cells %d too large for 32-bit cell keys",
+                (int)cells);
+    memset(stats, 0, sizeof *stats);
+    NCG_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    MeshState &M = ctx->mesh;
+    M.valid = false;
+    M.extra_pairs.clear();
+    M.big_visits.clear();
+    M.collided.clear();
+    if (n == 0) {
+        M.valid = true;
+        M.n = 0;
+        M.inserted = 0;
+        return NCG_OK;
+    }
+
+    DBody *db = dbuf<DBody>(ctx, "m_b", n);
+    int32_t *dex = exists ? dbuf<int32_t>(ctx, "m_ex", n) : nullptr;
+    unsigned *key = dbuf<unsigned>(ctx, "m_key", n);
+    int *idx = dbuf<int>(ctx, "m_idx", n);
+    unsigned *skey = dbuf<unsigned>(ctx, "m_skey", n);
+    int *sid = dbuf<int>(ctx, "m_sid", n);
+    int *cbi = dbuf<int>(ctx, "m_count", n);
+    double4 *pr = dbuf<double4>(ctx, "m_pr", n);
+    double4 *vm = dbuf<double4>(ctx, "m_vm", n);
+    int *cnt = dbuf<int>(ctx, "m_cnt", 4);
+    unsigned long long *tot = dbuf<unsigned long long>(ctx, "m_tot", 2);
+    const int big_cap = 1 << 16;
+    int *bigl = dbuf<int>(ctx, "m_big", big_cap);
+    if (!db || (exists && !dex) || !key || !idx || !skey || !sid || !cbi || !pr || !vm || !cnt ||
+        !tot || !bigl)
+        return NCG_ERR_NOMEM;
+    // Dense cell table; cleared once when (re)allocated, afterwards only the
+    // keys of the previous pass are reset.
+    if (M.table_keys < nkeys) {
+        M.table_keys = 0;
+        M.prev_ins = 0;
+        unsigned *cs0 = dbuf<unsigned>(ctx, "m_cs", nkeys);
+        unsigned *ce0 = dbuf<unsigned>(ctx, "m_ce", nkeys);
+        if (!cs0 || !ce0) return NCG_ERR_NOMEM;
+        NCG_HIP(hipMemsetAsync(cs0, 0, sizeof(unsigned) * nkeys, st));
+        NCG_HIP(hipMemsetAsync(ce0, 0, sizeof(unsigned) * nkeys, st));
+        M.table_keys = nkeys;
+    }
+    unsigned *cs = dbuf<unsigned>(ctx, "m_cs", nkeys);
+    unsigned *ce = dbuf<unsigned>(ctx, "m_ce", nkeys);
+    unsigned *prev = dbuf<unsigned>(ctx, "m_prev", n);
+    if (!prev) return NCG_ERR_NOMEM;
+    if (M.prev_ins > 0) {
+        hipLaunchKernelGGL(k_clear_cells, dim3(cdiv(M.prev_ins, MT)), dim3(MT), 0, st,
+                           (const unsigned *)dbuf<unsigned>(ctx, "m_prevkeys", M.prev_ins),
+                           M.prev_ins, cs, ce);
+        NCG_CHECK_LAUNCH();
+        M.prev_ins = 0;
+    }
+    NCG_HIP(hipMemcpyAsync(db, bodies, sizeof(DBody) * (size_t)n, hipMemcpyHostToDevice, st));
+    if (exists)
+        NCG_HIP(hipMemcpyAsync(dex, exists, sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, st));
+    NCG_HIP(hipMemsetAsync(cnt, 0, sizeof(int) * 4, st));
+    NCG_HIP(hipMemsetAsync(tot, 0, sizeof(unsigned long long) * 2, st));
+    {
+        ProfScope ps(ctx, st, "mesh_bin");
+        hipLaunchKernelGGL(k_mkey, dim3(cdiv(n, MT)), dim3(MT), 0, st, db, (int)n, dex, gam,
+                           collision_cube, (int)d, (int)mode, key, idx, cnt, bigl, big_cap);
+        NCG_CHECK_LAUNCH();
+    }
+    int hc4[4];
+    NCG_HIP(hipMemcpyAsync(hc4, cnt, sizeof hc4, hipMemcpyDeviceToHost, st));
+    size_t tmp_bytes = 0;
+    int end_bit = 1;
+    while (end_bit < 32 && (1ull << end_bit) < (unsigned long long)nkeys) ++end_bit;
+    NCG_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, key, skey, idx, sid, n, 0, 32, st));
+    void *tmp = ncg_dbuf(ctx, "m_sorttmp", tmp_bytes);
+    if (!tmp) return NCG_ERR_NOMEM;
+    // NOKEY (all ones) sorts last whatever end_bit, as long as every real key
+    // is below 2^end_bit: sort on all 32 bits to keep that simple.
+    NCG_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, key, skey, idx, sid, n, 0, 32, st));
+    NCG_HIP(hipStreamSynchronize(st));
+    const int ins = hc4[0];
+    const int nbig = std::min(hc4[3], big_cap);
+    NCG_REQUIRE(hc4[3] <= big_cap, "more than %d bodies larger than the mesh size", big_cap);
+    if (ins > 0) {
+        hipLaunchKernelGGL(k_bounds, dim3(cdiv(ins, MT)), dim3(MT), 0, st, skey, ins, cs, ce);
+        hipLaunchKernelGGL(k_msorted, dim3(cdiv(ins, MT)), dim3(MT), 0, st, db, sid, ins, pr, vm);
+        NCG_CHECK_LAUNCH();
+    }
+    // Remember the keys of this pass to clear them next time.
+    unsigned *pk = dbuf<unsigned>(ctx, "m_prevkeys", std::max(ins, 1));
+    if (!pk) return NCG_ERR_NOMEM;
+    if (ins) NCG_HIP(hipMemcpyAsync(pk, skey, sizeof(unsigned) * ins, hipMemcpyDeviceToDevice, st));
+    M.prev_ins = ins;
+
+    std::vector<int4> cand;
+    unsigned long long cap = 1 << 16;
+    unsigned long long total = 0;
+    for (int attempt = 0; attempt < 2 && ins > 0; ++attempt) {
+        int4 *dc = dbuf<int4>(ctx, "m_cand", cap);
+        if (!dc) return NCG_ERR_NOMEM;
+        NCG_HIP(hipMemsetAsync(tot, 0, sizeof(unsigned long long) * 2, st));
+        {
+            ProfScope ps(ctx, st, "mesh_count");
+            if (mode == 0)
+                hipLaunchKernelGGL(k_mcount<0>, dim3(cdiv(ins, MT)), dim3(MT), 0, st, skey, sid, pr,
+                                   vm, cs, ce, ins, (int)d, timestep, cbi, dc, cap, tot + 1, tot);
+            else
+                hipLaunchKernelGGL(k_mcount<1>, dim3(cdiv(ins, MT)), dim3(MT), 0, st, skey, sid, pr,
+                                   vm, cs, ce, ins, (int)d, timestep, cbi, dc, cap, tot + 1, tot);
+            NCG_CHECK_LAUNCH();
+        }
+        unsigned long long ht[2];
+        NCG_HIP(hipMemcpyAsync(ht, tot, sizeof ht, hipMemcpyDeviceToHost, st));
+        NCG_HIP(hipStreamSynchronize(st));
+        total = ht[0];
+        if (ht[1] <= cap) {
+            cand.resize(ht[1]);
+            if (ht[1]) {
+                NCG_HIP(hipMemcpyAsync(cand.data(), dc, sizeof(int4) * ht[1], hipMemcpyDeviceToHost, st));
+                NCG_HIP(hipStreamSynchronize(st));
+            }
+            break;
+        }
+        cap = ht[1];
+    }
+    std::vector<int> big(nbig);
+    if (nbig) {
+        NCG_HIP(hipMemcpyAsync(big.data(), bigl, sizeof(int) * nbig, hipMemcpyDeviceToHost, st));
+        NCG_HIP(hipStreamSynchronize(st));
+        std::sort(big.begin(), big.end());
+    }
+    stats->total_neighbours = (int64_t)total;
+    stats->how_many_small = mode == 0 ? hc4[1] : 0;
+    stats->how_many_big = mode == 0 ? hc4[2] : 0;
+    stats->candidates = (int64_t)cand.size();
+    M.n = n;
+    M.cells = cells;
+    M.mode = mode;
+    M.inserted = ins;
+    M.gam = gam;
+    M.cube = collision_cube;
+    M.valid = true;
+    if (mode == 1) return NCG_OK;
+
+    // ---- host replay of collision.c:339-415 ----------------------------------
+    // Candidates in visiting order: k ascending, cell 26..0, p ascending.
+    std::sort(cand.begin(), cand.end(), [](const int4 &a, const int4 &b) {
+        if (a.x != b.x) return a.x < b.x;
+        if (a.y != b.y) return a.y > b.y;
+        return a.z < b.z;
+    });
+    HostCells hc;
+    hc.d = d;
+    std::vector<char> dirty((size_t)n, 0);
+    std::map<int, std::vector<int>> pending;  // visit k -> dirty neighbours p < k
+    int64_t ncoll = 0, htests = 0;
+    auto live = [&](int32_t k) { return !exists || exists[k] != 0; };
+    ncg_status err = NCG_OK;
+    // Body b was modified during visit kv: later visits that have b as a
+    // neighbour must re-test it on the host.
+    auto mark_dirty = [&](int b, int kv) -> ncg_status {
+        if (dirty[b]) return NCG_OK;
+        dirty[b] = 1;
+        ncg_status r = load_cells(ctx, hc, n, ins, key, skey, sid);
+        if (r != NCG_OK) return r;
+        const unsigned kb = hc.key[b];
+        if (kb == NOKEY) return NCG_OK;  // not in the hash table: nobody's neighbour
+        for (int l = 0; l < 27; ++l) {
+            const auto rg = hc.range((unsigned)((long)kb + stencil_offset(l, d)));
+            for (size_t t = rg.first; t < rg.second; ++t) {
+                const int q = hc.sid[t];
+                if (q > kv && q > b) pending[q].push_back(b);
+            }
+        }
+        return NCG_OK;
+    };
+    auto collide = [&](int p, int k, int kv) -> ncg_status {
+        ops->resolve(ops->user, p, k);
+        ++ncoll;
+        M.collided.push_back({p, k});
+        ncg_status r = mark_dirty(p, kv);
+        if (r != NCG_OK) return r;
+        return mark_dirty(k, kv);
+    };
+    // Remaining pairs of visit k from traversal position `from` (cell order
+    // key (26 - l), then id), all decided on the host.
+    auto full_rest = [&](int k, int from_l, int from_p) -> ncg_status {
+        ncg_status r = load_cells(ctx, hc, n, ins, key, skey, sid);
+        if (r != NCG_OK) return r;
+        const unsigned kk = hc.key[k];
+        for (int l = from_l; l >= 0; --l) {
+            const auto rg = hc.range((unsigned)((long)kk + stencil_offset(l, d)));
+            for (size_t t = rg.first; t < rg.second; ++t) {
+                const int p = hc.sid[t];
+                if (p >= k) break;
+                if (l == from_l && p < from_p) continue;
+                ++htests;
+                if (ops->test(ops->user, p, k)) {
+                    r = collide(p, k, k);
+                    if (r != NCG_OK) return r;
+                }
+            }
+        }
+        return NCG_OK;
+    };
+    size_t ci = 0, bi = 0;
+    while (true) {
+        const int kc = ci < cand.size() ? cand[ci].x : INT32_MAX;
+        const int kb = bi < big.size() ? big[bi] : INT32_MAX;
+        const int kp = pending.empty() ? INT32_MAX : pending.begin()->first;
+        const int k = std::min(kc, std::min(kb, kp));
+        if (k == INT32_MAX) break;
+        if (k == kb) {
+            // Body larger than the mesh size (collision.c:381-414).
+            ++bi;
+            if (!live(k)) continue;
+            std::vector<ncg_pair> visit;
+            for (int32_t p = 0; p < n; ++p) {
+                if (!(live(p) && p != k)) continue;
+                const double Rp = bodies[p].radius;
+                if (!(Rp < gam || (Rp >= gam && p > k))) continue;
+                ++htests;
+                visit.push_back({k, p});
+                if (ops->test(ops->user, p, k)) {
+                    if (p > k && Rp <= gam && !dirty[p]) {
+                        ncg_status r = load_cells(ctx, hc, n, ins, key, skey, sid);
+                        if (r != NCG_OK) return r;
+                        if (hc.key[p] != NOKEY)
+                            return ncg_fail(ctx, NCG_ERR_UNSUPPORTED,
+                                            "body %d (larger than the mesh size) collides with "
+                                            "body %d before it is binned",
+                                            k, p);
+                    }
+                    ncg_status r = collide(p, k, k);
+                    if (r != NCG_OK) return r;
+                    visit.back() = {p, k};
+                }
+            }
+            M.big_visits.push_back({k, (int)M.big_pairs_all.size(), (int)visit.size()});
+            for (const ncg_pair &pq : visit) {
+                M.big_pairs_all.push_back(pq);
+                if (pq.fst > 2 && pq.snd > 2) M.extra_pairs.push_back(pq);
+            }
+            continue;
+        }
+        // Small body k: merge candidates and pending re-tests in traversal order.
+        std::vector<std::pair<long, int>> todo;  // ((26 - l) * 2^31 + p, p)
+        while (ci < cand.size() && cand[ci].x == k) {
+            todo.push_back({(long)(26 - cand[ci].y) * (1L << 31) + cand[ci].z, cand[ci].z});
+            ++ci;
+        }
+        auto it = pending.find(k);
+        if (it != pending.end()) {
+            ncg_status r = load_cells(ctx, hc, n, ins, key, skey, sid);
+            if (r != NCG_OK) return r;
+            for (int p : it->second) {
+                const int l = hc.stencil_index((long)hc.key[p] - (long)hc.key[k]);
+                if (l >= 0) todo.push_back({(long)(26 - l) * (1L << 31) + p, p});
+            }
+            pending.erase(it);
+        }
+        std::sort(todo.begin(), todo.end());
+        todo.erase(std::unique(todo.begin(), todo.end()), todo.end());
+        for (size_t q = 0; q < todo.size(); ++q) {
+            const int p = todo[q].second;
+            if (dirty[k]) {
+                // k changed earlier in this visit: every remaining pair on the host.
+                const int l = 26 - (int)(todo[q].first >> 31);
+                err = full_rest(k, l, p);
+                if (err != NCG_OK) return err;
+                break;
+            }
+            ++htests;
+            if (ops->test(ops->user, p, k)) {
+                err = collide(p, k, k);
+                if (err != NCG_OK) return err;
+                // The rest of this visit involves k, which changed: host.
+                const int l = 26 - (int)(todo[q].first >> 31);
+                err = full_rest(k, l, p + 1);
+                if (err != NCG_OK) return err;
+                break;
+            }
+        }
+    }
+    stats->collisions = ncoll;
+    stats->host_tests = htests;
+    return NCG_OK;
+}
+
+extern "C" ncg_status ncg_mesh_gravity(ncg_ctx *ctx, const ncg_body *bodies, int32_t n,
+                                       const int32_t *exists, const ncg_central *central,
+                                       double grav_const, double softening, ncg_precision prec,
+                                       double *acc, double *cm_acc)
+{
+    if (!ctx) return NCG_ERR_INVALID;
+    NCG_REQUIRE(n >= 0 && (n == 0 || (bodies && acc)), "invalid arguments");
+    NCG_REQUIRE(prec == NCG_F64_EXACT || prec == NCG_F64_FAST || prec == NCG_MIXED,
+                "unknown precision %d", (int)prec);
+    MeshState &M = ctx->mesh;
+    if (!M.valid || M.n != n)
+        return ncg_fail(ctx, NCG_ERR_INVALID,
+                        "ncg_mesh_gravity needs a ncg_mesh_pass over the same %d body slots", n);
+    if (n == 0) return NCG_OK;
+    NCG_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    const long d = M.cells;
+    const int ins = M.inserted;
+    const int cp = (central && central->present) ? 1 : 0;
+    DBody *db = dbuf<DBody>(ctx, "g_b", n);
+    int32_t *dex = exists ? dbuf<int32_t>(ctx, "g_ex", n) : nullptr;
+    double *dacc = dbuf<double>(ctx, "g_acc", 3 * (size_t)n);
+    double *dct = dbuf<double>(ctx, "g_cterm", 3 * (size_t)n);
+    double4 *pm = dbuf<double4>(ctx, "g_pm", std::max(ins, 1));
+    double *rr = dbuf<double>(ctx, "g_rr", std::max(ins, 1));
+    int *sx = dbuf<int>(ctx, "g_sx", std::max(ins, 1));
+    double *dcm = dbuf<double>(ctx, "g_cm", 4);
+    const int nbp = std::max(1, std::min(512, (int)cdiv(n, MT)));
+    double *bpart = dbuf<double>(ctx, "g_bpart", 9 * (size_t)nbp);
+    const size_t nex = std::max<size_t>(M.extra_pairs.size(), 1);
+    int2 *dextra = dbuf<int2>(ctx, "g_extra", nex);
+    if (!db || (exists && !dex) || !dacc || !dct || !pm || !rr || !sx || !dcm || !bpart || !dextra)
+        return NCG_ERR_NOMEM;
+    const unsigned *key = dbuf<unsigned>(ctx, "m_key", n);
+    const int *sid = dbuf<int>(ctx, "m_sid", n);
+    const unsigned *cs = dbuf<unsigned>(ctx, "m_cs", M.table_keys);
+    const unsigned *ce = dbuf<unsigned>(ctx, "m_ce", M.table_keys);
+    NCG_HIP(hipMemcpyAsync(db, bodies, sizeof(DBody) * (size_t)n, hipMemcpyHostToDevice, st));
+    if (exists)
+        NCG_HIP(hipMemcpyAsync(dex, exists, sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, st));
+    if (!M.extra_pairs.empty())
+        NCG_HIP(hipMemcpyAsync(dextra, M.extra_pairs.data(), sizeof(int2) * M.extra_pairs.size(),
+                               hipMemcpyHostToDevice, st));
+    if (ins > 0) {
+        hipLaunchKernelGGL(k_gsorted, dim3(cdiv(ins, MT)), dim3(MT), 0, st, db, dex, sid, ins, pm,
+                           rr, sx);
+        NCG_CHECK_LAUNCH();
+    }
+    const double G = grav_const, s = softening;
+    const double mu = cp ? G * central->mass : 0.0;
+    {
+        ProfScope ps(ctx, st, "mesh_gravity");
+        if (prec == NCG_F64_EXACT)
+            hipLaunchKernelGGL(k_mgrav<0>, dim3(cdiv(n, MT)), dim3(MT), 0, st, db, (int)n, dex, key,
+                               cs, ce, pm, rr, sx, sid, (int)d, cp, cp ? central->x : 0.0,
+                               cp ? central->y : 0.0, cp ? central->z : 0.0, mu, G, s, dacc, dct);
+        else
+            hipLaunchKernelGGL(k_mgrav<1>, dim3(cdiv(n, MT)), dim3(MT), 0, st, db, (int)n, dex, key,
+                               cs, ce, pm, rr, sx, sid, (int)d, cp, cp ? central->x : 0.0,
+                               cp ? central->y : 0.0, cp ? central->z : 0.0, mu, G, s, dacc, dct);
+        NCG_CHECK_LAUNCH();
+    }
+    hipLaunchKernelGGL(k_big_partial, dim3(nbp), dim3(MT), 0, st, db, (int)n, dex, G, s, bpart);
+    hipLaunchKernelGGL(k_big_final, dim3(1), dim3(MT), 0, st, db, (int)n, dex, bpart, nbp, G, s,
+                       dacc, dct, cp, dcm);
+    if (!M.extra_pairs.empty())
+        hipLaunchKernelGGL(k_extra_pairs, dim3(1), dim3(1), 0, st, db, dex, dextra,
+                           (int)M.extra_pairs.size(), G, s, dacc);
+    NCG_CHECK_LAUNCH();
+    double *hacc = hbuf<double>(ctx, "macc", 3 * (size_t)n);
+    double hcm[4] = {0, 0, 0, 0};
+    NCG_HIP(hipMemcpyAsync(hacc, dacc, sizeof(double) * 3 * (size_t)n, hipMemcpyDeviceToHost, st));
+    if (cp) NCG_HIP(hipMemcpyAsync(hcm, dcm, sizeof(double) * 3, hipMemcpyDeviceToHost, st));
+    NCG_HIP(hipStreamSynchronize(st));
+    for (int32_t k = 0; k < n; ++k) {
+        if (exists && !exists[k]) continue;
+        acc[3 * (size_t)k] = hacc[3 * (size_t)k];
+        acc[3 * (size_t)k + 1] = hacc[3 * (size_t)k + 1];
+        acc[3 * (size_t)k + 2] = hacc[3 * (size_t)k + 2];
+    }
+    if (cp && cm_acc) {
+        cm_acc[0] = hcm[0];
+        cm_acc[1] = hcm[1];
+        cm_acc[2] = hcm[2];
+    }
+    return NCG_OK;
+}
+
+extern "C" ncg_status ncg_mesh_pairs(ncg_ctx *ctx, ncg_pair *out, int64_t capacity, int64_t *count)
+{
+    if (!ctx) return NCG_ERR_INVALID;
+    NCG_REQUIRE(count && capacity >= 0 && (capacity == 0 || out), "invalid arguments");
+    MeshState &M = ctx->mesh;
+    NCG_REQUIRE(M.valid, "no mesh pass");
+    *count = 0;
+    const int32_t n = M.n;
+    if (n == 0) return NCG_OK;
+    NCG_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    const int ins = M.inserted;
+    const long d = M.cells;
+    // Per-body pair counts: small bodies from the GPU count, big bodies from
+    // the host visits; offsets in visiting order.
+    std::vector<int> cnt(n, 0);
+    std::vector<unsigned> hkey(n);
+    const int *cbi = dbuf<int>(ctx, "m_count", n);
+    const unsigned *key = dbuf<unsigned>(ctx, "m_key", n);
+    NCG_HIP(hipMemcpyAsync(cnt.data(), cbi, sizeof(int) * n, hipMemcpyDeviceToHost, st));
+    NCG_HIP(hipMemcpyAsync(hkey.data(), key, sizeof(unsigned) * n, hipMemcpyDeviceToHost, st));
+    NCG_HIP(hipStreamSynchronize(st));
+    for (int32_t k = 0; k < n; ++k)
+        if (hkey[k] == NOKEY) cnt[k] = 0;
+    for (const auto &bv : M.big_visits) cnt[bv.k] = bv.count;
+    std::vector<long long> off(n);
+    long long tot = 0;
+    for (int32_t k = 0; k < n; ++k) {
+        off[k] = tot;
+        tot += cnt[k];
+    }
+    *count = tot;
+    if (tot > capacity)
+        return ncg_fail(ctx, NCG_ERR_CAPACITY, "%lld pairs exceed capacity %lld", tot,
+                        (long long)capacity);
+    if (tot == 0) return NCG_OK;
+    long long *doff = dbuf<long long>(ctx, "p_off", n);
+    int2 *dout = dbuf<int2>(ctx, "p_out", (size_t)tot);
+    if (!doff || !dout) return NCG_ERR_NOMEM;
+    NCG_HIP(hipMemcpyAsync(doff, off.data(), sizeof(long long) * n, hipMemcpyHostToDevice, st));
+    if (ins > 0) {
+        hipLaunchKernelGGL(k_mpairs, dim3(cdiv(ins, MT)), dim3(MT), 0, st,
+                           (const unsigned *)dbuf<unsigned>(ctx, "m_skey", n),
+                           (const int *)dbuf<int>(ctx, "m_sid", n),
+                           (const unsigned *)dbuf<unsigned>(ctx, "m_cs", M.table_keys),
+                           (const unsigned *)dbuf<unsigned>(ctx, "m_ce", M.table_keys), ins,
+                           (int)d, doff, dout);
+        NCG_CHECK_LAUNCH();
+    }
+    NCG_HIP(hipMemcpyAsync(out, dout, sizeof(int2) * (size_t)tot, hipMemcpyDeviceToHost, st));
+    NCG_HIP(hipStreamSynchronize(st));
+    for (const auto &bv : M.big_visits)
+        for (int q = 0; q < bv.count; ++q) out[off[bv.k] + q] = M.big_pairs_all[bv.first + q];
+    // A collision registers (p, k) instead of (k, p) (physics.c:1065-1080).
+    for (const ncg_pair &c : M.collided) {
+        const int p = c.fst, k = c.snd;
+        for (long long q = off[k]; q < off[k] + cnt[k]; ++q)
+            if (out[q].fst == k && out[q].snd == p) {
+                out[q].fst = p;
+                out[q].snd = k;
+                break;
+            }
+    }
+    return NCG_OK;
 }

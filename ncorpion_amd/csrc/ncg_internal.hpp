@@ -23,17 +23,26 @@ struct DevBuf {
     size_t bytes = 0;
 };
 
+// Visit of a body larger than the mesh size: its registered pairs are
+// big_pairs_all[first, first + count).
+struct BigVisit {
+    int k, first, count;
+};
+
 // Mesh binning kept between ncg_mesh_pass and ncg_mesh_gravity.
 struct MeshState {
     bool valid = false;
     int32_t n = 0;           // body slots of the pass
     int32_t cells = 0;       // collision_cube_cells
-    int64_t nkeys = 0;       // dense key range of the start/end tables
+    int64_t table_keys = 0;  // allocated (and zeroed) entries of the cell tables
+    int32_t prev_ins = 0;    // keys of the previous pass still set in the tables
     int32_t mode = 0;        // 0 = mesh(), 1 = get_neighbours_mesh()
     int32_t inserted = 0;    // bodies inserted in the hash
     double gam = 0.0, cube = 0.0;
-    bool big_pairs = false;  // pass registered pairs of bodies larger than gam
-    std::vector<ncg_pair> extra_pairs;  // big-body pairs (host visit order)
+    std::vector<ncg_pair> extra_pairs;    // big-body pairs with both ids > 2
+    std::vector<ncg_pair> big_pairs_all;  // every pair registered by big visits
+    std::vector<BigVisit> big_visits;
+    std::vector<ncg_pair> collided;       // (p, k) in resolution order
 };
 
 struct ProfRec {

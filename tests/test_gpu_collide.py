@@ -52,7 +52,8 @@ def test_candidates_superset_random(ctx, seed, dt):
                 hits.add((i, j))
     assert hits <= cand
     assert len(cand - hits) <= 2  # only last-bit boundary cases may be extra
-    assert len(hits) > 10
+    if dt > 0:
+        assert len(hits) > 10
 
 
 @pytest.mark.parametrize("name", ["bf_inelastic_n300", "bf_elastic_once_n200",

@@ -1,0 +1,112 @@
+"""GPU mesh module (libncg.so) against the compiled reference and the oracle.
+
+mesh() / get_neighbours_mesh(): collision sequences, total_neighbours,
+how_many_small/big, final states and the registered pair list must be
+identical; the neighbour-cell gravity must match vector_field's mesh branch.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as orc
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def load(name):
+    return dict(np.load(os.path.join(GOLD, name + ".npz")))
+
+
+def bits(a):
+    return np.ascontiguousarray(a, dtype=np.float64).view(np.uint64)
+
+
+def run_mesh(ctx, g):
+    r = orc.Replay(g["bodies"], float(g["dt"]), int(g["ctype"]), float(g["f"]), int(g["one_coll"]),
+                   g.get("exists"))
+    st = ctx.mesh_pass(r.b, r.e if "exists" in g else None, float(g["dt"]), float(g["gam"]),
+                       float(g["cube"]), int(g["cells"]), 0, r.test, r.resolve)
+    return r, st
+
+
+@pytest.mark.parametrize("name", ["mesh_inelastic_n1500", "mesh_elastic_once_n1200"])
+def test_mesh_pass_matches_reference(ctx, name):
+    g = load(name)
+    r, st = run_mesh(ctx, g)
+    assert np.array_equal(np.array(r.log, dtype=np.int32).reshape(-1, 2), g["coll"])
+    assert st["total_neighbours"] == int(g["total_neighbours"])
+    assert st["how_many_small"] == int(g["how_many_small"])
+    assert st["how_many_big"] == int(g["how_many_big"])
+    assert np.array_equal(bits(r.b), bits(g["out_bodies"]))
+    pairs = ctx.mesh_pairs()
+    assert np.array_equal(pairs, g["pairs"])
+    assert len(g["coll"]) > 20
+
+
+def test_get_neighbours_mesh_matches_reference(ctx):
+    g = load("gnm_n1500")
+    b = np.array(g["bodies"])
+    st = ctx.mesh_pass(b, g["exists"], 0.0, float(g["gam"]), float(g["cube"]), int(g["cells"]), 1)
+    assert st["total_neighbours"] == int(g["total_neighbours"])
+    assert np.array_equal(ctx.mesh_pairs(), g["pairs"])
+
+
+def test_mesh_gravity_matches_reference(ctx):
+    # Binning of get_neighbours_mesh at the old positions, field at the new ones.
+    gn = load("gnm_n1500")
+    g = load("vf_mesh_n1500")
+    b = np.array(gn["bodies"])
+    ctx.mesh_pass(b, gn["exists"], 0.0, float(gn["gam"]), float(gn["cube"]), int(gn["cells"]), 1)
+    c8 = g["central"]
+    cen = (c8[0], c8[1], c8[2], c8[6])
+    ex = g["exists"].astype(bool)
+    for prec, tol in (("exact", 1e-13), ("fast64", 1e-12), ("mixed", 1e-12)):
+        acc, cm = ctx.mesh_gravity(g["bodies"], g["exists"], cen, float(g["G"]), float(g["soft"]),
+                                   prec)
+        ref = g["acc"]
+        err = np.abs(acc[ex] - ref[ex]).max() / np.abs(ref[ex]).max()
+        assert err < tol, (prec, err)
+        assert np.allclose(cm, g["cm_acc"], rtol=1e-12, atol=0)
+    # Mutual part alone against the oracle with the reference pair list.
+    accn, _ = ctx.mesh_gravity(g["bodies"], g["exists"], None, float(g["G"]), float(g["soft"]),
+                               "exact")
+    refn, _ = orc.vector_field_mesh(g["bodies"], g["pairs"], g["exists"], None, float(g["G"]),
+                                    float(g["soft"]))
+    d = np.linalg.norm(accn[ex] - refn[ex], axis=1)
+    assert np.max(d / np.linalg.norm(refn[ex], axis=1)) < 1e-12
+
+
+def blob(n, seed, half=0.07, rmin=5e-4, rmax=2e-3, vs=0.3):
+    rng = np.random.default_rng(seed)
+    b = np.zeros((n, 8))
+    b[:, 0:3] = rng.uniform(-half, half, (n, 3))
+    b[:, 3:6] = rng.normal(0, vs, (n, 3))
+    b[:, 6] = rng.uniform(1e-9, 1e-8, n)
+    b[:, 7] = rng.uniform(rmin, rmax, n)
+    return b
+
+
+@pytest.mark.parametrize("seed,cells,gam", [(1, 32, 0.004), (2, 64, 0.0025), (3, 16, 0.01)])
+def test_mesh_pass_matches_oracle_random(ctx, seed, cells, gam):
+    b = blob(6000, seed)
+    cube = cells * gam
+    ex = (np.random.default_rng(seed).uniform(size=6000) > 0.05).astype(np.int32)
+    ref = orc.mesh(b, 0.01, gam, cube, cells, 1, 1.5, 0, 1, ex)
+    r = orc.Replay(b, 0.01, 1, 1.5, 0, ex)
+    st = ctx.mesh_pass(r.b, r.e, 0.01, gam, cube, cells, 0, r.test, r.resolve)
+    assert np.array_equal(np.array(r.log, dtype=np.int32).reshape(-1, 2), ref["collisions"])
+    assert st["total_neighbours"] == ref["total_neighbours"]
+    assert st["how_many_small"] == ref["how_many_small"]
+    assert np.array_equal(bits(r.b), bits(ref["bodies"]))
+    assert np.array_equal(ctx.mesh_pairs(), ref["pairs"])
+
+
+def test_mesh_pass_empty_and_outside(ctx):
+    b = blob(50, 4, half=1.0)  # most bodies outside a small cube
+    st = ctx.mesh_pass(b, None, 0.01, 0.01, 0.16, 16, 1)
+    pairs, tn = orc.get_neighbours_mesh(b, 0.01, 0.16, 16)
+    assert st["total_neighbours"] == tn
+    st = ctx.mesh_pass(np.zeros((0, 8)), None, 0.01, 0.01, 0.16, 16, 1)
+    assert st["total_neighbours"] == 0
