@@ -132,13 +132,17 @@ CONFIGS = {
     # Same, denser (larger radii in init.txt) and shorter, for many collisions.
     "run_c1_dense": ("run", _run_cfg("brute", 2000, 40, DT1)),
     # Mesh, 16384 bodies, cells of side 0.01 (256 per axis), 10 steps of 2 pi/512.
+    # The initial mesh size comes from sma_min/sma_max/inclination_max
+    # (structure.c:517), so those are set to the disk's range.
     "run_mesh": ("run", _run_cfg("mesh", 16384, 10, DT2, {
         "collision_cube_cells": "256", "collision_cube_min": "2.56",
-        "how_many_neighbours": "100.0", "switch_to_brute_force": "110"})),
+        "how_many_neighbours": "100.0", "switch_to_brute_force": "110",
+        "sma_min": "0.9", "sma_max": "1.1", "inclination_max": "0.01"})),
     # Full-size mesh disk of 2^20 bodies (CPU baseline of the mesh module).
     "run_c2": ("run", _run_cfg("mesh", 1048576, 3, DT2, {
         "collision_cube_cells": "256", "collision_cube_min": "2.56",
-        "how_many_neighbours": "100.0", "switch_to_brute_force": "110"})),
+        "how_many_neighbours": "100.0", "switch_to_brute_force": "110",
+        "sma_min": "0.9", "sma_max": "1.1", "inclination_max": "0.01"})),
 }
 
 
