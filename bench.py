@@ -95,18 +95,20 @@ def main():
     dev = torch.device("cuda:0")
     interactions = float(n) * (n - 1)
 
-    def rk4_run(precision, steps, warmup):
+    def rk4_run(precision, steps, warmup, graph):
+        """Times `steps` RK4 steps after `warmup` untimed ones.  Without the
+        graph, the pair kernel is bracketed by HIP events on its stream."""
         bt = torch.from_numpy(b).to(dev)
         ct = torch.tensor([0, 0, 0, 0, 0, 0, 1.0, 1.0], dtype=torch.float64, device=dev)
         torch.cuda.synchronize()
-        ctx.rk4_steps_dev(bt, ct, warmup, dt, 1.0, soft, precision, graph=not args.no_graph)
+        ctx.rk4_steps_dev(bt, ct, warmup, dt, 1.0, soft, precision, graph=graph)
         ctx.synchronize()
         kname = "pairs_" + precision
-        ctx.profile(True)
+        ctx.profile(not graph)
         ctx.profile_read(kname, reset=True)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        ctx.rk4_steps_dev(bt, ct, steps, dt, 1.0, soft, precision, graph=not args.no_graph)
+        ctx.rk4_steps_dev(bt, ct, steps, dt, 1.0, soft, precision, graph=graph)
         ctx.synchronize()
         torch.cuda.synchronize()
         t = time.perf_counter() - t0
@@ -114,10 +116,16 @@ def main():
         ctx.profile(False)
         return t, kms, launches, bt, ct
 
-    t_mix, kms_mix, nl_mix, bt_mix, _ = rk4_run("mixed", args.steps, args.warmup)
+    # Timed region: eager launches, pair kernel timed with HIP events.
+    t_mix, kms_mix, nl_mix, bt_mix, _ = rk4_run("mixed", args.steps, args.warmup, False)
+    t_graph = None
+    if not args.no_graph:
+        t_graph = rk4_run("mixed", args.steps, 1, True)[0]
     evals = 4 * args.steps
     value = interactions * evals / t_mix
-    kavg_ms = kms_mix / max(nl_mix, 1)
+    if nl_mix == 0:
+        raise RuntimeError("no timed launches of the pair kernel were recorded")
+    kavg_ms = kms_mix / nl_mix
     achieved_tflops = FLOP_PER_INTERACTION * interactions / (kavg_ms * 1e-3) / 1e12
 
     out = {
@@ -135,9 +143,11 @@ def main():
         "data": "synthetic (seeded disk)",
         "config": {"workload": "brute-force all-pairs gravity, RK4 (4 evaluations/step), disk of "
                                "N=262144 bodies around a central mass, softening 1e-6, dt=2pi/256",
-                   "n": n, "precision": "mixed", "graph": not args.no_graph,
+                   "n": n, "precision": "mixed", "timed_launches": "eager (events)",
                    "parallelism": "single GPU"},
         "rk4_steps_per_s": args.steps / t_mix,
+        "rk4_steps_per_s_graph": None if t_graph is None else args.steps / t_graph,
+        "pair_interactions_per_s_graph": None if t_graph is None else interactions * evals / t_graph,
         "roofline": {"bound": "valu", "achieved": achieved_tflops,
                      "peak": PEAK_FP32_VECTOR_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved_tflops / PEAK_FP32_VECTOR_TFLOPS, "traffic": None,
@@ -148,9 +158,11 @@ def main():
 
     if not args.no_fp64:
         fsteps = max(1, args.steps // 4)
-        t64, kms64, nl64, bt64, _ = rk4_run("fast64", fsteps, 1)
+        t64, kms64, nl64, bt64, _ = rk4_run("fast64", fsteps, 1, False)
         v64 = interactions * 4 * fsteps / t64
-        k64 = kms64 / max(nl64, 1)
+        if nl64 == 0:
+            raise RuntimeError("no timed launches of the FP64 pair kernel were recorded")
+        k64 = kms64 / nl64
         out["fp64"] = {"pair_interactions_per_s": v64, "rk4_steps_per_s": fsteps / t64,
                        "kernel_avg_ms": k64,
                        "achieved_tflops": FLOP_PER_INTERACTION * interactions / (k64 * 1e-3) / 1e12,

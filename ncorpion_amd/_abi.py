@@ -66,6 +66,7 @@ _SIGNATURES = {
     "ncg_mesh_gravity": (S, [P, P, I32, P, ctypes.POINTER(Central), D, D, S, P, P]),
     "ncg_mesh_pairs": (S, [P, P, I64, ctypes.POINTER(I64)]),
     "ncg_rk4_steps_dev": (S, [P, P, I32, P, P, D, D, S, D, I32, I32, P]),
+    "ncg_set_tiling": (S, [P, I32, I32, D, I32]),
     "ncg_profile_enable": (S, [P, S]),
     "ncg_profile_read": (S, [P, ctypes.c_char_p, ctypes.POINTER(D), ctypes.POINTER(I64), S]),
 }
@@ -195,6 +196,10 @@ class Context:
             self.h, bodies_t.data_ptr(), n, None if exists_t is None else exists_t.data_ptr(),
             central_t.data_ptr(), float(G), float(softening), PRECISIONS[precision], float(dt),
             int(nsteps), 1 if graph else 0, stream))
+
+    def set_tiling(self, ipl=0, nsplit=-1, theta=0.0, source=-1):
+        self._check(self.lib.ncg_set_tiling(self.h, int(ipl), int(nsplit), float(theta),
+                                            int(source)))
 
     def profile(self, enable=True):
         self._check(self.lib.ncg_profile_enable(self.h, 1 if enable else 0))

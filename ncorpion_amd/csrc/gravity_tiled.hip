@@ -238,8 +238,52 @@ __global__ __launch_bounds__(NT) void k_tiles(const double4 *__restrict__ jd,
 }
 
 // ---- 6. pair kernel -------------------------------------------------------
+// One interaction of the FP64 form: a += G m_j (x_j - x_i) / (r^2 + eps^2)^(3/2),
+// hardware 1/sqrt refined by one Newton step (v_rsq_f64 is good to ~2^-29;
+// one step brings it to FP64 rounding).  r^2 is floored at TINY2 so that a
+// body against itself (r = 0, no softening) adds exactly 0.
+template <int IPL>
+__device__ inline void near_term(const double4 p, double sj, const double *xi, const double *yi,
+                                 const double *zi, const double *si, double *ax, double *ay,
+                                 double *az)
+{
+#pragma unroll
+    for (int q = 0; q < IPL; ++q) {
+        const double dx = p.x - xi[q], dy = p.y - yi[q], dz = p.z - zi[q];
+        const double e = si[q] + sj;
+        double r2 = fma(dx, dx, fma(dy, dy, fma(dz, dz, e * e)));
+        r2 = fmax(r2, TINY2);
+        double y = __builtin_amdgcn_rsq(r2);
+        const double h = 0.5 * r2;
+        y = y * fma(-h, y * y, 1.5);
+        const double sc3 = p.w * y * (y * y);
+        ax[q] = fma(sc3, dx, ax[q]);
+        ay[q] = fma(sc3, dy, ay[q]);
+        az[q] = fma(sc3, dz, az[q]);
+    }
+}
+
+// One interaction of the FP32 far form: separation w_i + u_j (w_i = C - u_i).
+template <int IPL>
+__device__ inline void far_term(const float4 u, const float *wx, const float *wy, const float *wz,
+                                float *fx, float *fy, float *fz)
+{
+#pragma unroll
+    for (int q = 0; q < IPL; ++q) {
+        const float dx = wx[q] + u.x, dy = wy[q] + u.y, dz = wz[q] + u.z;
+        const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+        const float y = __builtin_amdgcn_rsqf(r2);
+        const float s3 = (u.w * y) * (y * y);
+        fx[q] = fmaf(s3, dx, fx[q]);
+        fy[q] = fmaf(s3, dy, fy[q]);
+        fz[q] = fmaf(s3, dz, fz[q]);
+    }
+}
+
 // MODE 0: FP64-fast on every tile; MODE 1: mixed (near FP64, far FP32).
-template <int MODE, int IPL>
+// SRC 0: tile data read with wave-uniform addresses (scalar loads);
+// SRC 1: tile data staged in LDS by the workgroup, read as broadcasts.
+template <int MODE, int IPL, int SRC>
 __global__ __launch_bounds__(NT) void k_pairs(const double4 *__restrict__ jd,
                                               const double *__restrict__ jsig,
                                               const float4 *__restrict__ ju,
@@ -249,6 +293,9 @@ __global__ __launch_bounds__(NT) void k_pairs(const double4 *__restrict__ jd,
                                               double theta, double soft, int nslots,
                                               double *__restrict__ part)
 {
+    __shared__ float4 lu[SRC ? TJ : 1];
+    __shared__ double4 ld[SRC ? TJ : 1];
+    __shared__ double ls[SRC ? TJ : 1];
     const int nv = sc->nvalid;
     const int iblk = blockIdx.x;
     const int i0 = iblk * NT * IPL;
@@ -278,43 +325,49 @@ __global__ __launch_bounds__(NT) void k_pairs(const double4 *__restrict__ jd,
     // Largest softening length squared and the FP32 resolution test.
     const double eps = 2.0 * soft * sc->rmax;
     const double eps2x = eps * eps * 67108864.0;  // 2^26
-    const double hi2 = (double)ih.hx * ih.hx + (double)ih.hy * ih.hy + (double)ih.hz * ih.hz;
+    const double hi = sqrt((double)ih.hx * ih.hx + (double)ih.hy * ih.hy + (double)ih.hz * ih.hz);
 
     for (int t = t0; t < t1; ++t) {
         const TileHdr th = thdr[t];
         const int jb = t * TJ;
         const int cnt = th.count;
         bool near = true;
-        double Cx = th.ax - ih.ax, Cy = th.ay - ih.ay, Cz = th.az - ih.az;
+        const double Cx = th.ax - ih.ax, Cy = th.ay - ih.ay, Cz = th.az - ih.az;
         if (MODE == 1) {
             const double gx = fmax(0.0, fabs(Cx) - ((double)ih.hx + th.hx));
             const double gy = fmax(0.0, fabs(Cy) - ((double)ih.hy + th.hy));
             const double gz = fmax(0.0, fabs(Cz) - ((double)ih.hz + th.hz));
             const double dmin2 = gx * gx + gy * gy + gz * gz;
-            const double hj2 = (double)th.hx * th.hx + (double)th.hy * th.hy + (double)th.hz * th.hz;
-            const double size = sqrt(Cx * Cx + Cy * Cy + Cz * Cz) + sqrt(hi2) + sqrt(hj2);
+            const double hj = sqrt((double)th.hx * th.hx + (double)th.hy * th.hy +
+                                   (double)th.hz * th.hz);
+            const double size = sqrt(Cx * Cx + Cy * Cy + Cz * Cz) + hi + hj;
             near = (dmin2 < theta * theta * size * size) || (dmin2 <= eps2x);
         }
         if (near) {
-            for (int jj = 0; jj < cnt; ++jj) {
-                const double4 p = jd[jb + jj];
-                const double sj = jsig[jb + jj];
-#pragma unroll
-                for (int q = 0; q < IPL; ++q) {
-                    const double dx = p.x - xi[q], dy = p.y - yi[q], dz = p.z - zi[q];
-                    const double e = si[q] + sj;
-                    double r2 = fma(dx, dx, fma(dy, dy, fma(dz, dz, e * e)));
-                    r2 = fmax(r2, TINY2);
-                    double y = __builtin_amdgcn_rsq(r2);
-                    const double h = 0.5 * r2;
-                    y = y * fma(-h, y * y, 1.5);
-                    const double sc3 = p.w * y * (y * y);
-                    ax[q] = fma(sc3, dx, ax[q]);
-                    ay[q] = fma(sc3, dy, ay[q]);
-                    az[q] = fma(sc3, dz, az[q]);
+            if (SRC) {
+                __syncthreads();
+                if ((int)threadIdx.x < cnt) {
+                    ld[threadIdx.x] = jd[jb + threadIdx.x];
+                    ls[threadIdx.x] = jsig[jb + threadIdx.x];
                 }
+                __syncthreads();
+            }
+            if (cnt == TJ) {
+#pragma unroll 4
+                for (int jj = 0; jj < TJ; ++jj)
+                    near_term<IPL>(SRC ? ld[jj] : jd[jb + jj], SRC ? ls[jj] : jsig[jb + jj], xi,
+                                   yi, zi, si, ax, ay, az);
+            } else {
+                for (int jj = 0; jj < cnt; ++jj)
+                    near_term<IPL>(SRC ? ld[jj] : jd[jb + jj], SRC ? ls[jj] : jsig[jb + jj], xi,
+                                   yi, zi, si, ax, ay, az);
             }
         } else {
+            if (SRC) {
+                __syncthreads();
+                if ((int)threadIdx.x < cnt) lu[threadIdx.x] = ju[jb + threadIdx.x];
+                __syncthreads();
+            }
             const float cx = (float)Cx, cy = (float)Cy, cz = (float)Cz;
             float wx[IPL], wy[IPL], wz[IPL], fx[IPL], fy[IPL], fz[IPL];
 #pragma unroll
@@ -324,18 +377,13 @@ __global__ __launch_bounds__(NT) void k_pairs(const double4 *__restrict__ jd,
                 wz[q] = cz - uz[q];
                 fx[q] = fy[q] = fz[q] = 0.f;
             }
-            for (int jj = 0; jj < cnt; ++jj) {
-                const float4 u = ju[jb + jj];
-#pragma unroll
-                for (int q = 0; q < IPL; ++q) {
-                    const float dx = wx[q] + u.x, dy = wy[q] + u.y, dz = wz[q] + u.z;
-                    const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-                    const float y = __builtin_amdgcn_rsqf(r2);
-                    const float s3 = (u.w * y) * (y * y);
-                    fx[q] = fmaf(s3, dx, fx[q]);
-                    fy[q] = fmaf(s3, dy, fy[q]);
-                    fz[q] = fmaf(s3, dz, fz[q]);
-                }
+            if (cnt == TJ) {
+#pragma unroll 8
+                for (int jj = 0; jj < TJ; ++jj)
+                    far_term<IPL>(SRC ? lu[jj] : ju[jb + jj], wx, wy, wz, fx, fy, fz);
+            } else {
+                for (int jj = 0; jj < cnt; ++jj)
+                    far_term<IPL>(SRC ? lu[jj] : ju[jb + jj], wx, wy, wz, fx, fy, fz);
             }
 #pragma unroll
             for (int q = 0; q < IPL; ++q) {
@@ -441,18 +489,22 @@ __global__ __launch_bounds__(NT) void k_sum3_final(const double *__restrict__ pa
 }  // namespace
 
 // Tuning knobs of the tiled pipeline (chosen by the autotuner; defaults here).
-struct TiledCfg {
-    int ipl = 2;
-    int nsplit = 0;  // 0: pick from the CU count
-    double theta = 0.5;
-};
-static TiledCfg g_tiled_cfg;
 
-extern "C" void ncg_tiled_set_config(int ipl, int nsplit, double theta)
+extern "C" ncg_status ncg_set_tiling(ncg_ctx *ctx, int32_t ipl, int32_t nsplit, double theta,
+                                     int32_t source)
 {
-    if (ipl == 1 || ipl == 2 || ipl == 4) g_tiled_cfg.ipl = ipl;
-    if (nsplit >= 0) g_tiled_cfg.nsplit = nsplit;
-    if (theta > 0.0) g_tiled_cfg.theta = theta;
+    if (!ctx) return NCG_ERR_INVALID;
+    NCG_REQUIRE(ipl == 0 || ipl == 1 || ipl == 2 || ipl == 4, "ipl must be 1, 2 or 4 (0 keeps)");
+    NCG_REQUIRE(nsplit >= -1, "nsplit must be >= 0 (0 = automatic) or -1 (keep)");
+    NCG_REQUIRE(source >= -1 && source <= 1, "source must be 0 (scalar), 1 (LDS) or -1 (keep)");
+    TiledCfg &c = ctx->tiled;
+    ctx->tiled_explicit = true;
+    if (ipl > 0) c.ipl = ipl;
+    if (nsplit >= 0) c.nsplit = nsplit;
+    if (theta > 0.0) c.theta = theta;
+    if (source >= 0) c.src = source;
+    if (ctx->rk4) ctx->buf_gen++;  // captured graphs hold the old launch shape
+    return NCG_OK;
 }
 
 ncg_status ncg_grav_exact_launch(ncg_ctx *ctx, const DBody *d_b, int32_t n, const int32_t *d_ex,
@@ -460,30 +512,41 @@ ncg_status ncg_grav_exact_launch(ncg_ctx *ctx, const DBody *d_b, int32_t n, cons
                                  double *d_cterm, hipStream_t st);
 
 template <int MODE, int IPL>
-static void launch_pairs(dim3 grid, hipStream_t st, const double4 *jd, const double *jsig,
-                         const float4 *ju, const TileHdr *th, const TileHdr *ih, const Scal *sc,
-                         int nsplit, double theta, double soft, int n, double *part)
+static void launch_pairs(int src, dim3 grid, hipStream_t st, const double4 *jd,
+                         const double *jsig, const float4 *ju, const TileHdr *th,
+                         const TileHdr *ih, const Scal *sc, int nsplit, double theta, double soft,
+                         int n, double *part)
 {
-    hipLaunchKernelGGL((k_pairs<MODE, IPL>), grid, dim3(NT), 0, st, jd, jsig, ju, th, ih, sc,
-                       nsplit, theta, soft, n, part);
+    if (src)
+        hipLaunchKernelGGL((k_pairs<MODE, IPL, 1>), grid, dim3(NT), 0, st, jd, jsig, ju, th, ih, sc,
+                           nsplit, theta, soft, n, part);
+    else
+        hipLaunchKernelGGL((k_pairs<MODE, IPL, 0>), grid, dim3(NT), 0, st, jd, jsig, ju, th, ih, sc,
+                           nsplit, theta, soft, n, part);
 }
 
 // Grid shape of the tiled forms for n slots (same in order and eval).
 struct TiledShape {
-    int ipl, iblock, nib, ntiles, nsplit, nbb;
+    int ipl, iblock, nib, ntiles, nsplit, nbb, src;
+    double theta;
 };
 
-static TiledShape tiled_shape(ncg_ctx *ctx, int32_t n)
+static TiledShape tiled_shape(ncg_ctx *ctx, int32_t n, int prec)
 {
+    const TiledCfg cfg = ncg_tiled_cfg(ctx, prec, n);
     TiledShape t;
-    t.ipl = g_tiled_cfg.ipl;
+    t.ipl = cfg.ipl;
+    t.theta = cfg.theta;
+    t.src = cfg.src;
     t.iblock = NT * t.ipl;
     t.nib = (int)cdiv(n, t.iblock);
     t.ntiles = (int)cdiv(n, TJ);
-    int nsplit = g_tiled_cfg.nsplit;
+    int nsplit = cfg.nsplit;
     if (nsplit <= 0) {
+        // Enough workgroups for 64 per CU: the near tiles make the work per
+        // workgroup uneven, and small workgroups balance it.
         const int cus = ctx->num_cus > 0 ? ctx->num_cus : 256;
-        nsplit = (int)cdiv(4L * cus, t.nib);
+        nsplit = (int)cdiv(256L * cus, t.nib);
     }
     t.nsplit = std::max(1, std::min(nsplit, t.ntiles));
     t.nbb = std::min(1024, (int)cdiv(n, NT));
@@ -493,7 +556,7 @@ static TiledShape tiled_shape(ncg_ctx *ctx, int32_t n)
 ncg_status ncg_tiled_order(ncg_ctx *ctx, const DBody *d_b, int32_t n, const int32_t *d_ex,
                            hipStream_t st)
 {
-    const TiledShape t = tiled_shape(ctx, n);
+    const TiledShape t = tiled_shape(ctx, n, NCG_MIXED);
     Scal *sc = dbuf<Scal>(ctx, "t_scal", 1);
     double *bpart = dbuf<double>(ctx, "t_bpart", 8 * (size_t)t.nbb);
     unsigned *key = dbuf<unsigned>(ctx, "t_key", n);
@@ -519,7 +582,7 @@ ncg_status ncg_tiled_eval(ncg_ctx *ctx, const DBody *d_b, int32_t n, const int32
                           ncg_precision prec, double *d_acc, double *d_cterm, double *d_cm_acc,
                           hipStream_t st)
 {
-    const TiledShape t = tiled_shape(ctx, n);
+    const TiledShape t = tiled_shape(ctx, n, prec);
     Scal *sc = dbuf<Scal>(ctx, "t_scal", 1);
     int *perm = dbuf<int>(ctx, "t_perm", n);
     double4 *jd = dbuf<double4>(ctx, "t_jd", (size_t)t.ntiles * TJ + t.iblock);
@@ -540,17 +603,18 @@ ncg_status ncg_tiled_eval(ncg_ctx *ctx, const DBody *d_b, int32_t n, const int32
                        (float4 *)nullptr);
     NCG_CHECK_LAUNCH();
     const dim3 grid(t.nib, t.nsplit);
-    const double theta = g_tiled_cfg.theta;
+    const double theta = t.theta;
+    const int src = t.src;
     {
         ProfScope ps(ctx, st, prec == NCG_F64_FAST ? "pairs_fast64" : "pairs_mixed");
         if (prec == NCG_F64_FAST) {
-            if (t.ipl == 1) launch_pairs<0, 1>(grid, st, jd, jsig, ju, th, ih, sc, t.nsplit, theta, soft, n, part);
-            else if (t.ipl == 2) launch_pairs<0, 2>(grid, st, jd, jsig, ju, th, ih, sc, t.nsplit, theta, soft, n, part);
-            else launch_pairs<0, 4>(grid, st, jd, jsig, ju, th, ih, sc, t.nsplit, theta, soft, n, part);
+            if (t.ipl == 1) launch_pairs<0, 1>(src, grid, st, jd, jsig, ju, th, ih, sc, t.nsplit, theta, soft, n, part);
+            else if (t.ipl == 2) launch_pairs<0, 2>(src, grid, st, jd, jsig, ju, th, ih, sc, t.nsplit, theta, soft, n, part);
+            else launch_pairs<0, 4>(src, grid, st, jd, jsig, ju, th, ih, sc, t.nsplit, theta, soft, n, part);
         } else {
-            if (t.ipl == 1) launch_pairs<1, 1>(grid, st, jd, jsig, ju, th, ih, sc, t.nsplit, theta, soft, n, part);
-            else if (t.ipl == 2) launch_pairs<1, 2>(grid, st, jd, jsig, ju, th, ih, sc, t.nsplit, theta, soft, n, part);
-            else launch_pairs<1, 4>(grid, st, jd, jsig, ju, th, ih, sc, t.nsplit, theta, soft, n, part);
+            if (t.ipl == 1) launch_pairs<1, 1>(src, grid, st, jd, jsig, ju, th, ih, sc, t.nsplit, theta, soft, n, part);
+            else if (t.ipl == 2) launch_pairs<1, 2>(src, grid, st, jd, jsig, ju, th, ih, sc, t.nsplit, theta, soft, n, part);
+            else launch_pairs<1, 4>(src, grid, st, jd, jsig, ju, th, ih, sc, t.nsplit, theta, soft, n, part);
         }
         NCG_CHECK_LAUNCH();
     }

@@ -113,6 +113,12 @@ __global__ __launch_bounds__(MT) void k_clear_cells(const unsigned *__restrict__
     }
 }
 
+__global__ __launch_bounds__(MT) void k_iota(int *__restrict__ v, int n)
+{
+    const int s = blockIdx.x * MT + threadIdx.x;
+    if (s < n) v[s] = s;
+}
+
 __global__ __launch_bounds__(MT) void k_bounds(const unsigned *__restrict__ skey, int ins,
                                                unsigned *__restrict__ cs, unsigned *__restrict__ ce)
 {
@@ -440,18 +446,50 @@ __global__ __launch_bounds__(MT) void k_mpairs(const unsigned *__restrict__ skey
 
 namespace {
 
+// Host view of the binning, with the bodies a collision moved to another
+// cell before they were inserted (see the replay below).
 struct HostCells {
-    std::vector<unsigned> key;   // per body
+    std::vector<unsigned> key;   // per body, key of the GPU binning
     std::vector<unsigned> skey;  // per sorted position
     std::vector<int> sid;
+    std::map<int, unsigned> reloc;               // body -> key at its insertion
+    std::map<unsigned, std::vector<int>> added;  // cell -> relocated bodies (ascending)
     bool loaded = false;
     long d = 0;
-    // Ids of cell `ck`, ascending.
-    std::pair<size_t, size_t> range(unsigned ck) const
+    unsigned eff(int b) const
     {
+        auto it = reloc.find(b);
+        return it == reloc.end() ? key[b] : it->second;
+    }
+    // Bodies of cell `ck` (ascending id) at insertion time.
+    void members(unsigned ck, std::vector<int> &out) const
+    {
+        out.clear();
         auto lo = std::lower_bound(skey.begin(), skey.end(), ck);
         auto hi = std::upper_bound(lo, skey.end(), ck);
-        return {(size_t)(lo - skey.begin()), (size_t)(hi - skey.begin())};
+        for (auto it = lo; it != hi; ++it) {
+            const int q = sid[it - skey.begin()];
+            if (!reloc.count(q)) out.push_back(q);
+        }
+        auto ad = added.find(ck);
+        if (ad != added.end()) {
+            out.insert(out.end(), ad->second.begin(), ad->second.end());
+            std::sort(out.begin(), out.end());
+        }
+    }
+    void relocate(int b, unsigned nk)
+    {
+        const unsigned ok = eff(b);
+        if (ok == nk) return;
+        if (reloc.count(b) && ok != NOKEY) {
+            auto &v = added[ok];
+            v.erase(std::remove(v.begin(), v.end(), b), v.end());
+        }
+        reloc[b] = nk;
+        if (nk != NOKEY) {
+            auto &v = added[nk];
+            v.insert(std::upper_bound(v.begin(), v.end(), b), b);
+        }
     }
     int stencil_index(long delta) const
     {
@@ -460,6 +498,17 @@ struct HostCells {
         return -1;
     }
 };
+
+// Key of a body at position (x, y, z) as hash_table_cell_index computes it
+// (collision.c:197-207), NOKEY outside the collision cube.
+unsigned host_key(double x, double y, double z, double gam, double cube, long d)
+{
+    if (fabs(x) > cube / 2.0 || fabs(y) > cube / 2.0 || fabs(z) > cube / 2.0) return NOKEY;
+    const long i = d / 2 + 1 + (int)floor(x / gam);
+    const long j = d / 2 + 1 + (int)floor(y / gam);
+    const long k = d / 2 + 1 + (int)floor(z / gam);
+    return (unsigned)(i + j * d + k * d * d);
+}
 
 ncg_status load_cells(ncg_ctx *ctx, HostCells &hc, int32_t n, int ins, const unsigned *dkey,
                       const unsigned *dskey, const int *dsid)
@@ -646,27 +695,29 @@ extern "C" ncg_status ncg_mesh_pass(ncg_ctx *ctx, const ncg_body *bodies, int32_
     HostCells hc;
     hc.d = d;
     std::vector<char> dirty((size_t)n, 0);
-    std::map<int, std::vector<int>> pending;  // visit k -> dirty neighbours p < k
+    std::map<int, std::vector<int>> pending;  // visit k -> modified neighbours p < k
+    std::set<int> full_visit;                 // small bodies modified before their visit
     int64_t ncoll = 0, htests = 0;
-    auto live = [&](int32_t k) { return !exists || exists[k] != 0; };
     ncg_status err = NCG_OK;
-    // Body b was modified during visit kv: later visits that have b as a
-    // neighbour must re-test it on the host.
+    std::vector<int> mem;
+    auto live = [&](int32_t k) { return !exists || exists[k] != 0; };
+    // Later visits q > kv that have b as a neighbour re-test it on the host.
+    auto schedule = [&](int b, int kv) -> ncg_status {
+        ncg_status r = load_cells(ctx, hc, n, ins, key, skey, sid);
+        if (r != NCG_OK) return r;
+        const unsigned kb = hc.eff(b);
+        if (kb == NOKEY) return NCG_OK;  // not in the hash table: nobody's neighbour
+        for (int l = 0; l < 27; ++l) {
+            hc.members((unsigned)((long)kb + stencil_offset(l, d)), mem);
+            for (int q : mem)
+                if (q > kv && q > b) pending[q].push_back(b);
+        }
+        return NCG_OK;
+    };
     auto mark_dirty = [&](int b, int kv) -> ncg_status {
         if (dirty[b]) return NCG_OK;
         dirty[b] = 1;
-        ncg_status r = load_cells(ctx, hc, n, ins, key, skey, sid);
-        if (r != NCG_OK) return r;
-        const unsigned kb = hc.key[b];
-        if (kb == NOKEY) return NCG_OK;  // not in the hash table: nobody's neighbour
-        for (int l = 0; l < 27; ++l) {
-            const auto rg = hc.range((unsigned)((long)kb + stencil_offset(l, d)));
-            for (size_t t = rg.first; t < rg.second; ++t) {
-                const int q = hc.sid[t];
-                if (q > kv && q > b) pending[q].push_back(b);
-            }
-        }
-        return NCG_OK;
+        return schedule(b, kv);
     };
     auto collide = [&](int p, int k, int kv) -> ncg_status {
         ops->resolve(ops->user, p, k);
@@ -676,16 +727,17 @@ extern "C" ncg_status ncg_mesh_pass(ncg_ctx *ctx, const ncg_body *bodies, int32_
         if (r != NCG_OK) return r;
         return mark_dirty(k, kv);
     };
-    // Remaining pairs of visit k from traversal position `from` (cell order
-    // key (26 - l), then id), all decided on the host.
+    // Pairs of visit k from traversal position (cell from_l, id >= from_p)
+    // onwards, all decided on the host.
     auto full_rest = [&](int k, int from_l, int from_p) -> ncg_status {
         ncg_status r = load_cells(ctx, hc, n, ins, key, skey, sid);
         if (r != NCG_OK) return r;
-        const unsigned kk = hc.key[k];
+        const unsigned kk = hc.eff(k);
+        if (kk == NOKEY) return NCG_OK;
+        std::vector<int> cell;
         for (int l = from_l; l >= 0; --l) {
-            const auto rg = hc.range((unsigned)((long)kk + stencil_offset(l, d)));
-            for (size_t t = rg.first; t < rg.second; ++t) {
-                const int p = hc.sid[t];
+            hc.members((unsigned)((long)kk + stencil_offset(l, d)), cell);
+            for (int p : cell) {
                 if (p >= k) break;
                 if (l == from_l && p < from_p) continue;
                 ++htests;
@@ -702,7 +754,8 @@ extern "C" ncg_status ncg_mesh_pass(ncg_ctx *ctx, const ncg_body *bodies, int32_
         const int kc = ci < cand.size() ? cand[ci].x : INT32_MAX;
         const int kb = bi < big.size() ? big[bi] : INT32_MAX;
         const int kp = pending.empty() ? INT32_MAX : pending.begin()->first;
-        const int k = std::min(kc, std::min(kb, kp));
+        const int kf = full_visit.empty() ? INT32_MAX : *full_visit.begin();
+        const int k = std::min(std::min(kc, kb), std::min(kp, kf));
         if (k == INT32_MAX) break;
         if (k == kb) {
             // Body larger than the mesh size (collision.c:381-414).
@@ -716,18 +769,23 @@ extern "C" ncg_status ncg_mesh_pass(ncg_ctx *ctx, const ncg_body *bodies, int32_
                 ++htests;
                 visit.push_back({k, p});
                 if (ops->test(ops->user, p, k)) {
-                    if (p > k && Rp <= gam && !dirty[p]) {
-                        ncg_status r = load_cells(ctx, hc, n, ins, key, skey, sid);
-                        if (r != NCG_OK) return r;
-                        if (hc.key[p] != NOKEY)
-                            return ncg_fail(ctx, NCG_ERR_UNSUPPORTED,
-                                            "body %d (larger than the mesh size) collides with "
-                                            "body %d before it is binned",
-                                            k, p);
-                    }
                     ncg_status r = collide(p, k, k);
                     if (r != NCG_OK) return r;
                     visit.back() = {p, k};
+                    if (p > k && Rp <= gam) {
+                        // p is inserted later, at its new position: its cell
+                        // may change, and its own visit is decided on the host.
+                        r = load_cells(ctx, hc, n, ins, key, skey, sid);
+                        if (r != NCG_OK) return r;
+                        const unsigned nk = host_key(bodies[p].x, bodies[p].y, bodies[p].z, gam,
+                                                     collision_cube, d);
+                        if (nk != hc.eff(p)) {
+                            hc.relocate(p, nk);
+                            r = schedule(p, k);
+                            if (r != NCG_OK) return r;
+                        }
+                        full_visit.insert(p);
+                    }
                 }
             }
             M.big_visits.push_back({k, (int)M.big_pairs_all.size(), (int)visit.size()});
@@ -737,18 +795,39 @@ extern "C" ncg_status ncg_mesh_pass(ncg_ctx *ctx, const ncg_body *bodies, int32_
             }
             continue;
         }
-        // Small body k: merge candidates and pending re-tests in traversal order.
-        std::vector<std::pair<long, int>> todo;  // ((26 - l) * 2^31 + p, p)
+        // Small body k.
+        auto it = pending.find(k);
+        if (full_visit.count(k)) {
+            full_visit.erase(k);
+            while (ci < cand.size() && cand[ci].x == k) ++ci;
+            if (it != pending.end()) pending.erase(it);
+            if (live(k)) {
+                err = full_rest(k, 26, 0);
+                if (err != NCG_OK) return err;
+            }
+            continue;
+        }
+        // Candidates and pending re-tests, merged in traversal order
+        // ((26 - l) * 2^31 + p).  Pairs whose cells no longer match (a body
+        // moved before insertion) are dropped.
+        std::vector<std::pair<long, int>> todo;
+        const bool moved = !hc.reloc.empty();
         while (ci < cand.size() && cand[ci].x == k) {
-            todo.push_back({(long)(26 - cand[ci].y) * (1L << 31) + cand[ci].z, cand[ci].z});
+            const int p = cand[ci].z;
+            int l = cand[ci].y;
+            if (moved && (hc.reloc.count(p) || hc.reloc.count(k)))
+                l = hc.eff(p) == NOKEY || hc.eff(k) == NOKEY
+                        ? -1
+                        : hc.stencil_index((long)hc.eff(p) - (long)hc.eff(k));
+            if (l >= 0) todo.push_back({(long)(26 - l) * (1L << 31) + p, p});
             ++ci;
         }
-        auto it = pending.find(k);
         if (it != pending.end()) {
             ncg_status r = load_cells(ctx, hc, n, ins, key, skey, sid);
             if (r != NCG_OK) return r;
             for (int p : it->second) {
-                const int l = hc.stencil_index((long)hc.key[p] - (long)hc.key[k]);
+                if (hc.eff(p) == NOKEY || hc.eff(k) == NOKEY) continue;
+                const int l = hc.stencil_index((long)hc.eff(p) - (long)hc.eff(k));
                 if (l >= 0) todo.push_back({(long)(26 - l) * (1L << 31) + p, p});
             }
             pending.erase(it);
@@ -757,13 +836,6 @@ extern "C" ncg_status ncg_mesh_pass(ncg_ctx *ctx, const ncg_body *bodies, int32_
         todo.erase(std::unique(todo.begin(), todo.end()), todo.end());
         for (size_t q = 0; q < todo.size(); ++q) {
             const int p = todo[q].second;
-            if (dirty[k]) {
-                // k changed earlier in this visit: every remaining pair on the host.
-                const int l = 26 - (int)(todo[q].first >> 31);
-                err = full_rest(k, l, p);
-                if (err != NCG_OK) return err;
-                break;
-            }
             ++htests;
             if (ops->test(ops->user, p, k)) {
                 err = collide(p, k, k);
@@ -775,6 +847,37 @@ extern "C" ncg_status ncg_mesh_pass(ncg_ctx *ctx, const ncg_body *bodies, int32_
                 break;
             }
         }
+    }
+    if (!hc.reloc.empty()) {
+        // Re-bin with the insertion-time cells so that total_neighbours, the
+        // pair list and the mesh gravity see what the reference's hash saw.
+        std::vector<unsigned> nk(hc.key);
+        for (const auto &kv : hc.reloc) nk[kv.first] = kv.second;
+        int nins = 0;
+        for (unsigned v : nk) nins += v != NOKEY;
+        hipLaunchKernelGGL(k_clear_cells, dim3(cdiv(std::max(ins, 1), MT)), dim3(MT), 0, st,
+                           (const unsigned *)pk, ins, cs, ce);
+        NCG_CHECK_LAUNCH();
+        NCG_HIP(hipMemcpyAsync(key, nk.data(), sizeof(unsigned) * n, hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_iota, dim3(cdiv(n, MT)), dim3(MT), 0, st, idx, (int)n);
+        NCG_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, key, skey, idx, sid, n, 0, 32, st));
+        pk = dbuf<unsigned>(ctx, "m_prevkeys", std::max(nins, 1));
+        if (!pk) return NCG_ERR_NOMEM;
+        if (nins > 0) {
+            hipLaunchKernelGGL(k_bounds, dim3(cdiv(nins, MT)), dim3(MT), 0, st, skey, nins, cs, ce);
+            NCG_HIP(hipMemcpyAsync(pk, skey, sizeof(unsigned) * nins, hipMemcpyDeviceToDevice, st));
+            NCG_HIP(hipMemsetAsync(tot, 0, sizeof(unsigned long long) * 2, st));
+            hipLaunchKernelGGL(k_mcount<1>, dim3(cdiv(nins, MT)), dim3(MT), 0, st, skey, sid, pr,
+                               vm, cs, ce, nins, (int)d, timestep, cbi, (int4 *)nullptr, 0ull,
+                               tot + 1, tot);
+            NCG_CHECK_LAUNCH();
+        }
+        unsigned long long ht[2] = {0, 0};
+        NCG_HIP(hipMemcpyAsync(ht, tot, sizeof ht, hipMemcpyDeviceToHost, st));
+        NCG_HIP(hipStreamSynchronize(st));
+        M.prev_ins = nins;
+        M.inserted = nins;
+        stats->total_neighbours = (int64_t)ht[0];
     }
     stats->collisions = ncoll;
     stats->host_tests = htests;

@@ -1,7 +1,11 @@
 // Context, errors and buffer management of libncg.
 #include "ncg_internal.hpp"
 
+#include <dlfcn.h>
+
+#include <algorithm>
 #include <cstdlib>
+#include <cstring>
 #include <new>
 
 ncg_status ncg_fail(ncg_ctx *ctx, ncg_status code, const char *fmt, ...)
@@ -39,6 +43,60 @@ void *ncg_dbuf(ncg_ctx *ctx, const char *name, size_t bytes)
     return b.ptr;
 }
 
+// Reads the tuning table: $NCG_TUNING, else ncg_tuning.txt beside libncg.so.
+// Lines: <exact|fast64|mixed> <n_min> <ipl> <nsplit> <theta> <src>; '#' comments.
+static void load_tuning(ncg_ctx *ctx)
+{
+    std::string path;
+    if (const char *e = getenv("NCG_TUNING")) {
+        path = e;
+    } else {
+        Dl_info info;
+        if (dladdr((void *)&load_tuning, &info) && info.dli_fname) {
+            path = info.dli_fname;
+            const size_t slash = path.rfind('/');
+            path = (slash == std::string::npos ? std::string(".") : path.substr(0, slash)) +
+                   "/ncg_tuning.txt";
+        }
+    }
+    if (path.empty()) return;
+    FILE *f = fopen(path.c_str(), "r");
+    if (!f) return;
+    char line[512];
+    while (fgets(line, sizeof line, f)) {
+        if (line[0] == '#') continue;
+        char pname[32];
+        long nmin;
+        int ipl, nsplit, src;
+        double theta;
+        if (sscanf(line, "%31s %ld %d %d %lf %d", pname, &nmin, &ipl, &nsplit, &theta, &src) != 6)
+            continue;
+        TuneRow r;
+        r.prec = !strcmp(pname, "mixed") ? NCG_MIXED : !strcmp(pname, "fast64") ? NCG_F64_FAST : -1;
+        if (r.prec < 0 || (ipl != 1 && ipl != 2 && ipl != 4) || nsplit < 0 || theta <= 0.0 ||
+            (src != 0 && src != 1))
+            continue;
+        r.n_min = nmin;
+        r.cfg.ipl = ipl;
+        r.cfg.nsplit = nsplit;
+        r.cfg.theta = theta;
+        r.cfg.src = src;
+        ctx->tune.push_back(r);
+    }
+    fclose(f);
+    std::sort(ctx->tune.begin(), ctx->tune.end(),
+              [](const TuneRow &a, const TuneRow &b) { return a.n_min < b.n_min; });
+}
+
+TiledCfg ncg_tiled_cfg(const ncg_ctx *ctx, int prec, long n)
+{
+    if (ctx->tiled_explicit) return ctx->tiled;
+    TiledCfg c = ctx->tiled;
+    for (const TuneRow &r : ctx->tune)
+        if (r.prec == prec && r.n_min <= n) c = r.cfg;
+    return c;
+}
+
 extern "C" {
 
 int ncg_abi_version(void) { return NCG_ABI_VERSION; }
@@ -59,6 +117,7 @@ ncg_status ncg_create(ncg_ctx **out, int device)
     }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->num_cus = prop.multiProcessorCount;
+    load_tuning(ctx);
     *out = ctx;
     return NCG_OK;
 }

@@ -53,6 +53,22 @@ struct ProfRec {
 
 struct Rk4Graph;  // defined in rk4.hip
 
+// Launch shape of the tiled gravity forms.
+struct TiledCfg {
+    int ipl = 1;          // bodies per lane
+    int nsplit = 0;       // j-splits per i-block; 0 = from the CU count
+    double theta = 0.35;  // near/far threshold
+    int src = 0;          // 0 scalar loads, 1 LDS staging
+};
+
+// One row of the tuning table (ncg_tuning.txt next to libncg.so): the shape
+// to use for `prec` from n_min bodies up.
+struct TuneRow {
+    int prec;
+    long n_min;
+    TiledCfg cfg;
+};
+
 struct ncg_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -60,18 +76,20 @@ struct ncg_ctx {
     std::unordered_map<std::string, DevBuf> bufs;
     std::unordered_map<std::string, std::vector<unsigned char>> hbufs;
     MeshState mesh;
+    TiledCfg tiled;
+    bool tiled_explicit = false;     // ncg_set_tiling overrides the table
+    std::vector<TuneRow> tune;       // from the tuning table, sorted by n_min
     int num_cus = 0;
     bool profiling = false;
     std::unordered_map<std::string, ProfRec> prof;
     Rk4Graph *rk4 = nullptr;
     uint64_t buf_gen = 0;  // bumped whenever a device buffer is (re)allocated
-    // While a graph is captured with profiling on, the timing events become
-    // graph nodes; they are collected here and re-read after every replay.
-    bool capturing = false;
-    std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> cap_events;
+    bool capturing = false;  // a stream capture is in progress
 };
 
 void ncg_rk4_release(ncg_ctx *ctx);
+// Launch shape for `prec` at n bodies: ncg_set_tiling's, else the tuning table's.
+TiledCfg ncg_tiled_cfg(const ncg_ctx *ctx, int prec, long n);
 
 // Brackets one kernel launch with HIP events on its stream when profiling.
 struct ProfScope {
@@ -81,7 +99,10 @@ struct ProfScope {
     hipEvent_t a = nullptr, b = nullptr;
     ProfScope(ncg_ctx *c_, hipStream_t s_, const char *n_) : c(c_), s(s_), name(n_)
     {
-        if (c->profiling && hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess)
+        // No timing inside a stream capture: event records do not survive
+        // as queryable events in a replayed graph.
+        if (c->profiling && !c->capturing && hipEventCreate(&a) == hipSuccess &&
+            hipEventCreate(&b) == hipSuccess)
             (void)hipEventRecord(a, s);
         else
             a = b = nullptr;
@@ -90,10 +111,7 @@ struct ProfScope {
     {
         if (a && b) {
             (void)hipEventRecord(b, s);
-            if (c->capturing)
-                c->cap_events.emplace_back(name, std::make_pair(a, b));
-            else
-                c->prof[name].pending.emplace_back(a, b);
+            c->prof[name].pending.emplace_back(a, b);
         }
     }
 };

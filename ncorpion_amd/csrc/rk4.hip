@@ -23,16 +23,10 @@ struct Rk4Graph {
     hipStream_t stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> events;
     void release()
     {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
-        for (auto &e : events) {
-            (void)hipEventDestroy(e.second.first);
-            (void)hipEventDestroy(e.second.second);
-        }
-        events.clear();
         exec = nullptr;
         graph = nullptr;
         valid = false;
@@ -200,14 +194,12 @@ extern "C" ncg_status ncg_rk4_steps_dev(ncg_ctx *ctx, ncg_body *d_bodies, int32_
             if (!ctx->rk4) ctx->rk4 = new Rk4Graph();
             g = ctx->rk4;
             g->release();
-            ctx->cap_events.clear();
             ctx->capturing = true;
             NCG_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
             r = rk4_body(ctx, y0, n, d_exists, d_central, grav_const, softening, prec, dt, st, cp);
             hipGraph_t graph = nullptr;
             hipError_t e = hipStreamEndCapture(st, &graph);
             ctx->capturing = false;
-            g->events.swap(ctx->cap_events);
             if (r != NCG_OK) {
                 if (graph) (void)hipGraphDestroy(graph);
                 return r;
@@ -230,17 +222,6 @@ extern "C" ncg_status ncg_rk4_steps_dev(ncg_ctx *ctx, ncg_body *d_bodies, int32_
             continue;  // this step was done eagerly
         }
         NCG_HIP(hipGraphLaunch(g->exec, st));
-        if (ctx->profiling && !g->events.empty()) {
-            // Kernel durations of this replay, from the event nodes of the graph.
-            NCG_HIP(hipEventSynchronize(g->events.back().second.second));
-            for (auto &e : g->events) {
-                float ms = 0.f;
-                NCG_HIP(hipEventElapsedTime(&ms, e.second.first, e.second.second));
-                ProfRec &p = ctx->prof[e.first];
-                p.ms += ms;
-                p.launches++;
-            }
-        }
     }
     return NCG_OK;
 }
