@@ -67,6 +67,69 @@ def cpu_baseline(bodies, n_sample, timeout=300):
                       "of the workload, one evaluation of %.1f s" % (n, secs)}
 
 
+def ensemble_run(ctx, torch, nsys=512, steps=100):
+    """512 independent disks (seeds 0..511), N log-uniform in [256, 8192],
+    m = 1e-8, R = 2e-4, brute-force gravity + collision detection, mixed form,
+    RK4 dt = 2 pi / 256 with hipGraph-captured stages."""
+    import numpy as np
+    from ncorpion_amd import ic
+    sizes = ic.seeded_ensemble_sizes(nsys)
+    systems = [ic.disk(int(nn), seed=s, mass=1e-8, radius=2e-4) for s, nn in enumerate(sizes)]
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)
+    cen = np.zeros((nsys, 8))
+    cen[:, 6] = 1.0
+    cen[:, 7] = 1.0
+    bt = torch.from_numpy(np.concatenate(systems)).cuda()
+    ct = torch.from_numpy(cen).cuda()
+    dt = 2.0 * math.pi / 256.0
+    torch.cuda.synchronize()
+    ctx.ensemble_rk4_dev(bt, off, ct, 1, dt, 1.0, 0.0, "mixed", graph=True, collisions=True)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    nc = ctx.ensemble_rk4_dev(bt, off, ct, steps, dt, 1.0, 0.0, "mixed", graph=True,
+                              collisions=True)
+    ctx.synchronize()
+    t = time.perf_counter() - t0
+    inter = float(np.sum(sizes.astype(np.float64) * (sizes - 1))) * 4 * steps
+    return {"systems": nsys, "bodies": int(sizes.sum()), "steps": steps, "seconds": t,
+            "pair_interactions_per_s": inter / t, "system_steps_per_s": nsys * steps / t,
+            "collisions": nc, "precision": "mixed", "graph": True}
+
+
+def adjoint_run(ctx, torch, n=16384, steps=200, K=20):
+    """Collision-free disk of 16384 bodies, 200 FP64 RK4 steps dt = 2 pi/256;
+    loss = mean squared distance of the final positions to a seeded target
+    (sigma 1e-3); adjoint w.r.t. initial positions, velocities and masses with
+    a checkpoint every 20 steps."""
+    import numpy as np
+    from ncorpion_amd import ic
+    b = ic.disk(n, seed=0, mass=1e-8, radius=2e-4)
+    dt = 2.0 * math.pi / 256.0
+    bt = torch.from_numpy(b).cuda()
+    c0 = torch.tensor([0, 0, 0, 0, 0, 0, 1.0, 1.0], dtype=torch.float64, device="cuda")
+    fb, fc = bt.clone(), c0.clone()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ctx.rk4_steps_dev(fb, fc, steps, dt, 1.0, 0.0, "fast64", graph=True)
+    ctx.synchronize()
+    t_fwd = time.perf_counter() - t0
+    target = fb[:, 0:3] + 1e-3 * torch.tensor(np.random.default_rng(0).normal(size=(n, 3)),
+                                              device="cuda")
+    gfin = torch.zeros((n, 6), dtype=torch.float64, device="cuda")
+    gfin[:, 0:3] = 2.0 / n * (fb[:, 0:3] - target)
+    g0 = torch.zeros((n, 6), dtype=torch.float64, device="cuda")
+    gm = torch.zeros(n, dtype=torch.float64, device="cuda")
+    gc = torch.zeros(7, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ctx.rk4_adjoint_dev(bt, c0, gfin, g0, gm, gc, steps, dt, K, 1.0, 0.0)
+    ctx.synchronize()
+    t_adj = time.perf_counter() - t0
+    return {"n": n, "steps": steps, "checkpoint_every": K, "forward_s": t_fwd,
+            "adjoint_s": t_adj, "adjoint_over_forward": t_adj / t_fwd,
+            "note": "adjoint time includes its own forward pass and segment recomputation"}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,6 +140,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="also run a plain device copy of this size (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (the path is measured at one GPU's "
@@ -183,6 +250,18 @@ def main():
                            "p999_rel_acc_err": float(np.quantile(rel, 0.999)),
                            "normwise_max_err": float(d.max() / torch.sqrt(torch.mean(r * r))),
                            "reference": "FP64-fast form (mutual acceleration only)"}
+
+    if not args.no_ensemble:
+        out["ensemble"] = ensemble_run(ctx, torch)
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint_run(ctx, torch)
+    if args.copy_calibration_mib > 0:
+        x = torch.empty(args.copy_calibration_mib * (1 << 18), dtype=torch.float32, device=dev)
+        for _ in range(3):
+            y = x.clone()
+        torch.cuda.synchronize()
+        del y
+        out["copy_calibration_bytes"] = args.copy_calibration_mib * (1 << 20)
 
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(b, args.cpu_sample)

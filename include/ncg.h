@@ -211,6 +211,69 @@ ncg_status ncg_rk4_steps_dev(ncg_ctx *ctx, ncg_body *d_bodies, int32_t n, const 
                              void *stream);
 
 /* ---------------------------------------------------------------------------
+ * Batched ensembles: nsys independent systems of uneven size, concatenated in
+ * d_bodies (system s = records offsets[s] .. offsets[s+1]-1; offsets is a HOST
+ * array of nsys+1 ints, offsets[0] = 0), each with its own central body
+ * (d_centrals: 8 doubles per system, layout of ncg_rk4_steps_dev).  Every
+ * force evaluation of all systems is one persistent launch over a queue of
+ * (system, body block) work units ordered by cost.
+ *
+ * ncg_ensemble_rk4_dev advances every system by nsteps RK4 steps of dt.
+ * flags bit 0: replay each step from a hipGraph; bit 1: collisions -- each
+ * step starts with a brute-force collision pass per system (candidates from
+ * the same persistent launch as the stage-1 evaluation), replayed on the
+ * host in the reference's lexicographic order (collision.c:425-456) with the
+ * library's own restatement of closest_approach and of the elastic /
+ * inelastic outcome with parameter collision_f (physics.c:545-605).
+ * *n_collisions (may be NULL) receives the collisions resolved.
+ * ncg_ensemble_gravity_dev evaluates every system once (d_acc: 3 per body,
+ * d_cm_acc: 3 per system).  prec: NCG_F64_FAST or NCG_MIXED.
+ */
+ncg_status ncg_ensemble_rk4_dev(ncg_ctx *ctx, ncg_body *d_bodies, const int32_t *offsets,
+                                int32_t nsys, double *d_centrals, double grav_const,
+                                double softening, ncg_precision prec, double dt, int32_t nsteps,
+                                int32_t flags, double collision_f, int64_t *n_collisions,
+                                void *stream);
+ncg_status ncg_ensemble_gravity_dev(ncg_ctx *ctx, const ncg_body *d_bodies, const int32_t *offsets,
+                                    int32_t nsys, const double *d_centrals, double grav_const,
+                                    double softening, ncg_precision prec, double *d_acc,
+                                    double *d_cm_acc, void *stream);
+
+/* ---------------------------------------------------------------------------
+ * Adjoints (FP64).  No counterpart in NcorpiON; collision-free dynamics only
+ * (contact events are not differentiable).
+ *
+ * ncg_vjp_gravity_dev: vector-Jacobian product of the brute-force field
+ * (the mutual accelerations of ncg_gravity_allpairs plus the central term
+ * and the central body's acceleration CM_acc) at bodies d_bodies (n) and
+ * central body d_central (8 doubles, layout of ncg_rk4_steps_dev).  Inputs:
+ * d_lam (3n) cotangent of the body accelerations, d_lam_central (3, may be
+ * NULL) cotangent of CM_acc.  Outputs: d_gx (3n) gradient w.r.t. positions,
+ * d_gm (n) w.r.t. masses, d_gcentral (4) w.r.t. the central position and
+ * mass.
+ *
+ * ncg_rk4_adjoint_dev: discrete adjoint of nsteps FP64 RK4 steps of
+ * ncg_rk4_steps_dev (NCG_F64_FAST) from d_bodies0 / d_central0.  The forward
+ * pass keeps the state every checkpoint_every steps; the backward pass
+ * recomputes each segment from its checkpoint and reverses it stage by stage.
+ * d_gfinal (6n: per body x, y, z, vx, vy, vz) and d_gcentral_final (6, may
+ * be NULL) are the cotangents of the final state; outputs: d_g0 (6n)
+ * gradient w.r.t. the initial state, d_gm (n) w.r.t. masses, d_gcentral0
+ * (7: central x, y, z, vx, vy, vz, M).  d_final (n records, may be NULL)
+ * receives the final state.
+ */
+ncg_status ncg_vjp_gravity_dev(ncg_ctx *ctx, const ncg_body *d_bodies, int32_t n,
+                               const double *d_central, double grav_const, double softening,
+                               const double *d_lam, const double *d_lam_central, double *d_gx,
+                               double *d_gm, double *d_gcentral, void *stream);
+ncg_status ncg_rk4_adjoint_dev(ncg_ctx *ctx, const ncg_body *d_bodies0, int32_t n,
+                               const double *d_central0, double grav_const, double softening,
+                               double dt, int32_t nsteps, int32_t checkpoint_every,
+                               const double *d_gfinal, const double *d_gcentral_final,
+                               double *d_g0, double *d_gm, double *d_gcentral0, ncg_body *d_final,
+                               void *stream);
+
+/* ---------------------------------------------------------------------------
  * Launch shape of the tiled gravity forms (NCG_F64_FAST, NCG_MIXED): bodies
  * per lane (1, 2, 4; 0 keeps), j-splits per i-block (0 = from the CU count,
  * -1 keeps), near/far threshold theta (<= 0 keeps), tile source (0 scalar

@@ -67,6 +67,10 @@ _SIGNATURES = {
     "ncg_mesh_pairs": (S, [P, P, I64, ctypes.POINTER(I64)]),
     "ncg_rk4_steps_dev": (S, [P, P, I32, P, P, D, D, S, D, I32, I32, P]),
     "ncg_set_tiling": (S, [P, I32, I32, D, I32]),
+    "ncg_vjp_gravity_dev": (S, [P, P, I32, P, D, D, P, P, P, P, P, P]),
+    "ncg_rk4_adjoint_dev": (S, [P, P, I32, P, D, D, D, I32, I32, P, P, P, P, P, P, P]),
+    "ncg_ensemble_rk4_dev": (S, [P, P, P, I32, P, D, D, S, D, I32, I32, D, ctypes.POINTER(I64), P]),
+    "ncg_ensemble_gravity_dev": (S, [P, P, P, I32, P, D, D, S, P, P, P]),
     "ncg_profile_enable": (S, [P, S]),
     "ncg_profile_read": (S, [P, ctypes.c_char_p, ctypes.POINTER(D), ctypes.POINTER(I64), S]),
 }
@@ -196,6 +200,47 @@ class Context:
             self.h, bodies_t.data_ptr(), n, None if exists_t is None else exists_t.data_ptr(),
             central_t.data_ptr(), float(G), float(softening), PRECISIONS[precision], float(dt),
             int(nsteps), 1 if graph else 0, stream))
+
+    def ensemble_rk4_dev(self, bodies_t, offsets, centrals_t, nsteps, dt, G=1.0, softening=0.0,
+                         precision="mixed", graph=True, collisions=False, collision_f=1.5,
+                         stream=None):
+        """bodies_t: (sum N, 8) float64 device; offsets: host int sequence of
+        nsys+1; centrals_t: (nsys, 8) float64 device.  Returns collisions."""
+        off = np.ascontiguousarray(offsets, dtype=np.int32)
+        nc = I64()
+        flags = (1 if graph else 0) | (2 if collisions else 0)
+        self._check(self.lib.ncg_ensemble_rk4_dev(
+            self.h, bodies_t.data_ptr(), _ptr(off), len(off) - 1, centrals_t.data_ptr(),
+            float(G), float(softening), PRECISIONS[precision], float(dt), int(nsteps), flags,
+            float(collision_f), ctypes.byref(nc), stream))
+        return int(nc.value)
+
+    def ensemble_gravity_dev(self, bodies_t, offsets, centrals_t, acc_t, cm_t, G=1.0,
+                             softening=0.0, precision="mixed", stream=None):
+        off = np.ascontiguousarray(offsets, dtype=np.int32)
+        self._check(self.lib.ncg_ensemble_gravity_dev(
+            self.h, bodies_t.data_ptr(), _ptr(off), len(off) - 1, centrals_t.data_ptr(),
+            float(G), float(softening), PRECISIONS[precision], acc_t.data_ptr(), cm_t.data_ptr(),
+            stream))
+
+    def vjp_gravity_dev(self, bodies_t, central_t, lam_t, gx_t, gm_t, gc_t, lam_c_t=None, G=1.0,
+                        softening=0.0, stream=None):
+        n = int(bodies_t.shape[0])
+        self._check(self.lib.ncg_vjp_gravity_dev(
+            self.h, bodies_t.data_ptr(), n, central_t.data_ptr(), float(G), float(softening),
+            lam_t.data_ptr(), None if lam_c_t is None else lam_c_t.data_ptr(), gx_t.data_ptr(),
+            gm_t.data_ptr(), gc_t.data_ptr(), stream))
+
+    def rk4_adjoint_dev(self, bodies0_t, central0_t, gfinal_t, g0_t, gm_t, gc0_t, nsteps, dt,
+                        checkpoint_every=20, G=1.0, softening=0.0, gcentral_final_t=None,
+                        final_t=None, stream=None):
+        n = int(bodies0_t.shape[0])
+        self._check(self.lib.ncg_rk4_adjoint_dev(
+            self.h, bodies0_t.data_ptr(), n, central0_t.data_ptr(), float(G), float(softening),
+            float(dt), int(nsteps), int(checkpoint_every), gfinal_t.data_ptr(),
+            None if gcentral_final_t is None else gcentral_final_t.data_ptr(), g0_t.data_ptr(),
+            gm_t.data_ptr(), gc0_t.data_ptr(), None if final_t is None else final_t.data_ptr(),
+            stream))
 
     def set_tiling(self, ipl=0, nsplit=-1, theta=0.0, source=-1):
         self._check(self.lib.ncg_set_tiling(self.h, int(ipl), int(nsplit), float(theta),

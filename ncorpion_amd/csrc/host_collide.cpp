@@ -1,0 +1,139 @@
+// Built-in host collision decision and outcome for callers that are not
+// NcorpiON itself (the ensemble driver, Python users).  It restates, for one
+// body array, closest_approach (collision.c:63-172) and collision_treatment
+// with elastic (f = 2) or inelastic (f = collision_parameter) outcomes
+// (physics.c:545-605, 1043-1070), in the same FP64 operation order.  The
+// drop-in glue does not use it: there the reference's own functions decide.
+#include "ncg_internal.hpp"
+
+#pragma clang fp contract(off)
+
+namespace {
+
+struct Builtin {
+    ncg_body *b;
+    const int32_t *exists;
+    int32_t *did_collide;  // may be null
+    double dt;
+    double f;
+    int one_collision_only;
+    double approach[6];
+    double t_c;
+};
+
+int builtin_test(void *u, int32_t a, int32_t bb)
+{
+    Builtin *s = static_cast<Builtin *>(u);
+    if (s->exists && !(s->exists[a] && s->exists[bb])) return 0;
+    const ncg_body &A = s->b[a], &B = s->b[bb];
+    const double dx = A.x - B.x, dy = A.y - B.y, dz = A.z - B.z;
+    const double dvx = A.vx - B.vx, dvy = A.vy - B.vy, dvz = A.vz - B.vz;
+    const double dr_dot_dv = dx * dvx + dy * dvy + dz * dvz;
+    if (dr_dot_dv >= 0.0) return 0;
+    const double dr2 = dx * dx + dy * dy + dz * dz;
+    const double dv2 = dvx * dvx + dvy * dvy + dvz * dvz;
+    const double R = A.radius + B.radius;
+    const double disc = dr_dot_dv * dr_dot_dv + dv2 * (R * R - dr2);
+    if (disc <= 0.0) return 0;
+    const double t = -(dr_dot_dv + sqrt(disc)) / dv2;
+    s->t_c = t;
+    if (t > s->dt) return 0;
+    const double xa = A.x + t * A.vx, ya = A.y + t * A.vy, za = A.z + t * A.vz;
+    const double xb = B.x + t * B.vx, yb = B.y + t * B.vy, zb = B.z + t * B.vz;
+    if (A.mass > B.mass) {
+        s->approach[0] = xa; s->approach[1] = ya; s->approach[2] = za;
+        s->approach[3] = xb; s->approach[4] = yb; s->approach[5] = zb;
+    } else {
+        s->approach[0] = xb; s->approach[1] = yb; s->approach[2] = zb;
+        s->approach[3] = xa; s->approach[4] = ya; s->approach[5] = za;
+    }
+    if (s->did_collide && (s->did_collide[a] || s->did_collide[bb])) return 0;
+    return 1;
+}
+
+void builtin_resolve(void *u, int32_t a, int32_t bb)
+{
+    Builtin *s = static_cast<Builtin *>(u);
+    const int A = s->b[a].mass > s->b[bb].mass ? a : bb;
+    const int B = A == a ? bb : a;
+    ncg_body &P = s->b[A], &Q = s->b[B];
+    const double R = P.radius + Q.radius;
+    const double m_a = P.mass, m_b = Q.mass;
+    double vx_a = P.vx, vy_a = P.vy, vz_a = P.vz, vx_b = Q.vx, vy_b = Q.vy, vz_b = Q.vz;
+    const double *ap = s->approach;
+    const double xa = ap[0], ya = ap[1], za = ap[2], xb = ap[3], yb = ap[4], zb = ap[5];
+    const double dr_dot_dv = (xa - xb) * (vx_a - vx_b) + (ya - yb) * (vy_a - vy_b) + (za - zb) * (vz_a - vz_b);
+    const double alpha = s->f * m_a * m_b / ((m_a + m_b) * R * R);
+    const double Jx = alpha * dr_dot_dv * (xb - xa);
+    const double Jy = alpha * dr_dot_dv * (yb - ya);
+    const double Jz = alpha * dr_dot_dv * (zb - za);
+    vx_a += Jx / m_a;
+    vy_a += Jy / m_a;
+    vz_a += Jz / m_a;
+    vx_b -= Jx / m_b;
+    vy_b -= Jy / m_b;
+    vz_b -= Jz / m_b;
+    const double t = s->t_c;
+    P.x = xa - t * vx_a; P.y = ya - t * vy_a; P.z = za - t * vz_a;
+    P.vx = vx_a; P.vy = vy_a; P.vz = vz_a;
+    Q.x = xb - t * vx_b; Q.y = yb - t * vy_b; Q.z = zb - t * vz_b;
+    Q.vx = vx_b; Q.vy = vy_b; Q.vz = vz_b;
+    if (s->did_collide) {
+        s->did_collide[a] = s->one_collision_only;
+        s->did_collide[bb] = s->one_collision_only;
+    }
+}
+
+}  // namespace
+
+// Replays one brute-force collision pass over `bodies` (host) given the sorted
+// candidate list, with the built-in decision and outcome.  Same visiting rules
+// as ncg_brute_force_pass; returns the number of collisions.
+int64_t ncg_builtin_bf_replay(ncg_body *bodies, int32_t n, const int32_t *exists, double dt,
+                              double f, const std::vector<ncg_pair> &cand,
+                              std::vector<ncg_pair> *log)
+{
+    Builtin u{bodies, exists, nullptr, dt, f, 0, {0, 0, 0, 0, 0, 0}, 0.0};
+    std::vector<char> dirty((size_t)n, 0);
+    std::vector<int32_t> dlist;  // dirty bodies, kept sorted
+    int64_t ncoll = 0;
+    size_t ci = 0;
+    auto live = [&](int32_t k) { return !exists || exists[k] != 0; };
+    auto mark = [&](int32_t k) {
+        if (dirty[k]) return;
+        dirty[k] = 1;
+        dlist.insert(std::upper_bound(dlist.begin(), dlist.end(), k), k);
+    };
+    for (int32_t i = 0; i < n; ++i) {
+        while (ci < cand.size() && cand[ci].fst < i) ++ci;
+        if (!live(i)) continue;
+        int32_t jc = i + 1;
+        while (jc < n) {
+            if (dirty[i]) {
+                for (int32_t j = jc; j < n; ++j)
+                    if (live(j) && builtin_test(&u, i, j)) {
+                        builtin_resolve(&u, i, j);
+                        ++ncoll;
+                        mark(j);
+                        if (log) log->push_back({i, j});
+                    }
+                break;
+            }
+            while (ci < cand.size() && cand[ci].fst == i && cand[ci].snd < jc) ++ci;
+            const int32_t j_cand = (ci < cand.size() && cand[ci].fst == i) ? cand[ci].snd : n;
+            auto it = std::lower_bound(dlist.begin(), dlist.end(), jc);
+            const int32_t j_dirty = it == dlist.end() ? n : *it;
+            const int32_t j = std::min(j_cand, j_dirty);
+            if (j >= n) break;
+            if (live(j) && builtin_test(&u, i, j)) {
+                builtin_resolve(&u, i, j);
+                ++ncoll;
+                mark(i);
+                mark(j);
+                if (log) log->push_back({i, j});
+            }
+            jc = j + 1;
+        }
+    }
+    return ncoll;
+}

@@ -44,7 +44,9 @@ void *ncg_dbuf(ncg_ctx *ctx, const char *name, size_t bytes)
 }
 
 // Reads the tuning table: $NCG_TUNING, else ncg_tuning.txt beside libncg.so.
-// Lines: <exact|fast64|mixed> <n_min> <ipl> <nsplit> <theta> <src>; '#' comments.
+// Lines: <form> <n_min> <ipl> <nsplit> <theta> <src>; '#' comments.  Forms:
+// fast64, mixed (single-system tiled kernel, n_min = bodies) and ens_fast64,
+// ens_mixed (ensemble kernel, n_min = mean bodies per system).
 static void load_tuning(ncg_ctx *ctx)
 {
     std::string path;
@@ -72,7 +74,11 @@ static void load_tuning(ncg_ctx *ctx)
         if (sscanf(line, "%31s %ld %d %d %lf %d", pname, &nmin, &ipl, &nsplit, &theta, &src) != 6)
             continue;
         TuneRow r;
-        r.prec = !strcmp(pname, "mixed") ? NCG_MIXED : !strcmp(pname, "fast64") ? NCG_F64_FAST : -1;
+        r.prec = !strcmp(pname, "mixed")        ? NCG_MIXED
+                 : !strcmp(pname, "fast64")     ? NCG_F64_FAST
+                 : !strcmp(pname, "ens_mixed")  ? 100 + NCG_MIXED
+                 : !strcmp(pname, "ens_fast64") ? 100 + NCG_F64_FAST
+                                                : -1;
         if (r.prec < 0 || (ipl != 1 && ipl != 2 && ipl != 4) || nsplit < 0 || theta <= 0.0 ||
             (src != 0 && src != 1))
             continue;

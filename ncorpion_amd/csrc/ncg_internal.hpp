@@ -176,6 +176,11 @@ ncg_status ncg_tiled_eval(ncg_ctx *ctx, const DBody *d_b, int32_t n, const int32
                           const ncg_central *central, const double *d_central, double G,
                           double soft, ncg_precision prec, double *d_acc, double *d_cterm,
                           double *d_cm_acc, hipStream_t s);
+// Stage s (0..3) update of the RK4 driver (rk4.hip), see k_rk4_stage there.
+ncg_status ncg_rk4_stage_launch(ncg_ctx *ctx, int s, double dt, int32_t n, const int32_t *ex,
+                                DBody *y0, DBody *Y, const double *acc, double *sx, double *sv,
+                                double *c0, double *Yc, const double *cm, double *sc,
+                                hipStream_t st);
 ncg_status ncg_gravity_dev_impl(ncg_ctx *ctx, const DBody *d_bodies, int32_t n,
                                 const int32_t *d_exists, const ncg_central *central, double G,
                                 double soft, ncg_precision prec, double *d_acc,

@@ -124,6 +124,17 @@ __global__ __launch_bounds__(NT) void k_rk4_stage(int s, double dt, int n,
 
 }  // namespace
 
+ncg_status ncg_rk4_stage_launch(ncg_ctx *ctx, int s, double dt, int32_t n, const int32_t *ex,
+                                DBody *y0, DBody *Y, const double *acc, double *sx, double *sv,
+                                double *c0, double *Yc, const double *cm, double *sc,
+                                hipStream_t st)
+{
+    hipLaunchKernelGGL(k_rk4_stage, dim3(cdiv(n, NT)), dim3(NT), 0, st, s, dt, (int)n, ex, y0, Y, acc,
+                       sx, sv, c0, Yc, cm, sc);
+    NCG_CHECK_LAUNCH();
+    return NCG_OK;
+}
+
 // The 4 evaluations + stage updates of one step (no allocation, no sync).
 static ncg_status rk4_body(ncg_ctx *ctx, DBody *y0, int32_t n, const int32_t *ex, double *c0,
                            double G, double soft, ncg_precision prec, double dt, hipStream_t st,

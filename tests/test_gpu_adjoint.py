@@ -1,0 +1,156 @@
+"""Hand-written adjoints (libncg.so) against dense FP64 PyTorch autograd and
+central finite differences.  Collision-free dynamics only.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from ncorpion_amd import ic
+
+pytestmark = pytest.mark.gpu
+
+
+def tt():
+    import torch
+    return torch
+
+
+def torch_field(x, m, R, cen, G, soft):
+    """Dense FP64 reference: mutual + central accelerations and CM_acc."""
+    torch = tt()
+    n = x.shape[0]
+    d = x[None, :, :] - x[:, None, :]  # d[i, j] = x_j - x_i
+    e = soft * (R[:, None] + R[None, :])
+    eye = torch.eye(n, dtype=torch.float64, device=x.device)
+    D2 = (d * d).sum(-1) + e * e + eye
+    inv3 = (1.0 - eye) * D2 ** -1.5
+    a = G * (m[None, :, None] * d * inv3[..., None]).sum(1)
+    C, M = cen[0:3], cen[6]
+    ec = x - C
+    r3 = (ec * ec).sum(1) ** 1.5
+    a = a - G * M * ec / r3[:, None]
+    acm = G * (m[:, None] * ec / r3[:, None]).sum(0)
+    return a, acm
+
+
+def torch_rk4(x, v, m, R, C, Vc, M, G, soft, dt, nsteps):
+    torch = tt()
+
+    def f(x, v, C, Vc):
+        cen = torch.cat([C, Vc, M.reshape(1), torch.ones(1, dtype=torch.float64, device=x.device)])
+        a, acm = torch_field(x, m, R, cen, G, soft)
+        return v, a, Vc, acm
+
+    for _ in range(nsteps):
+        k1 = f(x, v, C, Vc)
+        k2 = f(*(s + 0.5 * dt * k for s, k in zip((x, v, C, Vc), k1)))
+        k3 = f(*(s + 0.5 * dt * k for s, k in zip((x, v, C, Vc), k2)))
+        k4 = f(*(s + dt * k for s, k in zip((x, v, C, Vc), k3)))
+        x, v, C, Vc = (s + dt / 6.0 * (a + 2 * b + 2 * c + d)
+                       for s, a, b, c, d in zip((x, v, C, Vc), k1, k2, k3, k4))
+    return x, v, C, Vc
+
+
+def disk(n, seed):
+    b = ic.disk(n, seed=seed, mass=1e-5, radius=1e-3)
+    return b
+
+
+def test_vjp_matches_torch_autograd(ctx):
+    torch = tt()
+    from ncorpion_amd.autograd import GravityField
+    n, G, soft = 300, 1.3, 0.02
+    b = disk(n, 1)
+    x = torch.tensor(b[:, 0:3], requires_grad=True, device="cuda")
+    m = torch.tensor(b[:, 6], requires_grad=True, device="cuda")
+    R = torch.tensor(b[:, 7], device="cuda")
+    cen = torch.tensor([0.01, -0.02, 0.003, 0, 0, 0, 1.0, 1.0], dtype=torch.float64,
+                       device="cuda", requires_grad=True)
+    rng = np.random.default_rng(3)
+    lam = torch.tensor(rng.normal(size=(n, 3)), device="cuda")
+    lamc = torch.tensor(rng.normal(size=3), device="cuda")
+    a, acm = torch_field(x, m, R, cen, G, soft)
+    ga = torch.autograd.grad((a * lam).sum() + (acm * lamc).sum(), (x, m, cen))
+    a2, acm2 = GravityField.apply(x, m, R, cen, G, soft, ctx)
+    assert torch.allclose(a2, a.detach(), rtol=1e-12, atol=1e-15)
+    assert torch.allclose(acm2, acm.detach(), rtol=1e-11, atol=1e-18)
+    gb = torch.autograd.grad((a2 * lam).sum() + (acm2 * lamc).sum(), (x, m, cen))
+    for u, w in zip(ga, gb):
+        assert torch.allclose(w, u, rtol=1e-10, atol=1e-12 * float(u.abs().max())), (u, w)
+
+
+@pytest.mark.parametrize("nsteps,K", [(12, 5), (7, 20)])
+def test_rk4_adjoint_matches_torch_autograd(ctx, nsteps, K):
+    torch = tt()
+    from ncorpion_amd.autograd import RK4Trajectory
+    n, G, soft, dt = 96, 1.0, 0.01, 2 * math.pi / 256
+    b = disk(n, 2)
+    x0 = torch.tensor(b[:, 0:3], requires_grad=True, device="cuda")
+    v0 = torch.tensor(b[:, 3:6], requires_grad=True, device="cuda")
+    m = torch.tensor(b[:, 6], requires_grad=True, device="cuda")
+    R = torch.tensor(b[:, 7], device="cuda")
+    C = torch.zeros(3, dtype=torch.float64, device="cuda", requires_grad=True)
+    Vc = torch.zeros(3, dtype=torch.float64, device="cuda", requires_grad=True)
+    M = torch.tensor(1.0, dtype=torch.float64, device="cuda", requires_grad=True)
+    rng = np.random.default_rng(4)
+    wx = torch.tensor(rng.normal(size=(n, 3)), device="cuda")
+    wv = torch.tensor(rng.normal(size=(n, 3)), device="cuda")
+    xT, vT, CT, VT = torch_rk4(x0, v0, m, R, C, Vc, M, G, soft, dt, nsteps)
+    L = (xT * wx).sum() + (vT * wv).sum() + CT.sum() * 0.3
+    gref = torch.autograd.grad(L, (x0, v0, m, C, Vc, M))
+    cen0 = torch.cat([C, Vc, M.reshape(1), torch.ones(1, dtype=torch.float64, device="cuda")])
+    xT2, vT2, cT2 = RK4Trajectory.apply(x0, v0, m, R, cen0, G, soft, dt, nsteps, K, ctx)
+    assert torch.allclose(xT2, xT.detach(), rtol=0, atol=1e-12)
+    L2 = (xT2 * wx).sum() + (vT2 * wv).sum() + cT2[0:3].sum() * 0.3
+    g = torch.autograd.grad(L2, (x0, v0, m, cen0))
+    gC, gVc, gM = g[3][0:3], g[3][3:6], g[3][6]
+    for u, w in zip(gref, (g[0], g[1], g[2], gC, gVc, gM)):
+        scale = float(u.abs().max()) + 1e-30
+        assert float((w - u).abs().max()) <= 1e-9 * scale, (u, w)
+
+
+def test_rk4_adjoint_matches_central_differences(ctx):
+    """Loss = mean squared distance of final positions to a seeded target;
+    adjoint directional derivatives vs central FD (h = 1e-6)."""
+    torch = tt()
+    n, G, soft, dt, nsteps, K = 512, 1.0, 0.0, 2 * math.pi / 256, 40, 10
+    b = ic.disk(n, seed=5, mass=1e-8, radius=2e-4)
+    bt0 = torch.from_numpy(b).cuda()
+    c0 = torch.tensor([0, 0, 0, 0, 0, 0, 1.0, 1.0], dtype=torch.float64, device="cuda")
+
+    def forward(bt):
+        bb, cc = bt.clone(), c0.clone()
+        torch.cuda.synchronize()
+        ctx.rk4_steps_dev(bb, cc, nsteps, dt, G, soft, "fast64", graph=False)
+        ctx.synchronize()
+        return bb
+
+    final = forward(bt0)
+    rng = np.random.default_rng(6)
+    target = final[:, 0:3] + 1e-3 * torch.tensor(rng.normal(size=(n, 3)), device="cuda")
+
+    def loss(bt):
+        f = forward(bt)
+        return float(((f[:, 0:3] - target) ** 2).sum() / n)
+
+    gfin = torch.zeros((n, 6), dtype=torch.float64, device="cuda")
+    gfin[:, 0:3] = 2.0 / n * (final[:, 0:3] - target)
+    g0 = torch.zeros((n, 6), dtype=torch.float64, device="cuda")
+    gm = torch.zeros(n, dtype=torch.float64, device="cuda")
+    gc = torch.zeros(7, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    ctx.rk4_adjoint_dev(bt0, c0, gfin, g0, gm, gc, nsteps, dt, K, G, soft)
+    ctx.synchronize()
+    h = 1e-6
+    worst = 0.0
+    for q in range(8):
+        r = np.random.default_rng(100 + q)
+        dirn = np.zeros((n, 8))
+        dirn[:, 0:6] = r.normal(size=(n, 6))
+        dirn[:, 6] = r.normal(size=n) * b[:, 6]   # relative mass perturbation
+        dt_ = torch.tensor(dirn, device="cuda")
+        fd = (loss(bt0 + h * dt_) - loss(bt0 - h * dt_)) / (2 * h)
+        ad = float((g0 * dt_[:, 0:6]).sum() + (gm * dt_[:, 6]).sum())
+        worst = max(worst, abs(fd - ad) / abs(fd))
+    assert worst < 1e-6, worst

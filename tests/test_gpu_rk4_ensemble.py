@@ -1,0 +1,172 @@
+"""Device RK4 driver and batched ensembles against the FP64 oracle.
+
+NcorpiON has no RK4 and no ensembles; these are pinned by the oracle's RK4
+(4th-order slope and two-body closed form, tests/test_oracle_closed_forms.py),
+by per-system equality with single-system runs, and by graph replays being
+bit-identical to eager launches.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle_lib as orc
+from ncorpion_amd import ic
+
+pytestmark = pytest.mark.gpu
+
+
+def torch_dev():
+    import torch
+    return torch
+
+
+def small_disk(n, seed, mass=1e-7, radius=1e-3):
+    return ic.disk(n, seed=seed, mass=mass, radius=radius)
+
+
+@pytest.mark.parametrize("prec,tol", [("fast64", 1e-12), ("mixed", 2e-7)])
+def test_rk4_driver_matches_oracle(ctx, prec, tol):
+    torch = torch_dev()
+    n, dt, steps = 600, 2 * math.pi / 256, 5
+    b = small_disk(n, 3)
+    c7 = np.array([0, 0, 0, 0, 0, 0, 1.0])
+    ref, rc = b.copy(), c7.copy()
+    for _ in range(steps):
+        ref, rc = orc.rk4_step(ref, rc, 1.0, 0.01, dt)
+    bt = torch.from_numpy(b).cuda()
+    ct = torch.tensor([0, 0, 0, 0, 0, 0, 1.0, 1.0], dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    ctx.rk4_steps_dev(bt, ct, steps, dt, 1.0, 0.01, prec, graph=False)
+    ctx.synchronize()
+    out = bt.cpu().numpy()
+    # Displacement over the run vs its error.
+    moved = np.abs(ref[:, :6] - b[:, :6]).max()
+    err = np.abs(out[:, :6] - ref[:, :6]).max()
+    assert err < tol * moved, (err, moved)
+    cen = ct.cpu().numpy()
+    assert np.allclose(cen[:6], rc[:6], rtol=0, atol=tol * max(1e-12, np.abs(rc[:6]).max()) + 1e-18)
+
+
+def test_rk4_graph_replays_bitwise_equal_to_eager(ctx):
+    torch = torch_dev()
+    n, dt = 3000, 2 * math.pi / 256
+    b = small_disk(n, 4)
+    res = []
+    for graph in (False, True):
+        bt = torch.from_numpy(b).cuda()
+        ct = torch.tensor([0, 0, 0, 0, 0, 0, 1.0, 1.0], dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        for _ in range(3):  # several replays with host synchronisation between them
+            ctx.rk4_steps_dev(bt, ct, 2, dt, 1.0, 1e-3, "mixed", graph=graph)
+            ctx.synchronize()
+        res.append((bt.cpu().numpy(), ct.cpu().numpy()))
+    assert np.array_equal(res[0][0].view(np.uint64), res[1][0].view(np.uint64))
+    assert np.array_equal(res[0][1].view(np.uint64), res[1][1].view(np.uint64))
+
+
+def make_ensemble(sizes, seed0=0, mass=1e-7, radius=1e-3):
+    systems = [small_disk(int(nn), seed0 + s, mass, radius) for s, nn in enumerate(sizes)]
+    off = np.concatenate([[0], np.cumsum([len(x) for x in systems])]).astype(np.int32)
+    return systems, np.concatenate(systems), off
+
+
+@pytest.mark.parametrize("prec", ["fast64", "mixed"])
+def test_ensemble_gravity_matches_per_system_oracle(ctx, prec):
+    torch = torch_dev()
+    sizes = [1, 7, 64, 255, 256, 257, 513, 1000, 2049, 3001]
+    systems, allb, off = make_ensemble(sizes)
+    S = len(sizes)
+    cen = np.zeros((S, 8))
+    cen[:, 6] = 1.0
+    cen[:, 7] = 1.0
+    cen[::2, 7] = 0.0  # half the systems without a central body
+    bt = torch.from_numpy(allb).cuda()
+    ct = torch.from_numpy(cen).cuda()
+    at = torch.zeros((len(allb), 3), dtype=torch.float64, device="cuda")
+    cm = torch.zeros((S, 3), dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    ctx.ensemble_gravity_dev(bt, off, ct, at, cm, 1.0, 0.01, prec)
+    ctx.synchronize()
+    acc, cma = at.cpu().numpy(), cm.cpu().numpy()
+    for s, sysb in enumerate(systems):
+        present = cen[s, 7] != 0
+        ref, rcm = orc.vector_field_brute(sysb, None, (0, 0, 0, 1.0) if present else None, 1.0, 0.01)
+        got = acc[off[s]:off[s + 1]]
+        if present:
+            tol = 1e-13 if prec == "fast64" else 1e-9
+            assert np.abs(got - ref).max() <= tol * np.abs(ref).max()
+            assert np.allclose(cma[s], rcm, rtol=1e-12, atol=0)
+        elif len(sysb) > 1:
+            d = np.linalg.norm(got - ref, axis=1) / np.linalg.norm(ref, axis=1)
+            assert d.max() < (1e-12 if prec == "fast64" else 5e-6), d.max()
+
+
+def test_ensemble_rk4_matches_single_systems_and_graph(ctx):
+    torch = torch_dev()
+    rng = np.random.default_rng(7)
+    sizes = np.rint(np.exp(rng.uniform(np.log(16), np.log(1500), 24))).astype(int)
+    systems, allb, off = make_ensemble(sizes, seed0=100)
+    S = len(sizes)
+    cen = np.zeros((S, 8))
+    cen[:, 6] = 1.0
+    cen[:, 7] = 1.0
+    dt, steps = 2 * math.pi / 256, 3
+    outs = []
+    for graph in (False, True):
+        bt = torch.from_numpy(allb).cuda()
+        ct = torch.from_numpy(cen).cuda()
+        torch.cuda.synchronize()
+        ctx.ensemble_rk4_dev(bt, off, ct, steps, dt, 1.0, 0.01, "fast64", graph=graph)
+        ctx.synchronize()
+        outs.append((bt.cpu().numpy(), ct.cpu().numpy()))
+    assert np.array_equal(outs[0][0].view(np.uint64), outs[1][0].view(np.uint64))
+    out, cout = outs[1]
+    for s, sysb in enumerate(systems):
+        ref, rc = sysb.copy(), np.array([0, 0, 0, 0, 0, 0, 1.0])
+        for _ in range(steps):
+            ref, rc = orc.rk4_step(ref, rc, 1.0, 0.01, dt)
+        got = out[off[s]:off[s + 1]]
+        moved = np.abs(ref[:, :6] - sysb[:, :6]).max()
+        assert np.abs(got[:, :6] - ref[:, :6]).max() < 1e-11 * moved
+        assert np.allclose(cout[s, :6], rc[:6], rtol=1e-9, atol=1e-16)
+
+
+def test_ensemble_collisions_match_oracle(ctx):
+    torch = torch_dev()
+    # Dense small systems so that collisions happen every step.
+    rng = np.random.default_rng(11)
+    sizes = rng.integers(50, 400, 12)
+    systems = []
+    for s, nn in enumerate(sizes):
+        b = np.zeros((nn, 8))
+        r2 = np.random.default_rng(200 + s)
+        b[:, 0:3] = r2.uniform(-0.03, 0.03, (nn, 3)) + np.array([1.0, 0, 0])
+        b[:, 3:6] = r2.normal(0, 0.3, (nn, 3)) + np.array([0, 1.0, 0])
+        b[:, 6] = r2.uniform(1e-12, 1e-11, nn)
+        b[:, 7] = r2.uniform(1e-3, 2e-3, nn)
+        systems.append(b)
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)
+    allb = np.concatenate(systems)
+    S = len(sizes)
+    cen = np.zeros((S, 8))
+    cen[:, 6] = 1.0
+    cen[:, 7] = 1.0
+    dt, steps = 0.002, 2
+    bt = torch.from_numpy(allb).cuda()
+    ct = torch.from_numpy(cen).cuda()
+    torch.cuda.synchronize()
+    nc = ctx.ensemble_rk4_dev(bt, off, ct, steps, dt, 1.0, 0.0, "fast64", graph=True,
+                              collisions=True, collision_f=1.5)
+    ctx.synchronize()
+    out = bt.cpu().numpy()
+    total = 0
+    for s, sysb in enumerate(systems):
+        ref, rc = sysb.copy(), np.array([0, 0, 0, 0, 0, 0, 1.0])
+        for _ in range(steps):
+            ref, _, pairs = orc.brute_force(ref, dt, 1, 1.5, 0)
+            total += len(pairs)
+            ref, rc = orc.rk4_step(ref, rc, 1.0, 0.0, dt)
+        got = out[off[s]:off[s + 1]]
+        assert np.abs(got[:, :6] - ref[:, :6]).max() < 1e-9
+    assert nc == total and total > 20
