@@ -32,6 +32,13 @@ PEAK_FP32_VECTOR_TFLOPS = 157.3    # MI355X published FP32 vector peak
 PEAK_FP64_VECTOR_TFLOPS = 78.6
 
 
+# HBM bytes per launch of the mixed pair kernel at N = 262144 with the tuned
+# shape (nsplit 128): FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (scales from a
+# 256 MiB copy), from rocprofv3 --pmc runs of this script, summarised in
+# profiles/k_pairs_mixed_n262144_pmc.md.  Mostly the j-split partial sums.
+PAIRS_TRAFFIC_BYTES = (110702 * 2.000 + 786432 * 1.000) * 1024
+
+
 def make_workload(n, seed=0):
     from ncorpion_amd import ic
     m, R = ic.disk_c3_masses(n, seed)
@@ -67,6 +74,121 @@ def cpu_baseline(bodies, n_sample, timeout=300):
                       "of the workload, one evaluation of %.1f s" % (n, secs)}
 
 
+MESH_N = 1 << 20
+MESH_GAM, MESH_CELLS, MESH_CUBE = 0.01, 256, 2.56
+# Algorithmic bytes per body of one binning pass: read the 64-byte record,
+# write a 64-byte sorted copy, write and read the 4-byte key, write the
+# 4-byte permutation.  (The dense cell table is not swept: only the cells of
+# the previous and current pass are touched.)
+MESH_BIN_BYTES_PER_BODY = 64 + 64 + 4 + 4 + 4
+
+
+def mesh_workload(n=MESH_N, seed=0):
+    from ncorpion_amd import ic
+    return ic.disk(n, seed=seed, mass=1e-10, radius=5e-5)
+
+
+def mesh_run(ctx, torch, passes=3):
+    """Mesh module on a disk of 2^20 bodies (m = 1e-10, R = 5e-5) with cells
+    of side 0.01 (256 per axis, cube 2.56), dt = 2 pi / 512:
+    get_neighbours_mesh passes (mode 1), mesh() passes with the library's
+    built-in inelastic bounce (mode 0, f = 1.5), and the neighbour-cell
+    gravity in its three forms.  Kernel times from HIP events."""
+    import numpy as np
+    b0 = mesh_workload()
+    n = b0.shape[0]
+    dt = 2.0 * math.pi / 512.0
+    ex = np.ones(n, np.int32)
+    out = {"n": n, "cells": MESH_CELLS, "gam": MESH_GAM}
+    # Warm-up: buffers, dense cell table.
+    ctx.mesh_pass(b0.copy(), ex, dt, MESH_GAM, MESH_CUBE, MESH_CELLS, 1)
+    ctx.profile(True)
+    for k in ("mesh_bin", "mesh_count", "mesh_gravity"):
+        ctx.profile_read(k, reset=True)
+    b = b0.copy()
+    t0 = time.perf_counter()
+    for _ in range(passes):
+        st = ctx.mesh_pass(b, ex, dt, MESH_GAM, MESH_CUBE, MESH_CELLS, 1)
+    t_gnm = (time.perf_counter() - t0) / passes
+    bin_ms, nb = ctx.profile_read("mesh_bin", reset=True)
+    cnt_ms, nc = ctx.profile_read("mesh_count", reset=True)
+    P = st["total_neighbours"]
+    out["get_neighbours_mesh"] = {
+        "seconds_per_pass": t_gnm, "total_neighbours": P,
+        "bin_ms": bin_ms / nb, "count_ms": cnt_ms / nc,
+        "bin_GBps": MESH_BIN_BYTES_PER_BODY * n / (bin_ms / nb * 1e-3) / 1e9,
+        "note": "pass time includes the 64 MiB host-to-device upload"}
+    # mesh() with collisions.
+    coll = 0
+    t_mesh = 0.0
+    for _ in range(passes):
+        b = b0.copy()
+        dc = np.zeros(n, np.int32)
+        ops = ctx.builtin_ops(b, dt, exists=ex, did_collide=dc, f=1.5)
+        t0 = time.perf_counter()
+        st = ctx.mesh_pass(b, ex, dt, MESH_GAM, MESH_CUBE, MESH_CELLS, 0, ops=ops)
+        t_mesh += time.perf_counter() - t0
+        coll = ops.collisions
+    bin_ms, nb = ctx.profile_read("mesh_bin", reset=True)
+    cnt_ms, nc = ctx.profile_read("mesh_count", reset=True)
+    out["mesh"] = {"seconds_per_pass": t_mesh / passes, "collisions": coll,
+                   "candidates": st["candidates"], "host_tests": st["host_tests"],
+                   "total_neighbours": st["total_neighbours"],
+                   "bin_ms": bin_ms / nb, "count_ms": cnt_ms / nc,
+                   "predicate_tests_per_s": st["total_neighbours"] / (cnt_ms / nc * 1e-3)}
+    # Neighbour-cell gravity at the post-pass state (every pair both ways).
+    grav = {}
+    ref = None
+    for prec in ("exact", "fast64", "mixed"):
+        acc, _ = ctx.mesh_gravity(b, ex, None, 1.0, 0.0, prec)
+        acc, _ = ctx.mesh_gravity(b, ex, None, 1.0, 0.0, prec)
+        ms, k = ctx.profile_read("mesh_gravity", reset=True)
+        ms /= k
+        g = {"kernel_ms": ms,
+             "pair_terms_per_s": 2.0 * st["total_neighbours"] / (ms * 1e-3)}
+        if prec == "exact":
+            ref = acc
+        else:
+            d = np.linalg.norm(acc - ref, axis=1)
+            r = np.linalg.norm(ref, axis=1)
+            g["max_rel_err_vs_exact"] = float((d / r).max())
+            g["normwise_err_vs_exact"] = float(d.max() / np.sqrt(np.mean(r * r)))
+        grav[prec] = g
+    out["gravity"] = grav
+    ctx.profile(False)
+    return out
+
+
+def cpu_mesh_baseline(timeout=300):
+    """NcorpiON's own get_neighbours_mesh (compiled from source, 1 thread) on
+    the full 2^20-body mesh workload, through the function-level probe."""
+    import numpy as np
+    probe = os.path.join(ROOT, "oracle", "_ref", "probe_c2", "ref_probe")
+    if not os.path.exists(probe):
+        return None
+    b = mesh_workload()
+    n = b.shape[0]
+    with tempfile.TemporaryDirectory() as td:
+        fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        with open(fin, "wb") as f:
+            f.write(struct.pack("<4i", 0x4E434750, 3 | 0x100, n, 0))
+            f.write(np.array([2.0 * math.pi / 512.0, MESH_GAM, MESH_CUBE, 0, 0, 0, 0, 0]).tobytes())
+            f.write(np.array([0, 0, 0, 0, 0, 0, 1.0, 0.05]).tobytes())
+            f.write(b.tobytes())
+            f.write(np.ones(n, np.int32).tobytes())
+            f.write(np.zeros(n, np.int32).tobytes())
+        r = subprocess.run([probe, fin, fout], stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                           text=True, timeout=timeout)
+        if r.returncode != 0:
+            return None
+        secs = float(r.stderr.strip().split("op_seconds=")[-1].split()[0])
+        hdr = np.fromfile(fout, dtype=np.int32, count=8)
+    return {"value": 1.0 / secs, "unit": "get_neighbours_mesh passes/s", "cores": 1,
+            "kind": "reference", "seconds": secs, "total_neighbours": int(hdr[4]),
+            "sample": "NcorpiON get_neighbours_mesh (gcc -O2, 1 thread), one call on the full "
+                      "2^20-body mesh workload"}
+
+
 def ensemble_run(ctx, torch, nsys=512, steps=100):
     """512 independent disks (seeds 0..511), N log-uniform in [256, 8192],
     m = 1e-8, R = 2e-4, brute-force gravity + collision detection, mixed form,
@@ -96,21 +218,57 @@ def ensemble_run(ctx, torch, nsys=512, steps=100):
             "collisions": nc, "precision": "mixed", "graph": True}
 
 
+def precision_run(ctx, torch, b, soft, dt, steps=20):
+    """Energy drift over `steps` RK4 steps of the workload in the mixed and
+    the FP64 form (the FP64 energy kernel measures both), and how far the two
+    final states are apart."""
+    dev = torch.device("cuda:0")
+    out, finals = {}, {}
+    for prec in ("mixed", "fast64"):
+        bt = torch.from_numpy(b).to(dev)
+        ct = torch.tensor([0, 0, 0, 0, 0, 0, 1.0, 1.0], dtype=torch.float64, device=dev)
+        torch.cuda.synchronize()
+        e0 = ctx.energy_dev(bt, ct, None, 1.0, soft).clone()
+        ctx.rk4_steps_dev(bt, ct, steps, dt, 1.0, soft, prec, graph=True)
+        e1 = ctx.energy_dev(bt, ct, None, 1.0, soft).clone()
+        ctx.synchronize()
+        e0v, e1v = e0.cpu().numpy(), e1.cpu().numpy()
+        out["energy_drift_" + prec] = float((e1v[3] - e0v[3]) / abs(e0v[3]))
+        finals[prec] = bt
+    out["energy_drift_difference"] = out["energy_drift_mixed"] - out["energy_drift_fast64"]
+    out["max_position_difference"] = float((finals["mixed"][:, 0:3] - finals["fast64"][:, 0:3])
+                                           .abs().max())
+    out["steps"] = steps
+    return out
+
+
+ADJ_SOFT = 1.0  # softening length R_i + R_j: smooth field without collisions
+
+
 def adjoint_run(ctx, torch, n=16384, steps=200, K=20):
-    """Collision-free disk of 16384 bodies, 200 FP64 RK4 steps dt = 2 pi/256;
-    loss = mean squared distance of the final positions to a seeded target
-    (sigma 1e-3); adjoint w.r.t. initial positions, velocities and masses with
-    a checkpoint every 20 steps."""
+    """Collision-free disk of 16384 bodies (m = 1e-8, R = 2e-4, softening
+    length R_i + R_j), 200 FP64 RK4 steps dt = 2 pi/256; loss = mean squared
+    distance of the final positions to a seeded target (sigma 1e-3); adjoint
+    w.r.t. initial positions, velocities and masses with a checkpoint every
+    20 steps."""
     import numpy as np
     from ncorpion_amd import ic
     b = ic.disk(n, seed=0, mass=1e-8, radius=2e-4)
     dt = 2.0 * math.pi / 256.0
     bt = torch.from_numpy(b).cuda()
     c0 = torch.tensor([0, 0, 0, 0, 0, 0, 1.0, 1.0], dtype=torch.float64, device="cuda")
+    # Warm-up (buffers, graph capture, adjoint work arrays), untimed.
+    wb, wc = bt.clone(), c0.clone()
+    ctx.rk4_steps_dev(wb, wc, 2, dt, 1.0, ADJ_SOFT, "fast64", graph=True)
+    wg = torch.zeros((n, 6), dtype=torch.float64, device="cuda")
+    wm = torch.zeros(n, dtype=torch.float64, device="cuda")
+    wcg = torch.zeros(7, dtype=torch.float64, device="cuda")
+    ctx.rk4_adjoint_dev(bt, c0, wg, wg.clone(), wm, wcg, K, dt, K, 1.0, ADJ_SOFT)
+    ctx.synchronize()
     fb, fc = bt.clone(), c0.clone()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    ctx.rk4_steps_dev(fb, fc, steps, dt, 1.0, 0.0, "fast64", graph=True)
+    ctx.rk4_steps_dev(fb, fc, steps, dt, 1.0, ADJ_SOFT, "fast64", graph=True)
     ctx.synchronize()
     t_fwd = time.perf_counter() - t0
     target = fb[:, 0:3] + 1e-3 * torch.tensor(np.random.default_rng(0).normal(size=(n, 3)),
@@ -122,7 +280,7 @@ def adjoint_run(ctx, torch, n=16384, steps=200, K=20):
     gc = torch.zeros(7, dtype=torch.float64, device="cuda")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    ctx.rk4_adjoint_dev(bt, c0, gfin, g0, gm, gc, steps, dt, K, 1.0, 0.0)
+    ctx.rk4_adjoint_dev(bt, c0, gfin, g0, gm, gc, steps, dt, K, 1.0, ADJ_SOFT)
     ctx.synchronize()
     t_adj = time.perf_counter() - t0
     return {"n": n, "steps": steps, "checkpoint_every": K, "forward_s": t_fwd,
@@ -142,6 +300,8 @@ def main():
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-precision", action="store_true")
+    ap.add_argument("--no-mesh", action="store_true")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run a plain device copy of this size (HBM counter calibration)")
     args = ap.parse_args()
@@ -217,7 +377,8 @@ def main():
         "pair_interactions_per_s_graph": None if t_graph is None else interactions * evals / t_graph,
         "roofline": {"bound": "valu", "achieved": achieved_tflops,
                      "peak": PEAK_FP32_VECTOR_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved_tflops / PEAK_FP32_VECTOR_TFLOPS, "traffic": None,
+                     "frac": achieved_tflops / PEAK_FP32_VECTOR_TFLOPS,
+                     "traffic": PAIRS_TRAFFIC_BYTES if n == 262144 else None,
                      "kernel": "k_pairs<mixed>", "kernel_avg_ms": kavg_ms,
                      "kernel_launches": nl_mix,
                      "flop_per_interaction": FLOP_PER_INTERACTION},
@@ -251,6 +412,10 @@ def main():
                            "normwise_max_err": float(d.max() / torch.sqrt(torch.mean(r * r))),
                            "reference": "FP64-fast form (mutual acceleration only)"}
 
+    if not args.no_precision:
+        out["precision_study"] = precision_run(ctx, torch, b, soft, dt)
+    if not args.no_mesh:
+        out["mesh_module"] = mesh_run(ctx, torch)
     if not args.no_ensemble:
         out["ensemble"] = ensemble_run(ctx, torch)
     if not args.no_adjoint:
@@ -267,6 +432,8 @@ def main():
 
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(b, args.cpu_sample)
+        if out["cpu_baseline"] is not None and not args.no_mesh:
+            out["cpu_baseline"]["mesh"] = cpu_mesh_baseline()
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out))

@@ -94,6 +94,27 @@ typedef struct {
     void (*resolve)(void *user, int32_t a, int32_t b);
 } ncg_replay_ops;
 
+/* Built-in decision and outcome for callers other than NcorpiON itself
+ * (NcorpiON's glue passes its own closest_approach and collision_treatment).
+ * They restate, on a host body array, closest_approach (collision.c:63-172)
+ * and the elastic (collision_f = 2) or inelastic bounce of
+ * collision_treatment (physics.c:545-605, 1043-1070), including the
+ * did_collide rule of one_collision_only_bool.  Fill the public fields, then
+ * ncg_builtin_replay_ops gives the ops to pass to ncg_brute_force_pass or
+ * ncg_mesh_pass; `collisions` counts resolved pairs. */
+typedef struct {
+    ncg_body *bodies;         /* host array, updated in place */
+    const int32_t *exists;    /* may be NULL */
+    int32_t *did_collide;     /* may be NULL */
+    double timestep;          /* dt of the pass (sign and size as the caller's) */
+    double collision_f;
+    int32_t one_collision_only;
+    int32_t pad_;
+    int64_t collisions;
+    double approach_[6];      /* set by test, read by resolve */
+    double t_c_;
+} ncg_builtin_collider;
+
 /* Per-pass counters of the mesh module (structure.h:130-137). */
 typedef struct {
     int64_t total_neighbours;
@@ -273,6 +294,21 @@ ncg_status ncg_rk4_adjoint_dev(ncg_ctx *ctx, const ncg_body *d_bodies0, int32_t 
                                double *d_g0, double *d_gm, double *d_gcentral0, ncg_body *d_final,
                                void *stream);
 
+ncg_status ncg_builtin_replay_ops(ncg_builtin_collider *state, ncg_replay_ops *ops);
+
+/* ---------------------------------------------------------------------------
+ * Total energy (FP64) of bodies d_bodies (n, d_exists may be NULL) and the
+ * central body d_central (8 doubles as in ncg_rk4_steps_dev, may be NULL):
+ * d_out[4] = {kinetic (bodies + central body), mutual potential
+ * -sum_{i<j} G m_i m_j / sqrt(r^2 + (s (R_i + R_j))^2), central potential
+ * -sum_i G M m_i / |x_i - C|, total}.  No counterpart in NcorpiON on this
+ * path; it is the energy conserved by the field of vector_field
+ * (physics.c:127-146, 175-201), for drift measurements.
+ */
+ncg_status ncg_energy_dev(ncg_ctx *ctx, const ncg_body *d_bodies, int32_t n,
+                          const int32_t *d_exists, const double *d_central, double grav_const,
+                          double softening, double *d_out, void *stream);
+
 /* ---------------------------------------------------------------------------
  * Launch shape of the tiled gravity forms (NCG_F64_FAST, NCG_MIXED): bodies
  * per lane (1, 2, 4; 0 keeps), j-splits per i-block (0 = from the CU count,
@@ -286,7 +322,8 @@ ncg_status ncg_set_tiling(ncg_ctx *ctx, int32_t ipl, int32_t nsplit, double thet
 /* ---------------------------------------------------------------------------
  * Kernel timing with HIP events recorded on the stream each kernel is
  * launched on.  Names: "exact", "pairs_fast64", "pairs_mixed",
- * "candidates", "mesh_bin", "mesh_count", "mesh_gravity".
+ * "candidates", "mesh_bin", "mesh_count", "mesh_gravity", "ens_mixed",
+ * "ens_fast64", "vjp", "energy".
  * ncg_profile_read waits for the recorded events and returns the summed
  * duration and launch count (reset != 0 clears them).
  */

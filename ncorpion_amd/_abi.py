@@ -37,6 +37,37 @@ class ReplayOps(ctypes.Structure):
     _fields_ = [("user", ctypes.c_void_p), ("test", TEST_FN), ("resolve", RESOLVE_FN)]
 
 
+class BuiltinCollider(ctypes.Structure):
+    _fields_ = [("bodies", ctypes.c_void_p), ("exists", ctypes.c_void_p),
+                ("did_collide", ctypes.c_void_p), ("timestep", ctypes.c_double),
+                ("collision_f", ctypes.c_double), ("one_collision_only", ctypes.c_int32),
+                ("pad_", ctypes.c_int32), ("collisions", ctypes.c_int64),
+                ("approach_", ctypes.c_double * 6), ("t_c_", ctypes.c_double)]
+
+
+class BuiltinOps:
+    """Library-side closest_approach + bounce on a host body array (kept
+    alive with the arrays it points to)."""
+
+    def __init__(self, lib, bodies, exists, did_collide, dt, f, one_collision_only):
+        self.bodies, self.exists, self.did_collide = bodies, exists, did_collide
+        self.state = BuiltinCollider()
+        self.state.bodies = bodies.ctypes.data
+        self.state.exists = None if exists is None else exists.ctypes.data
+        self.state.did_collide = None if did_collide is None else did_collide.ctypes.data
+        self.state.timestep = float(dt)
+        self.state.collision_f = float(f)
+        self.state.one_collision_only = int(one_collision_only)
+        self.ops = ReplayOps()
+        st = lib.ncg_builtin_replay_ops(ctypes.byref(self.state), ctypes.byref(self.ops))
+        if st != NCG_OK:
+            raise NcgError(st, "ncg_builtin_replay_ops failed")
+
+    @property
+    def collisions(self):
+        return int(self.state.collisions)
+
+
 class MeshStats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int64) for n in ("total_neighbours", "how_many_small", "how_many_big",
                                              "collisions", "candidates", "host_tests")]
@@ -71,6 +102,8 @@ _SIGNATURES = {
     "ncg_rk4_adjoint_dev": (S, [P, P, I32, P, D, D, D, I32, I32, P, P, P, P, P, P, P]),
     "ncg_ensemble_rk4_dev": (S, [P, P, P, I32, P, D, D, S, D, I32, I32, D, ctypes.POINTER(I64), P]),
     "ncg_ensemble_gravity_dev": (S, [P, P, P, I32, P, D, D, S, P, P, P]),
+    "ncg_energy_dev": (S, [P, P, I32, P, P, D, D, P, P]),
+    "ncg_builtin_replay_ops": (S, [ctypes.POINTER(BuiltinCollider), ctypes.POINTER(ReplayOps)]),
     "ncg_profile_enable": (S, [P, S]),
     "ncg_profile_read": (S, [P, ctypes.c_char_p, ctypes.POINTER(D), ctypes.POINTER(I64), S]),
 }
@@ -231,6 +264,20 @@ class Context:
             lam_t.data_ptr(), None if lam_c_t is None else lam_c_t.data_ptr(), gx_t.data_ptr(),
             gm_t.data_ptr(), gc_t.data_ptr(), stream))
 
+    def energy_dev(self, bodies_t, central_t=None, exists_t=None, G=1.0, softening=0.0,
+                   out_t=None, stream=None):
+        """Energy terms {kinetic, mutual potential, central potential, total}
+        as a (4,) float64 device tensor (written asynchronously on `stream`)."""
+        import torch
+        n = int(bodies_t.shape[0])
+        if out_t is None:
+            out_t = torch.empty(4, dtype=torch.float64, device=bodies_t.device)
+        self._check(self.lib.ncg_energy_dev(
+            self.h, bodies_t.data_ptr(), n, None if exists_t is None else exists_t.data_ptr(),
+            None if central_t is None else central_t.data_ptr(), float(G), float(softening),
+            out_t.data_ptr(), stream))
+        return out_t
+
     def rk4_adjoint_dev(self, bodies0_t, central0_t, gfinal_t, g0_t, gm_t, gc0_t, nsteps, dt,
                         checkpoint_every=20, G=1.0, softening=0.0, gcentral_final_t=None,
                         final_t=None, stream=None):
@@ -273,39 +320,64 @@ class Context:
             self._check(st)
             return out[: cnt.value].copy()
 
-    def brute_force_pass(self, bodies, exists, dt, test, resolve):
+    def builtin_ops(self, bodies, dt, exists=None, did_collide=None, f=1.5, one_collision_only=0):
+        """Built-in decision/outcome ops (ncg_builtin_replay_ops) acting on the
+        host array `bodies` in place; pass as ops= to brute_force_pass or
+        mesh_pass."""
+        if not (isinstance(bodies, np.ndarray) and bodies.dtype == np.float64
+                and bodies.flags.c_contiguous):
+            raise ValueError("bodies must be a C-contiguous float64 array (mutated in place)")
+        n = bodies.shape[0]
+        e = None if exists is None else as_exists(exists, n)
+        dc = None
+        if did_collide is not None:
+            dc = did_collide
+            if not (isinstance(dc, np.ndarray) and dc.dtype == np.int32 and dc.flags.c_contiguous):
+                raise ValueError("did_collide must be a C-contiguous int32 array")
+        return BuiltinOps(self.lib, bodies, e, dc, dt, f, one_collision_only)
+
+    def _ops(self, test, resolve, ops):
+        """(keepalive, pointer) of the replay ops: built-in or Python callbacks."""
+        if ops is not None:
+            return ops, ctypes.byref(ops.ops)
+        if test is None:
+            return None, None
+        tf = TEST_FN(lambda u, a, c: 1 if test(int(a), int(c)) else 0)
+        rf = RESOLVE_FN(lambda u, a, c: resolve(int(a), int(c)))
+        r = ReplayOps(None, tf, rf)
+        return (r, tf, rf), ctypes.byref(r)
+
+    def brute_force_pass(self, bodies, exists, dt, test=None, resolve=None, ops=None):
         """Replay of brute_force with Python callbacks test(a, b) -> bool and
-        resolve(a, b).  `bodies` must be the array the callbacks mutate."""
+        resolve(a, b), or built-in ops.  `bodies` must be the array they mutate."""
         b = bodies
         if not (isinstance(b, np.ndarray) and b.dtype == np.float64 and b.flags.c_contiguous):
             raise ValueError("bodies must be a C-contiguous float64 array (mutated in place)")
         n = b.shape[0]
         e = as_exists(exists, n)
-        tf = TEST_FN(lambda u, a, c: 1 if test(int(a), int(c)) else 0)
-        rf = RESOLVE_FN(lambda u, a, c: resolve(int(a), int(c)))
-        ops = ReplayOps(None, tf, rf)
+        keep, ops_p = self._ops(test, resolve, ops)
+        if ops_p is None:
+            raise ValueError("brute_force_pass needs test/resolve callbacks or ops")
         nc = I64()
         self._check(self.lib.ncg_brute_force_pass(self.h, _ptr(b), n, _ptr(e), float(dt),
-                                                  ctypes.byref(ops), ctypes.byref(nc)))
+                                                  ops_p, ctypes.byref(nc)))
+        del keep
         return int(nc.value)
 
     # -- mesh module -------------------------------------------------------------
-    def mesh_pass(self, bodies, exists, dt, gam, cube, cells, mode, test=None, resolve=None):
+    def mesh_pass(self, bodies, exists, dt, gam, cube, cells, mode, test=None, resolve=None,
+                  ops=None):
         b = bodies
         if not (isinstance(b, np.ndarray) and b.dtype == np.float64 and b.flags.c_contiguous):
             raise ValueError("bodies must be a C-contiguous float64 array (mutated in place)")
         n = b.shape[0]
         e = as_exists(exists, n)
         stats = MeshStats()
-        ops_p = None
-        if test is not None:
-            tf = TEST_FN(lambda u, a, c: 1 if test(int(a), int(c)) else 0)
-            rf = RESOLVE_FN(lambda u, a, c: resolve(int(a), int(c)))
-            ops = ReplayOps(None, tf, rf)
-            ops_p = ctypes.byref(ops)
+        keep, ops_p = self._ops(test, resolve, ops)
         self._check(self.lib.ncg_mesh_pass(self.h, _ptr(b), n, _ptr(e), float(dt), float(gam),
                                            float(cube), int(cells), int(mode), ops_p,
                                            ctypes.byref(stats)))
+        del keep
         return {f: int(getattr(stats, f)) for f, _ in MeshStats._fields_}
 
     def mesh_gravity(self, bodies, exists=None, central=None, G=1.0, softening=0.0,

@@ -25,60 +25,112 @@
 
 namespace {
 
-constexpr int AT = 256;  // threads per workgroup and LDS tile
+constexpr int AT = 256;  // threads per workgroup
+constexpr int VC = 64;   // j chunk: one wavefront's worth of bodies
 
-// Pair part of the VJP; one lane per body i.  g_x and g_m are written (=).
-__global__ __launch_bounds__(AT) void k_vjp_pairs(const DBody *__restrict__ b, int n,
-                                                  const double *__restrict__ lam, double G,
-                                                  double soft, double *__restrict__ gx,
-                                                  double *__restrict__ gm)
+// j records for the pair VJP: {x, y, z, m, lam x, lam y, lam z, s R}.
+__global__ __launch_bounds__(AT) void k_vjp_pack(const DBody *__restrict__ b, int n,
+                                                 const double *__restrict__ lam, double soft,
+                                                 double *__restrict__ jv)
 {
-    __shared__ double4 tp[AT];  // x, y, z, m
-    __shared__ double4 tl[AT];  // lam x, y, z, s R
     const int i = blockIdx.x * AT + threadIdx.x;
-    const bool mine = i < n;
-    const int ii = mine ? i : n - 1;
-    const double xi = b[ii].x, yi = b[ii].y, zi = b[ii].z, mi = b[ii].mass;
-    const double si = soft * b[ii].radius;
-    const double lx = lam[3 * (size_t)ii], ly = lam[3 * (size_t)ii + 1], lz = lam[3 * (size_t)ii + 2];
+    if (i >= n) return;
+    const DBody q = b[i];
+    double *o = jv + 8 * (size_t)i;
+    o[0] = q.x;
+    o[1] = q.y;
+    o[2] = q.z;
+    o[3] = q.mass;
+    o[4] = lam[3 * (size_t)i];
+    o[5] = lam[3 * (size_t)i + 1];
+    o[6] = lam[3 * (size_t)i + 2];
+    o[7] = soft * q.radius;
+}
+
+struct VjpI {
+    double x, y, z, m, lx, ly, lz, s;
+};
+
+// One ordered pair (i, j) of the VJP.  1/D from v_rsq_f64 plus one Newton
+// step (FP64 rounding).  SELF: the chunk holds this lane's own body; its
+// term is masked to 0 (D may be 0 there).
+template <bool SELF>
+__device__ inline void vjp_term(const double *__restrict__ q, int j, int i, const VjpI &a,
+                                double &gxx, double &gxy, double &gxz, double &gmi)
+{
+    const double dx = q[0] - a.x, dy = q[1] - a.y, dz = q[2] - a.z;
+    const double e = a.s + q[7];
+    const double D2 = fma(dx, dx, fma(dy, dy, fma(dz, dz, e * e)));
+    double y = __builtin_amdgcn_rsq(D2);
+    y = y * fma(-0.5 * D2, y * y, 1.5);
+    double i3 = y * y * y;
+    double i5 = i3 * y * y;
+    if (SELF) {
+        i3 = j == i ? 0.0 : i3;
+        i5 = j == i ? 0.0 : i5;
+    }
+    // u = m_i lam_j - m_j lam_i ; T u = u / D^3 - 3 d (d . u) / D^5
+    const double ux = a.m * q[4] - q[3] * a.lx;
+    const double uy = a.m * q[5] - q[3] * a.ly;
+    const double uz = a.m * q[6] - q[3] * a.lz;
+    const double du3 = 3.0 * (dx * ux + dy * uy + dz * uz) * i5;
+    gxx = fma(ux, i3, fma(-dx, du3, gxx));
+    gxy = fma(uy, i3, fma(-dy, du3, gxy));
+    gxz = fma(uz, i3, fma(-dz, du3, gxz));
+    gmi = fma(-(q[4] * dx + q[5] * dy + q[6] * dz), i3, gmi);
+}
+
+// Pair part of the VJP.  Grid (bodies / AT, nsplit): lane = body i, the
+// block's j range is [blockIdx.y * per, ...) with per a multiple of VC.  j
+// records are read with wave-uniform addresses (scalar loads).  Output: one
+// partial {g_x, g_m} per (split, body), summed in split order by k_vjp_reduce.
+__global__ __launch_bounds__(AT) void k_vjp_pairs(const double *__restrict__ jv, int n, int per,
+                                                  double4 *__restrict__ part)
+{
+    const int i = blockIdx.x * AT + threadIdx.x;
+    const int ii = min(i, n - 1);
+    VjpI a;
+    {
+        const double *p = jv + 8 * (size_t)ii;
+        a = VjpI{p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7]};
+    }
+    const int wave0 = __builtin_amdgcn_readfirstlane(blockIdx.x * AT + ((int)threadIdx.x & ~(VC - 1)));
+    const int j0 = blockIdx.y * per, j1 = min(n, j0 + per);
     double gxx = 0., gxy = 0., gxz = 0., gmi = 0.;
-    for (int base = 0; base < n; base += AT) {
-        const int j = base + (int)threadIdx.x;
-        __syncthreads();
-        if (j < n) {
-            const DBody q = b[j];
-            tp[threadIdx.x] = make_double4(q.x, q.y, q.z, q.mass);
-            tl[threadIdx.x] = make_double4(lam[3 * (size_t)j], lam[3 * (size_t)j + 1],
-                                           lam[3 * (size_t)j + 2], soft * q.radius);
-        }
-        __syncthreads();
-        const int cnt = min(AT, n - base);
-        for (int jj = 0; jj < cnt; ++jj) {
-            if (base + jj == i) continue;
-            const double4 p = tp[jj], l = tl[jj];
-            const double dx = p.x - xi, dy = p.y - yi, dz = p.z - zi;
-            const double e = si + l.w;
-            const double D2 = fma(dx, dx, fma(dy, dy, fma(dz, dz, e * e)));
-            double y = __builtin_amdgcn_rsq(D2);
-            y = y * fma(-0.5 * D2, y * y, 1.5);
-            y = y * fma(-0.5 * D2, y * y, 1.5);
-            const double i3 = y * y * y;
-            const double i5 = i3 * y * y;
-            // u = m_i lam_j - m_j lam_i ; T u = u / D^3 - 3 d (d . u) / D^5
-            const double ux = mi * l.x - p.w * lx, uy = mi * l.y - p.w * ly, uz = mi * l.z - p.w * lz;
-            const double du = dx * ux + dy * uy + dz * uz;
-            gxx += ux * i3 - 3.0 * dx * du * i5;
-            gxy += uy * i3 - 3.0 * dy * du * i5;
-            gxz += uz * i3 - 3.0 * dz * du * i5;
-            gmi -= (l.x * dx + l.y * dy + l.z * dz) * i3;
+    for (int jc = j0; jc < j1; jc += VC) {
+        const int je = min(j1, jc + VC);
+        if (jc == wave0) {
+            for (int j = jc; j < je; ++j) vjp_term<true>(jv + 8 * (size_t)j, j, i, a, gxx, gxy, gxz, gmi);
+        } else if (je - jc == VC) {
+#pragma unroll 4
+            for (int j = jc; j < jc + VC; ++j)
+                vjp_term<false>(jv + 8 * (size_t)j, j, i, a, gxx, gxy, gxz, gmi);
+        } else {
+            for (int j = jc; j < je; ++j) vjp_term<false>(jv + 8 * (size_t)j, j, i, a, gxx, gxy, gxz, gmi);
         }
     }
-    if (mine) {
-        gx[3 * (size_t)i + 0] = G * gxx;
-        gx[3 * (size_t)i + 1] = G * gxy;
-        gx[3 * (size_t)i + 2] = G * gxz;
-        gm[i] = G * gmi;
+    if (i < n) part[(size_t)blockIdx.y * n + i] = make_double4(gxx, gxy, gxz, gmi);
+}
+
+// g_x (=) and g_m (=) from the split partials, in split order.
+__global__ __launch_bounds__(AT) void k_vjp_reduce(const double4 *__restrict__ part, int nsplit,
+                                                   int n, double G, double *__restrict__ gx,
+                                                   double *__restrict__ gm)
+{
+    const int i = blockIdx.x * AT + threadIdx.x;
+    if (i >= n) return;
+    double4 t = part[i];
+    for (int s = 1; s < nsplit; ++s) {
+        const double4 u = part[(size_t)s * n + i];
+        t.x += u.x;
+        t.y += u.y;
+        t.z += u.z;
+        t.w += u.w;
     }
+    gx[3 * (size_t)i + 0] = G * t.x;
+    gx[3 * (size_t)i + 1] = G * t.y;
+    gx[3 * (size_t)i + 2] = G * t.z;
+    gm[i] = G * t.w;
 }
 
 // Central part, added to g_x / g_m; per-block partials of g_C (3) and g_M.
@@ -223,13 +275,23 @@ static ncg_status vjp_impl(ncg_ctx *ctx, const DBody *b, int32_t n, const double
     const bool accumulate = acc_gm;
     double *gm_tmp = accumulate ? dbuf<double>(ctx, "v_gm", n) : gm;
     const int nb = (int)cdiv(n, AT);
+    // j splits: about 8 workgroups per CU, each split a multiple of VC bodies.
+    const int cus = ctx->num_cus > 0 ? ctx->num_cus : 256;
+    const int nchunk = (int)cdiv(n, VC);
+    int nsplit = std::max(1, std::min(nchunk, (int)cdiv(8 * cus, nb)));
+    const int per = (int)cdiv(nchunk, nsplit) * VC;
+    nsplit = (int)cdiv(n, per);
     double *part = dbuf<double>(ctx, "v_part", 4 * (size_t)nb);
-    if (!gm_tmp || !part) return NCG_ERR_NOMEM;
+    double *jv = dbuf<double>(ctx, "v_jv", 8 * (size_t)n);
+    double4 *pp = dbuf<double4>(ctx, "v_pp", (size_t)nsplit * n);
+    if (!gm_tmp || !part || !jv || !pp) return NCG_ERR_NOMEM;
+    hipLaunchKernelGGL(k_vjp_pack, dim3(nb), dim3(AT), 0, st, b, (int)n, lam, soft, jv);
     {
         ProfScope ps(ctx, st, "vjp");
-        hipLaunchKernelGGL(k_vjp_pairs, dim3(nb), dim3(AT), 0, st, b, (int)n, lam, G, soft, gx, gm_tmp);
+        hipLaunchKernelGGL(k_vjp_pairs, dim3(nb, nsplit), dim3(AT), 0, st, jv, (int)n, per, pp);
         NCG_CHECK_LAUNCH();
     }
+    hipLaunchKernelGGL(k_vjp_reduce, dim3(nb), dim3(AT), 0, st, pp, nsplit, (int)n, G, gx, gm_tmp);
     hipLaunchKernelGGL(k_vjp_central, dim3(nb), dim3(AT), 0, st, b, (int)n, cen, lam, lamc, G, gx,
                        gm_tmp, part);
     hipLaunchKernelGGL(k_sum4, dim3(1), dim3(AT), 0, st, part, nb, gcm, 0);

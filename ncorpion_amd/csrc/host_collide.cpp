@@ -10,22 +10,13 @@
 
 namespace {
 
-struct Builtin {
-    ncg_body *b;
-    const int32_t *exists;
-    int32_t *did_collide;  // may be null
-    double dt;
-    double f;
-    int one_collision_only;
-    double approach[6];
-    double t_c;
-};
+using Builtin = ncg_builtin_collider;
 
 int builtin_test(void *u, int32_t a, int32_t bb)
 {
     Builtin *s = static_cast<Builtin *>(u);
     if (s->exists && !(s->exists[a] && s->exists[bb])) return 0;
-    const ncg_body &A = s->b[a], &B = s->b[bb];
+    const ncg_body &A = s->bodies[a], &B = s->bodies[bb];
     const double dx = A.x - B.x, dy = A.y - B.y, dz = A.z - B.z;
     const double dvx = A.vx - B.vx, dvy = A.vy - B.vy, dvz = A.vz - B.vz;
     const double dr_dot_dv = dx * dvx + dy * dvy + dz * dvz;
@@ -36,16 +27,16 @@ int builtin_test(void *u, int32_t a, int32_t bb)
     const double disc = dr_dot_dv * dr_dot_dv + dv2 * (R * R - dr2);
     if (disc <= 0.0) return 0;
     const double t = -(dr_dot_dv + sqrt(disc)) / dv2;
-    s->t_c = t;
-    if (t > s->dt) return 0;
+    s->t_c_ = t;
+    if (t > s->timestep) return 0;
     const double xa = A.x + t * A.vx, ya = A.y + t * A.vy, za = A.z + t * A.vz;
     const double xb = B.x + t * B.vx, yb = B.y + t * B.vy, zb = B.z + t * B.vz;
     if (A.mass > B.mass) {
-        s->approach[0] = xa; s->approach[1] = ya; s->approach[2] = za;
-        s->approach[3] = xb; s->approach[4] = yb; s->approach[5] = zb;
+        s->approach_[0] = xa; s->approach_[1] = ya; s->approach_[2] = za;
+        s->approach_[3] = xb; s->approach_[4] = yb; s->approach_[5] = zb;
     } else {
-        s->approach[0] = xb; s->approach[1] = yb; s->approach[2] = zb;
-        s->approach[3] = xa; s->approach[4] = ya; s->approach[5] = za;
+        s->approach_[0] = xb; s->approach_[1] = yb; s->approach_[2] = zb;
+        s->approach_[3] = xa; s->approach_[4] = ya; s->approach_[5] = za;
     }
     if (s->did_collide && (s->did_collide[a] || s->did_collide[bb])) return 0;
     return 1;
@@ -54,16 +45,16 @@ int builtin_test(void *u, int32_t a, int32_t bb)
 void builtin_resolve(void *u, int32_t a, int32_t bb)
 {
     Builtin *s = static_cast<Builtin *>(u);
-    const int A = s->b[a].mass > s->b[bb].mass ? a : bb;
+    const int A = s->bodies[a].mass > s->bodies[bb].mass ? a : bb;
     const int B = A == a ? bb : a;
-    ncg_body &P = s->b[A], &Q = s->b[B];
+    ncg_body &P = s->bodies[A], &Q = s->bodies[B];
     const double R = P.radius + Q.radius;
     const double m_a = P.mass, m_b = Q.mass;
     double vx_a = P.vx, vy_a = P.vy, vz_a = P.vz, vx_b = Q.vx, vy_b = Q.vy, vz_b = Q.vz;
-    const double *ap = s->approach;
+    const double *ap = s->approach_;
     const double xa = ap[0], ya = ap[1], za = ap[2], xb = ap[3], yb = ap[4], zb = ap[5];
     const double dr_dot_dv = (xa - xb) * (vx_a - vx_b) + (ya - yb) * (vy_a - vy_b) + (za - zb) * (vz_a - vz_b);
-    const double alpha = s->f * m_a * m_b / ((m_a + m_b) * R * R);
+    const double alpha = s->collision_f * m_a * m_b / ((m_a + m_b) * R * R);
     const double Jx = alpha * dr_dot_dv * (xb - xa);
     const double Jy = alpha * dr_dot_dv * (yb - ya);
     const double Jz = alpha * dr_dot_dv * (zb - za);
@@ -73,7 +64,7 @@ void builtin_resolve(void *u, int32_t a, int32_t bb)
     vx_b -= Jx / m_b;
     vy_b -= Jy / m_b;
     vz_b -= Jz / m_b;
-    const double t = s->t_c;
+    const double t = s->t_c_;
     P.x = xa - t * vx_a; P.y = ya - t * vy_a; P.z = za - t * vz_a;
     P.vx = vx_a; P.vy = vy_a; P.vz = vz_a;
     Q.x = xb - t * vx_b; Q.y = yb - t * vy_b; Q.z = zb - t * vz_b;
@@ -82,9 +73,19 @@ void builtin_resolve(void *u, int32_t a, int32_t bb)
         s->did_collide[a] = s->one_collision_only;
         s->did_collide[bb] = s->one_collision_only;
     }
+    ++s->collisions;
 }
 
 }  // namespace
+
+extern "C" ncg_status ncg_builtin_replay_ops(ncg_builtin_collider *state, ncg_replay_ops *ops)
+{
+    if (!state || !ops || !state->bodies) return NCG_ERR_INVALID;
+    ops->user = state;
+    ops->test = builtin_test;
+    ops->resolve = builtin_resolve;
+    return NCG_OK;
+}
 
 // Replays one brute-force collision pass over `bodies` (host) given the sorted
 // candidate list, with the built-in decision and outcome.  Same visiting rules
@@ -93,7 +94,11 @@ int64_t ncg_builtin_bf_replay(ncg_body *bodies, int32_t n, const int32_t *exists
                               double f, const std::vector<ncg_pair> &cand,
                               std::vector<ncg_pair> *log)
 {
-    Builtin u{bodies, exists, nullptr, dt, f, 0, {0, 0, 0, 0, 0, 0}, 0.0};
+    Builtin u{};
+    u.bodies = bodies;
+    u.exists = exists;
+    u.timestep = dt;
+    u.collision_f = f;
     std::vector<char> dirty((size_t)n, 0);
     std::vector<int32_t> dlist;  // dirty bodies, kept sorted
     int64_t ncoll = 0;

@@ -119,10 +119,16 @@ __global__ __launch_bounds__(MT) void k_iota(int *__restrict__ v, int n)
     if (s < n) v[s] = s;
 }
 
-__global__ __launch_bounds__(MT) void k_bounds(const unsigned *__restrict__ skey, int ins,
+// Inserted-body count: *dins on the device when given (launches that do not
+// wait for the host), else ins.
+__device__ inline int ins_count(int ins, const int *dins) { return dins ? *dins : ins; }
+
+__global__ __launch_bounds__(MT) void k_bounds(const unsigned *__restrict__ skey, int ins_h,
+                                               const int *__restrict__ dins,
                                                unsigned *__restrict__ cs, unsigned *__restrict__ ce)
 {
     const int s = blockIdx.x * MT + threadIdx.x;
+    const int ins = ins_count(ins_h, dins);
     if (s >= ins) return;
     const unsigned k = skey[s];
     if (s == 0 || skey[s - 1] != k) cs[k] = (unsigned)s;
@@ -131,14 +137,24 @@ __global__ __launch_bounds__(MT) void k_bounds(const unsigned *__restrict__ skey
 
 // Sorted copies {x, y, z, R} and {vx, vy, vz, m} of the inserted bodies.
 __global__ __launch_bounds__(MT) void k_msorted(const DBody *__restrict__ b,
-                                                const int *__restrict__ sid, int ins,
+                                                const int *__restrict__ sid, int ins_h,
+                                                const int *__restrict__ dins,
                                                 double4 *__restrict__ pr, double4 *__restrict__ vm)
 {
     const int s = blockIdx.x * MT + threadIdx.x;
-    if (s >= ins) return;
+    if (s >= ins_count(ins_h, dins)) return;
     const DBody q = b[sid[s]];
     pr[s] = make_double4(q.x, q.y, q.z, q.radius);
     vm[s] = make_double4(q.vx, q.vy, q.vz, q.mass);
+}
+
+// Keys of this pass, kept to clear their cells at the start of the next one.
+__global__ __launch_bounds__(MT) void k_copy_keys(const unsigned *__restrict__ skey,
+                                                  const int *__restrict__ dins,
+                                                  unsigned *__restrict__ out)
+{
+    const int s = blockIdx.x * MT + threadIdx.x;
+    if (s < *dins) out[s] = skey[s];
 }
 
 // closest_approach's predicate (collision.c:112-137) with a 2^-40 relative
@@ -166,7 +182,8 @@ __global__ __launch_bounds__(MT) void k_mcount(const unsigned *__restrict__ skey
                                                const double4 *__restrict__ pr,
                                                const double4 *__restrict__ vm,
                                                const unsigned *__restrict__ cs,
-                                               const unsigned *__restrict__ ce, int ins, int d,
+                                               const unsigned *__restrict__ ce, int ins_h,
+                                               const int *__restrict__ dins, int d,
                                                double dt, int *__restrict__ count_by_id,
                                                int4 *__restrict__ cand, unsigned long long cap,
                                                unsigned long long *__restrict__ ccnt,
@@ -174,6 +191,7 @@ __global__ __launch_bounds__(MT) void k_mcount(const unsigned *__restrict__ skey
 {
     __shared__ unsigned long long sh[MT];
     const int s = blockIdx.x * MT + threadIdx.x;
+    const int ins = ins_count(ins_h, dins);
     unsigned long long c = 0;
     if (s < ins) {
         const int k = sid[s];
@@ -255,57 +273,112 @@ __global__ __launch_bounds__(MT) void k_gsorted(const DBody *__restrict__ b,
     sx[s] = (ex == nullptr || ex[p] != 0) && p > 2;
 }
 
-// Per body k: central term, bodies 0..2 (k > 2), neighbour pairs (k > 2,
-// both inserted).  cterm receives the per-body CM_acc terms.
-template <int FAST>
+// Central term of body k (physics.c:127-146): acceleration and CM_acc term.
+__device__ inline void central_term(double x, double y, double z, double m, double cx, double cy,
+                                    double cz, double mu, double G, double &ax, double &ay,
+                                    double &az, double *ct)
+{
+    const double DX = x - cx, DY = y - cy, DZ = z - cz;
+    const double rk = sqrt(DX * DX + DY * DY + DZ * DZ);
+    const double r3 = rk * rk * rk;
+    ax = -mu * DX / r3;
+    ay = -mu * DY / r3;
+    az = -mu * DZ / r3;
+    ct[0] = G * m * DX / r3;
+    ct[1] = G * m * DY / r3;
+    ct[2] = G * m * DZ / r3;
+}
+
+// Bodies the sorted kernel does not cover: k <= 2 (central term only; their
+// mutual part is added by k_big_final) and k > 2 not binned (central term and
+// bodies 0..2).
+__global__ __launch_bounds__(MT) void k_mgrav_rest(const DBody *__restrict__ b, int n,
+                                                   const int32_t *__restrict__ ex,
+                                                   const unsigned *__restrict__ key, int cp,
+                                                   double cx, double cy, double cz, double mu,
+                                                   double G, double s, double *__restrict__ acc,
+                                                   double *__restrict__ cterm)
+{
+    const int k = blockIdx.x * MT + threadIdx.x;
+    if (k >= n || (ex && !ex[k])) return;
+    if (k > 2 && key[k] != NOKEY) return;
+    const DBody q = b[k];
+    double ax = 0., ay = 0., az = 0.;
+    if (cp) central_term(q.x, q.y, q.z, q.mass, cx, cy, cz, mu, G, ax, ay, az, cterm + 3 * (size_t)k);
+    if (k > 2)
+        for (int big = 0; big < 3 && big < n; ++big) {
+            if (ex && !ex[big]) continue;
+            const DBody g = b[big];
+            term_exact(q.x, q.y, q.z, q.radius, make_double4(g.x, g.y, g.z, g.mass), g.radius, G, s,
+                       ax, ay, az);
+        }
+    acc[3 * (size_t)k + 0] = ax;
+    acc[3 * (size_t)k + 1] = ay;
+    acc[3 * (size_t)k + 2] = az;
+}
+
+// Binned bodies k > 2, one lane per sorted position t (bodies of one cell
+// are adjacent lanes and walk the same 27 cells).  Per body: central term,
+// bodies 0..2, then the neighbour terms cell l = 0..26, ascending id within
+// a cell.  FORM 0: the reference's expression; 1: FP64 with hardware 1/sqrt
+// and a Newton step; 2: separations in FP64, 1/r^3 and forces in FP32,
+// summed in FP32 per neighbour cell and added to the FP64 total.
+template <int FORM>
 __global__ __launch_bounds__(MT) void k_mgrav(const DBody *__restrict__ b, int n,
-                                              const int32_t *__restrict__ ex,
-                                              const unsigned *__restrict__ key,
+                                              const int32_t *__restrict__ ex, int ins,
                                               const unsigned *__restrict__ cs,
                                               const unsigned *__restrict__ ce,
                                               const double4 *__restrict__ pm,
                                               const double *__restrict__ rr,
                                               const int *__restrict__ sx,
-                                              const int *__restrict__ sid, int d, int cp,
+                                              const int *__restrict__ sid,
+                                              const unsigned *__restrict__ skey, int d, int cp,
                                               double cx, double cy, double cz, double mu,
                                               double G, double s, double *__restrict__ acc,
                                               double *__restrict__ cterm)
 {
-    const int k = blockIdx.x * MT + threadIdx.x;
-    if (k >= n || (ex && !ex[k])) return;
-    const DBody q = b[k];
+    const int t = blockIdx.x * MT + threadIdx.x;
+    if (t >= ins || !sx[t]) return;
+    const int k = sid[t];
+    const double4 me = pm[t];
+    const double Rk = rr[t];
     double ax = 0., ay = 0., az = 0.;
-    if (cp) {
-        const double DX = q.x - cx, DY = q.y - cy, DZ = q.z - cz;
-        const double rk = sqrt(DX * DX + DY * DY + DZ * DZ);
-        const double r3 = rk * rk * rk;
-        ax = -mu * DX / r3;
-        ay = -mu * DY / r3;
-        az = -mu * DZ / r3;
-        cterm[3 * (size_t)k + 0] = G * q.mass * DX / r3;
-        cterm[3 * (size_t)k + 1] = G * q.mass * DY / r3;
-        cterm[3 * (size_t)k + 2] = G * q.mass * DZ / r3;
+    if (cp) central_term(me.x, me.y, me.z, me.w, cx, cy, cz, mu, G, ax, ay, az, cterm + 3 * (size_t)k);
+    for (int big = 0; big < 3 && big < n; ++big) {
+        if (ex && !ex[big]) continue;
+        const DBody g = b[big];
+        term_exact(me.x, me.y, me.z, Rk, make_double4(g.x, g.y, g.z, g.mass), g.radius, G, s, ax,
+                   ay, az);
     }
-    if (k > 2) {
-        // physics.c:209-260: pairs (0,k), (1,k), (2,k)
-        for (int big = 0; big < 3 && big < n; ++big) {
-            if (ex && !ex[big]) continue;
-            const DBody g = b[big];
-            term_exact(q.x, q.y, q.z, q.radius, make_double4(g.x, g.y, g.z, g.mass), g.radius, G,
-                       s, ax, ay, az);
-        }
-        const unsigned kk = key[k];
-        if (kk != NOKEY) {
-            for (int l = 0; l < 27; ++l) {
-                const unsigned ck = (unsigned)((long)kk + stencil_offset(l, d));
-                const unsigned a = cs[ck], e = ce[ck];
-                for (unsigned t = a; t < e; ++t) {
-                    if (!sx[t] || sid[t] == k) continue;
-                    if (FAST)
-                        term_fast(q.x, q.y, q.z, q.radius, pm[t], rr[t], G, s, ax, ay, az);
-                    else
-                        term_exact(q.x, q.y, q.z, q.radius, pm[t], rr[t], G, s, ax, ay, az);
-                }
+    const long kk = skey[t];
+    for (int l = 0; l < 27; ++l) {
+        const unsigned ck = (unsigned)(kk + stencil_offset(l, d));
+        const unsigned a = cs[ck], e = ce[ck];
+        if (FORM == 2) {
+            float fx = 0.f, fy = 0.f, fz = 0.f;
+            const float Gf = (float)G;
+            for (unsigned u = a; u < e; ++u) {
+                if (!sx[u] || u == (unsigned)t) continue;
+                const double4 p = pm[u];
+                const float dx = (float)(p.x - me.x), dy = (float)(p.y - me.y), dz = (float)(p.z - me.z);
+                const float ee = (float)(s * (Rk + rr[u]));
+                const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, ee * ee)));
+                const float y = __builtin_amdgcn_rsqf(fmaxf(r2, 1e-30f));
+                const float f = (Gf * (float)p.w) * (y * (y * y));
+                fx = fmaf(f, dx, fx);
+                fy = fmaf(f, dy, fy);
+                fz = fmaf(f, dz, fz);
+            }
+            ax += (double)fx;
+            ay += (double)fy;
+            az += (double)fz;
+        } else {
+            for (unsigned u = a; u < e; ++u) {
+                if (!sx[u] || u == (unsigned)t) continue;
+                if (FORM == 1)
+                    term_fast(me.x, me.y, me.z, Rk, pm[u], rr[u], G, s, ax, ay, az);
+                else
+                    term_exact(me.x, me.y, me.z, Rk, pm[u], rr[u], G, s, ax, ay, az);
             }
         }
     }
@@ -589,8 +662,6 @@ extern "C" ncg_status ncg_mesh_pass(ncg_ctx *ctx, const ncg_body *bodies, int32_
     }
     unsigned *cs = dbuf<unsigned>(ctx, "m_cs", nkeys);
     unsigned *ce = dbuf<unsigned>(ctx, "m_ce", nkeys);
-    unsigned *prev = dbuf<unsigned>(ctx, "m_prev", n);
-    if (!prev) return NCG_ERR_NOMEM;
     if (M.prev_ins > 0) {
         hipLaunchKernelGGL(k_clear_cells, dim3(cdiv(M.prev_ins, MT)), dim3(MT), 0, st,
                            (const unsigned *)dbuf<unsigned>(ctx, "m_prevkeys", M.prev_ins),
@@ -598,58 +669,47 @@ extern "C" ncg_status ncg_mesh_pass(ncg_ctx *ctx, const ncg_body *bodies, int32_
         NCG_CHECK_LAUNCH();
         M.prev_ins = 0;
     }
+    // Sized for every body (growth happens here, after the clear above).
+    unsigned *pk = dbuf<unsigned>(ctx, "m_prevkeys", n);
+    size_t tmp_bytes = 0;
+    NCG_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, key, skey, idx, sid, n, 0, 32, st));
+    void *tmp = ncg_dbuf(ctx, "m_sorttmp", tmp_bytes);
+    unsigned long long cap = 1 << 16;
+    int4 *dc = mode == 0 ? dbuf<int4>(ctx, "m_cand", cap) : nullptr;
+    if (!pk || !tmp || (mode == 0 && !dc)) return NCG_ERR_NOMEM;
     NCG_HIP(hipMemcpyAsync(db, bodies, sizeof(DBody) * (size_t)n, hipMemcpyHostToDevice, st));
     if (exists)
         NCG_HIP(hipMemcpyAsync(dex, exists, sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, st));
     NCG_HIP(hipMemsetAsync(cnt, 0, sizeof(int) * 4, st));
     NCG_HIP(hipMemsetAsync(tot, 0, sizeof(unsigned long long) * 2, st));
+    const unsigned g = (unsigned)cdiv(n, MT);
     {
+        // Binning: keys, (key, id) radix sort, cell ranges, sorted copies.
+        // The inserted count stays on the device (cnt[0]): no host wait.
         ProfScope ps(ctx, st, "mesh_bin");
-        hipLaunchKernelGGL(k_mkey, dim3(cdiv(n, MT)), dim3(MT), 0, st, db, (int)n, dex, gam,
-                           collision_cube, (int)d, (int)mode, key, idx, cnt, bigl, big_cap);
+        hipLaunchKernelGGL(k_mkey, dim3(g), dim3(MT), 0, st, db, (int)n, dex, gam, collision_cube,
+                           (int)d, (int)mode, key, idx, cnt, bigl, big_cap);
+        NCG_CHECK_LAUNCH();
+        // NOKEY (all ones) sorts last; every real key is below 2^32 - 1.
+        NCG_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, key, skey, idx, sid, n, 0, 32, st));
+        hipLaunchKernelGGL(k_bounds, dim3(g), dim3(MT), 0, st, skey, 0, cnt, cs, ce);
+        hipLaunchKernelGGL(k_msorted, dim3(g), dim3(MT), 0, st, db, sid, 0, cnt, pr, vm);
+        hipLaunchKernelGGL(k_copy_keys, dim3(g), dim3(MT), 0, st, skey, cnt, pk);
         NCG_CHECK_LAUNCH();
     }
     int hc4[4];
     NCG_HIP(hipMemcpyAsync(hc4, cnt, sizeof hc4, hipMemcpyDeviceToHost, st));
-    size_t tmp_bytes = 0;
-    int end_bit = 1;
-    while (end_bit < 32 && (1ull << end_bit) < (unsigned long long)nkeys) ++end_bit;
-    NCG_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, key, skey, idx, sid, n, 0, 32, st));
-    void *tmp = ncg_dbuf(ctx, "m_sorttmp", tmp_bytes);
-    if (!tmp) return NCG_ERR_NOMEM;
-    // NOKEY (all ones) sorts last whatever end_bit, as long as every real key
-    // is below 2^end_bit: sort on all 32 bits to keep that simple.
-    NCG_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, key, skey, idx, sid, n, 0, 32, st));
-    NCG_HIP(hipStreamSynchronize(st));
-    const int ins = hc4[0];
-    const int nbig = std::min(hc4[3], big_cap);
-    NCG_REQUIRE(hc4[3] <= big_cap, "more than %d bodies larger than the mesh size", big_cap);
-    if (ins > 0) {
-        hipLaunchKernelGGL(k_bounds, dim3(cdiv(ins, MT)), dim3(MT), 0, st, skey, ins, cs, ce);
-        hipLaunchKernelGGL(k_msorted, dim3(cdiv(ins, MT)), dim3(MT), 0, st, db, sid, ins, pr, vm);
-        NCG_CHECK_LAUNCH();
-    }
-    // Remember the keys of this pass to clear them next time.
-    unsigned *pk = dbuf<unsigned>(ctx, "m_prevkeys", std::max(ins, 1));
-    if (!pk) return NCG_ERR_NOMEM;
-    if (ins) NCG_HIP(hipMemcpyAsync(pk, skey, sizeof(unsigned) * ins, hipMemcpyDeviceToDevice, st));
-    M.prev_ins = ins;
-
     std::vector<int4> cand;
-    unsigned long long cap = 1 << 16;
     unsigned long long total = 0;
-    for (int attempt = 0; attempt < 2 && ins > 0; ++attempt) {
-        int4 *dc = dbuf<int4>(ctx, "m_cand", cap);
-        if (!dc) return NCG_ERR_NOMEM;
-        NCG_HIP(hipMemsetAsync(tot, 0, sizeof(unsigned long long) * 2, st));
+    for (int attempt = 0; attempt < 2; ++attempt) {
         {
             ProfScope ps(ctx, st, "mesh_count");
             if (mode == 0)
-                hipLaunchKernelGGL(k_mcount<0>, dim3(cdiv(ins, MT)), dim3(MT), 0, st, skey, sid, pr,
-                                   vm, cs, ce, ins, (int)d, timestep, cbi, dc, cap, tot + 1, tot);
+                hipLaunchKernelGGL(k_mcount<0>, dim3(g), dim3(MT), 0, st, skey, sid, pr, vm, cs, ce,
+                                   0, cnt, (int)d, timestep, cbi, dc, cap, tot + 1, tot);
             else
-                hipLaunchKernelGGL(k_mcount<1>, dim3(cdiv(ins, MT)), dim3(MT), 0, st, skey, sid, pr,
-                                   vm, cs, ce, ins, (int)d, timestep, cbi, dc, cap, tot + 1, tot);
+                hipLaunchKernelGGL(k_mcount<1>, dim3(g), dim3(MT), 0, st, skey, sid, pr, vm, cs, ce,
+                                   0, cnt, (int)d, timestep, cbi, (int4 *)nullptr, 0ull, tot + 1, tot);
             NCG_CHECK_LAUNCH();
         }
         unsigned long long ht[2];
@@ -665,7 +725,14 @@ extern "C" ncg_status ncg_mesh_pass(ncg_ctx *ctx, const ncg_body *bodies, int32_
             break;
         }
         cap = ht[1];
+        dc = dbuf<int4>(ctx, "m_cand", cap);
+        if (!dc) return NCG_ERR_NOMEM;
+        NCG_HIP(hipMemsetAsync(tot, 0, sizeof(unsigned long long) * 2, st));
     }
+    const int ins = hc4[0];
+    M.prev_ins = ins;
+    const int nbig = std::min(hc4[3], big_cap);
+    NCG_REQUIRE(hc4[3] <= big_cap, "more than %d bodies larger than the mesh size", big_cap);
     std::vector<int> big(nbig);
     if (nbig) {
         NCG_HIP(hipMemcpyAsync(big.data(), bigl, sizeof(int) * nbig, hipMemcpyDeviceToHost, st));
@@ -864,12 +931,13 @@ extern "C" ncg_status ncg_mesh_pass(ncg_ctx *ctx, const ncg_body *bodies, int32_
         pk = dbuf<unsigned>(ctx, "m_prevkeys", std::max(nins, 1));
         if (!pk) return NCG_ERR_NOMEM;
         if (nins > 0) {
-            hipLaunchKernelGGL(k_bounds, dim3(cdiv(nins, MT)), dim3(MT), 0, st, skey, nins, cs, ce);
+            hipLaunchKernelGGL(k_bounds, dim3(cdiv(nins, MT)), dim3(MT), 0, st, skey, nins,
+                               (const int *)nullptr, cs, ce);
             NCG_HIP(hipMemcpyAsync(pk, skey, sizeof(unsigned) * nins, hipMemcpyDeviceToDevice, st));
             NCG_HIP(hipMemsetAsync(tot, 0, sizeof(unsigned long long) * 2, st));
             hipLaunchKernelGGL(k_mcount<1>, dim3(cdiv(nins, MT)), dim3(MT), 0, st, skey, sid, pr,
-                               vm, cs, ce, nins, (int)d, timestep, cbi, (int4 *)nullptr, 0ull,
-                               tot + 1, tot);
+                               vm, cs, ce, nins, (const int *)nullptr, (int)d, timestep, cbi,
+                               (int4 *)nullptr, 0ull, tot + 1, tot);
             NCG_CHECK_LAUNCH();
         }
         unsigned long long ht[2] = {0, 0};
@@ -934,16 +1002,22 @@ extern "C" ncg_status ncg_mesh_gravity(ncg_ctx *ctx, const ncg_body *bodies, int
     }
     const double G = grav_const, s = softening;
     const double mu = cp ? G * central->mass : 0.0;
+    const double cxh = cp ? central->x : 0.0, cyh = cp ? central->y : 0.0, czh = cp ? central->z : 0.0;
     {
         ProfScope ps(ctx, st, "mesh_gravity");
+        const unsigned *skey = dbuf<unsigned>(ctx, "m_skey", n);
+        const dim3 gs(cdiv(std::max(ins, 1), MT));
         if (prec == NCG_F64_EXACT)
-            hipLaunchKernelGGL(k_mgrav<0>, dim3(cdiv(n, MT)), dim3(MT), 0, st, db, (int)n, dex, key,
-                               cs, ce, pm, rr, sx, sid, (int)d, cp, cp ? central->x : 0.0,
-                               cp ? central->y : 0.0, cp ? central->z : 0.0, mu, G, s, dacc, dct);
+            hipLaunchKernelGGL(k_mgrav<0>, gs, dim3(MT), 0, st, db, (int)n, dex, ins, cs, ce, pm, rr,
+                               sx, sid, skey, (int)d, cp, cxh, cyh, czh, mu, G, s, dacc, dct);
+        else if (prec == NCG_F64_FAST)
+            hipLaunchKernelGGL(k_mgrav<1>, gs, dim3(MT), 0, st, db, (int)n, dex, ins, cs, ce, pm, rr,
+                               sx, sid, skey, (int)d, cp, cxh, cyh, czh, mu, G, s, dacc, dct);
         else
-            hipLaunchKernelGGL(k_mgrav<1>, dim3(cdiv(n, MT)), dim3(MT), 0, st, db, (int)n, dex, key,
-                               cs, ce, pm, rr, sx, sid, (int)d, cp, cp ? central->x : 0.0,
-                               cp ? central->y : 0.0, cp ? central->z : 0.0, mu, G, s, dacc, dct);
+            hipLaunchKernelGGL(k_mgrav<2>, gs, dim3(MT), 0, st, db, (int)n, dex, ins, cs, ce, pm, rr,
+                               sx, sid, skey, (int)d, cp, cxh, cyh, czh, mu, G, s, dacc, dct);
+        hipLaunchKernelGGL(k_mgrav_rest, dim3(cdiv(n, MT)), dim3(MT), 0, st, db, (int)n, dex, key, cp,
+                           cxh, cyh, czh, mu, G, s, dacc, dct);
         NCG_CHECK_LAUNCH();
     }
     hipLaunchKernelGGL(k_big_partial, dim3(nbp), dim3(MT), 0, st, db, (int)n, dex, G, s, bpart);

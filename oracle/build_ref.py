@@ -126,6 +126,13 @@ CONFIGS = {
         "inelastic_collision_bool": "0", "one_collision_only_bool": "1",
         "collision_cube_cells": "32", "collision_cube_min": "0.32",
         "how_many_neighbours": "40.0", "N_max": "5000"}))),
+    # Mesh module at the 2^20-body disk size (cells of side 0.01, 256 per
+    # axis): CPU baseline of one mesh() / get_neighbours_mesh() call.
+    "probe_c2": ("probe", _mesh(dict(PROBE_COMMON, **{
+        "G": "1.", "softening_parameter": "0.", "inelastic_collision_bool": "1",
+        "elastic_collision_bool": "0", "collision_parameter": "1.5",
+        "collision_cube_cells": "256", "collision_cube_min": "2.56",
+        "how_many_neighbours": "100.0", "N_max": "1048600"}))),
     # --- whole runs (drop-in parity) ------------------------------------------
     # Brute force, 2000 bodies, 200 steps of 2 pi/256, inelastic bounces.
     "run_c1": ("run", _run_cfg("brute", 2000, 200, DT1)),

@@ -20,7 +20,8 @@
  *
  * ops: 0 vector_field (physics.c:63), 1 brute_force (collision.c:425),
  *      2 mesh (collision.c:324), 3 get_neighbours_mesh (physics.c:1094),
- *      4 closest_approach on every i<j (collision.c:63).
+ *      4 closest_approach on every i<j (collision.c:63); op | 0x100 writes
+ *      only the header and CM_acc (timing runs).
  */
 #define _POSIX_C_SOURCE 199309L
 #include <stdio.h>
@@ -69,7 +70,9 @@ int main(int argc, char **argv)
     int hdr[4];
     rd(hdr, sizeof(int), 4, f);
     if (hdr[0] != 0x4e434750) return 2;
-    int op = hdr[1], n = hdr[2], npin = hdr[3];
+    /* op | 0x100: write the header and CM_acc only (timing runs). */
+    const int header_only = (hdr[1] & 0x100) != 0;
+    int op = hdr[1] & 0xff, n = hdr[2], npin = hdr[3];
     double prm[8], cm[8];
     rd(prm, sizeof(double), 8, f);
     rd(cm, sizeof(double), 8, f);
@@ -160,6 +163,10 @@ int main(int argc, char **argv)
     fwrite(ohdr, sizeof(int), 8, o);
     double ca4[4] = {CM_acc[0], CM_acc[1], CM_acc[2], 0.0};
     fwrite(ca4, sizeof(double), 4, o);
+    if (header_only) {
+        fclose(o);
+        return 0;
+    }
     fwrite(bodies, sizeof(double), 8 * (size_t)n, o);
     fwrite(coll_log, sizeof(int), 2 * (size_t)n_coll_log, o);
     for (int j = 0; j < np_out; j++) {

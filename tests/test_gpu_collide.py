@@ -79,6 +79,20 @@ def test_replay_matches_oracle_dense(ctx, seed):
     assert len(pairs) > 100
 
 
+@pytest.mark.parametrize("seed,f,one", [(23, 1.5, 0), (24, 2.0, 1)])
+def test_replay_builtin_ops_match_oracle(ctx, seed, f, one):
+    """ncg_builtin_replay_ops: the library's restatement of closest_approach
+    and the bounce gives the reference's collisions and final state."""
+    b = cluster(1200, seed, box=0.06, radius=2.5e-3)
+    out, _, pairs = orc.brute_force(b, 0.01, 1 if f != 2.0 else 0, f, one)
+    bb = b.copy()
+    dc = np.zeros(len(b), np.int32)
+    ops = ctx.builtin_ops(bb, 0.01, did_collide=dc, f=f, one_collision_only=one)
+    nc = ctx.brute_force_pass(bb, None, 0.01, ops=ops)
+    assert nc == ops.collisions == len(pairs) > 50
+    assert np.array_equal(bits(bb), bits(out))
+
+
 def test_replay_sparse_collisions_few_host_tests(ctx):
     # A handful of collisions: the host only re-tests rows/columns of the
     # bodies involved, not all n(n-1)/2 pairs.

@@ -62,7 +62,7 @@ def test_mesh_gravity_matches_reference(ctx):
     c8 = g["central"]
     cen = (c8[0], c8[1], c8[2], c8[6])
     ex = g["exists"].astype(bool)
-    for prec, tol in (("exact", 1e-13), ("fast64", 1e-12), ("mixed", 1e-12)):
+    for prec, tol in (("exact", 1e-13), ("fast64", 1e-12)):
         acc, cm = ctx.mesh_gravity(g["bodies"], g["exists"], cen, float(g["G"]), float(g["soft"]),
                                    prec)
         ref = g["acc"]
@@ -76,6 +76,15 @@ def test_mesh_gravity_matches_reference(ctx):
                                     float(g["soft"]))
     d = np.linalg.norm(accn[ex] - refn[ex], axis=1)
     assert np.max(d / np.linalg.norm(refn[ex], axis=1)) < 1e-12
+    # Mixed form (FP32 forces, per-cell FP32 sums into FP64) on the mutual part.
+    accm, _ = ctx.mesh_gravity(g["bodies"], g["exists"], None, float(g["G"]), float(g["soft"]),
+                               "mixed")
+    dm = np.linalg.norm(accm[ex] - refn[ex], axis=1)
+    rn = np.linalg.norm(refn[ex], axis=1)
+    print("mesh mixed: max rel %.3e normwise %.3e" % ((dm / rn).max(),
+                                                      dm.max() / np.sqrt(np.mean(rn * rn))))
+    assert dm.max() / np.sqrt(np.mean(rn * rn)) < 1e-6
+    assert (dm / rn).max() < 1e-4
 
 
 def blob(n, seed, half=0.07, rmin=5e-4, rmax=2e-3, vs=0.3):
@@ -100,6 +109,24 @@ def test_mesh_pass_matches_oracle_random(ctx, seed, cells, gam):
     assert st["total_neighbours"] == ref["total_neighbours"]
     assert st["how_many_small"] == ref["how_many_small"]
     assert np.array_equal(bits(r.b), bits(ref["bodies"]))
+    assert np.array_equal(ctx.mesh_pairs(), ref["pairs"])
+
+
+@pytest.mark.parametrize("seed,cells,gam,one", [(5, 32, 0.004, 0), (6, 16, 0.01, 1)])
+def test_mesh_pass_builtin_ops_match_oracle(ctx, seed, cells, gam, one):
+    """The library's own decision/outcome (ncg_builtin_replay_ops) gives the
+    reference's collisions, counters, states and pairs."""
+    b = blob(6000, seed)
+    cube = cells * gam
+    ex = (np.random.default_rng(seed).uniform(size=6000) > 0.05).astype(np.int32)
+    ref = orc.mesh(b, 0.01, gam, cube, cells, 1, 1.5, one, 1, ex)
+    bb = b.copy()
+    dc = np.zeros(6000, np.int32)
+    ops = ctx.builtin_ops(bb, 0.01, exists=ex, did_collide=dc, f=1.5, one_collision_only=one)
+    st = ctx.mesh_pass(bb, ex, 0.01, gam, cube, cells, 0, ops=ops)
+    assert ops.collisions == len(ref["collisions"]) > 10
+    assert st["total_neighbours"] == ref["total_neighbours"]
+    assert np.array_equal(bits(bb), bits(ref["bodies"]))
     assert np.array_equal(ctx.mesh_pairs(), ref["pairs"])
 
 
