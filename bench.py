@@ -218,6 +218,24 @@ def ensemble_run(ctx, torch, nsys=512, steps=100):
             "collisions": nc, "precision": "mixed", "graph": True}
 
 
+def collision_filter_run(ctx, b, dt):
+    """closest_approach predicate on every pair i < j of the workload (the
+    GPU filter of brute_force), timed with HIP events."""
+    n = b.shape[0]
+    ctx.collide_candidates(b, None, dt)  # warm-up
+    ctx.profile(True)
+    ctx.profile_read("candidates", reset=True)
+    for _ in range(2):
+        cand = ctx.collide_candidates(b, None, dt)
+    ms, k = ctx.profile_read("candidates", reset=True)
+    ctx.profile(False)
+    ms /= k
+    tests = n * (n - 1) / 2.0
+    return {"kernel_ms": ms, "pair_tests_per_s": tests / (ms * 1e-3),
+            "candidates": int(cand.shape[0]), "n": n,
+            "flop_per_test": 27, "achieved_tflops": 27 * tests / (ms * 1e-3) / 1e12}
+
+
 def precision_run(ctx, torch, b, soft, dt, steps=20):
     """Energy drift over `steps` RK4 steps of the workload in the mixed and
     the FP64 form (the FP64 energy kernel measures both), and how far the two
@@ -302,6 +320,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--no-mesh", action="store_true")
+    ap.add_argument("--no-collisions", action="store_true")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run a plain device copy of this size (HBM counter calibration)")
     args = ap.parse_args()
@@ -412,6 +431,8 @@ def main():
                            "normwise_max_err": float(d.max() / torch.sqrt(torch.mean(r * r))),
                            "reference": "FP64-fast form (mutual acceleration only)"}
 
+    if not args.no_collisions:
+        out["collision_filter"] = collision_filter_run(ctx, b, dt)
     if not args.no_precision:
         out["precision_study"] = precision_run(ctx, torch, b, soft, dt)
     if not args.no_mesh:

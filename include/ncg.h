@@ -313,8 +313,9 @@ ncg_status ncg_energy_dev(ncg_ctx *ctx, const ncg_body *d_bodies, int32_t n,
  * Launch shape of the tiled gravity forms (NCG_F64_FAST, NCG_MIXED): bodies
  * per lane (1, 2, 4; 0 keeps), j-splits per i-block (0 = from the CU count,
  * -1 keeps), near/far threshold theta (<= 0 keeps), tile source (0 scalar
- * loads, 1 LDS, -1 keeps).  Results do not depend on the shape beyond FP
- * rounding of the split sums.  The autotuner (ncorpion_amd/autotune.py)
+ * loads, 1 LDS, 2 scalar loads issued one 8-body batch ahead with paired
+ * accumulators, 3 paired accumulators only; -1 keeps).  Results do not
+ * depend on the shape beyond FP rounding of the split sums.  The autotuner (ncorpion_amd/autotune.py)
  * picks these per N.
  */
 ncg_status ncg_set_tiling(ncg_ctx *ctx, int32_t ipl, int32_t nsplit, double theta, int32_t source);

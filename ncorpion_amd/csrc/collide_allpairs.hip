@@ -105,9 +105,12 @@ static ncg_status candidates_dev(ncg_ctx *ctx, const DBody *db, int32_t n, const
         unsigned long long *dcnt = dbuf<unsigned long long>(ctx, "cand_cnt", 1);
         if (!dout || !dcnt) return NCG_ERR_NOMEM;
         NCG_HIP(hipMemsetAsync(dcnt, 0, sizeof(unsigned long long), st));
-        hipLaunchKernelGGL(k_candidates, dim3((unsigned)nblk), dim3(CT), 0, st, db, (int)n, dex, dt,
-                           ntiles, dout, cap, dcnt);
-        NCG_CHECK_LAUNCH();
+        {
+            ProfScope ps(ctx, st, "candidates");
+            hipLaunchKernelGGL(k_candidates, dim3((unsigned)nblk), dim3(CT), 0, st, db, (int)n, dex,
+                               dt, ntiles, dout, cap, dcnt);
+            NCG_CHECK_LAUNCH();
+        }
         unsigned long long cnt = 0;
         NCG_HIP(hipMemcpyAsync(&cnt, dcnt, sizeof cnt, hipMemcpyDeviceToHost, st));
         NCG_HIP(hipStreamSynchronize(st));

@@ -21,6 +21,7 @@
 // reference's lexicographic order with the built-in outcome, and systems that
 // changed get their stage-1 evaluation again.
 #include <algorithm>
+#include <cstring>
 #include <numeric>
 
 #include "ncg_internal.hpp"
@@ -232,11 +233,14 @@ __global__ __launch_bounds__(ENT) void k_ens_pass(const EnsUnit *__restrict__ un
 }
 
 // Per-system CM_acc: the partials of the system's gravity units in unit order.
+// With `partial`, systems without units in this pass keep their value.
 __global__ void k_ens_cm(const int *__restrict__ ufirst, const int *__restrict__ ulist,
-                         const double *__restrict__ cpart, int nsys, double *__restrict__ cm)
+                         const double *__restrict__ cpart, int nsys, int partial,
+                         double *__restrict__ cm)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nsys) return;
+    if (partial && ufirst[s] == ufirst[s + 1]) return;
     double a[3] = {0., 0., 0.};
     for (int q = ufirst[s]; q < ufirst[s + 1]; ++q)
         for (int d = 0; d < 3; ++d) a[d] += cpart[3 * (size_t)ulist[q] + d];
@@ -388,14 +392,15 @@ static void launch_pass(int ipl, int nblk, hipStream_t st, const EnsUnit *units,
                            counter, b, off, cen, G, soft, dt, acc, cpart, cand, cap, ccnt);
 }
 
-// One evaluation of all systems (optionally fused with the collision pass).
-static ncg_status ens_eval(ncg_ctx *ctx, const EnsPlan &P, const EnsDev &D, const DBody *b,
+// One pass over the unit list du[nunits] (gravity units, optionally with
+// collision units) and the per-system CM_acc from its index lists.
+static ncg_status ens_pass(ncg_ctx *ctx, const EnsPlan &P, const int *doff, const EnsUnit *du,
+                           int nunits, const int *dfirst, const int *dlist, bool partial,
+                           const DBody *b,
                            const double *cen, double G, double soft, ncg_precision prec,
-                           double dt, bool with_coll, double *acc, double *cm, hipStream_t st)
+                           double dt, double *acc, double *cm, hipStream_t st)
 {
-    const std::vector<EnsUnit> &U = with_coll ? P.both : P.grav;
-    const EnsUnit *du = with_coll ? D.both : D.grav;
-    const int nunits = (int)U.size();
+    if (nunits == 0) return NCG_OK;
     unsigned *counter = dbuf<unsigned>(ctx, "e_counter", 2);
     double *cpart = dbuf<double>(ctx, "e_cpart", 3 * (size_t)std::max(nunits, 1));
     const unsigned cap = 1u << 20;
@@ -407,18 +412,27 @@ static ncg_status ens_eval(ncg_ctx *ctx, const EnsPlan &P, const EnsDev &D, cons
     {
         ProfScope ps(ctx, st, prec == NCG_MIXED ? "ens_mixed" : "ens_fast64");
         if (prec == NCG_MIXED)
-            launch_pass<1>(P.ipl, nblk, st, du, nunits, counter, b, D.off, cen, G, soft, dt, acc,
+            launch_pass<1>(P.ipl, nblk, st, du, nunits, counter, b, doff, cen, G, soft, dt, acc,
                            cpart, cand, cap, counter + 1);
         else
-            launch_pass<0>(P.ipl, nblk, st, du, nunits, counter, b, D.off, cen, G, soft, dt, acc,
+            launch_pass<0>(P.ipl, nblk, st, du, nunits, counter, b, doff, cen, G, soft, dt, acc,
                            cpart, cand, cap, counter + 1);
         NCG_CHECK_LAUNCH();
     }
-    hipLaunchKernelGGL(k_ens_cm, dim3(cdiv(P.nsys, 128)), dim3(128), 0, st,
-                       with_coll ? D.bfirst : D.gfirst, with_coll ? D.blist : D.glist, cpart,
-                       P.nsys, cm);
+    hipLaunchKernelGGL(k_ens_cm, dim3(cdiv(P.nsys, 128)), dim3(128), 0, st, dfirst, dlist, cpart,
+                       P.nsys, partial ? 1 : 0, cm);
     NCG_CHECK_LAUNCH();
     return NCG_OK;
+}
+
+// One evaluation of all systems (optionally fused with the collision pass).
+static ncg_status ens_eval(ncg_ctx *ctx, const EnsPlan &P, const EnsDev &D, const DBody *b,
+                           const double *cen, double G, double soft, ncg_precision prec,
+                           double dt, bool with_coll, double *acc, double *cm, hipStream_t st)
+{
+    const int nunits = (int)(with_coll ? P.both.size() : P.grav.size());
+    return ens_pass(ctx, P, D.off, with_coll ? D.both : D.grav, nunits, with_coll ? D.bfirst : D.gfirst,
+                    with_coll ? D.blist : D.glist, false, b, cen, G, soft, prec, dt, acc, cm, st);
 }
 
 extern "C" ncg_status ncg_ensemble_rk4_dev(ncg_ctx *ctx, ncg_body *d_bodies,
@@ -504,7 +518,10 @@ extern "C" ncg_status ncg_ensemble_rk4_dev(ncg_ctx *ctx, ncg_body *d_bodies,
 
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph = nullptr;
-    std::vector<ncg_body> host;
+    // Pinned staging for the systems that have candidates (allocated before
+    // any queued work can read it).
+    ncg_body *host = collide ? pbuf<ncg_body>(ctx, "e_host", ntot) : nullptr;
+    if (collide && !host) return NCG_ERR_NOMEM;
     int64_t ncoll_total = 0;
     for (int32_t step = 0; step < nsteps; ++step) {
         bool skip0 = false;
@@ -521,15 +538,36 @@ extern "C" ncg_status ncg_ensemble_rk4_dev(ncg_ctx *ctx, ncg_body *d_bodies,
             if (nc > 0) {
                 std::vector<int2> cv(nc);
                 NCG_HIP(hipMemcpyAsync(cv.data(), cand, sizeof(int2) * nc, hipMemcpyDeviceToHost, st));
-                if (host.empty()) host.resize(ntot);
-                NCG_HIP(hipMemcpyAsync(host.data(), y0, sizeof(DBody) * ntot, hipMemcpyDeviceToHost, st));
                 NCG_HIP(hipStreamSynchronize(st));
                 std::sort(cv.begin(), cv.end(), [](const int2 &a, const int2 &b) {
                     return a.x != b.x ? a.x < b.x : a.y < b.y;
                 });
+                // Download only the systems with candidates (adjacent ones in
+                // one copy).
+                {
+                    int r0 = -1, r1 = -1;  // current body range [r0, r1)
+                    for (size_t q = 0; q < cv.size();) {
+                        const int s = (int)(std::upper_bound(P.off.begin(), P.off.end(), cv[q].x) -
+                                            P.off.begin()) - 1;
+                        const int b0 = P.off[s], b1 = P.off[s + 1];
+                        if (b0 != r1) {
+                            if (r1 > r0)
+                                NCG_HIP(hipMemcpyAsync(host + r0, y0 + r0, sizeof(DBody) * (r1 - r0),
+                                                       hipMemcpyDeviceToHost, st));
+                            r0 = b0;
+                        }
+                        r1 = b1;
+                        while (q < cv.size() && cv[q].x < b1) ++q;
+                    }
+                    if (r1 > r0)
+                        NCG_HIP(hipMemcpyAsync(host + r0, y0 + r0, sizeof(DBody) * (r1 - r0),
+                                               hipMemcpyDeviceToHost, st));
+                    NCG_HIP(hipStreamSynchronize(st));
+                }
                 // Replay system by system (local indices).
                 size_t q = 0;
                 bool changed = false;
+                std::vector<char> chg(nsys, 0);
                 while (q < cv.size()) {
                     const int g = cv[q].x;
                     const int s = (int)(std::upper_bound(P.off.begin(), P.off.end(), g) - P.off.begin()) - 1;
@@ -539,17 +577,42 @@ extern "C" ncg_status ncg_ensemble_rk4_dev(ncg_ctx *ctx, ncg_body *d_bodies,
                         loc.push_back({cv[q].x - base, cv[q].y - base});
                         ++q;
                     }
-                    const int64_t k = ncg_builtin_bf_replay(host.data() + base, ns, nullptr, dt,
+                    const int64_t k = ncg_builtin_bf_replay(host + base, ns, nullptr, dt,
                                                             collision_f, loc, nullptr);
                     if (k) {
                         ncoll_total += k;
                         changed = true;
-                        NCG_HIP(hipMemcpyAsync(y0 + base, host.data() + base, sizeof(DBody) * ns,
+                        chg[s] = 1;
+                        NCG_HIP(hipMemcpyAsync(y0 + base, host + base, sizeof(DBody) * ns,
                                                hipMemcpyHostToDevice, st));
                     }
                 }
                 if (changed) {
-                    r = ens_eval(ctx, P, D, y0, d_centrals, G, soft, prec, dt, false, acc, cm, st);
+                    // Stage-1 evaluation again, for the changed systems only.
+                    std::vector<EnsUnit> sub;
+                    for (const EnsUnit &u : P.grav)
+                        if (chg[u.sys]) sub.push_back(u);
+                    std::vector<int> sfirst, slist;
+                    unit_index(P, sub, sfirst, slist);
+                    const size_t nu = sub.size();
+                    // Pinned staging: units, then first[nsys + 1], then list.
+                    const size_t bytes = sizeof(EnsUnit) * nu + sizeof(int) * (sfirst.size() + slist.size());
+                    unsigned char *hs = pbuf<unsigned char>(ctx, "e_subh", sizeof(EnsUnit) * P.grav.size() +
+                                                                               sizeof(int) * (nsys + 1 + P.grav.size()));
+                    unsigned char *ds = dbuf<unsigned char>(ctx, "e_sub", sizeof(EnsUnit) * P.grav.size() +
+                                                                              sizeof(int) * (nsys + 1 + P.grav.size()));
+                    if (!hs || !ds) return NCG_ERR_NOMEM;
+                    NCG_HIP(hipStreamSynchronize(st));  // the staging buffer is reused each step
+                    memcpy(hs, sub.data(), sizeof(EnsUnit) * nu);
+                    memcpy(hs + sizeof(EnsUnit) * nu, sfirst.data(), sizeof(int) * sfirst.size());
+                    memcpy(hs + sizeof(EnsUnit) * nu + sizeof(int) * sfirst.size(), slist.data(),
+                           sizeof(int) * slist.size());
+                    NCG_HIP(hipMemcpyAsync(ds, hs, bytes, hipMemcpyHostToDevice, st));
+                    const EnsUnit *du = reinterpret_cast<const EnsUnit *>(ds);
+                    const int *df = reinterpret_cast<const int *>(ds + sizeof(EnsUnit) * nu);
+                    const int *dl = df + sfirst.size();
+                    r = ens_pass(ctx, P, D.off, du, (int)nu, df, dl, true, y0, d_centrals, G, soft,
+                                 prec, dt, acc, cm, st);
                     if (r != NCG_OK) return r;
                 }
             }

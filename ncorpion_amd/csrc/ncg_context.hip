@@ -43,6 +43,26 @@ void *ncg_dbuf(ncg_ctx *ctx, const char *name, size_t bytes)
     return b.ptr;
 }
 
+void *ncg_pbuf(ncg_ctx *ctx, const char *name, size_t bytes)
+{
+    DevBuf &b = ctx->pbufs[name];
+    if (b.bytes >= bytes && b.ptr) return b.ptr;
+    if (b.ptr) {
+        hipStreamSynchronize(ctx->stream);
+        hipHostFree(b.ptr);
+        b.ptr = nullptr;
+        b.bytes = 0;
+    }
+    size_t want = bytes + bytes / 4 + 256;
+    if (hipHostMalloc(&b.ptr, want, hipHostMallocDefault) != hipSuccess) {
+        b.ptr = nullptr;
+        ncg_fail(ctx, NCG_ERR_NOMEM, "hipHostMalloc of %zu bytes for '%s' failed", want, name);
+        return nullptr;
+    }
+    b.bytes = want;
+    return b.ptr;
+}
+
 // Reads the tuning table: $NCG_TUNING, else ncg_tuning.txt beside libncg.so.
 // Lines: <form> <n_min> <ipl> <nsplit> <theta> <src>; '#' comments.  Forms:
 // fast64, mixed (single-system tiled kernel, n_min = bodies) and ens_fast64,
@@ -80,7 +100,7 @@ static void load_tuning(ncg_ctx *ctx)
                  : !strcmp(pname, "ens_fast64") ? 100 + NCG_F64_FAST
                                                 : -1;
         if (r.prec < 0 || (ipl != 1 && ipl != 2 && ipl != 4) || nsplit < 0 || theta <= 0.0 ||
-            (src != 0 && src != 1))
+            src < 0 || src > 3)
             continue;
         r.n_min = nmin;
         r.cfg.ipl = ipl;
@@ -141,6 +161,8 @@ void ncg_destroy(ncg_ctx *ctx)
         }
     for (auto &kv : ctx->bufs)
         if (kv.second.ptr) (void)hipFree(kv.second.ptr);
+    for (auto &kv : ctx->pbufs)
+        if (kv.second.ptr) (void)hipHostFree(kv.second.ptr);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }

@@ -75,6 +75,7 @@ struct ncg_ctx {
     std::string err;
     std::unordered_map<std::string, DevBuf> bufs;
     std::unordered_map<std::string, std::vector<unsigned char>> hbufs;
+    std::unordered_map<std::string, DevBuf> pbufs;  // pinned host buffers
     MeshState mesh;
     TiledCfg tiled;
     bool tiled_explicit = false;     // ncg_set_tiling overrides the table
@@ -126,6 +127,14 @@ template <class T>
 static inline T *dbuf(ncg_ctx *ctx, const char *name, size_t count)
 {
     return static_cast<T *>(ncg_dbuf(ctx, name, count * sizeof(T) + 16));
+}
+
+// Pinned host buffer `name` (hipHostMalloc), grown like ncg_dbuf.
+void *ncg_pbuf(ncg_ctx *ctx, const char *name, size_t bytes);
+template <class T>
+static inline T *pbuf(ncg_ctx *ctx, const char *name, size_t count)
+{
+    return static_cast<T *>(ncg_pbuf(ctx, name, count * sizeof(T) + 16));
 }
 
 // Host staging vector `name` of at least `count` elements.
