@@ -268,42 +268,55 @@ def adjoint_run(ctx, torch, n=16384, steps=200, K=20):
     length R_i + R_j), 200 FP64 RK4 steps dt = 2 pi/256; loss = mean squared
     distance of the final positions to a seeded target (sigma 1e-3); adjoint
     w.r.t. initial positions, velocities and masses with a checkpoint every
-    20 steps."""
+    20 steps.  Times: the plain forward run (RK4 driver, graph replay), the
+    forward run that keeps the checkpoints, the backward pass from them, and
+    the all-in-one entry point that runs its own forward pass."""
     import numpy as np
     from ncorpion_amd import ic
     b = ic.disk(n, seed=0, mass=1e-8, radius=2e-4)
     dt = 2.0 * math.pi / 256.0
     bt = torch.from_numpy(b).cuda()
     c0 = torch.tensor([0, 0, 0, 0, 0, 0, 1.0, 1.0], dtype=torch.float64, device="cuda")
+    g0 = torch.zeros((n, 6), dtype=torch.float64, device="cuda")
+    gm = torch.zeros(n, dtype=torch.float64, device="cuda")
+    gc = torch.zeros(7, dtype=torch.float64, device="cuda")
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = fn()
+        ctx.synchronize()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, r
+
     # Warm-up (buffers, graph capture, adjoint work arrays), untimed.
     wb, wc = bt.clone(), c0.clone()
-    ctx.rk4_steps_dev(wb, wc, 2, dt, 1.0, ADJ_SOFT, "fast64", graph=True)
-    wg = torch.zeros((n, 6), dtype=torch.float64, device="cuda")
-    wm = torch.zeros(n, dtype=torch.float64, device="cuda")
-    wcg = torch.zeros(7, dtype=torch.float64, device="cuda")
-    ctx.rk4_adjoint_dev(bt, c0, wg, wg.clone(), wm, wcg, K, dt, K, 1.0, ADJ_SOFT)
+    ck, ckc = ctx.rk4_forward_ckpt_dev(wb, wc, K, dt, K, 1.0, ADJ_SOFT)
+    ctx.rk4_adjoint_ckpt_dev(ck, ckc, g0, g0.clone(), gm, gc, K, dt, K, 1.0, ADJ_SOFT)
+    ctx.rk4_adjoint_dev(bt, c0, g0, g0.clone(), gm, gc, K, dt, K, 1.0, ADJ_SOFT)
     ctx.synchronize()
     fb, fc = bt.clone(), c0.clone()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    ctx.rk4_steps_dev(fb, fc, steps, dt, 1.0, ADJ_SOFT, "fast64", graph=True)
-    ctx.synchronize()
-    t_fwd = time.perf_counter() - t0
+    t_fwd, _ = timed(lambda: ctx.rk4_steps_dev(fb, fc, steps, dt, 1.0, ADJ_SOFT, "fast64",
+                                               graph=True))
     target = fb[:, 0:3] + 1e-3 * torch.tensor(np.random.default_rng(0).normal(size=(n, 3)),
                                               device="cuda")
     gfin = torch.zeros((n, 6), dtype=torch.float64, device="cuda")
     gfin[:, 0:3] = 2.0 / n * (fb[:, 0:3] - target)
-    g0 = torch.zeros((n, 6), dtype=torch.float64, device="cuda")
-    gm = torch.zeros(n, dtype=torch.float64, device="cuda")
-    gc = torch.zeros(7, dtype=torch.float64, device="cuda")
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    ctx.rk4_adjoint_dev(bt, c0, gfin, g0, gm, gc, steps, dt, K, 1.0, ADJ_SOFT)
-    ctx.synchronize()
-    t_adj = time.perf_counter() - t0
+    kb, kc = bt.clone(), c0.clone()
+    t_fck, (ck, ckc) = timed(lambda: ctx.rk4_forward_ckpt_dev(kb, kc, steps, dt, K, 1.0, ADJ_SOFT))
+    t_bwd, _ = timed(lambda: ctx.rk4_adjoint_ckpt_dev(ck, ckc, gfin, g0, gm, gc, steps, dt, K, 1.0,
+                                                      ADJ_SOFT))
+    g_ck = g0.clone()
+    t_all, _ = timed(lambda: ctx.rk4_adjoint_dev(bt, c0, gfin, g0, gm, gc, steps, dt, K, 1.0,
+                                                 ADJ_SOFT))
+    same = bool(torch.equal(g_ck, g0))
     return {"n": n, "steps": steps, "checkpoint_every": K, "forward_s": t_fwd,
-            "adjoint_s": t_adj, "adjoint_over_forward": t_adj / t_fwd,
-            "note": "adjoint time includes its own forward pass and segment recomputation"}
+            "forward_with_checkpoints_s": t_fck, "backward_s": t_bwd,
+            "adjoint_over_forward": t_bwd / t_fwd,
+            "forward_and_backward_over_forward": (t_fck + t_bwd) / t_fwd,
+            "all_in_one_s": t_all, "all_in_one_over_forward": t_all / t_fwd,
+            "gradients_identical_both_ways": same,
+            "note": "backward = segment recomputation from the checkpoints + 4 VJPs per step"}
 
 
 def main():

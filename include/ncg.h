@@ -294,6 +294,26 @@ ncg_status ncg_rk4_adjoint_dev(ncg_ctx *ctx, const ncg_body *d_bodies0, int32_t 
                                double *d_g0, double *d_gm, double *d_gcentral0, ncg_body *d_final,
                                void *stream);
 
+/* The same adjoint split in two, so that the forward run the caller needs
+ * anyway keeps the checkpoints:
+ * ncg_rk4_forward_ckpt_dev advances d_bodies / d_central by nsteps FP64 RK4
+ * steps (the ncg_rk4_steps_dev driver, NCG_F64_FAST, graph replay) and
+ * stores the state before step q * checkpoint_every in d_ckpt (n records per
+ * checkpoint) and d_ckpt_central (8 doubles per checkpoint), q = 0 ..
+ * (nsteps - 1) / checkpoint_every.
+ * ncg_rk4_adjoint_ckpt_dev runs the backward pass of ncg_rk4_adjoint_dev
+ * from those checkpoints (same cotangent inputs and gradient outputs). */
+ncg_status ncg_rk4_forward_ckpt_dev(ncg_ctx *ctx, ncg_body *d_bodies, int32_t n, double *d_central,
+                                    double grav_const, double softening, double dt, int32_t nsteps,
+                                    int32_t checkpoint_every, ncg_body *d_ckpt,
+                                    double *d_ckpt_central, void *stream);
+ncg_status ncg_rk4_adjoint_ckpt_dev(ncg_ctx *ctx, const ncg_body *d_ckpt, int32_t n,
+                                    const double *d_ckpt_central, double grav_const,
+                                    double softening, double dt, int32_t nsteps,
+                                    int32_t checkpoint_every, const double *d_gfinal,
+                                    const double *d_gcentral_final, double *d_g0, double *d_gm,
+                                    double *d_gcentral0, void *stream);
+
 ncg_status ncg_builtin_replay_ops(ncg_builtin_collider *state, ncg_replay_ops *ops);
 
 /* ---------------------------------------------------------------------------
@@ -313,9 +333,8 @@ ncg_status ncg_energy_dev(ncg_ctx *ctx, const ncg_body *d_bodies, int32_t n,
  * Launch shape of the tiled gravity forms (NCG_F64_FAST, NCG_MIXED): bodies
  * per lane (1, 2, 4; 0 keeps), j-splits per i-block (0 = from the CU count,
  * -1 keeps), near/far threshold theta (<= 0 keeps), tile source (0 scalar
- * loads, 1 LDS, 2 scalar loads issued one 8-body batch ahead with paired
- * accumulators, 3 paired accumulators only; -1 keeps).  Results do not
- * depend on the shape beyond FP rounding of the split sums.  The autotuner (ncorpion_amd/autotune.py)
+ * loads, 1 LDS, -1 keeps).  Results do not depend on the shape beyond FP
+ * rounding of the split sums.  The autotuner (ncorpion_amd/autotune.py)
  * picks these per N.
  */
 ncg_status ncg_set_tiling(ncg_ctx *ctx, int32_t ipl, int32_t nsplit, double theta, int32_t source);

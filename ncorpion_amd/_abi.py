@@ -103,6 +103,8 @@ _SIGNATURES = {
     "ncg_ensemble_rk4_dev": (S, [P, P, P, I32, P, D, D, S, D, I32, I32, D, ctypes.POINTER(I64), P]),
     "ncg_ensemble_gravity_dev": (S, [P, P, P, I32, P, D, D, S, P, P, P]),
     "ncg_energy_dev": (S, [P, P, I32, P, P, D, D, P, P]),
+    "ncg_rk4_forward_ckpt_dev": (S, [P, P, I32, P, D, D, D, I32, I32, P, P, P]),
+    "ncg_rk4_adjoint_ckpt_dev": (S, [P, P, I32, P, D, D, D, I32, I32, P, P, P, P, P, P]),
     "ncg_builtin_replay_ops": (S, [ctypes.POINTER(BuiltinCollider), ctypes.POINTER(ReplayOps)]),
     "ncg_profile_enable": (S, [P, S]),
     "ncg_profile_read": (S, [P, ctypes.c_char_p, ctypes.POINTER(D), ctypes.POINTER(I64), S]),
@@ -277,6 +279,34 @@ class Context:
             None if central_t is None else central_t.data_ptr(), float(G), float(softening),
             out_t.data_ptr(), stream))
         return out_t
+
+    def rk4_forward_ckpt_dev(self, bodies_t, central_t, nsteps, dt, checkpoint_every=20, G=1.0,
+                             softening=0.0, ckpt_t=None, ckpt_central_t=None, stream=None):
+        """Advances bodies_t / central_t in place by nsteps FP64 RK4 steps and
+        returns (ckpt_t, ckpt_central_t), the states every checkpoint_every
+        steps (allocated when not given)."""
+        import torch
+        n = int(bodies_t.shape[0])
+        nck = max(1, (int(nsteps) - 1) // int(checkpoint_every) + 1)
+        if ckpt_t is None:
+            ckpt_t = torch.empty((nck, n, 8), dtype=torch.float64, device=bodies_t.device)
+        if ckpt_central_t is None:
+            ckpt_central_t = torch.empty((nck, 8), dtype=torch.float64, device=bodies_t.device)
+        self._check(self.lib.ncg_rk4_forward_ckpt_dev(
+            self.h, bodies_t.data_ptr(), n, central_t.data_ptr(), float(G), float(softening),
+            float(dt), int(nsteps), int(checkpoint_every), ckpt_t.data_ptr(),
+            ckpt_central_t.data_ptr(), stream))
+        return ckpt_t, ckpt_central_t
+
+    def rk4_adjoint_ckpt_dev(self, ckpt_t, ckpt_central_t, gfinal_t, g0_t, gm_t, gc0_t, nsteps, dt,
+                             checkpoint_every=20, G=1.0, softening=0.0, gcentral_final_t=None,
+                             stream=None):
+        n = int(ckpt_t.shape[1])
+        self._check(self.lib.ncg_rk4_adjoint_ckpt_dev(
+            self.h, ckpt_t.data_ptr(), n, ckpt_central_t.data_ptr(), float(G), float(softening),
+            float(dt), int(nsteps), int(checkpoint_every), gfinal_t.data_ptr(),
+            None if gcentral_final_t is None else gcentral_final_t.data_ptr(), g0_t.data_ptr(),
+            gm_t.data_ptr(), gc0_t.data_ptr(), stream))
 
     def rk4_adjoint_dev(self, bodies0_t, central0_t, gfinal_t, g0_t, gm_t, gc0_t, nsteps, dt,
                         checkpoint_every=20, G=1.0, softening=0.0, gcentral_final_t=None,

@@ -6,9 +6,10 @@
 
 x, v: (n, 3) float64 CUDA tensors; m, radius: (n,); central: (8,) =
 {x, y, z, vx, vy, vz, M, present}.  Forward passes run the FP64-fast form on
-the GPU; backward passes call the hand-written VJP kernels
-(ncg_vjp_gravity_dev) and the checkpointed discrete RK4 adjoint
-(ncg_rk4_adjoint_dev).  Gradients flow to x/x0, v0, m and the central body;
+the GPU (RK4Trajectory keeps a checkpoint every checkpoint_every steps,
+ncg_rk4_forward_ckpt_dev); backward passes call the hand-written VJP kernels
+(ncg_vjp_gravity_dev) and the discrete RK4 adjoint from those checkpoints
+(ncg_rk4_adjoint_ckpt_dev).  Gradients flow to x/x0, v0, m and the central body;
 radii (softening lengths) are not differentiated.  Collision-free dynamics
 only: contact events are not differentiable.
 """
@@ -88,23 +89,22 @@ class RK4Trajectory(torch.autograd.Function):
     def forward(fctx, x0, v0, m, radius, central0, G=1.0, softening=0.0, dt=0.01, nsteps=1,
                 checkpoint_every=20, ctx=None):
         ctx = ctx or default_context()
-        b0 = pack_bodies(x0.detach(), v0.detach(), m.detach(), radius.detach())
-        c0 = central0.detach().contiguous().clone()
-        b = b0.clone()
-        c = c0.clone()
+        b = pack_bodies(x0.detach(), v0.detach(), m.detach(), radius.detach())
+        c = central0.detach().contiguous().clone()
         _sync(ctx)
-        ctx.rk4_steps_dev(b, c, nsteps, dt, G, softening, "fast64", graph=False)
+        # The forward run keeps a checkpoint every `checkpoint_every` steps.
+        ck, ckc = ctx.rk4_forward_ckpt_dev(b, c, nsteps, dt, checkpoint_every, G, softening)
         ctx.synchronize()
-        fctx.save_for_backward(b0, c0)
+        fctx.save_for_backward(ck, ckc)
         fctx.args = (G, softening, dt, nsteps, checkpoint_every, ctx)
         return b[:, 0:3].clone(), b[:, 3:6].clone(), c.clone()
 
     @staticmethod
     def backward(fctx, g_x, g_v, g_c):
-        b0, c0 = fctx.saved_tensors
+        ck, ckc = fctx.saved_tensors
         G, soft, dt, nsteps, K, ctx = fctx.args
-        n = b0.shape[0]
-        dev = b0.device
+        n = ck.shape[1]
+        dev = ck.device
         gfin = torch.zeros((n, 6), dtype=torch.float64, device=dev)
         if g_x is not None:
             gfin[:, 0:3] = g_x
@@ -117,8 +117,8 @@ class RK4Trajectory(torch.autograd.Function):
         gm = torch.zeros(n, dtype=torch.float64, device=dev)
         gc0 = torch.zeros(7, dtype=torch.float64, device=dev)
         _sync(ctx)
-        ctx.rk4_adjoint_dev(b0, c0, gfin.contiguous(), g0, gm, gc0, nsteps, dt, K, G, soft,
-                            gcf.contiguous())
+        ctx.rk4_adjoint_ckpt_dev(ck, ckc, gfin.contiguous(), g0, gm, gc0, nsteps, dt, K, G, soft,
+                                 gcf.contiguous())
         ctx.synchronize()
         gcen = torch.zeros(8, dtype=torch.float64, device=dev)
         gcen[0:6] = gc0[0:6]

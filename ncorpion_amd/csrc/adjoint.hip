@@ -347,26 +347,15 @@ static ncg_status fwd_step_store(ncg_ctx *ctx, DBody *y, int32_t n, double *c0, 
     return NCG_OK;
 }
 
-extern "C" ncg_status ncg_rk4_adjoint_dev(ncg_ctx *ctx, const ncg_body *d_bodies0, int32_t n,
-                                          const double *d_central0, double grav_const,
-                                          double softening, double dt, int32_t nsteps,
-                                          int32_t checkpoint_every, const double *d_gfinal,
-                                          const double *d_gcentral_final, double *d_g0,
-                                          double *d_gm, double *d_gcentral0, ncg_body *d_final,
-                                          void *stream)
+// Backward pass from checkpoints ck (state before step q K, q = 0..) and
+// ckc (central records): recompute each segment storing its stage states,
+// then reverse it stage by stage.
+static ncg_status adjoint_backward(ncg_ctx *ctx, const DBody *ck, const double *ckc, int32_t n,
+                                   double G, double soft, double h, int32_t nsteps, int K,
+                                   const double *d_gfinal, const double *d_gcentral_final,
+                                   double *d_g0, double *d_gm, double *d_gcentral0, hipStream_t st)
 {
-    if (!ctx) return NCG_ERR_INVALID;
-    NCG_REQUIRE(n >= 1 && d_bodies0 && d_central0 && nsteps >= 0 && checkpoint_every >= 1 &&
-                    d_gfinal && d_g0 && d_gm && d_gcentral0,
-                "invalid arguments");
-    NCG_HIP(hipSetDevice(ctx->device));
-    hipStream_t st = pick_stream(ctx, stream);
-    const double G = grav_const, soft = softening, h = dt;
-    const int K = checkpoint_every;
-    const int nck = nsteps / K + 1;
-    // Checkpoints (bodies + central) every K steps.
-    DBody *ck = dbuf<DBody>(ctx, "a_ck", (size_t)nck * n);
-    double *ckc = dbuf<double>(ctx, "a_ckc", 8 * (size_t)nck);
+    const double dt = h;
     DBody *y = dbuf<DBody>(ctx, "a_y", n);
     double *c = dbuf<double>(ctx, "a_c", 8);
     // Segment store: stage states of K steps.
@@ -377,21 +366,9 @@ extern "C" ncg_status ncg_rk4_adjoint_dev(ncg_ctx *ctx, const ncg_body *d_bodies
     double *Kx = dbuf<double>(ctx, "a_Kx", 3 * (size_t)n), *Kv = dbuf<double>(ctx, "a_Kv", 3 * (size_t)n);
     double *gX = dbuf<double>(ctx, "a_gX", 3 * (size_t)n);
     double *cst = dbuf<double>(ctx, "a_cst", 32);  // cL[6] cK[6] cA[6] gcm[4] lamc[3]
-    if (!ck || !ckc || !y || !c || !Ys || !Ycs || !Lx || !Lv || !Ax || !Av || !Kx || !Kv || !gX || !cst)
+    if (!y || !c || !Ys || !Ycs || !Lx || !Lv || !Ax || !Av || !Kx || !Kv || !gX || !cst)
         return NCG_ERR_NOMEM;
     double *cL = cst, *cK = cst + 6, *cA = cst + 12, *gcm = cst + 18, *lamc = cst + 22;
-    // Forward with checkpoints.
-    NCG_HIP(hipMemcpyAsync(y, d_bodies0, sizeof(DBody) * n, hipMemcpyDeviceToDevice, st));
-    NCG_HIP(hipMemcpyAsync(c, d_central0, sizeof(double) * 8, hipMemcpyDeviceToDevice, st));
-    for (int step = 0; step < nsteps; ++step) {
-        if (step % K == 0) {
-            NCG_HIP(hipMemcpyAsync(ck + (size_t)(step / K) * n, y, sizeof(DBody) * n, hipMemcpyDeviceToDevice, st));
-            NCG_HIP(hipMemcpyAsync(ckc + 8 * (step / K), c, sizeof(double) * 8, hipMemcpyDeviceToDevice, st));
-        }
-        ncg_status r = fwd_step_store(ctx, y, n, c, G, soft, dt, nullptr, nullptr, st);
-        if (r != NCG_OK) return r;
-    }
-    if (d_final) NCG_HIP(hipMemcpyAsync(d_final, y, sizeof(DBody) * n, hipMemcpyDeviceToDevice, st));
     // Final cotangent: d_gfinal is (x, v) per body, 6n, interleaved x,y,z,vx,vy,vz.
     std::vector<double> hg(6 * (size_t)n);
     NCG_HIP(hipMemcpyAsync(hg.data(), d_gfinal, sizeof(double) * 6 * n, hipMemcpyDeviceToHost, st));
@@ -412,7 +389,6 @@ extern "C" ncg_status ncg_rk4_adjoint_dev(ncg_ctx *ctx, const ncg_body *d_bodies
     }
     NCG_HIP(hipMemsetAsync(d_gm, 0, sizeof(double) * n, st));
     NCG_HIP(hipMemsetAsync(gcm, 0, sizeof(double) * 4, st));
-    // Backward over segments.
     for (int seg = (nsteps - 1) / K; seg >= 0 && nsteps > 0; --seg) {
         const int s0 = seg * K, s1 = std::min(nsteps, s0 + K);
         NCG_HIP(hipMemcpyAsync(y, ck + (size_t)seg * n, sizeof(DBody) * n, hipMemcpyDeviceToDevice, st));
@@ -461,4 +437,87 @@ extern "C" ncg_status ncg_rk4_adjoint_dev(ncg_ctx *ctx, const ncg_body *d_bodies
     NCG_HIP(hipMemcpyAsync(d_gcentral0, gc7, sizeof(double) * 7, hipMemcpyHostToDevice, st));
     NCG_HIP(hipStreamSynchronize(st));
     return NCG_OK;
+}
+
+extern "C" ncg_status ncg_rk4_adjoint_dev(ncg_ctx *ctx, const ncg_body *d_bodies0, int32_t n,
+                                          const double *d_central0, double grav_const,
+                                          double softening, double dt, int32_t nsteps,
+                                          int32_t checkpoint_every, const double *d_gfinal,
+                                          const double *d_gcentral_final, double *d_g0,
+                                          double *d_gm, double *d_gcentral0, ncg_body *d_final,
+                                          void *stream)
+{
+    if (!ctx) return NCG_ERR_INVALID;
+    NCG_REQUIRE(n >= 1 && d_bodies0 && d_central0 && nsteps >= 0 && checkpoint_every >= 1 &&
+                    d_gfinal && d_g0 && d_gm && d_gcentral0,
+                "invalid arguments");
+    NCG_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = pick_stream(ctx, stream);
+    const double G = grav_const, soft = softening;
+    const int K = checkpoint_every;
+    const int nck = nsteps / K + 1;
+    // Checkpoints (bodies + central) every K steps.
+    DBody *ck = dbuf<DBody>(ctx, "a_ck", (size_t)nck * n);
+    double *ckc = dbuf<double>(ctx, "a_ckc", 8 * (size_t)nck);
+    DBody *y = dbuf<DBody>(ctx, "a_fy", n);
+    double *c = dbuf<double>(ctx, "a_fc", 8);
+    if (!ck || !ckc || !y || !c) return NCG_ERR_NOMEM;
+    // Forward with checkpoints.
+    NCG_HIP(hipMemcpyAsync(y, d_bodies0, sizeof(DBody) * n, hipMemcpyDeviceToDevice, st));
+    NCG_HIP(hipMemcpyAsync(c, d_central0, sizeof(double) * 8, hipMemcpyDeviceToDevice, st));
+    for (int step = 0; step < nsteps; ++step) {
+        if (step % K == 0) {
+            NCG_HIP(hipMemcpyAsync(ck + (size_t)(step / K) * n, y, sizeof(DBody) * n, hipMemcpyDeviceToDevice, st));
+            NCG_HIP(hipMemcpyAsync(ckc + 8 * (step / K), c, sizeof(double) * 8, hipMemcpyDeviceToDevice, st));
+        }
+        ncg_status r = fwd_step_store(ctx, y, n, c, G, soft, dt, nullptr, nullptr, st);
+        if (r != NCG_OK) return r;
+    }
+    if (d_final) NCG_HIP(hipMemcpyAsync(d_final, y, sizeof(DBody) * n, hipMemcpyDeviceToDevice, st));
+    return adjoint_backward(ctx, ck, ckc, n, G, soft, dt, nsteps, K, d_gfinal, d_gcentral_final,
+                            d_g0, d_gm, d_gcentral0, st);
+}
+
+extern "C" ncg_status ncg_rk4_forward_ckpt_dev(ncg_ctx *ctx, ncg_body *d_bodies, int32_t n,
+                                               double *d_central, double grav_const,
+                                               double softening, double dt, int32_t nsteps,
+                                               int32_t checkpoint_every, ncg_body *d_ckpt,
+                                               double *d_ckpt_central, void *stream)
+{
+    if (!ctx) return NCG_ERR_INVALID;
+    NCG_REQUIRE(n >= 1 && d_bodies && d_central && nsteps >= 0 && checkpoint_every >= 1 &&
+                    d_ckpt && d_ckpt_central,
+                "invalid arguments");
+    NCG_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = pick_stream(ctx, stream);
+    const int K = checkpoint_every;
+    for (int32_t s0 = 0; s0 < nsteps; s0 += K) {
+        const int q = s0 / K;
+        NCG_HIP(hipMemcpyAsync(d_ckpt + (size_t)q * n, d_bodies, sizeof(DBody) * n,
+                               hipMemcpyDeviceToDevice, st));
+        NCG_HIP(hipMemcpyAsync(d_ckpt_central + 8 * (size_t)q, d_central, sizeof(double) * 8,
+                               hipMemcpyDeviceToDevice, st));
+        // FP64-fast steps, replayed from the RK4 driver's graph.
+        ncg_status r = ncg_rk4_steps_dev(ctx, d_bodies, n, nullptr, d_central, grav_const, softening,
+                                         NCG_F64_FAST, dt, std::min(K, nsteps - s0), 1, st);
+        if (r != NCG_OK) return r;
+    }
+    return NCG_OK;
+}
+
+extern "C" ncg_status ncg_rk4_adjoint_ckpt_dev(ncg_ctx *ctx, const ncg_body *d_ckpt, int32_t n,
+                                               const double *d_ckpt_central, double grav_const,
+                                               double softening, double dt, int32_t nsteps,
+                                               int32_t checkpoint_every, const double *d_gfinal,
+                                               const double *d_gcentral_final, double *d_g0,
+                                               double *d_gm, double *d_gcentral0, void *stream)
+{
+    if (!ctx) return NCG_ERR_INVALID;
+    NCG_REQUIRE(n >= 1 && d_ckpt && d_ckpt_central && nsteps >= 0 && checkpoint_every >= 1 &&
+                    d_gfinal && d_g0 && d_gm && d_gcentral0,
+                "invalid arguments");
+    NCG_HIP(hipSetDevice(ctx->device));
+    return adjoint_backward(ctx, reinterpret_cast<const DBody *>(d_ckpt), d_ckpt_central, n,
+                            grav_const, softening, dt, nsteps, checkpoint_every, d_gfinal,
+                            d_gcentral_final, d_g0, d_gm, d_gcentral0, pick_stream(ctx, stream));
 }

@@ -282,11 +282,7 @@ __device__ inline void far_term(const float4 u, const float *wx, const float *wy
 
 // MODE 0: FP64-fast on every tile; MODE 1: mixed (near FP64, far FP32).
 // SRC 0: tile data read with wave-uniform addresses (scalar loads);
-// SRC 1: tile data staged in LDS by the workgroup, read as broadcasts;
-// SRC 2: as 0, and full far tiles run in batches of 8 bodies whose scalar
-//        loads are issued one batch ahead, with two accumulator sets (even
-//        and odd j) so consecutive interactions do not wait on each other;
-// SRC 3: as 2 without the look-ahead loads.
+// SRC 1: tile data staged in LDS by the workgroup, read as broadcasts.
 template <int MODE, int IPL, int SRC>
 __global__ __launch_bounds__(NT) void k_pairs(const double4 *__restrict__ jd,
                                               const double *__restrict__ jsig,
@@ -382,41 +378,7 @@ __global__ __launch_bounds__(NT) void k_pairs(const double4 *__restrict__ jd,
                 wz[q] = cz - uz[q];
                 fx[q] = fy[q] = fz[q] = 0.f;
             }
-            if (SRC >= 2 && cnt == TJ) {
-                float gx[IPL], gy[IPL], gz[IPL];
-#pragma unroll
-                for (int q = 0; q < IPL; ++q) gx[q] = gy[q] = gz[q] = 0.f;
-                float4 cur[8];
-#pragma unroll
-                for (int k = 0; k < 8; ++k) cur[k] = ju[jb + k];
-                for (int jj = 0; jj < TJ; jj += 8) {
-                    float4 nxt[8];
-                    if (SRC == 2) {
-                        // The last batch re-reads itself: in bounds, unused.
-                        const int nb = jb + min(jj + 8, TJ - 8);
-#pragma unroll
-                        for (int k = 0; k < 8; ++k) nxt[k] = ju[nb + k];
-                    }
-#pragma unroll
-                    for (int k = 0; k < 8; k += 2) {
-                        far_term<IPL>(cur[k], wx, wy, wz, fx, fy, fz);
-                        far_term<IPL>(cur[k + 1], wx, wy, wz, gx, gy, gz);
-                    }
-                    if (SRC == 2) {
-#pragma unroll
-                        for (int k = 0; k < 8; ++k) cur[k] = nxt[k];
-                    } else if (jj + 8 < TJ) {
-#pragma unroll
-                        for (int k = 0; k < 8; ++k) cur[k] = ju[jb + jj + 8 + k];
-                    }
-                }
-#pragma unroll
-                for (int q = 0; q < IPL; ++q) {
-                    fx[q] += gx[q];
-                    fy[q] += gy[q];
-                    fz[q] += gz[q];
-                }
-            } else if (cnt == TJ) {
+            if (cnt == TJ) {
 #pragma unroll 8
                 for (int jj = 0; jj < TJ; ++jj)
                     far_term<IPL>(LDS ? lu[jj] : ju[jb + jj], wx, wy, wz, fx, fy, fz);
@@ -535,9 +497,7 @@ extern "C" ncg_status ncg_set_tiling(ncg_ctx *ctx, int32_t ipl, int32_t nsplit, 
     if (!ctx) return NCG_ERR_INVALID;
     NCG_REQUIRE(ipl == 0 || ipl == 1 || ipl == 2 || ipl == 4, "ipl must be 1, 2 or 4 (0 keeps)");
     NCG_REQUIRE(nsplit >= -1, "nsplit must be >= 0 (0 = automatic) or -1 (keep)");
-    NCG_REQUIRE(source >= -1 && source <= 3,
-                "source must be 0 (scalar), 1 (LDS), 2 (scalar, look-ahead), 3 (scalar, paired) "
-                "or -1 (keep)");
+    NCG_REQUIRE(source >= -1 && source <= 1, "source must be 0 (scalar), 1 (LDS) or -1 (keep)");
     TiledCfg &c = ctx->tiled;
     ctx->tiled_explicit = true;
     if (ipl > 0) c.ipl = ipl;
@@ -560,12 +520,6 @@ static void launch_pairs(int src, dim3 grid, hipStream_t st, const double4 *jd,
 {
     if (src == 1)
         hipLaunchKernelGGL((k_pairs<MODE, IPL, 1>), grid, dim3(NT), 0, st, jd, jsig, ju, th, ih, sc,
-                           nsplit, theta, soft, n, part);
-    else if (src == 2)
-        hipLaunchKernelGGL((k_pairs<MODE, IPL, 2>), grid, dim3(NT), 0, st, jd, jsig, ju, th, ih, sc,
-                           nsplit, theta, soft, n, part);
-    else if (src == 3)
-        hipLaunchKernelGGL((k_pairs<MODE, IPL, 3>), grid, dim3(NT), 0, st, jd, jsig, ju, th, ih, sc,
                            nsplit, theta, soft, n, part);
     else
         hipLaunchKernelGGL((k_pairs<MODE, IPL, 0>), grid, dim3(NT), 0, st, jd, jsig, ju, th, ih, sc,

@@ -23,6 +23,7 @@
 
 #include <algorithm>
 #include <map>
+#include <queue>
 #include <set>
 
 #include "ncg_internal.hpp"
@@ -587,16 +588,19 @@ ncg_status load_cells(ncg_ctx *ctx, HostCells &hc, int32_t n, int ins, const uns
                       const unsigned *dskey, const int *dsid)
 {
     if (hc.loaded) return NCG_OK;
-    hc.key.resize(n);
-    hc.skey.resize(ins);
-    hc.sid.resize(ins);
     hipStream_t st = ctx->stream;
-    NCG_HIP(hipMemcpyAsync(hc.key.data(), dkey, sizeof(unsigned) * n, hipMemcpyDeviceToHost, st));
+    // Through pinned staging (one copy of each array, then host copies).
+    unsigned *pk = pbuf<unsigned>(ctx, "m_hkey", (size_t)n + 2 * (size_t)ins);
+    if (!pk) return NCG_ERR_NOMEM;
+    NCG_HIP(hipMemcpyAsync(pk, dkey, sizeof(unsigned) * n, hipMemcpyDeviceToHost, st));
     if (ins) {
-        NCG_HIP(hipMemcpyAsync(hc.skey.data(), dskey, sizeof(unsigned) * ins, hipMemcpyDeviceToHost, st));
-        NCG_HIP(hipMemcpyAsync(hc.sid.data(), dsid, sizeof(int) * ins, hipMemcpyDeviceToHost, st));
+        NCG_HIP(hipMemcpyAsync(pk + n, dskey, sizeof(unsigned) * ins, hipMemcpyDeviceToHost, st));
+        NCG_HIP(hipMemcpyAsync(pk + n + ins, dsid, sizeof(int) * ins, hipMemcpyDeviceToHost, st));
     }
     NCG_HIP(hipStreamSynchronize(st));
+    hc.key.assign(pk, pk + n);
+    hc.skey.assign(pk + n, pk + n + ins);
+    hc.sid.assign(reinterpret_cast<const int *>(pk + n + ins), reinterpret_cast<const int *>(pk + n + 2 * (size_t)ins));
     hc.loaded = true;
     return NCG_OK;
 }
@@ -762,8 +766,13 @@ extern "C" ncg_status ncg_mesh_pass(ncg_ctx *ctx, const ncg_body *bodies, int32_
     HostCells hc;
     hc.d = d;
     std::vector<char> dirty((size_t)n, 0);
-    std::map<int, std::vector<int>> pending;  // visit k -> modified neighbours p < k
+    // Re-tests owed to later visits: (visit k, modified neighbour p < k), a
+    // min-heap on k (entries are only ever added for visits still ahead).
+    std::priority_queue<std::pair<int, int>, std::vector<std::pair<int, int>>,
+                        std::greater<std::pair<int, int>>>
+        pending;
     std::set<int> full_visit;                 // small bodies modified before their visit
+    std::vector<int> owed;                    // the current visit's entries of `pending`
     int64_t ncoll = 0, htests = 0;
     ncg_status err = NCG_OK;
     std::vector<int> mem;
@@ -777,7 +786,7 @@ extern "C" ncg_status ncg_mesh_pass(ncg_ctx *ctx, const ncg_body *bodies, int32_
         for (int l = 0; l < 27; ++l) {
             hc.members((unsigned)((long)kb + stencil_offset(l, d)), mem);
             for (int q : mem)
-                if (q > kv && q > b) pending[q].push_back(b);
+                if (q > kv && q > b) pending.push({q, b});
         }
         return NCG_OK;
     };
@@ -820,7 +829,7 @@ extern "C" ncg_status ncg_mesh_pass(ncg_ctx *ctx, const ncg_body *bodies, int32_
     while (true) {
         const int kc = ci < cand.size() ? cand[ci].x : INT32_MAX;
         const int kb = bi < big.size() ? big[bi] : INT32_MAX;
-        const int kp = pending.empty() ? INT32_MAX : pending.begin()->first;
+        const int kp = pending.empty() ? INT32_MAX : pending.top().first;
         const int kf = full_visit.empty() ? INT32_MAX : *full_visit.begin();
         const int k = std::min(std::min(kc, kb), std::min(kp, kf));
         if (k == INT32_MAX) break;
@@ -863,11 +872,14 @@ extern "C" ncg_status ncg_mesh_pass(ncg_ctx *ctx, const ncg_body *bodies, int32_
             continue;
         }
         // Small body k.
-        auto it = pending.find(k);
+        owed.clear();
+        while (!pending.empty() && pending.top().first == k) {
+            owed.push_back(pending.top().second);
+            pending.pop();
+        }
         if (full_visit.count(k)) {
             full_visit.erase(k);
             while (ci < cand.size() && cand[ci].x == k) ++ci;
-            if (it != pending.end()) pending.erase(it);
             if (live(k)) {
                 err = full_rest(k, 26, 0);
                 if (err != NCG_OK) return err;
@@ -889,15 +901,14 @@ extern "C" ncg_status ncg_mesh_pass(ncg_ctx *ctx, const ncg_body *bodies, int32_
             if (l >= 0) todo.push_back({(long)(26 - l) * (1L << 31) + p, p});
             ++ci;
         }
-        if (it != pending.end()) {
+        if (!owed.empty()) {
             ncg_status r = load_cells(ctx, hc, n, ins, key, skey, sid);
             if (r != NCG_OK) return r;
-            for (int p : it->second) {
+            for (int p : owed) {
                 if (hc.eff(p) == NOKEY || hc.eff(k) == NOKEY) continue;
                 const int l = hc.stencil_index((long)hc.eff(p) - (long)hc.eff(k));
                 if (l >= 0) todo.push_back({(long)(26 - l) * (1L << 31) + p, p});
             }
-            pending.erase(it);
         }
         std::sort(todo.begin(), todo.end());
         todo.erase(std::unique(todo.begin(), todo.end()), todo.end());

@@ -100,7 +100,7 @@ static void load_tuning(ncg_ctx *ctx)
                  : !strcmp(pname, "ens_fast64") ? 100 + NCG_F64_FAST
                                                 : -1;
         if (r.prec < 0 || (ipl != 1 && ipl != 2 && ipl != 4) || nsplit < 0 || theta <= 0.0 ||
-            src < 0 || src > 3)
+            (src != 0 && src != 1))
             continue;
         r.n_min = nmin;
         r.cfg.ipl = ipl;

@@ -174,3 +174,39 @@ def test_rk4_adjoint_full_size_central_differences(ctx):
     worst = _fd_check(ctx, 16384, 200, 20, 64, 1e-8, 2e-4, 0, 1.0)
     print("worst relative FD difference over 64 directions: %.3e" % worst)
     assert worst < 1e-6, worst
+
+
+def test_checkpointed_forward_and_backward_match_all_in_one(ctx):
+    """ncg_rk4_forward_ckpt_dev + ncg_rk4_adjoint_ckpt_dev give the final
+    state of the RK4 driver and the gradients of ncg_rk4_adjoint_dev."""
+    torch = tt()
+    n, nsteps, K, dt, soft = 700, 23, 5, 2 * math.pi / 256, 0.5
+    b = ic.disk(n, seed=9, mass=1e-7, radius=1e-3)
+    bt = torch.from_numpy(b).cuda()
+    c0 = torch.tensor([0.01, 0, 0, 0, 1e-3, 0, 1.0, 1.0], dtype=torch.float64, device="cuda")
+    ref_b, ref_c = bt.clone(), c0.clone()
+    torch.cuda.synchronize()
+    ctx.rk4_steps_dev(ref_b, ref_c, nsteps, dt, 1.0, soft, "fast64", graph=False)
+    kb, kc = bt.clone(), c0.clone()
+    ck, ckc = ctx.rk4_forward_ckpt_dev(kb, kc, nsteps, dt, K, 1.0, soft)
+    ctx.synchronize()
+    assert ck.shape[0] == (nsteps - 1) // K + 1
+    assert torch.equal(kb, ref_b) and torch.equal(kc, ref_c)
+    assert torch.equal(ck[0], bt) and torch.equal(ckc[0], c0)
+    rng = np.random.default_rng(10)
+    gfin = torch.tensor(rng.normal(size=(n, 6)), device="cuda")
+    gcf = torch.tensor(rng.normal(size=6), device="cuda")
+    outs = []
+    for mode in ("ckpt", "all"):
+        g0 = torch.zeros((n, 6), dtype=torch.float64, device="cuda")
+        gm = torch.zeros(n, dtype=torch.float64, device="cuda")
+        gc = torch.zeros(7, dtype=torch.float64, device="cuda")
+        if mode == "ckpt":
+            ctx.rk4_adjoint_ckpt_dev(ck, ckc, gfin, g0, gm, gc, nsteps, dt, K, 1.0, soft, gcf)
+        else:
+            ctx.rk4_adjoint_dev(bt, c0, gfin, g0, gm, gc, nsteps, dt, K, 1.0, soft, gcf)
+        ctx.synchronize()
+        outs.append((g0, gm, gc))
+    for u, w in zip(*outs):
+        scale = float(u.abs().max()) + 1e-300
+        assert float((u - w).abs().max()) <= 1e-13 * scale
