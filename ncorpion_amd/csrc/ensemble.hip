@@ -16,19 +16,15 @@
 // 64-body sub-tiles added into FP64 accumulators).
 //
 // RK4 (classic, as in rk4.hip) runs every stage of all systems at once; with
-// collisions on, each step starts with a collision pass fused with the
-// stage-1 evaluation, the host replays each system's candidates in the
-// reference's lexicographic order with the built-in outcome, and systems that
-// changed get their stage-1 evaluation again.
+// collisions on, each step starts with a collision pass of every system and a
+// replay kernel (one wavefront per system) that decides the candidates in the
+// reference's lexicographic order with the elastic / inelastic outcome, so a
+// whole step -- pass, replay, four stages -- is one captured graph.
 #include <algorithm>
 #include <cstring>
 #include <numeric>
 
 #include "ncg_internal.hpp"
-
-int64_t ncg_builtin_bf_replay(ncg_body *bodies, int32_t n, const int32_t *exists, double dt,
-                              double f, const std::vector<ncg_pair> &cand,
-                              std::vector<ncg_pair> *log);
 
 namespace {
 
@@ -232,15 +228,271 @@ __global__ __launch_bounds__(ENT) void k_ens_pass(const EnsUnit *__restrict__ un
     }
 }
 
+// ---- device replay of the collision pass -------------------------------------
+// closest_approach (collision.c:63-172) on bodies a < c of one system: hit
+// when the pair approaches (dr.dv < 0), meets (disc > 0) and meets within dt
+// (t_c <= dt); ap receives the positions at closest approach, heavier body
+// first.  Same FP64 operation order as the host restatement (this file is
+// compiled without FMA contraction).
+__device__ bool ens_approach(const DBody *__restrict__ b, int a, int c, double dt, double *ap,
+                             double *tc)
+{
+    const DBody A = b[a], B = b[c];
+    const double dx = A.x - B.x, dy = A.y - B.y, dz = A.z - B.z;
+    const double dvx = A.vx - B.vx, dvy = A.vy - B.vy, dvz = A.vz - B.vz;
+    const double dr_dot_dv = dx * dvx + dy * dvy + dz * dvz;
+    if (dr_dot_dv >= 0.0) return false;
+    const double dr2 = dx * dx + dy * dy + dz * dz;
+    const double dv2 = dvx * dvx + dvy * dvy + dvz * dvz;
+    const double R = A.radius + B.radius;
+    const double disc = dr_dot_dv * dr_dot_dv + dv2 * (R * R - dr2);
+    if (disc <= 0.0) return false;
+    const double t = -(dr_dot_dv + sqrt(disc)) / dv2;
+    *tc = t;
+    if (t > dt) return false;
+    const double xa = A.x + t * A.vx, ya = A.y + t * A.vy, za = A.z + t * A.vz;
+    const double xb = B.x + t * B.vx, yb = B.y + t * B.vy, zb = B.z + t * B.vz;
+    if (A.mass > B.mass) {
+        ap[0] = xa; ap[1] = ya; ap[2] = za; ap[3] = xb; ap[4] = yb; ap[5] = zb;
+    } else {
+        ap[0] = xb; ap[1] = yb; ap[2] = zb; ap[3] = xa; ap[4] = ya; ap[5] = za;
+    }
+    return true;
+}
+
+// The elastic (f = 2) / inelastic bounce of collision (physics.c:545-605)
+// from the approach of the same pair: impulse along the line of centres at
+// contact, then positions rewound to x_c - t_c v'.
+__device__ void ens_bounce(DBody *__restrict__ b, int a, int c, double f, const double *ap,
+                           double t)
+{
+    const int ia = b[a].mass > b[c].mass ? a : c;
+    const int ib = ia == a ? c : a;
+    DBody P = b[ia], Q = b[ib];
+    const double R = P.radius + Q.radius;
+    const double m_a = P.mass, m_b = Q.mass;
+    double vx_a = P.vx, vy_a = P.vy, vz_a = P.vz, vx_b = Q.vx, vy_b = Q.vy, vz_b = Q.vz;
+    const double xa = ap[0], ya = ap[1], za = ap[2], xb = ap[3], yb = ap[4], zb = ap[5];
+    const double dr_dot_dv = (xa - xb) * (vx_a - vx_b) + (ya - yb) * (vy_a - vy_b) + (za - zb) * (vz_a - vz_b);
+    const double alpha = f * m_a * m_b / ((m_a + m_b) * R * R);
+    const double Jx = alpha * dr_dot_dv * (xb - xa);
+    const double Jy = alpha * dr_dot_dv * (yb - ya);
+    const double Jz = alpha * dr_dot_dv * (zb - za);
+    vx_a += Jx / m_a;
+    vy_a += Jy / m_a;
+    vz_a += Jz / m_a;
+    vx_b -= Jx / m_b;
+    vy_b -= Jy / m_b;
+    vz_b -= Jz / m_b;
+    P.x = xa - t * vx_a; P.y = ya - t * vy_a; P.z = za - t * vz_a;
+    P.vx = vx_a; P.vy = vy_a; P.vz = vz_a;
+    Q.x = xb - t * vx_b; Q.y = yb - t * vy_b; Q.z = zb - t * vz_b;
+    Q.vx = vx_b; Q.vy = vy_b; Q.vz = vz_b;
+    b[ia] = P;
+    b[ib] = Q;
+}
+
+constexpr unsigned ENS_CAND_CAP = 1u << 20;  // candidates of one pass, all systems
+constexpr int RCAP = 1024;  // candidates of one system per pass (LDS)
+constexpr int DCAP = 256;   // bodies changed in one system per pass (LDS)
+
+// One wavefront per system replays brute_force's visiting order
+// (collision.c:425-456): rows i ascending, j > i ascending; a pair is tested
+// when the GPU pass listed it or when one of its bodies changed earlier in
+// this pass; a changed row i is scanned in full (64 j per step, the first
+// hit in j order wins, then the scan resumes after it).  err bit 0: too many
+// candidates overall, bit 1: too many for one system, bit 2: too many
+// changed bodies in one system.
+__global__ __launch_bounds__(64) void k_ens_replay(DBody *__restrict__ y0, const int *__restrict__ off,
+                                                   const int2 *__restrict__ cand,
+                                                   const unsigned *__restrict__ ccnt, unsigned cap,
+                                                   double dt, double f,
+                                                   unsigned char *__restrict__ dirty,
+                                                   unsigned *__restrict__ ncoll,
+                                                   unsigned *__restrict__ err)
+{
+    __shared__ int2 lc[RCAP];
+    __shared__ int dl[DCAP];
+    __shared__ int s_nc, s_nd, s_row, s_from, s_hit;
+    __shared__ double s_ap[6], s_tc;
+    const int lane = threadIdx.x;
+    const int sys = blockIdx.x;
+    const int base = off[sys], ns = off[sys + 1] - base;
+    DBody *b = y0 + base;
+    unsigned char *dt8 = dirty + base;
+    const unsigned total = *ccnt;
+    if (lane == 0) {
+        s_nc = 0;
+        s_nd = 0;
+        if (total > cap) atomicOr(err, 1u);
+    }
+    __syncthreads();
+    const unsigned m = min(total, cap);
+    for (unsigned k = lane; k < m; k += 64) {
+        const int2 c = cand[k];
+        if (c.x >= base && c.x < base + ns) {
+            const int slot = atomicAdd(&s_nc, 1);
+            if (slot < RCAP) lc[slot] = make_int2(c.x - base, c.y - base);
+        }
+    }
+    __syncthreads();
+    if (s_nc == 0) return;
+    if (lane == 0) {
+        if (s_nc > RCAP) {
+            atomicOr(err, 2u);
+            s_nc = RCAP;
+        }
+        // (i, j) order; lists are short.
+        for (int a = 1; a < s_nc; ++a) {
+            const int2 v = lc[a];
+            int q = a - 1;
+            while (q >= 0 && (lc[q].x > v.x || (lc[q].x == v.x && lc[q].y > v.y))) {
+                lc[q + 1] = lc[q];
+                --q;
+            }
+            lc[q + 1] = v;
+        }
+    }
+    __syncthreads();
+    unsigned hits = 0;
+    auto mark = [&](int q) {  // lane 0 only
+        if (dt8[q]) return;
+        dt8[q] = 1;
+        if (s_nd >= DCAP) {
+            atomicOr(err, 4u);
+            return;
+        }
+        int p = s_nd++;
+        while (p > 0 && dl[p - 1] > q) {
+            dl[p] = dl[p - 1];
+            --p;
+        }
+        dl[p] = q;
+    };
+    // First candidate of row r (candidates are in (i, j) order).
+    auto first_cand = [&](int r) {
+        int lo = 0, hi = s_nc;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (lc[mid].x < r) lo = mid + 1;
+            else hi = mid;
+        }
+        return lo;
+    };
+    int cur = 0;  // every row below cur is done
+    while (true) {
+        // Rows past the largest changed body only have candidate pairs.
+        if (lane == 0) {
+            const int maxd = s_nd ? dl[s_nd - 1] : -1;
+            int start = cur;
+            if (start > maxd) {
+                const int c = first_cand(start);
+                start = c < s_nc ? lc[c].x : ns;
+            }
+            s_row = start < ns ? start : -1;
+        }
+        __syncthreads();
+        const int start = s_row;
+        if (start < 0) break;
+        // 64 rows at once: does row r have anything to do?  A changed row
+        // is scanned in full; another row only if one of its listed pairs
+        // or of its pairs with a changed body hits.  Rows before the first
+        // such row have no hit and change nothing.
+        const int r = start + lane;
+        bool ev = false;
+        if (r < ns) {
+            if (dt8[r]) {
+                ev = true;
+            } else {
+                double ap[6], tc;
+                for (int q = 0; q < s_nd && !ev; ++q)
+                    if (dl[q] > r) ev = ens_approach(b, r, dl[q], dt, ap, &tc);
+                for (int c = first_cand(r); !ev && c < s_nc && lc[c].x == r; ++c)
+                    ev = ens_approach(b, r, lc[c].y, dt, ap, &tc);
+            }
+        }
+        const unsigned long long emask = __ballot(ev);
+        if (emask == 0ull) {
+            cur = start + 64;
+            __syncthreads();
+            continue;
+        }
+        const int row = start + __ffsll((long long)emask) - 1;
+        // That row in the reference's order, on lane 0 until its first hit.
+        if (lane == 0) {
+            s_from = -1;
+            if (dt8[row]) {
+                s_from = row + 1;  // changed row: scan it in full
+            } else {
+                int ci = first_cand(row);
+                int jc = row + 1;
+                while (jc < ns) {
+                    while (ci < s_nc && lc[ci].x == row && lc[ci].y < jc) ++ci;
+                    const int jcand = (ci < s_nc && lc[ci].x == row) ? lc[ci].y : ns;
+                    int jdirty = ns;
+                    for (int q = 0; q < s_nd; ++q)
+                        if (dl[q] >= jc) {
+                            jdirty = dl[q];
+                            break;
+                        }
+                    const int j = min(jcand, jdirty);
+                    if (j >= ns) break;
+                    double ap[6], tc;
+                    if (ens_approach(b, row, j, dt, ap, &tc)) {
+                        ens_bounce(b, row, j, f, ap, tc);
+                        ++hits;
+                        mark(row);
+                        mark(j);
+                        s_from = j + 1;  // row changed: the rest of it in full
+                        break;
+                    }
+                    jc = j + 1;
+                }
+            }
+        }
+        __threadfence_block();
+        __syncthreads();
+        int jc = s_from;
+        while (jc >= 0 && jc < ns) {
+            const int j = jc + lane;
+            double ap[6], tc;
+            const bool hit = j < ns && ens_approach(b, row, j, dt, ap, &tc);
+            const unsigned long long mask = __ballot(hit);
+            if (mask == 0ull) {
+                jc += 64;
+                continue;
+            }
+            const int first = __ffsll((long long)mask) - 1;
+            if (lane == first) {
+                for (int q = 0; q < 6; ++q) s_ap[q] = ap[q];
+                s_tc = tc;
+                s_hit = j;
+            }
+            __syncthreads();
+            if (lane == 0) {
+                ens_bounce(b, row, s_hit, f, s_ap, s_tc);
+                ++hits;
+                mark(s_hit);
+            }
+            jc = s_hit + 1;
+            __threadfence_block();
+            __syncthreads();
+        }
+        cur = row + 1;
+        __syncthreads();
+    }
+    if (lane == 0) {
+        for (int q = 0; q < s_nd; ++q) dt8[dl[q]] = 0;
+        if (hits) atomicAdd(ncoll, hits);
+    }
+}
+
 // Per-system CM_acc: the partials of the system's gravity units in unit order.
-// With `partial`, systems without units in this pass keep their value.
 __global__ void k_ens_cm(const int *__restrict__ ufirst, const int *__restrict__ ulist,
-                         const double *__restrict__ cpart, int nsys, int partial,
-                         double *__restrict__ cm)
+                         const double *__restrict__ cpart, int nsys, double *__restrict__ cm)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nsys) return;
-    if (partial && ufirst[s] == ufirst[s + 1]) return;
     double a[3] = {0., 0., 0.};
     for (int q = ufirst[s]; q < ufirst[s + 1]; ++q)
         for (int d = 0; d < 3; ++d) a[d] += cpart[3 * (size_t)ulist[q] + d];
@@ -331,8 +583,7 @@ __global__ void k_ens_stage(int s, double dt, int ntot, DBody *__restrict__ y0,
 struct EnsPlan {
     int nsys = 0, ntot = 0, ipl = 1;
     std::vector<int> off;
-    std::vector<EnsUnit> grav, coll, both;  // unit lists (cost-ordered)
-    std::vector<int> ufirst, ulist;        // per system: gravity unit ids (in `grav`/`both` order)
+    std::vector<EnsUnit> grav, coll;  // unit lists (cost-ordered)
 };
 
 static void make_units(EnsPlan &P, int ipl)
@@ -351,9 +602,7 @@ static void make_units(EnsPlan &P, int ipl)
     };
     auto by_cost = [&](const EnsUnit &a, const EnsUnit &b) { return cost(a) > cost(b); };
     std::stable_sort(P.grav.begin(), P.grav.end(), by_cost);
-    P.both = P.grav;
-    P.both.insert(P.both.end(), P.coll.begin(), P.coll.end());
-    std::stable_sort(P.both.begin(), P.both.end(), by_cost);
+    std::stable_sort(P.coll.begin(), P.coll.end(), by_cost);
 }
 
 // Gravity-unit index lists per system for a unit array `units`.
@@ -373,8 +622,8 @@ static void unit_index(const EnsPlan &P, const std::vector<EnsUnit> &units, std:
 }
 
 struct EnsDev {
-    EnsUnit *grav = nullptr, *both = nullptr;
-    int *gfirst = nullptr, *glist = nullptr, *bfirst = nullptr, *blist = nullptr;
+    EnsUnit *grav = nullptr, *coll = nullptr;
+    int *gfirst = nullptr, *glist = nullptr;
     int *off = nullptr;
 };
 
@@ -392,21 +641,20 @@ static void launch_pass(int ipl, int nblk, hipStream_t st, const EnsUnit *units,
                            counter, b, off, cen, G, soft, dt, acc, cpart, cand, cap, ccnt);
 }
 
-// One pass over the unit list du[nunits] (gravity units, optionally with
-// collision units) and the per-system CM_acc from its index lists.
+// One pass over the unit list du[nunits] (gravity and/or collision units);
+// with gravity units, the per-system CM_acc from the index lists.
 static ncg_status ens_pass(ncg_ctx *ctx, const EnsPlan &P, const int *doff, const EnsUnit *du,
-                           int nunits, const int *dfirst, const int *dlist, bool partial,
-                           const DBody *b,
+                           int nunits, const int *dfirst, const int *dlist, const DBody *b,
                            const double *cen, double G, double soft, ncg_precision prec,
                            double dt, double *acc, double *cm, hipStream_t st)
 {
-    if (nunits == 0) return NCG_OK;
     unsigned *counter = dbuf<unsigned>(ctx, "e_counter", 2);
     double *cpart = dbuf<double>(ctx, "e_cpart", 3 * (size_t)std::max(nunits, 1));
-    const unsigned cap = 1u << 20;
+    const unsigned cap = ENS_CAND_CAP;
     int2 *cand = dbuf<int2>(ctx, "e_cand", cap);
     if (!counter || !cpart || !cand) return NCG_ERR_NOMEM;
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, counter, 2);
+    if (nunits == 0) return NCG_OK;
     const int cus = ctx->num_cus > 0 ? ctx->num_cus : 256;
     const int nblk = std::min(nunits, 8 * cus);
     {
@@ -419,20 +667,21 @@ static ncg_status ens_pass(ncg_ctx *ctx, const EnsPlan &P, const int *doff, cons
                            cpart, cand, cap, counter + 1);
         NCG_CHECK_LAUNCH();
     }
-    hipLaunchKernelGGL(k_ens_cm, dim3(cdiv(P.nsys, 128)), dim3(128), 0, st, dfirst, dlist, cpart,
-                       P.nsys, partial ? 1 : 0, cm);
-    NCG_CHECK_LAUNCH();
+    if (dfirst) {
+        hipLaunchKernelGGL(k_ens_cm, dim3(cdiv(P.nsys, 128)), dim3(128), 0, st, dfirst, dlist,
+                           cpart, P.nsys, cm);
+        NCG_CHECK_LAUNCH();
+    }
     return NCG_OK;
 }
 
-// One evaluation of all systems (optionally fused with the collision pass).
+// One evaluation of all systems.
 static ncg_status ens_eval(ncg_ctx *ctx, const EnsPlan &P, const EnsDev &D, const DBody *b,
                            const double *cen, double G, double soft, ncg_precision prec,
-                           double dt, bool with_coll, double *acc, double *cm, hipStream_t st)
+                           double *acc, double *cm, hipStream_t st)
 {
-    const int nunits = (int)(with_coll ? P.both.size() : P.grav.size());
-    return ens_pass(ctx, P, D.off, with_coll ? D.both : D.grav, nunits, with_coll ? D.bfirst : D.gfirst,
-                    with_coll ? D.blist : D.glist, false, b, cen, G, soft, prec, dt, acc, cm, st);
+    return ens_pass(ctx, P, D.off, D.grav, (int)P.grav.size(), D.gfirst, D.glist, b, cen, G, soft,
+                    prec, 0.0, acc, cm, st);
 }
 
 extern "C" ncg_status ncg_ensemble_rk4_dev(ncg_ctx *ctx, ncg_body *d_bodies,
@@ -462,18 +711,15 @@ extern "C" ncg_status ncg_ensemble_rk4_dev(ncg_ctx *ctx, ncg_body *d_bodies,
                                       P.ntot / std::max(nsys, 1));
     P.ipl = tc.ipl == 2 ? 2 : 1;
     make_units(P, P.ipl);
-    std::vector<int> gfirst, glist, bfirst, blist;
+    std::vector<int> gfirst, glist;
     unit_index(P, P.grav, gfirst, glist);
-    unit_index(P, P.both, bfirst, blist);
 
     const int ntot = P.ntot;
     EnsDev D;
     D.grav = dbuf<EnsUnit>(ctx, "e_grav", P.grav.size());
-    D.both = dbuf<EnsUnit>(ctx, "e_both", P.both.size());
+    D.coll = dbuf<EnsUnit>(ctx, "e_coll", P.coll.size());
     D.gfirst = dbuf<int>(ctx, "e_gfirst", nsys + 1);
     D.glist = dbuf<int>(ctx, "e_glist", std::max<size_t>(glist.size(), 1));
-    D.bfirst = dbuf<int>(ctx, "e_bfirst", nsys + 1);
-    D.blist = dbuf<int>(ctx, "e_blist", std::max<size_t>(blist.size(), 1));
     D.off = dbuf<int>(ctx, "e_off", nsys + 1);
     DBody *Y = dbuf<DBody>(ctx, "e_Y", ntot);
     double *acc = dbuf<double>(ctx, "e_acc", 3 * (size_t)ntot);
@@ -482,33 +728,43 @@ extern "C" ncg_status ncg_ensemble_rk4_dev(ncg_ctx *ctx, ncg_body *d_bodies,
     double *Yc = dbuf<double>(ctx, "e_Yc", 8 * (size_t)nsys);
     double *scen = dbuf<double>(ctx, "e_sc", 6 * (size_t)nsys);
     double *cm = dbuf<double>(ctx, "e_cm", 3 * (size_t)nsys);
-    // Touch the pass buffers before any capture.
+    // Every buffer a captured step uses exists before the capture.
     unsigned *counter = dbuf<unsigned>(ctx, "e_counter", 2);
-    double *cpart = dbuf<double>(ctx, "e_cpart", 3 * std::max(P.both.size(), (size_t)1));
-    int2 *cand = dbuf<int2>(ctx, "e_cand", 1u << 20);
-    if (!D.grav || !D.both || !D.gfirst || !D.glist || !D.bfirst || !D.blist || !D.off || !Y ||
-        !acc || !sx || !sv || !Yc || !scen || !cm || !counter || !cpart || !cand)
+    double *cpart = dbuf<double>(ctx, "e_cpart", 3 * std::max(P.grav.size(), P.coll.size()) + 3);
+    int2 *cand = dbuf<int2>(ctx, "e_cand", ENS_CAND_CAP);
+    unsigned char *dirty = dbuf<unsigned char>(ctx, "e_dirty", (size_t)ntot + 4);
+    unsigned *stat = dbuf<unsigned>(ctx, "e_stat", 2);  // collisions, error bits
+    if (!D.grav || !D.coll || !D.gfirst || !D.glist || !D.off || !Y || !acc || !sx || !sv ||
+        !Yc || !scen || !cm || !counter || !cpart || !cand || !dirty || !stat)
         return NCG_ERR_NOMEM;
     NCG_HIP(hipMemcpyAsync(D.grav, P.grav.data(), sizeof(EnsUnit) * P.grav.size(), hipMemcpyHostToDevice, st));
-    NCG_HIP(hipMemcpyAsync(D.both, P.both.data(), sizeof(EnsUnit) * P.both.size(), hipMemcpyHostToDevice, st));
+    NCG_HIP(hipMemcpyAsync(D.coll, P.coll.data(), sizeof(EnsUnit) * P.coll.size(), hipMemcpyHostToDevice, st));
     NCG_HIP(hipMemcpyAsync(D.gfirst, gfirst.data(), sizeof(int) * gfirst.size(), hipMemcpyHostToDevice, st));
     NCG_HIP(hipMemcpyAsync(D.glist, glist.data(), sizeof(int) * glist.size(), hipMemcpyHostToDevice, st));
-    NCG_HIP(hipMemcpyAsync(D.bfirst, bfirst.data(), sizeof(int) * bfirst.size(), hipMemcpyHostToDevice, st));
-    NCG_HIP(hipMemcpyAsync(D.blist, blist.data(), sizeof(int) * blist.size(), hipMemcpyHostToDevice, st));
     NCG_HIP(hipMemcpyAsync(D.off, P.off.data(), sizeof(int) * (nsys + 1), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, stat, 2);
+    hipLaunchKernelGGL(k_zero_u32, dim3(cdiv(ntot / 4 + 1, 256)), dim3(256), 0, st,
+                       reinterpret_cast<unsigned *>(dirty), ntot / 4 + 1);
+    NCG_CHECK_LAUNCH();
     DBody *y0 = reinterpret_cast<DBody *>(d_bodies);
     const unsigned sgrid = cdiv(ntot + nsys, 256);
     const double G = grav_const, soft = softening;
 
-    // Stages s0..3 of one step; stage 0's evaluation is done by the caller
-    // when `skip_eval0` (collision pass fused with it).
-    auto step_body = [&](bool skip_eval0) -> ncg_status {
+    // One step: with collisions, the brute-force pass of every system and its
+    // replay on the device, then the four RK4 stages.
+    auto step_body = [&]() -> ncg_status {
+        if (collide) {
+            ncg_status r = ens_pass(ctx, P, D.off, D.coll, (int)P.coll.size(), nullptr, nullptr, y0,
+                                    d_centrals, G, soft, prec, dt, acc, cm, st);
+            if (r != NCG_OK) return r;
+            hipLaunchKernelGGL(k_ens_replay, dim3(nsys), dim3(64), 0, st, y0, D.off, cand,
+                               counter + 1, ENS_CAND_CAP, dt, collision_f, dirty, stat, stat + 1);
+            NCG_CHECK_LAUNCH();
+        }
         for (int s = 0; s < 4; ++s) {
-            if (!(s == 0 && skip_eval0)) {
-                ncg_status r = ens_eval(ctx, P, D, s == 0 ? y0 : Y, s == 0 ? d_centrals : Yc, G,
-                                        soft, prec, dt, false, acc, cm, st);
-                if (r != NCG_OK) return r;
-            }
+            ncg_status r = ens_eval(ctx, P, D, s == 0 ? y0 : Y, s == 0 ? d_centrals : Yc, G, soft,
+                                    prec, acc, cm, st);
+            if (r != NCG_OK) return r;
             hipLaunchKernelGGL(k_ens_stage, dim3(sgrid), dim3(256), 0, st, s, dt, ntot, y0, Y, acc,
                                sx, sv, nsys, d_centrals, Yc, cm, scen);
             NCG_CHECK_LAUNCH();
@@ -518,118 +774,18 @@ extern "C" ncg_status ncg_ensemble_rk4_dev(ncg_ctx *ctx, ncg_body *d_bodies,
 
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph = nullptr;
-    // Pinned staging for the systems that have candidates (allocated before
-    // any queued work can read it).
-    ncg_body *host = collide ? pbuf<ncg_body>(ctx, "e_host", ntot) : nullptr;
-    if (collide && !host) return NCG_ERR_NOMEM;
-    int64_t ncoll_total = 0;
     for (int32_t step = 0; step < nsteps; ++step) {
-        bool skip0 = false;
-        if (collide) {
-            // Collision candidates fused with the stage-1 evaluation.
-            ncg_status r = ens_eval(ctx, P, D, y0, d_centrals, G, soft, prec, dt, true, acc, cm, st);
-            if (r != NCG_OK) return r;
-            unsigned hc[2];
-            NCG_HIP(hipMemcpyAsync(hc, counter, sizeof hc, hipMemcpyDeviceToHost, st));
-            NCG_HIP(hipStreamSynchronize(st));
-            const unsigned nc = hc[1];
-            if (nc > (1u << 20))
-                return ncg_fail(ctx, NCG_ERR_CAPACITY, "more than 2^20 collision candidates");
-            if (nc > 0) {
-                std::vector<int2> cv(nc);
-                NCG_HIP(hipMemcpyAsync(cv.data(), cand, sizeof(int2) * nc, hipMemcpyDeviceToHost, st));
-                NCG_HIP(hipStreamSynchronize(st));
-                std::sort(cv.begin(), cv.end(), [](const int2 &a, const int2 &b) {
-                    return a.x != b.x ? a.x < b.x : a.y < b.y;
-                });
-                // Download only the systems with candidates (adjacent ones in
-                // one copy).
-                {
-                    int r0 = -1, r1 = -1;  // current body range [r0, r1)
-                    for (size_t q = 0; q < cv.size();) {
-                        const int s = (int)(std::upper_bound(P.off.begin(), P.off.end(), cv[q].x) -
-                                            P.off.begin()) - 1;
-                        const int b0 = P.off[s], b1 = P.off[s + 1];
-                        if (b0 != r1) {
-                            if (r1 > r0)
-                                NCG_HIP(hipMemcpyAsync(host + r0, y0 + r0, sizeof(DBody) * (r1 - r0),
-                                                       hipMemcpyDeviceToHost, st));
-                            r0 = b0;
-                        }
-                        r1 = b1;
-                        while (q < cv.size() && cv[q].x < b1) ++q;
-                    }
-                    if (r1 > r0)
-                        NCG_HIP(hipMemcpyAsync(host + r0, y0 + r0, sizeof(DBody) * (r1 - r0),
-                                               hipMemcpyDeviceToHost, st));
-                    NCG_HIP(hipStreamSynchronize(st));
-                }
-                // Replay system by system (local indices).
-                size_t q = 0;
-                bool changed = false;
-                std::vector<char> chg(nsys, 0);
-                while (q < cv.size()) {
-                    const int g = cv[q].x;
-                    const int s = (int)(std::upper_bound(P.off.begin(), P.off.end(), g) - P.off.begin()) - 1;
-                    const int base = P.off[s], ns = P.off[s + 1] - base;
-                    std::vector<ncg_pair> loc;
-                    while (q < cv.size() && cv[q].x < base + ns) {
-                        loc.push_back({cv[q].x - base, cv[q].y - base});
-                        ++q;
-                    }
-                    const int64_t k = ncg_builtin_bf_replay(host + base, ns, nullptr, dt,
-                                                            collision_f, loc, nullptr);
-                    if (k) {
-                        ncoll_total += k;
-                        changed = true;
-                        chg[s] = 1;
-                        NCG_HIP(hipMemcpyAsync(y0 + base, host + base, sizeof(DBody) * ns,
-                                               hipMemcpyHostToDevice, st));
-                    }
-                }
-                if (changed) {
-                    // Stage-1 evaluation again, for the changed systems only.
-                    std::vector<EnsUnit> sub;
-                    for (const EnsUnit &u : P.grav)
-                        if (chg[u.sys]) sub.push_back(u);
-                    std::vector<int> sfirst, slist;
-                    unit_index(P, sub, sfirst, slist);
-                    const size_t nu = sub.size();
-                    // Pinned staging: units, then first[nsys + 1], then list.
-                    const size_t bytes = sizeof(EnsUnit) * nu + sizeof(int) * (sfirst.size() + slist.size());
-                    unsigned char *hs = pbuf<unsigned char>(ctx, "e_subh", sizeof(EnsUnit) * P.grav.size() +
-                                                                               sizeof(int) * (nsys + 1 + P.grav.size()));
-                    unsigned char *ds = dbuf<unsigned char>(ctx, "e_sub", sizeof(EnsUnit) * P.grav.size() +
-                                                                              sizeof(int) * (nsys + 1 + P.grav.size()));
-                    if (!hs || !ds) return NCG_ERR_NOMEM;
-                    NCG_HIP(hipStreamSynchronize(st));  // the staging buffer is reused each step
-                    memcpy(hs, sub.data(), sizeof(EnsUnit) * nu);
-                    memcpy(hs + sizeof(EnsUnit) * nu, sfirst.data(), sizeof(int) * sfirst.size());
-                    memcpy(hs + sizeof(EnsUnit) * nu + sizeof(int) * sfirst.size(), slist.data(),
-                           sizeof(int) * slist.size());
-                    NCG_HIP(hipMemcpyAsync(ds, hs, bytes, hipMemcpyHostToDevice, st));
-                    const EnsUnit *du = reinterpret_cast<const EnsUnit *>(ds);
-                    const int *df = reinterpret_cast<const int *>(ds + sizeof(EnsUnit) * nu);
-                    const int *dl = df + sfirst.size();
-                    r = ens_pass(ctx, P, D.off, du, (int)nu, df, dl, true, y0, d_centrals, G, soft,
-                                 prec, dt, acc, cm, st);
-                    if (r != NCG_OK) return r;
-                }
-            }
-            skip0 = true;
-        }
-        if (!use_graph) {
-            ncg_status r = step_body(skip0);
+        if (!use_graph || step == 0) {
+            // Eager (the first step also allocates everything the graph uses).
+            ncg_status r = step_body();
             if (r != NCG_OK) return r;
             continue;
         }
         if (!exec) {
-            ncg_status r = step_body(skip0);  // eager first step allocates everything
-            if (r != NCG_OK) return r;
             NCG_HIP(hipStreamSynchronize(st));
             ctx->capturing = true;
             NCG_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-            r = step_body(skip0);
+            ncg_status r = step_body();
             hipError_t e = hipStreamEndCapture(st, &graph);
             ctx->capturing = false;
             if (r != NCG_OK) return r;
@@ -639,16 +795,23 @@ extern "C" ncg_status ncg_ensemble_rk4_dev(ncg_ctx *ctx, ncg_body *d_bodies,
                 (void)hipGraphDestroy(graph);
                 return ncg_fail(ctx, NCG_ERR_DEVICE, "graph instantiation failed");
             }
-            continue;
         }
         NCG_HIP(hipGraphLaunch(exec, st));
     }
+    unsigned hs[2] = {0, 0};
+    NCG_HIP(hipMemcpyAsync(hs, stat, sizeof hs, hipMemcpyDeviceToHost, st));
+    NCG_HIP(hipStreamSynchronize(st));
     if (exec) {
-        NCG_HIP(hipStreamSynchronize(st));
         (void)hipGraphExecDestroy(exec);
         (void)hipGraphDestroy(graph);
     }
-    if (n_collisions) *n_collisions = ncoll_total;
+    if (n_collisions) *n_collisions = hs[0];
+    if (hs[1])
+        return ncg_fail(ctx, NCG_ERR_CAPACITY,
+                        "collision replay capacity exceeded (flags %u: 1 = more than %u candidates "
+                        "in one pass, 2 = more than %d in one system, 4 = more than %d changed "
+                        "bodies in one system); results are not valid",
+                        hs[1], ENS_CAND_CAP, RCAP, DCAP);
     return NCG_OK;
 }
 
@@ -690,5 +853,5 @@ extern "C" ncg_status ncg_ensemble_gravity_dev(ncg_ctx *ctx, const ncg_body *d_b
     NCG_HIP(hipMemcpyAsync(D.glist, glist.data(), sizeof(int) * glist.size(), hipMemcpyHostToDevice, st));
     NCG_HIP(hipMemcpyAsync(D.off, P.off.data(), sizeof(int) * (nsys + 1), hipMemcpyHostToDevice, st));
     return ens_eval(ctx, P, D, reinterpret_cast<const DBody *>(d_bodies), d_centrals, grav_const,
-                    softening, prec, 0.0, false, d_acc, d_cm_acc, st);
+                    softening, prec, d_acc, d_cm_acc, st);
 }

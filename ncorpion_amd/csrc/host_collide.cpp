@@ -1,5 +1,5 @@
 // Built-in host collision decision and outcome for callers that are not
-// NcorpiON itself (the ensemble driver, Python users).  It restates, for one
+// NcorpiON itself (Python users, tests).  It restates, for one
 // body array, closest_approach (collision.c:63-172) and collision_treatment
 // with elastic (f = 2) or inelastic (f = collision_parameter) outcomes
 // (physics.c:545-605, 1043-1070), in the same FP64 operation order.  The
@@ -85,60 +85,4 @@ extern "C" ncg_status ncg_builtin_replay_ops(ncg_builtin_collider *state, ncg_re
     ops->test = builtin_test;
     ops->resolve = builtin_resolve;
     return NCG_OK;
-}
-
-// Replays one brute-force collision pass over `bodies` (host) given the sorted
-// candidate list, with the built-in decision and outcome.  Same visiting rules
-// as ncg_brute_force_pass; returns the number of collisions.
-int64_t ncg_builtin_bf_replay(ncg_body *bodies, int32_t n, const int32_t *exists, double dt,
-                              double f, const std::vector<ncg_pair> &cand,
-                              std::vector<ncg_pair> *log)
-{
-    Builtin u{};
-    u.bodies = bodies;
-    u.exists = exists;
-    u.timestep = dt;
-    u.collision_f = f;
-    std::vector<char> dirty((size_t)n, 0);
-    std::vector<int32_t> dlist;  // dirty bodies, kept sorted
-    int64_t ncoll = 0;
-    size_t ci = 0;
-    auto live = [&](int32_t k) { return !exists || exists[k] != 0; };
-    auto mark = [&](int32_t k) {
-        if (dirty[k]) return;
-        dirty[k] = 1;
-        dlist.insert(std::upper_bound(dlist.begin(), dlist.end(), k), k);
-    };
-    for (int32_t i = 0; i < n; ++i) {
-        while (ci < cand.size() && cand[ci].fst < i) ++ci;
-        if (!live(i)) continue;
-        int32_t jc = i + 1;
-        while (jc < n) {
-            if (dirty[i]) {
-                for (int32_t j = jc; j < n; ++j)
-                    if (live(j) && builtin_test(&u, i, j)) {
-                        builtin_resolve(&u, i, j);
-                        ++ncoll;
-                        mark(j);
-                        if (log) log->push_back({i, j});
-                    }
-                break;
-            }
-            while (ci < cand.size() && cand[ci].fst == i && cand[ci].snd < jc) ++ci;
-            const int32_t j_cand = (ci < cand.size() && cand[ci].fst == i) ? cand[ci].snd : n;
-            auto it = std::lower_bound(dlist.begin(), dlist.end(), jc);
-            const int32_t j_dirty = it == dlist.end() ? n : *it;
-            const int32_t j = std::min(j_cand, j_dirty);
-            if (j >= n) break;
-            if (live(j) && builtin_test(&u, i, j)) {
-                builtin_resolve(&u, i, j);
-                ++ncoll;
-                mark(i);
-                mark(j);
-                if (log) log->push_back({i, j});
-            }
-            jc = j + 1;
-        }
-    }
-    return ncoll;
 }

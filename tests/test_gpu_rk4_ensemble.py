@@ -158,7 +158,7 @@ def test_ensemble_collisions_match_oracle(ctx):
     cen = np.zeros((S, 8))
     cen[:, 6] = 1.0
     cen[:, 7] = 1.0
-    dt, steps = 0.002, 2
+    dt, steps = 0.002, 4
     bt = torch.from_numpy(allb).cuda()
     ct = torch.from_numpy(cen).cuda()
     torch.cuda.synchronize()
@@ -176,3 +176,37 @@ def test_ensemble_collisions_match_oracle(ctx):
         got = out[off[s]:off[s + 1]]
         assert np.abs(got[:, :6] - ref[:, :6]).max() < 1e-9
     assert nc == total and total > 20
+
+
+def test_ensemble_collision_steps_graph_equals_eager(ctx):
+    """The captured step (collision pass, device replay, four stages) replays
+    bit for bit like eager launches."""
+    torch = torch_dev()
+    rng = np.random.default_rng(12)
+    sizes = rng.integers(100, 700, 9)
+    systems = []
+    for s, nn in enumerate(sizes):
+        b = np.zeros((nn, 8))
+        r2 = np.random.default_rng(300 + s)
+        b[:, 0:3] = r2.uniform(-0.04, 0.04, (nn, 3)) + np.array([0, 1.0, 0])
+        b[:, 3:6] = r2.normal(0, 0.3, (nn, 3)) + np.array([-1.0, 0, 0])
+        b[:, 6] = r2.uniform(1e-12, 1e-11, nn)
+        b[:, 7] = r2.uniform(1e-3, 2e-3, nn)
+        systems.append(b)
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)
+    allb = np.concatenate(systems)
+    cen = np.zeros((len(sizes), 8))
+    cen[:, 6] = 1.0
+    cen[:, 7] = 1.0
+    res = []
+    for graph in (False, True):
+        bt = torch.from_numpy(allb).cuda()
+        ct = torch.from_numpy(cen).cuda()
+        torch.cuda.synchronize()
+        nc = ctx.ensemble_rk4_dev(bt, off, ct, 6, 0.002, 1.0, 0.0, "mixed", graph=graph,
+                                  collisions=True, collision_f=1.5)
+        ctx.synchronize()
+        res.append((nc, bt.cpu().numpy(), ct.cpu().numpy()))
+    assert res[0][0] == res[1][0] > 50
+    assert np.array_equal(res[0][1].view(np.uint64), res[1][1].view(np.uint64))
+    assert np.array_equal(res[0][2].view(np.uint64), res[1][2].view(np.uint64))
