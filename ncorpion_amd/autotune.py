@@ -1,15 +1,23 @@
-"""Autotuner of the tiled gravity launch shapes.
+"""Autotuner of the gravity launch shapes.
 
-For each body count N (a seeded disk of that size), every candidate shape --
-bodies per lane, j-splits per i-block, tile source (scalar loads or LDS) and,
-for the mixed form, the near/far threshold theta -- is timed with HIP events
-around the pair kernel.  A theta is admissible only if the mixed form's max
-relative error on the mutual acceleration, against the FP64 form, stays below
-`max_err`.  The fastest admissible shape per (form, N) is written to the tuning
-table that libncg.so reads at context creation (ncg_tuning.txt beside it).
+Single systems: for each body count N (a seeded disk of that size), every
+candidate shape -- bodies per lane, j-splits per i-block, tile source (scalar
+loads or LDS) and, for the mixed form, the near/far threshold theta -- is
+timed with HIP events around the pair kernel.  A theta is admissible only if
+the mixed form's max relative error on the mutual acceleration, against the
+FP64 form, stays below `max_err`.
+
+Ensembles: on 512 seeded disks of log-uniform N in [256, 8192], bodies per
+lane and persistent workgroups per CU of the ensemble pass are timed the same
+way.
+
+The fastest admissible shape per (form, N) is written to the tuning table
+that libncg.so reads at context creation (ncg_tuning.txt beside it); rows of
+forms that were not tuned are kept.
 
     python -m ncorpion_amd.autotune --n 4096 16384 65536 262144 \
         --out ncorpion_amd/ncg_tuning.txt
+    python -m ncorpion_amd.autotune --forms ensemble
 """
 import argparse
 import itertools
@@ -40,7 +48,7 @@ def _time(ctx, bt, acc, prec, reps, soft):
     return ms / max(cnt, 1)
 
 
-def tune(ctx, n, reps=3, max_err=5e-7, soft=1e-6, thetas=(0.25, 0.35, 0.5), log=print):
+def tune(ctx, n, reps=3, max_err=5e-7, soft=1e-6, thetas=(0.15, 0.2, 0.25, 0.35), log=print):
     """Returns {form: (ipl, nsplit, theta, src, ms)} for N = n."""
     import torch
     b = _workload(n)
@@ -76,12 +84,65 @@ def tune(ctx, n, reps=3, max_err=5e-7, soft=1e-6, thetas=(0.25, 0.35, 0.5), log=
     return out
 
 
+def tune_ensemble(ctx, reps=3, log=print):
+    """Returns {form: (ipl, workgroups_per_cu, theta, src, ms)} for the 512-disk
+    ensemble; the table row applies to every ensemble (n_min 1)."""
+    import numpy as np
+    import torch
+    from . import ic
+    sizes = ic.seeded_ensemble_sizes(512)
+    systems = [ic.disk(int(nn), seed=s, mass=1e-8, radius=2e-4) for s, nn in enumerate(sizes)]
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)
+    bt = torch.from_numpy(np.concatenate(systems)).cuda()
+    cen = np.zeros((len(sizes), 8))
+    cen[:, 6] = 1.0
+    cen[:, 7] = 1.0
+    ct = torch.from_numpy(cen).cuda()
+    acc = torch.zeros((bt.shape[0], 3), dtype=torch.float64, device="cuda")
+    cm = torch.zeros((len(sizes), 3), dtype=torch.float64, device="cuda")
+    out = {}
+    for prec in ("mixed", "fast64"):
+        best = None
+        for ipl, wg in itertools.product((1, 2), (2, 4, 8, 12, 16)):
+            ctx.set_tiling(ipl, wg, 0.35, 0)
+            ctx.ensemble_gravity_dev(bt, off, ct, acc, cm, 1.0, 0.0, prec)
+            ctx.synchronize()
+            ctx.profile(True)
+            ctx.profile_read("ens_" + prec)
+            for _ in range(reps):
+                ctx.ensemble_gravity_dev(bt, off, ct, acc, cm, 1.0, 0.0, prec)
+            ctx.synchronize()
+            ms, cnt = ctx.profile_read("ens_" + prec)
+            ctx.profile(False)
+            ms /= max(cnt, 1)
+            log("  ensemble %s ipl=%d wg/CU=%d %.3f ms" % (prec, ipl, wg, ms))
+            if best is None or ms < best[4]:
+                best = (ipl, wg, 0.35, 0, ms)  # theta column unused (must be > 0)
+        out[prec] = best
+    return out
+
+
+def read_table(path):
+    """Rows (form, n_min, (ipl, nsplit, theta, src, ms)) of an existing table."""
+    rows = []
+    if not os.path.exists(path):
+        return rows
+    for line in open(path):
+        if line.startswith("#") or not line.strip():
+            continue
+        f = line.split("#")[0].split()
+        ms = float(line.split("#")[1].split()[0]) if "#" in line else 0.0
+        rows.append((f[0], int(f[1]), (int(f[2]), int(f[3]), float(f[4]), int(f[5]), ms)))
+    return rows
+
+
 def write_table(rows, path):
     with open(path, "w") as f:
         f.write("# Launch shapes of the tiled gravity forms, read by libncg.so at ncg_create.\n")
         f.write("# Written by ncorpion_amd/autotune.py on an MI355X; one row per (form, N bucket):\n")
         f.write("# the row with the largest n_min <= N applies.\n")
         f.write("# form   n_min   ipl  nsplit  theta  src   (measured ms per evaluation at n_min)\n")
+        f.write("# (ens_* forms: nsplit is persistent workgroups per CU; theta, src unused)\n")
         for prec, nmin, (ipl, ns, theta, src, ms) in rows:
             f.write("%-7s  %-7d %-4d %-7d %-6.2f %-4d  # %.3f ms\n" % (prec, nmin, ipl, ns, theta, src, ms))
 
@@ -92,20 +153,33 @@ def main(argv=None):
     ap.add_argument("--out", default=DEFAULT_TABLE)
     ap.add_argument("--max-err", type=float, default=5e-7)
     ap.add_argument("--json", default=None)
+    ap.add_argument("--forms", nargs="+", default=["single", "ensemble"],
+                    choices=["single", "ensemble"])
     args = ap.parse_args(argv)
     from . import Context
     ctx = Context(0)
+    log = lambda s: print(s, flush=True)  # noqa: E731
+    tuned = set()
     rows, res = [], {}
-    ns = sorted(args.n)
-    for i, n in enumerate(ns):
-        r = tune(ctx, n, max_err=args.max_err, log=lambda s: print(s, flush=True))
-        res[n] = r
-        # A row applies from the geometric midpoint to the previous size.
-        nmin = 1 if i == 0 else int((ns[i - 1] * n) ** 0.5)
+    if "single" in args.forms:
+        tuned |= {"mixed", "fast64"}
+        ns = sorted(args.n)
+        for i, n in enumerate(ns):
+            r = tune(ctx, n, max_err=args.max_err, log=log)
+            res[n] = r
+            # A row applies from the geometric midpoint to the previous size.
+            nmin = 1 if i == 0 else int((ns[i - 1] * n) ** 0.5)
+            for prec in ("mixed", "fast64"):
+                if r[prec] is not None:
+                    rows.append((prec, nmin, r[prec]))
+    if "ensemble" in args.forms:
+        tuned |= {"ens_mixed", "ens_fast64"}
+        r = tune_ensemble(ctx, log=log)
+        res["ensemble"] = r
         for prec in ("mixed", "fast64"):
-            if r[prec] is not None:
-                rows.append((prec, nmin, r[prec]))
-    write_table(rows, args.out)
+            rows.append(("ens_" + prec, 1, r[prec]))
+    kept = [row for row in read_table(args.out) if row[0] not in tuned]
+    write_table(kept + rows, args.out)
     if args.json:
         with open(args.json, "w") as f:
             json.dump({str(k): v for k, v in res.items()}, f)

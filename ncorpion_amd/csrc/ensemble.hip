@@ -54,6 +54,50 @@ __global__ void k_zero_u32(unsigned *p, int n)
     if (i < n) p[i] = 0u;
 }
 
+// One FP32 interaction of the mixed form: separation rounded from FP64,
+// softening added only when SOFT.
+template <bool SOFT, int IPL>
+__device__ inline void ens_mixed_term(const double4 p, const float2 g, const double *xi,
+                                      const double *yi, const double *zi, const float *sif,
+                                      float *fx, float *fy, float *fz)
+{
+#pragma unroll
+    for (int q = 0; q < IPL; ++q) {
+        const float dx = (float)(p.x - xi[q]);
+        const float dy = (float)(p.y - yi[q]);
+        const float dz = (float)(p.z - zi[q]);
+        float r2;
+        if (SOFT) {
+            const float e = sif[q] + g.y;
+            r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, e * e)));
+        } else {
+            r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+        }
+        r2 = fmaxf(r2, 1e-20f);  // a body against itself adds 0 * finite
+        const float y = __builtin_amdgcn_rsqf(r2);
+        const float s3 = (g.x * y) * (y * y);
+        fx[q] = fmaf(s3, dx, fx[q]);
+        fy[q] = fmaf(s3, dy, fy[q]);
+        fz[q] = fmaf(s3, dz, fz[q]);
+    }
+}
+
+// Bodies j0 .. j1-1 (at most 64) of the staged tile into FP32 partials.
+template <bool SOFT, int IPL>
+__device__ inline void ens_mixed_run(const double4 *tp, const float2 *tf, int j0, int j1,
+                                     const double *xi, const double *yi, const double *zi,
+                                     const float *sif, float *fx, float *fy, float *fz)
+{
+    if (j1 - j0 == 64) {
+#pragma unroll 4
+        for (int jj = j0; jj < j0 + 64; ++jj)
+            ens_mixed_term<SOFT, IPL>(tp[jj], tf[jj], xi, yi, zi, sif, fx, fy, fz);
+    } else {
+        for (int jj = j0; jj < j1; ++jj)
+            ens_mixed_term<SOFT, IPL>(tp[jj], tf[jj], xi, yi, zi, sif, fx, fy, fz);
+    }
+}
+
 template <int MODE, int IPL>
 __global__ __launch_bounds__(ENT) void k_ens_pass(const EnsUnit *__restrict__ units, int nunits,
                                                   unsigned *__restrict__ counter,
@@ -68,6 +112,7 @@ __global__ __launch_bounds__(ENT) void k_ens_pass(const EnsUnit *__restrict__ un
 {
     __shared__ double4 tp[ETJ];  // x, y, z, G m
     __shared__ double ts[ETJ];   // s R
+    __shared__ float2 tf[ETJ];   // G m, s R in FP32 (mixed form)
     __shared__ double4 tv[ETJ];  // vx, vy, vz, R (collision units)
     __shared__ double red[3][ENT];
     while (true) {
@@ -96,6 +141,7 @@ __global__ __launch_bounds__(ENT) void k_ens_pass(const EnsUnit *__restrict__ un
                     const DBody q = b[base + jb + threadIdx.x];
                     tp[threadIdx.x] = make_double4(q.x, q.y, q.z, G * q.mass);
                     ts[threadIdx.x] = soft * q.radius;
+                    tf[threadIdx.x] = make_float2((float)(G * q.mass), (float)(soft * q.radius));
                 }
                 __syncthreads();
                 if (MODE == 0) {
@@ -117,30 +163,18 @@ __global__ __launch_bounds__(ENT) void k_ens_pass(const EnsUnit *__restrict__ un
                         }
                     }
                 } else {
+                    float sif[IPL];
+#pragma unroll
+                    for (int q = 0; q < IPL; ++q) sif[q] = (float)si[q];
                     for (int j0 = 0; j0 < cnt; j0 += 64) {
                         const int j1 = min(cnt, j0 + 64);
                         float fx[IPL], fy[IPL], fz[IPL];
 #pragma unroll
                         for (int q = 0; q < IPL; ++q) fx[q] = fy[q] = fz[q] = 0.f;
-                        for (int jj = j0; jj < j1; ++jj) {
-                            const double4 p = tp[jj];
-                            const float sj = (float)ts[jj];
-                            const float gm = (float)p.w;
-#pragma unroll
-                            for (int q = 0; q < IPL; ++q) {
-                                const float dx = (float)(p.x - xi[q]);
-                                const float dy = (float)(p.y - yi[q]);
-                                const float dz = (float)(p.z - zi[q]);
-                                const float e = (float)si[q] + sj;
-                                float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, e * e)));
-                                r2 = fmaxf(r2, 1e-20f);
-                                const float y = __builtin_amdgcn_rsqf(r2);
-                                const float s3 = (gm * y) * (y * y);
-                                fx[q] = fmaf(s3, dx, fx[q]);
-                                fy[q] = fmaf(s3, dy, fy[q]);
-                                fz[q] = fmaf(s3, dz, fz[q]);
-                            }
-                        }
+                        if (soft != 0.0)
+                            ens_mixed_run<true, IPL>(tp, tf, j0, j1, xi, yi, zi, sif, fx, fy, fz);
+                        else
+                            ens_mixed_run<false, IPL>(tp, tf, j0, j1, xi, yi, zi, sif, fx, fy, fz);
 #pragma unroll
                         for (int q = 0; q < IPL; ++q) {
                             ax[q] += (double)fx[q];
@@ -582,6 +616,7 @@ __global__ void k_ens_stage(int s, double dt, int ntot, DBody *__restrict__ y0,
 
 struct EnsPlan {
     int nsys = 0, ntot = 0, ipl = 1;
+    int wg_per_cu = 8;  // persistent workgroups per CU
     std::vector<int> off;
     std::vector<EnsUnit> grav, coll;  // unit lists (cost-ordered)
 };
@@ -656,7 +691,7 @@ static ncg_status ens_pass(ncg_ctx *ctx, const EnsPlan &P, const int *doff, cons
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, counter, 2);
     if (nunits == 0) return NCG_OK;
     const int cus = ctx->num_cus > 0 ? ctx->num_cus : 256;
-    const int nblk = std::min(nunits, 8 * cus);
+    const int nblk = std::min(nunits, P.wg_per_cu * cus);
     {
         ProfScope ps(ctx, st, prec == NCG_MIXED ? "ens_mixed" : "ens_fast64");
         if (prec == NCG_MIXED)
@@ -710,6 +745,8 @@ extern "C" ncg_status ncg_ensemble_rk4_dev(ncg_ctx *ctx, ncg_body *d_bodies,
     const TiledCfg tc = ncg_tiled_cfg(ctx, prec == NCG_MIXED ? 100 + NCG_MIXED : 100 + NCG_F64_FAST,
                                       P.ntot / std::max(nsys, 1));
     P.ipl = tc.ipl == 2 ? 2 : 1;
+    // For the ensemble forms the table's split column is workgroups per CU.
+    P.wg_per_cu = tc.nsplit > 0 ? std::min(tc.nsplit, 32) : 8;
     make_units(P, P.ipl);
     std::vector<int> gfirst, glist;
     unit_index(P, P.grav, gfirst, glist);
@@ -839,6 +876,8 @@ extern "C" ncg_status ncg_ensemble_gravity_dev(ncg_ctx *ctx, const ncg_body *d_b
     const TiledCfg tc = ncg_tiled_cfg(ctx, prec == NCG_MIXED ? 100 + NCG_MIXED : 100 + NCG_F64_FAST,
                                       P.ntot / std::max(nsys, 1));
     P.ipl = tc.ipl == 2 ? 2 : 1;
+    // For the ensemble forms the table's split column is workgroups per CU.
+    P.wg_per_cu = tc.nsplit > 0 ? std::min(tc.nsplit, 32) : 8;
     make_units(P, P.ipl);
     std::vector<int> gfirst, glist;
     unit_index(P, P.grav, gfirst, glist);
