@@ -232,6 +232,32 @@ ncg_status ncg_rk4_steps_dev(ncg_ctx *ctx, ncg_body *d_bodies, int32_t n, const 
                              void *stream);
 
 /* ---------------------------------------------------------------------------
+ * GPU-resident leapfrog of NcorpiON's brute-force integrator
+ * (integration_brute_force_SABA1, rk4.c:840-954): bodies and central body
+ * (d_central: 8 doubles {x, y, z, vx, vy, vz, M, present}) stay on the
+ * device; each of nsteps steps is kick(dt) with the field of vector_field
+ * (rk4.c:68-107), a brute_force collision pass with dt (collision.c:425-456:
+ * candidates on the GPU, then the reference's visiting order replayed on the
+ * GPU with closest_approach and the bounce of collision, physics.c:545-605),
+ * drift(dt) (rk4.c:110-127), and, with one_collision_only, did_collide
+ * cleared (rk4.c:348-353).  collision_f <= 0: no collisions; 2: elastic;
+ * otherwise the inelastic bounce with that parameter.  d_exists and
+ * d_did_collide may be NULL (no holes; no did_collide unless
+ * one_collision_only).  flags bit 0: replay steps from a hipGraph; bit 1:
+ * start with the half drift of rk4.c:871-879 (collision pass and drift with
+ * dt/2); bit 2: end with the backward half drift of rk4.c:938-942 (the
+ * central body included).  With NCG_F64_EXACT every step is the reference's
+ * arithmetic.  *n_collisions (may be NULL) receives the collisions resolved.
+ * No counterpart as one call in NcorpiON: it moves the reference's loop to
+ * the device.
+ */
+ncg_status ncg_leapfrog_dev(ncg_ctx *ctx, ncg_body *d_bodies, int32_t n, const int32_t *d_exists,
+                            int32_t *d_did_collide, double *d_central, double grav_const,
+                            double softening, ncg_precision prec, double dt, int32_t nsteps,
+                            double collision_f, int32_t one_collision_only, int32_t flags,
+                            int64_t *n_collisions, void *stream);
+
+/* ---------------------------------------------------------------------------
  * Batched ensembles: nsys independent systems of uneven size, concatenated in
  * d_bodies (system s = records offsets[s] .. offsets[s+1]-1; offsets is a HOST
  * array of nsys+1 ints, offsets[0] = 0), each with its own central body

@@ -103,6 +103,8 @@ _SIGNATURES = {
     "ncg_ensemble_rk4_dev": (S, [P, P, P, I32, P, D, D, S, D, I32, I32, D, ctypes.POINTER(I64), P]),
     "ncg_ensemble_gravity_dev": (S, [P, P, P, I32, P, D, D, S, P, P, P]),
     "ncg_energy_dev": (S, [P, P, I32, P, P, D, D, P, P]),
+    "ncg_leapfrog_dev": (S, [P, P, I32, P, P, P, D, D, S, D, I32, D, I32, I32,
+                             ctypes.POINTER(I64), P]),
     "ncg_rk4_forward_ckpt_dev": (S, [P, P, I32, P, D, D, D, I32, I32, P, P, P]),
     "ncg_rk4_adjoint_ckpt_dev": (S, [P, P, I32, P, D, D, D, I32, I32, P, P, P, P, P, P]),
     "ncg_builtin_replay_ops": (S, [ctypes.POINTER(BuiltinCollider), ctypes.POINTER(ReplayOps)]),
@@ -265,6 +267,22 @@ class Context:
             self.h, bodies_t.data_ptr(), n, central_t.data_ptr(), float(G), float(softening),
             lam_t.data_ptr(), None if lam_c_t is None else lam_c_t.data_ptr(), gx_t.data_ptr(),
             gm_t.data_ptr(), gc_t.data_ptr(), stream))
+
+    def leapfrog_dev(self, bodies_t, central_t, nsteps, dt, G=1.0, softening=0.0,
+                     precision="exact", collision_f=0.0, one_collision_only=0, exists_t=None,
+                     did_collide_t=None, graph=False, half_drift_start=False,
+                     half_drift_end=False, stream=None):
+        """NcorpiON's brute-force leapfrog on the device (see ncg_leapfrog_dev);
+        returns the number of collisions resolved."""
+        n = int(bodies_t.shape[0])
+        nc = I64()
+        flags = (1 if graph else 0) | (2 if half_drift_start else 0) | (4 if half_drift_end else 0)
+        self._check(self.lib.ncg_leapfrog_dev(
+            self.h, bodies_t.data_ptr(), n, None if exists_t is None else exists_t.data_ptr(),
+            None if did_collide_t is None else did_collide_t.data_ptr(), central_t.data_ptr(),
+            float(G), float(softening), PRECISIONS[precision], float(dt), int(nsteps),
+            float(collision_f), int(one_collision_only), flags, ctypes.byref(nc), stream))
+        return int(nc.value)
 
     def energy_dev(self, bodies_t, central_t=None, exists_t=None, G=1.0, softening=0.0,
                    out_t=None, stream=None):

@@ -92,6 +92,20 @@ __global__ __launch_bounds__(CT) void k_candidates(const DBody *__restrict__ b, 
 
 }  // namespace
 
+ncg_status ncg_candidates_launch(ncg_ctx *ctx, const DBody *d_b, int32_t n, const int32_t *d_ex,
+                                 double dt, int2 *d_out, unsigned long long cap,
+                                 unsigned long long *d_count, hipStream_t st)
+{
+    const int ntiles = (int)cdiv(n, CT);
+    const long nblk = (long)ntiles * (ntiles + 1) / 2;
+    if (nblk == 0) return NCG_OK;
+    ProfScope ps(ctx, st, "candidates");
+    hipLaunchKernelGGL(k_candidates, dim3((unsigned)nblk), dim3(CT), 0, st, d_b, (int)n, d_ex, dt,
+                       ntiles, d_out, cap, d_count);
+    NCG_CHECK_LAUNCH();
+    return NCG_OK;
+}
+
 // Runs the candidate kernel on device buffers; returns the sorted list in `out`.
 static ncg_status candidates_dev(ncg_ctx *ctx, const DBody *db, int32_t n, const int32_t *dex,
                                  double dt, std::vector<ncg_pair> &out)

@@ -29,6 +29,7 @@
 namespace {
 
 constexpr int ENT = 256;  // threads per workgroup
+constexpr unsigned ENS_CAND_CAP = 1u << 20;  // candidates of one pass, all systems
 constexpr int ETJ = 256;  // j tile (LDS)
 
 struct EnsUnit {
@@ -259,265 +260,6 @@ __global__ __launch_bounds__(ENT) void k_ens_pass(const EnsUnit *__restrict__ un
                 }
             }
         }
-    }
-}
-
-// ---- device replay of the collision pass -------------------------------------
-// closest_approach (collision.c:63-172) on bodies a < c of one system: hit
-// when the pair approaches (dr.dv < 0), meets (disc > 0) and meets within dt
-// (t_c <= dt); ap receives the positions at closest approach, heavier body
-// first.  Same FP64 operation order as the host restatement (this file is
-// compiled without FMA contraction).
-__device__ bool ens_approach(const DBody *__restrict__ b, int a, int c, double dt, double *ap,
-                             double *tc)
-{
-    const DBody A = b[a], B = b[c];
-    const double dx = A.x - B.x, dy = A.y - B.y, dz = A.z - B.z;
-    const double dvx = A.vx - B.vx, dvy = A.vy - B.vy, dvz = A.vz - B.vz;
-    const double dr_dot_dv = dx * dvx + dy * dvy + dz * dvz;
-    if (dr_dot_dv >= 0.0) return false;
-    const double dr2 = dx * dx + dy * dy + dz * dz;
-    const double dv2 = dvx * dvx + dvy * dvy + dvz * dvz;
-    const double R = A.radius + B.radius;
-    const double disc = dr_dot_dv * dr_dot_dv + dv2 * (R * R - dr2);
-    if (disc <= 0.0) return false;
-    const double t = -(dr_dot_dv + sqrt(disc)) / dv2;
-    *tc = t;
-    if (t > dt) return false;
-    const double xa = A.x + t * A.vx, ya = A.y + t * A.vy, za = A.z + t * A.vz;
-    const double xb = B.x + t * B.vx, yb = B.y + t * B.vy, zb = B.z + t * B.vz;
-    if (A.mass > B.mass) {
-        ap[0] = xa; ap[1] = ya; ap[2] = za; ap[3] = xb; ap[4] = yb; ap[5] = zb;
-    } else {
-        ap[0] = xb; ap[1] = yb; ap[2] = zb; ap[3] = xa; ap[4] = ya; ap[5] = za;
-    }
-    return true;
-}
-
-// The elastic (f = 2) / inelastic bounce of collision (physics.c:545-605)
-// from the approach of the same pair: impulse along the line of centres at
-// contact, then positions rewound to x_c - t_c v'.
-__device__ void ens_bounce(DBody *__restrict__ b, int a, int c, double f, const double *ap,
-                           double t)
-{
-    const int ia = b[a].mass > b[c].mass ? a : c;
-    const int ib = ia == a ? c : a;
-    DBody P = b[ia], Q = b[ib];
-    const double R = P.radius + Q.radius;
-    const double m_a = P.mass, m_b = Q.mass;
-    double vx_a = P.vx, vy_a = P.vy, vz_a = P.vz, vx_b = Q.vx, vy_b = Q.vy, vz_b = Q.vz;
-    const double xa = ap[0], ya = ap[1], za = ap[2], xb = ap[3], yb = ap[4], zb = ap[5];
-    const double dr_dot_dv = (xa - xb) * (vx_a - vx_b) + (ya - yb) * (vy_a - vy_b) + (za - zb) * (vz_a - vz_b);
-    const double alpha = f * m_a * m_b / ((m_a + m_b) * R * R);
-    const double Jx = alpha * dr_dot_dv * (xb - xa);
-    const double Jy = alpha * dr_dot_dv * (yb - ya);
-    const double Jz = alpha * dr_dot_dv * (zb - za);
-    vx_a += Jx / m_a;
-    vy_a += Jy / m_a;
-    vz_a += Jz / m_a;
-    vx_b -= Jx / m_b;
-    vy_b -= Jy / m_b;
-    vz_b -= Jz / m_b;
-    P.x = xa - t * vx_a; P.y = ya - t * vy_a; P.z = za - t * vz_a;
-    P.vx = vx_a; P.vy = vy_a; P.vz = vz_a;
-    Q.x = xb - t * vx_b; Q.y = yb - t * vy_b; Q.z = zb - t * vz_b;
-    Q.vx = vx_b; Q.vy = vy_b; Q.vz = vz_b;
-    b[ia] = P;
-    b[ib] = Q;
-}
-
-constexpr unsigned ENS_CAND_CAP = 1u << 20;  // candidates of one pass, all systems
-constexpr int RCAP = 1024;  // candidates of one system per pass (LDS)
-constexpr int DCAP = 256;   // bodies changed in one system per pass (LDS)
-
-// One wavefront per system replays brute_force's visiting order
-// (collision.c:425-456): rows i ascending, j > i ascending; a pair is tested
-// when the GPU pass listed it or when one of its bodies changed earlier in
-// this pass; a changed row i is scanned in full (64 j per step, the first
-// hit in j order wins, then the scan resumes after it).  err bit 0: too many
-// candidates overall, bit 1: too many for one system, bit 2: too many
-// changed bodies in one system.
-__global__ __launch_bounds__(64) void k_ens_replay(DBody *__restrict__ y0, const int *__restrict__ off,
-                                                   const int2 *__restrict__ cand,
-                                                   const unsigned *__restrict__ ccnt, unsigned cap,
-                                                   double dt, double f,
-                                                   unsigned char *__restrict__ dirty,
-                                                   unsigned *__restrict__ ncoll,
-                                                   unsigned *__restrict__ err)
-{
-    __shared__ int2 lc[RCAP];
-    __shared__ int dl[DCAP];
-    __shared__ int s_nc, s_nd, s_row, s_from, s_hit;
-    __shared__ double s_ap[6], s_tc;
-    const int lane = threadIdx.x;
-    const int sys = blockIdx.x;
-    const int base = off[sys], ns = off[sys + 1] - base;
-    DBody *b = y0 + base;
-    unsigned char *dt8 = dirty + base;
-    const unsigned total = *ccnt;
-    if (lane == 0) {
-        s_nc = 0;
-        s_nd = 0;
-        if (total > cap) atomicOr(err, 1u);
-    }
-    __syncthreads();
-    const unsigned m = min(total, cap);
-    for (unsigned k = lane; k < m; k += 64) {
-        const int2 c = cand[k];
-        if (c.x >= base && c.x < base + ns) {
-            const int slot = atomicAdd(&s_nc, 1);
-            if (slot < RCAP) lc[slot] = make_int2(c.x - base, c.y - base);
-        }
-    }
-    __syncthreads();
-    if (s_nc == 0) return;
-    if (lane == 0) {
-        if (s_nc > RCAP) {
-            atomicOr(err, 2u);
-            s_nc = RCAP;
-        }
-        // (i, j) order; lists are short.
-        for (int a = 1; a < s_nc; ++a) {
-            const int2 v = lc[a];
-            int q = a - 1;
-            while (q >= 0 && (lc[q].x > v.x || (lc[q].x == v.x && lc[q].y > v.y))) {
-                lc[q + 1] = lc[q];
-                --q;
-            }
-            lc[q + 1] = v;
-        }
-    }
-    __syncthreads();
-    unsigned hits = 0;
-    auto mark = [&](int q) {  // lane 0 only
-        if (dt8[q]) return;
-        dt8[q] = 1;
-        if (s_nd >= DCAP) {
-            atomicOr(err, 4u);
-            return;
-        }
-        int p = s_nd++;
-        while (p > 0 && dl[p - 1] > q) {
-            dl[p] = dl[p - 1];
-            --p;
-        }
-        dl[p] = q;
-    };
-    // First candidate of row r (candidates are in (i, j) order).
-    auto first_cand = [&](int r) {
-        int lo = 0, hi = s_nc;
-        while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (lc[mid].x < r) lo = mid + 1;
-            else hi = mid;
-        }
-        return lo;
-    };
-    int cur = 0;  // every row below cur is done
-    while (true) {
-        // Rows past the largest changed body only have candidate pairs.
-        if (lane == 0) {
-            const int maxd = s_nd ? dl[s_nd - 1] : -1;
-            int start = cur;
-            if (start > maxd) {
-                const int c = first_cand(start);
-                start = c < s_nc ? lc[c].x : ns;
-            }
-            s_row = start < ns ? start : -1;
-        }
-        __syncthreads();
-        const int start = s_row;
-        if (start < 0) break;
-        // 64 rows at once: does row r have anything to do?  A changed row
-        // is scanned in full; another row only if one of its listed pairs
-        // or of its pairs with a changed body hits.  Rows before the first
-        // such row have no hit and change nothing.
-        const int r = start + lane;
-        bool ev = false;
-        if (r < ns) {
-            if (dt8[r]) {
-                ev = true;
-            } else {
-                double ap[6], tc;
-                for (int q = 0; q < s_nd && !ev; ++q)
-                    if (dl[q] > r) ev = ens_approach(b, r, dl[q], dt, ap, &tc);
-                for (int c = first_cand(r); !ev && c < s_nc && lc[c].x == r; ++c)
-                    ev = ens_approach(b, r, lc[c].y, dt, ap, &tc);
-            }
-        }
-        const unsigned long long emask = __ballot(ev);
-        if (emask == 0ull) {
-            cur = start + 64;
-            __syncthreads();
-            continue;
-        }
-        const int row = start + __ffsll((long long)emask) - 1;
-        // That row in the reference's order, on lane 0 until its first hit.
-        if (lane == 0) {
-            s_from = -1;
-            if (dt8[row]) {
-                s_from = row + 1;  // changed row: scan it in full
-            } else {
-                int ci = first_cand(row);
-                int jc = row + 1;
-                while (jc < ns) {
-                    while (ci < s_nc && lc[ci].x == row && lc[ci].y < jc) ++ci;
-                    const int jcand = (ci < s_nc && lc[ci].x == row) ? lc[ci].y : ns;
-                    int jdirty = ns;
-                    for (int q = 0; q < s_nd; ++q)
-                        if (dl[q] >= jc) {
-                            jdirty = dl[q];
-                            break;
-                        }
-                    const int j = min(jcand, jdirty);
-                    if (j >= ns) break;
-                    double ap[6], tc;
-                    if (ens_approach(b, row, j, dt, ap, &tc)) {
-                        ens_bounce(b, row, j, f, ap, tc);
-                        ++hits;
-                        mark(row);
-                        mark(j);
-                        s_from = j + 1;  // row changed: the rest of it in full
-                        break;
-                    }
-                    jc = j + 1;
-                }
-            }
-        }
-        __threadfence_block();
-        __syncthreads();
-        int jc = s_from;
-        while (jc >= 0 && jc < ns) {
-            const int j = jc + lane;
-            double ap[6], tc;
-            const bool hit = j < ns && ens_approach(b, row, j, dt, ap, &tc);
-            const unsigned long long mask = __ballot(hit);
-            if (mask == 0ull) {
-                jc += 64;
-                continue;
-            }
-            const int first = __ffsll((long long)mask) - 1;
-            if (lane == first) {
-                for (int q = 0; q < 6; ++q) s_ap[q] = ap[q];
-                s_tc = tc;
-                s_hit = j;
-            }
-            __syncthreads();
-            if (lane == 0) {
-                ens_bounce(b, row, s_hit, f, s_ap, s_tc);
-                ++hits;
-                mark(s_hit);
-            }
-            jc = s_hit + 1;
-            __threadfence_block();
-            __syncthreads();
-        }
-        cur = row + 1;
-        __syncthreads();
-    }
-    if (lane == 0) {
-        for (int q = 0; q < s_nd; ++q) dt8[dl[q]] = 0;
-        if (hits) atomicAdd(ncoll, hits);
     }
 }
 
@@ -794,9 +536,9 @@ extern "C" ncg_status ncg_ensemble_rk4_dev(ncg_ctx *ctx, ncg_body *d_bodies,
             ncg_status r = ens_pass(ctx, P, D.off, D.coll, (int)P.coll.size(), nullptr, nullptr, y0,
                                     d_centrals, G, soft, prec, dt, acc, cm, st);
             if (r != NCG_OK) return r;
-            hipLaunchKernelGGL(k_ens_replay, dim3(nsys), dim3(64), 0, st, y0, D.off, cand,
-                               counter + 1, ENS_CAND_CAP, dt, collision_f, dirty, stat, stat + 1);
-            NCG_CHECK_LAUNCH();
+            r = ncg_bf_replay_launch(ctx, y0, nullptr, nullptr, 0, D.off, nsys, cand, counter + 1,
+                                     nullptr, ENS_CAND_CAP, dt, collision_f, dirty, stat, st);
+            if (r != NCG_OK) return r;
         }
         for (int s = 0; s < 4; ++s) {
             ncg_status r = ens_eval(ctx, P, D, s == 0 ? y0 : Y, s == 0 ? d_centrals : Yc, G, soft,
@@ -848,7 +590,7 @@ extern "C" ncg_status ncg_ensemble_rk4_dev(ncg_ctx *ctx, ncg_body *d_bodies,
                         "collision replay capacity exceeded (flags %u: 1 = more than %u candidates "
                         "in one pass, 2 = more than %d in one system, 4 = more than %d changed "
                         "bodies in one system); results are not valid",
-                        hs[1], ENS_CAND_CAP, RCAP, DCAP);
+                        hs[1], ENS_CAND_CAP, NCG_REPLAY_RCAP, NCG_REPLAY_DCAP);
     return NCG_OK;
 }
 

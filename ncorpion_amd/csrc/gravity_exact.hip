@@ -18,15 +18,25 @@ namespace {
 
 constexpr int EX_T = 256;
 
+// Central body from the host (cp, cx, cy, cz, mu) or, when dcen is given,
+// from a device record {x, y, z, vx, vy, vz, M, present}.
 __global__ __launch_bounds__(256) void k_grav_exact(const DBody *__restrict__ b, int n,
                                                     const int32_t *__restrict__ ex, int cp,
                                                     double cx, double cy, double cz, double mu,
+                                                    const double *__restrict__ dcen,
                                                     double G, double s,
                                                     double *__restrict__ acc,
                                                     double *__restrict__ cterm)
 {
     __shared__ double tx[EX_T], ty[EX_T], tz[EX_T], tgm[EX_T], tr[EX_T];
     __shared__ int te[EX_T];
+    if (dcen) {
+        cp = dcen[7] != 0.0;
+        cx = dcen[0];
+        cy = dcen[1];
+        cz = dcen[2];
+        mu = G * dcen[6];  // physics.c:131
+    }
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const bool mine = k < n && (ex == nullptr || ex[k] != 0);
     double X = 0., Y = 0., Z = 0., Rk = 0.;
@@ -99,6 +109,25 @@ __global__ __launch_bounds__(256) void k_grav_exact(const DBody *__restrict__ b,
     }
 }
 
+// CM_acc as vector_field forms it: the per-body reaction terms added in body
+// order, one thread (physics.c:141-146 accumulates CM_acc body by body).
+__global__ void k_cm_seq(const double *__restrict__ cterm, int n, const int32_t *__restrict__ ex,
+                         const double *__restrict__ dcen, double *__restrict__ cm)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double a = 0.0, c = 0.0, d = 0.0;
+    if (dcen[7] != 0.0)
+        for (int k = 0; k < n; ++k) {
+            if (ex && !ex[k]) continue;
+            a += cterm[3 * (size_t)k];
+            c += cterm[3 * (size_t)k + 1];
+            d += cterm[3 * (size_t)k + 2];
+        }
+    cm[0] = a;
+    cm[1] = c;
+    cm[2] = d;
+}
+
 }  // namespace
 
 // Launches the exact kernel; cterm (3n) receives the per-body central
@@ -110,8 +139,28 @@ ncg_status ncg_grav_exact_launch(ncg_ctx *ctx, const DBody *d_b, int32_t n, cons
     const int cp = (c && c->present) ? 1 : 0;
     const double mu = cp ? G * c->mass : 0.0;  // physics.c:131
     hipLaunchKernelGGL(k_grav_exact, dim3(cdiv(n, 256)), dim3(256), 0, st, d_b, (int)n, d_ex, cp,
-                       cp ? c->x : 0.0, cp ? c->y : 0.0, cp ? c->z : 0.0, mu, G, soft, d_acc,
-                       d_cterm);
+                       cp ? c->x : 0.0, cp ? c->y : 0.0, cp ? c->z : 0.0, mu,
+                       (const double *)nullptr, G, soft, d_acc, d_cterm);
     NCG_CHECK_LAUNCH();
+    return NCG_OK;
+}
+
+// The same with the central body in a device record (8 doubles), and
+// CM_acc summed on the device in the reference's order.
+ncg_status ncg_grav_exact_dev(ncg_ctx *ctx, const DBody *d_b, int32_t n, const int32_t *d_ex,
+                              const double *d_cen, double G, double soft, double *d_acc,
+                              double *d_cterm, double *d_cm_acc, hipStream_t st)
+{
+    {
+        ProfScope ps(ctx, st, "exact");
+        hipLaunchKernelGGL(k_grav_exact, dim3(cdiv(n, 256)), dim3(256), 0, st, d_b, (int)n, d_ex,
+                           0, 0.0, 0.0, 0.0, 0.0, d_cen, G, soft, d_acc, d_cterm);
+        NCG_CHECK_LAUNCH();
+    }
+    if (d_cm_acc) {
+        hipLaunchKernelGGL(k_cm_seq, dim3(1), dim3(64), 0, st, d_cterm, (int)n, d_ex, d_cen,
+                           d_cm_acc);
+        NCG_CHECK_LAUNCH();
+    }
     return NCG_OK;
 }

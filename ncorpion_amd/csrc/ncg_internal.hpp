@@ -89,6 +89,32 @@ struct ncg_ctx {
 };
 
 void ncg_rk4_release(ncg_ctx *ctx);
+
+// Exact brute-force field with the central body in a device record (8
+// doubles {x, y, z, vx, vy, vz, M, present}); CM_acc summed in body order.
+ncg_status ncg_grav_exact_dev(ncg_ctx *ctx, const DBody *d_b, int32_t n, const int32_t *d_ex,
+                              const double *d_cen, double G, double soft, double *d_acc,
+                              double *d_cterm, double *d_cm_acc, hipStream_t st);
+
+// Candidates of brute_force's pass (collide_allpairs.hip), appended to
+// d_out[0 .. cap) with the count in *d_count (zeroed by the caller).
+ncg_status ncg_candidates_launch(ncg_ctx *ctx, const DBody *d_b, int32_t n, const int32_t *d_ex,
+                                 double dt, int2 *d_out, unsigned long long cap,
+                                 unsigned long long *d_count, hipStream_t st);
+
+// Device replay of a brute-force collision pass (replay.hip): one wavefront
+// per system (systems d_off[0..nsys]); candidates d_cand[0 .. *count) with
+// the count in d_count32 or d_count64; d_ex, d_dc may be null.  d_stat[0] +=
+// collisions, d_stat[1] |= capacity error bits.  d_dirty: one zeroed byte
+// per body, left zeroed.
+constexpr int NCG_REPLAY_RCAP = 1024;  // candidates of one system per pass
+constexpr int NCG_REPLAY_DCAP = 256;   // bodies changed in one system per pass
+ncg_status ncg_bf_replay_launch(ncg_ctx *ctx, DBody *d_b, const int32_t *d_ex, int32_t *d_dc,
+                                int one_collision_only, const int *d_off, int nsys,
+                                const int2 *d_cand, const unsigned *d_count32,
+                                const unsigned long long *d_count64, unsigned cap, double dt,
+                                double f, unsigned char *d_dirty, unsigned *d_stat,
+                                hipStream_t st);
 // Launch shape for `prec` at n bodies: ncg_set_tiling's, else the tuning table's.
 TiledCfg ncg_tiled_cfg(const ncg_ctx *ctx, int prec, long n);
 

@@ -29,6 +29,8 @@
  *   oracle_rk4_step            4th-order convergence slope, two-body orbit
  *                              (no reference counterpart: NcorpiON has no RK4).
  *   oracle_energy              two-body closed form.
+ *   oracle_state_hash          the FNV-1a of oracle/ref_trace.c (a helper for
+ *                              comparing with traced runs, not a restatement).
  */
 #include <math.h>
 #include <stdint.h>
@@ -627,4 +629,26 @@ void oracle_resolve(double *bodies, const int *exists, int *did_collide, int a, 
 {
     oracle_treat(bodies, exists, did_collide, a, b, type, collision_parameter, one_collision_only,
                  approach, t_c);
+}
+
+/* FNV-1a 64 hash of every existing body (int32 index, then its 64-byte
+ * record), as oracle/ref_trace.c logs the state at each traced call; lets a
+ * device run be compared with a traced reference run step by step. */
+uint64_t oracle_state_hash(const double *bodies, int n, const int *exists)
+{
+    uint64_t h = 1469598103934665603ULL;
+    for (int k = 0; k < n; k++) {
+        if (exists && !exists[k]) continue;
+        const unsigned char *c = (const unsigned char *)&k;
+        for (size_t i = 0; i < sizeof k; i++) {
+            h ^= c[i];
+            h *= 1099511628211ULL;
+        }
+        c = (const unsigned char *)(bodies + 8 * (size_t)k);
+        for (size_t i = 0; i < 64; i++) {
+            h ^= c[i];
+            h *= 1099511628211ULL;
+        }
+    }
+    return h;
 }

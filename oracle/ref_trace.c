@@ -12,6 +12,12 @@
  *
  * The trace goes to $NCG_TRACE (default ./trace.txt).  Per-function wall time
  * (CLOCK_MONOTONIC around the target call) is appended at exit.
+ *
+ * With $NCG_TRACE_DUMP set, the input state of the first brute_force call (the
+ * half drift before the integration loop, rk4.c:871-879) is written there:
+ * int32 n = largest_id + 1; n 64-byte bodies; int32 exists[n];
+ * int32 did_collide[n]; double CM {x, y, z, vx, vy, vz, mass, radius};
+ * double timestep.  A device run can start from it.
  */
 #define _POSIX_C_SOURCE 199309L
 #include <stdint.h>
@@ -129,8 +135,31 @@ void __wrap_vector_field(struct moonlet *X)
             (unsigned long long)accel_hash(X), CM_acc[0], CM_acc[1], CM_acc[2]);
 }
 
+static void dump_state(const struct moonlet *X)
+{
+    const char *p = getenv("NCG_TRACE_DUMP");
+    if (!p) return;
+    FILE *f = fopen(p, "wb");
+    if (!f) abort();
+    int n = largest_id + 1;
+    fwrite(&n, sizeof n, 1, f);
+    fwrite(X, sizeof(struct moonlet), (size_t)n, f);
+    fwrite(exists, sizeof(int), (size_t)n, f);
+    fwrite(did_collide, sizeof(int), (size_t)n, f);
+    double cm[8] = {CM.x, CM.y, CM.z, CM.vx, CM.vy, CM.vz, CM.mass, CM.radius};
+    fwrite(cm, sizeof(double), 8, f);
+    double ts = timestep;
+    fwrite(&ts, sizeof ts, 1, f);
+    fclose(f);
+}
+
 static void collide_call(int which, void (*fn)(struct moonlet *), const char *tag, struct moonlet *X)
 {
+    static int dumped = 0;
+    if (which == 1 && !dumped) {
+        dump_state(X);
+        dumped = 1;
+    }
     call_no++;
     uint64_t hin = state_hash(X);
     fprintf(tf(), "%s %ld in %016llx dt %a\n", tag, call_no, (unsigned long long)hin, timestep);

@@ -41,6 +41,8 @@ def lib():
         _lib.oracle_rk4_step.argtypes = [P, I, P, I, P, D, D, D]
         _lib.oracle_energy.argtypes = [P, I, P, I, P, D, D]
         _lib.oracle_energy.restype = D
+        _lib.oracle_state_hash.argtypes = [P, I, P]
+        _lib.oracle_state_hash.restype = ctypes.c_uint64
         _lib.oracle_resolve.argtypes = [P, P, P, I, I, I, D, I, P, D]
     return _lib
 
@@ -184,3 +186,11 @@ class Replay:
         lib().oracle_resolve(_p(self.b), _p(self.e), _p(self.dc), a, b, self.ctype, self.f,
                              self.one, _p(self.ap), self.tc.value)
         self.log.append((a, b))
+
+
+def state_hash(bodies, exists=None):
+    """FNV-1a of the existing bodies as the reference trace logs it."""
+    b = np.ascontiguousarray(bodies, dtype=np.float64)
+    n = b.shape[0]
+    e = None if exists is None else np.ascontiguousarray(exists, dtype=np.int32)
+    return int(lib().oracle_state_hash(_p(b), n, None if e is None else _p(e)))

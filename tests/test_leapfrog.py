@@ -1,0 +1,202 @@
+"""GPU-resident leapfrog (ncg_leapfrog_dev) against NcorpiON's own integration
+loop (integration_brute_force_SABA1, rk4.c:840-954).
+
+`oracle_leapfrog` restates that loop on the CPU with the oracle's
+vector_field and brute_force: half drift (brute_force and drift with dt/2),
+then per step kick(dt), brute_force(dt), drift(dt).  The CPU test pins it to a
+traced run of the compiled reference (state hash at every vector_field call);
+the GPU tests compare the device loop with it bit for bit, and with the traced
+reference runs directly.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle_lib as orc
+from ncorpion_amd import ic
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+REF = os.path.join(ROOT, "oracle", "_ref")
+
+
+def oracle_leapfrog(bodies, exists, central8, G, soft, dt, nsteps, f, one_coll, half_start,
+                    did_collide=None, on_step=None):
+    """rk4.c:871-879 (half drift) and 896-905 (kick, brute_force, drift) with
+    kick/drift of rk4.c:68-127; did_collide cleared after each step when
+    one_coll (rk4.c:348-353).  Returns (bodies, central8, collisions)."""
+    b = np.array(bodies, dtype=np.float64)
+    ex = np.ones(b.shape[0], np.int32) if exists is None else np.array(exists, np.int32)
+    live = ex != 0
+    c = np.array(central8, dtype=np.float64)
+    dc = np.zeros(b.shape[0], np.int32) if did_collide is None else np.array(did_collide, np.int32)
+    ctype = 0 if f == 2.0 else 1
+    present = c[7] != 0
+    ncoll = 0
+
+    def drift(h):
+        b[live, 0:3] += h * b[live, 3:6]
+        if present:
+            c[0:3] += h * c[3:6]
+
+    if half_start:
+        if f > 0:
+            nb, dc, pairs = orc.brute_force(b, 0.5 * dt, ctype, f, one_coll, exists=ex, did_collide=dc)
+            b[:] = nb
+            ncoll += len(pairs)
+        drift(0.5 * dt)
+    for s in range(nsteps):
+        if on_step:
+            on_step(s, b)
+        acc, cm = orc.vector_field_brute(b, ex, (c[0], c[1], c[2], c[6]) if present else None, G,
+                                         soft)
+        b[live, 3:6] += dt * acc[live]
+        if present:
+            c[3:6] += dt * cm
+        if f > 0:
+            nb, dc, pairs = orc.brute_force(b, dt, ctype, f, one_coll, exists=ex, did_collide=dc)
+            b[:] = nb
+            ncoll += len(pairs)
+        drift(dt)
+        if one_coll:
+            dc[:] = 0
+    return b, c, ncoll
+
+
+def read_dump(path):
+    raw = open(path, "rb").read()
+    n = int(np.frombuffer(raw[:4], np.int32)[0])
+    o = 4
+    bodies = np.frombuffer(raw[o:o + 64 * n], np.float64).reshape(n, 8).copy()
+    o += 64 * n
+    ex = np.frombuffer(raw[o:o + 4 * n], np.int32).copy()
+    o += 4 * n
+    dc = np.frombuffer(raw[o:o + 4 * n], np.int32).copy()
+    o += 4 * n
+    cm = np.frombuffer(raw[o:o + 64], np.float64).copy()
+    o += 64
+    ts = float(np.frombuffer(raw[o:o + 8], np.float64)[0])
+    return bodies, ex, dc, cm, ts
+
+
+def traced_reference(cfg, bodies, tmp_path):
+    """Runs the traced reference; returns (dump, VF input hashes, collisions)."""
+    exe = os.path.join(REF, cfg, "ncorpion_trace")
+    if not os.path.exists(exe):
+        pytest.skip("reference builds missing (oracle/build_ref.py)")
+    ic.write_init_txt(str(tmp_path / "init.txt"), bodies)
+    env = dict(os.environ, NCG_TRACE=str(tmp_path / "trace.txt"),
+               NCG_TRACE_DUMP=str(tmp_path / "dump.bin"))
+    r = subprocess.run([exe], cwd=str(tmp_path), env=env, stdout=subprocess.PIPE,
+                       stderr=subprocess.STDOUT, text=True, timeout=600)
+    assert r.returncode == 0, r.stdout[-2000:]
+    lines = open(tmp_path / "trace.txt").read().splitlines()
+    vf = [int(l.split()[2], 16) for l in lines if l.startswith("VF ")]
+    ncoll = sum(1 for l in lines if l.startswith("C "))
+    return read_dump(str(tmp_path / "dump.bin")), vf, ncoll
+
+
+def test_oracle_leapfrog_reproduces_reference_trace(tmp_path):
+    """CPU: the restated loop hits the reference's state hash at every one of
+    the 40 vector_field calls of a dense run with collisions."""
+    b0 = ic.disk(2000, seed=1, mass=1e-8, radius=4e-3)
+    (bodies, ex, dc, cm, ts), vf, ncoll = traced_reference("run_c1_dense", b0, tmp_path)
+    dt = 2.0 * ts
+    seen = []
+    _, _, nc = oracle_leapfrog(bodies, ex, np.concatenate([cm[0:7], [1.0]]), 1.0, 0.0, dt,
+                               len(vf), 1.5, 0, True, did_collide=dc,
+                               on_step=lambda s, b: seen.append(orc.state_hash(b, ex)))
+    assert seen == vf
+    assert nc == ncoll and ncoll > 20
+
+
+def tt():
+    import torch
+    return torch
+
+
+def cluster(n, seed, radius=2e-3, box=0.05):
+    rng = np.random.default_rng(seed)
+    b = np.zeros((n, 8))
+    b[:, 0:3] = rng.uniform(-box, box, (n, 3)) + np.array([1.0, 0, 0])
+    b[:, 3:6] = rng.normal(0, 0.2, (n, 3)) + np.array([0, 1.0, 0])
+    b[:, 6] = rng.uniform(1e-9, 1e-8, n)
+    b[:, 7] = rng.uniform(0.5 * radius, radius, n)
+    return b
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("f,one,graph,holes", [(1.5, 0, False, False), (1.5, 0, True, True),
+                                               (2.0, 1, True, False), (1.25, 1, False, True)])
+def test_leapfrog_matches_oracle_leapfrog(ctx, f, one, graph, holes):
+    torch = tt()
+    n, dt, steps = 1500, 0.004, 6
+    b = cluster(n, 40 + int(10 * f) + one)
+    ex = np.ones(n, np.int32)
+    if holes:
+        ex[np.random.default_rng(5).choice(n, 100, replace=False)] = 0
+    c8 = np.array([0.001, 0.0, 0.0, 0.0, 1e-3, 0.0, 1.0, 1.0])
+    ref_b, ref_c, ref_n = oracle_leapfrog(b, ex, c8, 1.0, 0.01, dt, steps, f, one, True)
+    bt = torch.from_numpy(b).cuda()
+    ct = torch.from_numpy(c8).cuda()
+    et = torch.from_numpy(ex).cuda()
+    dct = torch.zeros(n, dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()
+    nc = ctx.leapfrog_dev(bt, ct, steps, dt, 1.0, 0.01, "exact", f, one, exists_t=et,
+                          did_collide_t=dct, graph=graph, half_drift_start=True)
+    ctx.synchronize()
+    got = bt.cpu().numpy()
+    live = ex != 0
+    assert nc == ref_n and ref_n > 10
+    assert np.array_equal(got[live].view(np.uint64), ref_b[live].view(np.uint64))
+    assert np.array_equal(ct.cpu().numpy().view(np.uint64), ref_c.view(np.uint64))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("cfg,seed,radius", [("run_c1", 0, 2e-4), ("run_c1_dense", 1, 4e-3)])
+def test_leapfrog_reproduces_reference_run(ctx, tmp_path, cfg, seed, radius):
+    """Start from the reference's own state before its half drift, then step
+    on the device: the state hash after every step equals the reference's at
+    the next vector_field call, and the collision counts agree."""
+    torch = tt()
+    b0 = ic.disk(2000, seed=seed, mass=1e-8, radius=radius)
+    (bodies, ex, dc, cm, ts), vf, ncoll = traced_reference(cfg, b0, tmp_path)
+    dt = 2.0 * ts
+    bt = torch.from_numpy(bodies).cuda()
+    ct = torch.from_numpy(np.concatenate([cm[0:7], [1.0]])).cuda()
+    et = torch.from_numpy(ex).cuda()
+    dct = torch.from_numpy(dc).cuda()
+    total = ctx.leapfrog_dev(bt, ct, 0, dt, 1.0, 0.0, "exact", 1.5, 0, exists_t=et,
+                             did_collide_t=dct, half_drift_start=True)
+    for s, h in enumerate(vf):
+        ctx.synchronize()
+        assert orc.state_hash(bt.cpu().numpy(), ex) == h, "step %d" % s
+        if s + 1 < len(vf):
+            total += ctx.leapfrog_dev(bt, ct, 1, dt, 1.0, 0.0, "exact", 1.5, 0, exists_t=et,
+                                      did_collide_t=dct)
+    # The reference's last step (after its last vector_field call) is not hashed.
+    total += ctx.leapfrog_dev(bt, ct, 1, dt, 1.0, 0.0, "exact", 1.5, 0, exists_t=et,
+                              did_collide_t=dct)
+    assert total == ncoll
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec,tol", [("fast64", 1e-12), ("mixed", 1e-8)])
+def test_leapfrog_tiled_forms(ctx, prec, tol):
+    """The tiled fields in the same loop stay close to the exact one."""
+    torch = tt()
+    n, dt, steps = 4096, 2 * np.pi / 256, 8
+    b = ic.disk(n, seed=3, mass=1e-9, radius=1e-4)
+    c8 = np.array([0, 0, 0, 0, 0, 0, 1.0, 1.0])
+    out = {}
+    for p in ("exact", prec):
+        bt = torch.from_numpy(b).cuda()
+        ct = torch.from_numpy(c8).cuda()
+        torch.cuda.synchronize()
+        ctx.leapfrog_dev(bt, ct, steps, dt, 1.0, 1e-6, p, 0.0, 0, graph=True,
+                         half_drift_start=True, half_drift_end=True)
+        ctx.synchronize()
+        out[p] = bt.cpu().numpy()
+    d = np.abs(out[prec][:, 0:6] - out["exact"][:, 0:6]).max()
+    assert d < tol, d
