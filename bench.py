@@ -33,10 +33,10 @@ PEAK_FP64_VECTOR_TFLOPS = 78.6
 
 
 # HBM bytes per launch of the mixed pair kernel at N = 262144 with the tuned
-# shape (nsplit 128): FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (scales from a
+# shape (nsplit 128, theta 0.20): FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (scales from a
 # 256 MiB copy), from rocprofv3 --pmc runs of this script, summarised in
 # profiles/k_pairs_mixed_n262144_pmc.md.  Mostly the j-split partial sums.
-PAIRS_TRAFFIC_BYTES = (110702 * 2.000 + 786432 * 1.000) * 1024
+PAIRS_TRAFFIC_BYTES = (110634 * 2.000 + 786432 * 1.000) * 1024
 
 
 def make_workload(n, seed=0):
@@ -189,6 +189,63 @@ def cpu_mesh_baseline(timeout=300):
                       "2^20-body mesh workload"}
 
 
+C1_N, C1_STEPS = 2000, 200
+
+
+def leapfrog_run(ctx, torch):
+    """NcorpiON's brute-force leapfrog on the device (ncg_leapfrog_dev): disk of
+    2000 bodies, m = 1e-8, R = 2e-4, dt = 2 pi / 256, 200 steps after the half
+    drift, inelastic bounces (f = 1.5); exact field (the reference's
+    arithmetic), and the FP64-fast and mixed fields."""
+    import numpy as np
+    from ncorpion_amd import ic
+    b = ic.disk(C1_N, seed=0, mass=1e-8, radius=2e-4)
+    dt = 2.0 * math.pi / 256.0
+    out = {"n": C1_N, "steps": C1_STEPS}
+    for prec in ("exact", "fast64", "mixed"):
+        bt = torch.from_numpy(b).cuda()
+        ct = torch.tensor([0, 0, 0, 0, 0, 0, 1.0, 1.0], dtype=torch.float64, device="cuda")
+        wb, wc = bt.clone(), ct.clone()
+        ctx.leapfrog_dev(wb, wc, 2, dt, 1.0, 0.0, prec, 1.5, 0, graph=True)  # warm-up
+        ctx.synchronize()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        nc = ctx.leapfrog_dev(bt, ct, C1_STEPS, dt, 1.0, 0.0, prec, 1.5, 0, graph=True,
+                              half_drift_start=True)
+        ctx.synchronize()
+        t = time.perf_counter() - t0
+        out[prec] = {"seconds": t, "steps_per_s": C1_STEPS / t, "collisions": nc}
+    return out
+
+
+def cpu_c1_baseline(timeout=300):
+    """NcorpiON's own brute-force run of the same workload (compiled from source,
+    1 thread): time inside vector_field and brute_force, from the traced build."""
+    import numpy as np
+    from ncorpion_amd import ic
+    exe = os.path.join(ROOT, "oracle", "_ref", "run_c1", "ncorpion_trace")
+    if not os.path.exists(exe):
+        return None
+    with tempfile.TemporaryDirectory() as td:
+        ic.write_init_txt(os.path.join(td, "init.txt"),
+                          ic.disk(C1_N, seed=0, mass=1e-8, radius=2e-4))
+        env = dict(os.environ, NCG_TRACE=os.path.join(td, "trace.txt"))
+        r = subprocess.run([exe], cwd=td, env=env, stdout=subprocess.PIPE,
+                           stderr=subprocess.STDOUT, text=True, timeout=timeout)
+        if r.returncode != 0:
+            return None
+        t = {}
+        for line in open(os.path.join(td, "trace.txt")):
+            if line.startswith("TIME"):
+                f = line.split()
+                t[f[1]] = float(f[3].split("=")[1])
+    hot = t.get("vector_field", 0.0) + t.get("brute_force", 0.0)
+    return {"value": C1_STEPS / hot, "unit": "steps/s", "cores": 1, "kind": "reference",
+            "seconds_in_vector_field_and_brute_force": hot,
+            "sample": "NcorpiON brute-force run, 2000 bodies, 200 steps (gcc -O2, 1 thread): "
+                      "wall time inside vector_field and brute_force"}
+
+
 def ensemble_run(ctx, torch, nsys=512, steps=100):
     """512 independent disks (seeds 0..511), N log-uniform in [256, 8192],
     m = 1e-8, R = 2e-4, brute-force gravity + collision detection, mixed form,
@@ -334,6 +391,7 @@ def main():
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--no-mesh", action="store_true")
     ap.add_argument("--no-collisions", action="store_true")
+    ap.add_argument("--no-leapfrog", action="store_true")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run a plain device copy of this size (HBM counter calibration)")
     args = ap.parse_args()
@@ -448,6 +506,8 @@ def main():
         out["collision_filter"] = collision_filter_run(ctx, b, dt)
     if not args.no_precision:
         out["precision_study"] = precision_run(ctx, torch, b, soft, dt)
+    if not args.no_leapfrog:
+        out["leapfrog_c1"] = leapfrog_run(ctx, torch)
     if not args.no_mesh:
         out["mesh_module"] = mesh_run(ctx, torch)
     if not args.no_ensemble:
@@ -468,6 +528,8 @@ def main():
         out["cpu_baseline"] = cpu_baseline(b, args.cpu_sample)
         if out["cpu_baseline"] is not None and not args.no_mesh:
             out["cpu_baseline"]["mesh"] = cpu_mesh_baseline()
+        if out["cpu_baseline"] is not None and not args.no_leapfrog:
+            out["cpu_baseline"]["c1_run"] = cpu_c1_baseline()
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out))

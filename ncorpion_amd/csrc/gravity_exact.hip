@@ -18,6 +18,24 @@ namespace {
 
 constexpr int EX_T = 256;
 
+// physics.c:186-201 for body k at (X, Y, Z) and body j: D = sqrt(DX*DX +
+// DY*DY + DZ*DZ + soft*soft), D3 = D*D*D, term G m_j DX / D3, with DX formed
+// as the reference forms it for j < k (X_k - X_j) and j > k (X_j - X_k).
+__device__ inline void exact_term(double X, double Y, double Z, double Rk, double xj, double yj,
+                                  double zj, double Rj, double gmj, double s, bool lower,
+                                  double &tX, double &tY, double &tZ)
+{
+    const double dx = lower ? X - xj : xj - X;
+    const double dy = lower ? Y - yj : yj - Y;
+    const double dz = lower ? Z - zj : zj - Z;
+    const double soft = s * (lower ? (Rk + Rj) : (Rj + Rk));
+    const double D = sqrt(dx * dx + dy * dy + dz * dz + soft * soft);
+    const double D3 = D * D * D;
+    tX = gmj * dx / D3;
+    tY = gmj * dy / D3;
+    tZ = gmj * dz / D3;
+}
+
 // Central body from the host (cp, cx, cy, cz, mu) or, when dcen is given,
 // from a device record {x, y, z, vx, vy, vz, M, present}.
 __global__ __launch_bounds__(256) void k_grav_exact(const DBody *__restrict__ b, int n,
@@ -77,20 +95,41 @@ __global__ __launch_bounds__(256) void k_grav_exact(const DBody *__restrict__ b,
         __syncthreads();
         if (!mine) continue;
         const int cnt = min(EX_T, n - base);
-        for (int jj = 0; jj < cnt; ++jj) {
+        // Terms of four j at a time are formed independently (their square
+        // roots and divisions overlap), then added in j order: the sum is the
+        // reference's, skipped terms leave the accumulators untouched.
+        int jj = 0;
+        for (; jj + 4 <= cnt; jj += 4) {
+            double tX[4], tY[4], tZ[4];
+            bool use[4], low[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int j = base + jj + u;
+                use[u] = te[jj + u] && j != k;
+                low[u] = j < k;
+                exact_term(X, Y, Z, Rk, tx[jj + u], ty[jj + u], tz[jj + u], tr[jj + u],
+                           tgm[jj + u], s, low[u], tX[u], tY[u], tZ[u]);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (!use[u]) continue;
+                if (low[u]) {
+                    aX -= tX[u];
+                    aY -= tY[u];
+                    aZ -= tZ[u];
+                } else {
+                    aX += tX[u];
+                    aY += tY[u];
+                    aZ += tZ[u];
+                }
+            }
+        }
+        for (; jj < cnt; ++jj) {
             const int j = base + jj;
             if (!te[jj] || j == k) continue;
             const bool lower = j < k;
-            // physics.c:186-201: D = sqrt(DX*DX+DY*DY+DZ*DZ+soft*soft), D3 = D*D*D
-            const double dx = lower ? X - tx[jj] : tx[jj] - X;
-            const double dy = lower ? Y - ty[jj] : ty[jj] - Y;
-            const double dz = lower ? Z - tz[jj] : tz[jj] - Z;
-            const double soft = s * (lower ? (Rk + tr[jj]) : (tr[jj] + Rk));
-            const double D = sqrt(dx * dx + dy * dy + dz * dz + soft * soft);
-            const double D3 = D * D * D;
-            const double tX = tgm[jj] * dx / D3;
-            const double tY = tgm[jj] * dy / D3;
-            const double tZ = tgm[jj] * dz / D3;
+            double tX, tY, tZ;
+            exact_term(X, Y, Z, Rk, tx[jj], ty[jj], tz[jj], tr[jj], tgm[jj], s, lower, tX, tY, tZ);
             if (lower) {
                 aX -= tX;
                 aY -= tY;
@@ -111,21 +150,41 @@ __global__ __launch_bounds__(256) void k_grav_exact(const DBody *__restrict__ b,
 
 // CM_acc as vector_field forms it: the per-body reaction terms added in body
 // order, one thread (physics.c:141-146 accumulates CM_acc body by body).
-__global__ void k_cm_seq(const double *__restrict__ cterm, int n, const int32_t *__restrict__ ex,
-                         const double *__restrict__ dcen, double *__restrict__ cm)
+// The workgroup stages 256 bodies' terms in LDS, one lane adds them in order.
+__global__ __launch_bounds__(256) void k_cm_seq(const double *__restrict__ cterm, int n,
+                                                const int32_t *__restrict__ ex,
+                                                const double *__restrict__ dcen,
+                                                double *__restrict__ cm)
 {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    __shared__ double st[3 * 256];
+    __shared__ int se[256];
     double a = 0.0, c = 0.0, d = 0.0;
-    if (dcen[7] != 0.0)
-        for (int k = 0; k < n; ++k) {
-            if (ex && !ex[k]) continue;
-            a += cterm[3 * (size_t)k];
-            c += cterm[3 * (size_t)k + 1];
-            d += cterm[3 * (size_t)k + 2];
+    const bool cp = dcen[7] != 0.0;
+    for (int base = 0; cp && base < n; base += 256) {
+        const int k = base + (int)threadIdx.x;
+        __syncthreads();
+        if (k < n) {
+            se[threadIdx.x] = ex ? (ex[k] != 0) : 1;
+            st[3 * threadIdx.x] = cterm[3 * (size_t)k];
+            st[3 * threadIdx.x + 1] = cterm[3 * (size_t)k + 1];
+            st[3 * threadIdx.x + 2] = cterm[3 * (size_t)k + 2];
         }
-    cm[0] = a;
-    cm[1] = c;
-    cm[2] = d;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const int cnt = min(256, n - base);
+            for (int q = 0; q < cnt; ++q) {
+                if (!se[q]) continue;
+                a += st[3 * q];
+                c += st[3 * q + 1];
+                d += st[3 * q + 2];
+            }
+        }
+    }
+    if (threadIdx.x == 0) {
+        cm[0] = a;
+        cm[1] = c;
+        cm[2] = d;
+    }
 }
 
 }  // namespace
@@ -158,7 +217,7 @@ ncg_status ncg_grav_exact_dev(ncg_ctx *ctx, const DBody *d_b, int32_t n, const i
         NCG_CHECK_LAUNCH();
     }
     if (d_cm_acc) {
-        hipLaunchKernelGGL(k_cm_seq, dim3(1), dim3(64), 0, st, d_cterm, (int)n, d_ex, d_cen,
+        hipLaunchKernelGGL(k_cm_seq, dim3(1), dim3(256), 0, st, d_cterm, (int)n, d_ex, d_cen,
                            d_cm_acc);
         NCG_CHECK_LAUNCH();
     }
