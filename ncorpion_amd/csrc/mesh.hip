@@ -514,6 +514,299 @@ __global__ __launch_bounds__(MT) void k_mpairs(const unsigned *__restrict__ skey
     }
 }
 
+// ---- ordered FP64 mesh gravity (NCG_F64_EXACT) ---------------------------------
+// vector_field's mesh branch (physics.c:204-261, 296-380) gives body k > 2, in
+// this order: the central term, bodies 0..2, then its terms of the pair list
+// (physics.c:346-379), which mesh() / get_neighbours_mesh() fill visit by
+// visit in ascending visiting id (collision.c:339-415, physics.c:1104-1116):
+//   (1) visits v < k that registered k: only visits of bodies larger than the
+//       mesh size (a small visit pairs only with bodies already inserted);
+//   (2) its own visit: cells l = 26..0, ascending id < k (small k), or its
+//       recorded list (big k);
+//   (3) visits v > k that registered k, ascending v: the bodies of its 27
+//       cells with id > k (a merge of 27 id-sorted ranges) and the recorded
+//       big visits that list k.
+// A pair's term has the same bits whichever body is `fst` (negating the
+// separation is exact), so each body needs only this order.  Bodies 0..2 and
+// CM_acc are sums in body order (k_mseq).  Big visits: bvk[v] ascending
+// visiting ids, the ids each one paired with in bvo[bvoff[v], bvoff[v + 1]),
+// ascending.
+
+// Is id `x` in the ascending list o[a, e)?
+__device__ inline bool bv_has(const int *__restrict__ o, int a, int e, int x)
+{
+    while (a < e) {
+        const int m = (a + e) >> 1;
+        const int v = o[m];
+        if (v == x) return true;
+        if (v < x) a = m + 1;
+        else e = m;
+    }
+    return false;
+}
+
+// Term of body v (by id) on the body at (X, Y, Z) of radius Rk; pairs with a
+// body 0..2 or an absent body are skipped as in physics.c:351.
+__device__ inline void term_id(const DBody *__restrict__ b, const int32_t *__restrict__ ex, int v,
+                               double X, double Y, double Z, double Rk, double G, double s,
+                               double &ax, double &ay, double &az)
+{
+    if (v <= 2 || (ex && !ex[v])) return;
+    const DBody q = b[v];
+    term_exact(X, Y, Z, Rk, make_double4(q.x, q.y, q.z, q.mass), q.radius, G, s, ax, ay, az);
+}
+
+// Next recorded big visit from index vi on that lists body k (NOKEY if none).
+__device__ inline unsigned next_big(const int *__restrict__ bvk, const int *__restrict__ bvoff,
+                                    const int *__restrict__ bvo, int nbv, int &vi, int k)
+{
+    for (; vi < nbv; ++vi)
+        if (bvk[vi] > 2 && bv_has(bvo, bvoff[vi], bvoff[vi + 1], k)) return (unsigned)bvk[vi];
+    return NOKEY;
+}
+
+// Binned bodies k > 2, one lane per sorted position t.
+__global__ __launch_bounds__(MT) void k_mord(const DBody *__restrict__ b, int n,
+                                             const int32_t *__restrict__ ex, int ins,
+                                             const unsigned *__restrict__ cs,
+                                             const unsigned *__restrict__ ce,
+                                             const double4 *__restrict__ pm,
+                                             const double *__restrict__ rr,
+                                             const int *__restrict__ sx,
+                                             const int *__restrict__ sid,
+                                             const unsigned *__restrict__ skey, int d, int cp,
+                                             double cx, double cy, double cz, double mu, double G,
+                                             double s, const int *__restrict__ bvk,
+                                             const int *__restrict__ bvoff,
+                                             const int *__restrict__ bvo, int nbv,
+                                             double *__restrict__ acc)
+{
+    const int t = blockIdx.x * MT + threadIdx.x;
+    if (t >= ins || !sx[t]) return;
+    const int k = sid[t];
+    const double4 me = pm[t];
+    const double Rk = rr[t];
+    double ax = 0., ay = 0., az = 0., ct[3];
+    if (cp) central_term(me.x, me.y, me.z, me.w, cx, cy, cz, mu, G, ax, ay, az, ct);
+    for (int big = 0; big < 3 && big < n; ++big) {
+        if (ex && !ex[big]) continue;
+        const DBody g = b[big];
+        term_exact(me.x, me.y, me.z, Rk, make_double4(g.x, g.y, g.z, g.mass), g.radius, G, s, ax,
+                   ay, az);
+    }
+    // (1) big visits before k.
+    int vi = 0;
+    for (; vi < nbv && bvk[vi] < k; ++vi)
+        if (bvk[vi] > 2 && bv_has(bvo, bvoff[vi], bvoff[vi + 1], k))
+            term_id(b, ex, bvk[vi], me.x, me.y, me.z, Rk, G, s, ax, ay, az);
+    // (2) own visit.
+    const long kk = skey[t];
+    for (int l = 26; l >= 0; --l) {
+        const unsigned ck = (unsigned)(kk + stencil_offset(l, d));
+        for (unsigned u = cs[ck], e = ce[ck]; u < e; ++u) {
+            if (sid[u] >= k) break;
+            if (sx[u]) term_exact(me.x, me.y, me.z, Rk, pm[u], rr[u], G, s, ax, ay, az);
+        }
+    }
+    // (3) later visits: per cell l the cursor ps[l] at the first id > k, its id
+    // in hd[l] (NOKEY when the range is done).  A cell that appears under two
+    // stencil indexes (only with 2 cells per axis) is registered twice, as in
+    // the reference, and is advanced in step.
+    unsigned hd[27], ps[27], en[27];
+#pragma unroll
+    for (int l = 0; l < 27; ++l) {
+        const unsigned ck = (unsigned)(kk + stencil_offset(l, d));
+        unsigned lo = cs[ck], hi = ce[ck];
+        en[l] = hi;
+        while (lo < hi) {
+            const unsigned mid = (lo + hi) >> 1;
+            if (sid[mid] <= k) lo = mid + 1;
+            else hi = mid;
+        }
+        ps[l] = lo;
+        hd[l] = lo < en[l] ? (unsigned)sid[lo] : NOKEY;
+    }
+    unsigned nb = next_big(bvk, bvoff, bvo, nbv, vi, k);
+    while (true) {
+        unsigned m = hd[0];
+#pragma unroll
+        for (int l = 1; l < 27; ++l) m = min(m, hd[l]);
+        if (nb < m) {
+            term_id(b, ex, (int)nb, me.x, me.y, me.z, Rk, G, s, ax, ay, az);
+            ++vi;
+            nb = next_big(bvk, bvoff, bvo, nbv, vi, k);
+            continue;
+        }
+        if (m == NOKEY) break;
+        unsigned sel = 0, sen = 0;
+        int rep = 0;
+#pragma unroll
+        for (int l = 0; l < 27; ++l)
+            if (hd[l] == m) {
+                sel = ps[l];
+                sen = en[l];
+                ++rep;
+            }
+        if (sx[sel]) {
+            const double4 p = pm[sel];
+            const double Rp = rr[sel];
+            for (int r = 0; r < rep; ++r) term_exact(me.x, me.y, me.z, Rk, p, Rp, G, s, ax, ay, az);
+        }
+        const unsigned nx = sel + 1;
+        const unsigned nh = nx < sen ? (unsigned)sid[nx] : NOKEY;
+#pragma unroll
+        for (int l = 0; l < 27; ++l)
+            if (hd[l] == m) {
+                ps[l] = nx;
+                hd[l] = nh;
+            }
+    }
+    acc[3 * (size_t)k + 0] = ax;
+    acc[3 * (size_t)k + 1] = ay;
+    acc[3 * (size_t)k + 2] = az;
+}
+
+// Existing bodies k > 2 that are not binned (outside the collision cube, or
+// larger than the mesh size in mesh()): central term, bodies 0..2, then the
+// recorded big visits in visiting order (their own one included).
+__global__ __launch_bounds__(MT) void k_mrest_ord(const DBody *__restrict__ b, int n,
+                                                  const int32_t *__restrict__ ex,
+                                                  const unsigned *__restrict__ key, int cp,
+                                                  double cx, double cy, double cz, double mu,
+                                                  double G, double s, const int *__restrict__ bvk,
+                                                  const int *__restrict__ bvoff,
+                                                  const int *__restrict__ bvo, int nbv,
+                                                  double *__restrict__ acc)
+{
+    const int k = 3 + blockIdx.x * MT + threadIdx.x;
+    if (k >= n || (ex && !ex[k]) || key[k] != NOKEY) return;
+    const DBody q = b[k];
+    double ax = 0., ay = 0., az = 0., ct[3];
+    if (cp) central_term(q.x, q.y, q.z, q.mass, cx, cy, cz, mu, G, ax, ay, az, ct);
+    for (int big = 0; big < 3; ++big) {
+        if (ex && !ex[big]) continue;
+        const DBody g = b[big];
+        term_exact(q.x, q.y, q.z, q.radius, make_double4(g.x, g.y, g.z, g.mass), g.radius, G, s, ax,
+                   ay, az);
+    }
+    for (int vi = 0; vi < nbv; ++vi) {
+        const int v = bvk[vi];
+        if (v == k) {
+            for (int u = bvoff[vi]; u < bvoff[vi + 1]; ++u)
+                term_id(b, ex, bvo[u], q.x, q.y, q.z, q.radius, G, s, ax, ay, az);
+        } else if (v > 2 && bv_has(bvo, bvoff[vi], bvoff[vi + 1], k)) {
+            term_id(b, ex, v, q.x, q.y, q.z, q.radius, G, s, ax, ay, az);
+        }
+    }
+    acc[3 * (size_t)k + 0] = ax;
+    acc[3 * (size_t)k + 1] = ay;
+    acc[3 * (size_t)k + 2] = az;
+}
+
+constexpr int SQ_CH = 192;  // bodies per staged chunk of k_mseq
+
+// Bodies 0..2 and CM_acc with the reference's sums in body order
+// (physics.c:121-146, 215-256, 296-341).  Chain j < 9 is component j % 3 of
+// body j / 3: it starts at that body's central term and adds
+// G m_k (X_k - X_j) / D^3 for every existing k > 2 in ascending k; chains
+// 9..11 are CM_acc's components over every existing k.  Waves 1..3 compute the
+// terms of the next chunk of bodies into LDS while lanes 0..11 of wave 0 add
+// the current chunk in order; absent terms are -0.0, which leaves every sum
+// (-0.0 included) unchanged.  Thread 0 then applies the pairs (0,1), (0,2),
+// (1,2).  One block of 256 threads; rows padded by one double so that the 12
+// summing lanes read distinct LDS banks.
+__global__ __launch_bounds__(256) void k_mseq(const DBody *__restrict__ b, int n,
+                                              const int32_t *__restrict__ ex, int cp, double cx,
+                                              double cy, double cz, double mu, double G, double s,
+                                              double *__restrict__ acc, double *__restrict__ cm)
+{
+    __shared__ double T[2][12][SQ_CH + 1];
+    __shared__ double fin[12];
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    bool bex[3];
+    DBody g[3];
+    for (int i = 0; i < 3; ++i) {
+        bex[i] = i < n && (!ex || ex[i]);
+        g[i] = bex[i] ? b[i] : DBody{0, 0, 0, 0, 0, 0, 0, 0};
+    }
+    auto produce = [&](int c, int buf) {
+        const int slot = tid - 64;
+        const int k = c * SQ_CH + slot;
+        double v[12];
+        for (int j = 0; j < 12; ++j) v[j] = -0.0;
+        if (k < n && (!ex || ex[k])) {
+            const DBody q = b[k];
+            if (cp) {
+                const double DX = q.x - cx, DY = q.y - cy, DZ = q.z - cz;
+                const double rk = sqrt(DX * DX + DY * DY + DZ * DZ);
+                const double r3 = rk * rk * rk;
+                v[9] = G * q.mass * DX / r3;
+                v[10] = G * q.mass * DY / r3;
+                v[11] = G * q.mass * DZ / r3;
+            }
+            if (k > 2)
+                for (int i = 0; i < 3; ++i) {
+                    if (!bex[i]) continue;
+                    const double dx = q.x - g[i].x, dy = q.y - g[i].y, dz = q.z - g[i].z;
+                    const double soft = s * (q.radius + g[i].radius);
+                    const double D = sqrt(dx * dx + dy * dy + dz * dz + soft * soft);
+                    const double D3 = D * D * D;
+                    v[3 * i + 0] = G * q.mass * dx / D3;
+                    v[3 * i + 1] = G * q.mass * dy / D3;
+                    v[3 * i + 2] = G * q.mass * dz / D3;
+                }
+        }
+        for (int j = 0; j < 12; ++j) T[buf][j][slot] = v[j];
+    };
+    double sum = 0.0;
+    if (w == 0 && lane < 9) {
+        const int i = lane / 3, c = lane % 3;
+        if (cp && bex[i]) {
+            double a[3], ct[3];
+            central_term(g[i].x, g[i].y, g[i].z, g[i].mass, cx, cy, cz, mu, G, a[0], a[1], a[2], ct);
+            sum = a[c];
+        }
+    }
+    const int nch = (n + SQ_CH - 1) / SQ_CH;
+    if (w > 0) produce(0, 0);
+    __syncthreads();
+    for (int c = 0; c < nch; ++c) {
+        if (w > 0) {
+            if (c + 1 < nch) produce(c + 1, (c + 1) & 1);
+        } else if (lane < 12) {
+            const double *row = T[c & 1][lane];
+#pragma unroll 16
+            for (int i = 0; i < SQ_CH; ++i) sum += row[i];
+        }
+        __syncthreads();
+    }
+    if (w == 0 && lane < 12) fin[lane] = sum;
+    __syncthreads();
+    if (tid != 0) return;
+    double a[9];
+    for (int j = 0; j < 9; ++j) a[j] = fin[j];
+    const int pa[3] = {0, 0, 1}, pb[3] = {1, 2, 2};
+    for (int q = 0; q < 3; ++q) {
+        const int i = pa[q], j = pb[q];
+        if (!bex[i] || !bex[j]) continue;
+        const double dx = g[i].x - g[j].x, dy = g[i].y - g[j].y, dz = g[i].z - g[j].z;
+        const double soft = s * (g[i].radius + g[j].radius);
+        const double D = sqrt(dx * dx + dy * dy + dz * dz + soft * soft);
+        const double D3 = D * D * D;
+        a[3 * i + 0] -= G * g[j].mass * dx / D3;
+        a[3 * i + 1] -= G * g[j].mass * dy / D3;
+        a[3 * i + 2] -= G * g[j].mass * dz / D3;
+        a[3 * j + 0] += G * g[i].mass * dx / D3;
+        a[3 * j + 1] += G * g[i].mass * dy / D3;
+        a[3 * j + 2] += G * g[i].mass * dz / D3;
+    }
+    for (int i = 0; i < 3; ++i)
+        if (bex[i])
+            for (int c = 0; c < 3; ++c) acc[3 * i + c] = a[3 * i + c];
+    if (cp && cm)
+        for (int c = 0; c < 3; ++c) cm[c] = fin[9 + c];
+}
+
 }  // namespace
 
 // ---- host side --------------------------------------------------------------
@@ -1014,14 +1307,56 @@ extern "C" ncg_status ncg_mesh_gravity(ncg_ctx *ctx, const ncg_body *bodies, int
     const double G = grav_const, s = softening;
     const double mu = cp ? G * central->mass : 0.0;
     const double cxh = cp ? central->x : 0.0, cyh = cp ? central->y : 0.0, czh = cp ? central->z : 0.0;
-    {
+    if (prec == NCG_F64_EXACT) {
+        // Pair-list order, bit for bit: recorded big visits as (visiting id,
+        // ascending partner ids), bodies 0..2 and CM_acc on a second stream.
+        std::vector<int> hbk, hbo(1, 0), hbp;
+        for (const BigVisit &bv : M.big_visits) {
+            hbk.push_back(bv.k);
+            for (int q = 0; q < bv.count; ++q) {
+                const ncg_pair &pq = M.big_pairs_all[bv.first + q];
+                hbp.push_back(pq.fst == bv.k ? pq.snd : pq.fst);
+            }
+            hbo.push_back((int)hbp.size());
+        }
+        const int nbv = (int)hbk.size();
+        int *bvk = dbuf<int>(ctx, "g_bvk", std::max(nbv, 1));
+        int *bvoff = dbuf<int>(ctx, "g_bvoff", nbv + 1);
+        int *bvo = dbuf<int>(ctx, "g_bvo", std::max<size_t>(hbp.size(), 1));
+        if (!bvk || !bvoff || !bvo) return NCG_ERR_NOMEM;
+        if (nbv) {
+            NCG_HIP(hipMemcpyAsync(bvk, hbk.data(), sizeof(int) * nbv, hipMemcpyHostToDevice, st));
+            NCG_HIP(hipMemcpyAsync(bvoff, hbo.data(), sizeof(int) * (nbv + 1), hipMemcpyHostToDevice, st));
+            if (!hbp.empty())
+                NCG_HIP(hipMemcpyAsync(bvo, hbp.data(), sizeof(int) * hbp.size(),
+                                       hipMemcpyHostToDevice, st));
+        }
+        if (!ctx->aux) {
+            NCG_HIP(hipStreamCreate(&ctx->aux));
+            NCG_HIP(hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
+            NCG_HIP(hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
+        }
+        ProfScope ps(ctx, st, "mesh_gravity");
+        NCG_HIP(hipEventRecord(ctx->ev_fork, st));
+        NCG_HIP(hipStreamWaitEvent(ctx->aux, ctx->ev_fork, 0));
+        hipLaunchKernelGGL(k_mseq, dim3(1), dim3(256), 0, ctx->aux, db, (int)n, dex, cp, cxh, cyh,
+                           czh, mu, G, s, dacc, dcm);
+        NCG_CHECK_LAUNCH();
+        NCG_HIP(hipEventRecord(ctx->ev_join, ctx->aux));
+        if (ins > 0)
+            hipLaunchKernelGGL(k_mord, dim3(cdiv(ins, MT)), dim3(MT), 0, st, db, (int)n, dex, ins,
+                               cs, ce, pm, rr, sx, sid, (const unsigned *)dbuf<unsigned>(ctx, "m_skey", n),
+                               (int)d, cp, cxh, cyh, czh, mu, G, s, bvk, bvoff, bvo, nbv, dacc);
+        if (n > 3)
+            hipLaunchKernelGGL(k_mrest_ord, dim3(cdiv(n - 3, MT)), dim3(MT), 0, st, db, (int)n, dex,
+                               key, cp, cxh, cyh, czh, mu, G, s, bvk, bvoff, bvo, nbv, dacc);
+        NCG_CHECK_LAUNCH();
+        NCG_HIP(hipStreamWaitEvent(st, ctx->ev_join, 0));
+    } else {
         ProfScope ps(ctx, st, "mesh_gravity");
         const unsigned *skey = dbuf<unsigned>(ctx, "m_skey", n);
         const dim3 gs(cdiv(std::max(ins, 1), MT));
-        if (prec == NCG_F64_EXACT)
-            hipLaunchKernelGGL(k_mgrav<0>, gs, dim3(MT), 0, st, db, (int)n, dex, ins, cs, ce, pm, rr,
-                               sx, sid, skey, (int)d, cp, cxh, cyh, czh, mu, G, s, dacc, dct);
-        else if (prec == NCG_F64_FAST)
+        if (prec == NCG_F64_FAST)
             hipLaunchKernelGGL(k_mgrav<1>, gs, dim3(MT), 0, st, db, (int)n, dex, ins, cs, ce, pm, rr,
                                sx, sid, skey, (int)d, cp, cxh, cyh, czh, mu, G, s, dacc, dct);
         else
@@ -1030,14 +1365,14 @@ extern "C" ncg_status ncg_mesh_gravity(ncg_ctx *ctx, const ncg_body *bodies, int
         hipLaunchKernelGGL(k_mgrav_rest, dim3(cdiv(n, MT)), dim3(MT), 0, st, db, (int)n, dex, key, cp,
                            cxh, cyh, czh, mu, G, s, dacc, dct);
         NCG_CHECK_LAUNCH();
+        hipLaunchKernelGGL(k_big_partial, dim3(nbp), dim3(MT), 0, st, db, (int)n, dex, G, s, bpart);
+        hipLaunchKernelGGL(k_big_final, dim3(1), dim3(MT), 0, st, db, (int)n, dex, bpart, nbp, G, s,
+                           dacc, dct, cp, dcm);
+        if (!M.extra_pairs.empty())
+            hipLaunchKernelGGL(k_extra_pairs, dim3(1), dim3(1), 0, st, db, dex, dextra,
+                               (int)M.extra_pairs.size(), G, s, dacc);
+        NCG_CHECK_LAUNCH();
     }
-    hipLaunchKernelGGL(k_big_partial, dim3(nbp), dim3(MT), 0, st, db, (int)n, dex, G, s, bpart);
-    hipLaunchKernelGGL(k_big_final, dim3(1), dim3(MT), 0, st, db, (int)n, dex, bpart, nbp, G, s,
-                       dacc, dct, cp, dcm);
-    if (!M.extra_pairs.empty())
-        hipLaunchKernelGGL(k_extra_pairs, dim3(1), dim3(1), 0, st, db, dex, dextra,
-                           (int)M.extra_pairs.size(), G, s, dacc);
-    NCG_CHECK_LAUNCH();
     double *hacc = hbuf<double>(ctx, "macc", 3 * (size_t)n);
     double hcm[4] = {0, 0, 0, 0};
     NCG_HIP(hipMemcpyAsync(hacc, dacc, sizeof(double) * 3 * (size_t)n, hipMemcpyDeviceToHost, st));

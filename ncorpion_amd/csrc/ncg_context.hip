@@ -27,8 +27,8 @@ void *ncg_dbuf(ncg_ctx *ctx, const char *name, size_t bytes)
     if (b.bytes >= bytes && b.ptr) return b.ptr;
     if (b.ptr) {
         // The old contents may still be read by queued work on the stream.
-        hipStreamSynchronize(ctx->stream);
-        hipFree(b.ptr);
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipFree(b.ptr);
         b.ptr = nullptr;
         b.bytes = 0;
     }
@@ -48,8 +48,8 @@ void *ncg_pbuf(ncg_ctx *ctx, const char *name, size_t bytes)
     DevBuf &b = ctx->pbufs[name];
     if (b.bytes >= bytes && b.ptr) return b.ptr;
     if (b.ptr) {
-        hipStreamSynchronize(ctx->stream);
-        hipHostFree(b.ptr);
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipHostFree(b.ptr);
         b.ptr = nullptr;
         b.bytes = 0;
     }
@@ -163,6 +163,12 @@ void ncg_destroy(ncg_ctx *ctx)
         if (kv.second.ptr) (void)hipFree(kv.second.ptr);
     for (auto &kv : ctx->pbufs)
         if (kv.second.ptr) (void)hipHostFree(kv.second.ptr);
+    if (ctx->aux) {
+        (void)hipStreamSynchronize(ctx->aux);
+        (void)hipStreamDestroy(ctx->aux);
+    }
+    if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+    if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }

@@ -72,6 +72,10 @@ struct TuneRow {
 struct ncg_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    // Second stream for work that runs beside the main stream within one call
+    // (forked and joined with the two events); created on first use.
+    hipStream_t aux = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     std::string err;
     std::unordered_map<std::string, DevBuf> bufs;
     std::unordered_map<std::string, std::vector<unsigned char>> hbufs;

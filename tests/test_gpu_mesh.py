@@ -62,20 +62,22 @@ def test_mesh_gravity_matches_reference(ctx):
     c8 = g["central"]
     cen = (c8[0], c8[1], c8[2], c8[6])
     ex = g["exists"].astype(bool)
-    for prec, tol in (("exact", 1e-13), ("fast64", 1e-12)):
-        acc, cm = ctx.mesh_gravity(g["bodies"], g["exists"], cen, float(g["G"]), float(g["soft"]),
-                                   prec)
-        ref = g["acc"]
-        err = np.abs(acc[ex] - ref[ex]).max() / np.abs(ref[ex]).max()
-        assert err < tol, (prec, err)
-        assert np.allclose(cm, g["cm_acc"], rtol=1e-12, atol=0)
+    # The exact form sums in the reference's pair-list order: bit for bit.
+    acc, cm = ctx.mesh_gravity(g["bodies"], g["exists"], cen, float(g["G"]), float(g["soft"]),
+                               "exact")
+    assert np.array_equal(bits(acc[ex]), bits(g["acc"][ex]))
+    assert np.array_equal(bits(cm), bits(g["cm_acc"]))
+    acc, cm = ctx.mesh_gravity(g["bodies"], g["exists"], cen, float(g["G"]), float(g["soft"]),
+                               "fast64")
+    ref = g["acc"]
+    assert np.abs(acc[ex] - ref[ex]).max() / np.abs(ref[ex]).max() < 1e-12
+    assert np.allclose(cm, g["cm_acc"], rtol=1e-12, atol=0)
     # Mutual part alone against the oracle with the reference pair list.
     accn, _ = ctx.mesh_gravity(g["bodies"], g["exists"], None, float(g["G"]), float(g["soft"]),
                                "exact")
     refn, _ = orc.vector_field_mesh(g["bodies"], g["pairs"], g["exists"], None, float(g["G"]),
                                     float(g["soft"]))
-    d = np.linalg.norm(accn[ex] - refn[ex], axis=1)
-    assert np.max(d / np.linalg.norm(refn[ex], axis=1)) < 1e-12
+    assert np.array_equal(bits(accn[ex]), bits(refn[ex]))
     # Mixed form (FP32 forces, per-cell FP32 sums into FP64) on the mutual part:
     # every neighbour term is a near one, so the FP32 rounding of each term
     # (~1e-7) is amplified by the cancellation of the neighbour sum.
@@ -139,3 +141,57 @@ def test_mesh_pass_empty_and_outside(ctx):
     assert st["total_neighbours"] == tn
     st = ctx.mesh_pass(np.zeros((0, 8)), None, 0.01, 0.01, 0.16, 16, 1)
     assert st["total_neighbours"] == 0
+
+
+@pytest.mark.parametrize("seed,mode,nbig,holes,cen,soft", [
+    (11, 0, 0, False, True, 0.0), (12, 0, 5, True, True, 0.3), (13, 1, 0, True, False, 0.1),
+    (14, 0, 3, False, False, 0.0), (15, 1, 2, False, True, 0.0)])
+def test_mesh_gravity_exact_in_pair_list_order(ctx, seed, mode, nbig, holes, cen, soft):
+    """Exact mesh gravity = vector_field's mesh branch over the pass's own pair
+    list, bit for bit: bodies larger than the mesh size (their visits pair
+    them with every body), bodies outside the collision cube, holes (body 0
+    absent too), collisions during mesh(), with and without central body."""
+    n, cells, gam = 5000, 32, 0.004
+    cube = cells * gam  # the blob reaches past the cube: some bodies are not binned
+    b = blob(n, seed)
+    rng = np.random.default_rng(seed)
+    if nbig:
+        big = rng.choice(np.arange(1, n), nbig, replace=False)
+        big[0] = 1  # a big body among 0..2 too
+        b[big, 7] = rng.uniform(1.5 * gam, 3 * gam, nbig)
+    ex = np.ones(n, np.int32)
+    if holes:
+        ex[rng.choice(n, 200, replace=False)] = 0
+        ex[0] = 0
+    if mode == 0:
+        r = orc.Replay(b, 0.01, 1, 1.5, 0, ex)
+        ctx.mesh_pass(r.b, ex, 0.01, gam, cube, cells, 0, r.test, r.resolve)
+        bb = r.b
+        assert len(r.log) > 5
+    else:
+        bb = b
+        ctx.mesh_pass(bb, ex, 0.0, gam, cube, cells, 1)
+    pairs = ctx.mesh_pairs()
+    # The field is evaluated after a drift, as in the integrator.
+    b2 = bb.copy()
+    b2[:, 0:3] += 0.002 * b2[:, 3:6]
+    central = (0.01, -0.02, 0.005, 1.0) if cen else None
+    acc, cm = ctx.mesh_gravity(b2, ex, central, 1.0, soft, "exact")
+    ref, rcm = orc.vector_field_mesh(b2, pairs, ex, central, 1.0, soft)
+    live = ex != 0
+    assert np.array_equal(bits(acc[live]), bits(ref[live]))
+    if cen:
+        assert np.array_equal(bits(cm), bits(rcm))
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 7])
+def test_mesh_gravity_exact_few_bodies(ctx, n):
+    b = blob(n, 20 + n, half=0.01)
+    ex = np.ones(n, np.int32)
+    ctx.mesh_pass(b, ex, 0.0, 0.004, 0.128, 32, 1)
+    pairs = ctx.mesh_pairs()
+    central = (0.0, 0.0, 0.0, 1.0)
+    acc, cm = ctx.mesh_gravity(b, ex, central, 1.0, 0.05, "exact")
+    ref, rcm = orc.vector_field_mesh(b, pairs, ex, central, 1.0, 0.05)
+    assert np.array_equal(bits(acc), bits(ref))
+    assert np.array_equal(bits(cm), bits(rcm))
