@@ -103,7 +103,7 @@ def mesh_run(ctx, torch, passes=3):
     # Warm-up: buffers, dense cell table.
     ctx.mesh_pass(b0.copy(), ex, dt, MESH_GAM, MESH_CUBE, MESH_CELLS, 1)
     ctx.profile(True)
-    for k in ("mesh_bin", "mesh_count", "mesh_gravity"):
+    for k in ("mesh_bin", "mesh_count", "mesh_gravity", "mesh_lists"):
         ctx.profile_read(k, reset=True)
     b = b0.copy()
     t0 = time.perf_counter()
@@ -148,6 +148,9 @@ def mesh_run(ctx, torch, passes=3):
              "pair_terms_per_s": 2.0 * st["total_neighbours"] / (ms * 1e-3)}
         if prec == "exact":
             ref = acc
+            lms, lk = ctx.profile_read("mesh_lists", reset=True)
+            g["order"] = "reference pair-list order (bit for bit)"
+            g["lists_ms_per_pass"] = lms / max(lk, 1)
         else:
             d = np.linalg.norm(acc - ref, axis=1)
             r = np.linalg.norm(ref, axis=1)
@@ -159,19 +162,18 @@ def mesh_run(ctx, torch, passes=3):
     return out
 
 
-def cpu_mesh_baseline(timeout=300):
-    """NcorpiON's own get_neighbours_mesh (compiled from source, 1 thread) on
-    the full 2^20-body mesh workload, through the function-level probe."""
+def _probe_mesh(op, b, timeout):
+    """Runs the function-level probe (mesh configuration of the 2^20-body
+    workload) with `op` on bodies b; returns (seconds, header, CM_acc, bodies)."""
     import numpy as np
     probe = os.path.join(ROOT, "oracle", "_ref", "probe_c2", "ref_probe")
     if not os.path.exists(probe):
         return None
-    b = mesh_workload()
     n = b.shape[0]
     with tempfile.TemporaryDirectory() as td:
         fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
         with open(fin, "wb") as f:
-            f.write(struct.pack("<4i", 0x4E434750, 3 | 0x100, n, 0))
+            f.write(struct.pack("<4i", 0x4E434750, op, n, 0))
             f.write(np.array([2.0 * math.pi / 512.0, MESH_GAM, MESH_CUBE, 0, 0, 0, 0, 0]).tobytes())
             f.write(np.array([0, 0, 0, 0, 0, 0, 1.0, 0.05]).tobytes())
             f.write(b.tobytes())
@@ -183,10 +185,47 @@ def cpu_mesh_baseline(timeout=300):
             return None
         secs = float(r.stderr.strip().split("op_seconds=")[-1].split()[0])
         hdr = np.fromfile(fout, dtype=np.int32, count=8)
-    return {"value": 1.0 / secs, "unit": "get_neighbours_mesh passes/s", "cores": 1,
-            "kind": "reference", "seconds": secs, "total_neighbours": int(hdr[4]),
-            "sample": "NcorpiON get_neighbours_mesh (gcc -O2, 1 thread), one call on the full "
-                      "2^20-body mesh workload"}
+        cm = np.fromfile(fout, dtype=np.float64, count=3, offset=32)
+        bodies = None
+        if not op & 0x100:
+            bodies = np.fromfile(fout, dtype=np.float64, count=8 * n, offset=64).reshape(n, 8)
+    return secs, hdr, cm, bodies
+
+
+def cpu_mesh_baseline(ctx=None, timeout=300):
+    """NcorpiON's own get_neighbours_mesh and vector_field (mesh branch over
+    the pairs it registered), compiled from source, 1 thread, on the full
+    2^20-body mesh workload through the function-level probe.  With a
+    context, the device's exact mesh field on the same state is compared with
+    the reference's bit for bit."""
+    import numpy as np
+    b = mesh_workload()
+    n = b.shape[0]
+    g = _probe_mesh(3 | 0x100, b, timeout)
+    if g is None:
+        return None
+    secs, hdr, _, _ = g
+    out = {"value": 1.0 / secs, "unit": "get_neighbours_mesh passes/s", "cores": 1,
+           "kind": "reference", "seconds": secs, "total_neighbours": int(hdr[4]),
+           "sample": "NcorpiON get_neighbours_mesh (gcc -O2, 1 thread), one call on the full "
+                     "2^20-body mesh workload"}
+    v = _probe_mesh(5, b, timeout)
+    if v is not None:
+        vsecs, _, cm, ob = v
+        out["vector_field"] = {
+            "seconds": vsecs, "pair_terms_per_s": 2.0 * int(hdr[4]) / vsecs,
+            "sample": "NcorpiON vector_field, mesh branch over the pairs of its own "
+                      "get_neighbours_mesh, central mass 1 (gcc -O2, 1 thread)"}
+        if ctx is not None:
+            ex = np.ones(n, np.int32)
+            ctx.mesh_pass(b.copy(), ex, 0.0, MESH_GAM, MESH_CUBE, MESH_CELLS, 1)
+            acc, gcm = ctx.mesh_gravity(b, ex, (0.0, 0.0, 0.0, 1.0), 1.0, 0.0, "exact")
+            bad = np.any(acc.view(np.uint64) != np.ascontiguousarray(ob[:, 3:6]).view(np.uint64),
+                         axis=1)
+            out["vector_field"]["gpu_exact_bodies_differing"] = int(bad.sum())
+            out["vector_field"]["gpu_exact_cm_acc_identical"] = bool(
+                np.array_equal(gcm.view(np.uint64), cm.view(np.uint64)))
+    return out
 
 
 C1_N, C1_STEPS = 2000, 200
@@ -527,7 +566,7 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(b, args.cpu_sample)
         if out["cpu_baseline"] is not None and not args.no_mesh:
-            out["cpu_baseline"]["mesh"] = cpu_mesh_baseline()
+            out["cpu_baseline"]["mesh"] = cpu_mesh_baseline(ctx)
         if out["cpu_baseline"] is not None and not args.no_leapfrog:
             out["cpu_baseline"]["c1_run"] = cpu_c1_baseline()
     else:

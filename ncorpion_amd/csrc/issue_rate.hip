@@ -7,8 +7,13 @@
 //     3 for r^2, v_rsq_f32, 3 multiplies, 3 FMAs), j records read with
 //     wave-uniform addresses from a 128-entry tile as in the kernel.
 //
+//   chain64 / chain32: one wavefront, one dependent chain of adds (the
+//     body-order sums of the exact mesh field are such chains): the
+//     latency of a dependent v_add_f64 / v_add_f32.
+//
 // Output: wave-instructions per CU per clock (at the reported clock) and,
-// for `far`, interactions per second: the ceiling of the far-field loop.
+// for `far`, interactions per second: the ceiling of the far-field loop;
+// for the chains, clocks per dependent add.
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -115,6 +120,17 @@ __global__ __launch_bounds__(WG) void k_far(const float4 *__restrict__ ju, int i
     out[blockIdx.x * WG + threadIdx.x] = fx + fy + fz;
 }
 
+template <class T>
+__global__ __launch_bounds__(64) void k_chain(T *out, int iters, T b)
+{
+    T a = (T)threadIdx.x;
+    for (int it = 0; it < iters; ++it) {
+#pragma unroll
+        for (int q = 0; q < 32; ++q) a = a + b;
+    }
+    out[threadIdx.x] = a;
+}
+
 }  // namespace
 
 int main()
@@ -178,6 +194,25 @@ int main()
             printf("  %.3e interactions/s (%.1f TFLOP/s at 20 flop)", inter, 20.0 * inter / 1e12);
         }
         printf("\n");
+    }
+    for (int kind = 0; kind < 2; ++kind) {
+        const int citers = 1 << 15;
+        auto launch = [&]() {
+            if (kind == 0) hipLaunchKernelGGL(k_chain<double>, dim3(1), dim3(64), 0, 0, o64, citers, 1e-3);
+            else hipLaunchKernelGGL(k_chain<float>, dim3(1), dim3(64), 0, 0, o32, citers, 1e-3f);
+        };
+        launch();
+        CK(hipGetLastError());
+        CK(hipDeviceSynchronize());
+        CK(hipEventRecord(e0));
+        launch();
+        CK(hipEventRecord(e1));
+        CK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        CK(hipEventElapsedTime(&ms, e0, e1));
+        const double adds = 32.0 * citers;
+        printf("%-12s %8.3f ms  %.3f ns per dependent add  %.2f clocks\n", kind ? "chain32" : "chain64",
+               ms, ms * 1e6 / adds, ms * 1e-3 / adds * ghz * 1e9);
     }
     CK(hipFree(o32));
     CK(hipFree(o64));

@@ -321,9 +321,10 @@ __global__ __launch_bounds__(MT) void k_mgrav_rest(const DBody *__restrict__ b, 
 // Binned bodies k > 2, one lane per sorted position t (bodies of one cell
 // are adjacent lanes and walk the same 27 cells).  Per body: central term,
 // bodies 0..2, then the neighbour terms cell l = 0..26, ascending id within
-// a cell.  FORM 0: the reference's expression; 1: FP64 with hardware 1/sqrt
-// and a Newton step; 2: separations in FP64, 1/r^3 and forces in FP32,
-// summed in FP32 per neighbour cell and added to the FP64 total.
+// a cell (cell order: accelerations agree with the reference's pair-list
+// order to rounding).  FORM 1: FP64 with hardware 1/sqrt and a Newton step;
+// 2: separations in FP64, 1/r^3 and forces in FP32, summed in FP32 per
+// neighbour cell and added to the FP64 total.
 template <int FORM>
 __global__ __launch_bounds__(MT) void k_mgrav(const DBody *__restrict__ b, int n,
                                               const int32_t *__restrict__ ex, int ins,
@@ -376,10 +377,7 @@ __global__ __launch_bounds__(MT) void k_mgrav(const DBody *__restrict__ b, int n
         } else {
             for (unsigned u = a; u < e; ++u) {
                 if (!sx[u] || u == (unsigned)t) continue;
-                if (FORM == 1)
-                    term_fast(me.x, me.y, me.z, Rk, pm[u], rr[u], G, s, ax, ay, az);
-                else
-                    term_exact(me.x, me.y, me.z, Rk, pm[u], rr[u], G, s, ax, ay, az);
+                term_fast(me.x, me.y, me.z, Rk, pm[u], rr[u], G, s, ax, ay, az);
             }
         }
     }
@@ -565,6 +563,87 @@ __device__ inline unsigned next_big(const int *__restrict__ bvk, const int *__re
     return NOKEY;
 }
 
+// Id-ordered neighbourhood lists, one per occupied cell c (ukey[i], i < *nu):
+// the sorted positions of every body in c's 27 cells, ascending id (a cell
+// that appears under two stencil indexes, only with 2 cells per axis, is
+// listed twice, as the reference registers it twice).  For a body of c these
+// are, after its own entry, the later visits that register it.  usz[i]: list
+// length (0 for i >= *nu, up to ins).
+__global__ __launch_bounds__(MT) void k_usize(const unsigned *__restrict__ ukey,
+                                              const int *__restrict__ nu, int ins,
+                                              const unsigned *__restrict__ cs,
+                                              const unsigned *__restrict__ ce, int d,
+                                              unsigned *__restrict__ usz)
+{
+    const int i = blockIdx.x * MT + threadIdx.x;
+    if (i > ins) return;
+    unsigned z = 0;
+    if (i < *nu) {
+        const long kk = ukey[i];
+        for (int l = 0; l < 27; ++l) {
+            const unsigned ck = (unsigned)(kk + stencil_offset(l, d));
+            z += ce[ck] - cs[ck];
+        }
+    }
+    usz[i] = z;
+}
+
+// One lane per occupied cell: 27-way merge by id into L[uoff[i], uoff[i+1]);
+// each body of the cell gets lw = index of its own (last) entry, le = end.
+__global__ __launch_bounds__(MT) void k_umerge(const unsigned *__restrict__ ukey,
+                                               const int *__restrict__ nu,
+                                               const unsigned *__restrict__ uoff,
+                                               const unsigned *__restrict__ cs,
+                                               const unsigned *__restrict__ ce,
+                                               const int *__restrict__ sid, int d,
+                                               unsigned *__restrict__ L, unsigned *__restrict__ lw,
+                                               unsigned *__restrict__ le)
+{
+    const int i = blockIdx.x * MT + threadIdx.x;
+    if (i >= *nu) return;
+    const long kk = ukey[i];
+    unsigned o = uoff[i];
+    const unsigned oe = uoff[i + 1];
+    unsigned hd[27], ps[27], en[27];
+#pragma unroll
+    for (int l = 0; l < 27; ++l) {
+        const unsigned ck = (unsigned)(kk + stencil_offset(l, d));
+        ps[l] = cs[ck];
+        en[l] = ce[ck];
+        hd[l] = ps[l] < en[l] ? (unsigned)sid[ps[l]] : NOKEY;
+    }
+    while (true) {
+        unsigned m = hd[0];
+#pragma unroll
+        for (int l = 1; l < 27; ++l) m = min(m, hd[l]);
+        if (m == NOKEY) break;
+        unsigned sel = 0, sen = 0;
+        int rep = 0;
+        bool own = false;
+#pragma unroll
+        for (int l = 0; l < 27; ++l)
+            if (hd[l] == m) {
+                sel = ps[l];
+                sen = en[l];
+                ++rep;
+                own |= l == 13;
+            }
+        for (int r = 0; r < rep; ++r) L[o++] = sel;
+        if (own) {
+            lw[sel] = o - 1;
+            le[sel] = oe;
+        }
+        const unsigned nx = sel + 1;
+        const unsigned nh = nx < sen ? (unsigned)sid[nx] : NOKEY;
+#pragma unroll
+        for (int l = 0; l < 27; ++l)
+            if (hd[l] == m) {
+                ps[l] = nx;
+                hd[l] = nh;
+            }
+    }
+}
+
 // Binned bodies k > 2, one lane per sorted position t.
 __global__ __launch_bounds__(MT) void k_mord(const DBody *__restrict__ b, int n,
                                              const int32_t *__restrict__ ex, int ins,
@@ -579,6 +658,9 @@ __global__ __launch_bounds__(MT) void k_mord(const DBody *__restrict__ b, int n,
                                              double s, const int *__restrict__ bvk,
                                              const int *__restrict__ bvoff,
                                              const int *__restrict__ bvo, int nbv,
+                                             const unsigned *__restrict__ L,
+                                             const unsigned *__restrict__ lw,
+                                             const unsigned *__restrict__ le,
                                              double *__restrict__ acc)
 {
     const int t = blockIdx.x * MT + threadIdx.x;
@@ -599,67 +681,48 @@ __global__ __launch_bounds__(MT) void k_mord(const DBody *__restrict__ b, int n,
     for (; vi < nbv && bvk[vi] < k; ++vi)
         if (bvk[vi] > 2 && bv_has(bvo, bvoff[vi], bvoff[vi + 1], k))
             term_id(b, ex, bvk[vi], me.x, me.y, me.z, Rk, G, s, ax, ay, az);
-    // (2) own visit.
+    // The lanes of one cell walk the same positions in step (so that their
+    // loads coincide), each lane adding only the terms that are its own.
     const long kk = skey[t];
+    // (2) own visit: cells l = 26..0, ascending id < k; the walk of each cell
+    // stops at the largest id of k's cell among this wavefront's lanes.
+    const unsigned t1 = min(ce[(unsigned)kk] - 1u, (unsigned)(t | 63));
+    const int kmax = sid[t1];
     for (int l = 26; l >= 0; --l) {
         const unsigned ck = (unsigned)(kk + stencil_offset(l, d));
-        for (unsigned u = cs[ck], e = ce[ck]; u < e; ++u) {
-            if (sid[u] >= k) break;
-            if (sx[u]) term_exact(me.x, me.y, me.z, Rk, pm[u], rr[u], G, s, ax, ay, az);
-        }
-    }
-    // (3) later visits: per cell l the cursor ps[l] at the first id > k, its id
-    // in hd[l] (NOKEY when the range is done).  A cell that appears under two
-    // stencil indexes (only with 2 cells per axis) is registered twice, as in
-    // the reference, and is advanced in step.
-    unsigned hd[27], ps[27], en[27];
-#pragma unroll
-    for (int l = 0; l < 27; ++l) {
-        const unsigned ck = (unsigned)(kk + stencil_offset(l, d));
-        unsigned lo = cs[ck], hi = ce[ck];
-        en[l] = hi;
+        const unsigned a = cs[ck];
+        unsigned lo = a, hi = ce[ck];
         while (lo < hi) {
             const unsigned mid = (lo + hi) >> 1;
-            if (sid[mid] <= k) lo = mid + 1;
+            if (sid[mid] < kmax) lo = mid + 1;
             else hi = mid;
         }
-        ps[l] = lo;
-        hd[l] = lo < en[l] ? (unsigned)sid[lo] : NOKEY;
+        for (unsigned u = a; u < lo; ++u)
+            if (sid[u] < k && sx[u]) term_exact(me.x, me.y, me.z, Rk, pm[u], rr[u], G, s, ax, ay, az);
     }
+    // (3) later visits: the cell's id-ordered neighbourhood list after k's own
+    // entry, merged with the recorded big visits that list k; the walk starts
+    // after the entry of the cell's first body in this wavefront.
+    const unsigned t0 = max(cs[(unsigned)kk], (unsigned)(t & ~63));
+    const unsigned mine = lw[t];
     unsigned nb = next_big(bvk, bvoff, bvo, nbv, vi, k);
-    while (true) {
-        unsigned m = hd[0];
-#pragma unroll
-        for (int l = 1; l < 27; ++l) m = min(m, hd[l]);
-        if (nb < m) {
-            term_id(b, ex, (int)nb, me.x, me.y, me.z, Rk, G, s, ax, ay, az);
-            ++vi;
-            nb = next_big(bvk, bvoff, bvo, nbv, vi, k);
-            continue;
-        }
-        if (m == NOKEY) break;
-        unsigned sel = 0, sen = 0;
-        int rep = 0;
-#pragma unroll
-        for (int l = 0; l < 27; ++l)
-            if (hd[l] == m) {
-                sel = ps[l];
-                sen = en[l];
-                ++rep;
+    for (unsigned j = lw[t0] + 1, e = le[t]; j < e; ++j) {
+        if (j <= mine) continue;
+        const unsigned u = L[j];
+        if (nb != NOKEY) {
+            const unsigned id = (unsigned)sid[u];
+            while (nb < id) {
+                term_id(b, ex, (int)nb, me.x, me.y, me.z, Rk, G, s, ax, ay, az);
+                ++vi;
+                nb = next_big(bvk, bvoff, bvo, nbv, vi, k);
             }
-        if (sx[sel]) {
-            const double4 p = pm[sel];
-            const double Rp = rr[sel];
-            for (int r = 0; r < rep; ++r) term_exact(me.x, me.y, me.z, Rk, p, Rp, G, s, ax, ay, az);
         }
-        const unsigned nx = sel + 1;
-        const unsigned nh = nx < sen ? (unsigned)sid[nx] : NOKEY;
-#pragma unroll
-        for (int l = 0; l < 27; ++l)
-            if (hd[l] == m) {
-                ps[l] = nx;
-                hd[l] = nh;
-            }
+        if (sx[u]) term_exact(me.x, me.y, me.z, Rk, pm[u], rr[u], G, s, ax, ay, az);
+    }
+    while (nb != NOKEY) {
+        term_id(b, ex, (int)nb, me.x, me.y, me.z, Rk, G, s, ax, ay, az);
+        ++vi;
+        nb = next_big(bvk, bvoff, bvo, nbv, vi, k);
     }
     acc[3 * (size_t)k + 0] = ax;
     acc[3 * (size_t)k + 1] = ay;
@@ -703,84 +766,186 @@ __global__ __launch_bounds__(MT) void k_mrest_ord(const DBody *__restrict__ b, i
     acc[3 * (size_t)k + 2] = az;
 }
 
-constexpr int SQ_CH = 192;  // bodies per staged chunk of k_mseq
+constexpr int SQ_CH = 768;  // bodies per chunk of the body-order sums
 
-// Bodies 0..2 and CM_acc with the reference's sums in body order
-// (physics.c:121-146, 215-256, 296-341).  Chain j < 9 is component j % 3 of
-// body j / 3: it starts at that body's central term and adds
-// G m_k (X_k - X_j) / D^3 for every existing k > 2 in ascending k; chains
-// 9..11 are CM_acc's components over every existing k.  Waves 1..3 compute the
-// terms of the next chunk of bodies into LDS while lanes 0..11 of wave 0 add
-// the current chunk in order; absent terms are -0.0, which leaves every sum
-// (-0.0 included) unchanged.  Thread 0 then applies the pairs (0,1), (0,2),
-// (1,2).  One block of 256 threads; rows padded by one double so that the 12
-// summing lanes read distinct LDS banks.
+// Terms of the body-order sums (physics.c:121-146, 215-256), one thread per
+// body slot k < nch * SQ_CH: chain j < 9 is component j % 3 of body j / 3 and
+// receives G m_k (X_k - X_j) / D^3 from every existing k > 2; chains 9..11
+// are CM_acc's components, G m_k (X_k - X_c) / r^3 from every existing k.
+// Absent terms are -0.0, which leaves every sum (-0.0 included) unchanged.
+// Layout: chunk c is a [12][SQ_CH] block, so that each chain's terms are
+// contiguous.
+__global__ __launch_bounds__(MT) void k_mterms(const DBody *__restrict__ b, int n,
+                                               const int32_t *__restrict__ ex, int cp, double cx,
+                                               double cy, double cz, double G, double s,
+                                               int nslots, double *__restrict__ T)
+{
+    const int k = blockIdx.x * MT + threadIdx.x;
+    if (k >= nslots) return;
+    double v[12];
+    for (int j = 0; j < 12; ++j) v[j] = -0.0;
+    if (k < n && (!ex || ex[k])) {
+        const DBody q = b[k];
+        if (cp) {
+            const double DX = q.x - cx, DY = q.y - cy, DZ = q.z - cz;
+            const double rk = sqrt(DX * DX + DY * DY + DZ * DZ);
+            const double r3 = rk * rk * rk;
+            v[9] = G * q.mass * DX / r3;
+            v[10] = G * q.mass * DY / r3;
+            v[11] = G * q.mass * DZ / r3;
+        }
+        if (k > 2)
+            for (int i = 0; i < 3; ++i) {
+                if (i >= n || (ex && !ex[i])) continue;
+                const DBody g = b[i];
+                const double dx = q.x - g.x, dy = q.y - g.y, dz = q.z - g.z;
+                const double soft = s * (q.radius + g.radius);
+                const double D = sqrt(dx * dx + dy * dy + dz * dz + soft * soft);
+                const double D3 = D * D * D;
+                v[3 * i + 0] = G * q.mass * dx / D3;
+                v[3 * i + 1] = G * q.mass * dy / D3;
+                v[3 * i + 2] = G * q.mass * dz / D3;
+            }
+    }
+    double *dst = T + (size_t)(k / SQ_CH) * (12 * SQ_CH) + (k % SQ_CH);
+    for (int j = 0; j < 12; ++j) dst[(size_t)j * SQ_CH] = v[j];
+}
+
+// Staging of k_mseq: producer thread pt (waves 1..3) holds double2 number
+// pt + 192 q (q < 24) of a chunk of terms in named registers (an indexed array
+// would be placed in memory) and stores them into an LDS buffer of 12 rows of
+// SQ_CH + 2 doubles.
+static_assert(12 * SQ_CH == 2 * 192 * 24, "k_mseq stages 24 double2 per producer thread");
+// Consumer groups: 12 paired reads (at most 15 LDS reads may be outstanding).
+constexpr int SQ_G = 12;
+static_assert((SQ_CH / 2) % (2 * SQ_G) == 0, "whole ping-pong rounds per chunk");
+#define SEQ_EACH(X)                                                                        \
+    X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) \
+    X(16) X(17) X(18) X(19) X(20) X(21) X(22) X(23)
+#define SEQ_DECL(q) double2 ra##q, rb##q;
+#define SEQ_LOAD_A(q) ra##q = src_[pt + 192 * (q)];
+#define SEQ_LOAD_B(q) rb##q = src_[pt + 192 * (q)];
+#define SEQ_STAGE_A(q)                                                       \
+    {                                                                        \
+        const int e_ = 2 * (pt + 192 * (q)), j_ = e_ / SQ_CH, c_ = e_ % SQ_CH; \
+        *reinterpret_cast<double2 *>(&buf_[j_][c_]) = ra##q;                 \
+    }
+#define SEQ_STAGE_B(q)                                                       \
+    {                                                                        \
+        const int e_ = 2 * (pt + 192 * (q)), j_ = e_ / SQ_CH, c_ = e_ % SQ_CH; \
+        *reinterpret_cast<double2 *>(&buf_[j_][c_]) = rb##q;                 \
+    }
+
+// The sums themselves, in ascending k.  Waves 1..3 stage the next chunk into
+// LDS from registers, with loads issued two chunks ahead; lanes 0..11 of wave 0 add the
+// current chunk from LDS, reading each group of 24 terms while adding the
+// previous one.  Chains 0..8 start at the central term of their body,
+// CM_acc's at 0.  Thread 0 then applies the pairs (0,1), (0,2), (1,2)
+// (physics.c:296-341).  One block of 256 threads; producer and consumer run
+// separate loops with the same barriers.
 __global__ __launch_bounds__(256) void k_mseq(const DBody *__restrict__ b, int n,
                                               const int32_t *__restrict__ ex, int cp, double cx,
                                               double cy, double cz, double mu, double G, double s,
+                                              const double2 *__restrict__ T, int nch,
                                               double *__restrict__ acc, double *__restrict__ cm)
 {
-    __shared__ double T[2][12][SQ_CH + 1];
+    // Rows padded by two doubles: 16-byte aligned for paired reads, and the
+    // 12 summing lanes start on different banks.
+    __shared__ double S[2][12][SQ_CH + 2];
     __shared__ double fin[12];
+    __shared__ DBody g[3];
+    __shared__ int bex[3];
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
-    bool bex[3];
-    DBody g[3];
-    for (int i = 0; i < 3; ++i) {
-        bex[i] = i < n && (!ex || ex[i]);
-        g[i] = bex[i] ? b[i] : DBody{0, 0, 0, 0, 0, 0, 0, 0};
+    if (tid < 3) {
+        bex[tid] = tid < n && (!ex || ex[tid]);
+        if (bex[tid]) g[tid] = b[tid];
     }
-    auto produce = [&](int c, int buf) {
-        const int slot = tid - 64;
-        const int k = c * SQ_CH + slot;
-        double v[12];
-        for (int j = 0; j < 12; ++j) v[j] = -0.0;
-        if (k < n && (!ex || ex[k])) {
-            const DBody q = b[k];
-            if (cp) {
-                const double DX = q.x - cx, DY = q.y - cy, DZ = q.z - cz;
-                const double rk = sqrt(DX * DX + DY * DY + DZ * DZ);
-                const double r3 = rk * rk * rk;
-                v[9] = G * q.mass * DX / r3;
-                v[10] = G * q.mass * DY / r3;
-                v[11] = G * q.mass * DZ / r3;
-            }
-            if (k > 2)
-                for (int i = 0; i < 3; ++i) {
-                    if (!bex[i]) continue;
-                    const double dx = q.x - g[i].x, dy = q.y - g[i].y, dz = q.z - g[i].z;
-                    const double soft = s * (q.radius + g[i].radius);
-                    const double D = sqrt(dx * dx + dy * dy + dz * dz + soft * soft);
-                    const double D3 = D * D * D;
-                    v[3 * i + 0] = G * q.mass * dx / D3;
-                    v[3 * i + 1] = G * q.mass * dy / D3;
-                    v[3 * i + 2] = G * q.mass * dz / D3;
-                }
-        }
-        for (int j = 0; j < 12; ++j) T[buf][j][slot] = v[j];
-    };
-    double sum = 0.0;
-    if (w == 0 && lane < 9) {
-        const int i = lane / 3, c = lane % 3;
-        if (cp && bex[i]) {
-            double a[3], ct[3];
-            central_term(g[i].x, g[i].y, g[i].z, g[i].mass, cx, cy, cz, mu, G, a[0], a[1], a[2], ct);
-            sum = a[c];
-        }
-    }
-    const int nch = (n + SQ_CH - 1) / SQ_CH;
-    if (w > 0) produce(0, 0);
     __syncthreads();
-    for (int c = 0; c < nch; ++c) {
-        if (w > 0) {
-            if (c + 1 < nch) produce(c + 1, (c + 1) & 1);
-        } else if (lane < 12) {
-            const double *row = T[c & 1][lane];
-#pragma unroll 16
-            for (int i = 0; i < SQ_CH; ++i) sum += row[i];
+    if (w == 0) {
+        double sum = 0.0;
+        if (lane < 9) {
+            const int i = lane / 3, c = lane % 3;
+            if (cp && bex[i]) {
+                double a[3], ct[3];
+                central_term(g[i].x, g[i].y, g[i].z, g[i].mass, cx, cy, cz, mu, G, a[0], a[1], a[2],
+                             ct);
+                sum = a[c];
+            }
         }
         __syncthreads();
+        for (int c = 0; c < nch; ++c) {
+            if (lane < 12) {
+                const double2 *row = reinterpret_cast<const double2 *>(S[c & 1][lane]);
+                double2 x[SQ_G], y[SQ_G];
+#pragma unroll
+                for (int m = 0; m < SQ_G; ++m) x[m] = row[m];
+#pragma unroll
+                for (int i = SQ_G; i < SQ_CH / 2; i += 2 * SQ_G) {
+#pragma unroll
+                    for (int m = 0; m < SQ_G; ++m) y[m] = row[i + m];
+                    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                    for (int m = 0; m < SQ_G; ++m) {
+                        sum += x[m].x;
+                        sum += x[m].y;
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
+                    if (i + SQ_G < SQ_CH / 2) {
+#pragma unroll
+                        for (int m = 0; m < SQ_G; ++m) x[m] = row[i + SQ_G + m];
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                    for (int m = 0; m < SQ_G; ++m) {
+                        sum += y[m].x;
+                        sum += y[m].y;
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+            }
+            __syncthreads();
+        }
+        if (lane < 12) fin[lane] = sum;
+    } else {
+        // Two register sets: chunk c + 1 is staged from one while the loads of
+        // chunk c + 2 (issued an iteration earlier) land in the other; the
+        // staged set is then refilled with chunk c + 3.
+        const int pt = tid - 64;
+        SEQ_EACH(SEQ_DECL)
+        {
+            const double2 *src_ = T;
+            SEQ_EACH(SEQ_LOAD_A)
+            double(*buf_)[SQ_CH + 2] = S[0];
+            SEQ_EACH(SEQ_STAGE_A)
+        }
+        if (nch > 1) {
+            const double2 *src_ = T + 6 * SQ_CH;
+            SEQ_EACH(SEQ_LOAD_B)
+        }
+        if (nch > 2) {
+            const double2 *src_ = T + 12 * SQ_CH;
+            SEQ_EACH(SEQ_LOAD_A)
+        }
+        __syncthreads();
+        for (int c = 0; c < nch; ++c) {
+            if (c + 1 < nch) {
+                double(*buf_)[SQ_CH + 2] = S[(c + 1) & 1];
+                const double2 *src_ = T + (size_t)(c + 3) * (6 * SQ_CH);
+                if ((c & 1) == 0) {
+                    SEQ_EACH(SEQ_STAGE_B)
+                    if (c + 3 < nch) {
+                        SEQ_EACH(SEQ_LOAD_B)
+                    }
+                } else {
+                    SEQ_EACH(SEQ_STAGE_A)
+                    if (c + 3 < nch) {
+                        SEQ_EACH(SEQ_LOAD_A)
+                    }
+                }
+            }
+            __syncthreads();
+        }
     }
-    if (w == 0 && lane < 12) fin[lane] = sum;
     __syncthreads();
     if (tid != 0) return;
     double a[9];
@@ -806,6 +971,13 @@ __global__ __launch_bounds__(256) void k_mseq(const DBody *__restrict__ b, int n
     if (cp && cm)
         for (int c = 0; c < 3; ++c) cm[c] = fin[9 + c];
 }
+
+#undef SEQ_EACH
+#undef SEQ_DECL
+#undef SEQ_LOAD_A
+#undef SEQ_LOAD_B
+#undef SEQ_STAGE_A
+#undef SEQ_STAGE_B
 
 }  // namespace
 
@@ -919,6 +1091,7 @@ extern "C" ncg_status ncg_mesh_pass(ncg_ctx *ctx, const ncg_body *bodies, int32_
     hipStream_t st = ctx->stream;
     MeshState &M = ctx->mesh;
     M.valid = false;
+    M.ord_lists = false;
     M.extra_pairs.clear();
     M.big_visits.clear();
     M.collided.clear();
@@ -1336,17 +1509,56 @@ extern "C" ncg_status ncg_mesh_gravity(ncg_ctx *ctx, const ncg_body *bodies, int
             NCG_HIP(hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
             NCG_HIP(hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
         }
+        // Id-ordered neighbourhood lists, once per pass (every gravity call
+        // of the pass shares them).
+        const unsigned *skey = dbuf<unsigned>(ctx, "m_skey", n);
+        NCG_REQUIRE(27.0 * ins < 4294967295.0, "%d binned bodies: neighbourhood lists too long", ins);
+        const size_t lcap = std::max<size_t>(27 * (size_t)ins, 1);
+        unsigned *L = dbuf<unsigned>(ctx, "g_L", lcap);
+        unsigned *lw = dbuf<unsigned>(ctx, "g_lw", std::max(ins, 1));
+        unsigned *le = dbuf<unsigned>(ctx, "g_le", std::max(ins, 1));
+        if (!L || !lw || !le) return NCG_ERR_NOMEM;
+        if (!M.ord_lists && ins > 0) {
+            unsigned *ukey = dbuf<unsigned>(ctx, "g_ukey", ins);
+            unsigned *ucnt = dbuf<unsigned>(ctx, "g_ucnt", ins);
+            int *nu = dbuf<int>(ctx, "g_nu", 1);
+            unsigned *usz = dbuf<unsigned>(ctx, "g_usz", (size_t)ins + 1);
+            unsigned *uoff = dbuf<unsigned>(ctx, "g_uoff", (size_t)ins + 1);
+            if (!ukey || !ucnt || !nu || !usz || !uoff) return NCG_ERR_NOMEM;
+            size_t b1 = 0, b2 = 0;
+            NCG_HIP(hipcub::DeviceRunLengthEncode::Encode(nullptr, b1, skey, ukey, ucnt, nu, ins, st));
+            NCG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, b2, usz, uoff, ins + 1, st));
+            void *tmp = ncg_dbuf(ctx, "g_utmp", std::max(b1, b2));
+            if (!tmp) return NCG_ERR_NOMEM;
+            ProfScope pl(ctx, st, "mesh_lists");
+            NCG_HIP(hipcub::DeviceRunLengthEncode::Encode(tmp, b1, skey, ukey, ucnt, nu, ins, st));
+            hipLaunchKernelGGL(k_usize, dim3(cdiv(ins + 1, MT)), dim3(MT), 0, st, ukey, nu, ins, cs,
+                               ce, (int)d, usz);
+            NCG_CHECK_LAUNCH();
+            NCG_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, b2, usz, uoff, ins + 1, st));
+            hipLaunchKernelGGL(k_umerge, dim3(cdiv(ins, MT)), dim3(MT), 0, st, ukey, nu, uoff, cs,
+                               ce, sid, (int)d, L, lw, le);
+            NCG_CHECK_LAUNCH();
+            M.ord_lists = true;
+        }
+        // Terms of the body-order sums, then the sums on the second stream.
+        const int nch = cdiv(n, SQ_CH);
+        double *T = dbuf<double>(ctx, "g_seqT", (size_t)nch * 12 * SQ_CH);
+        if (!T) return NCG_ERR_NOMEM;
         ProfScope ps(ctx, st, "mesh_gravity");
+        hipLaunchKernelGGL(k_mterms, dim3(cdiv(nch * SQ_CH, MT)), dim3(MT), 0, st, db, (int)n, dex,
+                           cp, cxh, cyh, czh, G, s, nch * SQ_CH, T);
+        NCG_CHECK_LAUNCH();
         NCG_HIP(hipEventRecord(ctx->ev_fork, st));
         NCG_HIP(hipStreamWaitEvent(ctx->aux, ctx->ev_fork, 0));
         hipLaunchKernelGGL(k_mseq, dim3(1), dim3(256), 0, ctx->aux, db, (int)n, dex, cp, cxh, cyh,
-                           czh, mu, G, s, dacc, dcm);
+                           czh, mu, G, s, reinterpret_cast<const double2 *>(T), nch, dacc, dcm);
         NCG_CHECK_LAUNCH();
         NCG_HIP(hipEventRecord(ctx->ev_join, ctx->aux));
         if (ins > 0)
             hipLaunchKernelGGL(k_mord, dim3(cdiv(ins, MT)), dim3(MT), 0, st, db, (int)n, dex, ins,
-                               cs, ce, pm, rr, sx, sid, (const unsigned *)dbuf<unsigned>(ctx, "m_skey", n),
-                               (int)d, cp, cxh, cyh, czh, mu, G, s, bvk, bvoff, bvo, nbv, dacc);
+                               cs, ce, pm, rr, sx, sid, skey, (int)d, cp, cxh, cyh, czh, mu, G, s,
+                               bvk, bvoff, bvo, nbv, L, lw, le, dacc);
         if (n > 3)
             hipLaunchKernelGGL(k_mrest_ord, dim3(cdiv(n - 3, MT)), dim3(MT), 0, st, db, (int)n, dex,
                                key, cp, cxh, cyh, czh, mu, G, s, bvk, bvoff, bvo, nbv, dacc);

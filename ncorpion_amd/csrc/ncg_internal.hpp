@@ -32,6 +32,7 @@ struct BigVisit {
 // Mesh binning kept between ncg_mesh_pass and ncg_mesh_gravity.
 struct MeshState {
     bool valid = false;
+    bool ord_lists = false;  // id-ordered neighbourhood lists built for this pass
     int32_t n = 0;           // body slots of the pass
     int32_t cells = 0;       // collision_cube_cells
     int64_t table_keys = 0;  // allocated (and zeroed) entries of the cell tables
