@@ -204,7 +204,11 @@ ncg_status ncg_mesh_pass(ncg_ctx *ctx, const ncg_body *bodies, int32_t n,
 /* Mesh branch of vector_field (physics.c:63-109, 120-146, 204-261, 296-380):
  * central term, bodies 0..2 against every body, and the neighbour pairs of
  * the last ncg_mesh_pass (bodies > 2 only).  Outputs as ncg_gravity_allpairs.
- * NCG_F64_EXACT reproduces the reference's pair-list summation order. */
+ * NCG_F64_EXACT is bit-identical to the reference: each body's terms are added
+ * in the pass's pair-list order and bodies 0..2 and CM_acc in body order (the
+ * first exact call after a pass builds id-ordered neighbourhood lists, 108
+ * bytes per binned body, reused by later calls of the pass).  NCG_F64_FAST
+ * and NCG_MIXED sum per neighbour cell. */
 ncg_status ncg_mesh_gravity(ncg_ctx *ctx, const ncg_body *bodies, int32_t n,
                             const int32_t *exists, const ncg_central *central, double grav_const,
                             double softening, ncg_precision prec, double *acc,
