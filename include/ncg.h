@@ -87,11 +87,20 @@ typedef struct {
  * (closest_approach(a, b) != NULL and neither did_collide flag set,
  * collision.c:436-437 / 351-352).  `resolve` applies the outcome
  * (collision_treatment, physics.c:1043).  Both are called with the argument
- * order of the reference: (i, j) for brute_force, (p, k) for mesh. */
+ * order of the reference: (i, j) for brute_force, (p, k) for mesh.
+ *   Outcomes that remove bodies (merger, fragmentation: lose_moonlet,
+ * structure.c:1077) clear their `exists` entries, which the library reads
+ * live.  Outcomes that create bodies (fragmentation: get_free_index,
+ * structure.c:849) must be reported by `created`, which the library calls
+ * after each resolve: it writes the slots brought into existence by that
+ * resolve (at most cap) and returns their number; the caller's `exists`
+ * array must cover them.  `created` may be NULL when no outcome creates
+ * bodies (elastic and inelastic bounces, mergers). */
 typedef struct {
     void *user;
     int (*test)(void *user, int32_t a, int32_t b);
     void (*resolve)(void *user, int32_t a, int32_t b);
+    int32_t (*created)(void *user, int32_t *ids, int32_t cap);
 } ncg_replay_ops;
 
 /* Built-in decision and outcome for callers other than NcorpiON itself
@@ -165,7 +174,10 @@ ncg_status ncg_gravity_allpairs_dev(ncg_ctx *ctx, const ncg_body *d_bodies, int3
  * ops->resolve; after a collision every later pair that involves a modified
  * body is re-tested on the host through ops->test.  The GPU reads `bodies`
  * once, at the start of the pass; the callbacks see and mutate the caller's
- * live array.
+ * live array.  Rows run over the n slots given (collision.c:432), columns
+ * also over slots created during the pass (collision.c:435 reads the live
+ * largest_id); a body removed during the pass takes part in no further pair,
+ * and a created one is tested on the host against every later row.
  * *n_collisions (may be NULL) receives the number of resolved collisions.
  */
 ncg_status ncg_brute_force_pass(ncg_ctx *ctx, const ncg_body *bodies, int32_t n,

@@ -31,10 +31,13 @@ class Pair(ctypes.Structure):
 
 TEST_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32)
 RESOLVE_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32)
+CREATED_FN = ctypes.CFUNCTYPE(ctypes.c_int32, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32),
+                              ctypes.c_int32)
 
 
 class ReplayOps(ctypes.Structure):
-    _fields_ = [("user", ctypes.c_void_p), ("test", TEST_FN), ("resolve", RESOLVE_FN)]
+    _fields_ = [("user", ctypes.c_void_p), ("test", TEST_FN), ("resolve", RESOLVE_FN),
+                ("created", CREATED_FN)]
 
 
 class BuiltinCollider(ctypes.Structure):
@@ -384,26 +387,48 @@ class Context:
                 raise ValueError("did_collide must be a C-contiguous int32 array")
         return BuiltinOps(self.lib, bodies, e, dc, dt, f, one_collision_only)
 
-    def _ops(self, test, resolve, ops):
-        """(keepalive, pointer) of the replay ops: built-in or Python callbacks."""
+    def _ops(self, test, resolve, ops, created=None):
+        """(keepalive, pointer) of the replay ops: built-in or Python callbacks.
+        created() returns the slots brought into existence by the last resolve."""
         if ops is not None:
             return ops, ctypes.byref(ops.ops)
         if test is None:
             return None, None
         tf = TEST_FN(lambda u, a, c: 1 if test(int(a), int(c)) else 0)
         rf = RESOLVE_FN(lambda u, a, c: resolve(int(a), int(c)))
-        r = ReplayOps(None, tf, rf)
-        return (r, tf, rf), ctypes.byref(r)
 
-    def brute_force_pass(self, bodies, exists, dt, test=None, resolve=None, ops=None):
-        """Replay of brute_force with Python callbacks test(a, b) -> bool and
-        resolve(a, b), or built-in ops.  `bodies` must be the array they mutate."""
+        def _created(u, ids, cap):
+            new = list(created())
+            if len(new) > cap:
+                return -1
+            for q, v in enumerate(new):
+                ids[q] = int(v)
+            return len(new)
+
+        cf = CREATED_FN(_created) if created is not None else CREATED_FN()
+        r = ReplayOps(None, tf, rf, cf)
+        return (r, tf, rf, cf), ctypes.byref(r)
+
+    def brute_force_pass(self, bodies, exists, dt, test=None, resolve=None, ops=None,
+                         created=None, n=None):
+        """Replay of brute_force with Python callbacks test(a, b) -> bool,
+        resolve(a, b) and, when outcomes create bodies, created() -> new slots;
+        or built-in ops.  `bodies` (and `exists`) must be the arrays they
+        mutate; n (default: all rows) is the number of slots of the pass, the
+        arrays may be longer to hold bodies created during it."""
         b = bodies
         if not (isinstance(b, np.ndarray) and b.dtype == np.float64 and b.flags.c_contiguous):
             raise ValueError("bodies must be a C-contiguous float64 array (mutated in place)")
-        n = b.shape[0]
-        e = as_exists(exists, n)
-        keep, ops_p = self._ops(test, resolve, ops)
+        n = b.shape[0] if n is None else int(n)
+        if n > b.shape[0]:
+            raise ValueError("n exceeds the body array")
+        if exists is not None:
+            if not (isinstance(exists, np.ndarray) and exists.dtype == np.int32
+                    and exists.flags.c_contiguous and exists.shape == (b.shape[0],)):
+                raise ValueError("exists must be a C-contiguous int32 array of one entry per "
+                                 "body slot (the live array the callbacks mutate)")
+        e = exists
+        keep, ops_p = self._ops(test, resolve, ops, created)
         if ops_p is None:
             raise ValueError("brute_force_pass needs test/resolve callbacks or ops")
         nc = I64()

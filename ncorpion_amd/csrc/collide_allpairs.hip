@@ -201,39 +201,68 @@ extern "C" ncg_status ncg_brute_force_pass(ncg_ctx *ctx, const ncg_body *bodies,
     r = candidates_dev(ctx, db, n, dex, timestep, cand);
     if (r != NCG_OK) return r;
 
-    // Replay of collision.c:433-455.  `dirty` holds every body modified so far
-    // in this pass; pairs touching it are decided on the host.
+    // Replay of collision.c:433-455.  `dirty` holds every body modified or
+    // created so far in this pass; pairs touching it are decided on the host.
+    // Rows stop at the n slots of the call (current_largest_id); columns run
+    // to the live end, which fragmentation can extend (largest_id).
     std::vector<char> dirty((size_t)n, 0);
     std::set<int32_t> dirty_set;
+    int32_t ncol = n;
     int64_t ncoll = 0;
     size_t ci = 0;
+    std::vector<int32_t> born(64);
     auto live = [&](int32_t k) { return !exists || exists[k] != 0; };
-    auto collide = [&](int32_t i, int32_t j) {
+    auto mark = [&](int32_t k) {
+        if ((size_t)k >= dirty.size()) dirty.resize((size_t)k + 1, 0);
+        if (!dirty[k]) {
+            dirty[k] = 1;
+            dirty_set.insert(k);
+        }
+    };
+    auto collide = [&](int32_t i, int32_t j) -> ncg_status {
         ops->resolve(ops->user, i, j);
         ++ncoll;
-        if (!dirty[i]) { dirty[i] = 1; dirty_set.insert(i); }
-        if (!dirty[j]) { dirty[j] = 1; dirty_set.insert(j); }
+        mark(i);
+        mark(j);
+        if (ops->created) {
+            const int32_t nb = ops->created(ops->user, born.data(), (int32_t)born.size());
+            NCG_REQUIRE(nb >= 0 && nb <= (int32_t)born.size(),
+                        "created callback reported %d bodies (at most %d)", nb, (int)born.size());
+            for (int32_t q = 0; q < nb; ++q) {
+                NCG_REQUIRE(born[q] >= 0, "created callback reported slot %d", born[q]);
+                mark(born[q]);
+                ncol = std::max(ncol, born[q] + 1);
+            }
+        }
+        return NCG_OK;
     };
     for (int32_t i = 0; i < n; ++i) {
         // Candidates of earlier rows are consumed; position the cursor on row i.
         while (ci < cand.size() && cand[ci].fst < i) ++ci;
         if (!live(i)) continue;
         int32_t jc = i + 1;  // next j to consider in this row
-        while (jc < n) {
+        // A row body removed by an outcome takes part in no further collision
+        // (collision_treatment returns at physics.c:1059-1061).
+        while (jc < ncol && live(i)) {
             if (dirty[i]) {
                 // Row of a modified body: every remaining pair on the host.
-                for (int32_t j = jc; j < n; ++j)
-                    if (live(j) && ops->test(ops->user, i, j)) collide(i, j);
-                jc = n;
+                for (int32_t j = jc; j < ncol && live(i); ++j)
+                    if (live(j) && ops->test(ops->user, i, j)) {
+                        r = collide(i, j);
+                        if (r != NCG_OK) return r;
+                    }
                 break;
             }
             while (ci < cand.size() && cand[ci].fst == i && cand[ci].snd < jc) ++ci;
-            const int32_t j_cand = (ci < cand.size() && cand[ci].fst == i) ? cand[ci].snd : n;
+            const int32_t j_cand = (ci < cand.size() && cand[ci].fst == i) ? cand[ci].snd : ncol;
             auto it = dirty_set.lower_bound(jc);
-            const int32_t j_dirty = it == dirty_set.end() ? n : *it;
+            const int32_t j_dirty = it == dirty_set.end() ? ncol : *it;
             const int32_t j = std::min(j_cand, j_dirty);
-            if (j >= n) break;
-            if (live(j) && ops->test(ops->user, i, j)) collide(i, j);
+            if (j >= ncol) break;
+            if (live(j) && ops->test(ops->user, i, j)) {
+                r = collide(i, j);
+                if (r != NCG_OK) return r;
+            }
             jc = j + 1;
         }
     }

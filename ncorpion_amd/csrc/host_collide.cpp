@@ -84,5 +84,6 @@ extern "C" ncg_status ncg_builtin_replay_ops(ncg_builtin_collider *state, ncg_re
     ops->user = state;
     ops->test = builtin_test;
     ops->resolve = builtin_resolve;
+    ops->created = nullptr;  // bounces create no bodies
     return NCG_OK;
 }

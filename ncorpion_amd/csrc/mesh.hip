@@ -1261,9 +1261,25 @@ extern "C" ncg_status ncg_mesh_pass(ncg_ctx *ctx, const ncg_body *bodies, int32_
         dirty[b] = 1;
         return schedule(b, kv);
     };
+    int32_t born[64];
+    // Small (radius <= gam) at the start of the pass: not in the big list.
+    auto was_small = [&](int b) { return !std::binary_search(big.begin(), big.end(), b); };
     auto collide = [&](int p, int k, int kv) -> ncg_status {
         ops->resolve(ops->user, p, k);
         ++ncoll;
+        // Outcomes that remove, create or reclassify bodies (mergers, full
+        // fragmentation: bodies lost mid-pass drop out of later gathers and
+        // new ones join later big visits, collision.c:296, 386) are outside
+        // what this replay reproduces.
+        const int32_t nborn = ops->created ? ops->created(ops->user, born, 64) : 0;
+        if (nborn != 0 || !live(p) || !live(k) ||
+            (p > kv && (bodies[p].radius <= gam) != was_small(p)) ||
+            (k > kv && (bodies[k].radius <= gam) != was_small(k)))
+            return ncg_fail(ctx, NCG_ERR_INVALID,
+                            "mesh(): the outcome of collision (%d, %d) removed, created or "
+                            "resized bodies across the mesh size; only outcomes that keep both "
+                            "bodies (bounces, partial fragmentation) are supported on the mesh path",
+                            p, k);
         M.collided.push_back({p, k});
         ncg_status r = mark_dirty(p, kv);
         if (r != NCG_OK) return r;

@@ -3,9 +3,9 @@
  *
  * Compiled together with an NcorpiON source tree (its parameters.h,
  * structure.h, physics.h and collision.h on the include path) and linked with
- * -Wl,--wrap=vector_field,--wrap=brute_force,--wrap=mesh,--wrap=get_neighbours_mesh
- * plus ncg_wrap.c, so that the integrator's calls from rk4.o reach the GPU
- * without editing any NcorpiON file.  The parameters.h switches are honoured
+ * -Wl,--wrap=vector_field,--wrap=brute_force,--wrap=mesh,--wrap=get_neighbours_mesh,
+ * --wrap=get_free_index plus ncg_wrap.c, so that the integrator's calls from
+ * rk4.o reach the GPU without editing any NcorpiON file.  The parameters.h switches are honoured
  * as compile-time constants, exactly as the reference reads them.
  *
  * What each replaced function does here:
@@ -14,7 +14,9 @@
  *                        writes X[k].vx/vy/vz and CM_acc
  *   brute_force          collision.c:425-456 -> ncg_brute_force_pass, with the
  *                        reference's closest_approach and collision_treatment as
- *                        the host decision and outcome
+ *                        the host decision and outcome (every outcome: bounces,
+ *                        mergers, fragmentation; created slots are reported
+ *                        from the get_free_index wrap)
  *   mesh                 collision.c:324-417 -> ncg_mesh_pass (mode 0)
  *   get_neighbours_mesh  physics.c:1094-1118 -> ncg_mesh_pass (mode 1)
  * The hash table of the reference is left untouched (all NULL) and
@@ -176,11 +178,37 @@ static void replay_resolve(void *u, int32_t a, int32_t b)
     collision_treatment(r->X, a, b, r->type);
 }
 
+/* Slots brought into existence by fragmentation (get_free_index,
+ * structure.c:849, called from physics.c:994-1001), recorded here through
+ * -Wl,--wrap=get_free_index and reported to the library after each outcome. */
+int __real_get_free_index(int should_be_put_at_the_end);
+#define NCG_BORN_CAP 64
+static int32_t g_born[NCG_BORN_CAP];
+static int g_nborn = 0;
+
+int __wrap_get_free_index(int should_be_put_at_the_end)
+{
+    int index = __real_get_free_index(should_be_put_at_the_end);
+    if (g_nborn < NCG_BORN_CAP) g_born[g_nborn] = index;
+    g_nborn++;
+    return index;
+}
+
+static int32_t replay_created(void *u, int32_t *ids, int32_t cap)
+{
+    (void)u;
+    int nb = g_nborn;
+    g_nborn = 0;
+    if (nb > NCG_BORN_CAP || nb > cap) return -1; /* the library reports the error */
+    memcpy(ids, g_born, sizeof(int32_t) * (size_t)nb);
+    return nb;
+}
+
 static void require_two_body_outcomes(void)
 {
     if (collision_type() > 1) {
-        fprintf(stderr, "Error : ncg drop-in: merger and fragmentation outcomes are not supported "
-                        "on the GPU collision path (use elastic or inelastic collisions).\n");
+        fprintf(stderr, "Error : ncg drop-in: merger and fragmentation outcomes are supported on "
+                        "the brute-force path only (mesh(): use elastic or inelastic collisions).\n");
         abort();
     }
 }
@@ -188,10 +216,10 @@ static void require_two_body_outcomes(void)
 void ncg_dropin_brute_force(struct moonlet *X)
 {
     ncg_ctx *c = ctx();
-    require_two_body_outcomes();
     const int n = largest_id + 1;
     struct replay_user u = {X, collision_type()};
-    ncg_replay_ops ops = {&u, replay_test, replay_resolve};
+    ncg_replay_ops ops = {&u, replay_test, replay_resolve, replay_created};
+    g_nborn = 0;
     int64_t ncoll = 0;
     ncg_status st = ncg_brute_force_pass(c, (const ncg_body *)X, n, exists, timestep, &ops, &ncoll);
     if (st != NCG_OK) ncg_die("brute_force", st);
@@ -203,8 +231,9 @@ static void mesh_common(struct moonlet *X, int mode)
     ncg_ctx *c = ctx();
     const int n = largest_id + 1;
     struct replay_user u = {X, collision_type()};
-    ncg_replay_ops ops = {&u, replay_test, replay_resolve};
+    ncg_replay_ops ops = {&u, replay_test, replay_resolve, replay_created};
     ncg_mesh_stats stats;
+    g_nborn = 0;
     memset(&stats, 0, sizeof stats);
     ncg_status st = ncg_mesh_pass(c, (const ncg_body *)X, n, exists, timestep, gam, collision_cube,
                                   collision_cube_cells, mode, mode == 0 ? &ops : NULL, &stats);

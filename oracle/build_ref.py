@@ -138,6 +138,13 @@ CONFIGS = {
     "run_c1": ("run", _run_cfg("brute", 2000, 200, DT1)),
     # Same, denser (larger radii in init.txt) and shorter, for many collisions.
     "run_c1_dense": ("run", _run_cfg("brute", 2000, 40, DT1)),
+    # Brute force with the reference's default collision model (mergers,
+    # partial and full fragmentation: bodies removed and created mid-pass),
+    # and with instant mergers; 2000 bodies, 40 steps.
+    "run_c1_frag": ("run", _run_cfg("brute", 2000, 40, DT1, {
+        "inelastic_collision_bool": "0", "fragmentation_bool": "1", "N_max": "4000"})),
+    "run_c1_merge": ("run", _run_cfg("brute", 2000, 40, DT1, {
+        "inelastic_collision_bool": "0", "instant_merger_bool": "1"})),
     # Mesh, 16384 bodies, cells of side 0.01 (256 per axis), 10 steps of 2 pi/512.
     # The initial mesh size comes from sma_min/sma_max/inclination_max
     # (structure.c:517), so those are set to the disk's range.
@@ -204,6 +211,9 @@ def build(name):
     trace_wraps = wraps + ",--wrap=collision_treatment"
     run([CC] + CFLAGS + inc + [os.path.join(HERE, "ref_trace.c"), main_o] + objs +
         [trace_wraps, "-lm", "-o", os.path.join(out, "ncorpion_trace")])
+    # The glue also records the slots fragmentation creates (get_free_index).
+    wraps += ",--wrap=get_free_index"
+    trace_wraps += ",--wrap=get_free_index"
     lib_dir = os.path.join(REPO, "ncorpion_amd")
     if not os.path.exists(os.path.join(lib_dir, "libncg.so")):
         print("  (libncg.so not built: skipping drop-in executables)")

@@ -104,3 +104,87 @@ def test_replay_sparse_collisions_few_host_tests(ctx):
     assert np.array_equal(bits(r.b), bits(out))
     assert 0 < len(pairs) < 50
     assert r.tests < 4 * 3000 * (len(pairs) + 1)
+
+
+class ToyWorld:
+    """A host world whose outcomes remove and create bodies (test-only rules:
+    merge the second body into the first, swap velocities, or split off a new
+    body at a free slot, reused last-freed-first, or appended), with arrays
+    of `cap` slots like NcorpiON's N_max."""
+
+    def __init__(self, b, cap, dt):
+        n = b.shape[0]
+        self.b = np.zeros((cap, 8))
+        self.b[:n] = b
+        self.e = np.zeros(cap, np.int32)
+        self.e[:n] = 1
+        self.largest = n - 1
+        self.free = []
+        self.born = []
+        self.log = []
+        self.dt = dt
+
+    def test(self, a, c):
+        return orc.closest_approach(self.b, a, c, self.dt, self.e)[0]
+
+    def resolve(self, a, c):
+        if not (self.e[a] and self.e[c]):
+            return  # as collision_treatment (physics.c:1059-1061)
+        self.log.append((a, c))
+        b = self.b
+        kind = (7 * a + c) % 3
+        if kind == 0:
+            m = b[a, 6] + b[c, 6]
+            b[a, 3:6] = (b[a, 6] * b[a, 3:6] + b[c, 6] * b[c, 3:6]) / m
+            b[a, 6] = m
+            self.e[c] = 0
+            self.free.append(c)
+        elif kind == 1:
+            va = b[a, 3:6].copy()
+            b[a, 3:6] = b[c, 3:6]
+            b[c, 3:6] = va
+        else:
+            s = self.free.pop() if self.free else self.largest + 1
+            self.largest = max(self.largest, s)
+            b[s] = b[a]
+            b[s, 0] += 3.0 * b[a, 7]
+            b[s, 3:6] = -b[a, 3:6]
+            b[s, 6] *= 0.5
+            b[a, 6] *= 0.5
+            self.e[s] = 1
+            self.born.append(s)
+
+    def created(self):
+        out, self.born = self.born, []
+        return out
+
+
+def toy_brute_force(w, n0):
+    """collision.c:432-455 with the live largest_id for the columns."""
+    for i in range(n0):
+        if not w.e[i]:
+            continue
+        j = i + 1
+        while j <= w.largest:
+            if w.e[j] and w.test(i, j):
+                w.resolve(i, j)
+            j += 1
+
+
+@pytest.mark.parametrize("seed", [11, 12])
+def test_replay_with_removed_and_created_bodies(ctx, seed):
+    """Outcomes that remove bodies and create new ones at free or appended
+    slots, as mergers and fragmentation do: the replay gives the reference
+    loop's collision sequence and final state."""
+    n0, cap, dt = 400, 1200, 0.01
+    b = cluster(n0, seed, box=0.06, radius=2.5e-3)
+    ref = ToyWorld(b, cap, dt)
+    toy_brute_force(ref, n0)
+    got = ToyWorld(b, cap, dt)
+    nc = ctx.brute_force_pass(got.b, got.e, dt, got.test, got.resolve, created=got.created, n=n0)
+    assert got.log == ref.log
+    assert nc == len(ref.log)
+    assert np.array_equal(got.e, ref.e)
+    assert np.array_equal(bits(got.b[got.e != 0]), bits(ref.b[ref.e != 0]))
+    kinds = {(7 * a + c) % 3 for a, c in ref.log}
+    assert kinds == {0, 1, 2} and ref.largest >= n0  # removals, reuse and appends happened

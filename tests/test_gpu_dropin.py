@@ -57,6 +57,20 @@ def test_dense_run_identical_with_collisions(tmp_path):
     assert ncoll > 20
 
 
+@pytest.mark.parametrize("cfg", ["run_c1_frag", "run_c1_merge"])
+def test_fragmentation_and_merger_runs_identical(tmp_path, cfg):
+    # The reference's default collision model (mergers, partial and full
+    # fragmentation creating up to 15 bodies per collision at free or appended
+    # slots) and instant mergers: 2000 bodies, 40 steps, every traced call
+    # identical.
+    b = ic.disk(2000, seed=1, mass=1e-8, radius=4e-3)
+    tr = compare(cfg, b, tmp_path, 40)
+    coll = [l.split() for l in tr if l.startswith("C ")]
+    assert len(coll) > 100
+    if cfg == "run_c1_frag":
+        assert any(int(c[2]) >= 2000 or int(c[3]) >= 2000 for c in coll)  # created bodies collide
+
+
 def test_mesh_run_identical(tmp_path):
     # Mesh module, 16384 bodies (radius 1e-3 so that collisions occur), cells
     # of side 0.01, 10 steps: every traced call identical, the mesh gravity
