@@ -213,6 +213,23 @@ ncg_status ncg_mesh_pass(ncg_ctx *ctx, const ncg_body *bodies, int32_t n,
                          double collision_cube, int32_t cells, int32_t mode,
                          const ncg_replay_ops *ops, ncg_mesh_stats *stats);
 
+/* ncg_mesh_pass on device arrays for callers that keep the bodies on the GPU:
+ * d_bodies (n records, modified in place by mode 0), d_exists (n, or NULL),
+ * d_did_collide (n, or NULL unless one_collision_only).  Mode 0 replays
+ * mesh()'s visiting order on the device with the library's own outcome, the
+ * bounce of collision (physics.c:545-605) with restitution collision_f (2:
+ * elastic), and needs every body to be at most the mesh size gam (bodies
+ * larger than gam: NCG_ERR_INVALID, use ncg_mesh_pass).  Enqueued on
+ * `stream` (NULL: the context's stream); returns after the counters are
+ * read back (one synchronisation).  stats->host_tests is 0.  The binning is
+ * kept for ncg_mesh_gravity / ncg_mesh_gravity_dev; ncg_mesh_pairs is not
+ * available after it. */
+ncg_status ncg_mesh_pass_dev(ncg_ctx *ctx, ncg_body *d_bodies, int32_t n, const int32_t *d_exists,
+                             int32_t *d_did_collide, double timestep, double gam,
+                             double collision_cube, int32_t cells, int32_t mode,
+                             double collision_f, int32_t one_collision_only,
+                             ncg_mesh_stats *stats, void *stream);
+
 /* Mesh branch of vector_field (physics.c:63-109, 120-146, 204-261, 296-380):
  * central term, bodies 0..2 against every body, and the neighbour pairs of
  * the last ncg_mesh_pass (bodies > 2 only).  Outputs as ncg_gravity_allpairs.
@@ -225,6 +242,16 @@ ncg_status ncg_mesh_gravity(ncg_ctx *ctx, const ncg_body *bodies, int32_t n,
                             const int32_t *exists, const ncg_central *central, double grav_const,
                             double softening, ncg_precision prec, double *acc,
                             double *cm_acc);
+
+/* ncg_mesh_gravity on device arrays, enqueued on `stream` (NULL: the
+ * context's stream) without synchronisation: d_bodies (n records), d_exists
+ * (n, or NULL), d_central (8 doubles {x, y, z, vx, vy, vz, M, present}, or
+ * NULL), outputs d_acc (3 per slot; absent slots untouched) and d_cm_acc (3,
+ * may be NULL).  Uses the binning of the last mesh pass over the same n. */
+ncg_status ncg_mesh_gravity_dev(ncg_ctx *ctx, const ncg_body *d_bodies, int32_t n,
+                                const int32_t *d_exists, const double *d_central,
+                                double grav_const, double softening, ncg_precision prec,
+                                double *d_acc, double *d_cm_acc, void *stream);
 
 /* Neighbour pairs of the last ncg_mesh_pass in the reference's registration
  * order, (k, p) with k the visited body (what mesh()/get_neighbours_mesh()

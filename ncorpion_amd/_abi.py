@@ -98,6 +98,9 @@ _SIGNATURES = {
     "ncg_mesh_pass": (S, [P, P, I32, P, D, D, D, I32, I32, ctypes.POINTER(ReplayOps),
                           ctypes.POINTER(MeshStats)]),
     "ncg_mesh_gravity": (S, [P, P, I32, P, ctypes.POINTER(Central), D, D, S, P, P]),
+    "ncg_mesh_gravity_dev": (S, [P, P, I32, P, P, D, D, S, P, P, P]),
+    "ncg_mesh_pass_dev": (S, [P, P, I32, P, P, D, D, D, I32, I32, D, I32,
+                              ctypes.POINTER(MeshStats), P]),
     "ncg_mesh_pairs": (S, [P, P, I64, ctypes.POINTER(I64)]),
     "ncg_rk4_steps_dev": (S, [P, P, I32, P, P, D, D, S, D, I32, I32, P]),
     "ncg_set_tiling": (S, [P, I32, I32, D, I32]),
@@ -286,6 +289,29 @@ class Context:
             float(G), float(softening), PRECISIONS[precision], float(dt), int(nsteps),
             float(collision_f), int(one_collision_only), flags, ctypes.byref(nc), stream))
         return int(nc.value)
+
+    def mesh_pass_dev(self, bodies_t, dt, gam, cube, cells, mode, collision_f=1.5,
+                      one_collision_only=0, exists_t=None, did_collide_t=None, stream=None):
+        """mesh() (mode 0, device replay with the library's bounce) or
+        get_neighbours_mesh() (mode 1) on device tensors; returns the counters."""
+        n = int(bodies_t.shape[0])
+        stats = MeshStats()
+        self._check(self.lib.ncg_mesh_pass_dev(
+            self.h, bodies_t.data_ptr(), n, None if exists_t is None else exists_t.data_ptr(),
+            None if did_collide_t is None else did_collide_t.data_ptr(), float(dt), float(gam),
+            float(cube), int(cells), int(mode), float(collision_f), int(one_collision_only),
+            ctypes.byref(stats), stream))
+        return {f: int(getattr(stats, f)) for f, _ in MeshStats._fields_}
+
+    def mesh_gravity_dev(self, bodies_t, acc_t, central_t=None, cm_t=None, exists_t=None, G=1.0,
+                         softening=0.0, precision="exact", stream=None):
+        """Mesh field of device tensors over the last mesh pass (asynchronous)."""
+        n = int(bodies_t.shape[0])
+        self._check(self.lib.ncg_mesh_gravity_dev(
+            self.h, bodies_t.data_ptr(), n, None if exists_t is None else exists_t.data_ptr(),
+            None if central_t is None else central_t.data_ptr(), float(G), float(softening),
+            PRECISIONS[precision], acc_t.data_ptr(), None if cm_t is None else cm_t.data_ptr(),
+            stream))
 
     def energy_dev(self, bodies_t, central_t=None, exists_t=None, G=1.0, softening=0.0,
                    out_t=None, stream=None):

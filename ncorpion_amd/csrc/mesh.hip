@@ -27,6 +27,7 @@
 #include <set>
 
 #include "ncg_internal.hpp"
+#include "collide_dev.hpp"
 
 #pragma clang fp contract(off)
 
@@ -228,6 +229,243 @@ __global__ __launch_bounds__(MT) void k_mcount(const unsigned *__restrict__ skey
     if (threadIdx.x == 0 && t) atomicAdd(total, t);
 }
 
+// ---- device replay of mesh() ---------------------------------------------------
+// For callers that keep the bodies on the GPU, with the library's own bounce
+// (elastic or inelastic, collision.c:351-366 with physics.c:545-605) and no
+// body larger than the mesh size.  Then a body changes only during its own
+// visit or as the partner p < k of a later visit, so every body still ahead
+// of the current visit holds its start-of-pass state.  The replay walks the
+// visits with something to decide, in ascending id:
+//   - events of visit k: candidate pairs (k, p) of the pass and pending
+//     re-tests, in traversal order (cell l = 26..0, ascending p), each tested
+//     exactly with the current states; the first hit is resolved;
+//   - after a hit, the rest of visit k is scanned in full (k changed), 64
+//     pairs at a time, the first hit in traversal order winning;
+//   - whenever a body changes, its later visitors q > k (the members of its
+//     27 cells) are tested against its new state at once; the ones that may
+//     hit become pending re-tests of visit q.  A visitor's own state is its
+//     start-of-pass state (it is still ahead), and a re-test is exact when it
+//     runs, so the pending list only has to be a superset.
+// One wavefront.  stat[0] += collisions; stat[1] |= 1 (more candidates than
+// cap), 2 (more than MR_PCAP pending re-tests), 4 (a body larger than the
+// mesh size).
+constexpr int MR_PCAP = 8192;
+
+__global__ void k_lf_fill_u32(unsigned *__restrict__ p, unsigned n)
+{
+    if (threadIdx.x < n) p[threadIdx.x] = 0u;
+}
+
+// Candidates (k, l, p) as sort keys (k, 26 - l, p), for k, p < 2^29.
+__global__ __launch_bounds__(MT) void k_mcand_keys(const int4 *__restrict__ cand,
+                                                   const unsigned long long *__restrict__ ccnt,
+                                                   unsigned cap, unsigned long long *__restrict__ keys,
+                                                   unsigned *__restrict__ vals)
+{
+    const unsigned i = blockIdx.x * MT + threadIdx.x;
+    if (i >= cap) return;
+    const unsigned long long m = *ccnt;
+    unsigned long long k = ~0ull;
+    if (i < m) {
+        const int4 c = cand[i];
+        k = ((unsigned long long)c.x << 34) | ((unsigned long long)(26 - c.y) << 29) |
+            (unsigned long long)c.z;
+    }
+    keys[i] = k;
+    vals[i] = i;
+}
+
+__device__ inline bool mr_before(int4 a, int4 b)  // (q, 26 - l, d) order
+{
+    if (a.x != b.x) return a.x < b.x;
+    if (a.y != b.y) return a.y > b.y;
+    return a.z < b.z;
+}
+
+__global__ __launch_bounds__(64) void k_mesh_replay(
+    DBody *__restrict__ b, const int32_t *__restrict__ ex, int32_t *__restrict__ dcf, int one_coll,
+    double f, double dt, const unsigned *__restrict__ key, const int *__restrict__ sid,
+    const unsigned *__restrict__ cs, const unsigned *__restrict__ ce,
+    const double4 *__restrict__ pr, const double4 *__restrict__ vm, int d,
+    const int4 *__restrict__ cand, const unsigned *__restrict__ order,
+    const unsigned long long *__restrict__ ccnt, unsigned cap, const int *__restrict__ cnt,
+    unsigned *__restrict__ stat)
+{
+    __shared__ int4 P[MR_PCAP];
+    __shared__ int s_np, s_pi, s_ci, s_k, s_hl, s_hp, s_hit;
+    __shared__ int s_q[64];
+    __shared__ double s_ap[6], s_tc;
+    const int lane = threadIdx.x;
+    if (cnt[3] > 0) {
+        if (lane == 0) atomicOr(stat + 1, 4u);
+        return;
+    }
+    const unsigned long long total = *ccnt;
+    if (total > cap) {
+        if (lane == 0) atomicOr(stat + 1, 1u);
+        return;
+    }
+    const int m = (int)total;
+    if (lane == 0) {
+        s_np = 0;
+        s_pi = 0;
+        s_ci = 0;
+    }
+    __syncthreads();
+    unsigned hits = 0;
+    auto cand_at = [&](int i) { return cand[order[i]]; };
+    // Later visitors of body dd (just changed during visit kc) that may hit it.
+    auto eager = [&](int dd, int kc) {
+        const unsigned kd = key[dd];
+        if (kd == NOKEY) return;
+        const DBody D = b[dd];
+        const double4 pd = make_double4(D.x, D.y, D.z, D.radius);
+        const double4 vd = make_double4(D.vx, D.vy, D.vz, D.mass);
+        for (int l = 0; l < 27; ++l) {
+            const unsigned c = (unsigned)((long)kd + stencil_offset(l, d));
+            const unsigned a = cs[c], e = ce[c];
+            for (unsigned base = a; base < e; base += 64) {
+                const unsigned u = base + lane;
+                bool h = false;
+                if (u < e) {
+                    const int q = sid[u];
+                    s_q[lane] = q;
+                    if (q > kc && (!ex || ex[q])) h = approach_hit(pd, vd, pr[u], vm[u], dt);
+                }
+                unsigned long long mask = __ballot(h);
+                __syncthreads();
+                if (lane == 0) {
+                    while (mask) {
+                        const int bit = __ffsll((long long)mask) - 1;
+                        mask &= mask - 1;
+                        // In visit q, dd sits in cell 26 - l (the stencil is symmetric).
+                        const int4 ev = make_int4(s_q[bit], 26 - l, dd, 0);
+                        if (s_np >= MR_PCAP) {
+                            atomicOr(stat + 1, 2u);
+                            continue;
+                        }
+                        int w = s_np++;
+                        while (w > s_pi && mr_before(ev, P[w - 1])) {
+                            P[w] = P[w - 1];
+                            --w;
+                        }
+                        P[w] = ev;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+    };
+    auto resolve = [&](int p, int k) {  // lane 0
+        bf_bounce(b, p, k, f, s_ap, s_tc);
+        if (dcf) {
+            dcf[p] = one_coll;
+            dcf[k] = one_coll;
+        }
+        ++hits;
+    };
+    while (true) {
+        if (lane == 0) {
+            if (s_pi >= MR_PCAP / 2) {  // drop the consumed re-tests
+                for (int q = s_pi; q < s_np; ++q) P[q - s_pi] = P[q];
+                s_np -= s_pi;
+                s_pi = 0;
+            }
+            const int kc = s_ci < m ? cand_at(s_ci).x : INT_MAX;
+            const int kp = s_pi < s_np ? P[s_pi].x : INT_MAX;
+            const int k = min(kc, kp);
+            s_k = k;
+            s_hit = 0;
+            if (k != INT_MAX) {
+                // Events of visit k in traversal order until the first hit.
+                int ci = s_ci, pi = s_pi;
+                int4 prev = make_int4(-1, -1, -1, -1);
+                while (true) {
+                    const bool hc = ci < m && cand_at(ci).x == k;
+                    const bool hp = pi < s_np && P[pi].x == k;
+                    if (!hc && !hp) break;
+                    int4 ev;
+                    if (hc && (!hp || !mr_before(P[pi], make_int4(k, cand_at(ci).y, cand_at(ci).z, 0)))) {
+                        const int4 c = cand_at(ci++);
+                        ev = make_int4(k, c.y, c.z, 0);
+                    } else {
+                        ev = P[pi++];
+                    }
+                    if (ev.y == prev.y && ev.z == prev.z) continue;
+                    prev = ev;
+                    double ap[6], tc;
+                    if (bf_hit(b, ex, dcf, ev.z, k, dt, ap, &tc)) {
+                        for (int q = 0; q < 6; ++q) s_ap[q] = ap[q];
+                        s_tc = tc;
+                        s_hl = ev.y;
+                        s_hp = ev.z;
+                        s_hit = 1;
+                        resolve(ev.z, k);
+                        break;
+                    }
+                }
+                // Every event of visit k is consumed (after a hit the full scan
+                // below covers the rest of the visit).
+                while (s_ci < m && cand_at(s_ci).x == k) ++s_ci;
+                while (s_pi < s_np && P[s_pi].x == k) ++s_pi;
+            }
+        }
+        __threadfence_block();
+        __syncthreads();
+        const int k = s_k;
+        if (k == INT_MAX) break;
+        if (!s_hit) continue;
+        eager(s_hp, k);
+        // The rest of visit k, in traversal order, from after (s_hl, s_hp).
+        const unsigned kk = key[k];
+        int l = s_hl;
+        unsigned from;
+        {
+            const unsigned c = (unsigned)((long)kk + stencil_offset(l, d));
+            unsigned lo = cs[c], hi = ce[c];
+            const int p0 = s_hp;
+            while (lo < hi) {
+                const unsigned mid = (lo + hi) >> 1;
+                if (sid[mid] <= p0) lo = mid + 1;
+                else hi = mid;
+            }
+            from = lo;
+        }
+        for (; l >= 0; --l) {
+            const unsigned c = (unsigned)((long)kk + stencil_offset(l, d));
+            const unsigned e = ce[c];
+            unsigned base = (l == s_hl) ? from : cs[c];
+            while (base < e) {
+                const unsigned u = base + lane;
+                const int p = u < e ? sid[u] : INT_MAX;
+                double ap[6], tc;
+                const bool h = p < k && bf_hit(b, ex, dcf, p, k, dt, ap, &tc);
+                const unsigned long long hm = __ballot(h);
+                const unsigned long long past = __ballot(u < e && p >= k);
+                if (hm == 0ull) {
+                    if (past) break;  // ids ascend: the rest of the cell is above k
+                    base += 64;
+                    continue;
+                }
+                const int first = __ffsll((long long)hm) - 1;
+                if (lane == first) {
+                    for (int q = 0; q < 6; ++q) s_ap[q] = ap[q];
+                    s_tc = tc;
+                    s_hp = p;
+                }
+                __syncthreads();
+                if (lane == 0) resolve(s_hp, k);
+                __threadfence_block();
+                __syncthreads();
+                eager(s_hp, k);
+                base = base + first + 1;
+            }
+        }
+        eager(k, k);
+    }
+    if (lane == 0 && hits) atomicAdd(stat, hits);
+}
+
 // ---- neighbour-cell gravity ---------------------------------------------------
 // Term G m_p (X_p - X_k) / D^3 of physics.c:186-200 (reference expression).
 __device__ inline void term_exact(double X, double Y, double Z, double Rk, double4 p, double Rp,
@@ -295,11 +533,13 @@ __device__ inline void central_term(double x, double y, double z, double m, doub
 // bodies 0..2).
 __global__ __launch_bounds__(MT) void k_mgrav_rest(const DBody *__restrict__ b, int n,
                                                    const int32_t *__restrict__ ex,
-                                                   const unsigned *__restrict__ key, int cp,
-                                                   double cx, double cy, double cz, double mu,
+                                                   const unsigned *__restrict__ key, const double *__restrict__ cen,
                                                    double G, double s, double *__restrict__ acc,
                                                    double *__restrict__ cterm)
 {
+    const int cp = cen && cen[7] != 0.0;
+    const double cx = cp ? cen[0] : 0.0, cy = cp ? cen[1] : 0.0, cz = cp ? cen[2] : 0.0;
+    const double mu = cp ? G * cen[6] : 0.0;
     const int k = blockIdx.x * MT + threadIdx.x;
     if (k >= n || (ex && !ex[k])) return;
     if (k > 2 && key[k] != NOKEY) return;
@@ -334,11 +574,13 @@ __global__ __launch_bounds__(MT) void k_mgrav(const DBody *__restrict__ b, int n
                                               const double *__restrict__ rr,
                                               const int *__restrict__ sx,
                                               const int *__restrict__ sid,
-                                              const unsigned *__restrict__ skey, int d, int cp,
-                                              double cx, double cy, double cz, double mu,
+                                              const unsigned *__restrict__ skey, int d, const double *__restrict__ cen,
                                               double G, double s, double *__restrict__ acc,
                                               double *__restrict__ cterm)
 {
+    const int cp = cen && cen[7] != 0.0;
+    const double cx = cp ? cen[0] : 0.0, cy = cp ? cen[1] : 0.0, cz = cp ? cen[2] : 0.0;
+    const double mu = cp ? G * cen[6] : 0.0;
     const int t = blockIdx.x * MT + threadIdx.x;
     if (t >= ins || !sx[t]) return;
     const int k = sid[t];
@@ -422,9 +664,10 @@ __global__ __launch_bounds__(MT) void k_big_final(const DBody *__restrict__ b, i
                                                   const int32_t *__restrict__ ex,
                                                   const double *__restrict__ part, int np,
                                                   double G, double s, double *__restrict__ acc,
-                                                  const double *__restrict__ cterm, int cp,
+                                                  const double *__restrict__ cterm, const double *__restrict__ cen,
                                                   double *__restrict__ cm)
 {
+    const int cp = cen && cen[7] != 0.0;
     __shared__ double sh[MT];
     double a[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     for (int q = threadIdx.x; q < np; q += MT)
@@ -653,8 +896,7 @@ __global__ __launch_bounds__(MT) void k_mord(const DBody *__restrict__ b, int n,
                                              const double *__restrict__ rr,
                                              const int *__restrict__ sx,
                                              const int *__restrict__ sid,
-                                             const unsigned *__restrict__ skey, int d, int cp,
-                                             double cx, double cy, double cz, double mu, double G,
+                                             const unsigned *__restrict__ skey, int d, const double *__restrict__ cen, double G,
                                              double s, const int *__restrict__ bvk,
                                              const int *__restrict__ bvoff,
                                              const int *__restrict__ bvo, int nbv,
@@ -663,6 +905,9 @@ __global__ __launch_bounds__(MT) void k_mord(const DBody *__restrict__ b, int n,
                                              const unsigned *__restrict__ le,
                                              double *__restrict__ acc)
 {
+    const int cp = cen && cen[7] != 0.0;
+    const double cx = cp ? cen[0] : 0.0, cy = cp ? cen[1] : 0.0, cz = cp ? cen[2] : 0.0;
+    const double mu = cp ? G * cen[6] : 0.0;
     const int t = blockIdx.x * MT + threadIdx.x;
     if (t >= ins || !sx[t]) return;
     const int k = sid[t];
@@ -734,13 +979,15 @@ __global__ __launch_bounds__(MT) void k_mord(const DBody *__restrict__ b, int n,
 // recorded big visits in visiting order (their own one included).
 __global__ __launch_bounds__(MT) void k_mrest_ord(const DBody *__restrict__ b, int n,
                                                   const int32_t *__restrict__ ex,
-                                                  const unsigned *__restrict__ key, int cp,
-                                                  double cx, double cy, double cz, double mu,
+                                                  const unsigned *__restrict__ key, const double *__restrict__ cen,
                                                   double G, double s, const int *__restrict__ bvk,
                                                   const int *__restrict__ bvoff,
                                                   const int *__restrict__ bvo, int nbv,
                                                   double *__restrict__ acc)
 {
+    const int cp = cen && cen[7] != 0.0;
+    const double cx = cp ? cen[0] : 0.0, cy = cp ? cen[1] : 0.0, cz = cp ? cen[2] : 0.0;
+    const double mu = cp ? G * cen[6] : 0.0;
     const int k = 3 + blockIdx.x * MT + threadIdx.x;
     if (k >= n || (ex && !ex[k]) || key[k] != NOKEY) return;
     const DBody q = b[k];
@@ -776,10 +1023,11 @@ constexpr int SQ_CH = 768;  // bodies per chunk of the body-order sums
 // Layout: chunk c is a [12][SQ_CH] block, so that each chain's terms are
 // contiguous.
 __global__ __launch_bounds__(MT) void k_mterms(const DBody *__restrict__ b, int n,
-                                               const int32_t *__restrict__ ex, int cp, double cx,
-                                               double cy, double cz, double G, double s,
+                                               const int32_t *__restrict__ ex, const double *__restrict__ cen, double G, double s,
                                                int nslots, double *__restrict__ T)
 {
+    const int cp = cen && cen[7] != 0.0;
+    const double cx = cp ? cen[0] : 0.0, cy = cp ? cen[1] : 0.0, cz = cp ? cen[2] : 0.0;
     const int k = blockIdx.x * MT + threadIdx.x;
     if (k >= nslots) return;
     double v[12];
@@ -844,11 +1092,13 @@ static_assert((SQ_CH / 2) % (2 * SQ_G) == 0, "whole ping-pong rounds per chunk")
 // (physics.c:296-341).  One block of 256 threads; producer and consumer run
 // separate loops with the same barriers.
 __global__ __launch_bounds__(256) void k_mseq(const DBody *__restrict__ b, int n,
-                                              const int32_t *__restrict__ ex, int cp, double cx,
-                                              double cy, double cz, double mu, double G, double s,
+                                              const int32_t *__restrict__ ex, const double *__restrict__ cen, double G, double s,
                                               const double2 *__restrict__ T, int nch,
                                               double *__restrict__ acc, double *__restrict__ cm)
 {
+    const int cp = cen && cen[7] != 0.0;
+    const double cx = cp ? cen[0] : 0.0, cy = cp ? cen[1] : 0.0, cz = cp ? cen[2] : 0.0;
+    const double mu = cp ? G * cen[6] : 0.0;
     // Rows padded by two doubles: 16-byte aligned for paired reads, and the
     // 12 summing lanes start on different banks.
     __shared__ double S[2][12][SQ_CH + 2];
@@ -1049,6 +1299,88 @@ unsigned host_key(double x, double y, double z, double gam, double cube, long d)
     return (unsigned)(i + j * d + k * d * d);
 }
 
+constexpr int MESH_BIG_CAP = 1 << 16;
+
+// Device buffers of one binning (kept in the context between the pass and
+// the field).
+struct MeshBin {
+    unsigned *key, *skey, *cs, *ce, *pk;
+    int *idx, *sid, *cbi, *cnt, *bigl;
+    double4 *pr, *vm;
+    unsigned long long *tot;
+    void *tmp;
+    size_t tmp_bytes;
+};
+
+// Bins the bodies db (device) in the reference's cells: keys, (key, id) radix
+// sort, cell ranges, sorted copies; counters cnt (inserted, small, big,
+// big-list length) and tot stay on the device.  Clears the cells of the
+// previous pass first.
+ncg_status mesh_bin(ncg_ctx *ctx, const DBody *db, int32_t n, const int32_t *dex, double gam,
+                    double cube, long d, long long nkeys, int mode, MeshBin &B, hipStream_t st)
+{
+    MeshState &M = ctx->mesh;
+    unsigned *key = dbuf<unsigned>(ctx, "m_key", n);
+    int *idx = dbuf<int>(ctx, "m_idx", n);
+    unsigned *skey = dbuf<unsigned>(ctx, "m_skey", n);
+    int *sid = dbuf<int>(ctx, "m_sid", n);
+    int *cbi = dbuf<int>(ctx, "m_count", n);
+    double4 *pr = dbuf<double4>(ctx, "m_pr", n);
+    double4 *vm = dbuf<double4>(ctx, "m_vm", n);
+    int *cnt = dbuf<int>(ctx, "m_cnt", 4);
+    unsigned long long *tot = dbuf<unsigned long long>(ctx, "m_tot", 2);
+    const int big_cap = MESH_BIG_CAP;
+    int *bigl = dbuf<int>(ctx, "m_big", big_cap);
+    if (!key || !idx || !skey || !sid || !cbi || !pr || !vm || !cnt || !tot || !bigl)
+        return NCG_ERR_NOMEM;
+    // Dense cell table; cleared once when (re)allocated, afterwards only the
+    // keys of the previous pass are reset.
+    if (M.table_keys < nkeys) {
+        M.table_keys = 0;
+        M.prev_ins = 0;
+        unsigned *cs0 = dbuf<unsigned>(ctx, "m_cs", nkeys);
+        unsigned *ce0 = dbuf<unsigned>(ctx, "m_ce", nkeys);
+        if (!cs0 || !ce0) return NCG_ERR_NOMEM;
+        NCG_HIP(hipMemsetAsync(cs0, 0, sizeof(unsigned) * nkeys, st));
+        NCG_HIP(hipMemsetAsync(ce0, 0, sizeof(unsigned) * nkeys, st));
+        M.table_keys = nkeys;
+    }
+    unsigned *cs = dbuf<unsigned>(ctx, "m_cs", nkeys);
+    unsigned *ce = dbuf<unsigned>(ctx, "m_ce", nkeys);
+    if (M.prev_ins > 0) {
+        hipLaunchKernelGGL(k_clear_cells, dim3(cdiv(M.prev_ins, MT)), dim3(MT), 0, st,
+                           (const unsigned *)dbuf<unsigned>(ctx, "m_prevkeys", M.prev_ins),
+                           M.prev_ins, cs, ce);
+        NCG_CHECK_LAUNCH();
+        M.prev_ins = 0;
+    }
+    // Sized for every body (growth happens here, after the clear above).
+    unsigned *pk = dbuf<unsigned>(ctx, "m_prevkeys", n);
+    size_t tmp_bytes = 0;
+    NCG_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, key, skey, idx, sid, n, 0, 32, st));
+    void *tmp = ncg_dbuf(ctx, "m_sorttmp", tmp_bytes);
+    if (!pk || !tmp) return NCG_ERR_NOMEM;
+    NCG_HIP(hipMemsetAsync(cnt, 0, sizeof(int) * 4, st));
+    NCG_HIP(hipMemsetAsync(tot, 0, sizeof(unsigned long long) * 2, st));
+    const unsigned g = (unsigned)cdiv(n, MT);
+    {
+        // Binning: keys, (key, id) radix sort, cell ranges, sorted copies.
+        // The inserted count stays on the device (cnt[0]): no host wait.
+        ProfScope ps(ctx, st, "mesh_bin");
+        hipLaunchKernelGGL(k_mkey, dim3(g), dim3(MT), 0, st, db, (int)n, dex, gam, cube,
+                           (int)d, (int)mode, key, idx, cnt, bigl, big_cap);
+        NCG_CHECK_LAUNCH();
+        // NOKEY (all ones) sorts last; every real key is below 2^32 - 1.
+        NCG_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, key, skey, idx, sid, n, 0, 32, st));
+        hipLaunchKernelGGL(k_bounds, dim3(g), dim3(MT), 0, st, skey, 0, cnt, cs, ce);
+        hipLaunchKernelGGL(k_msorted, dim3(g), dim3(MT), 0, st, db, sid, 0, cnt, pr, vm);
+        hipLaunchKernelGGL(k_copy_keys, dim3(g), dim3(MT), 0, st, skey, cnt, pk);
+        NCG_CHECK_LAUNCH();
+    }
+    B = MeshBin{key, skey, cs, ce, pk, idx, sid, cbi, cnt, bigl, pr, vm, tot, tmp, tmp_bytes};
+    return NCG_OK;
+}
+
 ncg_status load_cells(ncg_ctx *ctx, HostCells &hc, int32_t n, int ins, const unsigned *dkey,
                       const unsigned *dskey, const int *dsid)
 {
@@ -1091,12 +1423,14 @@ extern "C" ncg_status ncg_mesh_pass(ncg_ctx *ctx, const ncg_body *bodies, int32_
     hipStream_t st = ctx->stream;
     MeshState &M = ctx->mesh;
     M.valid = false;
+    M.pairs_known = false;
     M.ord_lists = false;
     M.extra_pairs.clear();
     M.big_visits.clear();
     M.collided.clear();
     if (n == 0) {
         M.valid = true;
+        M.pairs_known = true;
         M.n = 0;
         M.inserted = 0;
         return NCG_OK;
@@ -1104,69 +1438,24 @@ extern "C" ncg_status ncg_mesh_pass(ncg_ctx *ctx, const ncg_body *bodies, int32_
 
     DBody *db = dbuf<DBody>(ctx, "m_b", n);
     int32_t *dex = exists ? dbuf<int32_t>(ctx, "m_ex", n) : nullptr;
-    unsigned *key = dbuf<unsigned>(ctx, "m_key", n);
-    int *idx = dbuf<int>(ctx, "m_idx", n);
-    unsigned *skey = dbuf<unsigned>(ctx, "m_skey", n);
-    int *sid = dbuf<int>(ctx, "m_sid", n);
-    int *cbi = dbuf<int>(ctx, "m_count", n);
-    double4 *pr = dbuf<double4>(ctx, "m_pr", n);
-    double4 *vm = dbuf<double4>(ctx, "m_vm", n);
-    int *cnt = dbuf<int>(ctx, "m_cnt", 4);
-    unsigned long long *tot = dbuf<unsigned long long>(ctx, "m_tot", 2);
-    const int big_cap = 1 << 16;
-    int *bigl = dbuf<int>(ctx, "m_big", big_cap);
-    if (!db || (exists && !dex) || !key || !idx || !skey || !sid || !cbi || !pr || !vm || !cnt ||
-        !tot || !bigl)
-        return NCG_ERR_NOMEM;
-    // Dense cell table; cleared once when (re)allocated, afterwards only the
-    // keys of the previous pass are reset.
-    if (M.table_keys < nkeys) {
-        M.table_keys = 0;
-        M.prev_ins = 0;
-        unsigned *cs0 = dbuf<unsigned>(ctx, "m_cs", nkeys);
-        unsigned *ce0 = dbuf<unsigned>(ctx, "m_ce", nkeys);
-        if (!cs0 || !ce0) return NCG_ERR_NOMEM;
-        NCG_HIP(hipMemsetAsync(cs0, 0, sizeof(unsigned) * nkeys, st));
-        NCG_HIP(hipMemsetAsync(ce0, 0, sizeof(unsigned) * nkeys, st));
-        M.table_keys = nkeys;
-    }
-    unsigned *cs = dbuf<unsigned>(ctx, "m_cs", nkeys);
-    unsigned *ce = dbuf<unsigned>(ctx, "m_ce", nkeys);
-    if (M.prev_ins > 0) {
-        hipLaunchKernelGGL(k_clear_cells, dim3(cdiv(M.prev_ins, MT)), dim3(MT), 0, st,
-                           (const unsigned *)dbuf<unsigned>(ctx, "m_prevkeys", M.prev_ins),
-                           M.prev_ins, cs, ce);
-        NCG_CHECK_LAUNCH();
-        M.prev_ins = 0;
-    }
-    // Sized for every body (growth happens here, after the clear above).
-    unsigned *pk = dbuf<unsigned>(ctx, "m_prevkeys", n);
-    size_t tmp_bytes = 0;
-    NCG_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, key, skey, idx, sid, n, 0, 32, st));
-    void *tmp = ncg_dbuf(ctx, "m_sorttmp", tmp_bytes);
-    unsigned long long cap = 1 << 16;
-    int4 *dc = mode == 0 ? dbuf<int4>(ctx, "m_cand", cap) : nullptr;
-    if (!pk || !tmp || (mode == 0 && !dc)) return NCG_ERR_NOMEM;
+    if (!db || (exists && !dex)) return NCG_ERR_NOMEM;
     NCG_HIP(hipMemcpyAsync(db, bodies, sizeof(DBody) * (size_t)n, hipMemcpyHostToDevice, st));
     if (exists)
         NCG_HIP(hipMemcpyAsync(dex, exists, sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, st));
-    NCG_HIP(hipMemsetAsync(cnt, 0, sizeof(int) * 4, st));
-    NCG_HIP(hipMemsetAsync(tot, 0, sizeof(unsigned long long) * 2, st));
+    MeshBin B;
+    ncg_status rb = mesh_bin(ctx, db, n, dex, gam, collision_cube, d, nkeys, mode, B, st);
+    if (rb != NCG_OK) return rb;
+    unsigned *key = B.key, *skey = B.skey, *cs = B.cs, *ce = B.ce, *pk = B.pk;
+    int *idx = B.idx, *sid = B.sid, *cbi = B.cbi, *cnt = B.cnt, *bigl = B.bigl;
+    double4 *pr = B.pr, *vm = B.vm;
+    unsigned long long *tot = B.tot;
+    void *tmp = B.tmp;
+    size_t tmp_bytes = B.tmp_bytes;
+    const int big_cap = MESH_BIG_CAP;
+    unsigned long long cap = 1 << 16;
+    int4 *dc = mode == 0 ? dbuf<int4>(ctx, "m_cand", cap) : nullptr;
+    if (mode == 0 && !dc) return NCG_ERR_NOMEM;
     const unsigned g = (unsigned)cdiv(n, MT);
-    {
-        // Binning: keys, (key, id) radix sort, cell ranges, sorted copies.
-        // The inserted count stays on the device (cnt[0]): no host wait.
-        ProfScope ps(ctx, st, "mesh_bin");
-        hipLaunchKernelGGL(k_mkey, dim3(g), dim3(MT), 0, st, db, (int)n, dex, gam, collision_cube,
-                           (int)d, (int)mode, key, idx, cnt, bigl, big_cap);
-        NCG_CHECK_LAUNCH();
-        // NOKEY (all ones) sorts last; every real key is below 2^32 - 1.
-        NCG_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, key, skey, idx, sid, n, 0, 32, st));
-        hipLaunchKernelGGL(k_bounds, dim3(g), dim3(MT), 0, st, skey, 0, cnt, cs, ce);
-        hipLaunchKernelGGL(k_msorted, dim3(g), dim3(MT), 0, st, db, sid, 0, cnt, pr, vm);
-        hipLaunchKernelGGL(k_copy_keys, dim3(g), dim3(MT), 0, st, skey, cnt, pk);
-        NCG_CHECK_LAUNCH();
-    }
     int hc4[4];
     NCG_HIP(hipMemcpyAsync(hc4, cnt, sizeof hc4, hipMemcpyDeviceToHost, st));
     std::vector<int4> cand;
@@ -1220,6 +1509,7 @@ extern "C" ncg_status ncg_mesh_pass(ncg_ctx *ctx, const ncg_body *bodies, int32_
     M.gam = gam;
     M.cube = collision_cube;
     M.valid = true;
+    M.pairs_known = true;
     if (mode == 1) return NCG_OK;
 
     // ---- host replay of collision.c:339-415 ----------------------------------
@@ -1445,46 +1735,146 @@ extern "C" ncg_status ncg_mesh_pass(ncg_ctx *ctx, const ncg_body *bodies, int32_
     return NCG_OK;
 }
 
-extern "C" ncg_status ncg_mesh_gravity(ncg_ctx *ctx, const ncg_body *bodies, int32_t n,
-                                       const int32_t *exists, const ncg_central *central,
-                                       double grav_const, double softening, ncg_precision prec,
-                                       double *acc, double *cm_acc)
+extern "C" ncg_status ncg_mesh_pass_dev(ncg_ctx *ctx, ncg_body *d_bodies, int32_t n,
+                                        const int32_t *d_exists, int32_t *d_did_collide,
+                                        double timestep, double gam, double collision_cube,
+                                        int32_t cells, int32_t mode, double collision_f,
+                                        int32_t one_collision_only, ncg_mesh_stats *stats,
+                                        void *stream)
 {
     if (!ctx) return NCG_ERR_INVALID;
-    NCG_REQUIRE(n >= 0 && (n == 0 || (bodies && acc)), "invalid arguments");
-    NCG_REQUIRE(prec == NCG_F64_EXACT || prec == NCG_F64_FAST || prec == NCG_MIXED,
-                "unknown precision %d", (int)prec);
-    MeshState &M = ctx->mesh;
-    if (!M.valid || M.n != n)
-        return ncg_fail(ctx, NCG_ERR_INVALID,
-                        "ncg_mesh_gravity needs a ncg_mesh_pass over the same %d body slots", n);
-    if (n == 0) return NCG_OK;
+    NCG_REQUIRE(n >= 0 && (n == 0 || d_bodies) && (mode == 0 || mode == 1) && stats,
+                "invalid arguments");
+    NCG_REQUIRE(cells >= 3 && gam > 0.0 && collision_cube > 0.0, "invalid mesh geometry");
+    NCG_REQUIRE(mode == 1 || collision_f > 0.0, "mesh() on the device needs collision_f > 0");
+    NCG_REQUIRE(n < (1 << 29), "the device replay takes fewer than 2^29 body slots");
+    NCG_REQUIRE(!(mode == 0 && one_collision_only && !d_did_collide),
+                "one_collision_only needs a did_collide array");
+    const long d = cells;
+    const long long nkeys = (long long)(d + 2) * (1 + d + d * d) + 1;
+    NCG_REQUIRE(nkeys < 0xffffffffLL, "collision_cube_cells %d too large for 32-bit cell keys",
+                (int)cells);
+    memset(stats, 0, sizeof *stats);
     NCG_HIP(hipSetDevice(ctx->device));
-    hipStream_t st = ctx->stream;
+    hipStream_t st = pick_stream(ctx, stream);
+    MeshState &M = ctx->mesh;
+    M.valid = false;
+    M.pairs_known = false;
+    M.ord_lists = false;
+    M.extra_pairs.clear();
+    M.big_visits.clear();
+    M.big_pairs_all.clear();
+    M.collided.clear();
+    if (n == 0) {
+        M.valid = true;
+        M.pairs_known = true;
+        M.n = 0;
+        M.inserted = 0;
+        return NCG_OK;
+    }
+    DBody *db = reinterpret_cast<DBody *>(d_bodies);
+    MeshBin B;
+    ncg_status r = mesh_bin(ctx, db, n, d_exists, gam, collision_cube, d, nkeys, mode, B, st);
+    if (r != NCG_OK) return r;
+    const unsigned cap = 1u << 16;
+    int4 *dc = mode == 0 ? dbuf<int4>(ctx, "m_cand", cap) : nullptr;
+    unsigned *stat = dbuf<unsigned>(ctx, "m_rstat", 2);
+    unsigned *hs = pbuf<unsigned>(ctx, "m_hstat", 16);
+    if ((mode == 0 && !dc) || !stat || !hs) return NCG_ERR_NOMEM;
+    const unsigned g = (unsigned)cdiv(n, MT);
+    {
+        ProfScope ps(ctx, st, "mesh_count");
+        if (mode == 0)
+            hipLaunchKernelGGL(k_mcount<0>, dim3(g), dim3(MT), 0, st, B.skey, B.sid, B.pr, B.vm, B.cs,
+                               B.ce, 0, B.cnt, (int)d, timestep, B.cbi, dc, (unsigned long long)cap,
+                               B.tot + 1, B.tot);
+        else
+            hipLaunchKernelGGL(k_mcount<1>, dim3(g), dim3(MT), 0, st, B.skey, B.sid, B.pr, B.vm, B.cs,
+                               B.ce, 0, B.cnt, (int)d, timestep, B.cbi, (int4 *)nullptr, 0ull,
+                               B.tot + 1, B.tot);
+        NCG_CHECK_LAUNCH();
+    }
+    hipLaunchKernelGGL(k_lf_fill_u32, dim3(1), dim3(64), 0, st, stat, 2u);
+    if (mode == 0) {
+        // Candidates in visiting order, then the replay.
+        unsigned long long *kin = dbuf<unsigned long long>(ctx, "m_ckey", cap);
+        unsigned long long *kout = dbuf<unsigned long long>(ctx, "m_ckey2", cap);
+        unsigned *vin = dbuf<unsigned>(ctx, "m_cval", cap);
+        unsigned *vout = dbuf<unsigned>(ctx, "m_cval2", cap);
+        size_t tb = 0;
+        NCG_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, kin, kout, vin, vout, (int)cap, 0, 64, st));
+        void *tmp = ncg_dbuf(ctx, "m_csorttmp", tb);
+        if (!kin || !kout || !vin || !vout || !tmp) return NCG_ERR_NOMEM;
+        ProfScope ps(ctx, st, "mesh_replay");
+        hipLaunchKernelGGL(k_mcand_keys, dim3(cdiv(cap, MT)), dim3(MT), 0, st, dc, B.tot + 1, cap, kin,
+                           vin);
+        NCG_CHECK_LAUNCH();
+        NCG_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tb, kin, kout, vin, vout, (int)cap, 0, 64, st));
+        hipLaunchKernelGGL(k_mesh_replay, dim3(1), dim3(64), 0, st, db, d_exists, d_did_collide,
+                           (int)one_collision_only, collision_f, timestep, B.key, B.sid, B.cs, B.ce,
+                           B.pr, B.vm, (int)d, dc, vout, B.tot + 1, cap, B.cnt, stat);
+        NCG_CHECK_LAUNCH();
+    }
+    // Counters: inserted, small, big, big-list length; neighbours,
+    // candidates; collisions, error bits.
+    NCG_HIP(hipMemcpyAsync(hs, B.cnt, sizeof(int) * 4, hipMemcpyDeviceToHost, st));
+    NCG_HIP(hipMemcpyAsync(hs + 4, B.tot, sizeof(unsigned long long) * 2, hipMemcpyDeviceToHost, st));
+    NCG_HIP(hipMemcpyAsync(hs + 8, stat, sizeof(unsigned) * 2, hipMemcpyDeviceToHost, st));
+    NCG_HIP(hipStreamSynchronize(st));
+    const int *hc4 = reinterpret_cast<const int *>(hs);
+    unsigned long long ht[2];
+    memcpy(ht, hs + 4, sizeof ht);
+    const int ins = hc4[0];
+    M.prev_ins = ins;
+    M.n = n;
+    M.cells = cells;
+    M.mode = mode;
+    M.inserted = ins;
+    M.gam = gam;
+    M.cube = collision_cube;
+    stats->total_neighbours = (int64_t)ht[0];
+    stats->how_many_small = mode == 0 ? hc4[1] : 0;
+    stats->how_many_big = mode == 0 ? hc4[2] : 0;
+    stats->candidates = (int64_t)ht[1];
+    stats->collisions = hs[8];
+    if (hs[9] & 4u)
+        return ncg_fail(ctx, NCG_ERR_INVALID,
+                        "ncg_mesh_pass_dev: %d bodies are larger than the mesh size (gam = %g); "
+                        "use ncg_mesh_pass", hc4[2], gam);
+    if (hs[9])
+        return ncg_fail(ctx, NCG_ERR_CAPACITY,
+                        "ncg_mesh_pass_dev: replay capacity exceeded (flags %u: 1 = more than %u "
+                        "candidates, 2 = more than %d pending re-tests); the state was not changed "
+                        "only if flag 1 is set alone",
+                        hs[9], cap, MR_PCAP);
+    M.valid = true;
+    M.pairs_known = mode == 1;
+    return NCG_OK;
+}
+
+// Mesh field of the bodies db (device), central body dcen (8-double record
+// {x, y, z, vx, vy, vz, M, present} or null), over the binning of the last
+// pass, into dacc (3 per slot) and dcm (CM_acc).
+static ncg_status mesh_gravity_core(ncg_ctx *ctx, const DBody *db, int32_t n, const int32_t *dex,
+                                    const double *dcen, double G, double s, ncg_precision prec,
+                                    double *dacc, double *dcm, hipStream_t st)
+{
+    MeshState &M = ctx->mesh;
     const long d = M.cells;
     const int ins = M.inserted;
-    const int cp = (central && central->present) ? 1 : 0;
-    DBody *db = dbuf<DBody>(ctx, "g_b", n);
-    int32_t *dex = exists ? dbuf<int32_t>(ctx, "g_ex", n) : nullptr;
-    double *dacc = dbuf<double>(ctx, "g_acc", 3 * (size_t)n);
     double *dct = dbuf<double>(ctx, "g_cterm", 3 * (size_t)n);
     double4 *pm = dbuf<double4>(ctx, "g_pm", std::max(ins, 1));
     double *rr = dbuf<double>(ctx, "g_rr", std::max(ins, 1));
     int *sx = dbuf<int>(ctx, "g_sx", std::max(ins, 1));
-    double *dcm = dbuf<double>(ctx, "g_cm", 4);
     const int nbp = std::max(1, std::min(512, (int)cdiv(n, MT)));
     double *bpart = dbuf<double>(ctx, "g_bpart", 9 * (size_t)nbp);
     const size_t nex = std::max<size_t>(M.extra_pairs.size(), 1);
     int2 *dextra = dbuf<int2>(ctx, "g_extra", nex);
-    if (!db || (exists && !dex) || !dacc || !dct || !pm || !rr || !sx || !dcm || !bpart || !dextra)
-        return NCG_ERR_NOMEM;
+    if (!dct || !pm || !rr || !sx || !bpart || !dextra) return NCG_ERR_NOMEM;
     const unsigned *key = dbuf<unsigned>(ctx, "m_key", n);
     const int *sid = dbuf<int>(ctx, "m_sid", n);
     const unsigned *cs = dbuf<unsigned>(ctx, "m_cs", M.table_keys);
     const unsigned *ce = dbuf<unsigned>(ctx, "m_ce", M.table_keys);
-    NCG_HIP(hipMemcpyAsync(db, bodies, sizeof(DBody) * (size_t)n, hipMemcpyHostToDevice, st));
-    if (exists)
-        NCG_HIP(hipMemcpyAsync(dex, exists, sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, st));
     if (!M.extra_pairs.empty())
         NCG_HIP(hipMemcpyAsync(dextra, M.extra_pairs.data(), sizeof(int2) * M.extra_pairs.size(),
                                hipMemcpyHostToDevice, st));
@@ -1493,9 +1883,6 @@ extern "C" ncg_status ncg_mesh_gravity(ncg_ctx *ctx, const ncg_body *bodies, int
                            rr, sx);
         NCG_CHECK_LAUNCH();
     }
-    const double G = grav_const, s = softening;
-    const double mu = cp ? G * central->mass : 0.0;
-    const double cxh = cp ? central->x : 0.0, cyh = cp ? central->y : 0.0, czh = cp ? central->z : 0.0;
     if (prec == NCG_F64_EXACT) {
         // Pair-list order, bit for bit: recorded big visits as (visiting id,
         // ascending partner ids), bodies 0..2 and CM_acc on a second stream.
@@ -1563,21 +1950,21 @@ extern "C" ncg_status ncg_mesh_gravity(ncg_ctx *ctx, const ncg_body *bodies, int
         if (!T) return NCG_ERR_NOMEM;
         ProfScope ps(ctx, st, "mesh_gravity");
         hipLaunchKernelGGL(k_mterms, dim3(cdiv(nch * SQ_CH, MT)), dim3(MT), 0, st, db, (int)n, dex,
-                           cp, cxh, cyh, czh, G, s, nch * SQ_CH, T);
+                           dcen, G, s, nch * SQ_CH, T);
         NCG_CHECK_LAUNCH();
         NCG_HIP(hipEventRecord(ctx->ev_fork, st));
         NCG_HIP(hipStreamWaitEvent(ctx->aux, ctx->ev_fork, 0));
-        hipLaunchKernelGGL(k_mseq, dim3(1), dim3(256), 0, ctx->aux, db, (int)n, dex, cp, cxh, cyh,
-                           czh, mu, G, s, reinterpret_cast<const double2 *>(T), nch, dacc, dcm);
+        hipLaunchKernelGGL(k_mseq, dim3(1), dim3(256), 0, ctx->aux, db, (int)n, dex, dcen, G, s,
+                           reinterpret_cast<const double2 *>(T), nch, dacc, dcm);
         NCG_CHECK_LAUNCH();
         NCG_HIP(hipEventRecord(ctx->ev_join, ctx->aux));
         if (ins > 0)
             hipLaunchKernelGGL(k_mord, dim3(cdiv(ins, MT)), dim3(MT), 0, st, db, (int)n, dex, ins,
-                               cs, ce, pm, rr, sx, sid, skey, (int)d, cp, cxh, cyh, czh, mu, G, s,
+                               cs, ce, pm, rr, sx, sid, skey, (int)d, dcen, G, s,
                                bvk, bvoff, bvo, nbv, L, lw, le, dacc);
         if (n > 3)
             hipLaunchKernelGGL(k_mrest_ord, dim3(cdiv(n - 3, MT)), dim3(MT), 0, st, db, (int)n, dex,
-                               key, cp, cxh, cyh, czh, mu, G, s, bvk, bvoff, bvo, nbv, dacc);
+                               key, dcen, G, s, bvk, bvoff, bvo, nbv, dacc);
         NCG_CHECK_LAUNCH();
         NCG_HIP(hipStreamWaitEvent(st, ctx->ev_join, 0));
     } else {
@@ -1586,21 +1973,61 @@ extern "C" ncg_status ncg_mesh_gravity(ncg_ctx *ctx, const ncg_body *bodies, int
         const dim3 gs(cdiv(std::max(ins, 1), MT));
         if (prec == NCG_F64_FAST)
             hipLaunchKernelGGL(k_mgrav<1>, gs, dim3(MT), 0, st, db, (int)n, dex, ins, cs, ce, pm, rr,
-                               sx, sid, skey, (int)d, cp, cxh, cyh, czh, mu, G, s, dacc, dct);
+                               sx, sid, skey, (int)d, dcen, G, s, dacc, dct);
         else
             hipLaunchKernelGGL(k_mgrav<2>, gs, dim3(MT), 0, st, db, (int)n, dex, ins, cs, ce, pm, rr,
-                               sx, sid, skey, (int)d, cp, cxh, cyh, czh, mu, G, s, dacc, dct);
-        hipLaunchKernelGGL(k_mgrav_rest, dim3(cdiv(n, MT)), dim3(MT), 0, st, db, (int)n, dex, key, cp,
-                           cxh, cyh, czh, mu, G, s, dacc, dct);
+                               sx, sid, skey, (int)d, dcen, G, s, dacc, dct);
+        hipLaunchKernelGGL(k_mgrav_rest, dim3(cdiv(n, MT)), dim3(MT), 0, st, db, (int)n, dex, key, dcen,
+                           G, s, dacc, dct);
         NCG_CHECK_LAUNCH();
         hipLaunchKernelGGL(k_big_partial, dim3(nbp), dim3(MT), 0, st, db, (int)n, dex, G, s, bpart);
         hipLaunchKernelGGL(k_big_final, dim3(1), dim3(MT), 0, st, db, (int)n, dex, bpart, nbp, G, s,
-                           dacc, dct, cp, dcm);
+                           dacc, dct, dcen, dcm);
         if (!M.extra_pairs.empty())
             hipLaunchKernelGGL(k_extra_pairs, dim3(1), dim3(1), 0, st, db, dex, dextra,
                                (int)M.extra_pairs.size(), G, s, dacc);
         NCG_CHECK_LAUNCH();
     }
+    return NCG_OK;
+}
+
+extern "C" ncg_status ncg_mesh_gravity(ncg_ctx *ctx, const ncg_body *bodies, int32_t n,
+                                       const int32_t *exists, const ncg_central *central,
+                                       double grav_const, double softening, ncg_precision prec,
+                                       double *acc, double *cm_acc)
+{
+    if (!ctx) return NCG_ERR_INVALID;
+    NCG_REQUIRE(n >= 0 && (n == 0 || (bodies && acc)), "invalid arguments");
+    NCG_REQUIRE(prec == NCG_F64_EXACT || prec == NCG_F64_FAST || prec == NCG_MIXED,
+                "unknown precision %d", (int)prec);
+    MeshState &M = ctx->mesh;
+    if (!M.valid || M.n != n)
+        return ncg_fail(ctx, NCG_ERR_INVALID,
+                        "ncg_mesh_gravity needs a ncg_mesh_pass over the same %d body slots", n);
+    if (n == 0) return NCG_OK;
+    NCG_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    const int cp = (central && central->present) ? 1 : 0;
+    DBody *db = dbuf<DBody>(ctx, "g_b", n);
+    int32_t *dex = exists ? dbuf<int32_t>(ctx, "g_ex", n) : nullptr;
+    double *dacc = dbuf<double>(ctx, "g_acc", 3 * (size_t)n);
+    double *dcm = dbuf<double>(ctx, "g_cm", 4);
+    double *dcen = dbuf<double>(ctx, "g_cen", 8);
+    double *hcen = pbuf<double>(ctx, "g_hcen", 8);
+    if (!db || (exists && !dex) || !dacc || !dcm || !dcen || !hcen) return NCG_ERR_NOMEM;
+    NCG_HIP(hipMemcpyAsync(db, bodies, sizeof(DBody) * (size_t)n, hipMemcpyHostToDevice, st));
+    if (exists)
+        NCG_HIP(hipMemcpyAsync(dex, exists, sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, st));
+    if (cp) {
+        // Central body as the device record the kernels read (through pinned
+        // memory; every call ends with a synchronisation, so it is free).
+        const double rec[8] = {central->x, central->y, central->z, 0.0, 0.0, 0.0, central->mass, 1.0};
+        for (int q = 0; q < 8; ++q) hcen[q] = rec[q];
+        NCG_HIP(hipMemcpyAsync(dcen, hcen, sizeof(double) * 8, hipMemcpyHostToDevice, st));
+    }
+    ncg_status r = mesh_gravity_core(ctx, db, n, dex, cp ? dcen : nullptr, grav_const, softening,
+                                     prec, dacc, dcm, st);
+    if (r != NCG_OK) return r;
     double *hacc = hbuf<double>(ctx, "macc", 3 * (size_t)n);
     double hcm[4] = {0, 0, 0, 0};
     NCG_HIP(hipMemcpyAsync(hacc, dacc, sizeof(double) * 3 * (size_t)n, hipMemcpyDeviceToHost, st));
@@ -1620,12 +2047,35 @@ extern "C" ncg_status ncg_mesh_gravity(ncg_ctx *ctx, const ncg_body *bodies, int
     return NCG_OK;
 }
 
+extern "C" ncg_status ncg_mesh_gravity_dev(ncg_ctx *ctx, const ncg_body *d_bodies, int32_t n,
+                                           const int32_t *d_exists, const double *d_central,
+                                           double grav_const, double softening, ncg_precision prec,
+                                           double *d_acc, double *d_cm_acc, void *stream)
+{
+    if (!ctx) return NCG_ERR_INVALID;
+    NCG_REQUIRE(n >= 0 && (n == 0 || (d_bodies && d_acc)), "invalid arguments");
+    NCG_REQUIRE(prec == NCG_F64_EXACT || prec == NCG_F64_FAST || prec == NCG_MIXED,
+                "unknown precision %d", (int)prec);
+    MeshState &M = ctx->mesh;
+    if (!M.valid || M.n != n)
+        return ncg_fail(ctx, NCG_ERR_INVALID,
+                        "ncg_mesh_gravity_dev needs a mesh pass over the same %d body slots", n);
+    if (n == 0) return NCG_OK;
+    NCG_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = pick_stream(ctx, stream);
+    double *dcm = d_cm_acc ? d_cm_acc : dbuf<double>(ctx, "g_cm", 4);
+    if (!dcm) return NCG_ERR_NOMEM;
+    return mesh_gravity_core(ctx, reinterpret_cast<const DBody *>(d_bodies), n, d_exists,
+                             d_central, grav_const, softening, prec, d_acc, dcm, st);
+}
+
 extern "C" ncg_status ncg_mesh_pairs(ncg_ctx *ctx, ncg_pair *out, int64_t capacity, int64_t *count)
 {
     if (!ctx) return NCG_ERR_INVALID;
     NCG_REQUIRE(count && capacity >= 0 && (capacity == 0 || out), "invalid arguments");
     MeshState &M = ctx->mesh;
     NCG_REQUIRE(M.valid, "no mesh pass");
+    NCG_REQUIRE(M.pairs_known, "the pair list of a device-replayed mesh() pass is not kept");
     *count = 0;
     const int32_t n = M.n;
     if (n == 0) return NCG_OK;
