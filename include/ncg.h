@@ -230,6 +230,40 @@ ncg_status ncg_mesh_pass_dev(ncg_ctx *ctx, ncg_body *d_bodies, int32_t n, const 
                              double collision_f, int32_t one_collision_only,
                              ncg_mesh_stats *stats, void *stream);
 
+/* Mesh geometry of NcorpiON's mesh integrator (rk4.c:662-837). */
+typedef struct {
+    double mesh_size;              /* gam: updated after every step as end_of_timestep does
+                                      (rk4.c:370-377) and written back */
+    double mesh_size_min;          /* gam_min = collision_cube_min / collision_cube_cells */
+    double target_neighbours;      /* how_many_neighbours: target average of neighbours */
+    double r_disruption;           /* disruption_threshold: checked, no body may come this
+                                      close to the central body */
+    double r_high_dumping;         /* high_dumping_threshold: checked, no body may go this far */
+    int32_t cells;                 /* collision_cube_cells */
+    int32_t pad_;
+} ncg_mesh_loop_params;
+
+/* GPU-resident mesh loop: NcorpiON's integration_mesh (SBAB1, rk4.c:662-837)
+ * with bodies and central body on the device.  Every step: three_largest_moonlets
+ * and three_largest_three_first (structure.c:914-1064) on the device; with
+ * collision_f > 0, mesh() (ncg_mesh_pass_dev, the library's bounce) then
+ * drift; without, drift then get_neighbours_mesh(); kick with the mesh field
+ * of `prec` (NCG_F64_EXACT: the reference's arithmetic); end_of_timestep's
+ * did_collide reset (one_collision_only) and mesh-size update.  flags bit 1:
+ * the half kick before the loop (rk4.c:697-717); bit 2: the backward half
+ * kick after it (rk4.c:807-812, the central record taking CM_buffer's kick).
+ * Needs every body at most the mesh size and no body crossing the disruption
+ * or high-dumping thresholds (end_of_timestep's removals are not applied:
+ * NCG_ERR_INVALID).  One synchronisation per mesh pass (its counters set the
+ * next mesh size).  d_exists may be written (a body moved into an empty slot
+ * 0..2).  *n_collisions (may be NULL): collisions resolved. */
+ncg_status ncg_mesh_leapfrog_dev(ncg_ctx *ctx, ncg_body *d_bodies, int32_t n, int32_t *d_exists,
+                                 int32_t *d_did_collide, double *d_central, double grav_const,
+                                 double softening, ncg_precision prec, double dt, int32_t nsteps,
+                                 ncg_mesh_loop_params *params, double collision_f,
+                                 int32_t one_collision_only, int32_t flags, int64_t *n_collisions,
+                                 void *stream);
+
 /* Mesh branch of vector_field (physics.c:63-109, 120-146, 204-261, 296-380):
  * central term, bodies 0..2 against every body, and the neighbour pairs of
  * the last ncg_mesh_pass (bodies > 2 only).  Outputs as ncg_gravity_allpairs.

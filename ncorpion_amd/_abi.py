@@ -40,6 +40,13 @@ class ReplayOps(ctypes.Structure):
                 ("created", CREATED_FN)]
 
 
+class MeshLoopParams(ctypes.Structure):
+    _fields_ = [("mesh_size", ctypes.c_double), ("mesh_size_min", ctypes.c_double),
+                ("target_neighbours", ctypes.c_double), ("r_disruption", ctypes.c_double),
+                ("r_high_dumping", ctypes.c_double), ("cells", ctypes.c_int32),
+                ("pad_", ctypes.c_int32)]
+
+
 class BuiltinCollider(ctypes.Structure):
     _fields_ = [("bodies", ctypes.c_void_p), ("exists", ctypes.c_void_p),
                 ("did_collide", ctypes.c_void_p), ("timestep", ctypes.c_double),
@@ -99,6 +106,9 @@ _SIGNATURES = {
                           ctypes.POINTER(MeshStats)]),
     "ncg_mesh_gravity": (S, [P, P, I32, P, ctypes.POINTER(Central), D, D, S, P, P]),
     "ncg_mesh_gravity_dev": (S, [P, P, I32, P, P, D, D, S, P, P, P]),
+    "ncg_mesh_leapfrog_dev": (S, [P, P, I32, P, P, P, D, D, S, D, I32,
+                                  ctypes.POINTER(MeshLoopParams), D, I32, I32,
+                                  ctypes.POINTER(I64), P]),
     "ncg_mesh_pass_dev": (S, [P, P, I32, P, P, D, D, D, I32, I32, D, I32,
                               ctypes.POINTER(MeshStats), P]),
     "ncg_mesh_pairs": (S, [P, P, I64, ctypes.POINTER(I64)]),
@@ -302,6 +312,30 @@ class Context:
             float(cube), int(cells), int(mode), float(collision_f), int(one_collision_only),
             ctypes.byref(stats), stream))
         return {f: int(getattr(stats, f)) for f, _ in MeshStats._fields_}
+
+    def mesh_leapfrog_dev(self, bodies_t, central_t, nsteps, dt, params, G=1.0, softening=0.0,
+                          precision="exact", collision_f=0.0, one_collision_only=0,
+                          exists_t=None, did_collide_t=None, half_kick_start=False,
+                          half_kick_end=False, stream=None):
+        """NcorpiON's mesh integrator on the device (see ncg_mesh_leapfrog_dev).
+        params: dict with gam, gam_min, how_many_neighbours, disruption_threshold,
+        high_dumping_threshold, cells; params["gam"] is updated in place.
+        Returns the number of collisions resolved."""
+        n = int(bodies_t.shape[0])
+        mp = MeshLoopParams(float(params["gam"]), float(params["gam_min"]),
+                            float(params["how_many_neighbours"]),
+                            float(params["disruption_threshold"]),
+                            float(params["high_dumping_threshold"]), int(params["cells"]), 0)
+        nc = I64()
+        flags = (2 if half_kick_start else 0) | (4 if half_kick_end else 0)
+        self._check(self.lib.ncg_mesh_leapfrog_dev(
+            self.h, bodies_t.data_ptr(), n, None if exists_t is None else exists_t.data_ptr(),
+            None if did_collide_t is None else did_collide_t.data_ptr(), central_t.data_ptr(),
+            float(G), float(softening), PRECISIONS[precision], float(dt), int(nsteps),
+            ctypes.byref(mp), float(collision_f), int(one_collision_only), flags,
+            ctypes.byref(nc), stream))
+        params["gam"] = mp.mesh_size
+        return int(nc.value)
 
     def mesh_gravity_dev(self, bodies_t, acc_t, central_t=None, cm_t=None, exists_t=None, G=1.0,
                          softening=0.0, precision="exact", stream=None):

@@ -70,7 +70,279 @@ __global__ void k_lf_off(int *__restrict__ off, int n)
     off[1] = n;
 }
 
+// three_largest_moonlets (structure.c:914-998) then three_largest_three_first
+// (structure.c:1001-1064), one block.  The sequential scan keeps, after its
+// first three existing bodies (ordered by its own rules, under which equal
+// radii put the second body first), a list sorted by decreasing radius in
+// which a later body displaces an earlier one only when strictly larger; so
+// the result is those three merged with the three largest of the rest by
+// (radius descending, index ascending), the first three winning ties.  Then
+// the swaps into slots 0, 1, 2 (moving a body into an empty slot frees its
+// old one, structure.c:1019-1060).  err |= 1 if fewer than three bodies exist.
+constexpr int T3 = 1024;
+
+struct Top3 {
+    double r[3];
+    int i[3];
+};
+
+__device__ inline bool t3_better(double ra, int ia, double rb, int ib)
+{
+    return ra > rb || (ra == rb && ia < ib);
+}
+
+__device__ inline void t3_insert(Top3 &t, double r, int i)
+{
+    if (i < 0) return;
+    if (t.i[0] < 0 || t3_better(r, i, t.r[0], t.i[0])) {
+        t.r[2] = t.r[1]; t.i[2] = t.i[1];
+        t.r[1] = t.r[0]; t.i[1] = t.i[0];
+        t.r[0] = r; t.i[0] = i;
+    } else if (t.i[1] < 0 || t3_better(r, i, t.r[1], t.i[1])) {
+        t.r[2] = t.r[1]; t.i[2] = t.i[1];
+        t.r[1] = r; t.i[1] = i;
+    } else if (t.i[2] < 0 || t3_better(r, i, t.r[2], t.i[2])) {
+        t.r[2] = r; t.i[2] = i;
+    }
+}
+
+__global__ __launch_bounds__(T3) void k_top3_first(DBody *__restrict__ b, int n,
+                                                   int32_t *__restrict__ ex,
+                                                   unsigned *__restrict__ err)
+{
+    __shared__ Top3 sh[T3];
+    __shared__ int s_j[3];
+    const int t = threadIdx.x;
+    if (t == 0) {
+        int c = 0;
+        for (int j = 0; j < n && c < 3; ++j)
+            if (!ex || ex[j]) s_j[c++] = j;
+        for (; c < 3; ++c) s_j[c] = -1;
+    }
+    __syncthreads();
+    if (s_j[2] < 0) {
+        if (t == 0) atomicOr(err, 1u);
+        return;
+    }
+    Top3 mine;
+    for (int q = 0; q < 3; ++q) {
+        mine.r[q] = 0.0;
+        mine.i[q] = -1;
+    }
+    for (int j = s_j[2] + 1 + t; j < n; j += T3)
+        if (!ex || ex[j]) t3_insert(mine, b[j].radius, j);
+    sh[t] = mine;
+    __syncthreads();
+    for (int w = T3 / 2; w > 0; w >>= 1) {
+        if (t < w) {
+            Top3 a = sh[t];
+            const Top3 &o = sh[t + w];
+            for (int q = 0; q < 3; ++q) t3_insert(a, o.r[q], o.i[q]);
+            sh[t] = a;
+        }
+        __syncthreads();
+    }
+    if (t != 0) return;
+    // The first three existing bodies by the scan's own rules (structure.c:932-970).
+    int L[3];
+    double R[3];
+    const int j0 = s_j[0], j1 = s_j[1], j2 = s_j[2];
+    const double r0 = b[j0].radius, r1 = b[j1].radius, r2 = b[j2].radius;
+    if (r0 > r1) {
+        L[0] = j0; R[0] = r0; L[1] = j1; R[1] = r1;
+    } else {
+        L[0] = j1; R[0] = r1; L[1] = j0; R[1] = r0;
+    }
+    if (R[1] >= r2) {
+        L[2] = j2; R[2] = r2;
+    } else if (r2 > R[1] && r2 <= R[0]) {
+        L[2] = L[1]; R[2] = R[1]; L[1] = j2; R[1] = r2;
+    } else {
+        L[2] = L[1]; R[2] = R[1]; L[1] = L[0]; R[1] = R[0]; L[0] = j2; R[0] = r2;
+    }
+    // Merge with the rest: a body of the rest enters only if strictly larger.
+    const Top3 rest = sh[0];
+    for (int q = 0; q < 3; ++q) {
+        const int i = rest.i[q];
+        if (i < 0) continue;
+        const double r = rest.r[q];
+        if (r > R[0]) {
+            L[2] = L[1]; R[2] = R[1]; L[1] = L[0]; R[1] = R[0]; L[0] = i; R[0] = r;
+        } else if (r > R[1]) {
+            L[2] = L[1]; R[2] = R[1]; L[1] = i; R[1] = r;
+        } else if (r > R[2]) {
+            L[2] = i; R[2] = r;
+        }
+    }
+    // three_largest_three_first.
+    int first = L[0], second = L[1], third = L[2];
+    auto place = [&](int slot, int from) {
+        if (!ex || ex[slot]) {
+            const DBody tmp = b[slot];
+            b[slot] = b[from];
+            b[from] = tmp;
+        } else {
+            b[slot] = b[from];
+            ex[from] = 0;
+            ex[slot] = 1;
+        }
+    };
+    if (first != 0) {
+        place(0, first);
+        if (second == 0) second = first;
+        else if (third == 0) third = first;
+    }
+    if (second != 1) {
+        place(1, second);
+        if (third == 1) third = second;
+    }
+    if (third != 2) place(2, third);
+}
+
+// Existing bodies (how_many_moonlets), and whether one is within
+// disruption_threshold of the central body or beyond high_dumping_threshold
+// (end_of_timestep would remove it, rk4.c:159-183).
+__global__ __launch_bounds__(LT) void k_lf_census(const DBody *__restrict__ b, int n,
+                                                  const int32_t *__restrict__ ex,
+                                                  const double *__restrict__ cen, double rlo,
+                                                  double rhi, int *__restrict__ count,
+                                                  unsigned *__restrict__ err)
+{
+    const int i = blockIdx.x * LT + threadIdx.x;
+    if (i >= n || (ex && !ex[i])) return;
+    atomicAdd(count, 1);
+    const int cp = cen[7] != 0.0;
+    const double DX = b[i].x - (cp ? cen[0] : 0.0), DY = b[i].y - (cp ? cen[1] : 0.0),
+                 DZ = b[i].z - (cp ? cen[2] : 0.0);
+    const double r2 = DX * DX + DY * DY + DZ * DZ;
+    if ((cp && r2 < rlo * rlo) || r2 > rhi * rhi) atomicOr(err, 2u);
+}
+
+__global__ void k_lf_zero_i32_one(int *__restrict__ p) { *p = 0; }
+
 }  // namespace
+
+extern "C" ncg_status ncg_mesh_leapfrog_dev(ncg_ctx *ctx, ncg_body *d_bodies, int32_t n,
+                                            int32_t *d_exists, int32_t *d_did_collide,
+                                            double *d_central, double grav_const,
+                                            double softening, ncg_precision prec, double dt,
+                                            int32_t nsteps, ncg_mesh_loop_params *mp,
+                                            double collision_f, int32_t one_collision_only,
+                                            int32_t flags, int64_t *n_collisions, void *stream)
+{
+    if (!ctx) return NCG_ERR_INVALID;
+    NCG_REQUIRE(n >= 3 && d_bodies && d_central && mp && nsteps >= 0, "invalid arguments");
+    NCG_REQUIRE(prec == NCG_F64_EXACT || prec == NCG_F64_FAST || prec == NCG_MIXED,
+                "unknown precision %d", (int)prec);
+    NCG_REQUIRE(mp->mesh_size > 0.0 && mp->mesh_size_min > 0.0 && mp->cells >= 3 &&
+                    mp->target_neighbours > 0.0,
+                "invalid mesh parameters");
+    NCG_REQUIRE(!(collision_f > 0.0 && one_collision_only && !d_did_collide),
+                "one_collision_only needs a did_collide array");
+    NCG_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = pick_stream(ctx, stream);
+    if (n_collisions) *n_collisions = 0;
+    DBody *b = reinterpret_cast<DBody *>(d_bodies);
+    const bool coll = collision_f > 0.0;
+    double *acc = dbuf<double>(ctx, "ml_acc", 3 * (size_t)n);
+    double *cm = dbuf<double>(ctx, "ml_cm", 4);
+    unsigned *err = dbuf<unsigned>(ctx, "ml_err", 2);
+    int *count = dbuf<int>(ctx, "ml_count", 1);
+    int *hpin = pbuf<int>(ctx, "ml_hpin", 4);
+    if (!acc || !cm || !err || !count || !hpin) return NCG_ERR_NOMEM;
+    const unsigned g = (unsigned)cdiv(n + 1, LT);
+    const double G = grav_const, soft = softening;
+    int64_t ncoll = 0;
+    // Status of the device-side checks (top-3 and census) read with the
+    // counters of each step.
+    auto census = [&](int &how_many) -> ncg_status {
+        hipLaunchKernelGGL(k_lf_zero_i32_one, dim3(1), dim3(1), 0, st, count);
+        hipLaunchKernelGGL(k_lf_census, dim3(cdiv(n, LT)), dim3(LT), 0, st, b, (int)n, d_exists,
+                           d_central, mp->r_disruption, mp->r_high_dumping, count, err);
+        NCG_CHECK_LAUNCH();
+        NCG_HIP(hipMemcpyAsync(hpin, count, sizeof(int), hipMemcpyDeviceToHost, st));
+        NCG_HIP(hipMemcpyAsync(hpin + 1, err, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+        NCG_HIP(hipStreamSynchronize(st));
+        how_many = hpin[0];
+        if (hpin[1] & 1u)
+            return ncg_fail(ctx, NCG_ERR_INVALID, "fewer than three bodies: three_largest_moonlets "
+                                                  "is undefined");
+        if (hpin[1] & 2u)
+            return ncg_fail(ctx, NCG_ERR_INVALID,
+                            "a body crossed disruption_threshold or high_dumping_threshold: "
+                            "end_of_timestep would remove it, which this loop does not do");
+        return NCG_OK;
+    };
+    auto largest3 = [&]() {
+        hipLaunchKernelGGL(k_top3_first, dim3(1), dim3(T3), 0, st, b, (int)n, d_exists, err);
+    };
+    // Mesh pass at the current gam (rk4.c:697-700, 765-767, 774-777).
+    int64_t tn = 0;
+    auto pass = [&](int mode, double h) -> ncg_status {
+        largest3();
+        NCG_CHECK_LAUNCH();
+        ncg_mesh_stats ms;
+        const double cube = mp->mesh_size * (double)mp->cells;
+        ncg_status r = ncg_mesh_pass_dev(ctx, d_bodies, n, d_exists, d_did_collide, h, mp->mesh_size, cube,
+                                         mp->cells, mode, collision_f, one_collision_only, &ms,
+                                         (void *)st);
+        if (r != NCG_OK) return r;
+        tn = ms.total_neighbours;
+        ncoll += ms.collisions;
+        return NCG_OK;
+    };
+    auto kick = [&](double h) -> ncg_status {
+        ncg_status r = ncg_mesh_gravity_dev(ctx, d_bodies, n, d_exists, d_central, G, soft, prec, acc,
+                                            cm, (void *)st);
+        if (r != NCG_OK) return r;
+        hipLaunchKernelGGL(k_lf_kick, dim3(g), dim3(LT), 0, st, (int)n, d_exists, h, b, acc,
+                           d_central, cm);
+        NCG_CHECK_LAUNCH();
+        return NCG_OK;
+    };
+    // end_of_timestep's mesh-size update (rk4.c:370-377).
+    auto update_gam = [&](int how_many) {
+        const double avg = 2.0 * (double)tn / (double)how_many;
+        double gam = mp->mesh_size * pow(mp->target_neighbours / avg, 1.0 / 3.0);
+        if (gam < mp->mesh_size_min) gam = mp->mesh_size_min;
+        mp->mesh_size = gam;
+    };
+    hipLaunchKernelGGL(k_lf_zero_i32, dim3(1), dim3(64), 0, st, reinterpret_cast<int32_t *>(err), 2);
+    NCG_CHECK_LAUNCH();
+    int how_many = 0;
+    ncg_status r = census(how_many);
+    if (r != NCG_OK) return r;
+    if (flags & 2) {
+        // Half kick before the loop (rk4.c:697-717).
+        if ((r = pass(1, 0.5 * dt)) != NCG_OK) return r;
+        if ((r = kick(0.5 * dt)) != NCG_OK) return r;
+        if ((r = census(how_many)) != NCG_OK) return r;
+        update_gam(how_many);
+    }
+    for (int32_t s = 0; s < nsteps; ++s) {
+        // SBAB1 (rk4.c:761-780): mesh() or get_neighbours_mesh(), drift, kick.
+        if (coll && (r = pass(0, dt)) != NCG_OK) return r;
+        hipLaunchKernelGGL(k_lf_drift, dim3(g), dim3(LT), 0, st, (int)n, d_exists, dt, b, d_central);
+        NCG_CHECK_LAUNCH();
+        if (!coll && (r = pass(1, dt)) != NCG_OK) return r;
+        if ((r = kick(dt)) != NCG_OK) return r;
+        // end_of_timestep (rk4.c:130-393): did_collide, census, mesh size.
+        if (coll && one_collision_only)
+            hipLaunchKernelGGL(k_lf_zero_i32, dim3(cdiv(n, 256)), dim3(256), 0, st, d_did_collide,
+                               (int)n);
+        if ((r = census(how_many)) != NCG_OK) return r;
+        update_gam(how_many);
+    }
+    if (flags & 4) {
+        // Backward half kick after the loop (rk4.c:807-812), the central
+        // record taking CM_buffer's kick.
+        if ((r = pass(1, -0.5 * dt)) != NCG_OK) return r;
+        if ((r = kick(-0.5 * dt)) != NCG_OK) return r;
+    }
+    NCG_HIP(hipStreamSynchronize(st));
+    if (n_collisions) *n_collisions = ncoll;
+    return NCG_OK;
+}
 
 extern "C" ncg_status ncg_leapfrog_dev(ncg_ctx *ctx, ncg_body *d_bodies, int32_t n,
                                        const int32_t *d_exists, int32_t *d_did_collide,

@@ -208,7 +208,7 @@ def build(name):
     main_o = os.path.join(scratch, "main.o")
     run([CC] + CFLAGS + inc + ["-c", os.path.join(REF_SRC, "main.c"), "-o", main_o])
     wraps = "-Wl,--wrap=vector_field,--wrap=brute_force,--wrap=mesh,--wrap=get_neighbours_mesh"
-    trace_wraps = wraps + ",--wrap=collision_treatment"
+    trace_wraps = wraps + ",--wrap=collision_treatment,--wrap=three_largest_moonlets"
     run([CC] + CFLAGS + inc + [os.path.join(HERE, "ref_trace.c"), main_o] + objs +
         [trace_wraps, "-lm", "-o", os.path.join(out, "ncorpion_trace")])
     # The glue also records the slots fragmentation creates (get_free_index).

@@ -13,11 +13,14 @@
  * The trace goes to $NCG_TRACE (default ./trace.txt).  Per-function wall time
  * (CLOCK_MONOTONIC around the target call) is appended at exit.
  *
- * With $NCG_TRACE_DUMP set, the input state of the first brute_force call (the
- * half drift before the integration loop, rk4.c:871-879) is written there:
- * int32 n = largest_id + 1; n 64-byte bodies; int32 exists[n];
- * int32 did_collide[n]; double CM {x, y, z, vx, vy, vz, mass, radius};
- * double timestep.  A device run can start from it.
+ * With $NCG_TRACE_DUMP set, the state at the start of the integration is
+ * written there: for brute force the input of the first brute_force call (the
+ * half drift before the loop, rk4.c:871-879), for the mesh the input of the
+ * first three_largest_moonlets call (the half kick, rk4.c:697-700; linked
+ * with --wrap=three_largest_moonlets): int32 n = largest_id + 1; n 64-byte
+ * bodies; int32 exists[n]; int32 did_collide[n]; double CM {x, y, z, vx, vy,
+ * vz, mass, radius}; double timestep; double gam; double collision_cube;
+ * int32 how_many_moonlets.  A device run can start from it.
  */
 #define _POSIX_C_SOURCE 199309L
 #include <stdint.h>
@@ -150,12 +153,27 @@ static void dump_state(const struct moonlet *X)
     fwrite(cm, sizeof(double), 8, f);
     double ts = timestep;
     fwrite(&ts, sizeof ts, 1, f);
+    double g2[2] = {gam, collision_cube};
+    fwrite(g2, sizeof(double), 2, f);
+    int hm = how_many_moonlets;
+    fwrite(&hm, sizeof hm, 1, f);
     fclose(f);
+}
+
+static int dumped = 0;
+
+void __real_three_largest_moonlets(struct moonlet *moonlets);
+void __wrap_three_largest_moonlets(struct moonlet *X)
+{
+    if (!dumped) {
+        dump_state(X);
+        dumped = 1;
+    }
+    __real_three_largest_moonlets(X);
 }
 
 static void collide_call(int which, void (*fn)(struct moonlet *), const char *tag, struct moonlet *X)
 {
-    static int dumped = 0;
     if (which == 1 && !dumped) {
         dump_state(X);
         dumped = 1;

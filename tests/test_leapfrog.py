@@ -80,6 +80,82 @@ def read_dump(path):
     return bodies, ex, dc, cm, ts
 
 
+def read_dump_mesh(path):
+    """read_dump plus the mesh fields the tracer appends: gam, collision_cube,
+    how_many_moonlets."""
+    bodies, ex, dc, cm, ts = read_dump(path)
+    raw = open(path, "rb").read()
+    o = 4 + bodies.shape[0] * (64 + 8) + 64 + 8
+    gam, cube = np.frombuffer(raw[o:o + 16], np.float64)
+    hm = int(np.frombuffer(raw[o + 16:o + 20], np.int32)[0])
+    return bodies, ex, dc, cm, ts, float(gam), float(cube), hm
+
+
+def three_largest_first(b, ex):
+    """three_largest_moonlets + three_largest_three_first (structure.c:914-1064)
+    restated on a numpy state (moves bodies in place)."""
+    L = [-1, -1, -1]
+    R0 = R1 = R2 = 0.0
+    i = 0
+    for j in range(b.shape[0]):
+        if not ex[j]:
+            continue
+        r = b[j, 7]
+        if i == 0:
+            R0, L[0] = r, j
+        elif i == 1:
+            R1 = r
+            if R0 > R1:
+                L[1] = j
+            else:
+                L[1], L[0] = L[0], j
+                R0, R1 = R1, R0
+        elif i == 2:
+            R2 = r
+            if R1 >= R2:
+                L[2] = j
+            elif R2 > R1 and R2 <= R0:
+                L[2], L[1] = L[1], j
+                R1, R2 = R2, R1
+            else:
+                L[2], L[1], L[0] = L[1], L[0], j
+                R0, R1, R2 = R2, R0, R1
+        else:
+            if r > R0:
+                L[2], L[1], L[0] = L[1], L[0], j
+                R2, R1, R0 = R1, R0, r
+            elif r > R1:
+                L[2], L[1] = L[1], j
+                R2, R1 = R1, r
+            elif r > R2:
+                L[2], R2 = j, r
+        i += 1
+    first, second, third = L
+
+    def place(slot, frm):
+        if ex[slot]:
+            tmp = b[slot].copy()
+            b[slot] = b[frm]
+            b[frm] = tmp
+        else:
+            b[slot] = b[frm]
+            ex[frm] = 0
+            ex[slot] = 1
+
+    if first != 0:
+        place(0, first)
+        if second == 0:
+            second = first
+        elif third == 0:
+            third = first
+    if second != 1:
+        place(1, second)
+        if third == 1:
+            third = second
+    if third != 2:
+        place(2, third)
+
+
 def traced_reference(cfg, bodies, tmp_path):
     """Runs the traced reference; returns (dump, VF input hashes, collisions)."""
     exe = os.path.join(REF, cfg, "ncorpion_trace")
@@ -200,3 +276,52 @@ def test_leapfrog_tiled_forms(ctx, prec, tol):
         out[p] = bt.cpu().numpy()
     d = np.abs(out[prec][:, 0:6] - out["exact"][:, 0:6]).max()
     assert d < tol, d
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["exact"])
+def test_mesh_loop_reproduces_reference_run(ctx, tmp_path, prec):
+    """NcorpiON's mesh integrator (16,384 bodies, radius 1e-3, inelastic
+    bounces, 10 steps) from the reference's own starting state: after every
+    device step, the state the reference's next mesh() sees (after its
+    three_largest swap) has the same hash, and the collision counts agree."""
+    torch = tt()
+    b0 = ic.disk(16384, seed=2, mass=1e-10, radius=1e-3)
+    exe = os.path.join(REF, "run_mesh", "ncorpion_trace")
+    if not os.path.exists(exe):
+        pytest.skip("reference builds missing (oracle/build_ref.py)")
+    ic.write_init_txt(str(tmp_path / "init.txt"), b0)
+    env = dict(os.environ, NCG_TRACE=str(tmp_path / "trace.txt"),
+               NCG_TRACE_DUMP=str(tmp_path / "dump.bin"))
+    r = subprocess.run([exe], cwd=str(tmp_path), env=env, stdout=subprocess.PIPE,
+                       stderr=subprocess.STDOUT, text=True, timeout=600)
+    assert r.returncode == 0, r.stdout[-2000:]
+    lines = open(tmp_path / "trace.txt").read().splitlines()
+    mesh_in, coll = [], []
+    for l in lines:
+        f = l.split()
+        if f[0] == "MESH" and f[2] == "in":
+            mesh_in.append(int(f[3], 16))
+            coll.append(0)
+        elif f[0] == "C" and coll:
+            coll[-1] += 1
+    bodies, ex, dc, cm, ts, gam, cube, hm = read_dump_mesh(str(tmp_path / "dump.bin"))
+    assert len(mesh_in) == 10 and sum(coll) > 10
+    dt = 2.0 * ts  # dumped inside the half kick (timestep halved)
+    params = dict(gam=gam, gam_min=2.56 / 256, how_many_neighbours=100.0,
+                  disruption_threshold=0.05, high_dumping_threshold=100.0, cells=256)
+    bt = torch.from_numpy(bodies).cuda()
+    ct = torch.from_numpy(np.concatenate([cm[0:7], [1.0]])).cuda()
+    et = torch.from_numpy(ex).cuda()
+    dct = torch.from_numpy(dc).cuda()
+    ctx.mesh_leapfrog_dev(bt, ct, 0, dt, params, 1.0, 0.0, prec, 1.5, 0, exists_t=et,
+                          did_collide_t=dct, half_kick_start=True)
+    for s, h in enumerate(mesh_in):
+        ctx.synchronize()
+        st = bt.cpu().numpy().copy()
+        e = et.cpu().numpy().copy()
+        three_largest_first(st, e)
+        assert orc.state_hash(st, e) == h, "step %d" % s
+        nc = ctx.mesh_leapfrog_dev(bt, ct, 1, dt, params, 1.0, 0.0, prec, 1.5, 0, exists_t=et,
+                                   did_collide_t=dct)
+        assert nc == coll[s], "step %d" % s
