@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define NCG_ABI_VERSION 1
+#define NCG_ABI_VERSION 2
 
 typedef struct ncg_ctx ncg_ctx;
 

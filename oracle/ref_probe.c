@@ -21,9 +21,9 @@
  * ops: 0 vector_field (physics.c:63), 1 brute_force (collision.c:425),
  *      2 mesh (collision.c:324), 3 get_neighbours_mesh (physics.c:1094),
  *      4 closest_approach on every i<j (collision.c:63), 5 get_neighbours_mesh
- *      then vector_field with the pairs it registered (only vector_field is
- *      timed; no pairs written); op | 0x100 writes only the header and CM_acc
- *      (timing runs).
+ *      then vector_field with the pairs it registered (each timed: gnm_seconds
+ *      and op_seconds on stderr; no pairs written); op | 0x100 writes only the
+ *      header and CM_acc (timing runs).
  */
 #define _POSIX_C_SOURCE 199309L
 #include <stdio.h>
@@ -129,11 +129,16 @@ int main(int argc, char **argv)
     case 1: brute_force(bodies); break;
     case 2: mesh(bodies); break;
     case 3: get_neighbours_mesh(bodies); break;
-    case 5:
+    case 5: {
         get_neighbours_mesh(bodies);
-        clock_gettime(CLOCK_MONOTONIC, &t0);
+        struct timespec tg;
+        clock_gettime(CLOCK_MONOTONIC, &tg);
+        fprintf(stderr, "gnm_seconds=%.6f\n",
+                (double)(tg.tv_sec - t0.tv_sec) + 1e-9 * (double)(tg.tv_nsec - t0.tv_nsec));
+        t0 = tg;
         vector_field(bodies);
         break;
+    }
     case 4: {
         long np = (long)n * (n - 1) / 2;
         ca_ab = (int *)malloc(sizeof(int) * 2 * (size_t)(np > 0 ? np : 1));
