@@ -293,7 +293,13 @@ __global__ __launch_bounds__(64) void k_mesh_replay(
 {
     __shared__ int4 P[MR_PCAP];
     __shared__ int s_np, s_pi, s_ci, s_k, s_hl, s_hp, s_hit;
-    __shared__ int s_q[64];
+    __shared__ int s_q[64], s_l[64];
+    // 27 position ranges [ra[i], ra[i] + rn[i]) walked as one index space,
+    // pre[i] = positions before range i, rl[i] = its stencil cell: set 0 for
+    // the later-visitor scans, set 1 for the rest of a visit (which calls
+    // those scans).
+    __shared__ unsigned s_ra[2][27], s_rn[2][27], s_pre[2][28];
+    __shared__ int s_rl[2][27];
     __shared__ double s_ap[6], s_tc;
     const int lane = threadIdx.x;
     if (cnt[3] > 0) {
@@ -314,46 +320,84 @@ __global__ __launch_bounds__(64) void k_mesh_replay(
     __syncthreads();
     unsigned hits = 0;
     auto cand_at = [&](int i) { return cand[order[i]]; };
-    // Later visitors of body dd (just changed during visit kc) that may hit it.
+    // Prefix of the 27 ranges of set r written by lanes 0..26 (every lane calls it).
+    auto flatten = [&](int r) {
+        __syncthreads();
+        if (lane == 0) {
+            unsigned t = 0;
+            for (int i = 0; i < 27; ++i) {
+                s_pre[r][i] = t;
+                t += s_rn[r][i];
+            }
+            s_pre[r][27] = t;
+        }
+        __syncthreads();
+    };
+    // Range of flat index x in set r.
+    auto range_of = [&](int r, unsigned x) {
+        int lo = 0, hi = 27;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (s_pre[r][mid] <= x) lo = mid;
+            else hi = mid;
+        }
+        return lo;
+    };
+    // Later visitors of body dd (just changed during visit kc) that may hit
+    // it: the members above kc of its 27 cells, 64 at a time across cells.
     auto eager = [&](int dd, int kc) {
         const unsigned kd = key[dd];
         if (kd == NOKEY) return;
         const DBody D = b[dd];
         const double4 pd = make_double4(D.x, D.y, D.z, D.radius);
         const double4 vd = make_double4(D.vx, D.vy, D.vz, D.mass);
-        for (int l = 0; l < 27; ++l) {
-            const unsigned c = (unsigned)((long)kd + stencil_offset(l, d));
-            const unsigned a = cs[c], e = ce[c];
-            for (unsigned base = a; base < e; base += 64) {
-                const unsigned u = base + lane;
-                bool h = false;
-                if (u < e) {
-                    const int q = sid[u];
-                    s_q[lane] = q;
-                    if (q > kc && (!ex || ex[q])) h = approach_hit(pd, vd, pr[u], vm[u], dt);
-                }
-                unsigned long long mask = __ballot(h);
-                __syncthreads();
-                if (lane == 0) {
-                    while (mask) {
-                        const int bit = __ffsll((long long)mask) - 1;
-                        mask &= mask - 1;
-                        // In visit q, dd sits in cell 26 - l (the stencil is symmetric).
-                        const int4 ev = make_int4(s_q[bit], 26 - l, dd, 0);
-                        if (s_np >= MR_PCAP) {
-                            atomicOr(stat + 1, 2u);
-                            continue;
-                        }
-                        int w = s_np++;
-                        while (w > s_pi && mr_before(ev, P[w - 1])) {
-                            P[w] = P[w - 1];
-                            --w;
-                        }
-                        P[w] = ev;
-                    }
-                }
-                __syncthreads();
+        if (lane < 27) {
+            const unsigned c = (unsigned)((long)kd + stencil_offset(lane, d));
+            unsigned lo = cs[c], hi = ce[c];
+            const unsigned e = hi;
+            while (lo < hi) {  // first id above kc
+                const unsigned mid = (lo + hi) >> 1;
+                if (sid[mid] <= kc) lo = mid + 1;
+                else hi = mid;
             }
+            s_ra[0][lane] = lo;
+            s_rn[0][lane] = e - lo;
+            s_rl[0][lane] = lane;
+        }
+        flatten(0);
+        const unsigned total = s_pre[0][27];
+        for (unsigned base = 0; base < total; base += 64) {
+            const unsigned x = base + lane;
+            bool h = false;
+            if (x < total) {
+                const int i = range_of(0, x);
+                const unsigned u = s_ra[0][i] + (x - s_pre[0][i]);
+                const int q = sid[u];
+                s_q[lane] = q;
+                s_l[lane] = s_rl[0][i];
+                if (!ex || ex[q]) h = approach_hit(pd, vd, pr[u], vm[u], dt);
+            }
+            unsigned long long mask = __ballot(h);
+            __syncthreads();
+            if (lane == 0) {
+                while (mask) {
+                    const int bit = __ffsll((long long)mask) - 1;
+                    mask &= mask - 1;
+                    // In visit q, dd sits in cell 26 - l (the stencil is symmetric).
+                    const int4 ev = make_int4(s_q[bit], 26 - s_l[bit], dd, 0);
+                    if (s_np >= MR_PCAP) {
+                        atomicOr(stat + 1, 2u);
+                        continue;
+                    }
+                    int w = s_np++;
+                    while (w > s_pi && mr_before(ev, P[w - 1])) {
+                        P[w] = P[w - 1];
+                        --w;
+                    }
+                    P[w] = ev;
+                }
+            }
+            __syncthreads();
         }
     };
     auto resolve = [&](int p, int k) {  // lane 0
@@ -416,50 +460,69 @@ __global__ __launch_bounds__(64) void k_mesh_replay(
         if (k == INT_MAX) break;
         if (!s_hit) continue;
         eager(s_hp, k);
-        // The rest of visit k, in traversal order, from after (s_hl, s_hp).
+        // The rest of visit k in traversal order -- cells s_hl down to 0,
+        // ids below k, after s_hp in cell s_hl -- 64 pairs at a time across
+        // cells; the first hit wins and the walk resumes after it.
         const unsigned kk = key[k];
-        int l = s_hl;
-        unsigned from;
-        {
-            const unsigned c = (unsigned)((long)kk + stencil_offset(l, d));
-            unsigned lo = cs[c], hi = ce[c];
-            const int p0 = s_hp;
-            while (lo < hi) {
-                const unsigned mid = (lo + hi) >> 1;
-                if (sid[mid] <= p0) lo = mid + 1;
-                else hi = mid;
+        const int l0 = s_hl, p0 = s_hp;
+        if (lane < 27) {
+            const int l = l0 - lane;  // range `lane` is cell l0 - lane
+            unsigned lo = 0, e = 0;
+            if (l >= 0) {
+                const unsigned c = (unsigned)((long)kk + stencil_offset(l, d));
+                lo = cs[c];
+                unsigned hi = ce[c];
+                if (l == l0) {  // after p0
+                    unsigned a2 = lo, b2 = hi;
+                    while (a2 < b2) {
+                        const unsigned mid = (a2 + b2) >> 1;
+                        if (sid[mid] <= p0) a2 = mid + 1;
+                        else b2 = mid;
+                    }
+                    lo = a2;
+                }
+                unsigned a3 = lo, b3 = hi;  // ids below k
+                while (a3 < b3) {
+                    const unsigned mid = (a3 + b3) >> 1;
+                    if (sid[mid] < k) a3 = mid + 1;
+                    else b3 = mid;
+                }
+                e = a3;
             }
-            from = lo;
+            s_ra[1][lane] = lo;
+            s_rn[1][lane] = e - lo;
+            s_rl[1][lane] = l;
         }
-        for (; l >= 0; --l) {
-            const unsigned c = (unsigned)((long)kk + stencil_offset(l, d));
-            const unsigned e = ce[c];
-            unsigned base = (l == s_hl) ? from : cs[c];
-            while (base < e) {
-                const unsigned u = base + lane;
-                const int p = u < e ? sid[u] : INT_MAX;
-                double ap[6], tc;
-                const bool h = p < k && bf_hit(b, ex, dcf, p, k, dt, ap, &tc);
-                const unsigned long long hm = __ballot(h);
-                const unsigned long long past = __ballot(u < e && p >= k);
-                if (hm == 0ull) {
-                    if (past) break;  // ids ascend: the rest of the cell is above k
-                    base += 64;
-                    continue;
-                }
-                const int first = __ffsll((long long)hm) - 1;
-                if (lane == first) {
-                    for (int q = 0; q < 6; ++q) s_ap[q] = ap[q];
-                    s_tc = tc;
-                    s_hp = p;
-                }
-                __syncthreads();
-                if (lane == 0) resolve(s_hp, k);
-                __threadfence_block();
-                __syncthreads();
-                eager(s_hp, k);
-                base = base + first + 1;
+        flatten(1);
+        const unsigned total = s_pre[1][27];
+        unsigned base = 0;
+        while (base < total) {
+            const unsigned x = base + lane;
+            int p = -1;
+            double ap[6], tc;
+            bool h = false;
+            if (x < total) {
+                const int i = range_of(1, x);
+                p = sid[s_ra[1][i] + (x - s_pre[1][i])];
+                h = bf_hit(b, ex, dcf, p, k, dt, ap, &tc);
             }
+            const unsigned long long hm = __ballot(h);
+            if (hm == 0ull) {
+                base += 64;
+                continue;
+            }
+            const int first = __ffsll((long long)hm) - 1;
+            if (lane == first) {
+                for (int q = 0; q < 6; ++q) s_ap[q] = ap[q];
+                s_tc = tc;
+                s_hp = p;
+            }
+            __syncthreads();
+            if (lane == 0) resolve(s_hp, k);
+            __threadfence_block();
+            __syncthreads();
+            eager(s_hp, k);
+            base = base + first + 1;
         }
         eager(k, k);
     }
