@@ -206,6 +206,14 @@ ncg_status ncg_collide_candidates_allpairs(ncg_ctx *ctx, const ncg_body *bodies,
  * in collision.c:381-414.  The binning is kept in the context for the next
  * ncg_mesh_gravity call, which evaluates the pairs registered by the pass at
  * the positions it is given (physics.c:296-378).
+ *   Outcomes may remove bodies (mergers; seen through the live `exists`, a
+ * removed body leaves the later visits' neighbours, collision.c:296) and
+ * create bodies, which mesh mode appends at slots n, n + 1, ... (fragmentation,
+ * physics.c:994-996 and structure.c:857-861; reported through ops->created,
+ * the arrays must cover them) and whose pairs it registers in the pair list
+ * (physics.c:1002-1010).  A pass with bodies larger than gam accepts only
+ * outcomes that keep both bodies.  The next ncg_mesh_gravity then takes the
+ * n slots of the pass plus the appended ones.
  *   ops may be NULL in mode 1.
  */
 ncg_status ncg_mesh_pass(ncg_ctx *ctx, const ncg_body *bodies, int32_t n,
@@ -271,7 +279,8 @@ ncg_status ncg_mesh_leapfrog_dev(ncg_ctx *ctx, ncg_body *d_bodies, int32_t n, in
  * in the pass's pair-list order and bodies 0..2 and CM_acc in body order (the
  * first exact call after a pass builds id-ordered neighbourhood lists, 108
  * bytes per binned body, reused by later calls of the pass).  NCG_F64_FAST
- * and NCG_MIXED sum per neighbour cell. */
+ * and NCG_MIXED sum per neighbour cell.  n: the slots of the pass plus the
+ * bodies it appended; pairs of absent bodies are skipped (physics.c:351). */
 ncg_status ncg_mesh_gravity(ncg_ctx *ctx, const ncg_body *bodies, int32_t n,
                             const int32_t *exists, const ncg_central *central, double grav_const,
                             double softening, ncg_precision prec, double *acc,
@@ -289,7 +298,9 @@ ncg_status ncg_mesh_gravity_dev(ncg_ctx *ctx, const ncg_body *d_bodies, int32_t 
 
 /* Neighbour pairs of the last ncg_mesh_pass in the reference's registration
  * order, (k, p) with k the visited body (what mesh()/get_neighbours_mesh()
- * append to `pairs`, collision.c:353-358, physics.c:1108-1111).  Returns
+ * append to `pairs`, collision.c:353-358, physics.c:1108-1111), each
+ * collision's entry replaced by what its outcome registered
+ * (physics.c:1043-1091).  Returns
  * NCG_ERR_CAPACITY with *count when the buffer is too small. */
 ncg_status ncg_mesh_pairs(ncg_ctx *ctx, ncg_pair *out, int64_t capacity, int64_t *count);
 

@@ -499,14 +499,30 @@ class Context:
 
     # -- mesh module -------------------------------------------------------------
     def mesh_pass(self, bodies, exists, dt, gam, cube, cells, mode, test=None, resolve=None,
-                  ops=None):
+                  ops=None, created=None, n=None):
+        """mesh() (mode 0) or get_neighbours_mesh() (mode 1).  Mode 0 takes
+        Python callbacks test(a, b) -> bool, resolve(a, b) and, when outcomes
+        append bodies, created() -> new slots; or built-in ops.  n (default:
+        all rows) is the number of slots of the pass; the arrays may be longer
+        to hold the bodies appended during it, and `exists` must then be the
+        live int32 array the callbacks mutate."""
         b = bodies
         if not (isinstance(b, np.ndarray) and b.dtype == np.float64 and b.flags.c_contiguous):
             raise ValueError("bodies must be a C-contiguous float64 array (mutated in place)")
-        n = b.shape[0]
-        e = as_exists(exists, n)
+        if n is None:
+            n = b.shape[0]
+            e = as_exists(exists, n)
+        else:
+            n = int(n)
+            if n > b.shape[0]:
+                raise ValueError("n exceeds the body array")
+            e = exists
+            if e is not None and not (isinstance(e, np.ndarray) and e.dtype == np.int32
+                                      and e.flags.c_contiguous and e.shape == (b.shape[0],)):
+                raise ValueError("exists must be a C-contiguous int32 array of one entry per "
+                                 "body slot (the live array the callbacks mutate)")
         stats = MeshStats()
-        keep, ops_p = self._ops(test, resolve, ops)
+        keep, ops_p = self._ops(test, resolve, ops, created)
         self._check(self.lib.ncg_mesh_pass(self.h, _ptr(b), n, _ptr(e), float(dt), float(gam),
                                            float(cube), int(cells), int(mode), ops_p,
                                            ctypes.byref(stats)))

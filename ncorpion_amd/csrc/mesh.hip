@@ -25,6 +25,7 @@
 #include <map>
 #include <queue>
 #include <set>
+#include <tuple>
 
 #include "ncg_internal.hpp"
 #include "collide_dev.hpp"
@@ -592,11 +593,12 @@ __device__ inline void central_term(double x, double y, double z, double m, doub
 }
 
 // Bodies the sorted kernel does not cover: k <= 2 (central term only; their
-// mutual part is added by k_big_final) and k > 2 not binned (central term and
-// bodies 0..2).
+// mutual part is added by k_big_final) and k > 2 not binned, appended ones
+// (k >= nkey) included (central term and bodies 0..2).
 __global__ __launch_bounds__(MT) void k_mgrav_rest(const DBody *__restrict__ b, int n,
                                                    const int32_t *__restrict__ ex,
-                                                   const unsigned *__restrict__ key, const double *__restrict__ cen,
+                                                   const unsigned *__restrict__ key, int nkey,
+                                                   const double *__restrict__ cen,
                                                    double G, double s, double *__restrict__ acc,
                                                    double *__restrict__ cterm)
 {
@@ -605,7 +607,7 @@ __global__ __launch_bounds__(MT) void k_mgrav_rest(const DBody *__restrict__ b, 
     const double mu = cp ? G * cen[6] : 0.0;
     const int k = blockIdx.x * MT + threadIdx.x;
     if (k >= n || (ex && !ex[k])) return;
-    if (k > 2 && key[k] != NOKEY) return;
+    if (k > 2 && k < nkey && key[k] != NOKEY) return;
     const DBody q = b[k];
     double ax = 0., ay = 0., az = 0.;
     if (cp) central_term(q.x, q.y, q.z, q.mass, cx, cy, cz, mu, G, ax, ay, az, cterm + 3 * (size_t)k);
@@ -1037,12 +1039,14 @@ __global__ __launch_bounds__(MT) void k_mord(const DBody *__restrict__ b, int n,
     acc[3 * (size_t)k + 2] = az;
 }
 
-// Existing bodies k > 2 that are not binned (outside the collision cube, or
-// larger than the mesh size in mesh()): central term, bodies 0..2, then the
-// recorded big visits in visiting order (their own one included).
+// Existing bodies k > 2 that are not binned (outside the collision cube,
+// larger than the mesh size in mesh(), or appended during the pass: slots
+// from nkey on): central term, bodies 0..2, then the recorded big visits in
+// visiting order (their own one included).
 __global__ __launch_bounds__(MT) void k_mrest_ord(const DBody *__restrict__ b, int n,
                                                   const int32_t *__restrict__ ex,
-                                                  const unsigned *__restrict__ key, const double *__restrict__ cen,
+                                                  const unsigned *__restrict__ key, int nkey,
+                                                  const double *__restrict__ cen,
                                                   double G, double s, const int *__restrict__ bvk,
                                                   const int *__restrict__ bvoff,
                                                   const int *__restrict__ bvo, int nbv,
@@ -1052,7 +1056,7 @@ __global__ __launch_bounds__(MT) void k_mrest_ord(const DBody *__restrict__ b, i
     const double cx = cp ? cen[0] : 0.0, cy = cp ? cen[1] : 0.0, cz = cp ? cen[2] : 0.0;
     const double mu = cp ? G * cen[6] : 0.0;
     const int k = 3 + blockIdx.x * MT + threadIdx.x;
-    if (k >= n || (ex && !ex[k]) || key[k] != NOKEY) return;
+    if (k >= n || (ex && !ex[k]) || (k < nkey && key[k] != NOKEY)) return;
     const DBody q = b[k];
     double ax = 0., ay = 0., az = 0., ct[3];
     if (cp) central_term(q.x, q.y, q.z, q.mass, cx, cy, cz, mu, G, ax, ay, az, ct);
@@ -1071,6 +1075,39 @@ __global__ __launch_bounds__(MT) void k_mrest_ord(const DBody *__restrict__ b, i
             term_id(b, ex, v, q.x, q.y, q.z, q.radius, G, s, ax, ay, az);
         }
     }
+    acc[3 * (size_t)k + 0] = ax;
+    acc[3 * (size_t)k + 1] = ay;
+    acc[3 * (size_t)k + 2] = az;
+}
+
+// Bodies whose term order the host lists (tg: id, first, count triples into
+// pl): central term, bodies 0..2, then the listed partners in order, each
+// term as in k_mord.  Runs after k_mord / k_mrest_ord and replaces theirs.
+__global__ __launch_bounds__(MT) void k_mexplicit(const DBody *__restrict__ b,
+                                                  const int32_t *__restrict__ ex,
+                                                  const double *__restrict__ cen, double G, double s,
+                                                  const int *__restrict__ tg, int ntg,
+                                                  const int *__restrict__ pl,
+                                                  double *__restrict__ acc)
+{
+    const int cp = cen && cen[7] != 0.0;
+    const double cx = cp ? cen[0] : 0.0, cy = cp ? cen[1] : 0.0, cz = cp ? cen[2] : 0.0;
+    const double mu = cp ? G * cen[6] : 0.0;
+    const int i = blockIdx.x * MT + threadIdx.x;
+    if (i >= ntg) return;
+    const int k = tg[3 * i], first = tg[3 * i + 1], cnt = tg[3 * i + 2];
+    if (k <= 2 || (ex && !ex[k])) return;
+    const DBody q = b[k];
+    double ax = 0., ay = 0., az = 0., ct[3];
+    if (cp) central_term(q.x, q.y, q.z, q.mass, cx, cy, cz, mu, G, ax, ay, az, ct);
+    for (int big = 0; big < 3; ++big) {
+        if (ex && !ex[big]) continue;
+        const DBody g = b[big];
+        term_exact(q.x, q.y, q.z, q.radius, make_double4(g.x, g.y, g.z, g.mass), g.radius, G, s, ax,
+                   ay, az);
+    }
+    for (int u = first; u < first + cnt; ++u)
+        term_id(b, ex, pl[u], q.x, q.y, q.z, q.radius, G, s, ax, ay, az);
     acc[3 * (size_t)k + 0] = ax;
     acc[3 * (size_t)k + 1] = ay;
     acc[3 * (size_t)k + 2] = az;
@@ -1444,6 +1481,33 @@ ncg_status mesh_bin(ncg_ctx *ctx, const DBody *db, int32_t n, const int32_t *dex
     return NCG_OK;
 }
 
+// Forget the previous pass (its binning stays until the next mesh_bin).
+void reset_pass(MeshState &M, int32_t n)
+{
+    M.valid = false;
+    M.pairs_known = false;
+    M.ord_lists = false;
+    M.extra_pairs.clear();
+    M.big_visits.clear();
+    M.big_pairs_all.clear();
+    M.events.clear();
+    M.frag_ids.clear();
+    M.removed.clear();
+    M.x_targets.clear();
+    M.x_partners.clear();
+    M.n_end = n;
+    M.rm_drop = 0;
+}
+
+// Number of stencil indexes l' in (l, 26] naming the same cell as l (only
+// with 2 cells per axis): a body listed twice in a visit is entry `occ` of it.
+int stencil_occ(int l, long d)
+{
+    int c = 0;
+    for (int q = l + 1; q < 27; ++q) c += stencil_offset(q, d) == stencil_offset(l, d);
+    return c;
+}
+
 ncg_status load_cells(ncg_ctx *ctx, HostCells &hc, int32_t n, int ins, const unsigned *dkey,
                       const unsigned *dskey, const int *dsid)
 {
@@ -1462,6 +1526,84 @@ ncg_status load_cells(ncg_ctx *ctx, HostCells &hc, int32_t n, int ins, const uns
     hc.skey.assign(pk + n, pk + n + ins);
     hc.sid.assign(reinterpret_cast<const int *>(pk + n + ins), reinterpret_cast<const int *>(pk + n + 2 * (size_t)ins));
     hc.loaded = true;
+    return NCG_OK;
+}
+
+// Term lists of the bodies that full fragmentations touched, for the exact
+// field (physics.c:347-378 sums each body's pairs in list order): for each
+// survivor (id > 2), its own visit (cells 26..0, ascending id below it) and
+// the later visits that gather it (ascending visitor id; within a visit,
+// stencil index 26..0), with each fragmentation's entry replaced by the
+// block's appended bodies (physics.c:1002-1010: pairs (id[p], id[q]), q < p);
+// for each appended body, the other bodies of its block in block order.
+// Absent partners stay listed: the field skips them as physics.c:351 does.
+// FP64-fast and mixed forms add the block pairs after the rest.
+ncg_status explicit_lists(ncg_ctx *ctx, MeshState &M, HostCells &hc, int32_t n, int ins,
+                          const unsigned *key, const unsigned *skey, const int *sid)
+{
+    std::map<std::tuple<int, int, int>, const MeshEvent *> blk;  // (visit, l, partner)
+    std::set<int> surv;
+    for (const MeshEvent &e : M.events)
+        if (e.kind == MeshEvent::BLOCK) {
+            blk[std::make_tuple(e.k, e.l, e.p)] = &e;
+            if (M.frag_ids[e.first] > 2) surv.insert(M.frag_ids[e.first]);
+        }
+    if (blk.empty()) return NCG_OK;
+    ncg_status r = load_cells(ctx, hc, n, ins, key, skey, sid);
+    if (r != NCG_OK) return r;
+    const long d = hc.d;
+    std::vector<int> &P = M.x_partners;
+    auto target = [&](int id, int first) {
+        M.x_targets.push_back(id);
+        M.x_targets.push_back(first);
+        M.x_targets.push_back((int)P.size() - first);
+    };
+    auto emit = [&](int v, int l, int p, int other) {
+        auto it = blk.find(std::make_tuple(v, l, p));
+        if (it == blk.end()) {
+            P.push_back(other);
+            return;
+        }
+        const MeshEvent &e = *it->second;
+        for (int q = 1; q < e.count; ++q) P.push_back(M.frag_ids[e.first + q]);
+    };
+    std::vector<int> mem;
+    std::vector<std::pair<int, int>> later;
+    for (int A : surv) {
+        const int first = (int)P.size();
+        const unsigned ka = hc.eff(A);
+        if (ka != NOKEY) {
+            for (int l = 26; l >= 0; --l) {
+                hc.members((unsigned)((long)ka + stencil_offset(l, d)), mem);
+                for (int p : mem) {
+                    if (p >= A) break;
+                    emit(A, l, p, p);
+                }
+            }
+            later.clear();
+            for (int l = 0; l < 27; ++l) {
+                hc.members((unsigned)((long)ka + stencil_offset(l, d)), mem);
+                for (int q : mem)
+                    if (q > A) later.push_back({q, -(26 - l)});  // A at index 26 - l of q's visit
+            }
+            std::sort(later.begin(), later.end());
+            for (const auto &ql : later) emit(ql.first, -ql.second, A, ql.first);
+        }
+        target(A, first);
+    }
+    for (const auto &kv : blk) {
+        const MeshEvent &e = *kv.second;
+        for (int a = 1; a < e.count; ++a) {
+            const int first = (int)P.size();
+            for (int b = 0; b < e.count; ++b)
+                if (b != a) P.push_back(M.frag_ids[e.first + b]);
+            target(M.frag_ids[e.first + a], first);
+            for (int b = 0; b < a; ++b) {
+                const int u = M.frag_ids[e.first + a], w = M.frag_ids[e.first + b];
+                if (u > 2 && w > 2) M.extra_pairs.push_back({u, w});
+            }
+        }
+    }
     return NCG_OK;
 }
 
@@ -1485,12 +1627,7 @@ extern "C" ncg_status ncg_mesh_pass(ncg_ctx *ctx, const ncg_body *bodies, int32_
     NCG_HIP(hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;
     MeshState &M = ctx->mesh;
-    M.valid = false;
-    M.pairs_known = false;
-    M.ord_lists = false;
-    M.extra_pairs.clear();
-    M.big_visits.clear();
-    M.collided.clear();
+    reset_pass(M, n);
     if (n == 0) {
         M.valid = true;
         M.pairs_known = true;
@@ -1596,6 +1733,15 @@ extern "C" ncg_status ncg_mesh_pass(ncg_ctx *ctx, const ncg_body *bodies, int32_
     ncg_status err = NCG_OK;
     std::vector<int> mem;
     auto live = [&](int32_t k) { return !exists || exists[k] != 0; };
+    // Visit during which each body was removed (sized at the first removal).
+    // A visit gathers its neighbours when it starts (collision.c:283-300): a
+    // body removed during an earlier visit is not among them, one removed
+    // during this visit still is.
+    std::vector<int> rm_at;
+    auto gathered = [&](int p, int kv) { return rm_at.empty() || rm_at[p] >= kv; };
+    // Bodies created by an outcome are appended in mesh mode (get_free_index(1),
+    // physics.c:994-996, structure.c:857-861): slots n, n + 1, ...
+    int32_t ncol = n;
     // Later visits q > kv that have b as a neighbour re-test it on the host.
     auto schedule = [&](int b, int kv) -> ncg_status {
         ncg_status r = load_cells(ctx, hc, n, ins, key, skey, sid);
@@ -1617,26 +1763,93 @@ extern "C" ncg_status ncg_mesh_pass(ncg_ctx *ctx, const ncg_body *bodies, int32_
     int32_t born[64];
     // Small (radius <= gam) at the start of the pass: not in the big list.
     auto was_small = [&](int b) { return !std::binary_search(big.begin(), big.end(), b); };
-    auto collide = [&](int p, int k, int kv) -> ncg_status {
+    // Resolves entry (k, p) of visit kv, found at stencil index l (-1: a big
+    // visit), and records what the outcome leaves in the pair list.
+    auto collide = [&](int p, int k, int kv, int l) -> ncg_status {
+        MeshEvent e{kv, l, p, l >= 0 ? stencil_occ(l, d) : 0, MeshEvent::PAIR, {p, k}, 0, 0};
+        if (!live(p) || !live(k)) {
+            // collision_treatment returns at once (physics.c:1053-1055): the
+            // visitor, or a partner listed twice, went earlier in this visit.
+            e.kind = MeshEvent::NONE;
+            M.events.push_back(e);
+            return NCG_OK;
+        }
+        const double mp = bodies[p].mass, mk = bodies[k].mass;
         ops->resolve(ops->user, p, k);
         ++ncoll;
-        // Outcomes that remove, create or reclassify bodies (mergers, full
-        // fragmentation: bodies lost mid-pass drop out of later gathers and
-        // new ones join later big visits, collision.c:296, 386) are outside
-        // what this replay reproduces.
         const int32_t nborn = ops->created ? ops->created(ops->user, born, 64) : 0;
-        if (nborn != 0 || !live(p) || !live(k) ||
-            (p > kv && (bodies[p].radius <= gam) != was_small(p)) ||
-            (k > kv && (bodies[k].radius <= gam) != was_small(k)))
-            return ncg_fail(ctx, NCG_ERR_INVALID,
-                            "mesh(): the outcome of collision (%d, %d) removed, created or "
-                            "resized bodies across the mesh size; only outcomes that keep both "
-                            "bodies (bounces, partial fragmentation) are supported on the mesh path",
+        if (nborn < 0)
+            return ncg_fail(ctx, NCG_ERR_CAPACITY,
+                            "mesh(): more than 64 bodies created by the outcome of collision (%d, %d)",
                             p, k);
-        M.collided.push_back({p, k});
-        ncg_status r = mark_dirty(p, kv);
-        if (r != NCG_OK) return r;
-        return mark_dirty(k, kv);
+        for (int q = 0; q < nborn; ++q)
+            if (born[q] != ncol + q)
+                return ncg_fail(ctx, NCG_ERR_INVALID,
+                                "mesh(): body %d created by collision (%d, %d) is not appended at "
+                                "slot %d (mesh() appends, structure.c:857-861)",
+                                born[q], p, k, ncol + q);
+        const bool lp = live(p), lk = live(k);
+        // Bodies larger than the mesh size scan every body (collision.c:386):
+        // there, removed and created bodies would change later big visits.
+        if (!big.empty() && (nborn != 0 || !lp || !lk ||
+                             (p > kv && (bodies[p].radius <= gam) != was_small(p)) ||
+                             (k > kv && (bodies[k].radius <= gam) != was_small(k))))
+            return ncg_fail(ctx, NCG_ERR_INVALID,
+                            "mesh(): the outcome of collision (%d, %d) removed, created or resized "
+                            "bodies across the mesh size in a pass with bodies larger than the mesh "
+                            "size; only outcomes that keep both bodies are supported there",
+                            p, k);
+        if (lp && lk) {
+            if (nborn != 0)
+                return ncg_fail(ctx, NCG_ERR_INVALID,
+                                "mesh(): the outcome of collision (%d, %d) kept both bodies and "
+                                "created %d (no outcome of the reference does)",
+                                p, k, nborn);
+            // A bounce registers (p, k) (physics.c:1063-1080); a partial
+            // fragmentation, which changes the masses, (heavier, lighter)
+            // (physics.c:1049-1051, 886-890).
+            if (bodies[p].mass != mp || bodies[k].mass != mk)
+                e.reg = mp > mk ? ncg_pair{p, k} : ncg_pair{k, p};
+            M.events.push_back(e);
+            ncg_status r = mark_dirty(p, kv);
+            if (r != NCG_OK) return r;
+            return mark_dirty(k, kv);
+        }
+        if (rm_at.empty()) rm_at.assign((size_t)n, INT32_MAX);
+        if (!lp && !lk) {
+            // A super-catastrophic disruption that leaves no body
+            // (physics.c:819-823) registers nothing.
+            if (nborn != 0)
+                return ncg_fail(ctx, NCG_ERR_INVALID,
+                                "mesh(): the outcome of collision (%d, %d) removed both bodies and "
+                                "created %d (no outcome of the reference does)",
+                                p, k, nborn);
+            rm_at[p] = rm_at[k] = kv;
+            M.removed.push_back({p, kv});
+            M.removed.push_back({k, kv});
+            e.kind = MeshEvent::NONE;
+            M.events.push_back(e);
+            return NCG_OK;
+        }
+        // A merger (physics.c:628-690) or a super-catastrophic disruption
+        // (physics.c:797-840) registers nothing; a full fragmentation
+        // (physics.c:899-1013) registers the pairs of its bodies, survivor
+        // first, then the appended ones.
+        const int A = lp ? p : k, gone = lp ? k : p;
+        rm_at[gone] = kv;
+        M.removed.push_back({gone, kv});
+        if (nborn != 0) {
+            e.kind = MeshEvent::BLOCK;
+            e.first = (int)M.frag_ids.size();
+            e.count = nborn + 1;
+            M.frag_ids.push_back(A);
+            for (int q = 0; q < nborn; ++q) M.frag_ids.push_back(born[q]);
+            ncol += nborn;
+        } else {
+            e.kind = MeshEvent::NONE;
+        }
+        M.events.push_back(e);
+        return mark_dirty(A, kv);
     };
     // Pairs of visit k from traversal position (cell from_l, id >= from_p)
     // onwards, all decided on the host.
@@ -1650,10 +1863,10 @@ extern "C" ncg_status ncg_mesh_pass(ncg_ctx *ctx, const ncg_body *bodies, int32_
             hc.members((unsigned)((long)kk + stencil_offset(l, d)), cell);
             for (int p : cell) {
                 if (p >= k) break;
-                if (l == from_l && p < from_p) continue;
+                if ((l == from_l && p < from_p) || !gathered(p, k)) continue;
                 ++htests;
                 if (ops->test(ops->user, p, k)) {
-                    r = collide(p, k, k);
+                    r = collide(p, k, k, l);
                     if (r != NCG_OK) return r;
                 }
             }
@@ -1680,9 +1893,11 @@ extern "C" ncg_status ncg_mesh_pass(ncg_ctx *ctx, const ncg_body *bodies, int32_
                 ++htests;
                 visit.push_back({k, p});
                 if (ops->test(ops->user, p, k)) {
-                    ncg_status r = collide(p, k, k);
+                    ncg_status r = collide(p, k, k, -1);
                     if (r != NCG_OK) return r;
-                    visit.back() = {p, k};
+                    const MeshEvent &ev = M.events.back();
+                    if (ev.kind == MeshEvent::NONE) visit.pop_back();
+                    else visit.back() = ev.reg;
                     if (p > k && Rp <= gam) {
                         // p is inserted later, at its new position: its cell
                         // may change, and its own visit is decided on the host.
@@ -1733,14 +1948,14 @@ extern "C" ncg_status ncg_mesh_pass(ncg_ctx *ctx, const ncg_body *bodies, int32_
                 l = hc.eff(p) == NOKEY || hc.eff(k) == NOKEY
                         ? -1
                         : hc.stencil_index((long)hc.eff(p) - (long)hc.eff(k));
-            if (l >= 0) todo.push_back({(long)(26 - l) * (1L << 31) + p, p});
+            if (l >= 0 && gathered(p, k)) todo.push_back({(long)(26 - l) * (1L << 31) + p, p});
             ++ci;
         }
         if (!owed.empty()) {
             ncg_status r = load_cells(ctx, hc, n, ins, key, skey, sid);
             if (r != NCG_OK) return r;
             for (int p : owed) {
-                if (hc.eff(p) == NOKEY || hc.eff(k) == NOKEY) continue;
+                if (hc.eff(p) == NOKEY || hc.eff(k) == NOKEY || !gathered(p, k)) continue;
                 const int l = hc.stencil_index((long)hc.eff(p) - (long)hc.eff(k));
                 if (l >= 0) todo.push_back({(long)(26 - l) * (1L << 31) + p, p});
             }
@@ -1751,10 +1966,10 @@ extern "C" ncg_status ncg_mesh_pass(ncg_ctx *ctx, const ncg_body *bodies, int32_
             const int p = todo[q].second;
             ++htests;
             if (ops->test(ops->user, p, k)) {
-                err = collide(p, k, k);
+                const int l = 26 - (int)(todo[q].first >> 31);
+                err = collide(p, k, k, l);
                 if (err != NCG_OK) return err;
                 // The rest of this visit involves k, which changed: host.
-                const int l = 26 - (int)(todo[q].first >> 31);
                 err = full_rest(k, l, p + 1);
                 if (err != NCG_OK) return err;
                 break;
@@ -1793,6 +2008,26 @@ extern "C" ncg_status ncg_mesh_pass(ncg_ctx *ctx, const ncg_body *bodies, int32_
         M.inserted = nins;
         stats->total_neighbours = (int64_t)ht[0];
     }
+    M.n_end = ncol;
+    if (!M.removed.empty()) {
+        // Later visits do not gather a removed body (collision.c:296): one
+        // neighbour fewer for every later visit whose cells hold it.
+        ncg_status r = load_cells(ctx, hc, n, ins, key, skey, sid);
+        if (r != NCG_OK) return r;
+        int64_t drop = 0;
+        for (const auto &rv : M.removed) {
+            const unsigned kb = hc.eff(rv.first);
+            if (kb == NOKEY) continue;
+            for (int l = 0; l < 27; ++l) {
+                hc.members((unsigned)((long)kb + stencil_offset(l, d)), mem);
+                for (int q : mem) drop += q > rv.second;
+            }
+        }
+        M.rm_drop = drop;
+        stats->total_neighbours -= drop;
+    }
+    err = explicit_lists(ctx, M, hc, n, ins, key, skey, sid);
+    if (err != NCG_OK) return err;
     stats->collisions = ncoll;
     stats->host_tests = htests;
     return NCG_OK;
@@ -1821,13 +2056,7 @@ extern "C" ncg_status ncg_mesh_pass_dev(ncg_ctx *ctx, ncg_body *d_bodies, int32_
     NCG_HIP(hipSetDevice(ctx->device));
     hipStream_t st = pick_stream(ctx, stream);
     MeshState &M = ctx->mesh;
-    M.valid = false;
-    M.pairs_known = false;
-    M.ord_lists = false;
-    M.extra_pairs.clear();
-    M.big_visits.clear();
-    M.big_pairs_all.clear();
-    M.collided.clear();
+    reset_pass(M, n);
     if (n == 0) {
         M.valid = true;
         M.pairs_known = true;
@@ -1934,8 +2163,11 @@ static ncg_status mesh_gravity_core(ncg_ctx *ctx, const DBody *db, int32_t n, co
     const size_t nex = std::max<size_t>(M.extra_pairs.size(), 1);
     int2 *dextra = dbuf<int2>(ctx, "g_extra", nex);
     if (!dct || !pm || !rr || !sx || !bpart || !dextra) return NCG_ERR_NOMEM;
-    const unsigned *key = dbuf<unsigned>(ctx, "m_key", n);
-    const int *sid = dbuf<int>(ctx, "m_sid", n);
+    // The pass's arrays cover its M.n slots; bodies appended during the pass
+    // (slots M.n .. n - 1) are not binned.
+    const int32_t nkey = M.n;
+    const unsigned *key = dbuf<unsigned>(ctx, "m_key", nkey);
+    const int *sid = dbuf<int>(ctx, "m_sid", nkey);
     const unsigned *cs = dbuf<unsigned>(ctx, "m_cs", M.table_keys);
     const unsigned *ce = dbuf<unsigned>(ctx, "m_ce", M.table_keys);
     if (!M.extra_pairs.empty())
@@ -1977,7 +2209,7 @@ static ncg_status mesh_gravity_core(ncg_ctx *ctx, const DBody *db, int32_t n, co
         }
         // Id-ordered neighbourhood lists, once per pass (every gravity call
         // of the pass shares them).
-        const unsigned *skey = dbuf<unsigned>(ctx, "m_skey", n);
+        const unsigned *skey = dbuf<unsigned>(ctx, "m_skey", nkey);
         NCG_REQUIRE(27.0 * ins < 4294967295.0, "%d binned bodies: neighbourhood lists too long", ins);
         const size_t lcap = std::max<size_t>(27 * (size_t)ins, 1);
         unsigned *L = dbuf<unsigned>(ctx, "g_L", lcap);
@@ -2027,12 +2259,26 @@ static ncg_status mesh_gravity_core(ncg_ctx *ctx, const DBody *db, int32_t n, co
                                bvk, bvoff, bvo, nbv, L, lw, le, dacc);
         if (n > 3)
             hipLaunchKernelGGL(k_mrest_ord, dim3(cdiv(n - 3, MT)), dim3(MT), 0, st, db, (int)n, dex,
-                               key, dcen, G, s, bvk, bvoff, bvo, nbv, dacc);
+                               key, (int)nkey, dcen, G, s, bvk, bvoff, bvo, nbv, dacc);
         NCG_CHECK_LAUNCH();
+        if (!M.x_targets.empty()) {
+            const int ntg = (int)M.x_targets.size() / 3;
+            int *tg = dbuf<int>(ctx, "g_xtg", M.x_targets.size());
+            int *pl = dbuf<int>(ctx, "g_xpl", std::max<size_t>(M.x_partners.size(), 1));
+            if (!tg || !pl) return NCG_ERR_NOMEM;
+            NCG_HIP(hipMemcpyAsync(tg, M.x_targets.data(), sizeof(int) * M.x_targets.size(),
+                                   hipMemcpyHostToDevice, st));
+            if (!M.x_partners.empty())
+                NCG_HIP(hipMemcpyAsync(pl, M.x_partners.data(), sizeof(int) * M.x_partners.size(),
+                                       hipMemcpyHostToDevice, st));
+            hipLaunchKernelGGL(k_mexplicit, dim3(cdiv(ntg, MT)), dim3(MT), 0, st, db, dex, dcen, G, s,
+                               (const int *)tg, ntg, (const int *)pl, dacc);
+            NCG_CHECK_LAUNCH();
+        }
         NCG_HIP(hipStreamWaitEvent(st, ctx->ev_join, 0));
     } else {
         ProfScope ps(ctx, st, "mesh_gravity");
-        const unsigned *skey = dbuf<unsigned>(ctx, "m_skey", n);
+        const unsigned *skey = dbuf<unsigned>(ctx, "m_skey", nkey);
         const dim3 gs(cdiv(std::max(ins, 1), MT));
         if (prec == NCG_F64_FAST)
             hipLaunchKernelGGL(k_mgrav<1>, gs, dim3(MT), 0, st, db, (int)n, dex, ins, cs, ce, pm, rr,
@@ -2040,7 +2286,8 @@ static ncg_status mesh_gravity_core(ncg_ctx *ctx, const DBody *db, int32_t n, co
         else
             hipLaunchKernelGGL(k_mgrav<2>, gs, dim3(MT), 0, st, db, (int)n, dex, ins, cs, ce, pm, rr,
                                sx, sid, skey, (int)d, dcen, G, s, dacc, dct);
-        hipLaunchKernelGGL(k_mgrav_rest, dim3(cdiv(n, MT)), dim3(MT), 0, st, db, (int)n, dex, key, dcen,
+        hipLaunchKernelGGL(k_mgrav_rest, dim3(cdiv(n, MT)), dim3(MT), 0, st, db, (int)n, dex, key,
+                           (int)nkey, dcen,
                            G, s, dacc, dct);
         NCG_CHECK_LAUNCH();
         hipLaunchKernelGGL(k_big_partial, dim3(nbp), dim3(MT), 0, st, db, (int)n, dex, G, s, bpart);
@@ -2064,9 +2311,10 @@ extern "C" ncg_status ncg_mesh_gravity(ncg_ctx *ctx, const ncg_body *bodies, int
     NCG_REQUIRE(prec == NCG_F64_EXACT || prec == NCG_F64_FAST || prec == NCG_MIXED,
                 "unknown precision %d", (int)prec);
     MeshState &M = ctx->mesh;
-    if (!M.valid || M.n != n)
+    if (!M.valid || M.n_end != n)
         return ncg_fail(ctx, NCG_ERR_INVALID,
-                        "ncg_mesh_gravity needs a ncg_mesh_pass over the same %d body slots", n);
+                        "ncg_mesh_gravity needs a ncg_mesh_pass over the same %d body slots "
+                        "(those of the pass and the bodies it appended)", n);
     if (n == 0) return NCG_OK;
     NCG_HIP(hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;
@@ -2120,7 +2368,7 @@ extern "C" ncg_status ncg_mesh_gravity_dev(ncg_ctx *ctx, const ncg_body *d_bodie
     NCG_REQUIRE(prec == NCG_F64_EXACT || prec == NCG_F64_FAST || prec == NCG_MIXED,
                 "unknown precision %d", (int)prec);
     MeshState &M = ctx->mesh;
-    if (!M.valid || M.n != n)
+    if (!M.valid || M.n_end != n)
         return ncg_fail(ctx, NCG_ERR_INVALID,
                         "ncg_mesh_gravity_dev needs a mesh pass over the same %d body slots", n);
     if (n == 0) return NCG_OK;
@@ -2164,14 +2412,28 @@ extern "C" ncg_status ncg_mesh_pairs(ncg_ctx *ctx, ncg_pair *out, int64_t capaci
         off[k] = tot;
         tot += cnt[k];
     }
-    *count = tot;
-    if (tot > capacity)
-        return ncg_fail(ctx, NCG_ERR_CAPACITY, "%lld pairs exceed capacity %lld", tot,
+    // Registered pairs: the gathered entries less those of bodies removed
+    // before the visit, each collision's entry replaced by what its outcome
+    // registered (physics.c:1043-1091).
+    long long fin = tot - M.rm_drop;
+    bool simple = M.rm_drop == 0;
+    for (const MeshEvent &e : M.events) {
+        if (e.l < 0) continue;  // big visits: recorded as registered
+        if (e.kind == MeshEvent::NONE) --fin;
+        if (e.kind == MeshEvent::BLOCK) fin += (long long)e.count * (e.count - 1) / 2 - 1;
+        simple &= e.kind == MeshEvent::PAIR;
+    }
+    *count = fin;
+    if (fin > capacity)
+        return ncg_fail(ctx, NCG_ERR_CAPACITY, "%lld pairs exceed capacity %lld", fin,
                         (long long)capacity);
     if (tot == 0) return NCG_OK;
     long long *doff = dbuf<long long>(ctx, "p_off", n);
     int2 *dout = dbuf<int2>(ctx, "p_out", (size_t)tot);
     if (!doff || !dout) return NCG_ERR_NOMEM;
+    std::vector<ncg_pair> all;
+    if (!simple) all.resize((size_t)tot);
+    ncg_pair *gl = simple ? out : all.data();
     NCG_HIP(hipMemcpyAsync(doff, off.data(), sizeof(long long) * n, hipMemcpyHostToDevice, st));
     if (ins > 0) {
         hipLaunchKernelGGL(k_mpairs, dim3(cdiv(ins, MT)), dim3(MT), 0, st,
@@ -2182,19 +2444,50 @@ extern "C" ncg_status ncg_mesh_pairs(ncg_ctx *ctx, ncg_pair *out, int64_t capaci
                            (int)d, doff, dout);
         NCG_CHECK_LAUNCH();
     }
-    NCG_HIP(hipMemcpyAsync(out, dout, sizeof(int2) * (size_t)tot, hipMemcpyDeviceToHost, st));
+    NCG_HIP(hipMemcpyAsync(gl, dout, sizeof(int2) * (size_t)tot, hipMemcpyDeviceToHost, st));
     NCG_HIP(hipStreamSynchronize(st));
     for (const auto &bv : M.big_visits)
-        for (int q = 0; q < bv.count; ++q) out[off[bv.k] + q] = M.big_pairs_all[bv.first + q];
-    // A collision registers (p, k) instead of (k, p) (physics.c:1065-1080).
-    for (const ncg_pair &c : M.collided) {
-        const int p = c.fst, k = c.snd;
-        for (long long q = off[k]; q < off[k] + cnt[k]; ++q)
-            if (out[q].fst == k && out[q].snd == p) {
-                out[q].fst = p;
-                out[q].snd = k;
+        for (int q = 0; q < bv.count; ++q) gl[off[bv.k] + q] = M.big_pairs_all[bv.first + q];
+    // The entry of each collision of a small visit: the occ-th entry of its
+    // partner in the visit.
+    std::map<long long, const MeshEvent *> at;
+    for (const MeshEvent &e : M.events) {
+        if (e.l < 0) continue;
+        int seen = 0;
+        long long pos = -1;
+        for (long long q = off[e.k]; q < off[e.k] + cnt[e.k]; ++q)
+            if (gl[q].snd == e.p && seen++ == e.occ) {
+                pos = q;
                 break;
             }
+        if (pos < 0)
+            return ncg_fail(ctx, NCG_ERR_INVALID, "collision (%d, %d) not among the pairs of visit %d",
+                            e.p, e.k, e.k);
+        at[pos] = &e;
     }
+    if (simple) {
+        for (const auto &pe : at) out[pe.first] = pe.second->reg;
+        return NCG_OK;
+    }
+    std::vector<int> rmv((size_t)n, INT32_MAX);
+    for (const auto &rv : M.removed) rmv[rv.first] = rv.second;
+    long long o = 0;
+    for (int32_t k = 0; k < n; ++k)
+        for (long long q = off[k]; q < off[k] + cnt[k]; ++q) {
+            auto it = at.find(q);
+            if (it != at.end()) {
+                const MeshEvent &e = *it->second;
+                if (e.kind == MeshEvent::PAIR) out[o++] = e.reg;
+                if (e.kind == MeshEvent::BLOCK)
+                    for (int a = 1; a < e.count; ++a)
+                        for (int b = 0; b < a; ++b)
+                            out[o++] = {M.frag_ids[e.first + a], M.frag_ids[e.first + b]};
+                continue;
+            }
+            if (rmv[gl[q].snd] < k) continue;  // removed before visit k gathered
+            out[o++] = gl[q];
+        }
+    if (o != fin)
+        return ncg_fail(ctx, NCG_ERR_INVALID, "pair list of %lld entries, %lld expected", o, fin);
     return NCG_OK;
 }

@@ -29,6 +29,16 @@ struct BigVisit {
     int k, first, count;
 };
 
+// What a resolved collision of mesh() left at its entry (k, p) of visit k
+// (stencil index l, occ-th entry of p in that visit) of the pair list.
+struct MeshEvent {
+    enum Kind { PAIR, NONE, BLOCK };
+    int k, l, p, occ;
+    Kind kind;
+    ncg_pair reg;       // PAIR: the registered pair (physics.c:1065-1080, 886-890)
+    int first, count;   // BLOCK: frag_ids[first, first + count), survivor first
+};
+
 // Mesh binning kept between ncg_mesh_pass and ncg_mesh_gravity.
 struct MeshState {
     bool valid = false;
@@ -44,7 +54,14 @@ struct MeshState {
     std::vector<ncg_pair> extra_pairs;    // big-body pairs with both ids > 2
     std::vector<ncg_pair> big_pairs_all;  // every pair registered by big visits
     std::vector<BigVisit> big_visits;
-    std::vector<ncg_pair> collided;       // (p, k) in resolution order
+    std::vector<MeshEvent> events;        // collisions in resolution order
+    std::vector<int32_t> frag_ids;        // bodies of the fragmentation blocks
+    std::vector<std::pair<int, int>> removed;  // (body, visit during which it was removed)
+    int32_t n_end = 0;                    // n + bodies appended during the pass
+    int64_t rm_drop = 0;                  // gathered entries that removed bodies left out
+    // Bodies whose exact-field sums the host lists term by term (survivors and
+    // fragments of full fragmentations): (id, first, count) into x_partners.
+    std::vector<int> x_targets, x_partners;
 };
 
 struct ProfRec {

@@ -17,12 +17,16 @@
  *                        the host decision and outcome (every outcome: bounces,
  *                        mergers, fragmentation; created slots are reported
  *                        from the get_free_index wrap)
- *   mesh                 collision.c:324-417 -> ncg_mesh_pass (mode 0)
+ *   mesh                 collision.c:324-417 -> ncg_mesh_pass (mode 0), with the
+ *                        same host decision and outcomes (every outcome;
+ *                        fragments are appended, as mesh mode requires)
  *   get_neighbours_mesh  physics.c:1094-1118 -> ncg_mesh_pass (mode 1)
  * The hash table of the reference is left untouched (all NULL) and
  * how_many_modified stays 0, so end_of_timestep's clearing loop
- * (rk4.c:361-364) has nothing to do.  The `pairs` array is not filled: the
- * replaced vector_field evaluates the same pairs from the GPU binning.
+ * (rk4.c:361-364) has nothing to do.  The `pairs` array holds only what
+ * collision_treatment registers: the replaced vector_field evaluates the
+ * pass's pairs from the GPU binning and the library's record of each
+ * collision's outcome.
  *
  * Environment:
  *   NCG_DEVICE     GPU ordinal (default 0)
@@ -204,15 +208,6 @@ static int32_t replay_created(void *u, int32_t *ids, int32_t cap)
     return nb;
 }
 
-static void require_two_body_outcomes(void)
-{
-    if (collision_type() > 1) {
-        fprintf(stderr, "Error : ncg drop-in: merger and fragmentation outcomes are supported on "
-                        "the brute-force path only (mesh(): use elastic or inelastic collisions).\n");
-        abort();
-    }
-}
-
 void ncg_dropin_brute_force(struct moonlet *X)
 {
     ncg_ctx *c = ctx();
@@ -249,7 +244,6 @@ static void mesh_common(struct moonlet *X, int mode)
 void ncg_dropin_mesh(struct moonlet *X)
 {
 #if mesh_bool
-    require_two_body_outcomes();
     mesh_common(X, 0);
 #else
     (void)X;

@@ -152,6 +152,19 @@ CONFIGS = {
         "collision_cube_cells": "256", "collision_cube_min": "2.56",
         "how_many_neighbours": "100.0", "switch_to_brute_force": "110",
         "sma_min": "0.9", "sma_max": "1.1", "inclination_max": "0.01"})),
+    # The same mesh run with the reference's default collision model
+    # (mergers, partial and full fragmentation: bodies removed mid-pass,
+    # fragments appended with their pairs) and with instant mergers.
+    "run_mesh_frag": ("run", _run_cfg("mesh", 16384, 10, DT2, {
+        "collision_cube_cells": "256", "collision_cube_min": "2.56",
+        "how_many_neighbours": "100.0", "switch_to_brute_force": "110",
+        "sma_min": "0.9", "sma_max": "1.1", "inclination_max": "0.01",
+        "inelastic_collision_bool": "0", "fragmentation_bool": "1", "N_max": "40000"})),
+    "run_mesh_merge": ("run", _run_cfg("mesh", 16384, 10, DT2, {
+        "collision_cube_cells": "256", "collision_cube_min": "2.56",
+        "how_many_neighbours": "100.0", "switch_to_brute_force": "110",
+        "sma_min": "0.9", "sma_max": "1.1", "inclination_max": "0.01",
+        "inelastic_collision_bool": "0", "instant_merger_bool": "1"})),
     # Full-size mesh disk of 2^20 bodies (CPU baseline of the mesh module).
     "run_c2": ("run", _run_cfg("mesh", 1048576, 3, DT2, {
         "collision_cube_cells": "256", "collision_cube_min": "2.56",

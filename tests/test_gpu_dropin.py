@@ -79,3 +79,20 @@ def test_mesh_run_identical(tmp_path):
     tr = compare("run_mesh", b, tmp_path, 10)
     assert sum(1 for l in tr if l.startswith("C ")) > 10
     assert any(l.startswith("MESH") for l in tr) and any(l.startswith("GNM") for l in tr)
+
+
+@pytest.mark.parametrize("cfg,mass,radius", [("run_mesh_frag", 1e-8, 2e-3),
+                                             ("run_mesh_frag", 1e-9, 1e-3),
+                                             ("run_mesh_merge", 1e-10, 1e-3)])
+def test_mesh_fragmentation_and_merger_runs_identical(tmp_path, cfg, mass, radius):
+    # Mesh module with the reference's default collision model (mergers;
+    # partial, full and super-catastrophic fragmentation: bodies removed
+    # mid-pass, fragments appended with their pairs) and with instant
+    # mergers: 16384 bodies, 10 steps, every traced call identical, the exact
+    # mesh gravity included.
+    b = ic.disk(16384, seed=2, mass=mass, radius=radius)
+    tr = compare(cfg, b, tmp_path, 10)
+    coll = [l.split() for l in tr if l.startswith("C ")]
+    assert len(coll) > 100
+    if cfg == "run_mesh_frag":
+        assert any(int(c[2]) >= 16384 or int(c[3]) >= 16384 for c in coll)  # fragments collide

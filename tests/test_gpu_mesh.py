@@ -195,3 +195,146 @@ def test_mesh_gravity_exact_few_bodies(ctx, n):
     ref, rcm = orc.vector_field_mesh(b, pairs, ex, central, 1.0, 0.05)
     assert np.array_equal(bits(acc), bits(ref))
     assert np.array_equal(bits(cm), bits(rcm))
+
+
+class MeshToyWorld:
+    """A host world for mesh(): arrays of `cap` slots like NcorpiON's N_max,
+    closest_approach as the decision, and test-only outcomes in the four
+    shapes of the reference's (physics.c:1043-1091): a bounce registering
+    (a, b); a merger into the heavier body registering nothing; a partial
+    fragmentation changing both masses and registering (heavier, lighter); a
+    full fragmentation keeping the heavier body, removing the other and
+    appending two bodies, registering the pairs of its three bodies
+    (id[p], id[q]), q < p."""
+
+    def __init__(self, b, cap, dt):
+        n = b.shape[0]
+        self.b = np.zeros((cap, 8))
+        self.b[:n] = b
+        self.e = np.zeros(cap, np.int32)
+        self.e[:n] = 1
+        self.largest = n - 1
+        self.born = []
+        self.log = []
+        self.pairs = []
+        self.kinds = set()
+        self.dt = dt
+
+    def test(self, a, c):
+        return orc.closest_approach(self.b, a, c, self.dt, self.e)[0]
+
+    def resolve(self, a, c):
+        b = self.b
+        A, B = (a, c) if b[a, 6] > b[c, 6] else (c, a)
+        if not (self.e[a] and self.e[c]):
+            return
+        self.log.append((a, c))
+        kind = (7 * a + c) % 4
+        self.kinds.add(kind)
+        m = b[A, 6] + b[B, 6]
+        if kind == 0:
+            va = b[a, 3:6].copy()
+            b[a, 3:6] = b[c, 3:6]
+            b[c, 3:6] = va
+            self.pairs.append((a, c))
+        elif kind == 1:
+            b[A, 3:6] = (b[A, 6] * b[A, 3:6] + b[B, 6] * b[B, 3:6]) / m
+            b[A, 6] = m
+            b[A, 7] = np.cbrt(b[A, 7] ** 3 + b[B, 7] ** 3)
+            self.e[B] = 0
+        elif kind == 2:
+            b[A, 6], b[B, 6] = 0.75 * m, 0.25 * m
+            b[A, 3:6], b[B, 3:6] = b[B, 3:6].copy(), b[A, 3:6].copy()
+            self.pairs.append((A, B))
+        else:
+            ids = [A]
+            for q in range(2):
+                s = self.largest + 1
+                self.largest = s
+                b[s] = b[A]
+                b[s, 0] += (3.0 + 3.0 * q) * b[A, 7]
+                b[s, 3:6] = -b[A, 3:6]
+                b[s, 6] = 0.1 * m
+                self.e[s] = 1
+                self.born.append(s)
+                ids.append(s)
+            b[A, 6] = 0.8 * m
+            self.e[B] = 0
+            for p in range(len(ids)):
+                for q in range(p):
+                    self.pairs.append((ids[p], ids[q]))
+
+    def created(self):
+        out, self.born = self.born, []
+        return out
+
+
+def toy_mesh(w, n0, gam, cube, cells):
+    """mesh(), collision.c:324-417 (every body at most the mesh size): the
+    hash holds each cell's visited bodies in insertion order; a visit gathers
+    the existing bodies of its 27 cells (newest first per cell) and pops them
+    last-in first-out (collision.c:264-316, structure.c:712-841)."""
+    cells_of = {}
+    total = small = 0
+    for k in range(n0):
+        if not w.e[k]:
+            continue
+        small += 1
+        idx = orc.cell_indexes(w.b[k, 0], w.b[k, 1], w.b[k, 2], gam, cube, cells)
+        if idx is None:
+            continue
+        nghb = []
+        for l in range(27):
+            for q in reversed(cells_of.get(int(idx[l]), [])):
+                if w.e[q]:
+                    nghb.append(q)
+        cells_of.setdefault(int(idx[13]), []).append(k)
+        while nghb:
+            p = nghb.pop()
+            if w.test(p, k):
+                w.resolve(p, k)
+            else:
+                w.pairs.append((k, p))
+            total += 1
+    return total, small
+
+
+@pytest.mark.parametrize("seed", [31, 32])
+def test_mesh_pass_with_removed_and_appended_bodies(ctx, seed):
+    """Outcomes that remove bodies and append new ones, as mergers and
+    fragmentation do in mesh mode: the replay gives mesh()'s collision
+    sequence, counters, final state and pair list, and the exact field over
+    the appended slots is vector_field's over that list, bit for bit."""
+    n0, cap, dt, cells, gam = 3000, 6000, 0.01, 16, 0.01
+    cube = cells * gam
+    rng = np.random.default_rng(seed)
+    b = blob(n0, seed, half=0.06, rmin=1.5e-3, rmax=3e-3)
+    b[:, 6] = rng.uniform(1e-9, 1e-8, n0)
+    ref = MeshToyWorld(b, cap, dt)
+    total, small = toy_mesh(ref, n0, gam, cube, cells)
+    got = MeshToyWorld(b, cap, dt)
+    st = ctx.mesh_pass(got.b, got.e, dt, gam, cube, cells, 0, got.test, got.resolve,
+                       created=got.created, n=n0)
+    assert got.log == ref.log
+    assert ref.kinds == {0, 1, 2, 3} and ref.largest >= n0
+    assert st["collisions"] == len(ref.log)
+    assert st["total_neighbours"] == total
+    assert st["how_many_small"] == small
+    assert np.array_equal(got.e, ref.e)
+    live = ref.e != 0
+    assert np.array_equal(bits(got.b[live]), bits(ref.b[live]))
+    pairs = ctx.mesh_pairs()
+    assert np.array_equal(pairs, np.array(ref.pairs, np.int32).reshape(-1, 2))
+    # The field after a drift over the n0 slots and the appended ones.
+    ne = ref.largest + 1
+    b2 = ref.b[:ne].copy()
+    b2[:, 0:3] += 0.002 * b2[:, 3:6]
+    ex = ref.e[:ne].copy()
+    central = (0.01, -0.02, 0.005, 1.0)
+    acc, cm = ctx.mesh_gravity(b2, ex, central, 1.0, 0.1, "exact")
+    racc, rcm = orc.vector_field_mesh(b2, pairs, ex, central, 1.0, 0.1)
+    lv = ex != 0
+    assert np.array_equal(bits(acc[lv]), bits(racc[lv]))
+    assert np.array_equal(bits(cm), bits(rcm))
+    acc, cm = ctx.mesh_gravity(b2, ex, central, 1.0, 0.1, "fast64")
+    assert np.abs(acc[lv] - racc[lv]).max() / np.abs(racc[lv]).max() < 1e-12
