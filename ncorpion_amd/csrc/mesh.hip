@@ -694,15 +694,22 @@ __global__ __launch_bounds__(MT) void k_mgrav(const DBody *__restrict__ b, int n
 }
 
 // Bodies 0..2 receive += G m_k (X_k - X_big)/D^3 from every k > 2
-// (physics.c:221-224, 237-240, 253-256): per-block partial sums.
+// (physics.c:221-224, 237-240, 253-256), and CM_acc the central terms of
+// every body (physics.c:143-145): per-block partial sums, 12 per block.
 __global__ __launch_bounds__(MT) void k_big_partial(const DBody *__restrict__ b, int n,
                                                     const int32_t *__restrict__ ex, double G,
-                                                    double s, double *__restrict__ part)
+                                                    double s, const double *__restrict__ cterm,
+                                                    const double *__restrict__ cen,
+                                                    double *__restrict__ part)
 {
+    const int cp = cen && cen[7] != 0.0;
     __shared__ double sh[MT];
-    double a[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int k = 3 + blockIdx.x * MT + threadIdx.x; k < n; k += gridDim.x * MT) {
+    double a[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int k = blockIdx.x * MT + threadIdx.x; k < n; k += gridDim.x * MT) {
         if (ex && !ex[k]) continue;
+        if (cp)
+            for (int c = 0; c < 3; ++c) a[9 + c] += cterm[3 * (size_t)k + c];
+        if (k < 3) continue;
         const DBody q = b[k];
         for (int big = 0; big < 3; ++big) {
             if (ex && !ex[big]) continue;
@@ -717,39 +724,30 @@ __global__ __launch_bounds__(MT) void k_big_partial(const DBody *__restrict__ b,
             a[3 * big + 2] += gm * dz / D3;
         }
     }
-    for (int c = 0; c < 9; ++c) {
+    for (int c = 0; c < 12; ++c) {
         const double r = bsum(a[c], sh);
-        if (threadIdx.x == 0) part[9 * blockIdx.x + c] = r;
+        if (threadIdx.x == 0) part[12 * blockIdx.x + c] = r;
     }
 }
 
 // Bodies 0..2: add the partials, then the pairs (0,1), (0,2), (1,2)
-// (physics.c:300-341); also sums CM_acc.  One block.
+// (physics.c:300-341); CM_acc from its partials.  One block.
 __global__ __launch_bounds__(MT) void k_big_final(const DBody *__restrict__ b, int n,
                                                   const int32_t *__restrict__ ex,
                                                   const double *__restrict__ part, int np,
                                                   double G, double s, double *__restrict__ acc,
-                                                  const double *__restrict__ cterm, const double *__restrict__ cen,
-                                                  double *__restrict__ cm)
+                                                  const double *__restrict__ cen, double *__restrict__ cm)
 {
     const int cp = cen && cen[7] != 0.0;
     __shared__ double sh[MT];
-    double a[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double a[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     for (int q = threadIdx.x; q < np; q += MT)
-        for (int c = 0; c < 9; ++c) a[c] += part[9 * q + c];
-    double tot[9];
-    for (int c = 0; c < 9; ++c) tot[c] = bsum(a[c], sh);
-    double cma[3] = {0, 0, 0};
-    if (cp)
-        for (int k = threadIdx.x; k < n; k += MT) {
-            if (ex && !ex[k]) continue;
-            for (int c = 0; c < 3; ++c) cma[c] += cterm[3 * (size_t)k + c];
-        }
-    double cmt[3];
-    for (int c = 0; c < 3; ++c) cmt[c] = bsum(cma[c], sh);
+        for (int c = 0; c < 12; ++c) a[c] += part[12 * q + c];
+    double tot[12];
+    for (int c = 0; c < 12; ++c) tot[c] = bsum(a[c], sh);
     if (threadIdx.x != 0) return;
     if (cp && cm)
-        for (int c = 0; c < 3; ++c) cm[c] = cmt[c];
+        for (int c = 0; c < 3; ++c) cm[c] = tot[9 + c];
     for (int big = 0; big < 3 && big < n; ++big) {
         if (ex && !ex[big]) continue;
         for (int c = 0; c < 3; ++c) acc[3 * big + c] += tot[3 * big + c];
@@ -2159,7 +2157,7 @@ static ncg_status mesh_gravity_core(ncg_ctx *ctx, const DBody *db, int32_t n, co
     double *rr = dbuf<double>(ctx, "g_rr", std::max(ins, 1));
     int *sx = dbuf<int>(ctx, "g_sx", std::max(ins, 1));
     const int nbp = std::max(1, std::min(512, (int)cdiv(n, MT)));
-    double *bpart = dbuf<double>(ctx, "g_bpart", 9 * (size_t)nbp);
+    double *bpart = dbuf<double>(ctx, "g_bpart", 12 * (size_t)nbp);
     const size_t nex = std::max<size_t>(M.extra_pairs.size(), 1);
     int2 *dextra = dbuf<int2>(ctx, "g_extra", nex);
     if (!dct || !pm || !rr || !sx || !bpart || !dextra) return NCG_ERR_NOMEM;
@@ -2180,7 +2178,7 @@ static ncg_status mesh_gravity_core(ncg_ctx *ctx, const DBody *db, int32_t n, co
     }
     if (prec == NCG_F64_EXACT) {
         // Pair-list order, bit for bit: recorded big visits as (visiting id,
-        // ascending partner ids), bodies 0..2 and CM_acc on a second stream.
+        // ascending partner ids); the per-body walks on a second stream.
         std::vector<int> hbk, hbo(1, 0), hbp;
         for (const BigVisit &bv : M.big_visits) {
             hbk.push_back(bv.k);
@@ -2239,42 +2237,48 @@ static ncg_status mesh_gravity_core(ncg_ctx *ctx, const DBody *db, int32_t n, co
             NCG_CHECK_LAUNCH();
             M.ord_lists = true;
         }
-        // Terms of the body-order sums, then the sums on the second stream.
+        // Terms of the body-order sums; the sums themselves stay on the
+        // caller's stream and the per-body walks go to the second stream.
+        // Dispatched right behind k_mterms in the same queue, the single
+        // latency-bound k_mseq workgroup is placed before the thousands of
+        // k_mord workgroups; placed after them it runs about twice as long.
         const int nch = cdiv(n, SQ_CH);
         double *T = dbuf<double>(ctx, "g_seqT", (size_t)nch * 12 * SQ_CH);
-        if (!T) return NCG_ERR_NOMEM;
+        const int ntg = (int)M.x_targets.size() / 3;
+        int *tg = dbuf<int>(ctx, "g_xtg", std::max<size_t>(M.x_targets.size(), 1));
+        int *pl = dbuf<int>(ctx, "g_xpl", std::max<size_t>(M.x_partners.size(), 1));
+        if (!T || !tg || !pl) return NCG_ERR_NOMEM;
+        if (ntg) {
+            NCG_HIP(hipMemcpyAsync(tg, M.x_targets.data(), sizeof(int) * M.x_targets.size(),
+                                   hipMemcpyHostToDevice, st));
+            NCG_HIP(hipMemcpyAsync(pl, M.x_partners.data(), sizeof(int) * M.x_partners.size(),
+                                   hipMemcpyHostToDevice, st));
+        }
         ProfScope ps(ctx, st, "mesh_gravity");
         hipLaunchKernelGGL(k_mterms, dim3(cdiv(nch * SQ_CH, MT)), dim3(MT), 0, st, db, (int)n, dex,
                            dcen, G, s, nch * SQ_CH, T);
         NCG_CHECK_LAUNCH();
         NCG_HIP(hipEventRecord(ctx->ev_fork, st));
         NCG_HIP(hipStreamWaitEvent(ctx->aux, ctx->ev_fork, 0));
-        hipLaunchKernelGGL(k_mseq, dim3(1), dim3(256), 0, ctx->aux, db, (int)n, dex, dcen, G, s,
+        hipLaunchKernelGGL(k_mseq, dim3(1), dim3(256), 0, st, db, (int)n, dex, dcen, G, s,
                            reinterpret_cast<const double2 *>(T), nch, dacc, dcm);
         NCG_CHECK_LAUNCH();
-        NCG_HIP(hipEventRecord(ctx->ev_join, ctx->aux));
+        hipStream_t ax = ctx->aux;
         if (ins > 0)
-            hipLaunchKernelGGL(k_mord, dim3(cdiv(ins, MT)), dim3(MT), 0, st, db, (int)n, dex, ins,
+            hipLaunchKernelGGL(k_mord, dim3(cdiv(ins, MT)), dim3(MT), 0, ax, db, (int)n, dex, ins,
                                cs, ce, pm, rr, sx, sid, skey, (int)d, dcen, G, s,
                                bvk, bvoff, bvo, nbv, L, lw, le, dacc);
         if (n > 3)
-            hipLaunchKernelGGL(k_mrest_ord, dim3(cdiv(n - 3, MT)), dim3(MT), 0, st, db, (int)n, dex,
+            hipLaunchKernelGGL(k_mrest_ord, dim3(cdiv(n - 3, MT)), dim3(MT), 0, ax, db, (int)n, dex,
                                key, (int)nkey, dcen, G, s, bvk, bvoff, bvo, nbv, dacc);
         NCG_CHECK_LAUNCH();
-        if (!M.x_targets.empty()) {
-            const int ntg = (int)M.x_targets.size() / 3;
-            int *tg = dbuf<int>(ctx, "g_xtg", M.x_targets.size());
-            int *pl = dbuf<int>(ctx, "g_xpl", std::max<size_t>(M.x_partners.size(), 1));
-            if (!tg || !pl) return NCG_ERR_NOMEM;
-            NCG_HIP(hipMemcpyAsync(tg, M.x_targets.data(), sizeof(int) * M.x_targets.size(),
-                                   hipMemcpyHostToDevice, st));
-            if (!M.x_partners.empty())
-                NCG_HIP(hipMemcpyAsync(pl, M.x_partners.data(), sizeof(int) * M.x_partners.size(),
-                                       hipMemcpyHostToDevice, st));
-            hipLaunchKernelGGL(k_mexplicit, dim3(cdiv(ntg, MT)), dim3(MT), 0, st, db, dex, dcen, G, s,
+        // Survivors and fragments of full fragmentations: host-listed order.
+        if (ntg) {
+            hipLaunchKernelGGL(k_mexplicit, dim3(cdiv(ntg, MT)), dim3(MT), 0, ax, db, dex, dcen, G, s,
                                (const int *)tg, ntg, (const int *)pl, dacc);
             NCG_CHECK_LAUNCH();
         }
+        NCG_HIP(hipEventRecord(ctx->ev_join, ax));
         NCG_HIP(hipStreamWaitEvent(st, ctx->ev_join, 0));
     } else {
         ProfScope ps(ctx, st, "mesh_gravity");
@@ -2290,9 +2294,10 @@ static ncg_status mesh_gravity_core(ncg_ctx *ctx, const DBody *db, int32_t n, co
                            (int)nkey, dcen,
                            G, s, dacc, dct);
         NCG_CHECK_LAUNCH();
-        hipLaunchKernelGGL(k_big_partial, dim3(nbp), dim3(MT), 0, st, db, (int)n, dex, G, s, bpart);
+        hipLaunchKernelGGL(k_big_partial, dim3(nbp), dim3(MT), 0, st, db, (int)n, dex, G, s,
+                           (const double *)dct, dcen, bpart);
         hipLaunchKernelGGL(k_big_final, dim3(1), dim3(MT), 0, st, db, (int)n, dex, bpart, nbp, G, s,
-                           dacc, dct, dcen, dcm);
+                           dacc, dcen, dcm);
         if (!M.extra_pairs.empty())
             hipLaunchKernelGGL(k_extra_pairs, dim3(1), dim3(1), 0, st, db, dex, dextra,
                                (int)M.extra_pairs.size(), G, s, dacc);
