@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define NCG_ABI_VERSION 2
+#define NCG_ABI_VERSION 3
 
 typedef struct ncg_ctx ncg_ctx;
 
@@ -442,6 +442,31 @@ ncg_status ncg_builtin_replay_ops(ncg_builtin_collider *state, ncg_replay_ops *o
 ncg_status ncg_energy_dev(ncg_ctx *ctx, const ncg_body *d_bodies, int32_t n,
                           const int32_t *d_exists, const double *d_central, double grav_const,
                           double softening, double *d_out, void *stream);
+
+/* ---------------------------------------------------------------------------
+ * Output path: what display() (display.c:239-451) writes for every body.
+ *
+ * ncg_format_fixed: the text fprintf(file, "%.<decimals>lf ", v) writes for
+ * each of `count` values (display.c:323-351 use 14 decimals), concatenated
+ * without terminator into `text`: the exact binary value rounded to nearest
+ * with ties to even, a '-' for every negative sign bit (-0.0 included),
+ * "nan" / "inf" for non-finite values, one space after each value, as the
+ * C library prints them.  decimals 0..17, count < 2^30.  *length receives the
+ * byte count; NCG_ERR_CAPACITY when it exceeds `capacity` (nothing written).
+ */
+ncg_status ncg_format_fixed(ncg_ctx *ctx, const double *values, int64_t count, int32_t decimals,
+                            char *text, int64_t capacity, int64_t *length);
+
+/* cart2ell (structure.c:172-284) of every existing body k of the n slots
+ * with mu + G m_k (display.c:340: mu = G (central mass [+ inner fluid disk
+ * mass])), relative to `central` = {x, y, z, vx, vy, vz} of CM_buffer (NULL:
+ * the origin at rest, central_mass_bool 0): alkhqp[6k .. 6k+5] = a, lambda,
+ * k, h, q, p, bit-identical to the reference (J2 correction not applied).
+ * warn (n entries, or NULL): 1 where the reference prints its (k, h)
+ * consistency warning (structure.c:245-247).  Absent slots are untouched. */
+ncg_status ncg_cart2ell(ncg_ctx *ctx, const ncg_body *bodies, int32_t n, const int32_t *exists,
+                        const double *central, double mu, double grav_const, double *alkhqp,
+                        int32_t *warn);
 
 /* ---------------------------------------------------------------------------
  * Launch shape of the tiled gravity forms (NCG_F64_FAST, NCG_MIXED): bodies

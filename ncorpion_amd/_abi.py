@@ -124,6 +124,8 @@ _SIGNATURES = {
     "ncg_rk4_forward_ckpt_dev": (S, [P, P, I32, P, D, D, D, I32, I32, P, P, P]),
     "ncg_rk4_adjoint_ckpt_dev": (S, [P, P, I32, P, D, D, D, I32, I32, P, P, P, P, P, P]),
     "ncg_builtin_replay_ops": (S, [ctypes.POINTER(BuiltinCollider), ctypes.POINTER(ReplayOps)]),
+    "ncg_format_fixed": (S, [P, P, I64, I32, P, I64, ctypes.POINTER(I64)]),
+    "ncg_cart2ell": (S, [P, P, I32, P, P, D, D, P, P]),
     "ncg_profile_enable": (S, [P, S]),
     "ncg_profile_read": (S, [P, ctypes.c_char_p, ctypes.POINTER(D), ctypes.POINTER(I64), S]),
 }
@@ -541,6 +543,35 @@ class Context:
                                               float(G), float(softening), PRECISIONS[precision],
                                               _ptr(out), _ptr(cm)))
         return out, cm
+
+    # -- output path -------------------------------------------------------------
+    def format_fixed(self, values, decimals=14):
+        """Text of printf("%.<decimals>f ") for each value, concatenated (bytes)."""
+        v = np.ascontiguousarray(values, dtype=np.float64).ravel()
+        cap = max(64, 24 * v.size)
+        while True:
+            out = ctypes.create_string_buffer(cap)
+            ln = I64()
+            st = self.lib.ncg_format_fixed(self.h, _ptr(v), v.size, int(decimals), out, cap,
+                                           ctypes.byref(ln))
+            if st == NCG_ERR_CAPACITY:
+                cap = int(ln.value)
+                continue
+            self._check(st)
+            return out.raw[: ln.value]
+
+    def cart2ell(self, bodies, exists=None, central=None, mu=1.0, G=1.0):
+        """(alkhqp (n, 6), warn (n,)) of cart2ell for every existing body;
+        central: (x, y, z, vx, vy, vz) or None."""
+        b = as_bodies(bodies)
+        n = b.shape[0]
+        e = as_exists(exists, n)
+        c = None if central is None else np.ascontiguousarray(central, dtype=np.float64)
+        out = np.zeros((n, 6))
+        warn = np.zeros(n, np.int32)
+        self._check(self.lib.ncg_cart2ell(self.h, _ptr(b), n, _ptr(e), _ptr(c), float(mu), float(G),
+                                          _ptr(out), _ptr(warn)))
+        return out, warn
 
     def mesh_pairs(self):
         cap = 1 << 20

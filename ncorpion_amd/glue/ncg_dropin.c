@@ -262,3 +262,235 @@ void ncg_dropin_get_neighbours_mesh(struct moonlet *X)
     abort();
 #endif
 }
+
+/* ---- display() (display.c:239-451) -------------------------------------------
+ * The per-body columns (x ... vz, a ... p, radius, mass) are formatted on the
+ * GPU (ncg_format_fixed: the bytes fprintf's "%.14lf " writes) and their
+ * elliptic elements computed there (ncg_cart2ell, cart2ell's arithmetic; the
+ * bodies it warns about go through the reference's cart2ell again so that
+ * the warning is printed as it would be).  The central body's values, the
+ * statistics line and the collision lines are few and written here with the
+ * reference's own format strings. */
+#if write_to_files_bool
+#include "display.h"
+
+static char *g_text = NULL;
+static int64_t g_text_cap = 0;
+static double *g_cols = NULL;
+static size_t g_cols_cap = 0;
+
+static void write_column(FILE *f, const double *v, int64_t count)
+{
+    if (count == 0) return;
+    ncg_ctx *c = ctx();
+    int64_t len = 0;
+    ncg_status st = ncg_format_fixed(c, v, count, 14, g_text, g_text_cap, &len);
+    if (st == NCG_ERR_CAPACITY) {
+        free(g_text);
+        g_text_cap = len + 4096;
+        g_text = (char *)malloc((size_t)g_text_cap);
+        if (!g_text) {
+            fprintf(stderr, "Error : ncg drop-in: cannot allocate the output text buffer.\n");
+            abort();
+        }
+        st = ncg_format_fixed(c, v, count, 14, g_text, g_text_cap, &len);
+    }
+    if (st != NCG_OK) ncg_die("display", st);
+    fwrite(g_text, 1, (size_t)len, f);
+}
+#endif
+
+void ncg_dropin_display(struct moonlet *moonlets)
+{
+#if write_to_files_bool
+    FILE *filex = NULL, *filey = NULL, *filez = NULL, *filevx = NULL, *filevy = NULL, *filevz = NULL;
+    FILE *filer = NULL, *filem = NULL, *filea = NULL, *filel = NULL, *filek = NULL, *fileh = NULL;
+    FILE *fileq = NULL, *filep = NULL, *filestat, *filecol = NULL;
+    const int collision = collision_bool && write_collisions_bool ? 1 : 0;
+    filestat = *files;
+    if (collision) filecol = *(files + 1);
+    if (write_cartesian_bool) {
+        filem = files[1 + collision];
+        filer = files[2 + collision];
+        filex = files[3 + collision];
+        filey = files[4 + collision];
+        filez = files[5 + collision];
+        filevx = files[6 + collision];
+        filevy = files[7 + collision];
+        filevz = files[8 + collision];
+        if (write_elliptic_bool) {
+            filea = files[9 + collision];
+            filel = files[10 + collision];
+            filek = files[11 + collision];
+            fileh = files[12 + collision];
+            fileq = files[13 + collision];
+            filep = files[14 + collision];
+        }
+    } else if (write_elliptic_bool) {
+        filem = files[1 + collision];
+        filer = files[2 + collision];
+        filea = files[3 + collision];
+        filel = files[4 + collision];
+        filek = files[5 + collision];
+        fileh = files[6 + collision];
+        fileq = files[7 + collision];
+        filep = files[8 + collision];
+    }
+    typ alkhqp[6];
+    typ total_mass = 0.0, mu;
+    typ M1 = 0., M2 = 0., R1, R2, a1, a2, l1, l2, K1, K2, H1, H2, Q1, Q2, P1, P2;
+    int i1 = 0, i2 = 0, p;
+    mu = central_mass_bool ? CM_buffer.mass : M_unit;
+    mu *= G;
+    if (central_mass_bool && write_cartesian_bool) {
+        fprintf(filex, "%.14lf ", CM_buffer.x);
+        fprintf(filey, "%.14lf ", CM_buffer.y);
+        fprintf(filez, "%.14lf ", CM_buffer.z);
+        fprintf(filevx, "%.14lf ", CM_buffer.vx);
+        fprintf(filevy, "%.14lf ", CM_buffer.vy);
+        fprintf(filevz, "%.14lf ", CM_buffer.vz);
+    }
+    const int n = largest_id + 1;
+    int cnt = 0;
+    for (p = 0; p < n; p++) cnt += exists[p] != 0;
+    /* Columns of the existing bodies: 6 cartesian, 6 elliptic, radius, mass. */
+    if ((size_t)cnt * 14 > g_cols_cap) {
+        free(g_cols);
+        g_cols_cap = (size_t)cnt * 14 + 1024;
+        g_cols = (double *)malloc(sizeof(double) * g_cols_cap);
+        if (!g_cols) {
+            fprintf(stderr, "Error : ncg drop-in: cannot allocate the output columns.\n");
+            abort();
+        }
+    }
+    double *col[14];
+    for (int c = 0; c < 14; c++) col[c] = g_cols + (size_t)c * (size_t)cnt;
+    int q = 0;
+    for (p = 0; p < n; p++) {
+        if (!exists[p]) continue;
+        const struct moonlet *b = moonlets + p;
+        col[0][q] = b->x;
+        col[1][q] = b->y;
+        col[2][q] = b->z;
+        col[3][q] = b->vx;
+        col[4][q] = b->vy;
+        col[5][q] = b->vz;
+        col[12][q] = b->radius;
+        col[13][q] = b->mass;
+        const typ m = b->mass;
+        total_mass += m;
+        if (m > M1) {
+            M2 = M1;
+            i2 = i1;
+            M1 = m;
+            i1 = p;
+        } else if (m > M2) {
+            M2 = m;
+            i2 = p;
+        }
+        q++;
+    }
+    if (write_cartesian_bool) {
+        write_column(filex, col[0], cnt);
+        write_column(filey, col[1], cnt);
+        write_column(filez, col[2], cnt);
+        write_column(filevx, col[3], cnt);
+        write_column(filevy, col[4], cnt);
+        write_column(filevz, col[5], cnt);
+    }
+    if (write_elliptic_bool && n > 0) {
+        double *el = (double *)malloc(sizeof(double) * 6 * (size_t)n);
+        int32_t *warn = (int32_t *)malloc(sizeof(int32_t) * (size_t)n);
+        if (!el || !warn) {
+            fprintf(stderr, "Error : ncg drop-in: cannot allocate the elliptic elements.\n");
+            abort();
+        }
+        const double cm6[6] = {CM_buffer.x, CM_buffer.y, CM_buffer.z,
+                               CM_buffer.vx, CM_buffer.vy, CM_buffer.vz};
+        ncg_status st = ncg_cart2ell(ctx(), (const ncg_body *)moonlets, n, exists,
+                                     central_mass_bool ? cm6 : NULL, mu, G, el, warn);
+        if (st != NCG_OK) ncg_die("display", st);
+        q = 0;
+        for (p = 0; p < n; p++) {
+            if (!exists[p]) continue;
+            if (warn[p]) cart2ell(moonlets, p, alkhqp, mu + G * moonlets[p].mass);
+            for (int c = 0; c < 6; c++) col[6 + c][q] = el[6 * (size_t)p + c];
+            q++;
+        }
+        free(el);
+        free(warn);
+        write_column(filea, col[6], cnt);
+        write_column(filel, col[7], cnt);
+        write_column(filek, col[8], cnt);
+        write_column(fileh, col[9], cnt);
+        write_column(fileq, col[10], cnt);
+        write_column(filep, col[11], cnt);
+    }
+    if (write_cartesian_bool || write_elliptic_bool) {
+        write_column(filer, col[12], cnt);
+        write_column(filem, col[13], cnt);
+    }
+    /* Statistics regarding the two most massive bodies (display.c:373-396). */
+    if (how_many_moonlets == 0) {
+        M1 = 0.; M2 = 0.; R1 = 0.; R2 = 0.; a1 = 0.; a2 = 0.; l1 = 0.; l2 = 0.;
+        K1 = 0.; K2 = 0.; H1 = 0.; H2 = 0.; Q1 = 0.; Q2 = 0.; P1 = 0.; P2 = 0.;
+    } else if (how_many_moonlets == 1) {
+        M2 = 0.; R2 = 0.; a2 = 0.; l2 = 0.; K2 = 0.; H2 = 0.; Q2 = 0.; P2 = 0.;
+        R1 = moonlets[i1].radius;
+        cart2ell(moonlets, i1, alkhqp, mu + G * M1);
+        a1 = alkhqp[0]; l1 = alkhqp[1]; K1 = alkhqp[2]; H1 = alkhqp[3]; Q1 = alkhqp[4]; P1 = alkhqp[5];
+    } else {
+        R1 = moonlets[i1].radius;
+        R2 = moonlets[i2].radius;
+        cart2ell(moonlets, i1, alkhqp, mu + G * M1);
+        a1 = alkhqp[0]; l1 = alkhqp[1]; K1 = alkhqp[2]; H1 = alkhqp[3]; Q1 = alkhqp[4]; P1 = alkhqp[5];
+        cart2ell(moonlets, i2, alkhqp, mu + G * M2);
+        a2 = alkhqp[0]; l2 = alkhqp[1]; K2 = alkhqp[2]; H2 = alkhqp[3]; Q2 = alkhqp[4]; P2 = alkhqp[5];
+    }
+    /* The statistics line (display.c:398-414). */
+    fprintf(filestat, "%.14lf %d %d", time_elapsed + t_init, how_many_moonlets, collision_count);
+    if (central_mass_bool) fprintf(filestat, " %.14lf", CM_buffer.mass);
+    else fprintf(filestat, " %.1lf", 0.);
+    fprintf(filestat, " %.14lf", total_mass);
+    fprintf(filestat, " %.1lf %.1lf", 0., 0.);
+    if (central_tides_bool) fprintf(filestat, " %.14lf", SideralOmega);
+    else fprintf(filestat, " %.1lf", 0.);
+    fprintf(filestat, " %.14lf %.14lf %.14lf %.14lf %.14lf %.14lf %.14lf %.14lf %.14lf %.14lf %.14lf "
+                      "%.14lf %.14lf %.14lf %.14lf %.14lf",
+            M1, M2, R1, R2, a1, a2, l1, l2, K1, K2, H1, H2, Q1, Q2, P1, P2);
+    fprintf(filestat, "\n");
+    if (write_cartesian_bool) {
+        fprintf(filex, "\n");
+        fprintf(filey, "\n");
+        fprintf(filez, "\n");
+        fprintf(filevx, "\n");
+        fprintf(filevy, "\n");
+        fprintf(filevz, "\n");
+    }
+    if (write_elliptic_bool) {
+        fprintf(filea, "\n");
+        fprintf(filel, "\n");
+        fprintf(filek, "\n");
+        fprintf(fileh, "\n");
+        fprintf(fileq, "\n");
+        fprintf(filep, "\n");
+    }
+    if (write_cartesian_bool || write_elliptic_bool) {
+        fprintf(filer, "\n");
+        fprintf(filem, "\n");
+    }
+    /* Collision data (display.c:433-447). */
+    if (collision) {
+        for (p = 0; p < indexCollision; p++) {
+            const struct collisionData *cd = collisionDatas + p;
+            fprintf(filecol, "%.14lf %.14lf %.14lf %.14lf %.14lf %.14lf %.14lf %.14lf\n", cd->time,
+                    cd->m1, cd->m2, cd->R1, cd->R2, cd->DeltaV, cd->impact_angle, cd->m_tilde);
+        }
+        indexCollision = 0;
+    }
+#else
+    (void)moonlets;
+    fprintf(stderr, "Error : ncg drop-in: display called without write_to_files_bool.\n");
+    abort();
+#endif
+}

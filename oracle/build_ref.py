@@ -165,6 +165,19 @@ CONFIGS = {
         "how_many_neighbours": "100.0", "switch_to_brute_force": "110",
         "sma_min": "0.9", "sma_max": "1.1", "inclination_max": "0.01",
         "inelastic_collision_bool": "0", "instant_merger_bool": "1"})),
+    # Output files (display(), every 4th step: cartesian and elliptic
+    # elements, radii, masses, statistics, collision data) of a brute-force
+    # run with the fragmentation model, and of a mesh run.
+    "run_c1_out": ("run", _run_cfg("brute", 2000, 40, DT1, {
+        "inelastic_collision_bool": "0", "fragmentation_bool": "1", "N_max": "4000",
+        "write_to_files_bool": "1", "write_cartesian_bool": "1", "write_elliptic_bool": "1",
+        "write_collisions_bool": "1", "output_step": "4"})),
+    "run_mesh_out": ("run", _run_cfg("mesh", 16384, 10, DT2, {
+        "collision_cube_cells": "256", "collision_cube_min": "2.56",
+        "how_many_neighbours": "100.0", "switch_to_brute_force": "110",
+        "sma_min": "0.9", "sma_max": "1.1", "inclination_max": "0.01",
+        "write_to_files_bool": "1", "write_cartesian_bool": "1", "write_elliptic_bool": "1",
+        "output_step": "2"})),
     # Full-size mesh disk of 2^20 bodies (CPU baseline of the mesh module).
     "run_c2": ("run", _run_cfg("mesh", 1048576, 3, DT2, {
         "collision_cube_cells": "256", "collision_cube_min": "2.56",
@@ -220,7 +233,8 @@ def build(name):
         return
     main_o = os.path.join(scratch, "main.o")
     run([CC] + CFLAGS + inc + ["-c", os.path.join(REF_SRC, "main.c"), "-o", main_o])
-    wraps = "-Wl,--wrap=vector_field,--wrap=brute_force,--wrap=mesh,--wrap=get_neighbours_mesh"
+    wraps = ("-Wl,--wrap=vector_field,--wrap=brute_force,--wrap=mesh,--wrap=get_neighbours_mesh,"
+             "--wrap=display")
     trace_wraps = wraps + ",--wrap=collision_treatment,--wrap=three_largest_moonlets"
     run([CC] + CFLAGS + inc + [os.path.join(HERE, "ref_trace.c"), main_o] + objs +
         [trace_wraps, "-lm", "-o", os.path.join(out, "ncorpion_trace")])

@@ -157,6 +157,18 @@ int main(int argc, char **argv)
             }
         break;
     }
+    case 6: {
+        /* cart2ell (structure.c:172) of every existing body with mu = prm[3] + G m_k
+           (display.c:340), relative to CM; the six elements replace x .. vz. */
+        CM_buffer = CM;
+        double *el = (double *)malloc(sizeof(double) * 6 * (size_t)(n > 0 ? n : 1));
+        for (int k = 0; k < n; k++)
+            if (exists[k]) cart2ell(bodies, k, el + 6 * (size_t)k, prm[3] + G * bodies[k].mass);
+        for (int k = 0; k < n; k++)
+            if (exists[k]) memcpy(&bodies[k].x, el + 6 * (size_t)k, 6 * sizeof(double));
+        free(el);
+        break;
+    }
     default: return 2;
     }
     clock_gettime(CLOCK_MONOTONIC, &t1);

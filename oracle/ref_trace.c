@@ -42,6 +42,8 @@ void ncg_dropin_get_neighbours_mesh(struct moonlet *X);
 #define T_BF ncg_dropin_brute_force
 #define T_MESH ncg_dropin_mesh
 #define T_GNM ncg_dropin_get_neighbours_mesh
+void ncg_dropin_display(struct moonlet *X);
+#define T_DISP ncg_dropin_display
 #else
 void __real_vector_field(struct moonlet *X);
 void __real_brute_force(struct moonlet *X);
@@ -51,13 +53,15 @@ void __real_get_neighbours_mesh(struct moonlet *X);
 #define T_BF __real_brute_force
 #define T_MESH __real_mesh
 #define T_GNM __real_get_neighbours_mesh
+void __real_display(struct moonlet *X);
+#define T_DISP __real_display
 #endif
 void __real_collision_treatment(struct moonlet *moonlets, int a, int b, int type_of_collision);
 
 static FILE *trace_fp = NULL;
 static long call_no = 0;
-static double t_acc[4] = {0, 0, 0, 0};
-static long n_acc[4] = {0, 0, 0, 0};
+static double t_acc[5] = {0, 0, 0, 0, 0};
+static long n_acc[5] = {0, 0, 0, 0, 0};
 
 static double now_s(void)
 {
@@ -69,8 +73,9 @@ static double now_s(void)
 static void trace_close(void)
 {
     if (!trace_fp) return;
-    static const char *nm[4] = {"vector_field", "brute_force", "mesh", "get_neighbours_mesh"};
-    for (int q = 0; q < 4; q++)
+    static const char *nm[5] = {"vector_field", "brute_force", "mesh", "get_neighbours_mesh",
+                                "display"};
+    for (int q = 0; q < 5; q++)
         if (n_acc[q]) fprintf(trace_fp, "TIME %s calls=%ld total_s=%.6f\n", nm[q], n_acc[q], t_acc[q]);
     fclose(trace_fp);
     trace_fp = NULL;
@@ -194,3 +199,14 @@ static void collide_call(int which, void (*fn)(struct moonlet *), const char *ta
 void __wrap_brute_force(struct moonlet *X) { collide_call(1, T_BF, "BF", X); }
 void __wrap_mesh(struct moonlet *X) { collide_call(2, T_MESH, "MESH", X); }
 void __wrap_get_neighbours_mesh(struct moonlet *X) { collide_call(3, T_GNM, "GNM", X); }
+
+/* display() (display.c:239): the output files are compared as files. */
+void __wrap_display(struct moonlet *X)
+{
+    call_no++;
+    fprintf(tf(), "DISP %ld in %016llx\n", call_no, (unsigned long long)state_hash(X));
+    double t0 = now_s();
+    T_DISP(X);
+    t_acc[4] += now_s() - t0;
+    n_acc[4]++;
+}

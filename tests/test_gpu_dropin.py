@@ -96,3 +96,35 @@ def test_mesh_fragmentation_and_merger_runs_identical(tmp_path, cfg, mass, radiu
     assert len(coll) > 100
     if cfg == "run_mesh_frag":
         assert any(int(c[2]) >= 16384 or int(c[3]) >= 16384 for c in coll)  # fragments collide
+
+
+@pytest.mark.parametrize("cfg,n,seed,mass,radius,steps", [
+    ("run_c1_out", 2000, 1, 1e-8, 4e-3, 40), ("run_mesh_out", 16384, 2, 1e-10, 1e-3, 10)])
+def test_output_files_identical(tmp_path, cfg, n, seed, mass, radius, steps):
+    # display() with cartesian and elliptic elements, radii, masses,
+    # statistics and collision data: the reference and the drop-in write
+    # every output file byte for byte alike (brute force with the
+    # fragmentation model, output every 4th step; mesh, every 2nd step).
+    if not os.path.exists(os.path.join(REF, cfg, "ncorpion_dropin_trace")):
+        pytest.skip("reference builds missing (oracle/build_ref.py)")
+    b = ic.disk(n, seed=seed, mass=mass, radius=radius)
+    dirs = {}
+    traces = {}
+    for kind, exe in (("ref", "ncorpion_trace"), ("gpu", "ncorpion_dropin_trace")):
+        d = tmp_path / kind
+        d.mkdir()
+        ic.write_init_txt(str(d / "init.txt"), b)
+        traces[kind], out = run(os.path.join(REF, cfg, exe), str(d))
+        assert "total number of timestep performed : %d" % steps in out
+        dirs[kind] = d
+    assert traces["ref"] == traces["gpu"]
+    assert sum(1 for l in traces["ref"] if l.startswith("DISP")) > 5
+    names = sorted(f for f in os.listdir(dirs["ref"]) if f.endswith(".txt") and f != "init.txt")
+    assert sorted(f for f in os.listdir(dirs["gpu"]) if f.endswith(".txt") and f != "init.txt") == names
+    assert {"x.txt", "lambda.txt", "stat.txt", "mass.txt"} <= set(names)
+    for f in names:
+        a = (dirs["ref"] / f).read_bytes()
+        g = (dirs["gpu"] / f).read_bytes()
+        assert a == g, f
+    if cfg == "run_c1_out":
+        assert len((dirs["ref"] / "collision.txt").read_bytes()) > 1000
