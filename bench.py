@@ -266,6 +266,71 @@ def cpu_mesh_baseline(ctx=None, timeout=300):
 C1_N, C1_STEPS = 2000, 200
 
 
+OUT_COLUMNS = ("x", "y", "z", "vx", "vy", "vz", "a", "lambda", "k", "h", "q", "p", "radius", "mass")
+
+
+def output_run(ctx, passes=3):
+    """Output path of display() on the 2^20-body disk (central mass 1 at
+    rest at the origin, G = 1): elliptic elements (ncg_cart2ell) and the
+    "%.14lf " text of the 14 per-body columns (ncg_format_fixed), written to
+    files in a temporary directory.  Seconds per display, end to end."""
+    import numpy as np
+    b = mesh_workload()
+    n = b.shape[0]
+    cen = np.zeros(6)
+    ctx.cart2ell(b[:1024], None, central=cen, mu=1.0, G=1.0)  # warm-up
+    ctx.format_fixed(b[:1024, 0])
+    t_lib = t_all = 0.0
+    nbytes = 0
+    with tempfile.TemporaryDirectory() as td:
+        for _ in range(passes):
+            t0 = time.perf_counter()
+            el, _ = ctx.cart2ell(b, None, central=cen, mu=1.0, G=1.0)
+            cols = [b[:, 0], b[:, 1], b[:, 2], b[:, 3], b[:, 4], b[:, 5], el[:, 0], el[:, 1],
+                    el[:, 2], el[:, 3], el[:, 4], el[:, 5], b[:, 7], b[:, 6]]
+            texts = [ctx.format_fixed(c) for c in cols]
+            t1 = time.perf_counter()
+            for name, t in zip(OUT_COLUMNS, texts):
+                with open(os.path.join(td, name + ".txt"), "ab") as f:
+                    f.write(t)
+                    f.write(b"\n")
+            t2 = time.perf_counter()
+            t_lib += t1 - t0
+            t_all += t2 - t0
+            nbytes = sum(len(t) for t in texts)
+    return {"n": n, "seconds_per_display": t_all / passes, "library_seconds": t_lib / passes,
+            "text_bytes": nbytes, "columns": len(OUT_COLUMNS)}
+
+
+def cpu_output_baseline(timeout=300):
+    """NcorpiON's display() (gcc -O2, 1 thread) on the same bodies, through
+    the function-level probe (op 7: display then file_closing, files in a
+    temporary directory)."""
+    import numpy as np
+    probe = os.path.join(ROOT, "oracle", "_ref", "probe_out", "ref_probe")
+    if not os.path.exists(probe):
+        return None
+    b = mesh_workload()
+    n = b.shape[0]
+    with tempfile.TemporaryDirectory() as td:
+        fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        with open(fin, "wb") as f:
+            f.write(struct.pack("<4i", 0x4E434750, 7 | 0x100, n, 0))
+            f.write(np.zeros(8).tobytes())
+            f.write(np.array([0, 0, 0, 0, 0, 0, 1.0, 0.05]).tobytes())
+            f.write(b.tobytes())
+            f.write(np.ones(n, np.int32).tobytes())
+            f.write(np.zeros(n, np.int32).tobytes())
+        r = subprocess.run([probe, fin, fout], cwd=td, stdout=subprocess.PIPE,
+                           stderr=subprocess.PIPE, text=True, timeout=timeout)
+        if r.returncode != 0:
+            return None
+        secs = float(r.stderr.strip().split("op_seconds=")[-1].split()[0])
+    return {"seconds_per_display": secs, "cores": 1, "kind": "reference",
+            "sample": "NcorpiON display() of the %d-body disk: cartesian and elliptic elements, "
+                      "radii, masses (one call, files in a temporary directory)" % n}
+
+
 def leapfrog_run(ctx, torch):
     """NcorpiON's brute-force leapfrog on the device (ncg_leapfrog_dev): disk of
     2000 bodies, m = 1e-8, R = 2e-4, dt = 2 pi / 256, 200 steps after the half
@@ -466,6 +531,7 @@ def main():
     ap.add_argument("--no-mesh", action="store_true")
     ap.add_argument("--no-collisions", action="store_true")
     ap.add_argument("--no-leapfrog", action="store_true")
+    ap.add_argument("--no-output", action="store_true")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run a plain device copy of this size (HBM counter calibration)")
     args = ap.parse_args()
@@ -585,6 +651,8 @@ def main():
     if not args.no_mesh:
         out["mesh_module"] = mesh_run(ctx, torch)
         out["mesh_loop"] = mesh_loop_run(ctx, torch)
+    if not args.no_output:
+        out["output"] = output_run(ctx)
     if not args.no_ensemble:
         out["ensemble"] = ensemble_run(ctx, torch)
     if not args.no_adjoint:
@@ -605,6 +673,8 @@ def main():
             out["cpu_baseline"]["mesh"] = cpu_mesh_baseline(ctx)
         if out["cpu_baseline"] is not None and not args.no_leapfrog:
             out["cpu_baseline"]["c1_run"] = cpu_c1_baseline()
+        if out["cpu_baseline"] is not None and not args.no_output:
+            out["cpu_baseline"]["output"] = cpu_output_baseline()
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out))

@@ -133,6 +133,11 @@ CONFIGS = {
         "elastic_collision_bool": "0", "collision_parameter": "1.5",
         "collision_cube_cells": "256", "collision_cube_min": "2.56",
         "how_many_neighbours": "100.0", "N_max": "1048600"}))),
+    # display() of the 2^20-body disk (CPU baseline of the output path):
+    # cartesian and elliptic elements, radii and masses.
+    "probe_out": ("probe", _brute(dict(PROBE_COMMON, **{
+        "G": "1.", "write_to_files_bool": "1", "write_cartesian_bool": "1",
+        "write_elliptic_bool": "1", "write_collisions_bool": "0", "N_max": "1048600"}))),
     # --- whole runs (drop-in parity) ------------------------------------------
     # Brute force, 2000 bodies, 200 steps of 2 pi/256, inelastic bounces.
     "run_c1": ("run", _run_cfg("brute", 2000, 200, DT1)),

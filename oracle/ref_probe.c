@@ -34,6 +34,7 @@
 #include "structure.h"
 #include "physics.h"
 #include "collision.h"
+#include "display.h"
 
 static int *coll_log = NULL;
 static int n_coll_log = 0, cap_coll_log = 0;
@@ -167,6 +168,20 @@ int main(int argc, char **argv)
         for (int k = 0; k < n; k++)
             if (exists[k]) memcpy(&bodies[k].x, el + 6 * (size_t)k, 6 * sizeof(double));
         free(el);
+        break;
+    }
+    case 7: {
+        /* display() (display.c:239) of the bodies into the output files of
+           the working directory, then file_closing() (its final flush). */
+#if write_to_files_bool
+        file_opening();
+        CM_buffer = CM;
+        clock_gettime(CLOCK_MONOTONIC, &t0);
+        display(bodies);
+        file_closing();
+#else
+        return 2;
+#endif
         break;
     }
     default: return 2;
