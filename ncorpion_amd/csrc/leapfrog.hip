@@ -106,8 +106,52 @@ __device__ inline void t3_insert(Top3 &t, double r, int i)
     }
 }
 
+// Per block of k_top3_part: the three largest (radius descending, index
+// ascending) existing bodies after the first three existing ones, over a
+// grid-stride share of the slots.
+constexpr int T3B = 256;
+constexpr int T3_PARTS = 1024;
+
+__global__ __launch_bounds__(T3B) void k_top3_part(const DBody *__restrict__ b, int n,
+                                                   const int32_t *__restrict__ ex,
+                                                   Top3 *__restrict__ part)
+{
+    __shared__ Top3 sh[T3B];
+    __shared__ int s_start;
+    const int t = threadIdx.x;
+    if (t == 0) {
+        int c = 0, j = 0;
+        for (; j < n && c < 3; ++j)
+            if (!ex || ex[j]) ++c;
+        s_start = c == 3 ? j : n;
+    }
+    __syncthreads();
+    Top3 mine;
+    for (int q = 0; q < 3; ++q) {
+        mine.r[q] = 0.0;
+        mine.i[q] = -1;
+    }
+    for (int j = s_start + blockIdx.x * T3B + t; j < n; j += gridDim.x * T3B)
+        if (!ex || ex[j]) t3_insert(mine, b[j].radius, j);
+    sh[t] = mine;
+    __syncthreads();
+    for (int w = T3B / 2; w > 0; w >>= 1) {
+        if (t < w) {
+            Top3 a = sh[t];
+            const Top3 &o = sh[t + w];
+            for (int q = 0; q < 3; ++q) t3_insert(a, o.r[q], o.i[q]);
+            sh[t] = a;
+        }
+        __syncthreads();
+    }
+    if (t == 0) part[blockIdx.x] = sh[0];
+}
+
+// The first three existing bodies, the partials of k_top3_part merged, then
+// the swaps.  One block.
 __global__ __launch_bounds__(T3) void k_top3_first(DBody *__restrict__ b, int n,
                                                    int32_t *__restrict__ ex,
+                                                   const Top3 *__restrict__ part, int nparts,
                                                    unsigned *__restrict__ err)
 {
     __shared__ Top3 sh[T3];
@@ -129,8 +173,8 @@ __global__ __launch_bounds__(T3) void k_top3_first(DBody *__restrict__ b, int n,
         mine.r[q] = 0.0;
         mine.i[q] = -1;
     }
-    for (int j = s_j[2] + 1 + t; j < n; j += T3)
-        if (!ex || ex[j]) t3_insert(mine, b[j].radius, j);
+    for (int q = t; q < nparts; q += T3)
+        for (int c = 0; c < 3; ++c) t3_insert(mine, part[q].r[c], part[q].i[c]);
     sh[t] = mine;
     __syncthreads();
     for (int w = T3 / 2; w > 0; w >>= 1) {
@@ -208,14 +252,28 @@ __global__ __launch_bounds__(LT) void k_lf_census(const DBody *__restrict__ b, i
                                                   double rhi, int *__restrict__ count,
                                                   unsigned *__restrict__ err)
 {
-    const int i = blockIdx.x * LT + threadIdx.x;
-    if (i >= n || (ex && !ex[i])) return;
-    atomicAdd(count, 1);
+    // Grid-stride share of the slots; one atomic per block for the count.
+    __shared__ int wc[LT / 64];
     const int cp = cen[7] != 0.0;
-    const double DX = b[i].x - (cp ? cen[0] : 0.0), DY = b[i].y - (cp ? cen[1] : 0.0),
-                 DZ = b[i].z - (cp ? cen[2] : 0.0);
-    const double r2 = DX * DX + DY * DY + DZ * DZ;
-    if ((cp && r2 < rlo * rlo) || r2 > rhi * rhi) atomicOr(err, 2u);
+    const double cx = cp ? cen[0] : 0.0, cy = cp ? cen[1] : 0.0, cz = cp ? cen[2] : 0.0;
+    int c = 0;
+    bool out = false;
+    for (int i = blockIdx.x * LT + threadIdx.x; i < n; i += gridDim.x * LT) {
+        if (ex && !ex[i]) continue;
+        ++c;
+        const double DX = b[i].x - cx, DY = b[i].y - cy, DZ = b[i].z - cz;
+        const double r2 = DX * DX + DY * DY + DZ * DZ;
+        out |= (cp && r2 < rlo * rlo) || r2 > rhi * rhi;
+    }
+    if (out) atomicOr(err, 2u);
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o, 64);
+    if ((threadIdx.x & 63) == 0) wc[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int tot = 0;
+        for (int w = 0; w < LT / 64; ++w) tot += wc[w];
+        if (tot) atomicAdd(count, tot);
+    }
 }
 
 __global__ void k_lf_zero_i32_one(int *__restrict__ p) { *p = 0; }
@@ -257,7 +315,8 @@ extern "C" ncg_status ncg_mesh_leapfrog_dev(ncg_ctx *ctx, ncg_body *d_bodies, in
     // counters of each step.
     auto census = [&](int &how_many) -> ncg_status {
         hipLaunchKernelGGL(k_lf_zero_i32_one, dim3(1), dim3(1), 0, st, count);
-        hipLaunchKernelGGL(k_lf_census, dim3(cdiv(n, LT)), dim3(LT), 0, st, b, (int)n, d_exists,
+        hipLaunchKernelGGL(k_lf_census, dim3(std::min<long>(1024, cdiv(n, LT))), dim3(LT), 0, st, b,
+                           (int)n, d_exists,
                            d_central, mp->r_disruption, mp->r_high_dumping, count, err);
         NCG_CHECK_LAUNCH();
         NCG_HIP(hipMemcpyAsync(hpin, count, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -273,8 +332,14 @@ extern "C" ncg_status ncg_mesh_leapfrog_dev(ncg_ctx *ctx, ncg_body *d_bodies, in
                             "end_of_timestep would remove it, which this loop does not do");
         return NCG_OK;
     };
+    const int nparts = (int)std::min<long>(T3_PARTS, cdiv(n, T3B));
+    Top3 *parts = dbuf<Top3>(ctx, "ml_top3", (size_t)nparts);
+    if (!parts) return NCG_ERR_NOMEM;
     auto largest3 = [&]() {
-        hipLaunchKernelGGL(k_top3_first, dim3(1), dim3(T3), 0, st, b, (int)n, d_exists, err);
+        hipLaunchKernelGGL(k_top3_part, dim3(nparts), dim3(T3B), 0, st, (const DBody *)b, (int)n,
+                           (const int32_t *)d_exists, parts);
+        hipLaunchKernelGGL(k_top3_first, dim3(1), dim3(T3), 0, st, b, (int)n, d_exists,
+                           (const Top3 *)parts, nparts, err);
     };
     // Mesh pass at the current gam (rk4.c:697-700, 765-767, 774-777).
     int64_t tn = 0;
