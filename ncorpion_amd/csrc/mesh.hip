@@ -178,7 +178,14 @@ __device__ inline bool approach_hit(double4 pa, double4 va, double4 pb, double4 
 }
 
 // One thread per inserted body (sorted position s): neighbour count (p < k in
-// the 27 cells) and, for mesh(), the collision candidates (k, l, p).
+// the 27 cells) and, for mesh(), the collision candidates (k, l, p): every
+// neighbour pair is tested once.  With at least 3 cells per axis the 27
+// offsets are distinct and the 13 cells l < 13 lie below the own one, so a
+// body tests every member of those cells and the lower ids of its own cell;
+// a pair found from its lower id is recorded from its higher id's side
+// (stencil index 26 - l).  With 2 cells per axis (a cell can appear twice in
+// the stencil, and the reference then gathers its members twice) a body
+// tests the lower ids of all 27 cells.
 template <int MODE>
 __global__ __launch_bounds__(MT) void k_mcount(const unsigned *__restrict__ skey,
                                                const int *__restrict__ sid,
@@ -208,6 +215,21 @@ __global__ __launch_bounds__(MT) void k_mcount(const unsigned *__restrict__ skey
             const unsigned ck = (unsigned)(key + stencil_offset(l, d));
             const unsigned a = cs[ck], e = ce[ck];
             if (a == e) continue;
+            if (MODE == 0 && d >= 3 && l < 13) {
+                // The whole cell, counting the lower ids on the way.
+                for (unsigned q = a; q < e; ++q) {
+                    const int p = sid[q];
+                    const bool lower = p < k;
+                    c += lower;
+                    if (lower ? approach_hit(pr[q], vm[q], pk, vk, dt)
+                              : approach_hit(pk, vk, pr[q], vm[q], dt)) {
+                        const unsigned long long slot = atomicAdd(ccnt, 1ull);
+                        if (slot < cap)
+                            cand[slot] = lower ? make_int4(k, l, p, 0) : make_int4(p, 26 - l, k, 0);
+                    }
+                }
+                continue;
+            }
             unsigned lo = a, hi = e;  // first position with id >= k
             while (lo < hi) {
                 const unsigned mid = (lo + hi) >> 1;
@@ -215,7 +237,7 @@ __global__ __launch_bounds__(MT) void k_mcount(const unsigned *__restrict__ skey
                 else hi = mid;
             }
             c += lo - a;
-            if (MODE == 0) {
+            if (MODE == 0 && (d < 3 || l == 13)) {
                 for (unsigned q = a; q < lo; ++q) {
                     if (approach_hit(pr[q], vm[q], pk, vk, dt)) {
                         const unsigned long long slot = atomicAdd(ccnt, 1ull);
