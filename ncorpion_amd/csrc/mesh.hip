@@ -269,10 +269,11 @@ __global__ __launch_bounds__(MT) void k_mcount(const unsigned *__restrict__ skey
 //     hit become pending re-tests of visit q.  A visitor's own state is its
 //     start-of-pass state (it is still ahead), and a re-test is exact when it
 //     runs, so the pending list only has to be a superset.
-// One wavefront.  stat[0] += collisions; stat[1] |= 1 (more candidates than
+// One block of MR_T threads.  stat[0] += collisions; stat[1] |= 1 (more candidates than
 // cap), 2 (more than MR_PCAP pending re-tests), 4 (a body larger than the
 // mesh size).
 constexpr int MR_PCAP = 8192;
+constexpr int MR_T = 512;  // threads of k_mesh_replay: scans take MR_T entries at a time
 
 __global__ void k_lf_fill_u32(unsigned *__restrict__ p, unsigned n)
 {
@@ -305,7 +306,7 @@ __device__ inline bool mr_before(int4 a, int4 b)  // (q, 26 - l, d) order
     return a.z < b.z;
 }
 
-__global__ __launch_bounds__(64) void k_mesh_replay(
+__global__ __launch_bounds__(MR_T) void k_mesh_replay(
     DBody *__restrict__ b, const int32_t *__restrict__ ex, int32_t *__restrict__ dcf, int one_coll,
     double f, double dt, const unsigned *__restrict__ key, const int *__restrict__ sid,
     const unsigned *__restrict__ cs, const unsigned *__restrict__ ce,
@@ -316,7 +317,8 @@ __global__ __launch_bounds__(64) void k_mesh_replay(
 {
     __shared__ int4 P[MR_PCAP];
     __shared__ int s_np, s_pi, s_ci, s_k, s_hl, s_hp, s_hit;
-    __shared__ int s_q[64], s_l[64];
+    __shared__ int s_q[MR_T], s_l[MR_T];
+    __shared__ unsigned long long s_m[MR_T / 64];  // per-wavefront hit masks of a scan round
     // 27 position ranges [ra[i], ra[i] + rn[i]) walked as one index space,
     // pre[i] = positions before range i, rl[i] = its stencil cell: set 0 for
     // the later-visitor scans, set 1 for the rest of a visit (which calls
@@ -324,7 +326,7 @@ __global__ __launch_bounds__(64) void k_mesh_replay(
     __shared__ unsigned s_ra[2][27], s_rn[2][27], s_pre[2][28];
     __shared__ int s_rl[2][27];
     __shared__ double s_ap[6], s_tc;
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x, wv = lane >> 6;
     if (cnt[3] > 0) {
         if (lane == 0) atomicOr(stat + 1, 4u);
         return;
@@ -367,7 +369,7 @@ __global__ __launch_bounds__(64) void k_mesh_replay(
         return lo;
     };
     // Later visitors of body dd (just changed during visit kc) that may hit
-    // it: the members above kc of its 27 cells, 64 at a time across cells.
+    // it: the members above kc of its 27 cells, MR_T at a time across cells.
     auto eager = [&](int dd, int kc) {
         const unsigned kd = key[dd];
         if (kd == NOKEY) return;
@@ -389,7 +391,7 @@ __global__ __launch_bounds__(64) void k_mesh_replay(
         }
         flatten(0);
         const unsigned total = s_pre[0][27];
-        for (unsigned base = 0; base < total; base += 64) {
+        for (unsigned base = 0; base < total; base += MR_T) {
             const unsigned x = base + lane;
             bool h = false;
             if (x < total) {
@@ -400,11 +402,14 @@ __global__ __launch_bounds__(64) void k_mesh_replay(
                 s_l[lane] = s_rl[0][i];
                 if (!ex || ex[q]) h = approach_hit(pd, vd, pr[u], vm[u], dt);
             }
-            unsigned long long mask = __ballot(h);
+            const unsigned long long wm = __ballot(h);
+            if ((lane & 63) == 0) s_m[wv] = wm;
             __syncthreads();
             if (lane == 0) {
-                while (mask) {
-                    const int bit = __ffsll((long long)mask) - 1;
+                for (int w = 0; w < MR_T / 64; ++w) {
+                  unsigned long long mask = s_m[w];
+                  while (mask) {
+                    const int bit = w * 64 + __ffsll((long long)mask) - 1;
                     mask &= mask - 1;
                     // In visit q, dd sits in cell 26 - l (the stencil is symmetric).
                     const int4 ev = make_int4(s_q[bit], 26 - s_l[bit], dd, 0);
@@ -412,12 +417,13 @@ __global__ __launch_bounds__(64) void k_mesh_replay(
                         atomicOr(stat + 1, 2u);
                         continue;
                     }
-                    int w = s_np++;
-                    while (w > s_pi && mr_before(ev, P[w - 1])) {
-                        P[w] = P[w - 1];
-                        --w;
+                    int at = s_np++;
+                    while (at > s_pi && mr_before(ev, P[at - 1])) {
+                        P[at] = P[at - 1];
+                        --at;
                     }
-                    P[w] = ev;
+                    P[at] = ev;
+                  }
                 }
             }
             __syncthreads();
@@ -484,7 +490,7 @@ __global__ __launch_bounds__(64) void k_mesh_replay(
         if (!s_hit) continue;
         eager(s_hp, k);
         // The rest of visit k in traversal order -- cells s_hl down to 0,
-        // ids below k, after s_hp in cell s_hl -- 64 pairs at a time across
+        // ids below k, after s_hp in cell s_hl -- MR_T pairs at a time across
         // cells; the first hit wins and the walk resumes after it.
         const unsigned kk = key[k];
         const int l0 = s_hl, p0 = s_hp;
@@ -530,11 +536,16 @@ __global__ __launch_bounds__(64) void k_mesh_replay(
                 h = bf_hit(b, ex, dcf, p, k, dt, ap, &tc);
             }
             const unsigned long long hm = __ballot(h);
-            if (hm == 0ull) {
-                base += 64;
+            __syncthreads();  // the masks of the previous round have been read
+            if ((lane & 63) == 0) s_m[wv] = hm;
+            __syncthreads();
+            int first = -1;
+            for (int w = 0; w < MR_T / 64 && first < 0; ++w)
+                if (s_m[w]) first = w * 64 + __ffsll((long long)s_m[w]) - 1;
+            if (first < 0) {
+                base += MR_T;
                 continue;
             }
-            const int first = __ffsll((long long)hm) - 1;
             if (lane == first) {
                 for (int q = 0; q < 6; ++q) s_ap[q] = ap[q];
                 s_tc = tc;
@@ -2122,7 +2133,7 @@ extern "C" ncg_status ncg_mesh_pass_dev(ncg_ctx *ctx, ncg_body *d_bodies, int32_
                            vin);
         NCG_CHECK_LAUNCH();
         NCG_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tb, kin, kout, vin, vout, (int)cap, 0, 64, st));
-        hipLaunchKernelGGL(k_mesh_replay, dim3(1), dim3(64), 0, st, db, d_exists, d_did_collide,
+        hipLaunchKernelGGL(k_mesh_replay, dim3(1), dim3(MR_T), 0, st, db, d_exists, d_did_collide,
                            (int)one_collision_only, collision_f, timestep, B.key, B.sid, B.cs, B.ce,
                            B.pr, B.vm, (int)d, dc, vout, B.tot + 1, cap, B.cnt, stat);
         NCG_CHECK_LAUNCH();
