@@ -216,13 +216,21 @@ __global__ __launch_bounds__(MT) void k_mcount(const unsigned *__restrict__ skey
             const unsigned a = cs[ck], e = ce[ck];
             if (a == e) continue;
             if (MODE == 0 && d >= 3 && l < 13) {
-                // The whole cell, counting the lower ids on the way.
+                // The whole cell, counting the lower ids on the way; the next
+                // member's record is loaded while the current one is tested.
+                int np = sid[a];
+                double4 nr = pr[a], nv = vm[a];
                 for (unsigned q = a; q < e; ++q) {
-                    const int p = sid[q];
+                    const int p = np;
+                    const double4 rq = nr, vq = nv;
+                    if (q + 1 < e) {
+                        np = sid[q + 1];
+                        nr = pr[q + 1];
+                        nv = vm[q + 1];
+                    }
                     const bool lower = p < k;
                     c += lower;
-                    if (lower ? approach_hit(pr[q], vm[q], pk, vk, dt)
-                              : approach_hit(pk, vk, pr[q], vm[q], dt)) {
+                    if (lower ? approach_hit(rq, vq, pk, vk, dt) : approach_hit(pk, vk, rq, vq, dt)) {
                         const unsigned long long slot = atomicAdd(ccnt, 1ull);
                         if (slot < cap)
                             cand[slot] = lower ? make_int4(k, l, p, 0) : make_int4(p, 26 - l, k, 0);
@@ -1038,8 +1046,23 @@ __global__ __launch_bounds__(MT) void k_mord(const DBody *__restrict__ b, int n,
             if (sid[mid] < kmax) lo = mid + 1;
             else hi = mid;
         }
-        for (unsigned u = a; u < lo; ++u)
-            if (sid[u] < k && sx[u]) term_exact(me.x, me.y, me.z, Rk, pm[u], rr[u], G, s, ax, ay, az);
+        if (a == lo) continue;
+        // The next member's record is loaded while the current term is added.
+        int nid = sid[a], nx = sx[a];
+        double4 np = pm[a];
+        double nr = rr[a];
+        for (unsigned u = a; u < lo; ++u) {
+            const int idu = nid, xu = nx;
+            const double4 pu = np;
+            const double ru = nr;
+            if (u + 1 < lo) {
+                nid = sid[u + 1];
+                nx = sx[u + 1];
+                np = pm[u + 1];
+                nr = rr[u + 1];
+            }
+            if (idu < k && xu) term_exact(me.x, me.y, me.z, Rk, pu, ru, G, s, ax, ay, az);
+        }
     }
     // (3) later visits: the cell's id-ordered neighbourhood list after k's own
     // entry, merged with the recorded big visits that list k; the walk starts
@@ -1047,9 +1070,26 @@ __global__ __launch_bounds__(MT) void k_mord(const DBody *__restrict__ b, int n,
     const unsigned t0 = max(cs[(unsigned)kk], (unsigned)(t & ~63));
     const unsigned mine = lw[t];
     unsigned nb = next_big(bvk, bvoff, bvo, nbv, vi, k);
-    for (unsigned j = lw[t0] + 1, e = le[t]; j < e; ++j) {
+    // The list entry two ahead and the record one ahead are loaded while the
+    // current term is added.
+    const unsigned j0 = lw[t0] + 1, e = le[t];
+    unsigned cu = j0 < e ? L[j0] : 0u, nu2 = j0 + 1 < e ? L[j0 + 1] : 0u;
+    int cxs = j0 < e ? sx[cu] : 0;
+    double4 cpm = j0 < e ? pm[cu] : make_double4(0., 0., 0., 0.);
+    double crr = j0 < e ? rr[cu] : 0.;
+    for (unsigned j = j0; j < e; ++j) {
+        const unsigned u = cu;
+        const int xu = cxs;
+        const double4 pu = cpm;
+        const double ru = crr;
+        cu = nu2;
+        if (j + 1 < e) {
+            cxs = sx[cu];
+            cpm = pm[cu];
+            crr = rr[cu];
+        }
+        if (j + 2 < e) nu2 = L[j + 2];
         if (j <= mine) continue;
-        const unsigned u = L[j];
         if (nb != NOKEY) {
             const unsigned id = (unsigned)sid[u];
             while (nb < id) {
@@ -1058,7 +1098,7 @@ __global__ __launch_bounds__(MT) void k_mord(const DBody *__restrict__ b, int n,
                 nb = next_big(bvk, bvoff, bvo, nbv, vi, k);
             }
         }
-        if (sx[u]) term_exact(me.x, me.y, me.z, Rk, pm[u], rr[u], G, s, ax, ay, az);
+        if (xu) term_exact(me.x, me.y, me.z, Rk, pu, ru, G, s, ax, ay, az);
     }
     while (nb != NOKEY) {
         term_id(b, ex, (int)nb, me.x, me.y, me.z, Rk, G, s, ax, ay, az);
