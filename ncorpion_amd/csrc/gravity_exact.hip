@@ -187,6 +187,83 @@ __global__ __launch_bounds__(256) void k_cm_seq(const double *__restrict__ cterm
     }
 }
 
+// Small n: one wavefront per body k.  The lanes form the terms of 64 bodies j
+// at a time (exact_term, as k_grav_exact does), and lanes 0, 1, 2 add the x,
+// y, z terms in j order: the same sums, with 64 lanes forming the terms of one
+// body instead of one lane forming them all.  A term to subtract is stored
+// negated and a skipped one as -0.0, so the sum is a run of additions:
+// a - t and a + (-t) are one operation, and a + (-0.0) is a for every a.
+constexpr int EXW_MAX = 32768;  // bodies up to which the wavefront-per-body form is used
+
+__global__ __launch_bounds__(256) void k_grav_exact_w(const DBody *__restrict__ b, int n,
+                                                      const int32_t *__restrict__ ex, int cp,
+                                                      double cx, double cy, double cz, double mu,
+                                                      const double *__restrict__ dcen,
+                                                      double G, double s,
+                                                      double *__restrict__ acc,
+                                                      double *__restrict__ cterm)
+{
+    __shared__ double st[4][3][64];
+    if (dcen) {
+        cp = dcen[7] != 0.0;
+        cx = dcen[0];
+        cy = dcen[1];
+        cz = dcen[2];
+        mu = G * dcen[6];  // physics.c:131
+    }
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int k = blockIdx.x * 4 + w;
+    if (k >= n || (ex != nullptr && ex[k] == 0)) return;  // the whole wavefront
+    const DBody me = b[k];
+    const double X = me.x, Y = me.y, Z = me.z, Rk = me.radius;
+    double a = 0.;  // component `lane` (lanes 0..2)
+    if (cp) {
+        // physics.c:121-146
+        const double DX = X - cx, DY = Y - cy, DZ = Z - cz;
+        const double rk = sqrt(DX * DX + DY * DY + DZ * DZ);
+        const double r3 = rk * rk * rk;
+        const double aX = -mu * DX / r3, aY = -mu * DY / r3, aZ = -mu * DZ / r3;
+        a = lane == 0 ? aX : lane == 1 ? aY : aZ;
+        if (lane == 0) {
+            cterm[3 * (size_t)k + 0] = G * me.mass * DX / r3;
+            cterm[3 * (size_t)k + 1] = G * me.mass * DY / r3;
+            cterm[3 * (size_t)k + 2] = G * me.mass * DZ / r3;
+        }
+    }
+    for (int base = 0; base < n; base += 64) {
+        const int j = base + lane;
+        double tX = -0.0, tY = -0.0, tZ = -0.0;
+        if (j < n && j != k && (ex == nullptr || ex[j] != 0)) {
+            const DBody q = b[j];
+            const bool lower = j < k;
+            exact_term(X, Y, Z, Rk, q.x, q.y, q.z, q.radius, G * q.mass, s, lower, tX, tY, tZ);
+            if (lower) {
+                tX = -tX;
+                tY = -tY;
+                tZ = -tZ;
+            }
+        }
+        st[w][0][lane] = tX;
+        st[w][1][lane] = tY;
+        st[w][2][lane] = tZ;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        if (lane < 3) {
+            const double *row = st[w][lane];
+            for (int g = 0; g < 64; g += 16) {
+                double v[16];
+#pragma unroll
+                for (int i = 0; i < 16; ++i) v[i] = row[g + i];
+#pragma unroll
+                for (int i = 0; i < 16; ++i) a += v[i];
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+    }
+    if (lane < 3) acc[3 * (size_t)k + lane] = a;
+}
+
 }  // namespace
 
 // Launches the exact kernel; cterm (3n) receives the per-body central
@@ -197,9 +274,14 @@ ncg_status ncg_grav_exact_launch(ncg_ctx *ctx, const DBody *d_b, int32_t n, cons
 {
     const int cp = (c && c->present) ? 1 : 0;
     const double mu = cp ? G * c->mass : 0.0;  // physics.c:131
-    hipLaunchKernelGGL(k_grav_exact, dim3(cdiv(n, 256)), dim3(256), 0, st, d_b, (int)n, d_ex, cp,
-                       cp ? c->x : 0.0, cp ? c->y : 0.0, cp ? c->z : 0.0, mu,
-                       (const double *)nullptr, G, soft, d_acc, d_cterm);
+    if (n <= EXW_MAX)
+        hipLaunchKernelGGL(k_grav_exact_w, dim3(cdiv(n, 4)), dim3(256), 0, st, d_b, (int)n, d_ex, cp,
+                           cp ? c->x : 0.0, cp ? c->y : 0.0, cp ? c->z : 0.0, mu,
+                           (const double *)nullptr, G, soft, d_acc, d_cterm);
+    else
+        hipLaunchKernelGGL(k_grav_exact, dim3(cdiv(n, 256)), dim3(256), 0, st, d_b, (int)n, d_ex, cp,
+                           cp ? c->x : 0.0, cp ? c->y : 0.0, cp ? c->z : 0.0, mu,
+                           (const double *)nullptr, G, soft, d_acc, d_cterm);
     NCG_CHECK_LAUNCH();
     return NCG_OK;
 }
@@ -212,8 +294,12 @@ ncg_status ncg_grav_exact_dev(ncg_ctx *ctx, const DBody *d_b, int32_t n, const i
 {
     {
         ProfScope ps(ctx, st, "exact");
-        hipLaunchKernelGGL(k_grav_exact, dim3(cdiv(n, 256)), dim3(256), 0, st, d_b, (int)n, d_ex,
-                           0, 0.0, 0.0, 0.0, 0.0, d_cen, G, soft, d_acc, d_cterm);
+        if (n <= EXW_MAX)
+            hipLaunchKernelGGL(k_grav_exact_w, dim3(cdiv(n, 4)), dim3(256), 0, st, d_b, (int)n, d_ex,
+                               0, 0.0, 0.0, 0.0, 0.0, d_cen, G, soft, d_acc, d_cterm);
+        else
+            hipLaunchKernelGGL(k_grav_exact, dim3(cdiv(n, 256)), dim3(256), 0, st, d_b, (int)n, d_ex,
+                               0, 0.0, 0.0, 0.0, 0.0, d_cen, G, soft, d_acc, d_cterm);
         NCG_CHECK_LAUNCH();
     }
     if (d_cm_acc) {
