@@ -343,6 +343,13 @@ extern "C" ncg_status ncg_mesh_leapfrog_dev(ncg_ctx *ctx, ncg_body *d_bodies, in
     };
     // Mesh pass at the current gam (rk4.c:697-700, 765-767, 774-777).
     int64_t tn = 0;
+    // With the exact field, each pass starts the field's neighbourhood lists
+    // on the second stream during its candidate search and replay.
+    struct ListsFlag {
+        MeshState &m;
+        ~ListsFlag() { m.want_lists = false; }
+    } lists_flag{ctx->mesh};
+    ctx->mesh.want_lists = prec == NCG_F64_EXACT;
     auto pass = [&](int mode, double h) -> ncg_status {
         largest3();
         NCG_CHECK_LAUNCH();

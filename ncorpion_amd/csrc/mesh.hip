@@ -2104,6 +2104,11 @@ extern "C" ncg_status ncg_mesh_pass(ncg_ctx *ctx, const ncg_body *bodies, int32_
     return NCG_OK;
 }
 
+static ncg_status build_ord_lists(ncg_ctx *ctx, int ins, long d, const unsigned *skey,
+                                  const unsigned *cs, const unsigned *ce, const int *sid,
+                                  hipStream_t st);
+static ncg_status ensure_aux(ncg_ctx *ctx);
+
 extern "C" ncg_status ncg_mesh_pass_dev(ncg_ctx *ctx, ncg_body *d_bodies, int32_t n,
                                         const int32_t *d_exists, int32_t *d_did_collide,
                                         double timestep, double gam, double collision_cube,
@@ -2156,6 +2161,22 @@ extern "C" ncg_status ncg_mesh_pass_dev(ncg_ctx *ctx, ncg_body *d_bodies, int32_
                                B.ce, 0, B.cnt, (int)d, timestep, B.cbi, (int4 *)nullptr, 0ull,
                                B.tot + 1, B.tot);
         NCG_CHECK_LAUNCH();
+    }
+    if (M.want_lists) {
+        // The exact field's lists depend on the binning only: built on the
+        // second stream while the replay (one workgroup) runs here.
+        int hins = 0;
+        NCG_HIP(hipMemcpyAsync(&hins, B.cnt, sizeof(int), hipMemcpyDeviceToHost, st));
+        NCG_HIP(hipStreamSynchronize(st));
+        if (hins > 0) {
+            r = ensure_aux(ctx);
+            if (r != NCG_OK) return r;
+            NCG_HIP(hipEventRecord(ctx->ev_fork, st));
+            NCG_HIP(hipStreamWaitEvent(ctx->aux, ctx->ev_fork, 0));
+            r = build_ord_lists(ctx, hins, d, B.skey, B.cs, B.ce, B.sid, ctx->aux);
+            if (r != NCG_OK) return r;
+            M.ord_lists = true;
+        }
     }
     hipLaunchKernelGGL(k_lf_fill_u32, dim3(1), dim3(64), 0, st, stat, 2u);
     if (mode == 0) {
@@ -2212,6 +2233,49 @@ extern "C" ncg_status ncg_mesh_pass_dev(ncg_ctx *ctx, ncg_body *d_bodies, int32_
                         hs[9], cap, MR_PCAP);
     M.valid = true;
     M.pairs_known = mode == 1;
+    return NCG_OK;
+}
+
+// Id-ordered neighbourhood lists of the binning (k_usize, k_umerge), on
+// stream st: L (27 entries per binned body), lw, le.
+static ncg_status build_ord_lists(ncg_ctx *ctx, int ins, long d, const unsigned *skey,
+                                  const unsigned *cs, const unsigned *ce, const int *sid,
+                                  hipStream_t st)
+{
+    NCG_REQUIRE(27.0 * ins < 4294967295.0, "%d binned bodies: neighbourhood lists too long", ins);
+    const size_t lcap = std::max<size_t>(27 * (size_t)ins, 1);
+    unsigned *L = dbuf<unsigned>(ctx, "g_L", lcap);
+    unsigned *lw = dbuf<unsigned>(ctx, "g_lw", std::max(ins, 1));
+    unsigned *le = dbuf<unsigned>(ctx, "g_le", std::max(ins, 1));
+    unsigned *ukey = dbuf<unsigned>(ctx, "g_ukey", ins);
+    unsigned *ucnt = dbuf<unsigned>(ctx, "g_ucnt", ins);
+    int *nu = dbuf<int>(ctx, "g_nu", 1);
+    unsigned *usz = dbuf<unsigned>(ctx, "g_usz", (size_t)ins + 1);
+    unsigned *uoff = dbuf<unsigned>(ctx, "g_uoff", (size_t)ins + 1);
+    if (!L || !lw || !le || !ukey || !ucnt || !nu || !usz || !uoff) return NCG_ERR_NOMEM;
+    size_t b1 = 0, b2 = 0;
+    NCG_HIP(hipcub::DeviceRunLengthEncode::Encode(nullptr, b1, skey, ukey, ucnt, nu, ins, st));
+    NCG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, b2, usz, uoff, ins + 1, st));
+    void *tmp = ncg_dbuf(ctx, "g_utmp", std::max(b1, b2));
+    if (!tmp) return NCG_ERR_NOMEM;
+    ProfScope pl(ctx, st, "mesh_lists");
+    NCG_HIP(hipcub::DeviceRunLengthEncode::Encode(tmp, b1, skey, ukey, ucnt, nu, ins, st));
+    hipLaunchKernelGGL(k_usize, dim3(cdiv(ins + 1, MT)), dim3(MT), 0, st, ukey, nu, ins, cs, ce,
+                       (int)d, usz);
+    NCG_CHECK_LAUNCH();
+    NCG_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, b2, usz, uoff, ins + 1, st));
+    hipLaunchKernelGGL(k_umerge, dim3(cdiv(ins, MT)), dim3(MT), 0, st, ukey, nu, uoff, cs, ce, sid,
+                       (int)d, L, lw, le);
+    NCG_CHECK_LAUNCH();
+    return NCG_OK;
+}
+
+static ncg_status ensure_aux(ncg_ctx *ctx)
+{
+    if (ctx->aux) return NCG_OK;
+    NCG_HIP(hipStreamCreate(&ctx->aux));
+    NCG_HIP(hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
+    NCG_HIP(hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
     return NCG_OK;
 }
 
@@ -2273,11 +2337,8 @@ static ncg_status mesh_gravity_core(ncg_ctx *ctx, const DBody *db, int32_t n, co
                 NCG_HIP(hipMemcpyAsync(bvo, hbp.data(), sizeof(int) * hbp.size(),
                                        hipMemcpyHostToDevice, st));
         }
-        if (!ctx->aux) {
-            NCG_HIP(hipStreamCreate(&ctx->aux));
-            NCG_HIP(hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
-            NCG_HIP(hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
-        }
+        ncg_status ra = ensure_aux(ctx);
+        if (ra != NCG_OK) return ra;
         // Id-ordered neighbourhood lists, once per pass (every gravity call
         // of the pass shares them).
         const unsigned *skey = dbuf<unsigned>(ctx, "m_skey", nkey);
@@ -2288,26 +2349,8 @@ static ncg_status mesh_gravity_core(ncg_ctx *ctx, const DBody *db, int32_t n, co
         unsigned *le = dbuf<unsigned>(ctx, "g_le", std::max(ins, 1));
         if (!L || !lw || !le) return NCG_ERR_NOMEM;
         if (!M.ord_lists && ins > 0) {
-            unsigned *ukey = dbuf<unsigned>(ctx, "g_ukey", ins);
-            unsigned *ucnt = dbuf<unsigned>(ctx, "g_ucnt", ins);
-            int *nu = dbuf<int>(ctx, "g_nu", 1);
-            unsigned *usz = dbuf<unsigned>(ctx, "g_usz", (size_t)ins + 1);
-            unsigned *uoff = dbuf<unsigned>(ctx, "g_uoff", (size_t)ins + 1);
-            if (!ukey || !ucnt || !nu || !usz || !uoff) return NCG_ERR_NOMEM;
-            size_t b1 = 0, b2 = 0;
-            NCG_HIP(hipcub::DeviceRunLengthEncode::Encode(nullptr, b1, skey, ukey, ucnt, nu, ins, st));
-            NCG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, b2, usz, uoff, ins + 1, st));
-            void *tmp = ncg_dbuf(ctx, "g_utmp", std::max(b1, b2));
-            if (!tmp) return NCG_ERR_NOMEM;
-            ProfScope pl(ctx, st, "mesh_lists");
-            NCG_HIP(hipcub::DeviceRunLengthEncode::Encode(tmp, b1, skey, ukey, ucnt, nu, ins, st));
-            hipLaunchKernelGGL(k_usize, dim3(cdiv(ins + 1, MT)), dim3(MT), 0, st, ukey, nu, ins, cs,
-                               ce, (int)d, usz);
-            NCG_CHECK_LAUNCH();
-            NCG_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, b2, usz, uoff, ins + 1, st));
-            hipLaunchKernelGGL(k_umerge, dim3(cdiv(ins, MT)), dim3(MT), 0, st, ukey, nu, uoff, cs,
-                               ce, sid, (int)d, L, lw, le);
-            NCG_CHECK_LAUNCH();
+            ncg_status rl = build_ord_lists(ctx, ins, d, skey, cs, ce, sid, st);
+            if (rl != NCG_OK) return rl;
             M.ord_lists = true;
         }
         // Terms of the body-order sums; the sums themselves stay on the

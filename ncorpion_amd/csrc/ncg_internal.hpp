@@ -44,6 +44,7 @@ struct MeshState {
     bool valid = false;
     bool ord_lists = false;  // id-ordered neighbourhood lists built for this pass
     bool pairs_known = false;  // the host knows the registered pair list (host replay)
+    bool want_lists = false;   // device passes start the exact field's lists (mesh loop)
     int32_t n = 0;           // body slots of the pass
     int32_t cells = 0;       // collision_cube_cells
     int64_t table_keys = 0;  // allocated (and zeroed) entries of the cell tables
