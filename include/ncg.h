@@ -263,7 +263,8 @@ typedef struct {
  * Needs every body at most the mesh size and no body crossing the disruption
  * or high-dumping thresholds (end_of_timestep's removals are not applied:
  * NCG_ERR_INVALID).  One synchronisation per mesh pass (its counters set the
- * next mesh size).  d_exists may be written (a body moved into an empty slot
+ * next mesh size), two with NCG_F64_EXACT (the binned count sizes the field's
+ * neighbourhood lists, built on a second stream during the pass).  d_exists may be written (a body moved into an empty slot
  * 0..2).  *n_collisions (may be NULL): collisions resolved. */
 ncg_status ncg_mesh_leapfrog_dev(ncg_ctx *ctx, ncg_body *d_bodies, int32_t n, int32_t *d_exists,
                                  int32_t *d_did_collide, double *d_central, double grav_const,
