@@ -1,7 +1,8 @@
 // Device-resident classic RK4 integrator over the tiled gravity forms.
 //
 // NcorpiON itself integrates with a leapfrog (rk4.c:68-127 kick/drift); this
-// driver is the RK4 the measurement plan asks for.  State: the body array
+// driver is a classic fourth-order Runge-Kutta added beside it, on which the
+// pair-interaction throughput and the energy drift are measured.  State: the body array
 // (x, v in the struct moonlet layout) and the central body
 // {x, y, z, vx, vy, vz, M, present} in device memory; the central body moves
 // under CM_acc exactly as kick() applies it (rk4.c:103-105).
