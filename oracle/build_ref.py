@@ -177,6 +177,15 @@ CONFIGS = {
         "inelastic_collision_bool": "0", "fragmentation_bool": "1", "N_max": "4000",
         "write_to_files_bool": "1", "write_cartesian_bool": "1", "write_elliptic_bool": "1",
         "write_collisions_bool": "1", "output_step": "4"})),
+    # Elliptic elements only (no collision file), and cartesian only with the
+    # collision file: the other layouts of the files array.
+    "run_c1_out_ell": ("run", _run_cfg("brute", 2000, 20, DT1, {
+        "write_to_files_bool": "1", "write_cartesian_bool": "0", "write_elliptic_bool": "1",
+        "write_collisions_bool": "0", "output_step": "4"})),
+    "run_c1_out_cart": ("run", _run_cfg("brute", 2000, 20, DT1, {
+        "inelastic_collision_bool": "0", "instant_merger_bool": "1",
+        "write_to_files_bool": "1", "write_cartesian_bool": "1", "write_elliptic_bool": "0",
+        "write_collisions_bool": "1", "output_step": "4"})),
     "run_mesh_out": ("run", _run_cfg("mesh", 16384, 10, DT2, {
         "collision_cube_cells": "256", "collision_cube_min": "2.56",
         "how_many_neighbours": "100.0", "switch_to_brute_force": "110",

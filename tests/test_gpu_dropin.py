@@ -99,7 +99,8 @@ def test_mesh_fragmentation_and_merger_runs_identical(tmp_path, cfg, mass, radiu
 
 
 @pytest.mark.parametrize("cfg,n,seed,mass,radius,steps", [
-    ("run_c1_out", 2000, 1, 1e-8, 4e-3, 40), ("run_mesh_out", 16384, 2, 1e-10, 1e-3, 10)])
+    ("run_c1_out", 2000, 1, 1e-8, 4e-3, 40), ("run_mesh_out", 16384, 2, 1e-10, 1e-3, 10),
+    ("run_c1_out_ell", 2000, 3, 1e-8, 4e-3, 20), ("run_c1_out_cart", 2000, 4, 1e-8, 4e-3, 20)])
 def test_output_files_identical(tmp_path, cfg, n, seed, mass, radius, steps):
     # display() with cartesian and elliptic elements, radii, masses,
     # statistics and collision data: the reference and the drop-in write
@@ -121,10 +122,14 @@ def test_output_files_identical(tmp_path, cfg, n, seed, mass, radius, steps):
     assert sum(1 for l in traces["ref"] if l.startswith("DISP")) > 5
     names = sorted(f for f in os.listdir(dirs["ref"]) if f.endswith(".txt") and f != "init.txt")
     assert sorted(f for f in os.listdir(dirs["gpu"]) if f.endswith(".txt") and f != "init.txt") == names
-    assert {"x.txt", "lambda.txt", "stat.txt", "mass.txt"} <= set(names)
+    want = {"stat.txt", "mass.txt", "radius.txt"}
+    want |= {"lambda.txt", "a.txt"} if cfg != "run_c1_out_cart" else {"x.txt", "collision.txt"}
+    if cfg in ("run_c1_out", "run_mesh_out"):
+        want |= {"x.txt", "lambda.txt"}
+    assert want <= set(names), names
     for f in names:
         a = (dirs["ref"] / f).read_bytes()
         g = (dirs["gpu"] / f).read_bytes()
         assert a == g, f
-    if cfg == "run_c1_out":
+    if cfg in ("run_c1_out", "run_c1_out_cart"):
         assert len((dirs["ref"] / "collision.txt").read_bytes()) > 1000
