@@ -11,6 +11,7 @@
 // the q > k terms likewise (X_q - X_k, then added), so signed zeros match too.
 // This translation unit is compiled with -ffp-contract=off: no FMA fusion.
 #include "ncg_internal.hpp"
+#include "exact_div.hpp"
 
 #pragma clang fp contract(off)
 
@@ -31,9 +32,13 @@ __device__ inline void exact_term(double X, double Y, double Z, double Rk, doubl
     const double soft = s * (lower ? (Rk + Rj) : (Rj + Rk));
     const double D = sqrt(dx * dx + dy * dy + dz * dz + soft * soft);
     const double D3 = D * D * D;
-    tX = gmj * dx / D3;
-    tY = gmj * dy / D3;
-    tZ = gmj * dz / D3;
+    {
+        double q0, q1, q2;
+        div3(gmj * dx, gmj * dy, gmj * dz, D3, q0, q1, q2);
+        tX = q0;
+        tY = q1;
+        tZ = q2;
+    }
 }
 
 // Central body from the host (cp, cx, cy, cz, mu) or, when dcen is given,

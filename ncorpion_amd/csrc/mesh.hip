@@ -28,6 +28,7 @@
 #include <tuple>
 
 #include "ncg_internal.hpp"
+#include "exact_div.hpp"
 #include "collide_dev.hpp"
 
 #pragma clang fp contract(off)
@@ -581,9 +582,13 @@ __device__ inline void term_exact(double X, double Y, double Z, double Rk, doubl
     const double D = sqrt(dx * dx + dy * dy + dz * dz + soft * soft);
     const double D3 = D * D * D;
     const double gm = G * p.w;
-    ax -= gm * dx / D3;
-    ay -= gm * dy / D3;
-    az -= gm * dz / D3;
+    {
+        double q0, q1, q2;
+        div3(gm * dx, gm * dy, gm * dz, D3, q0, q1, q2);
+        ax -= q0;
+        ay -= q1;
+        az -= q2;
+    }
 }
 
 __device__ inline void term_fast(double X, double Y, double Z, double Rk, double4 p, double Rp,
@@ -760,9 +765,13 @@ __global__ __launch_bounds__(MT) void k_big_partial(const DBody *__restrict__ b,
             const double D = sqrt(dx * dx + dy * dy + dz * dz + soft * soft);
             const double D3 = D * D * D;
             const double gm = G * q.mass;
-            a[3 * big + 0] += gm * dx / D3;
-            a[3 * big + 1] += gm * dy / D3;
-            a[3 * big + 2] += gm * dz / D3;
+            {
+                double q0, q1, q2;
+                div3(gm * dx, gm * dy, gm * dz, D3, q0, q1, q2);
+                a[3 * big + 0] += q0;
+                a[3 * big + 1] += q1;
+                a[3 * big + 2] += q2;
+            }
         }
     }
     for (int c = 0; c < 12; ++c) {
@@ -802,12 +811,20 @@ __global__ __launch_bounds__(MT) void k_big_final(const DBody *__restrict__ b, i
         const double soft = s * (A.radius + B.radius);
         const double D = sqrt(dx * dx + dy * dy + dz * dz + soft * soft);
         const double D3 = D * D * D;
-        acc[3 * i + 0] -= G * B.mass * dx / D3;
-        acc[3 * i + 1] -= G * B.mass * dy / D3;
-        acc[3 * i + 2] -= G * B.mass * dz / D3;
-        acc[3 * j + 0] += G * A.mass * dx / D3;
-        acc[3 * j + 1] += G * A.mass * dy / D3;
-        acc[3 * j + 2] += G * A.mass * dz / D3;
+        {
+            double q0, q1, q2;
+            div3(G * B.mass * dx, G * B.mass * dy, G * B.mass * dz, D3, q0, q1, q2);
+            acc[3 * i + 0] -= q0;
+            acc[3 * i + 1] -= q1;
+            acc[3 * i + 2] -= q2;
+        }
+        {
+            double q0, q1, q2;
+            div3(G * A.mass * dx, G * A.mass * dy, G * A.mass * dz, D3, q0, q1, q2);
+            acc[3 * j + 0] += q0;
+            acc[3 * j + 1] += q1;
+            acc[3 * j + 2] += q2;
+        }
     }
 }
 
@@ -826,12 +843,20 @@ __global__ void k_extra_pairs(const DBody *__restrict__ b, const int32_t *__rest
         const double soft = s * (A.radius + B.radius);
         const double D = sqrt(dx * dx + dy * dy + dz * dz + soft * soft);
         const double D3 = D * D * D;
-        acc[3 * k + 0] -= G * B.mass * dx / D3;
-        acc[3 * k + 1] -= G * B.mass * dy / D3;
-        acc[3 * k + 2] -= G * B.mass * dz / D3;
-        acc[3 * p + 0] += G * A.mass * dx / D3;
-        acc[3 * p + 1] += G * A.mass * dy / D3;
-        acc[3 * p + 2] += G * A.mass * dz / D3;
+        {
+            double q0, q1, q2;
+            div3(G * B.mass * dx, G * B.mass * dy, G * B.mass * dz, D3, q0, q1, q2);
+            acc[3 * k + 0] -= q0;
+            acc[3 * k + 1] -= q1;
+            acc[3 * k + 2] -= q2;
+        }
+        {
+            double q0, q1, q2;
+            div3(G * A.mass * dx, G * A.mass * dy, G * A.mass * dz, D3, q0, q1, q2);
+            acc[3 * p + 0] += q0;
+            acc[3 * p + 1] += q1;
+            acc[3 * p + 2] += q2;
+        }
     }
 }
 
@@ -1221,9 +1246,13 @@ __global__ __launch_bounds__(MT) void k_mterms(const DBody *__restrict__ b, int 
                 const double soft = s * (q.radius + g.radius);
                 const double D = sqrt(dx * dx + dy * dy + dz * dz + soft * soft);
                 const double D3 = D * D * D;
-                v[3 * i + 0] = G * q.mass * dx / D3;
-                v[3 * i + 1] = G * q.mass * dy / D3;
-                v[3 * i + 2] = G * q.mass * dz / D3;
+                {
+                    double q0, q1, q2;
+                    div3(G * q.mass * dx, G * q.mass * dy, G * q.mass * dz, D3, q0, q1, q2);
+                    v[3 * i + 0] = q0;
+                    v[3 * i + 1] = q1;
+                    v[3 * i + 2] = q2;
+                }
             }
     }
     double *dst = T + (size_t)(k / SQ_CH) * (12 * SQ_CH) + (k % SQ_CH);
@@ -1379,12 +1408,20 @@ __global__ __launch_bounds__(256) void k_mseq(const DBody *__restrict__ b, int n
         const double soft = s * (g[i].radius + g[j].radius);
         const double D = sqrt(dx * dx + dy * dy + dz * dz + soft * soft);
         const double D3 = D * D * D;
-        a[3 * i + 0] -= G * g[j].mass * dx / D3;
-        a[3 * i + 1] -= G * g[j].mass * dy / D3;
-        a[3 * i + 2] -= G * g[j].mass * dz / D3;
-        a[3 * j + 0] += G * g[i].mass * dx / D3;
-        a[3 * j + 1] += G * g[i].mass * dy / D3;
-        a[3 * j + 2] += G * g[i].mass * dz / D3;
+        {
+            double q0, q1, q2;
+            div3(G * g[j].mass * dx, G * g[j].mass * dy, G * g[j].mass * dz, D3, q0, q1, q2);
+            a[3 * i + 0] -= q0;
+            a[3 * i + 1] -= q1;
+            a[3 * i + 2] -= q2;
+        }
+        {
+            double q0, q1, q2;
+            div3(G * g[i].mass * dx, G * g[i].mass * dy, G * g[i].mass * dz, D3, q0, q1, q2);
+            a[3 * j + 0] += q0;
+            a[3 * j + 1] += q1;
+            a[3 * j + 2] += q2;
+        }
     }
     for (int i = 0; i < 3; ++i)
         if (bex[i])
