@@ -36,7 +36,7 @@ PEAK_FP64_VECTOR_TFLOPS = 78.6
 # shape (nsplit 128, theta 0.20): FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (scales from a
 # 256 MiB copy), from rocprofv3 --pmc runs of this script, summarised in
 # profiles/k_pairs_mixed_n262144_pmc.md.  Mostly the j-split partial sums.
-PAIRS_TRAFFIC_BYTES = (110634 * 2.000 + 786432 * 1.000) * 1024
+PAIRS_TRAFFIC_BYTES = (110639 * 2.000 + 786432 * 1.000) * 1024
 
 
 def make_workload(n, seed=0):

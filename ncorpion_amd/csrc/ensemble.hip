@@ -67,14 +67,15 @@ __device__ inline void ens_mixed_term(const double4 p, const float2 g, const dou
         const float dx = (float)(p.x - xi[q]);
         const float dy = (float)(p.y - yi[q]);
         const float dz = (float)(p.z - zi[q]);
+        // 1e-20 in the chain: a body against itself adds 0 * finite; it is
+        // below half an ulp of any r^2 above ~2e-13.
         float r2;
         if (SOFT) {
             const float e = sif[q] + g.y;
-            r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, e * e)));
+            r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, fmaf(e, e, 1e-20f))));
         } else {
-            r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+            r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, 1e-20f)));
         }
-        r2 = fmaxf(r2, 1e-20f);  // a body against itself adds 0 * finite
         const float y = __builtin_amdgcn_rsqf(r2);
         const float s3 = (g.x * y) * (y * y);
         fx[q] = fmaf(s3, dx, fx[q]);
@@ -153,8 +154,7 @@ __global__ __launch_bounds__(ENT) void k_ens_pass(const EnsUnit *__restrict__ un
                         for (int q = 0; q < IPL; ++q) {
                             const double dx = p.x - xi[q], dy = p.y - yi[q], dz = p.z - zi[q];
                             const double e = si[q] + sj;
-                            double r2 = fma(dx, dx, fma(dy, dy, fma(dz, dz, e * e)));
-                            r2 = fmax(r2, 1e-200);
+                            const double r2 = fma(dx, dx, fma(dy, dy, fma(dz, dz, fma(e, e, 1e-200))));
                             double y = __builtin_amdgcn_rsq(r2);
                             y = y * fma(-0.5 * r2, y * y, 1.5);
                             const double s3 = p.w * y * (y * y);

@@ -240,8 +240,10 @@ __global__ __launch_bounds__(NT) void k_tiles(const double4 *__restrict__ jd,
 // ---- 6. pair kernel -------------------------------------------------------
 // One interaction of the FP64 form: a += G m_j (x_j - x_i) / (r^2 + eps^2)^(3/2),
 // hardware 1/sqrt refined by one Newton step (v_rsq_f64 is good to ~2^-29;
-// one step brings it to FP64 rounding).  r^2 is floored at TINY2 so that a
-// body against itself (r = 0, no softening) adds exactly 0.
+// one step brings it to FP64 rounding).  TINY2 enters the FMA chain with
+// eps^2, so that a body against itself (r = 0, no softening) has r^2 = TINY2
+// and adds exactly 0; above r^2 ~ 1e-184 it is below half an ulp and leaves
+// r^2 as it is (one FMA instead of a multiply and a floor).
 template <int IPL>
 __device__ inline void near_term(const double4 p, double sj, const double *xi, const double *yi,
                                  const double *zi, const double *si, double *ax, double *ay,
@@ -251,8 +253,7 @@ __device__ inline void near_term(const double4 p, double sj, const double *xi, c
     for (int q = 0; q < IPL; ++q) {
         const double dx = p.x - xi[q], dy = p.y - yi[q], dz = p.z - zi[q];
         const double e = si[q] + sj;
-        double r2 = fma(dx, dx, fma(dy, dy, fma(dz, dz, e * e)));
-        r2 = fmax(r2, TINY2);
+        const double r2 = fma(dx, dx, fma(dy, dy, fma(dz, dz, fma(e, e, TINY2))));
         double y = __builtin_amdgcn_rsq(r2);
         const double h = 0.5 * r2;
         y = y * fma(-h, y * y, 1.5);

@@ -596,8 +596,9 @@ __device__ inline void term_fast(double X, double Y, double Z, double Rk, double
 {
     const double dx = p.x - X, dy = p.y - Y, dz = p.z - Z;
     const double e = s * (Rk + Rp);
-    double r2 = __builtin_fma(dx, dx, __builtin_fma(dy, dy, __builtin_fma(dz, dz, e * e)));
-    r2 = fmax(r2, 1e-200);
+    // 1e-200 in the chain: a body against itself adds 0 * finite.
+    const double r2 =
+        __builtin_fma(dx, dx, __builtin_fma(dy, dy, __builtin_fma(dz, dz, __builtin_fma(e, e, 1e-200))));
     double y = __builtin_amdgcn_rsq(r2);
     y = y * __builtin_fma(-0.5 * r2, y * y, 1.5);
     const double f = (G * p.w) * y * (y * y);
