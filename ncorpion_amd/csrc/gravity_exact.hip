@@ -154,42 +154,54 @@ __global__ __launch_bounds__(256) void k_grav_exact(const DBody *__restrict__ b,
 }
 
 // CM_acc as vector_field forms it: the per-body reaction terms added in body
-// order, one thread (physics.c:141-146 accumulates CM_acc body by body).
-// The workgroup stages 256 bodies' terms in LDS, one lane adds them in order.
+// order (physics.c:141-146 accumulates CM_acc body by body).  Waves 1..3
+// stage chunks of CMS_CH bodies' terms in LDS, a term of an absent body (or
+// past n) stored as -0.0, while lanes 0, 1, 2 of wave 0 add the x, y, z terms
+// of the previous chunk: a run of additions with no branch (a + (-0.0) is a
+// for every a, so a skipped body leaves the sum as it was).
+constexpr int CMS_CH = 1024;
+
 __global__ __launch_bounds__(256) void k_cm_seq(const double *__restrict__ cterm, int n,
                                                 const int32_t *__restrict__ ex,
                                                 const double *__restrict__ dcen,
                                                 double *__restrict__ cm)
 {
-    __shared__ double st[3 * 256];
-    __shared__ int se[256];
-    double a = 0.0, c = 0.0, d = 0.0;
-    const bool cp = dcen[7] != 0.0;
-    for (int base = 0; cp && base < n; base += 256) {
-        const int k = base + (int)threadIdx.x;
-        __syncthreads();
-        if (k < n) {
-            se[threadIdx.x] = ex ? (ex[k] != 0) : 1;
-            st[3 * threadIdx.x] = cterm[3 * (size_t)k];
-            st[3 * threadIdx.x + 1] = cterm[3 * (size_t)k + 1];
-            st[3 * threadIdx.x + 2] = cterm[3 * (size_t)k + 2];
+    // Rows padded by two doubles so that the three summing lanes read
+    // different banks.
+    __shared__ double st[2][3][CMS_CH + 2];
+    const int tid = threadIdx.x, w = tid >> 6;
+    if (dcen[7] == 0.0) {
+        if (tid < 3) cm[tid] = 0.0;
+        return;
+    }
+    const int nch = (n + CMS_CH - 1) / CMS_CH;
+    auto stage = [&](int buf, int base) {
+        for (int q = tid - 64; q < CMS_CH; q += 192) {
+            const int k = base + q;
+            const bool on = k < n && (!ex || ex[k] != 0);
+            st[buf][0][q] = on ? cterm[3 * (size_t)k] : -0.0;
+            st[buf][1][q] = on ? cterm[3 * (size_t)k + 1] : -0.0;
+            st[buf][2][q] = on ? cterm[3 * (size_t)k + 2] : -0.0;
         }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            const int cnt = min(256, n - base);
-            for (int q = 0; q < cnt; ++q) {
-                if (!se[q]) continue;
-                a += st[3 * q];
-                c += st[3 * q + 1];
-                d += st[3 * q + 2];
+    };
+    if (w > 0 && nch > 0) stage(0, 0);
+    __syncthreads();
+    double a = 0.0;
+    for (int c = 0; c < nch; ++c) {
+        if (w > 0) {
+            if (c + 1 < nch) stage((c + 1) & 1, (c + 1) * CMS_CH);
+        } else if (tid < 3) {
+            // Whole groups of 16 (the padding past n is -0.0).
+            const int len = (min(CMS_CH, n - c * CMS_CH) + 15) & ~15;
+            const double *row = st[c & 1][tid];
+            for (int q = 0; q < len; q += 16) {
+#pragma unroll
+                for (int u = 0; u < 16; ++u) a += row[q + u];
             }
         }
+        __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        cm[0] = a;
-        cm[1] = c;
-        cm[2] = d;
-    }
+    if (tid < 3) cm[tid] = a;
 }
 
 // Small n: one wavefront per body k.  The lanes form the terms of 64 bodies j
