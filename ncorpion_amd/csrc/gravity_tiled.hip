@@ -407,6 +407,86 @@ __global__ __launch_bounds__(NT) void k_pairs(const double4 *__restrict__ jd,
     }
 }
 
+// ---- 6b. small n: direct form ------------------------------------------------
+// Up to SMALL_N bodies both fast forms skip the spatial order and the tiles:
+// one wavefront per body, the workgroup's four bodies sharing the j records
+// staged in LDS NT at a time.  Each lane takes the j of its columns in the
+// FP64 form (near_term); at this size that costs less than the tiled
+// pipeline's set-up launches.  An absent j (or one past n) is staged with
+// zero mass at the origin and softening length 1, so its term is exactly 0.
+// The lanes' partials are added by a fixed butterfly, then the central term
+// as k_finish adds it.
+constexpr int SMALL_N = 4096;
+
+__global__ __launch_bounds__(NT) void k_small(const DBody *__restrict__ b, int n,
+                                              const int32_t *__restrict__ ex, int cp, double cx,
+                                              double cy, double cz, double mu,
+                                              const double *__restrict__ dcen, double G,
+                                              double soft, double *__restrict__ acc,
+                                              double *__restrict__ cterm)
+{
+    __shared__ double4 sp[NT];
+    __shared__ double ss[NT];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int k = blockIdx.x * (NT / 64) + w;
+    const bool mine = k < n && (!ex || ex[k] != 0);
+    double X = 0.0, Y = 0.0, Z = 0.0, Sk = 0.0;
+    if (mine) {
+        const DBody q = b[k];
+        X = q.x;
+        Y = q.y;
+        Z = q.z;
+        Sk = soft * q.radius;
+    }
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    for (int base = 0; base < n; base += NT) {
+        const int j = base + (int)threadIdx.x;
+        __syncthreads();
+        if (j < n && (!ex || ex[j] != 0)) {
+            const DBody q = b[j];
+            sp[threadIdx.x] = make_double4(q.x, q.y, q.z, G * q.mass);
+            ss[threadIdx.x] = soft * q.radius;
+        } else {
+            sp[threadIdx.x] = make_double4(0.0, 0.0, 0.0, 0.0);
+            ss[threadIdx.x] = 1.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < NT / 64; ++u)
+            near_term<1>(sp[lane + 64 * u], ss[lane + 64 * u], &X, &Y, &Z, &Sk, &ax, &ay, &az);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        ax += __shfl_xor(ax, off);
+        ay += __shfl_xor(ay, off);
+        az += __shfl_xor(az, off);
+    }
+    if (!mine || lane != 0) return;
+    if (dcen) {
+        cx = dcen[0];
+        cy = dcen[1];
+        cz = dcen[2];
+        mu = G * dcen[6];
+        cp = dcen[7] != 0.0;
+    }
+    if (cp) {
+        const DBody q = b[k];
+        const double DX = q.x - cx, DY = q.y - cy, DZ = q.z - cz;
+        const double rk = sqrt(DX * DX + DY * DY + DZ * DZ);
+        const double r3 = rk * rk * rk;
+        ax += -mu * DX / r3;
+        ay += -mu * DY / r3;
+        az += -mu * DZ / r3;
+        if (cterm) {
+            cterm[3 * (size_t)k + 0] = G * q.mass * DX / r3;
+            cterm[3 * (size_t)k + 1] = G * q.mass * DY / r3;
+            cterm[3 * (size_t)k + 2] = G * q.mass * DZ / r3;
+        }
+    }
+    acc[3 * (size_t)k + 0] = ax;
+    acc[3 * (size_t)k + 1] = ay;
+    acc[3 * (size_t)k + 2] = az;
+}
+
 // ---- 7. finish ------------------------------------------------------------
 __global__ __launch_bounds__(NT) void k_finish(const DBody *__restrict__ b,
                                                const int *__restrict__ perm,
@@ -558,6 +638,7 @@ static TiledShape tiled_shape(ncg_ctx *ctx, int32_t n, int prec)
 ncg_status ncg_tiled_order(ncg_ctx *ctx, const DBody *d_b, int32_t n, const int32_t *d_ex,
                            hipStream_t st)
 {
+    if (n <= SMALL_N) return NCG_OK;  // the direct form needs no order
     const TiledShape t = tiled_shape(ctx, n, NCG_MIXED);
     Scal *sc = dbuf<Scal>(ctx, "t_scal", 1);
     double *bpart = dbuf<double>(ctx, "t_bpart", 8 * (size_t)t.nbb);
@@ -584,6 +665,27 @@ ncg_status ncg_tiled_eval(ncg_ctx *ctx, const DBody *d_b, int32_t n, const int32
                           ncg_precision prec, double *d_acc, double *d_cterm, double *d_cm_acc,
                           hipStream_t st)
 {
+    const bool want_cm = d_cm_acc != nullptr;
+    if (n <= SMALL_N) {
+        double *cterm = d_cterm ? d_cterm : (want_cm ? dbuf<double>(ctx, "cterm", 3 * (size_t)n) : nullptr);
+        const int nb = std::max(1, std::min(1024, (int)cdiv(n, NT)));
+        double *p3 = want_cm ? dbuf<double>(ctx, "sum3", 3 * (size_t)nb) : nullptr;
+        if (want_cm && (!cterm || !p3)) return NCG_ERR_NOMEM;
+        const int cp = (c && c->present) ? 1 : 0;
+        {
+            ProfScope ps(ctx, st, prec == NCG_F64_FAST ? "pairs_fast64" : "pairs_mixed");
+            hipLaunchKernelGGL(k_small, dim3(std::max(1, (int)cdiv(n, NT / 64))), dim3(NT), 0, st, d_b,
+                               n, d_ex, cp, cp ? c->x : 0.0, cp ? c->y : 0.0, cp ? c->z : 0.0,
+                               cp ? G * c->mass : 0.0, d_central, G, soft, d_acc, cterm);
+            NCG_CHECK_LAUNCH();
+        }
+        if (want_cm) {
+            hipLaunchKernelGGL(k_sum3, dim3(nb), dim3(NT), 0, st, cterm, n, d_ex, p3);
+            hipLaunchKernelGGL(k_sum3_final, dim3(1), dim3(NT), 0, st, p3, nb, d_cm_acc);
+            NCG_CHECK_LAUNCH();
+        }
+        return NCG_OK;
+    }
     const TiledShape t = tiled_shape(ctx, n, prec);
     Scal *sc = dbuf<Scal>(ctx, "t_scal", 1);
     int *perm = dbuf<int>(ctx, "t_perm", n);
@@ -593,7 +695,6 @@ ncg_status ncg_tiled_eval(ncg_ctx *ctx, const DBody *d_b, int32_t n, const int32
     TileHdr *th = dbuf<TileHdr>(ctx, "t_th", t.ntiles);
     TileHdr *ih = dbuf<TileHdr>(ctx, "t_ih", t.nib);
     double *part = dbuf<double>(ctx, "t_part", (size_t)t.nsplit * 3 * n);
-    const bool want_cm = d_cm_acc != nullptr;
     double *cterm = d_cterm ? d_cterm : (want_cm ? dbuf<double>(ctx, "cterm", 3 * (size_t)n) : nullptr);
     double *p3 = want_cm ? dbuf<double>(ctx, "sum3", 3 * (size_t)t.nbb) : nullptr;
     if (!sc || !perm || !jd || !jsig || !ju || !th || !ih || !part || (want_cm && (!cterm || !p3)))
