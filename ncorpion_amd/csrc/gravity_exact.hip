@@ -247,11 +247,18 @@ __global__ __launch_bounds__(256) void k_grav_exact_w(const DBody *__restrict__ 
             cterm[3 * (size_t)k + 2] = G * me.mass * DZ / r3;
         }
     }
+    // The next group's records are loaded before the current group is
+    // summed, so their latency overlaps the ordered additions.
+    DBody q = {};
+    int e = 0;
+    if (lane < n) {
+        q = b[lane];
+        e = ex == nullptr ? 1 : ex[lane];
+    }
     for (int base = 0; base < n; base += 64) {
         const int j = base + lane;
         double tX = -0.0, tY = -0.0, tZ = -0.0;
-        if (j < n && j != k && (ex == nullptr || ex[j] != 0)) {
-            const DBody q = b[j];
+        if (j < n && j != k && e != 0) {
             const bool lower = j < k;
             exact_term(X, Y, Z, Rk, q.x, q.y, q.z, q.radius, G * q.mass, s, lower, tX, tY, tZ);
             if (lower) {
@@ -259,6 +266,11 @@ __global__ __launch_bounds__(256) void k_grav_exact_w(const DBody *__restrict__ 
                 tY = -tY;
                 tZ = -tZ;
             }
+        }
+        const int jn = j + 64;
+        if (jn < n) {
+            q = b[jn];
+            e = ex == nullptr ? 1 : ex[jn];
         }
         st[w][0][lane] = tX;
         st[w][1][lane] = tY;
