@@ -17,7 +17,12 @@
 
 namespace {
 
-constexpr int CT = 128;  // tile edge (bodies) of the upper-triangular tiling
+// Tile edge (bodies) of the upper-triangular tiling: 128, or 64 up to
+// CT_SMALL_N bodies, where 128-body tiles leave most SIMDs idle (n = 2000:
+// 136 workgroups of 128 lanes each walking 128 partners; with 64, 528
+// wavefronts walking 64).
+constexpr int CT_SMALL_N = 16384;
+__host__ __device__ constexpr int cand_tile(int n) { return n <= CT_SMALL_N ? 64 : 128; }
 
 // closest_approach's predicate (collision.c:112-137), same operations in the
 // same order.  The final time test keeps a relative slack of 2^-40 so the set
@@ -40,6 +45,7 @@ __device__ inline bool approach_hit(double xa, double ya, double za, double vxa,
     return t <= dt + fabs(dt) * 9.094947017729282e-13;
 }
 
+template <int CT>
 __global__ __launch_bounds__(CT) void k_candidates(const DBody *__restrict__ b, int n,
                                                    const int32_t *__restrict__ ex, double dt,
                                                    int ntiles, int2 *__restrict__ out,
@@ -96,12 +102,17 @@ ncg_status ncg_candidates_launch(ncg_ctx *ctx, const DBody *d_b, int32_t n, cons
                                  double dt, int2 *d_out, unsigned long long cap,
                                  unsigned long long *d_count, hipStream_t st)
 {
-    const int ntiles = (int)cdiv(n, CT);
+    const int ct = cand_tile(n);
+    const int ntiles = (int)cdiv(n, ct);
     const long nblk = (long)ntiles * (ntiles + 1) / 2;
     if (nblk == 0) return NCG_OK;
     ProfScope ps(ctx, st, "candidates");
-    hipLaunchKernelGGL(k_candidates, dim3((unsigned)nblk), dim3(CT), 0, st, d_b, (int)n, d_ex, dt,
-                       ntiles, d_out, cap, d_count);
+    if (ct == 64)
+        hipLaunchKernelGGL(k_candidates<64>, dim3((unsigned)nblk), dim3(64), 0, st, d_b, (int)n,
+                           d_ex, dt, ntiles, d_out, cap, d_count);
+    else
+        hipLaunchKernelGGL(k_candidates<128>, dim3((unsigned)nblk), dim3(128), 0, st, d_b, (int)n,
+                           d_ex, dt, ntiles, d_out, cap, d_count);
     NCG_CHECK_LAUNCH();
     return NCG_OK;
 }
@@ -111,7 +122,8 @@ static ncg_status candidates_dev(ncg_ctx *ctx, const DBody *db, int32_t n, const
                                  double dt, std::vector<ncg_pair> &out)
 {
     hipStream_t st = ctx->stream;
-    const int ntiles = (int)cdiv(n, CT);
+    const int ct = cand_tile(n);
+    const int ntiles = (int)cdiv(n, ct);
     const long nblk = (long)ntiles * (ntiles + 1) / 2;
     unsigned long long cap = 1ull << 16;
     for (int attempt = 0; attempt < 2; ++attempt) {
@@ -121,8 +133,12 @@ static ncg_status candidates_dev(ncg_ctx *ctx, const DBody *db, int32_t n, const
         NCG_HIP(hipMemsetAsync(dcnt, 0, sizeof(unsigned long long), st));
         {
             ProfScope ps(ctx, st, "candidates");
-            hipLaunchKernelGGL(k_candidates, dim3((unsigned)nblk), dim3(CT), 0, st, db, (int)n, dex,
-                               dt, ntiles, dout, cap, dcnt);
+            if (ct == 64)
+                hipLaunchKernelGGL(k_candidates<64>, dim3((unsigned)nblk), dim3(64), 0, st, db,
+                                   (int)n, dex, dt, ntiles, dout, cap, dcnt);
+            else
+                hipLaunchKernelGGL(k_candidates<128>, dim3((unsigned)nblk), dim3(128), 0, st, db,
+                                   (int)n, dex, dt, ntiles, dout, cap, dcnt);
             NCG_CHECK_LAUNCH();
         }
         unsigned long long cnt = 0;
