@@ -74,7 +74,9 @@ def c3_disk(n, seed=0):
     return ic.disk(n, seed=seed, masses=m, radii=R)
 
 
-@pytest.mark.parametrize("n", [1000, 8192])
+# 1000 and 4096 run the direct form (up to 4096 bodies), 4097 and 8192 the
+# tiled one.
+@pytest.mark.parametrize("n", [1000, 4096, 4097, 8192])
 def test_fast64_matches_oracle(ctx, n):
     b = c3_disk(n, 11)
     ref, rcm = orc.vector_field_brute(b, None, None, 1.0, 1e-6)
@@ -89,7 +91,7 @@ def test_fast64_matches_oracle(ctx, n):
     assert np.allclose(cm2, rcm2, rtol=1e-12, atol=0)
 
 
-@pytest.mark.parametrize("n", [1000, 16384])
+@pytest.mark.parametrize("n", [1000, 4096, 4097, 16384])
 def test_mixed_within_tolerance(ctx, n):
     # Mutual part only (the central term is FP64 in every form).
     b = c3_disk(n, 12)
