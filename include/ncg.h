@@ -159,7 +159,8 @@ ncg_status ncg_gravity_allpairs(ncg_ctx *ctx, const ncg_body *bodies, int32_t n,
                                 double *acc, double *cm_acc);
 
 /* Same on device buffers: bodies (n records), exists (n or NULL), acc (3n),
- * cm_acc (3 doubles or NULL). */
+ * cm_acc (3 doubles or NULL; in the reference's order for NCG_F64_EXACT, as
+ * above). */
 ncg_status ncg_gravity_allpairs_dev(ncg_ctx *ctx, const ncg_body *d_bodies, int32_t n,
                                     const int32_t *d_exists, const ncg_central *central,
                                     double grav_const, double softening, ncg_precision prec,

@@ -56,19 +56,23 @@ __global__ void k_zero_u32(unsigned *p, int n)
 }
 
 // One FP32 interaction of the mixed form: separation rounded from FP64,
-// softening added only when SOFT.
-template <bool SOFT, int IPL>
-__device__ inline void ens_mixed_term(const double4 p, const float2 g, const double *xi,
-                                      const double *yi, const double *zi, const float *sif,
-                                      float *fx, float *fy, float *fz)
+// softening added only when SOFT.  With SELF the body at local index js is
+// compared with the lane's own (li): a body against itself has r^2 = 1e-20
+// without softening, so its mass is zeroed there (G m y^3 would overflow for
+// G m > ~3e8 and inf * 0 poison the sum).
+template <bool SOFT, bool SELF, int IPL>
+__device__ inline void ens_mixed_term(const double4 p, const float2 g, int js, const int *li,
+                                      const double *xi, const double *yi, const double *zi,
+                                      const float *sif, float *fx, float *fy, float *fz)
 {
 #pragma unroll
     for (int q = 0; q < IPL; ++q) {
+        const float gm = (SELF && js == li[q]) ? 0.f : g.x;
         const float dx = (float)(p.x - xi[q]);
         const float dy = (float)(p.y - yi[q]);
         const float dz = (float)(p.z - zi[q]);
-        // 1e-20 in the chain: a body against itself adds 0 * finite; it is
-        // below half an ulp of any r^2 above ~2e-13.
+        // 1e-20 in the chain keeps a self pair's y finite; it is below half
+        // an ulp of any r^2 above ~2e-13.
         float r2;
         if (SOFT) {
             const float e = sif[q] + g.y;
@@ -77,26 +81,30 @@ __device__ inline void ens_mixed_term(const double4 p, const float2 g, const dou
             r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, 1e-20f)));
         }
         const float y = __builtin_amdgcn_rsqf(r2);
-        const float s3 = (g.x * y) * (y * y);
+        const float s3 = (gm * y) * (y * y);
         fx[q] = fmaf(s3, dx, fx[q]);
         fy[q] = fmaf(s3, dy, fy[q]);
         fz[q] = fmaf(s3, dz, fz[q]);
     }
 }
 
-// Bodies j0 .. j1-1 (at most 64) of the staged tile into FP32 partials.
-template <bool SOFT, int IPL>
-__device__ inline void ens_mixed_run(const double4 *tp, const float2 *tf, int j0, int j1,
-                                     const double *xi, const double *yi, const double *zi,
-                                     const float *sif, float *fx, float *fy, float *fz)
+// Bodies j0 .. j1-1 (at most 64) of the staged tile (first local index jb)
+// into FP32 partials.
+template <bool SOFT, bool SELF, int IPL>
+__device__ inline void ens_mixed_run(const double4 *tp, const float2 *tf, int j0, int j1, int jb,
+                                     const int *li, const double *xi, const double *yi,
+                                     const double *zi, const float *sif, float *fx, float *fy,
+                                     float *fz)
 {
-    if (j1 - j0 == 64) {
+    if (!SELF && j1 - j0 == 64) {
 #pragma unroll 4
         for (int jj = j0; jj < j0 + 64; ++jj)
-            ens_mixed_term<SOFT, IPL>(tp[jj], tf[jj], xi, yi, zi, sif, fx, fy, fz);
+            ens_mixed_term<SOFT, false, IPL>(tp[jj], tf[jj], jb + jj, li, xi, yi, zi, sif, fx, fy,
+                                             fz);
     } else {
         for (int jj = j0; jj < j1; ++jj)
-            ens_mixed_term<SOFT, IPL>(tp[jj], tf[jj], xi, yi, zi, sif, fx, fy, fz);
+            ens_mixed_term<SOFT, SELF, IPL>(tp[jj], tf[jj], jb + jj, li, xi, yi, zi, sif, fx, fy,
+                                            fz);
     }
 }
 
@@ -146,6 +154,10 @@ __global__ __launch_bounds__(ENT) void k_ens_pass(const EnsUnit *__restrict__ un
                     tf[threadIdx.x] = make_float2((float)(G * q.mass), (float)(soft * q.radius));
                 }
                 __syncthreads();
+                // The tile holds some of the unit's own bodies: their self
+                // terms are zeroed (r^2 = 1e-200 without softening gives
+                // y^3 = 1e300, which G m > ~1.8e8 would overflow).
+                const bool diag = jb < U.i0 + ENT * IPL && jb + cnt > U.i0;
                 if (MODE == 0) {
                     for (int jj = 0; jj < cnt; ++jj) {
                         const double4 p = tp[jj];
@@ -157,7 +169,8 @@ __global__ __launch_bounds__(ENT) void k_ens_pass(const EnsUnit *__restrict__ un
                             const double r2 = fma(dx, dx, fma(dy, dy, fma(dz, dz, fma(e, e, 1e-200))));
                             double y = __builtin_amdgcn_rsq(r2);
                             y = y * fma(-0.5 * r2, y * y, 1.5);
-                            const double s3 = p.w * y * (y * y);
+                            const double w = (diag && jb + jj == li[q]) ? 0.0 : p.w;
+                            const double s3 = w * y * (y * y);
                             ax[q] = fma(s3, dx, ax[q]);
                             ay[q] = fma(s3, dy, ay[q]);
                             az[q] = fma(s3, dz, az[q]);
@@ -172,10 +185,20 @@ __global__ __launch_bounds__(ENT) void k_ens_pass(const EnsUnit *__restrict__ un
                         float fx[IPL], fy[IPL], fz[IPL];
 #pragma unroll
                         for (int q = 0; q < IPL; ++q) fx[q] = fy[q] = fz[q] = 0.f;
-                        if (soft != 0.0)
-                            ens_mixed_run<true, IPL>(tp, tf, j0, j1, xi, yi, zi, sif, fx, fy, fz);
-                        else
-                            ens_mixed_run<false, IPL>(tp, tf, j0, j1, xi, yi, zi, sif, fx, fy, fz);
+                        if (diag) {
+                            if (soft != 0.0)
+                                ens_mixed_run<true, true, IPL>(tp, tf, j0, j1, jb, li, xi, yi, zi,
+                                                               sif, fx, fy, fz);
+                            else
+                                ens_mixed_run<false, true, IPL>(tp, tf, j0, j1, jb, li, xi, yi, zi,
+                                                                sif, fx, fy, fz);
+                        } else if (soft != 0.0) {
+                            ens_mixed_run<true, false, IPL>(tp, tf, j0, j1, jb, li, xi, yi, zi,
+                                                            sif, fx, fy, fz);
+                        } else {
+                            ens_mixed_run<false, false, IPL>(tp, tf, j0, j1, jb, li, xi, yi, zi,
+                                                             sif, fx, fy, fz);
+                        }
 #pragma unroll
                         for (int q = 0; q < IPL; ++q) {
                             ax[q] += (double)fx[q];

@@ -161,16 +161,18 @@ __global__ __launch_bounds__(256) void k_grav_exact(const DBody *__restrict__ b,
 // for every a, so a skipped body leaves the sum as it was).
 constexpr int CMS_CH = 1024;
 
+// The central body's presence comes from the device record dcen (its
+// present field) or, when dcen is null, from cp.
 __global__ __launch_bounds__(256) void k_cm_seq(const double *__restrict__ cterm, int n,
                                                 const int32_t *__restrict__ ex,
-                                                const double *__restrict__ dcen,
+                                                const double *__restrict__ dcen, int cp,
                                                 double *__restrict__ cm)
 {
     // Rows padded by two doubles so that the three summing lanes read
     // different banks.
     __shared__ double st[2][3][CMS_CH + 2];
     const int tid = threadIdx.x, w = tid >> 6;
-    if (dcen[7] == 0.0) {
+    if (dcen ? dcen[7] == 0.0 : cp == 0) {
         if (tid < 3) cm[tid] = 0.0;
         return;
     }
@@ -332,9 +334,20 @@ ncg_status ncg_grav_exact_dev(ncg_ctx *ctx, const DBody *d_b, int32_t n, const i
         NCG_CHECK_LAUNCH();
     }
     if (d_cm_acc) {
-        hipLaunchKernelGGL(k_cm_seq, dim3(1), dim3(256), 0, st, d_cterm, (int)n, d_ex, d_cen,
+        hipLaunchKernelGGL(k_cm_seq, dim3(1), dim3(256), 0, st, d_cterm, (int)n, d_ex, d_cen, 0,
                            d_cm_acc);
         NCG_CHECK_LAUNCH();
     }
+    return NCG_OK;
+}
+
+// CM_acc from the per-body terms d_cterm of an exact evaluation with the
+// central body given on the host (cp: present), summed in body order.
+ncg_status ncg_cm_seq_launch(ncg_ctx *ctx, const double *d_cterm, int32_t n, const int32_t *d_ex,
+                             int cp, double *d_cm_acc, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_cm_seq, dim3(1), dim3(256), 0, st, d_cterm, (int)n, d_ex,
+                       (const double *)nullptr, cp, d_cm_acc);
+    NCG_CHECK_LAUNCH();
     return NCG_OK;
 }

@@ -38,7 +38,7 @@ namespace {
 
 constexpr int TJ = 128;     // bodies per j tile
 constexpr int NT = 256;     // threads per workgroup
-constexpr double TINY2 = 1e-200;  // floor of r^2 in the FP64 form (self pairs give 0 * finite)
+constexpr double TINY2 = 1e-200;  // floor of r^2 in the FP64 form (keeps a self pair's y finite)
 
 struct TileHdr {
     double ax, ay, az;  // anchor (box centre)
@@ -241,9 +241,10 @@ __global__ __launch_bounds__(NT) void k_tiles(const double4 *__restrict__ jd,
 // One interaction of the FP64 form: a += G m_j (x_j - x_i) / (r^2 + eps^2)^(3/2),
 // hardware 1/sqrt refined by one Newton step (v_rsq_f64 is good to ~2^-29;
 // one step brings it to FP64 rounding).  TINY2 enters the FMA chain with
-// eps^2, so that a body against itself (r = 0, no softening) has r^2 = TINY2
-// and adds exactly 0; above r^2 ~ 1e-184 it is below half an ulp and leaves
-// r^2 as it is (one FMA instead of a multiply and a floor).
+// eps^2, so that a body against itself (r = 0, no softening) has a finite
+// y = 1e100 (its mass is zeroed by near_term_self, so it adds exactly 0);
+// above r^2 ~ 1e-184 it is below half an ulp and leaves r^2 as it is (one
+// FMA instead of a multiply and a floor).
 template <int IPL>
 __device__ inline void near_term(const double4 p, double sj, const double *xi, const double *yi,
                                  const double *zi, const double *si, double *ax, double *ay,
@@ -261,6 +262,24 @@ __device__ inline void near_term(const double4 p, double sj, const double *xi, c
         ax[q] = fma(sc3, dx, ax[q]);
         ay[q] = fma(sc3, dy, ay[q]);
         az[q] = fma(sc3, dz, az[q]);
+    }
+}
+
+// near_term for a j slot that may be one of the lane's own bodies (i slot q
+// is i0 + q NT + lane): that pair's mass is zeroed.  Without softening its
+// r^2 is TINY2 and y^3 = 1e300, so G m y^3 would overflow to inf for
+// G m > ~1.8e8 and inf * 0 poison the sum; with the mass zeroed the term is
+// 0 * 1e300 * 0 = 0 whatever G m is.  Used only where the j tile overlaps
+// the lane's i slots.
+template <int IPL>
+__device__ inline void near_term_self(const double4 p, double sj, int js, int is0,
+                                      const double *xi, const double *yi, const double *zi,
+                                      const double *si, double *ax, double *ay, double *az)
+{
+#pragma unroll
+    for (int q = 0; q < IPL; ++q) {
+        const double4 pq = make_double4(p.x, p.y, p.z, js == is0 + q * NT ? 0.0 : p.w);
+        near_term<1>(pq, sj, xi + q, yi + q, zi + q, si + q, ax + q, ay + q, az + q);
     }
 }
 
@@ -354,7 +373,13 @@ __global__ __launch_bounds__(NT) void k_pairs(const double4 *__restrict__ jd,
                 }
                 __syncthreads();
             }
-            if (cnt == TJ) {
+            if (jb < i0 + NT * IPL && jb + TJ > i0) {
+                // The tile holds some of the workgroup's own bodies.
+                const int is0 = i0 + (int)threadIdx.x;
+                for (int jj = 0; jj < cnt; ++jj)
+                    near_term_self<IPL>(LDS ? ld[jj] : jd[jb + jj], LDS ? ls[jj] : jsig[jb + jj],
+                                        jb + jj, is0, xi, yi, zi, si, ax, ay, az);
+            } else if (cnt == TJ) {
 #pragma unroll 4
                 for (int jj = 0; jj < TJ; ++jj)
                     near_term<IPL>(LDS ? ld[jj] : jd[jb + jj], LDS ? ls[jj] : jsig[jb + jj], xi,
@@ -451,9 +476,18 @@ __global__ __launch_bounds__(NT) void k_small(const DBody *__restrict__ b, int n
             ss[threadIdx.x] = 1.0;
         }
         __syncthreads();
+        if (k >= base && k < base + NT) {
+            // This chunk holds body k itself: its own term is zeroed
+            // (near_term_self).
 #pragma unroll
-        for (int u = 0; u < NT / 64; ++u)
-            near_term<1>(sp[lane + 64 * u], ss[lane + 64 * u], &X, &Y, &Z, &Sk, &ax, &ay, &az);
+            for (int u = 0; u < NT / 64; ++u)
+                near_term_self<1>(sp[lane + 64 * u], ss[lane + 64 * u], base + lane + 64 * u, k,
+                                  &X, &Y, &Z, &Sk, &ax, &ay, &az);
+        } else {
+#pragma unroll
+            for (int u = 0; u < NT / 64; ++u)
+                near_term<1>(sp[lane + 64 * u], ss[lane + 64 * u], &X, &Y, &Z, &Sk, &ax, &ay, &az);
+        }
     }
     for (int off = 32; off > 0; off >>= 1) {
         ax += __shfl_xor(ax, off);
@@ -589,9 +623,6 @@ extern "C" ncg_status ncg_set_tiling(ncg_ctx *ctx, int32_t ipl, int32_t nsplit, 
     return NCG_OK;
 }
 
-ncg_status ncg_grav_exact_launch(ncg_ctx *ctx, const DBody *d_b, int32_t n, const int32_t *d_ex,
-                                 const ncg_central *c, double G, double soft, double *d_acc,
-                                 double *d_cterm, hipStream_t st);
 
 template <int MODE, int IPL>
 static void launch_pairs(int src, dim3 grid, hipStream_t st, const double4 *jd,
@@ -660,6 +691,28 @@ ncg_status ncg_tiled_order(ncg_ctx *ctx, const DBody *d_b, int32_t n, const int3
     return NCG_OK;
 }
 
+int ncg_small_partials(int32_t n) { return std::max(1, std::min(1024, (int)cdiv(n, NT))); }
+
+ncg_status ncg_small_launch(ncg_ctx *ctx, const DBody *d_b, int32_t n, const int32_t *d_ex,
+                            const ncg_central *c, const double *d_central, double G, double soft,
+                            double *d_acc, double *d_cterm, double *d_p3, double *d_cm_acc,
+                            hipStream_t st)
+{
+    NCG_REQUIRE(n >= 1 && n <= SMALL_N, "the direct form takes 1..%d bodies", SMALL_N);
+    const int cp = (c && c->present) ? 1 : 0;
+    hipLaunchKernelGGL(k_small, dim3(std::max(1, (int)cdiv(n, NT / 64))), dim3(NT), 0, st, d_b, n,
+                       d_ex, cp, cp ? c->x : 0.0, cp ? c->y : 0.0, cp ? c->z : 0.0,
+                       cp ? G * c->mass : 0.0, d_central, G, soft, d_acc, d_cterm);
+    NCG_CHECK_LAUNCH();
+    if (d_cm_acc) {
+        const int nb = ncg_small_partials(n);
+        hipLaunchKernelGGL(k_sum3, dim3(nb), dim3(NT), 0, st, d_cterm, n, d_ex, d_p3);
+        hipLaunchKernelGGL(k_sum3_final, dim3(1), dim3(NT), 0, st, d_p3, nb, d_cm_acc);
+        NCG_CHECK_LAUNCH();
+    }
+    return NCG_OK;
+}
+
 ncg_status ncg_tiled_eval(ncg_ctx *ctx, const DBody *d_b, int32_t n, const int32_t *d_ex,
                           const ncg_central *c, const double *d_central, double G, double soft,
                           ncg_precision prec, double *d_acc, double *d_cterm, double *d_cm_acc,
@@ -668,23 +721,11 @@ ncg_status ncg_tiled_eval(ncg_ctx *ctx, const DBody *d_b, int32_t n, const int32
     const bool want_cm = d_cm_acc != nullptr;
     if (n <= SMALL_N) {
         double *cterm = d_cterm ? d_cterm : (want_cm ? dbuf<double>(ctx, "cterm", 3 * (size_t)n) : nullptr);
-        const int nb = std::max(1, std::min(1024, (int)cdiv(n, NT)));
-        double *p3 = want_cm ? dbuf<double>(ctx, "sum3", 3 * (size_t)nb) : nullptr;
+        double *p3 = want_cm ? dbuf<double>(ctx, "sum3", 3 * (size_t)ncg_small_partials(n)) : nullptr;
         if (want_cm && (!cterm || !p3)) return NCG_ERR_NOMEM;
-        const int cp = (c && c->present) ? 1 : 0;
-        {
-            ProfScope ps(ctx, st, prec == NCG_F64_FAST ? "pairs_fast64" : "pairs_mixed");
-            hipLaunchKernelGGL(k_small, dim3(std::max(1, (int)cdiv(n, NT / 64))), dim3(NT), 0, st, d_b,
-                               n, d_ex, cp, cp ? c->x : 0.0, cp ? c->y : 0.0, cp ? c->z : 0.0,
-                               cp ? G * c->mass : 0.0, d_central, G, soft, d_acc, cterm);
-            NCG_CHECK_LAUNCH();
-        }
-        if (want_cm) {
-            hipLaunchKernelGGL(k_sum3, dim3(nb), dim3(NT), 0, st, cterm, n, d_ex, p3);
-            hipLaunchKernelGGL(k_sum3_final, dim3(1), dim3(NT), 0, st, p3, nb, d_cm_acc);
-            NCG_CHECK_LAUNCH();
-        }
-        return NCG_OK;
+        ProfScope ps(ctx, st, prec == NCG_F64_FAST ? "pairs_fast64" : "pairs_mixed");
+        return ncg_small_launch(ctx, d_b, n, d_ex, c, d_central, G, soft, d_acc, cterm, p3,
+                                d_cm_acc, st);
     }
     const TiledShape t = tiled_shape(ctx, n, prec);
     Scal *sc = dbuf<Scal>(ctx, "t_scal", 1);
@@ -754,16 +795,8 @@ ncg_status ncg_gravity_dev_impl(ncg_ctx *ctx, const DBody *d_b, int32_t n, const
             r = ncg_grav_exact_launch(ctx, d_b, n, d_ex, c, G, soft, d_acc, cterm, st);
         }
         if (r != NCG_OK) return r;
-        if (cp && d_cm_acc) {
-            // Device tree sum; the host entry point redoes this sum in the
-            // reference's sequential order from the per-body terms.
-            const int nb = std::min(1024, (int)cdiv(n, NT));
-            double *p3 = dbuf<double>(ctx, "sum3", 3 * (size_t)nb);
-            if (!p3) return NCG_ERR_NOMEM;
-            hipLaunchKernelGGL(k_sum3, dim3(nb), dim3(NT), 0, st, cterm, n, d_ex, p3);
-            hipLaunchKernelGGL(k_sum3_final, dim3(1), dim3(NT), 0, st, p3, nb, d_cm_acc);
-            NCG_CHECK_LAUNCH();
-        }
+        // CM_acc in the reference's order: the terms added body by body.
+        if (cp && d_cm_acc) return ncg_cm_seq_launch(ctx, cterm, n, d_ex, 1, d_cm_acc, st);
         return NCG_OK;
     }
     NCG_REQUIRE(prec == NCG_F64_FAST || prec == NCG_MIXED, "unknown precision %d", (int)prec);

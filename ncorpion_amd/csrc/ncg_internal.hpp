@@ -54,6 +54,7 @@ struct MeshState {
     double gam = 0.0, cube = 0.0;
     std::vector<ncg_pair> extra_pairs;    // big-body pairs with both ids > 2
     std::vector<ncg_pair> big_pairs_all;  // every pair registered by big visits
+    std::vector<char> big_scan;           // per entry: 1 the visitor's own pair, 0 a block pair
     std::vector<BigVisit> big_visits;
     std::vector<MeshEvent> events;        // collisions in resolution order
     std::vector<int32_t> frag_ids;        // bodies of the fragmentation blocks
@@ -244,3 +245,19 @@ ncg_status ncg_gravity_dev_impl(ncg_ctx *ctx, const DBody *d_bodies, int32_t n,
                                 const int32_t *d_exists, const ncg_central *central, double G,
                                 double soft, ncg_precision prec, double *d_acc,
                                 double *d_cterm, double *d_cm_acc, hipStream_t s);
+// Direct form of the fast gravity forms for 1..4096 bodies (k_small): one
+// wavefront per body; with d_cm_acc, CM_acc as a fixed-shape tree sum of the
+// terms d_cterm (3n) through d_p3 (3 * ncg_small_partials(n) doubles).
+int ncg_small_partials(int32_t n);
+ncg_status ncg_small_launch(ncg_ctx *ctx, const DBody *d_b, int32_t n, const int32_t *d_ex,
+                            const ncg_central *central, const double *d_central, double G,
+                            double soft, double *d_acc, double *d_cterm, double *d_p3,
+                            double *d_cm_acc, hipStream_t st);
+// Exact form (gravity_exact.hip): the field with per-body CM_acc terms d_cterm
+// (3n, written when the central body is present), and CM_acc summed from
+// them in body order.
+ncg_status ncg_grav_exact_launch(ncg_ctx *ctx, const DBody *d_b, int32_t n, const int32_t *d_ex,
+                                 const ncg_central *c, double G, double soft, double *d_acc,
+                                 double *d_cterm, hipStream_t st);
+ncg_status ncg_cm_seq_launch(ncg_ctx *ctx, const double *d_cterm, int32_t n, const int32_t *d_ex,
+                             int cp, double *d_cm_acc, hipStream_t st);

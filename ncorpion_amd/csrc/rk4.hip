@@ -20,6 +20,7 @@ struct Rk4Graph {
     int32_t n = 0;
     uint64_t buf_gen = 0;
     int prec = -1;
+    bool central_present = false;  // the captured stages include the central body or not
     double G = 0, soft = 0, dt = 0;
     hipStream_t stream = nullptr;
     hipGraph_t graph = nullptr;
@@ -195,7 +196,7 @@ extern "C" ncg_status ncg_rk4_steps_dev(ncg_ctx *ctx, ncg_body *d_bodies, int32_
         Rk4Graph *g = ctx->rk4;
         const bool same = g && g->valid && g->key_b == y0 && g->key_c == d_central &&
                           g->key_ex == d_exists && g->n == n && g->prec == (int)prec &&
-                          g->buf_gen == ctx->buf_gen &&
+                          g->central_present == cp && g->buf_gen == ctx->buf_gen &&
                           g->G == grav_const && g->soft == softening && g->dt == dt &&
                           g->stream == st;
         if (!same) {
@@ -227,6 +228,7 @@ extern "C" ncg_status ncg_rk4_steps_dev(ncg_ctx *ctx, ncg_body *d_bodies, int32_
             g->buf_gen = ctx->buf_gen;
             g->n = n;
             g->prec = (int)prec;
+            g->central_present = cp;
             g->G = grav_const;
             g->soft = softening;
             g->dt = dt;

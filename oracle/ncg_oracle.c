@@ -19,6 +19,8 @@
  *   oracle_vector_field_brute  golden vectors from the compiled reference
  *                              (tests/golden), Newton's third law, two-body
  *                              closed form.
+ *   oracle_vector_field_rows   equal, bit for bit, to the rows of
+ *                              oracle_vector_field_brute (tests/test_oracle_golden.py).
  *   oracle_vector_field_mesh   golden vectors from the compiled reference.
  *   oracle_closest_approach    analytic head-on / tangent / receding cases and
  *                              golden vectors.
@@ -111,6 +113,60 @@ void oracle_vector_field_brute(const double *bodies, int n, const int *exists,
 
 /* One symmetric pair update of the mesh branch (physics.c:208-221 pattern):
  * body `a` (outer) -= G*m_b*D/D3, body `b` += G*m_a*D/D3 with D = X_a - X_b. */
+/* The same field for the bodies rows[0..nrows) only, each summed over all n
+ * bodies: acc[3 r ..] receives row r's acceleration, bit-identical to what
+ * oracle_vector_field_brute writes for that body (the sum vector_field forms
+ * for body k: its central term, minus the terms of p < k in ascending p while
+ * k is the outer body, then plus the terms it receives from every q > k in
+ * ascending q, physics.c:175-201).  O(nrows n): the checker of large
+ * fields on a sample of bodies.  Rows of absent bodies are left untouched. */
+void oracle_vector_field_rows(const double *bodies, int n, const int *exists,
+                              int central_present, const double *central, double G,
+                              double softening_parameter, const int *rows, int nrows, double *acc)
+{
+    const double mu = G * (central_present ? central[3] : 0.);
+    for (int r = 0; r < nrows; r++) {
+        const int k = rows[r];
+        if (k < 0 || k >= n || !exists[k]) continue;
+        const double X = BX(bodies, k), Y = BY(bodies, k), Z = BZ(bodies, k);
+        const double Rk = BR(bodies, k);
+        double aX = 0., aY = 0., aZ = 0.;
+        if (central_present) {
+            double DX = X - central[0], DY = Y - central[1], DZ = Z - central[2];
+            double rk = sqrt(DX * DX + DY * DY + DZ * DZ);
+            double r3 = rk * rk * rk;
+            aX = -mu * DX / r3;
+            aY = -mu * DY / r3;
+            aZ = -mu * DZ / r3;
+        }
+        for (int p = 0; p < k; p++) {
+            if (!exists[p]) continue;
+            double mp = BM(bodies, p);
+            double dx = X - BX(bodies, p), dy = Y - BY(bodies, p), dz = Z - BZ(bodies, p);
+            double softening = softening_parameter * (Rk + BR(bodies, p));
+            double D = sqrt(dx * dx + dy * dy + dz * dz + softening * softening);
+            double D3 = D * D * D;
+            aX -= G * mp * dx / D3;
+            aY -= G * mp * dy / D3;
+            aZ -= G * mp * dz / D3;
+        }
+        for (int q = k + 1; q < n; q++) {
+            if (!exists[q]) continue;
+            double mq = BM(bodies, q);
+            double dx = BX(bodies, q) - X, dy = BY(bodies, q) - Y, dz = BZ(bodies, q) - Z;
+            double softening = softening_parameter * (BR(bodies, q) + Rk);
+            double D = sqrt(dx * dx + dy * dy + dz * dz + softening * softening);
+            double D3 = D * D * D;
+            aX += G * mq * dx / D3;
+            aY += G * mq * dy / D3;
+            aZ += G * mq * dz / D3;
+        }
+        acc[3 * (size_t)r + 0] = aX;
+        acc[3 * (size_t)r + 1] = aY;
+        acc[3 * (size_t)r + 2] = aZ;
+    }
+}
+
 static void pair_update(const double *bodies, int a, int b, double G, double s,
                         double *acc)
 {

@@ -23,7 +23,8 @@
  *      4 closest_approach on every i<j (collision.c:63), 5 get_neighbours_mesh
  *      then vector_field with the pairs it registered (each timed: gnm_seconds
  *      and op_seconds on stderr; no pairs written); op | 0x100 writes only the
- *      header and CM_acc (timing runs).
+ *      header and CM_acc (timing runs); op | 0x200 leaves the pair list out
+ *      (n_pairs 0: large mesh passes, whose list runs to gigabytes).
  */
 #define _POSIX_C_SOURCE 199309L
 #include <stdio.h>
@@ -75,6 +76,7 @@ int main(int argc, char **argv)
     if (hdr[0] != 0x4e434750) return 2;
     /* op | 0x100: write the header and CM_acc only (timing runs). */
     const int header_only = (hdr[1] & 0x100) != 0;
+    const int no_pairs = (hdr[1] & 0x200) != 0;
     int op = hdr[1] & 0xff, n = hdr[2], npin = hdr[3];
     double prm[8], cm[8];
     rd(prm, sizeof(double), 8, f);
@@ -192,7 +194,7 @@ int main(int argc, char **argv)
 
     FILE *o = fopen(argv[2], "wb");
     if (!o) return 2;
-    int np_out = (mesh_bool && (op == 2 || op == 3)) ? how_many_pairs : 0;
+    int np_out = (mesh_bool && (op == 2 || op == 3) && !no_pairs) ? how_many_pairs : 0;
     int ohdr[8] = {op, n, n_coll_log, np_out, 0, 0, 0, n_ca};
     if (!brute_force_bool) {
         ohdr[4] = total_neighbours;

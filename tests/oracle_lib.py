@@ -28,6 +28,7 @@ def lib():
         _lib = ctypes.CDLL(LIB)
         _lib.oracle_vector_field_brute.argtypes = [P, I, P, I, P, D, D, P, P]
         _lib.oracle_vector_field_mesh.argtypes = [P, I, P, I, P, D, D, P, L, P, P]
+        _lib.oracle_vector_field_rows.argtypes = [P, I, P, I, P, D, D, P, I, P]
         _lib.oracle_closest_approach.argtypes = [P, P, I, I, D, P, P]
         _lib.oracle_closest_approach.restype = I
         _lib.oracle_brute_force.argtypes = [P, I, P, P, D, I, D, I, P, L]
@@ -67,6 +68,20 @@ def vector_field_brute(bodies, exists=None, central=None, G=1.0, soft=0.0):
     lib().oracle_vector_field_brute(_p(b), n, _p(e), int(central is not None), _p(c), G, soft,
                                     _p(acc), _p(cm))
     return acc, cm
+
+
+def vector_field_rows(bodies, rows, exists=None, central=None, G=1.0, soft=0.0):
+    """Field of the bodies `rows` only (each summed over all bodies), as
+    vector_field_brute gives it for those rows."""
+    b = np.ascontiguousarray(bodies, dtype=np.float64)
+    n = b.shape[0]
+    e = _ex(exists, n)
+    r = np.ascontiguousarray(rows, dtype=np.int32)
+    acc = np.zeros((len(r), 3))
+    c = np.array(central if central is not None else (0, 0, 0, 0), dtype=np.float64)
+    lib().oracle_vector_field_rows(_p(b), n, _p(e), int(central is not None), _p(c), G, soft,
+                                   _p(r), len(r), _p(acc))
+    return acc
 
 
 def vector_field_mesh(bodies, pairs, exists=None, central=None, G=1.0, soft=0.0):

@@ -128,6 +128,44 @@ def test_device_entry_point_matches_host(ctx):
     assert np.array_equal(ct.cpu().numpy(), cm_h)
 
 
+@pytest.mark.parametrize("n", [1000, 4096, 4097, 8192])
+@pytest.mark.parametrize("prec,tol", [("fast64", 1e-12), ("mixed", 1e-6)])
+def test_no_softening_with_large_gm(ctx, n, prec, tol):
+    # Without softening (NcorpiON's default) a body's pair with itself has
+    # r = 0.  With G m far above 1 (SI or cgs units: here G m spans 1e7..1e9)
+    # that term must still add exactly nothing rather than inf * 0.  Both
+    # sides of the direct form's 4096-body limit.
+    b = c3_disk(n, 15)
+    G = 1e18
+    ref, _ = orc.vector_field_brute(b, None, None, G, 0.0)
+    acc, _ = ctx.gravity_allpairs(b, None, None, G, 0.0, prec)
+    assert np.all(np.isfinite(acc))
+    mx, _ = rel_err(acc, ref)
+    assert mx <= tol, mx
+
+
+def test_exact_device_entry_point_cm_acc_in_reference_order(ctx):
+    # The device entry point sums CM_acc body by body as vector_field does
+    # (physics.c:141-146): bit-identical to the oracle, holes included.
+    import torch
+    n = 3000
+    b = ic.disk(n, seed=16, mass=1e-6, radius=1e-3)
+    b[:, 6] *= np.random.default_rng(16).uniform(0.5, 2.0, n)
+    ex = (np.random.default_rng(17).uniform(size=n) > 0.1).astype(np.int32)
+    cen = (1e-3, 2e-3, -1e-3, 1.0)
+    ref, rcm = orc.vector_field_brute(b, ex, cen, 1.0, 0.01)
+    bt = torch.from_numpy(b).cuda()
+    et = torch.from_numpy(ex).cuda()
+    at = torch.zeros((n, 3), dtype=torch.float64, device="cuda")
+    ct = torch.zeros(3, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    ctx.gravity_allpairs_dev(bt, at, et, cen, 1.0, 0.01, "exact", ct)
+    ctx.synchronize()
+    keep = ex.astype(bool)
+    assert np.array_equal(bits(at.cpu().numpy()[keep]), bits(ref[keep]))
+    assert np.array_equal(bits(ct.cpu().numpy()), bits(rcm))
+
+
 def test_empty_and_single(ctx):
     acc, cm = ctx.gravity_allpairs(np.zeros((0, 8)), None, (0, 0, 0, 1.0), 1.0, 0.0, "mixed")
     assert acc.shape == (0, 3)

@@ -108,6 +108,32 @@ def test_ensemble_gravity_matches_per_system_oracle(ctx, prec):
             assert np.all(err <= bound), (err / bound).max()
 
 
+@pytest.mark.parametrize("prec,tol", [("fast64", 1e-12), ("mixed", 5e-6)])
+def test_ensemble_no_softening_with_large_gm(ctx, prec, tol):
+    # Without softening a body's pair with itself has r = 0; with G m up to
+    # 1e3 times the FP32 form's overflow point it must still add nothing.
+    torch = torch_dev()
+    sizes = [7, 300, 1000, 2049]
+    systems, allb, off = make_ensemble(sizes, seed0=50, mass=1e-6)
+    S = len(sizes)
+    cen = np.zeros((S, 8))
+    G = 1e15
+    bt = torch.from_numpy(allb).cuda()
+    ct = torch.from_numpy(cen).cuda()
+    at = torch.zeros((len(allb), 3), dtype=torch.float64, device="cuda")
+    cm = torch.zeros((S, 3), dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    ctx.ensemble_gravity_dev(bt, off, ct, at, cm, G, 0.0, prec)
+    ctx.synchronize()
+    acc = at.cpu().numpy()
+    assert np.all(np.isfinite(acc))
+    for s, sysb in enumerate(systems):
+        ref, _ = orc.vector_field_brute(sysb, None, None, G, 0.0)
+        got = acc[off[s]:off[s + 1]]
+        err = np.linalg.norm(got - ref, axis=1) / np.linalg.norm(ref, axis=1)
+        assert err.max() <= tol, (s, err.max())
+
+
 def test_ensemble_rk4_matches_single_systems_and_graph(ctx):
     torch = torch_dev()
     rng = np.random.default_rng(7)

@@ -60,6 +60,21 @@ def test_closest_approach_every_pair():
     assert g["hit"].sum() > 10
 
 
+@pytest.mark.parametrize("name", ["vf_bf_g1_n257_holes", "vf_bf_soft_n64", "vf_bf_nocm_n200",
+                                  "vf_c3_n4096"])
+def test_vector_field_rows_equal_reference_rows(name):
+    # The row-subset form (the checker of fields too large for the full
+    # O(N^2) oracle) against the reference's own accelerations of those rows.
+    g = load(name)
+    cen = central4(g["central"]) if "central" in g else None
+    n = len(g["bodies"])
+    ex = g.get("exists")
+    keep = ex.astype(bool) if ex is not None else np.ones(n, bool)
+    rows = np.flatnonzero(keep)[np.random.default_rng(n).permutation(int(keep.sum()))[:97]]
+    acc = orc.vector_field_rows(g["bodies"], rows, ex, cen, float(g["G"]), float(g["soft"]))
+    assert np.array_equal(bits(acc), bits(g["acc"][rows]))
+
+
 @pytest.mark.parametrize("name", ["bf_inelastic_n300", "bf_elastic_once_n200",
                                   "bf_inelastic125_n250"])
 def test_brute_force_pass(name):
