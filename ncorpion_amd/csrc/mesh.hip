@@ -1454,8 +1454,11 @@ struct HostCells {
     std::map<unsigned, std::vector<int>> added;  // cell -> relocated bodies (ascending)
     bool loaded = false;
     long d = 0;
+    // Key at insertion time; NOKEY for bodies appended during the pass,
+    // which are never inserted.
     unsigned eff(int b) const
     {
+        if (b < 0 || b >= (int)key.size()) return NOKEY;
         auto it = reloc.find(b);
         return it == reloc.end() ? key[b] : it->second;
     }
@@ -1599,6 +1602,7 @@ void reset_pass(MeshState &M, int32_t n)
     M.extra_pairs.clear();
     M.big_visits.clear();
     M.big_pairs_all.clear();
+    M.big_scan.clear();
     M.events.clear();
     M.frag_ids.clear();
     M.removed.clear();
@@ -1639,79 +1643,103 @@ ncg_status load_cells(ncg_ctx *ctx, HostCells &hc, int32_t n, int ins, const uns
 }
 
 // Term lists of the bodies that full fragmentations touched, for the exact
-// field (physics.c:347-378 sums each body's pairs in list order): for each
-// survivor (id > 2), its own visit (cells 26..0, ascending id below it) and
-// the later visits that gather it (ascending visitor id; within a visit,
-// stencil index 26..0), with each fragmentation's entry replaced by the
-// block's appended bodies (physics.c:1002-1010: pairs (id[p], id[q]), q < p);
-// for each appended body, the other bodies of its block in block order.
+// field (physics.c:347-378 sums each body's pairs in list order).  For each
+// body X of a fragmentation block (survivor and appended bodies, ids > 2),
+// X's entries of the pair list in order, i.e. by visit, then by position in
+// the visit:
+//   * visits of bodies larger than the mesh size: their recorded pairs
+//     (block pairs included) that hold X;
+//   * when X was inserted: its own visit (cells 26..0, ascending id below
+//     it) and the later visits that gather it (stencil index 26..0 of their
+//     own walk), each fragmentation's entry replaced by the block's other
+//     bodies (physics.c:1002-1010: pairs (id[p], id[q]), q < p);
+//   * for an appended body, the entry of the small visit whose block made it:
+//     the other bodies of the block in block order.
 // Absent partners stay listed: the field skips them as physics.c:351 does.
-// FP64-fast and mixed forms add the block pairs after the rest.
+// FP64-fast and mixed forms add the block pairs of small visits after the
+// rest (big visits' pairs are among the extra pairs already).
 ncg_status explicit_lists(ncg_ctx *ctx, MeshState &M, HostCells &hc, int32_t n, int ins,
                           const unsigned *key, const unsigned *skey, const int *sid)
 {
-    std::map<std::tuple<int, int, int>, const MeshEvent *> blk;  // (visit, l, partner)
-    std::set<int> surv;
+    std::map<std::tuple<int, int, int>, const MeshEvent *> blk;  // small visits: (visit, l, partner)
+    std::set<int> targets;
     for (const MeshEvent &e : M.events)
         if (e.kind == MeshEvent::BLOCK) {
-            blk[std::make_tuple(e.k, e.l, e.p)] = &e;
-            if (M.frag_ids[e.first] > 2) surv.insert(M.frag_ids[e.first]);
+            if (e.l >= 0) blk[std::make_tuple(e.k, e.l, e.p)] = &e;
+            for (int q = 0; q < e.count; ++q)
+                if (M.frag_ids[e.first + q] > 2) targets.insert(M.frag_ids[e.first + q]);
         }
-    if (blk.empty()) return NCG_OK;
+    if (targets.empty()) return NCG_OK;
     ncg_status r = load_cells(ctx, hc, n, ins, key, skey, sid);
     if (r != NCG_OK) return r;
     const long d = hc.d;
     std::vector<int> &P = M.x_partners;
-    auto target = [&](int id, int first) {
-        M.x_targets.push_back(id);
-        M.x_targets.push_back(first);
-        M.x_targets.push_back((int)P.size() - first);
-    };
-    auto emit = [&](int v, int l, int p, int other) {
+    // (visit, position in the visit, order among equal positions, partner)
+    std::vector<std::tuple<int, long, int, int>> ent;
+    auto emit = [&](int v, int l, int p, int other, int self) {
+        const long pos = (long)(26 - l) * (1L << 31) + p;
         auto it = blk.find(std::make_tuple(v, l, p));
         if (it == blk.end()) {
-            P.push_back(other);
+            ent.emplace_back(v, pos, 0, other);
             return;
         }
         const MeshEvent &e = *it->second;
-        for (int q = 1; q < e.count; ++q) P.push_back(M.frag_ids[e.first + q]);
+        for (int q = 0; q < e.count; ++q)
+            if (M.frag_ids[e.first + q] != self)
+                ent.emplace_back(v, pos, q, M.frag_ids[e.first + q]);
     };
     std::vector<int> mem;
-    std::vector<std::pair<int, int>> later;
-    for (int A : surv) {
-        const int first = (int)P.size();
-        const unsigned ka = hc.eff(A);
-        if (ka != NOKEY) {
+    for (int X : targets) {
+        ent.clear();
+        for (const BigVisit &bv : M.big_visits)
+            for (int q = 0; q < bv.count; ++q) {
+                const ncg_pair &pq = M.big_pairs_all[bv.first + q];
+                if (pq.fst == X) ent.emplace_back(bv.k, (long)q, 0, pq.snd);
+                else if (pq.snd == X) ent.emplace_back(bv.k, (long)q, 0, pq.fst);
+            }
+        const unsigned kx = hc.eff(X);
+        if (kx != NOKEY) {
             for (int l = 26; l >= 0; --l) {
-                hc.members((unsigned)((long)ka + stencil_offset(l, d)), mem);
+                hc.members((unsigned)((long)kx + stencil_offset(l, d)), mem);
                 for (int p : mem) {
-                    if (p >= A) break;
-                    emit(A, l, p, p);
+                    if (p >= X) break;
+                    emit(X, l, p, p, X);
                 }
             }
-            later.clear();
             for (int l = 0; l < 27; ++l) {
-                hc.members((unsigned)((long)ka + stencil_offset(l, d)), mem);
+                hc.members((unsigned)((long)kx + stencil_offset(l, d)), mem);
                 for (int q : mem)
-                    if (q > A) later.push_back({q, -(26 - l)});  // A at index 26 - l of q's visit
+                    if (q > X) emit(q, 26 - l, X, q, X);  // X at index 26 - l of q's visit
             }
-            std::sort(later.begin(), later.end());
-            for (const auto &ql : later) emit(ql.first, -ql.second, A, ql.first);
         }
-        target(A, first);
+        for (const auto &kv : blk) {
+            const MeshEvent &e = *kv.second;
+            for (int a = 1; a < e.count; ++a)
+                if (M.frag_ids[e.first + a] == X) {
+                    const long pos = (long)(26 - e.l) * (1L << 31) + e.p;
+                    for (int b = 0; b < e.count; ++b)
+                        if (b != a) ent.emplace_back(e.k, pos, b, M.frag_ids[e.first + b]);
+                }
+        }
+        std::stable_sort(ent.begin(), ent.end(), [](const std::tuple<int, long, int, int> &u,
+                                                    const std::tuple<int, long, int, int> &w) {
+            if (std::get<0>(u) != std::get<0>(w)) return std::get<0>(u) < std::get<0>(w);
+            if (std::get<1>(u) != std::get<1>(w)) return std::get<1>(u) < std::get<1>(w);
+            return std::get<2>(u) < std::get<2>(w);
+        });
+        const int first = (int)P.size();
+        for (const auto &t : ent) P.push_back(std::get<3>(t));
+        M.x_targets.push_back(X);
+        M.x_targets.push_back(first);
+        M.x_targets.push_back((int)P.size() - first);
     }
     for (const auto &kv : blk) {
         const MeshEvent &e = *kv.second;
-        for (int a = 1; a < e.count; ++a) {
-            const int first = (int)P.size();
-            for (int b = 0; b < e.count; ++b)
-                if (b != a) P.push_back(M.frag_ids[e.first + b]);
-            target(M.frag_ids[e.first + a], first);
+        for (int a = 1; a < e.count; ++a)
             for (int b = 0; b < a; ++b) {
                 const int u = M.frag_ids[e.first + a], w = M.frag_ids[e.first + b];
                 if (u > 2 && w > 2) M.extra_pairs.push_back({u, w});
             }
-        }
     }
     return NCG_OK;
 }
@@ -1807,6 +1835,10 @@ extern "C" ncg_status ncg_mesh_pass(ncg_ctx *ctx, const ncg_body *bodies, int32_
         NCG_HIP(hipStreamSynchronize(st));
         std::sort(big.begin(), big.end());
     }
+    // Visits of bodies larger than the mesh size still ahead (a body's class
+    // is its radius at its visit, collision.c:342-344: an outcome may move an
+    // unvisited body across gam).
+    std::set<int> bigset(big.begin(), big.end());
     stats->total_neighbours = (int64_t)total;
     stats->how_many_small = mode == 0 ? hc4[1] : 0;
     stats->how_many_big = mode == 0 ? hc4[2] : 0;
@@ -1865,13 +1897,64 @@ extern "C" ncg_status ncg_mesh_pass(ncg_ctx *ctx, const ncg_body *bodies, int32_
         return NCG_OK;
     };
     auto mark_dirty = [&](int b, int kv) -> ncg_status {
-        if (dirty[b]) return NCG_OK;
+        // Bodies appended during the pass are never inserted: nobody's neighbour.
+        if (b >= n || dirty[b]) return NCG_OK;
         dirty[b] = 1;
         return schedule(b, kv);
     };
     int32_t born[64];
-    // Small (radius <= gam) at the start of the pass: not in the big list.
-    auto was_small = [&](int b) { return !std::binary_search(big.begin(), big.end(), b); };
+    int64_t small_adj = 0, big_adj = 0;
+    // Class at the visit of the bodies an outcome changed before it: 0 small,
+    // 1 big, 2 removed (not visited); others keep their start-of-pass class.
+    std::map<int, int> cls;
+    auto class_of = [&](int b) {
+        auto it = cls.find(b);
+        if (it != cls.end()) return it->second;
+        return bigset.count(b) ? 1 : 0;
+    };
+    // Body b (kv < b < n, not yet visited) was changed by an outcome of visit
+    // kv.  At its visit the reference reads its live state: skipped when it no
+    // longer exists, inserted at its current cell when small, a scan of every
+    // body when big (collision.c:339-344).  Its visit is then decided on the
+    // host, and the later visits that gather it re-test it.
+    auto unvisited_changed = [&](int b, int kv) -> ncg_status {
+        ncg_status r = load_cells(ctx, hc, n, ins, key, skey, sid);
+        if (r != NCG_OK) return r;
+        const int was = class_of(b);
+        if (!live(b)) {
+            if (was == 2) return NCG_OK;
+            if (hc.eff(b) != NOKEY) hc.relocate(b, NOKEY);
+            (was == 0 ? small_adj : big_adj) -= 1;
+            bigset.erase(b);
+            full_visit.erase(b);
+            cls[b] = 2;
+            return NCG_OK;
+        }
+        if (bodies[b].radius <= gam) {
+            const unsigned nk = host_key(bodies[b].x, bodies[b].y, bodies[b].z, gam,
+                                         collision_cube, d);
+            const bool moved_cell = nk != hc.eff(b);
+            if (moved_cell) hc.relocate(b, nk);
+            if (was == 1) {
+                bigset.erase(b);
+                ++small_adj;
+                --big_adj;
+            }
+            cls[b] = 0;
+            full_visit.insert(b);
+            if (moved_cell || was == 1) return schedule(b, kv);
+            return NCG_OK;
+        }
+        if (hc.eff(b) != NOKEY) hc.relocate(b, NOKEY);
+        if (was == 0) {
+            bigset.insert(b);
+            --small_adj;
+            ++big_adj;
+        }
+        cls[b] = 1;
+        full_visit.erase(b);
+        return NCG_OK;
+    };
     // Resolves entry (k, p) of visit kv, found at stencil index l (-1: a big
     // visit), and records what the outcome leaves in the pair list.
     auto collide = [&](int p, int k, int kv, int l) -> ncg_status {
@@ -1898,16 +1981,8 @@ extern "C" ncg_status ncg_mesh_pass(ncg_ctx *ctx, const ncg_body *bodies, int32_
                                 "slot %d (mesh() appends, structure.c:857-861)",
                                 born[q], p, k, ncol + q);
         const bool lp = live(p), lk = live(k);
-        // Bodies larger than the mesh size scan every body (collision.c:386):
-        // there, removed and created bodies would change later big visits.
-        if (!big.empty() && (nborn != 0 || !lp || !lk ||
-                             (p > kv && (bodies[p].radius <= gam) != was_small(p)) ||
-                             (k > kv && (bodies[k].radius <= gam) != was_small(k))))
-            return ncg_fail(ctx, NCG_ERR_INVALID,
-                            "mesh(): the outcome of collision (%d, %d) removed, created or resized "
-                            "bodies across the mesh size in a pass with bodies larger than the mesh "
-                            "size; only outcomes that keep both bodies are supported there",
-                            p, k);
+        if ((int32_t)rm_at.size() < ncol + nborn && !rm_at.empty())
+            rm_at.resize((size_t)(ncol + nborn), INT32_MAX);
         if (lp && lk) {
             if (nborn != 0)
                 return ncg_fail(ctx, NCG_ERR_INVALID,
@@ -1924,7 +1999,7 @@ extern "C" ncg_status ncg_mesh_pass(ncg_ctx *ctx, const ncg_body *bodies, int32_
             if (r != NCG_OK) return r;
             return mark_dirty(k, kv);
         }
-        if (rm_at.empty()) rm_at.assign((size_t)n, INT32_MAX);
+        if (rm_at.empty()) rm_at.assign((size_t)(ncol + nborn), INT32_MAX);
         if (!lp && !lk) {
             // A super-catastrophic disruption that leaves no body
             // (physics.c:819-823) registers nothing.
@@ -1982,50 +2057,62 @@ extern "C" ncg_status ncg_mesh_pass(ncg_ctx *ctx, const ncg_body *bodies, int32_
         }
         return NCG_OK;
     };
-    size_t ci = 0, bi = 0;
+    size_t ci = 0;
     while (true) {
         const int kc = ci < cand.size() ? cand[ci].x : INT32_MAX;
-        const int kb = bi < big.size() ? big[bi] : INT32_MAX;
+        const int kb = bigset.empty() ? INT32_MAX : *bigset.begin();
         const int kp = pending.empty() ? INT32_MAX : pending.top().first;
         const int kf = full_visit.empty() ? INT32_MAX : *full_visit.begin();
         const int k = std::min(std::min(kc, kb), std::min(kp, kf));
         if (k == INT32_MAX) break;
         if (k == kb) {
-            // Body larger than the mesh size (collision.c:381-414).
-            ++bi;
+            // Body larger than the mesh size (collision.c:381-414): every
+            // existing body, small or a larger id, up to the live largest_id
+            // (bodies appended during the pass included), at its live radius.
+            bigset.erase(bigset.begin());
+            while (ci < cand.size() && cand[ci].x == k) ++ci;
+            while (!pending.empty() && pending.top().first == k) pending.pop();
             if (!live(k)) continue;
             std::vector<ncg_pair> visit;
-            for (int32_t p = 0; p < n; ++p) {
+            std::vector<char> scan;
+            for (int32_t p = 0; p < ncol; ++p) {
                 if (!(live(p) && p != k)) continue;
                 const double Rp = bodies[p].radius;
                 if (!(Rp < gam || (Rp >= gam && p > k))) continue;
                 ++htests;
                 visit.push_back({k, p});
+                scan.push_back(1);
                 if (ops->test(ops->user, p, k)) {
                     ncg_status r = collide(p, k, k, -1);
                     if (r != NCG_OK) return r;
                     const MeshEvent &ev = M.events.back();
-                    if (ev.kind == MeshEvent::NONE) visit.pop_back();
-                    else visit.back() = ev.reg;
-                    if (p > k && Rp <= gam) {
-                        // p is inserted later, at its new position: its cell
-                        // may change, and its own visit is decided on the host.
-                        r = load_cells(ctx, hc, n, ins, key, skey, sid);
+                    if (ev.kind == MeshEvent::NONE) {
+                        visit.pop_back();
+                        scan.pop_back();
+                    } else if (ev.kind == MeshEvent::PAIR) {
+                        visit.back() = ev.reg;
+                    } else {
+                        // Full fragmentation: the pairs of its bodies
+                        // (physics.c:1002-1010), survivor first.
+                        visit.pop_back();
+                        scan.pop_back();
+                        for (int a = 1; a < ev.count; ++a)
+                            for (int b2 = 0; b2 < a; ++b2) {
+                                visit.push_back({M.frag_ids[ev.first + a], M.frag_ids[ev.first + b2]});
+                                scan.push_back(0);
+                            }
+                    }
+                    if (p > k && p < n) {
+                        r = unvisited_changed(p, k);
                         if (r != NCG_OK) return r;
-                        const unsigned nk = host_key(bodies[p].x, bodies[p].y, bodies[p].z, gam,
-                                                     collision_cube, d);
-                        if (nk != hc.eff(p)) {
-                            hc.relocate(p, nk);
-                            r = schedule(p, k);
-                            if (r != NCG_OK) return r;
-                        }
-                        full_visit.insert(p);
                     }
                 }
             }
             M.big_visits.push_back({k, (int)M.big_pairs_all.size(), (int)visit.size()});
-            for (const ncg_pair &pq : visit) {
+            for (size_t q = 0; q < visit.size(); ++q) {
+                const ncg_pair &pq = visit[q];
                 M.big_pairs_all.push_back(pq);
+                M.big_scan.push_back(scan[q]);
                 if (pq.fst > 2 && pq.snd > 2) M.extra_pairs.push_back(pq);
             }
             continue;
@@ -2035,6 +2122,13 @@ extern "C" ncg_status ncg_mesh_pass(ncg_ctx *ctx, const ncg_body *bodies, int32_
         while (!pending.empty() && pending.top().first == k) {
             owed.push_back(pending.top().second);
             pending.pop();
+        }
+        if (!live(k) || class_of(k) != 0) {
+            // Removed before its visit, or now larger than the mesh size: no
+            // small visit (collision.c:340-344).
+            full_visit.erase(k);
+            while (ci < cand.size() && cand[ci].x == k) ++ci;
+            continue;
         }
         if (full_visit.count(k)) {
             full_visit.erase(k);
@@ -2118,6 +2212,8 @@ extern "C" ncg_status ncg_mesh_pass(ncg_ctx *ctx, const ncg_body *bodies, int32_
         stats->total_neighbours = (int64_t)ht[0];
     }
     M.n_end = ncol;
+    stats->how_many_small += small_adj;
+    stats->how_many_big += big_adj;
     if (!M.removed.empty()) {
         // Later visits do not gather a removed body (collision.c:296): one
         // neighbour fewer for every later visit whose cells hold it.
@@ -2355,9 +2451,12 @@ static ncg_status mesh_gravity_core(ncg_ctx *ctx, const DBody *db, int32_t n, co
         // Pair-list order, bit for bit: recorded big visits as (visiting id,
         // ascending partner ids); the per-body walks on a second stream.
         std::vector<int> hbk, hbo(1, 0), hbp;
+        // Only the visitor's own pairs (ascending partner ids); the bodies of
+        // fragmentation blocks recorded there have explicit term lists.
         for (const BigVisit &bv : M.big_visits) {
             hbk.push_back(bv.k);
             for (int q = 0; q < bv.count; ++q) {
+                if (!M.big_scan[bv.first + q]) continue;
                 const ncg_pair &pq = M.big_pairs_all[bv.first + q];
                 hbp.push_back(pq.fst == bv.k ? pq.snd : pq.fst);
             }
@@ -2629,7 +2728,8 @@ extern "C" ncg_status ncg_mesh_pairs(ncg_ctx *ctx, ncg_pair *out, int64_t capaci
         return NCG_OK;
     }
     std::vector<int> rmv((size_t)n, INT32_MAX);
-    for (const auto &rv : M.removed) rmv[rv.first] = rv.second;
+    for (const auto &rv : M.removed)
+        if (rv.first < n) rmv[rv.first] = rv.second;
     long long o = 0;
     for (int32_t k = 0; k < n; ++k)
         for (long long q = off[k]; q < off[k] + cnt[k]; ++q) {

@@ -98,6 +98,29 @@ def test_mesh_fragmentation_and_merger_runs_identical(tmp_path, cfg, mass, radiu
         assert any(int(c[2]) >= 16384 or int(c[3]) >= 16384 for c in coll)  # fragments collide
 
 
+def test_mesh_fragmentation_with_big_bodies_run_identical(tmp_path):
+    # The reference's default collision model with bodies larger than the
+    # mesh size: 24 bodies of radius 0.03-0.06 (same density as the rest)
+    # in a 16384-body disk, half of them crossing it fast enough to fragment
+    # what they hit.  Their visits scan every body (collision.c:381-414):
+    # mergers remove bodies from later scans, fragments are appended and
+    # scanned by later big visits.  Every traced call identical.
+    import numpy as np
+    n = 16384
+    b = ic.disk(n, seed=6, mass=1e-8, radius=2e-3)
+    rng = np.random.default_rng(6)
+    big = rng.choice(np.arange(n), 24, replace=False)
+    b[big, 7] = rng.uniform(0.03, 0.06, 24)
+    b[big, 6] = 1e-8 * (b[big, 7] / 2e-3) ** 3
+    b[big[:12], 3:6] += rng.normal(0.0, 0.05, (12, 3))
+    tr = compare("run_mesh_frag_big", b, tmp_path, 10)
+    outs = [l.split() for l in tr if l.startswith("MESH") and " out " in l]
+    assert outs and all(int(o[o.index("big") + 1]) > 0 for o in outs)
+    coll = [l.split() for l in tr if l.startswith("C ")]
+    assert len(coll) > 100
+    assert any(int(c[2]) >= n or int(c[3]) >= n for c in coll)  # fragments collide
+
+
 @pytest.mark.parametrize("cfg,n,seed,mass,radius,steps", [
     ("run_c1_out", 2000, 1, 1e-8, 4e-3, 40), ("run_mesh_out", 16384, 2, 1e-10, 1e-3, 10),
     ("run_c1_out_ell", 2000, 3, 1e-8, 4e-3, 20), ("run_c1_out_cart", 2000, 4, 1e-8, 4e-3, 20)])

@@ -219,6 +219,7 @@ class MeshToyWorld:
         self.pairs = []
         self.kinds = set()
         self.dt = dt
+        self.visits = []  # (id, 1 if visited as larger than the mesh size)
 
     def test(self, a, c):
         return orc.closest_approach(self.b, a, c, self.dt, self.e)[0]
@@ -243,6 +244,10 @@ class MeshToyWorld:
             b[A, 7] = np.cbrt(b[A, 7] ** 3 + b[B, 7] ** 3)
             self.e[B] = 0
         elif kind == 2:
+            # Radii follow the masses (constant density), so a body may move
+            # across the mesh size before its own visit.
+            b[A, 7] *= np.cbrt(0.75 * m / b[A, 6])
+            b[B, 7] *= np.cbrt(0.25 * m / b[B, 6])
             b[A, 6], b[B, 6] = 0.75 * m, 0.25 * m
             b[A, 3:6], b[B, 3:6] = b[B, 3:6].copy(), b[A, 3:6].copy()
             self.pairs.append((A, B))
@@ -269,17 +274,39 @@ class MeshToyWorld:
         return out
 
 
-def toy_mesh(w, n0, gam, cube, cells):
-    """mesh(), collision.c:324-417 (every body at most the mesh size): the
-    hash holds each cell's visited bodies in insertion order; a visit gathers
-    the existing bodies of its 27 cells (newest first per cell) and pops them
-    last-in first-out (collision.c:264-316, structure.c:712-841)."""
+def toy_mesh(w, n0, gam, cube, cells, with_big=False):
+    """mesh(), collision.c:324-417: bodies visited by ascending id with their
+    live state.  A body at most the mesh size (radius read at its visit)
+    joins the hash, which holds each cell's visited bodies in insertion
+    order; its visit gathers the existing bodies of its 27 cells (newest
+    first per cell) and pops them last-in first-out (collision.c:264-316,
+    structure.c:712-841).  A larger one (with_big) tests every existing body
+    up to the live largest id that is smaller than the mesh size or larger
+    with a larger id (collision.c:381-414).  Returns (total_neighbours,
+    how_many_small[, how_many_big])."""
     cells_of = {}
-    total = small = 0
+    total = small = big = 0
     for k in range(n0):
         if not w.e[k]:
             continue
+        if with_big and w.b[k, 7] > gam:
+            big += 1
+            if hasattr(w, "visits"):
+                w.visits.append((k, 1))
+            p = 0
+            while p <= w.largest:
+                if w.e[p] and p != k:
+                    Rp = w.b[p, 7]
+                    if Rp < gam or (Rp >= gam and p > k):
+                        if w.test(p, k):
+                            w.resolve(p, k)
+                        else:
+                            w.pairs.append((k, p))
+                p += 1
+            continue
         small += 1
+        if with_big and hasattr(w, "visits"):
+            w.visits.append((k, 0))
         idx = orc.cell_indexes(w.b[k, 0], w.b[k, 1], w.b[k, 2], gam, cube, cells)
         if idx is None:
             continue
@@ -296,7 +323,7 @@ def toy_mesh(w, n0, gam, cube, cells):
             else:
                 w.pairs.append((k, p))
             total += 1
-    return total, small
+    return (total, small, big) if with_big else (total, small)
 
 
 @pytest.mark.parametrize("seed", [31, 32])
@@ -338,3 +365,62 @@ def test_mesh_pass_with_removed_and_appended_bodies(ctx, seed):
     assert np.array_equal(bits(cm), bits(rcm))
     acc, cm = ctx.mesh_gravity(b2, ex, central, 1.0, 0.1, "fast64")
     assert np.abs(acc[lv] - racc[lv]).max() / np.abs(racc[lv]).max() < 1e-12
+
+
+def test_mesh_pass_with_big_bodies_and_all_outcomes(ctx):
+    """Bodies larger than the mesh size together with outcomes that remove,
+    create and resize bodies (collision.c:381-414 with physics.c:628-1040's
+    shapes): a big visit scans every body up to the live largest id, so bodies
+    removed earlier in the pass leave it and appended ones join it; a body an
+    outcome moves across the mesh size before its own visit is visited in its
+    new class.  Collision sequence, counters, final state, pair list and the
+    exact field over the pass's own list (appended slots included) equal the
+    restatement's (pinned to the oracle's mesh() with big bodies on the CPU)."""
+    n0, cap, dt, cells, gam = 3000, 9000, 0.01, 16, 0.01
+    cube = cells * gam
+    seen = {"big_coll": 0, "to_big": 0, "to_small": 0, "appended_in_big": 0}
+    for seed in (41, 42, 43):
+        rng = np.random.default_rng(seed)
+        b = blob(n0, seed, half=0.06, rmin=1.5e-3, rmax=3e-3)
+        b[:, 6] = rng.uniform(1e-9, 1e-8, n0)
+        near = rng.choice(np.arange(n0), 80, replace=False)
+        b[near, 7] = rng.uniform(0.85 * gam, 1.3 * gam, 80)
+        b[near, 6] = rng.uniform(1e-8, 4e-8, 80)
+        start_big = set(np.flatnonzero(b[:, 7] > gam).tolist())
+        ref = MeshToyWorld(b, cap, dt)
+        total, small, big = toy_mesh(ref, n0, gam, cube, cells, with_big=True)
+        got = MeshToyWorld(b, cap, dt)
+        print("big bodies, seed %d: %d collisions, %d big visits" % (seed, len(ref.log), big),
+              flush=True)
+        st = ctx.mesh_pass(got.b, got.e, dt, gam, cube, cells, 0, got.test, got.resolve,
+                           created=got.created, n=n0)
+        assert got.log == ref.log
+        assert st["collisions"] == len(ref.log)
+        assert (st["total_neighbours"], st["how_many_small"], st["how_many_big"]) == \
+            (total, small, big)
+        assert np.array_equal(got.e, ref.e)
+        live = ref.e != 0
+        assert np.array_equal(bits(got.b[live]), bits(ref.b[live]))
+        pairs = ctx.mesh_pairs()
+        assert np.array_equal(pairs, np.array(ref.pairs, np.int32).reshape(-1, 2))
+        ne = ref.largest + 1
+        b2 = ref.b[:ne].copy()
+        b2[:, 0:3] += 0.002 * b2[:, 3:6]
+        ex = ref.e[:ne].copy()
+        central = (0.01, -0.02, 0.005, 1.0)
+        for soft in (0.0, 0.1):
+            acc, cm = ctx.mesh_gravity(b2, ex, central, 1.0, soft, "exact")
+            racc, rcm = orc.vector_field_mesh(b2, pairs, ex, central, 1.0, soft)
+            lv = ex != 0
+            assert np.array_equal(bits(acc[lv]), bits(racc[lv]))
+            assert np.array_equal(bits(cm), bits(rcm))
+        acc, _ = ctx.mesh_gravity(b2, ex, central, 1.0, 0.1, "fast64")
+        assert np.abs(acc[lv] - racc[lv]).max() / np.abs(racc[lv]).max() < 1e-12
+        bigv = {k for k, c in ref.visits if c == 1}
+        seen["big_coll"] += sum(1 for a, c in ref.log if c in bigv or a in bigv)
+        seen["to_big"] += len(bigv - start_big)
+        seen["to_small"] += len({k for k, c in ref.visits if c == 0} & start_big)
+        seen["appended_in_big"] += sum(1 for a, c in ref.pairs if c >= n0 and a in bigv)
+    print(seen)
+    assert all(v > 0 for v in seen.values()), seen
+
