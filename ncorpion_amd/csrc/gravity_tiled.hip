@@ -7,27 +7,36 @@
 //
 // Pipeline of one evaluation (all on one stream, no host synchronisation):
 //   1. k_bbox_*      bounding box, largest radius and count of existing bodies
-//   2. k_morton      30-bit Morton key per body (holes sort last)
+//   2. k_morton      30-bit Morton key per body over the bounding CUBE (the
+//                    same cell size on every axis, so that runs of keys are
+//                    compact in space also for a thin disk); holes sort last
 //   3. radix sort    (key, index) pairs -> spatially coherent order
 //   4. k_gather      sorted FP64 records {x, y, z, G m} and s R
 //   5. k_tiles       per tile of TJ bodies: FP64 anchor (box centre), FP32
-//                    half extent, FP32 offsets from the anchor and FP32 G m;
-//                    the same for i-blocks of 256*IPL bodies
+//                    half extent, the range of (G m)^(-1/2) over the tile and
+//                    the scaled far-form records (below); the anchor and
+//                    extent also per wavefront group of 64*IPL i-bodies
 //   6. k_pairs<MODE> one workgroup per (i-block, j-split); every lane owns IPL
-//                    bodies and walks the j tiles of its split.  Tile data is
+//                    bodies, every wavefront a contiguous group of 64*IPL of
+//                    them, and walks the j tiles of its split.  Tile data is
 //                    read with wave-uniform addresses (scalar loads).  For each
-//                    (i-block, tile) pair a uniform test picks the form:
+//                    (wavefront group, tile) pair a uniform test picks the form:
 //                      near: FP64 separations, FP64 rsqrt refined by one Newton
 //                            step, FP64 accumulation (also every tile in
 //                            NCG_F64_FAST);
-//                      far:  separations C + u_j - u_i from the FP64 anchor
-//                            difference C and FP32 offsets, FP32 1/r^3 and pair
-//                            force, FP32 tile partial added into FP64.
-//                    A tile pair is far when its box distance is at least
-//                    theta times (anchor distance + half extents), which bounds
-//                    the FP32 separation error, and when the largest softening
-//                    length is below FP32 resolution of that distance, so the
-//                    far form can omit it without changing an FP32 result.
+//                      far:  scaled FP32 separations t = s_j (C + u_j - u_i)
+//                            with s_j = (G m_j)^(-1/2), C the FP64 anchor
+//                            difference and u the FP32 offsets from the
+//                            anchors; then G m_j d / r^3 = t / |t|^3, so a
+//                            term is 12 FP32 operations; FP32 tile partial
+//                            added into FP64.
+//                    A pair is far when its box distance is at least theta
+//                    times (anchor distance + half extents), which bounds the
+//                    FP32 separation error, when the largest softening length
+//                    is below FP32 resolution of that distance, so the far
+//                    form can omit it without changing an FP32 result, and
+//                    when every scaled distance |t|^2 lies in [1e-24, 1e24],
+//                    so t / |t|^3 neither overflows nor underflows in FP32.
 //   7. k_finish      sums the j-split partials in fixed order, adds the FP64
 //                    central term and scatters back to body order.
 #include <hipcub/hipcub.hpp>
@@ -40,10 +49,16 @@ constexpr int TJ = 128;     // bodies per j tile
 constexpr int NT = 256;     // threads per workgroup
 constexpr double TINY2 = 1e-200;  // floor of r^2 in the FP64 form (keeps a self pair's y finite)
 
+constexpr int WV = 64;      // lanes per wavefront
+
 struct TileHdr {
     double ax, ay, az;  // anchor (box centre)
     float hx, hy, hz;   // half extents, rounded up
     int count;          // valid bodies in the tile
+    // Largest and smallest s = (G m)^(-1/2) over the tile's bodies with
+    // G m > 0 (j tiles only); smax is +inf when a mass is negative or not
+    // finite, which keeps the tile in the FP64 form.
+    float smax, smin;
 };
 
 struct Scal {           // per-evaluation scalars produced on the device
@@ -151,9 +166,13 @@ __global__ __launch_bounds__(NT) void k_morton(const DBody *__restrict__ b, int 
     }
     unsigned q[3];
     const double p[3] = {b[k].x, b[k].y, b[k].z};
+    // Cells of one size on all axes: the bounding cube of edge w around the
+    // box centre.
+    const double w = fmax(sc->bmax[0] - sc->bmin[0],
+                          fmax(sc->bmax[1] - sc->bmin[1], sc->bmax[2] - sc->bmin[2]));
     for (int c = 0; c < 3; ++c) {
-        const double w = sc->bmax[c] - sc->bmin[c];
-        double f = w > 0.0 ? (p[c] - sc->bmin[c]) / w : 0.0;
+        const double lo = 0.5 * (sc->bmin[c] + sc->bmax[c]) - 0.5 * w;
+        double f = w > 0.0 ? (p[c] - lo) / w : 0.0;
         int v = (int)(f * 1023.999);
         v = v < 0 ? 0 : (v > 1023 ? 1023 : v);
         q[c] = (unsigned)v;
@@ -179,7 +198,12 @@ __global__ __launch_bounds__(NT) void k_gather(const DBody *__restrict__ b, int 
     jsig[s] = soft * q.radius;
 }
 
-// ---- 5. tile anchors and FP32 offsets -------------------------------------
+// ---- 5. tile anchors and scaled FP32 offsets --------------------------------
+// Far-form record of body j in a tile with anchor A: {s (x - A), s G m > 0 ?
+// s : 0} with s = (G m)^(-1/2), the product formed in FP64 and rounded once.
+// A body without mass (and a padding slot) gets {1e30, 1e30, 1e30, 0}: its
+// scaled separation is 1e30 whatever the i-body, |t|^2 overflows to +inf,
+// 1/|t| = 0 and the term t / |t|^3 is exactly 0.
 __global__ __launch_bounds__(NT) void k_tiles(const double4 *__restrict__ jd,
                                               const Scal *__restrict__ sc, int tsize,
                                               TileHdr *__restrict__ hdr,
@@ -192,6 +216,8 @@ __global__ __launch_bounds__(NT) void k_tiles(const double4 *__restrict__ jd,
     if (b0 >= nv) return;
     const int cnt = min(tsize, nv - b0);
     double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
+    double gmin = 1e300, gmax = 0.0;  // over G m > 0
+    bool bad = false;                 // a negative or non-finite G m
     for (int q = threadIdx.x; q < cnt; q += NT) {
         const double4 p = jd[b0 + q];
         mn[0] = fmin(mn[0], p.x);
@@ -200,6 +226,12 @@ __global__ __launch_bounds__(NT) void k_tiles(const double4 *__restrict__ jd,
         mx[0] = fmax(mx[0], p.x);
         mx[1] = fmax(mx[1], p.y);
         mx[2] = fmax(mx[2], p.z);
+        if (p.w > 0.0 && p.w < 1e300) {
+            gmin = fmin(gmin, p.w);
+            gmax = fmax(gmax, p.w);
+        } else if (!(p.w == 0.0)) {
+            bad = true;
+        }
     }
     double A[3];
     float H[3];
@@ -212,6 +244,9 @@ __global__ __launch_bounds__(NT) void k_tiles(const double4 *__restrict__ jd,
         if ((double)hf < h) hf = nextafterf(hf, 3.0e38f);
         H[c] = hf;
     }
+    const double gl = block_reduce(gmin, sh, 0);
+    const double gh = block_reduce(gmax, sh, 1);
+    const double nbad = block_reduce(bad ? 1.0 : 0.0, sh, 2);
     if (threadIdx.x == 0) {
         TileHdr th;
         th.ax = A[0];
@@ -221,18 +256,31 @@ __global__ __launch_bounds__(NT) void k_tiles(const double4 *__restrict__ jd,
         th.hy = H[1];
         th.hz = H[2];
         th.count = cnt;
+        if (nbad != 0.0) {
+            th.smax = __int_as_float(0x7f800000);
+            th.smin = 0.f;
+        } else if (gh == 0.0) {  // no mass in the tile: every term is 0
+            th.smax = th.smin = 0.f;
+        } else {
+            // Rounded outwards, so the bounds hold for the FP32 values.
+            th.smax = (float)(1.0 / sqrt(gl)) * 1.0000002f;
+            th.smin = (float)(1.0 / sqrt(gh)) * 0.9999998f;
+        }
         hdr[t] = th;
     }
     if (ju) {
         for (int q = threadIdx.x; q < tsize; q += NT) {
             const int s = b0 + q;
+            float4 r = make_float4(1e30f, 1e30f, 1e30f, 0.f);
             if (q < cnt) {
                 const double4 p = jd[s];
-                ju[s] = make_float4((float)(p.x - A[0]), (float)(p.y - A[1]), (float)(p.z - A[2]),
-                                    (float)p.w);
-            } else {
-                ju[s] = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (p.w > 0.0) {
+                    const double sj = 1.0 / sqrt(p.w);
+                    r = make_float4((float)(sj * (p.x - A[0])), (float)(sj * (p.y - A[1])),
+                                    (float)(sj * (p.z - A[2])), (float)sj);
+                }
             }
+            ju[s] = r;
         }
     }
 }
@@ -266,43 +314,53 @@ __device__ inline void near_term(const double4 p, double sj, const double *xi, c
 }
 
 // near_term for a j slot that may be one of the lane's own bodies (i slot q
-// is i0 + q NT + lane): that pair's mass is zeroed.  Without softening its
+// is is0 + q stride): that pair's mass is zeroed.  Without softening its
 // r^2 is TINY2 and y^3 = 1e300, so G m y^3 would overflow to inf for
 // G m > ~1.8e8 and inf * 0 poison the sum; with the mass zeroed the term is
 // 0 * 1e300 * 0 = 0 whatever G m is.  Used only where the j tile overlaps
 // the lane's i slots.
 template <int IPL>
-__device__ inline void near_term_self(const double4 p, double sj, int js, int is0,
+__device__ inline void near_term_self(const double4 p, double sj, int js, int is0, int stride,
                                       const double *xi, const double *yi, const double *zi,
                                       const double *si, double *ax, double *ay, double *az)
 {
 #pragma unroll
     for (int q = 0; q < IPL; ++q) {
-        const double4 pq = make_double4(p.x, p.y, p.z, js == is0 + q * NT ? 0.0 : p.w);
+        const double4 pq = make_double4(p.x, p.y, p.z, js == is0 + q * stride ? 0.0 : p.w);
         near_term<1>(pq, sj, xi + q, yi + q, zi + q, si + q, ax + q, ay + q, az + q);
     }
 }
 
-// One interaction of the FP32 far form: separation w_i + u_j (w_i = C - u_i).
+// One interaction of the FP32 far form from the scaled record u = {s u_j, s}
+// (k_tiles) and the lane's w_i = C - u_i: t = s w_i + s u_j, y = 1/|t|, and
+// the term G m_j d / r^3 = y^3 t.
 template <int IPL>
 __device__ inline void far_term(const float4 u, const float *wx, const float *wy, const float *wz,
                                 float *fx, float *fy, float *fz)
 {
 #pragma unroll
     for (int q = 0; q < IPL; ++q) {
-        const float dx = wx[q] + u.x, dy = wy[q] + u.y, dz = wz[q] + u.z;
-        const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+        const float tx = fmaf(u.w, wx[q], u.x);
+        const float ty = fmaf(u.w, wy[q], u.y);
+        const float tz = fmaf(u.w, wz[q], u.z);
+        const float r2 = fmaf(tz, tz, fmaf(ty, ty, tx * tx));
         const float y = __builtin_amdgcn_rsqf(r2);
-        const float s3 = (u.w * y) * (y * y);
-        fx[q] = fmaf(s3, dx, fx[q]);
-        fy[q] = fmaf(s3, dy, fy[q]);
-        fz[q] = fmaf(s3, dz, fz[q]);
+        const float y3 = (y * y) * y;
+        fx[q] = fmaf(y3, tx, fx[q]);
+        fy[q] = fmaf(y3, ty, fy[q]);
+        fz[q] = fmaf(y3, tz, fz[q]);
     }
 }
 
 // MODE 0: FP64-fast on every tile; MODE 1: mixed (near FP64, far FP32).
-// SRC 0: tile data read with wave-uniform addresses (scalar loads);
-// SRC 1: tile data staged in LDS by the workgroup, read as broadcasts.
+// SRC 0: the form picked per wavefront; near tiles read with wave-uniform
+//        addresses (scalar loads), far tiles staged by the wavefront in its
+//        own LDS slice and read as broadcasts (a far term takes three per-j
+//        operands in one instruction, more than the one scalar operand a
+//        VALU instruction can read);
+// SRC 1: tile data staged in LDS by the workgroup, read as broadcasts; the
+//        form picked per workgroup (near when any of its wavefronts is near).
+// Lane `lane` of wavefront w owns bodies i0 + w 64 IPL + q 64 + lane.
 template <int MODE, int IPL, int SRC>
 __global__ __launch_bounds__(NT) void k_pairs(const double4 *__restrict__ jd,
                                               const double *__restrict__ jsig,
@@ -314,7 +372,8 @@ __global__ __launch_bounds__(NT) void k_pairs(const double4 *__restrict__ jd,
                                               double *__restrict__ part)
 {
     constexpr bool LDS = SRC == 1;
-    __shared__ float4 lu[LDS ? TJ : 1];
+    constexpr int GW = WV * IPL;  // bodies of one wavefront group
+    __shared__ float4 lu[LDS ? TJ : (NT / WV) * TJ];
     __shared__ double4 ld[LDS ? TJ : 1];
     __shared__ double ls[LDS ? TJ : 1];
     const int nv = sc->nvalid;
@@ -325,14 +384,19 @@ __global__ __launch_bounds__(NT) void k_pairs(const double4 *__restrict__ jd,
     const int per = (ntiles + nsplit - 1) / nsplit;
     const int t0 = blockIdx.y * per;
     const int t1 = min(ntiles, t0 + per);
+    const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x / WV);
+    const int lane = (int)threadIdx.x % WV;
+    const int g0 = i0 + w * GW;  // the wavefront group's first body
+    const bool gvalid = g0 < nv;
+    if (!LDS && !gvalid) return;
 
-    const TileHdr ih = ihdr[iblk];
+    const TileHdr ih = ihdr[gvalid ? iblk * (NT / WV) + w : 0];
     double xi[IPL], yi[IPL], zi[IPL], si[IPL];
     float ux[IPL], uy[IPL], uz[IPL];
     double ax[IPL], ay[IPL], az[IPL];
 #pragma unroll
     for (int q = 0; q < IPL; ++q) {
-        const int i = min(i0 + q * NT + (int)threadIdx.x, nv - 1);
+        const int i = min(g0 + q * WV + lane, nv - 1);
         const double4 p = jd[i];
         xi[q] = p.x;
         yi[q] = p.y;
@@ -362,7 +426,13 @@ __global__ __launch_bounds__(NT) void k_pairs(const double4 *__restrict__ jd,
             const double hj = sqrt((double)th.hx * th.hx + (double)th.hy * th.hy +
                                    (double)th.hz * th.hz);
             const double size = sqrt(Cx * Cx + Cy * Cy + Cz * Cz) + hi + hj;
-            near = (dmin2 < theta * theta * size * size) || (dmin2 <= eps2x);
+            const double smax = th.smax, smin = th.smin;
+            // Scaled distances |t|^2 = s^2 r^2 of every pair in [1e-24, 1e24].
+            const bool range = size * size * (smax * smax) <= 1e24 &&
+                               (smin == 0.0 || dmin2 * (smin * smin) >= 1e-24);
+            near = (dmin2 < theta * theta * size * size) || (dmin2 <= eps2x) || !range ||
+                   !gvalid;
+            if (LDS) near = __syncthreads_or(near && gvalid);
         }
         if (near) {
             if (LDS) {
@@ -373,12 +443,12 @@ __global__ __launch_bounds__(NT) void k_pairs(const double4 *__restrict__ jd,
                 }
                 __syncthreads();
             }
-            if (jb < i0 + NT * IPL && jb + TJ > i0) {
-                // The tile holds some of the workgroup's own bodies.
-                const int is0 = i0 + (int)threadIdx.x;
+            if (jb < g0 + GW && jb + TJ > g0) {
+                // The tile holds some of the wavefront's own bodies.
+                const int is0 = g0 + lane;
                 for (int jj = 0; jj < cnt; ++jj)
                     near_term_self<IPL>(LDS ? ld[jj] : jd[jb + jj], LDS ? ls[jj] : jsig[jb + jj],
-                                        jb + jj, is0, xi, yi, zi, si, ax, ay, az);
+                                        jb + jj, is0, WV, xi, yi, zi, si, ax, ay, az);
             } else if (cnt == TJ) {
 #pragma unroll 4
                 for (int jj = 0; jj < TJ; ++jj)
@@ -390,10 +460,20 @@ __global__ __launch_bounds__(NT) void k_pairs(const double4 *__restrict__ jd,
                                    yi, zi, si, ax, ay, az);
             }
         } else {
+            float4 *tu = LDS ? lu : lu + w * TJ;
             if (LDS) {
                 __syncthreads();
-                if ((int)threadIdx.x < cnt) lu[threadIdx.x] = ju[jb + threadIdx.x];
+                if ((int)threadIdx.x < TJ) lu[threadIdx.x] = ju[jb + threadIdx.x];
                 __syncthreads();
+            } else {
+                // The wavefront's own slice: ordered after its reads of the
+                // previous far tile and before the reads below.
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+#pragma unroll
+                for (int r = 0; r < TJ / WV; ++r) tu[r * WV + lane] = ju[jb + r * WV + lane];
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                __builtin_amdgcn_wave_barrier();
             }
             const float cx = (float)Cx, cy = (float)Cy, cz = (float)Cz;
             float wx[IPL], wy[IPL], wz[IPL], fx[IPL], fy[IPL], fz[IPL];
@@ -404,14 +484,10 @@ __global__ __launch_bounds__(NT) void k_pairs(const double4 *__restrict__ jd,
                 wz[q] = cz - uz[q];
                 fx[q] = fy[q] = fz[q] = 0.f;
             }
-            if (cnt == TJ) {
+            // Slots past cnt hold massless records (exact zeros), so whole
+            // tiles are walked.
 #pragma unroll 8
-                for (int jj = 0; jj < TJ; ++jj)
-                    far_term<IPL>(LDS ? lu[jj] : ju[jb + jj], wx, wy, wz, fx, fy, fz);
-            } else {
-                for (int jj = 0; jj < cnt; ++jj)
-                    far_term<IPL>(LDS ? lu[jj] : ju[jb + jj], wx, wy, wz, fx, fy, fz);
-            }
+            for (int jj = 0; jj < TJ; ++jj) far_term<IPL>(tu[jj], wx, wy, wz, fx, fy, fz);
 #pragma unroll
             for (int q = 0; q < IPL; ++q) {
                 ax[q] += (double)fx[q];
@@ -423,7 +499,7 @@ __global__ __launch_bounds__(NT) void k_pairs(const double4 *__restrict__ jd,
     double *o = part + (size_t)blockIdx.y * 3 * (size_t)nslots;
 #pragma unroll
     for (int q = 0; q < IPL; ++q) {
-        const int i = i0 + q * NT + (int)threadIdx.x;
+        const int i = g0 + q * WV + lane;
         if (i < nv) {
             o[3 * (size_t)i + 0] = ax[q];
             o[3 * (size_t)i + 1] = ay[q];
@@ -481,7 +557,7 @@ __global__ __launch_bounds__(NT) void k_small(const DBody *__restrict__ b, int n
             // (near_term_self).
 #pragma unroll
             for (int u = 0; u < NT / 64; ++u)
-                near_term_self<1>(sp[lane + 64 * u], ss[lane + 64 * u], base + lane + 64 * u, k,
+                near_term_self<1>(sp[lane + 64 * u], ss[lane + 64 * u], base + lane + 64 * u, k, 0,
                                   &X, &Y, &Z, &Sk, &ax, &ay, &az);
         } else {
 #pragma unroll
@@ -640,7 +716,7 @@ static void launch_pairs(int src, dim3 grid, hipStream_t st, const double4 *jd,
 
 // Grid shape of the tiled forms for n slots (same in order and eval).
 struct TiledShape {
-    int ipl, iblock, nib, ntiles, nsplit, nbb, src;
+    int ipl, iblock, nib, ngroups, ntiles, nsplit, nbb, src;
     double theta;
 };
 
@@ -653,6 +729,7 @@ static TiledShape tiled_shape(ncg_ctx *ctx, int32_t n, int prec)
     t.src = cfg.src;
     t.iblock = NT * t.ipl;
     t.nib = (int)cdiv(n, t.iblock);
+    t.ngroups = t.nib * (NT / WV);  // wavefront groups of 64 ipl bodies
     t.ntiles = (int)cdiv(n, TJ);
     int nsplit = cfg.nsplit;
     if (nsplit <= 0) {
@@ -734,7 +811,7 @@ ncg_status ncg_tiled_eval(ncg_ctx *ctx, const DBody *d_b, int32_t n, const int32
     double *jsig = dbuf<double>(ctx, "t_jsig", (size_t)t.ntiles * TJ + t.iblock);
     float4 *ju = dbuf<float4>(ctx, "t_ju", (size_t)t.ntiles * TJ);
     TileHdr *th = dbuf<TileHdr>(ctx, "t_th", t.ntiles);
-    TileHdr *ih = dbuf<TileHdr>(ctx, "t_ih", t.nib);
+    TileHdr *ih = dbuf<TileHdr>(ctx, "t_ih", t.ngroups);
     double *part = dbuf<double>(ctx, "t_part", (size_t)t.nsplit * 3 * n);
     double *cterm = d_cterm ? d_cterm : (want_cm ? dbuf<double>(ctx, "cterm", 3 * (size_t)n) : nullptr);
     double *p3 = want_cm ? dbuf<double>(ctx, "sum3", 3 * (size_t)t.nbb) : nullptr;
@@ -743,7 +820,7 @@ ncg_status ncg_tiled_eval(ncg_ctx *ctx, const DBody *d_b, int32_t n, const int32
     hipLaunchKernelGGL(k_gather, dim3(cdiv(n, NT)), dim3(NT), 0, st, d_b, n, perm, sc, G, soft, jd,
                        jsig);
     hipLaunchKernelGGL(k_tiles, dim3(t.ntiles), dim3(NT), 0, st, jd, sc, TJ, th, ju);
-    hipLaunchKernelGGL(k_tiles, dim3(t.nib), dim3(NT), 0, st, jd, sc, t.iblock, ih,
+    hipLaunchKernelGGL(k_tiles, dim3(t.ngroups), dim3(NT), 0, st, jd, sc, WV * t.ipl, ih,
                        (float4 *)nullptr);
     NCG_CHECK_LAUNCH();
     const dim3 grid(t.nib, t.nsplit);
