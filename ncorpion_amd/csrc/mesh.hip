@@ -271,7 +271,7 @@ __global__ __launch_bounds__(MT) void k_mcount(const unsigned *__restrict__ skey
 //   - events of visit k: candidate pairs (k, p) of the pass and pending
 //     re-tests, in traversal order (cell l = 26..0, ascending p), each tested
 //     exactly with the current states; the first hit is resolved;
-//   - after a hit, the rest of visit k is scanned in full (k changed), 64
+//   - after a hit, the rest of visit k is scanned in full (k changed), MR_T
 //     pairs at a time, the first hit in traversal order winning;
 //   - whenever a body changes, its later visitors q > k (the members of its
 //     27 cells) are tested against its new state at once; the ones that may
@@ -385,17 +385,13 @@ __global__ __launch_bounds__(MR_T) void k_mesh_replay(
         const DBody D = b[dd];
         const double4 pd = make_double4(D.x, D.y, D.z, D.radius);
         const double4 vd = make_double4(D.vx, D.vy, D.vz, D.mass);
+        // Whole cells: the members at or below kc are skipped in the scan
+        // (cheaper than locating them first with dependent loads).
         if (lane < 27) {
             const unsigned c = (unsigned)((long)kd + stencil_offset(lane, d));
-            unsigned lo = cs[c], hi = ce[c];
-            const unsigned e = hi;
-            while (lo < hi) {  // first id above kc
-                const unsigned mid = (lo + hi) >> 1;
-                if (sid[mid] <= kc) lo = mid + 1;
-                else hi = mid;
-            }
+            const unsigned lo = cs[c];
             s_ra[0][lane] = lo;
-            s_rn[0][lane] = e - lo;
+            s_rn[0][lane] = ce[c] - lo;
             s_rl[0][lane] = lane;
         }
         flatten(0);
@@ -409,7 +405,7 @@ __global__ __launch_bounds__(MR_T) void k_mesh_replay(
                 const int q = sid[u];
                 s_q[lane] = q;
                 s_l[lane] = s_rl[0][i];
-                if (!ex || ex[q]) h = approach_hit(pd, vd, pr[u], vm[u], dt);
+                if (q > kc && (!ex || ex[q])) h = approach_hit(pd, vd, pr[u], vm[u], dt);
             }
             const unsigned long long wm = __ballot(h);
             if ((lane & 63) == 0) s_m[wv] = wm;
@@ -503,29 +499,16 @@ __global__ __launch_bounds__(MR_T) void k_mesh_replay(
         // cells; the first hit wins and the walk resumes after it.
         const unsigned kk = key[k];
         const int l0 = s_hl, p0 = s_hp;
+        // Whole cells, walked in traversal order; the members outside the
+        // rest of the visit (ids at or above k, and up to p0 in cell l0) are
+        // skipped in the scan.
         if (lane < 27) {
             const int l = l0 - lane;  // range `lane` is cell l0 - lane
             unsigned lo = 0, e = 0;
             if (l >= 0) {
                 const unsigned c = (unsigned)((long)kk + stencil_offset(l, d));
                 lo = cs[c];
-                unsigned hi = ce[c];
-                if (l == l0) {  // after p0
-                    unsigned a2 = lo, b2 = hi;
-                    while (a2 < b2) {
-                        const unsigned mid = (a2 + b2) >> 1;
-                        if (sid[mid] <= p0) a2 = mid + 1;
-                        else b2 = mid;
-                    }
-                    lo = a2;
-                }
-                unsigned a3 = lo, b3 = hi;  // ids below k
-                while (a3 < b3) {
-                    const unsigned mid = (a3 + b3) >> 1;
-                    if (sid[mid] < k) a3 = mid + 1;
-                    else b3 = mid;
-                }
-                e = a3;
+                e = ce[c];
             }
             s_ra[1][lane] = lo;
             s_rn[1][lane] = e - lo;
@@ -542,7 +525,8 @@ __global__ __launch_bounds__(MR_T) void k_mesh_replay(
             if (x < total) {
                 const int i = range_of(1, x);
                 p = sid[s_ra[1][i] + (x - s_pre[1][i])];
-                h = bf_hit(b, ex, dcf, p, k, dt, ap, &tc);
+                if (p < k && (s_rl[1][i] != l0 || p > p0))
+                    h = bf_hit(b, ex, dcf, p, k, dt, ap, &tc);
             }
             const unsigned long long hm = __ballot(h);
             __syncthreads();  // the masks of the previous round have been read
@@ -670,19 +654,35 @@ __global__ __launch_bounds__(MT) void k_mgrav_rest(const DBody *__restrict__ b, 
     acc[3 * (size_t)k + 2] = az;
 }
 
+// Neighbour records of the fast forms: {x, y, z, G m} of the sorted bodies,
+// G m = 0 for the bodies that take no part in the sorted walk (absent ones,
+// and bodies 0..2, whose terms are added apart), so the walk needs no test.
+__global__ __launch_bounds__(MT) void k_gweights(const double4 *__restrict__ pm,
+                                                 const int *__restrict__ sx, int ins, double G,
+                                                 double4 *__restrict__ pw)
+{
+    const int s = blockIdx.x * MT + threadIdx.x;
+    if (s >= ins) return;
+    const double4 q = pm[s];
+    pw[s] = make_double4(q.x, q.y, q.z, sx[s] ? G * q.w : 0.0);
+}
+
 // Binned bodies k > 2, one lane per sorted position t (bodies of one cell
-// are adjacent lanes and walk the same 27 cells).  Per body: central term,
+// are adjacent lanes and walk the same cells).  Per body: central term,
 // bodies 0..2, then the neighbour terms cell l = 0..26, ascending id within
 // a cell (cell order: accelerations agree with the reference's pair-list
-// order to rounding).  FORM 1: FP64 with hardware 1/sqrt and a Newton step;
-// 2: separations in FP64, 1/r^3 and forces in FP32, summed in FP32 per
-// neighbour cell and added to the FP64 total.
-template <int FORM>
+// order to rounding).  The three cells (i-1, i, i+1) of a stencil row have
+// consecutive keys, so their members are one contiguous run of the sorted
+// order: the walk is 9 runs.  The lane's own record enters with weight 0.
+// FORM 1: FP64 with hardware 1/sqrt and a Newton step; 2: separations in
+// FP64, 1/r^3 and forces in FP32, summed in FP32 per stencil row and added
+// to the FP64 total.  SOFT: softening lengths s (R_k + R_p) (else 0).
+template <int FORM, bool SOFT>
 __global__ __launch_bounds__(MT) void k_mgrav(const DBody *__restrict__ b, int n,
                                               const int32_t *__restrict__ ex, int ins,
                                               const unsigned *__restrict__ cs,
                                               const unsigned *__restrict__ ce,
-                                              const double4 *__restrict__ pm,
+                                              const double4 *__restrict__ pw,
                                               const double *__restrict__ rr,
                                               const int *__restrict__ sx,
                                               const int *__restrict__ sid,
@@ -696,31 +696,36 @@ __global__ __launch_bounds__(MT) void k_mgrav(const DBody *__restrict__ b, int n
     const int t = blockIdx.x * MT + threadIdx.x;
     if (t >= ins || !sx[t]) return;
     const int k = sid[t];
-    const double4 me = pm[t];
-    const double Rk = rr[t];
+    const DBody me = b[k];
+    const double Rk = SOFT ? rr[t] : 0.0;
     double ax = 0., ay = 0., az = 0.;
-    if (cp) central_term(me.x, me.y, me.z, me.w, cx, cy, cz, mu, G, ax, ay, az, cterm + 3 * (size_t)k);
+    if (cp) central_term(me.x, me.y, me.z, me.mass, cx, cy, cz, mu, G, ax, ay, az, cterm + 3 * (size_t)k);
     for (int big = 0; big < 3 && big < n; ++big) {
         if (ex && !ex[big]) continue;
         const DBody g = b[big];
-        term_exact(me.x, me.y, me.z, Rk, make_double4(g.x, g.y, g.z, g.mass), g.radius, G, s, ax,
-                   ay, az);
+        term_exact(me.x, me.y, me.z, me.radius, make_double4(g.x, g.y, g.z, g.mass), g.radius, G,
+                   s, ax, ay, az);
     }
     const long kk = skey[t];
-    for (int l = 0; l < 27; ++l) {
-        const unsigned ck = (unsigned)(kk + stencil_offset(l, d));
-        const unsigned a = cs[ck], e = ce[ck];
+    for (int r = 0; r < 9; ++r) {
+        const long kr = kk + (long)(r % 3 - 1) * d + (long)(r / 3 - 1) * d * d;
+        const unsigned s0 = cs[kr - 1], e0 = ce[kr - 1], s1 = cs[kr], e1 = ce[kr];
+        const unsigned s2 = cs[kr + 1], e2 = ce[kr + 1];
+        // Empty cells hold [0, 0).
+        const unsigned a = e0 > s0 ? s0 : (e1 > s1 ? s1 : s2);
+        const unsigned e = e2 > s2 ? e2 : (e1 > s1 ? e1 : e0);
         if (FORM == 2) {
             float fx = 0.f, fy = 0.f, fz = 0.f;
-            const float Gf = (float)G;
+#pragma unroll 2
             for (unsigned u = a; u < e; ++u) {
-                if (!sx[u] || u == (unsigned)t) continue;
-                const double4 p = pm[u];
+                const double4 p = pw[u];
+                const float w = u == (unsigned)t ? 0.f : (float)p.w;
                 const float dx = (float)(p.x - me.x), dy = (float)(p.y - me.y), dz = (float)(p.z - me.z);
-                const float ee = (float)(s * (Rk + rr[u]));
-                const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, ee * ee)));
-                const float y = __builtin_amdgcn_rsqf(fmaxf(r2, 1e-30f));
-                const float f = (Gf * (float)p.w) * (y * (y * y));
+                const float ee = SOFT ? (float)(s * (Rk + rr[u])) : 0.f;
+                // The floor keeps y^3 finite for the lane's own record (weight 0).
+                const float r2 = fmaxf(fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, ee * ee))), 1e-20f);
+                const float y = __builtin_amdgcn_rsqf(r2);
+                const float f = w * (y * (y * y));
                 fx = fmaf(f, dx, fx);
                 fy = fmaf(f, dy, fy);
                 fz = fmaf(f, dz, fz);
@@ -729,9 +734,21 @@ __global__ __launch_bounds__(MT) void k_mgrav(const DBody *__restrict__ b, int n
             ay += (double)fy;
             az += (double)fz;
         } else {
+#pragma unroll 2
             for (unsigned u = a; u < e; ++u) {
-                if (!sx[u] || u == (unsigned)t) continue;
-                term_fast(me.x, me.y, me.z, Rk, pm[u], rr[u], G, s, ax, ay, az);
+                const double4 p = pw[u];
+                const double w = u == (unsigned)t ? 0.0 : p.w;
+                const double dx = p.x - me.x, dy = p.y - me.y, dz = p.z - me.z;
+                const double ee = SOFT ? s * (Rk + rr[u]) : 0.0;
+                // 1e-200 in the chain: the lane's own record adds 0 * finite.
+                const double r2 = __builtin_fma(
+                    dx, dx, __builtin_fma(dy, dy, __builtin_fma(dz, dz, __builtin_fma(ee, ee, 1e-200))));
+                double y = __builtin_amdgcn_rsq(r2);
+                y = y * __builtin_fma(-0.5 * r2, y * y, 1.5);
+                const double f = w * y * (y * y);
+                ax = __builtin_fma(f, dx, ax);
+                ay = __builtin_fma(f, dy, ay);
+                az = __builtin_fma(f, dz, az);
             }
         }
     }
@@ -2536,13 +2553,25 @@ static ncg_status mesh_gravity_core(ncg_ctx *ctx, const DBody *db, int32_t n, co
     } else {
         ProfScope ps(ctx, st, "mesh_gravity");
         const unsigned *skey = dbuf<unsigned>(ctx, "m_skey", nkey);
+        double4 *pw = dbuf<double4>(ctx, "g_pw", std::max(ins, 1));
+        if (!pw) return NCG_ERR_NOMEM;
         const dim3 gs(cdiv(std::max(ins, 1), MT));
-        if (prec == NCG_F64_FAST)
-            hipLaunchKernelGGL(k_mgrav<1>, gs, dim3(MT), 0, st, db, (int)n, dex, ins, cs, ce, pm, rr,
-                               sx, sid, skey, (int)d, dcen, G, s, dacc, dct);
+        if (ins > 0)
+            hipLaunchKernelGGL(k_gweights, gs, dim3(MT), 0, st, (const double4 *)pm, (const int *)sx,
+                               ins, G, pw);
+        const bool soft = s != 0.0;
+        if (prec == NCG_F64_FAST && soft)
+            hipLaunchKernelGGL((k_mgrav<1, true>), gs, dim3(MT), 0, st, db, (int)n, dex, ins, cs, ce,
+                               pw, rr, sx, sid, skey, (int)d, dcen, G, s, dacc, dct);
+        else if (prec == NCG_F64_FAST)
+            hipLaunchKernelGGL((k_mgrav<1, false>), gs, dim3(MT), 0, st, db, (int)n, dex, ins, cs, ce,
+                               pw, rr, sx, sid, skey, (int)d, dcen, G, s, dacc, dct);
+        else if (soft)
+            hipLaunchKernelGGL((k_mgrav<2, true>), gs, dim3(MT), 0, st, db, (int)n, dex, ins, cs, ce,
+                               pw, rr, sx, sid, skey, (int)d, dcen, G, s, dacc, dct);
         else
-            hipLaunchKernelGGL(k_mgrav<2>, gs, dim3(MT), 0, st, db, (int)n, dex, ins, cs, ce, pm, rr,
-                               sx, sid, skey, (int)d, dcen, G, s, dacc, dct);
+            hipLaunchKernelGGL((k_mgrav<2, false>), gs, dim3(MT), 0, st, db, (int)n, dex, ins, cs, ce,
+                               pw, rr, sx, sid, skey, (int)d, dcen, G, s, dacc, dct);
         hipLaunchKernelGGL(k_mgrav_rest, dim3(cdiv(n, MT)), dim3(MT), 0, st, db, (int)n, dex, key,
                            (int)nkey, dcen,
                            G, s, dacc, dct);
