@@ -32,11 +32,93 @@ PEAK_FP32_VECTOR_TFLOPS = 157.3    # MI355X published FP32 vector peak
 PEAK_FP64_VECTOR_TFLOPS = 78.6
 
 
-# HBM bytes per launch of the mixed pair kernel at N = 262144 with the tuned
-# shape (nsplit 128, theta 0.20): FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (scales from a
-# 256 MiB copy), from rocprofv3 --pmc runs of this script, summarised in
-# profiles/k_pairs_mixed_n262144_pmc.md.  Mostly the j-split partial sums.
-PAIRS_TRAFFIC_BYTES = (110639 * 2.000 + 786432 * 1.000) * 1024
+COPY_MIB = 256  # plain copy that calibrates the HBM byte counters
+
+
+def traffic_child(n):
+    """Run under rocprofv3 --pmc by traffic_run: one mixed evaluation of the
+    workload (its pair kernel is the launch measured) and three passes of a
+    plain elementwise copy of COPY_MIB MiB (reads and writes that many bytes
+    once each), the reference point of the counters."""
+    import torch
+    from ncorpion_amd import Context
+    b = make_workload(n)
+    ctx = Context(0)
+    bt = torch.from_numpy(b).cuda()
+    acc = torch.zeros((n, 3), dtype=torch.float64, device="cuda")
+    ctx.gravity_allpairs_dev(bt, acc, None, None, 1.0, 1e-6, "mixed")
+    ctx.synchronize()
+    x = torch.zeros(COPY_MIB * (1 << 18), dtype=torch.float32, device="cuda")
+    for _ in range(3):
+        y = x * 1.0
+    torch.cuda.synchronize()
+    del y
+    ctx.close()
+    return 0
+
+
+def _pmc_pass(counter, n, timeout):
+    """Mean value of `counter` per dispatch of the pair kernel and of the copy
+    kernel, from one rocprofv3 pass over traffic_child."""
+    import csv
+    import glob
+    import shutil
+    prof = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    if not os.path.exists(prof):
+        return None
+    with tempfile.TemporaryDirectory(dir="/tmp") as td:
+        cmd = [prof, "--pmc", counter, "--output-format", "csv", "-d", td, "-o", "pmc", "--",
+               sys.executable, os.path.abspath(__file__), "--traffic-child", "--n", str(n)]
+        env = dict(os.environ, TMPDIR="/tmp")
+        try:
+            r = subprocess.run(cmd, cwd=td, env=env, stdout=subprocess.PIPE,
+                               stderr=subprocess.STDOUT, text=True, timeout=timeout)
+        except subprocess.TimeoutExpired:
+            return None
+        files = glob.glob(os.path.join(td, "**", "*counter_collection.csv"), recursive=True)
+        if not files:
+            return None
+        per = {}  # dispatch -> (kernel name, summed value)
+        for fn in files:
+            with open(fn, newline="") as f:
+                for row in csv.DictReader(f):
+                    if row.get("Counter_Name") != counter:
+                        continue
+                    key = (fn, row.get("Dispatch_Id"))
+                    name = row.get("Kernel_Name", "")
+                    v = float(row.get("Counter_Value", 0.0))
+                    per[key] = (name, per.get(key, (name, 0.0))[1] + v)
+    pairs = [v for name, v in per.values() if "k_pairs" in name]
+    copies = [v for name, v in per.values() if "elementwise" in name or "vectorized" in name]
+    if not pairs or not copies:
+        return None
+    copies = sorted(copies)[-3:]  # the three copy passes (the largest elementwise launches)
+    return sum(pairs) / len(pairs), sum(copies) / len(copies), r.returncode
+
+
+def traffic_run(n, timeout=150):
+    """HBM bytes per launch of the mixed pair kernel: FETCH_SIZE and
+    WRITE_SIZE (KiB per dispatch, each from its own rocprofv3 --pmc pass),
+    each scaled by what the same counter reads for a copy of known size."""
+    out = {}
+    copy_kib = COPY_MIB * 1024.0
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        r = _pmc_pass(counter, n, timeout)
+        if r is None:
+            return None
+        kib, copy, _ = r
+        scale = copy_kib / copy if copy > 0 else float("nan")
+        out[counter] = {"kib_per_dispatch": kib, "copy_kib": copy, "scale": scale,
+                        "bytes": kib * scale * 1024.0}
+    out["bytes"] = out["FETCH_SIZE"]["bytes"] + out["WRITE_SIZE"]["bytes"]
+    return out
+
+
+def pairs_algorithmic_bytes(n):
+    """Bytes the pair kernel must move per launch: the sorted FP64 records
+    (32 B) and softening lengths (8 B) and the FP32 far-form records (16 B)
+    read once, 24 B of acceleration partials written per body."""
+    return n * (32 + 8 + 16 + 24)
 
 
 def make_workload(n, seed=0):
@@ -385,6 +467,80 @@ def cpu_c1_baseline(timeout=300):
                       "wall time inside vector_field and brute_force"}
 
 
+DROPIN = os.path.join(ROOT, "ncorpion_amd", "glue", "_build")
+
+
+def _timing_lines(text):
+    """{("call" | "kernel", name): {field: value}} from the glue's NCG_TIMING lines."""
+    out = {}
+    for line in text.splitlines():
+        f = line.split()
+        if len(f) >= 4 and f[0] == "ncg_timing":
+            out[(f[1], f[2])] = {k: float(v) for k, v in (x.split("=") for x in f[3:])}
+    return out
+
+
+def _run_dropin(cfg, bodies, precision, timeout=600):
+    """Runs the drop-in NcorpiON of configuration `cfg` (NcorpiON's own main
+    and integrator, its hot-path calls redirected to libncg through the glue)
+    on `bodies` with NCG_TIMING; returns (wall seconds, timing lines)."""
+    from ncorpion_amd import ic
+    exe = os.path.join(DROPIN, cfg, "ncorpion_gpu")
+    if not os.path.exists(exe):
+        return None
+    with tempfile.TemporaryDirectory() as td:
+        ic.write_init_txt(os.path.join(td, "init.txt"), bodies)
+        env = dict(os.environ, NCG_TIMING="1", NCG_PRECISION=precision)
+        t0 = time.perf_counter()
+        r = subprocess.run([exe], cwd=td, env=env, stdout=subprocess.PIPE,
+                           stderr=subprocess.PIPE, text=True, timeout=timeout)
+        wall = time.perf_counter() - t0
+    if r.returncode != 0:
+        raise RuntimeError("drop-in %s failed: %s" % (cfg, r.stderr[-2000:]))
+    return wall, _timing_lines(r.stderr)
+
+
+def dropin_run():
+    """The drop-in NcorpiON a user builds (ncorpion_amd/glue/build_dropin.py),
+    whole runs with per-call uploads: the brute-force workload (2000 bodies,
+    200 steps, inelastic bounces), exact and mixed fields, steps/s counted
+    over the time inside vector_field and brute_force (as the reference's
+    baseline is); and the mesh workload (2^20 bodies, 3 steps), with the time
+    inside mesh(), vector_field and get_neighbours_mesh split into device
+    kernel time and the rest (uploads, host replay of mesh()'s visiting
+    order, glue)."""
+    from ncorpion_amd import ic
+    out = {}
+    b1 = ic.disk(C1_N, seed=0, mass=1e-8, radius=2e-4)
+    c1 = {"n": C1_N, "steps": C1_STEPS}
+    for prec in ("exact", "mixed"):
+        r = _run_dropin("c1", b1, prec)
+        if r is None:
+            return None
+        wall, t = r
+        hot = sum(t.get(("call", k), {}).get("seconds", 0.0) for k in ("vector_field", "brute_force"))
+        c1[prec] = {"steps_per_s": C1_STEPS / hot, "seconds_in_vector_field_and_brute_force": hot,
+                    "process_wall_seconds": wall,
+                    "vector_field_calls": int(t.get(("call", "vector_field"), {}).get("calls", 0))}
+    out["c1"] = c1
+    r = _run_dropin("c2", mesh_workload(), "exact")
+    if r is not None:
+        wall, t = r
+        calls = {k: t[("call", k)] for k in ("mesh", "vector_field", "get_neighbours_mesh")
+                 if ("call", k) in t}
+        kern = {k[1]: v for k, v in t.items() if k[0] == "kernel"}
+        hot = sum(v["seconds"] for v in calls.values())
+        dev = sum(v["ms"] for v in kern.values()) * 1e-3
+        out["c2"] = {"n": MESH_N, "steps": 3, "precision": "exact",
+                     "seconds_per_step": hot / 3, "device_kernel_seconds_per_step": dev / 3,
+                     "host_and_transfer_seconds_per_step": (hot - dev) / 3,
+                     "calls": {k: {"calls": int(v["calls"]), "seconds": v["seconds"]}
+                               for k, v in calls.items()},
+                     "kernels_ms": {k: v["ms"] for k, v in kern.items()},
+                     "process_wall_seconds": wall}
+    return out
+
+
 def ensemble_run(ctx, torch, nsys=512, steps=100):
     """512 independent disks (seeds 0..511), N log-uniform in [256, 8192],
     m = 1e-8, R = 2e-4, brute-force gravity + collision detection, mixed form,
@@ -532,9 +688,14 @@ def main():
     ap.add_argument("--no-collisions", action="store_true")
     ap.add_argument("--no-leapfrog", action="store_true")
     ap.add_argument("--no-output", action="store_true")
+    ap.add_argument("--no-dropin", action="store_true")
+    ap.add_argument("--no-traffic", action="store_true")
+    ap.add_argument("--traffic-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run a plain device copy of this size (HBM counter calibration)")
     args = ap.parse_args()
+    if args.traffic_child:
+        return traffic_child(args.n)
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (the path is measured at one GPU's "
               "share); --gpus %d is not supported" % args.gpus, file=sys.stderr)
@@ -608,11 +769,18 @@ def main():
         "roofline": {"bound": "valu", "achieved": achieved_tflops,
                      "peak": PEAK_FP32_VECTOR_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved_tflops / PEAK_FP32_VECTOR_TFLOPS,
-                     "traffic": PAIRS_TRAFFIC_BYTES if n == 262144 else None,
+                     "traffic": None,
                      "kernel": "k_pairs<mixed>", "kernel_avg_ms": kavg_ms,
                      "kernel_launches": nl_mix,
                      "flop_per_interaction": FLOP_PER_INTERACTION},
     }
+
+    if not args.no_traffic:
+        tr = traffic_run(n)
+        if tr is not None:
+            out["roofline"]["traffic"] = tr["bytes"]
+            out["roofline"]["traffic_counters"] = tr
+            out["roofline"]["algorithmic_bytes"] = pairs_algorithmic_bytes(n)
 
     if not args.no_fp64:
         fsteps = max(1, args.steps // 4)
@@ -653,6 +821,8 @@ def main():
         out["mesh_loop"] = mesh_loop_run(ctx, torch)
     if not args.no_output:
         out["output"] = output_run(ctx)
+    if not args.no_dropin:
+        out["dropin"] = dropin_run()
     if not args.no_ensemble:
         out["ensemble"] = ensemble_run(ctx, torch)
     if not args.no_adjoint:

@@ -31,6 +31,8 @@
  * Environment:
  *   NCG_DEVICE     GPU ordinal (default 0)
  *   NCG_PRECISION  exact (default; bit-identical to the reference), fast64, mixed
+ *   NCG_TIMING     when set: wall time per replaced function (ncg_wrap.c) and
+ *                  kernel time per kernel group, printed to stderr at exit
  *
  * Errors follow the reference convention: message on stderr, then abort().
  */
@@ -61,8 +63,26 @@ static void ncg_die(const char *what, ncg_status st)
     abort();
 }
 
+/* With NCG_TIMING set, the library times its kernels with HIP events and the
+ * totals are printed to stderr at exit, one line per kernel group:
+ *   ncg_timing kernel <name> launches=<count> ms=<sum> */
+static int g_timing = 0;
+
 static void ncg_release(void)
 {
+    static const char *const kernels[] = {"exact", "pairs_fast64", "pairs_mixed",
+                                          "candidates", "mesh_bin", "mesh_count",
+                                          "mesh_gravity", "mesh_lists", "mesh_replay"};
+    size_t k;
+    if (g_ctx && g_timing) {
+        for (k = 0; k < sizeof kernels / sizeof kernels[0]; ++k) {
+            double ms = 0.0;
+            int64_t launches = 0;
+            if (ncg_profile_read(g_ctx, kernels[k], &ms, &launches, 1) == NCG_OK && launches > 0)
+                fprintf(stderr, "ncg_timing kernel %s launches=%lld ms=%.6f\n", kernels[k],
+                        (long long)launches, ms);
+        }
+    }
     if (g_ctx) ncg_destroy(g_ctx);
     g_ctx = NULL;
 }
@@ -87,6 +107,11 @@ static ncg_ctx *ctx(void)
     else if (p && strcmp(p, "exact")) {
         fprintf(stderr, "Error : ncg drop-in: NCG_PRECISION must be exact, fast64 or mixed.\n");
         abort();
+    }
+    if (getenv("NCG_TIMING")) {
+        g_timing = 1;
+        st = ncg_profile_enable(g_ctx, 1);
+        if (st != NCG_OK) ncg_die("ncg_profile_enable", st);
     }
     atexit(ncg_release);
     return g_ctx;

@@ -89,7 +89,4 @@ def write_init_txt(path, bodies):
     """Write cartesian initial conditions in the format NcorpiON's populate()
     reads with fscanf("%lf") (structure.c:408-427): 8 numbers per body, %.17g
     so every double round-trips exactly."""
-    with open(path, "w") as f:
-        for row in bodies:
-            f.write(" ".join("%.17g" % v for v in row))
-            f.write("\n")
+    np.savetxt(path, np.asarray(bodies, dtype=np.float64), fmt="%.17g")

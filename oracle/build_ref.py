@@ -13,7 +13,9 @@ repository; only the linked executables land in oracle/_ref/<config>/:
   ref_probe               function-level probe (oracle/ref_probe.c)
   ncorpion_trace          the reference's main + call tracer (oracle/ref_trace.c)
   ncorpion_dropin_trace   same, with the GPU drop-in (needs ncorpion_amd/libncg.so)
-  ncorpion_dropin         the drop-in NcorpiON a user would build (no tracer)
+
+(The drop-in NcorpiON a user builds, without the tracer, is made by
+ncorpion_amd/glue/build_dropin.py.)
 
 Run it from any directory:  python3 oracle/build_ref.py [config ...]
 """
@@ -277,8 +279,6 @@ def build(name):
     link = ["-L" + lib_dir, "-lncg", rpath, "-lm"]
     run([CC] + CFLAGS + inc + ["-DTRACE_DROPIN", os.path.join(HERE, "ref_trace.c"), glue_o, main_o]
         + objs + [trace_wraps] + link + ["-o", os.path.join(out, "ncorpion_dropin_trace")])
-    run([CC] + CFLAGS + [os.path.join(glue, "ncg_wrap.c"), glue_o, main_o] + objs + [wraps] + link
-        + ["-o", os.path.join(out, "ncorpion_dropin")])
 
 
 def main(argv):

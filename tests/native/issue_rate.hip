@@ -6,6 +6,9 @@
 //   far: the FP32 far-field interaction of the mixed pair kernel (3 adds,
 //     3 for r^2, v_rsq_f32, 3 multiplies, 3 FMAs), j records read with
 //     wave-uniform addresses from a 128-entry tile as in the kernel.
+//   add64 / add64+cvt+add32: FP64 adds alone, and each followed by its
+//     rounding to FP32 and an FP32 add (the separations of the ensemble's
+//     mixed form): the difference prices v_cvt_f32_f64.
 //
 //   chain64 / chain32: one wavefront, one dependent chain of adds (the
 //     body-order sums of the exact mesh field are such chains): the
@@ -120,6 +123,41 @@ __global__ __launch_bounds__(WG) void k_far(const float4 *__restrict__ ju, int i
     out[blockIdx.x * WG + threadIdx.x] = fx + fy + fz;
 }
 
+__global__ __launch_bounds__(WG) void k_add64(double *out, int iters, double b)
+{
+    double a[8];
+    for (int q = 0; q < 8; ++q) a[q] = threadIdx.x * 1e-3 + q;
+    for (int it = 0; it < iters; ++it) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) a[q] = a[q] + b;
+    }
+    double s = 0.;
+    for (int q = 0; q < 8; ++q) s += a[q];
+    out[blockIdx.x * WG + threadIdx.x] = s;
+}
+
+// 8 chains of (v_add_f64, v_cvt_f32_f64, v_add_f32): the FP64 separation
+// rounded to FP32 of the ensemble's mixed form.
+__global__ __launch_bounds__(WG) void k_cvt(float *out, int iters, double b)
+{
+    double d[8];
+    float a[8];
+    for (int q = 0; q < 8; ++q) {
+        d[q] = threadIdx.x * 1e-3 + q;
+        a[q] = 0.f;
+    }
+    for (int it = 0; it < iters; ++it) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            d[q] = d[q] + b;
+            a[q] += (float)d[q];
+        }
+    }
+    float s = 0.f;
+    for (int q = 0; q < 8; ++q) s += a[q];
+    out[blockIdx.x * WG + threadIdx.x] = s;
+}
+
 template <class T>
 __global__ __launch_bounds__(64) void k_chain(T *out, int iters, T b)
 {
@@ -161,11 +199,13 @@ int main()
     CK(hipEventCreate(&e1));
     const int iters = 4096, far_iters = 64;
     const double waves = blocks * (WG / 64.0);
-    // kind: 0 fma32 x16, 1 add32, 2 mul32, 3 fma64, 4 rsq32, 5 far
-    const char *names[6] = {"fma32", "add32", "mul32", "fma64", "rsq32+add32", "far"};
-    const double instr[6] = {16.0 * iters, 8.0 * iters, 8.0 * iters, 8.0 * iters, 16.0 * iters,
-                             13.0 * 128 * far_iters};
-    for (int kind = 0; kind < 6; ++kind) {
+    // kind: 0 fma32 x16, 1 add32, 2 mul32, 3 fma64, 4 rsq32, 5 far, 6 add64,
+    // 7 add64+cvt+add32
+    const char *names[8] = {"fma32", "add32", "mul32", "fma64", "rsq32+add32", "far", "add64",
+                            "add64+cvt+add32"};
+    const double instr[8] = {16.0 * iters, 8.0 * iters, 8.0 * iters, 8.0 * iters, 16.0 * iters,
+                             13.0 * 128 * far_iters, 8.0 * iters, 24.0 * iters};
+    for (int kind = 0; kind < 8; ++kind) {
         auto launch = [&]() {
             switch (kind) {
             case 0: hipLaunchKernelGGL(k_fma32<16>, dim3(blocks), dim3(WG), 0, 0, o32, iters, 0.999f, 1e-3f); break;
@@ -173,6 +213,8 @@ int main()
             case 2: hipLaunchKernelGGL(k_mul32, dim3(blocks), dim3(WG), 0, 0, o32, iters, 0.999f); break;
             case 3: hipLaunchKernelGGL(k_fma64, dim3(blocks), dim3(WG), 0, 0, o64, iters, 0.999, 1e-3); break;
             case 4: hipLaunchKernelGGL(k_rsq32, dim3(blocks), dim3(WG), 0, 0, o32, iters, 0.5f); break;
+            case 6: hipLaunchKernelGGL(k_add64, dim3(blocks), dim3(WG), 0, 0, o64, iters, 1e-3); break;
+            case 7: hipLaunchKernelGGL(k_cvt, dim3(blocks), dim3(WG), 0, 0, o32, iters, 1e-3); break;
             default: hipLaunchKernelGGL(k_far, dim3(blocks), dim3(WG), 0, 0, ju, far_iters, o32); break;
             }
         };
