@@ -199,7 +199,7 @@ ncg_status ncg_collide_candidates_allpairs(ncg_ctx *ctx, const ncg_body *bodies,
  * get_neighbours_mesh() (mode 1, physics.c:1094-1118).  It bins the bodies in
  * the reference's hash-table cells (collision.c:175-261: cell side gam,
  * collision_cube_cells cells per axis plus one guard layer, key i + j*d + k*d^2)
- * by a counting sort on the GPU, counts total_neighbours exactly, finds the
+ * by a (key, id) radix sort on the GPU, counts total_neighbours exactly, finds the
  * collision candidates among neighbour pairs (mode 0) and replays them on the
  * host through ops in the reference's visiting order (bodies by ascending id;
  * for each, neighbour cells 26..0 and ascending id within a cell).  Bodies
@@ -212,9 +212,10 @@ ncg_status ncg_collide_candidates_allpairs(ncg_ctx *ctx, const ncg_body *bodies,
  * create bodies, which mesh mode appends at slots n, n + 1, ... (fragmentation,
  * physics.c:994-996 and structure.c:857-861; reported through ops->created,
  * the arrays must cover them) and whose pairs it registers in the pair list
- * (physics.c:1002-1010).  A pass with bodies larger than gam accepts only
- * outcomes that keep both bodies.  The next ncg_mesh_gravity then takes the
- * n slots of the pass plus the appended ones.
+ * (physics.c:1002-1010); a big body's later visit scans every body up to
+ * the live largest id, appended ones included, and a body an outcome moves
+ * across gam is visited in its new class.  The next ncg_mesh_gravity then
+ * takes the n slots of the pass plus the appended ones.
  *   ops may be NULL in mode 1.
  */
 ncg_status ncg_mesh_pass(ncg_ctx *ctx, const ncg_body *bodies, int32_t n,
@@ -249,7 +250,7 @@ typedef struct {
                                       close to the central body */
     double r_high_dumping;         /* high_dumping_threshold: checked, no body may go this far */
     int32_t cells;                 /* collision_cube_cells */
-    int32_t pad_;
+    int32_t host_passes;           /* out: mesh() passes of the call replayed on the host */
 } ncg_mesh_loop_params;
 
 /* GPU-resident mesh loop: NcorpiON's integration_mesh (SBAB1, rk4.c:662-837)
@@ -261,12 +262,17 @@ typedef struct {
  * did_collide reset (one_collision_only) and mesh-size update.  flags bit 1:
  * the half kick before the loop (rk4.c:697-717); bit 2: the backward half
  * kick after it (rk4.c:807-812, the central record taking CM_buffer's kick).
- * Needs every body at most the mesh size and no body crossing the disruption
- * or high-dumping thresholds (end_of_timestep's removals are not applied:
- * NCG_ERR_INVALID).  One synchronisation per mesh pass (its counters set the
- * next mesh size), two with NCG_F64_EXACT (the binned count sizes the field's
- * neighbourhood lists, built on a second stream during the pass).  d_exists may be written (a body moved into an empty slot
- * 0..2).  *n_collisions (may be NULL): collisions resolved. */
+ * A mesh() pass the device replay cannot take (a body larger than the mesh
+ * size, or more candidates or pending re-tests than the replay holds) is
+ * undone from a snapshot and replayed on the host with the same decision and
+ * bounce (ncg_mesh_pass with the built-in ops): the results are the same,
+ * the pass slower; params->host_passes counts them.  No body may cross the
+ * disruption or high-dumping thresholds (end_of_timestep's removals are not
+ * applied: NCG_ERR_INVALID).  One synchronisation per mesh pass (its
+ * counters set the next mesh size), two with NCG_F64_EXACT (the binned count
+ * sizes the field's neighbourhood lists, built on a second stream during the
+ * pass).  d_exists may be written (a body moved into an empty slot 0..2).
+ * *n_collisions (may be NULL): collisions resolved. */
 ncg_status ncg_mesh_leapfrog_dev(ncg_ctx *ctx, ncg_body *d_bodies, int32_t n, int32_t *d_exists,
                                  int32_t *d_did_collide, double *d_central, double grav_const,
                                  double softening, ncg_precision prec, double dt, int32_t nsteps,

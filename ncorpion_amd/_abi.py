@@ -44,7 +44,7 @@ class MeshLoopParams(ctypes.Structure):
     _fields_ = [("mesh_size", ctypes.c_double), ("mesh_size_min", ctypes.c_double),
                 ("target_neighbours", ctypes.c_double), ("r_disruption", ctypes.c_double),
                 ("r_high_dumping", ctypes.c_double), ("cells", ctypes.c_int32),
-                ("pad_", ctypes.c_int32)]
+                ("host_passes", ctypes.c_int32)]
 
 
 class BuiltinCollider(ctypes.Structure):
@@ -337,6 +337,7 @@ class Context:
             ctypes.byref(mp), float(collision_f), int(one_collision_only), flags,
             ctypes.byref(nc), stream))
         params["gam"] = mp.mesh_size
+        params["host_passes"] = int(mp.host_passes)
         return int(nc.value)
 
     def mesh_gravity_dev(self, bodies_t, acc_t, central_t=None, cm_t=None, exists_t=None, G=1.0,

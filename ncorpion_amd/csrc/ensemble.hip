@@ -450,7 +450,7 @@ static ncg_status ens_pass(ncg_ctx *ctx, const EnsPlan &P, const int *doff, cons
 {
     unsigned *counter = dbuf<unsigned>(ctx, "e_counter", 2);
     double *cpart = dbuf<double>(ctx, "e_cpart", 3 * (size_t)std::max(nunits, 1));
-    const unsigned cap = ENS_CAND_CAP;
+    const unsigned cap = ncg_replay_cap(ctx, ENS_CAND_CAP);
     int2 *cand = dbuf<int2>(ctx, "e_cand", cap);
     if (!counter || !cpart || !cand) return NCG_ERR_NOMEM;
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, counter, 2);
@@ -552,15 +552,83 @@ extern "C" ncg_status ncg_ensemble_rk4_dev(ncg_ctx *ctx, ncg_body *d_bodies,
     const unsigned sgrid = cdiv(ntot + nsys, 256);
     const double G = grav_const, soft = softening;
 
+    // The replay's capacity (candidates of a pass, per system: candidates and
+    // changed bodies in LDS) is checked every ENS_CHECK steps.  A window in
+    // which a pass exceeded it runs again from a snapshot taken at its start,
+    // step by step with each collision pass checked at once: a pass that
+    // exceeds the capacity is undone and every system's pass is replayed on
+    // the host with the same decision and bounce (ncg_brute_force_pass with
+    // the built-in ops), so results stay those of an unbounded replay.
+    constexpr int ENS_CHECK = 16;
+    DBody *snap_b = collide ? dbuf<DBody>(ctx, "e_snap_b", ntot) : nullptr;
+    DBody *pass_b = collide ? dbuf<DBody>(ctx, "e_pass_b", ntot) : nullptr;
+    double *snap_c = collide ? dbuf<double>(ctx, "e_snap_c", 8 * (size_t)nsys) : nullptr;
+    unsigned *hst = pbuf<unsigned>(ctx, "e_hstat", 4);
+    if ((collide && (!snap_b || !pass_b || !snap_c)) || !hst) return NCG_ERR_NOMEM;
+    int64_t host_coll = 0;
+    unsigned snap_coll = 0;
+    auto read_stat = [&](unsigned *hs) -> ncg_status {
+        NCG_HIP(hipMemcpyAsync(hst, stat, sizeof(unsigned) * 2, hipMemcpyDeviceToHost, st));
+        NCG_HIP(hipStreamSynchronize(st));
+        hs[0] = hst[0];
+        hs[1] = hst[1];
+        return NCG_OK;
+    };
+    auto write_stat = [&](unsigned coll) -> ncg_status {
+        hst[2] = coll;
+        hst[3] = 0u;
+        NCG_HIP(hipMemcpyAsync(stat, hst + 2, sizeof(unsigned) * 2, hipMemcpyHostToDevice, st));
+        // A replay that ran out of room for changed bodies leaves marks set.
+        hipLaunchKernelGGL(k_zero_u32, dim3(cdiv(ntot / 4 + 1, 256)), dim3(256), 0, st,
+                           reinterpret_cast<unsigned *>(dirty), ntot / 4 + 1);
+        NCG_CHECK_LAUNCH();
+        NCG_HIP(hipStreamSynchronize(st));
+        return NCG_OK;
+    };
+    auto device_pass = [&]() -> ncg_status {
+        ncg_status r = ens_pass(ctx, P, D.off, D.coll, (int)P.coll.size(), nullptr, nullptr, y0,
+                                d_centrals, G, soft, prec, dt, acc, cm, st);
+        if (r != NCG_OK) return r;
+        return ncg_bf_replay_launch(ctx, y0, nullptr, nullptr, 0, D.off, nsys, cand, counter + 1,
+                                    nullptr, ncg_replay_cap(ctx, ENS_CAND_CAP), dt, collision_f,
+                                    dirty, stat, st);
+    };
+    auto checked_pass = [&]() -> ncg_status {
+        unsigned hs[2];
+        ncg_status r = read_stat(hs);
+        if (r != NCG_OK) return r;
+        const unsigned before = hs[0];
+        NCG_HIP(hipMemcpyAsync(pass_b, y0, sizeof(DBody) * (size_t)ntot, hipMemcpyDeviceToDevice, st));
+        if ((r = device_pass()) != NCG_OK) return r;
+        if ((r = read_stat(hs)) != NCG_OK) return r;
+        if (hs[1] == 0u) return NCG_OK;
+        NCG_HIP(hipMemcpyAsync(y0, pass_b, sizeof(DBody) * (size_t)ntot, hipMemcpyDeviceToDevice, st));
+        if ((r = write_stat(before)) != NCG_OK) return r;
+        std::vector<DBody> hb((size_t)ntot);
+        NCG_HIP(hipMemcpyAsync(hb.data(), y0, sizeof(DBody) * (size_t)ntot, hipMemcpyDeviceToHost, st));
+        NCG_HIP(hipStreamSynchronize(st));
+        for (int sy = 0; sy < nsys; ++sy) {
+            const int ns = P.off[sy + 1] - P.off[sy];
+            if (ns < 2) continue;
+            ncg_builtin_collider bc = {};
+            bc.bodies = reinterpret_cast<ncg_body *>(hb.data() + P.off[sy]);
+            bc.timestep = dt;
+            bc.collision_f = collision_f;
+            ncg_replay_ops ops;
+            if ((r = ncg_builtin_replay_ops(&bc, &ops)) != NCG_OK) return r;
+            if ((r = ncg_brute_force_pass(ctx, bc.bodies, ns, nullptr, dt, &ops, nullptr)) != NCG_OK)
+                return r;
+            host_coll += bc.collisions;
+        }
+        NCG_HIP(hipMemcpyAsync(y0, hb.data(), sizeof(DBody) * (size_t)ntot, hipMemcpyHostToDevice, st));
+        NCG_HIP(hipStreamSynchronize(st));
+        return NCG_OK;
+    };
     // One step: with collisions, the brute-force pass of every system and its
     // replay on the device, then the four RK4 stages.
-    auto step_body = [&]() -> ncg_status {
+    auto step_body = [&](bool checked) -> ncg_status {
         if (collide) {
-            ncg_status r = ens_pass(ctx, P, D.off, D.coll, (int)P.coll.size(), nullptr, nullptr, y0,
-                                    d_centrals, G, soft, prec, dt, acc, cm, st);
-            if (r != NCG_OK) return r;
-            r = ncg_bf_replay_launch(ctx, y0, nullptr, nullptr, 0, D.off, nsys, cand, counter + 1,
-                                     nullptr, ENS_CAND_CAP, dt, collision_f, dirty, stat, st);
+            ncg_status r = checked ? checked_pass() : device_pass();
             if (r != NCG_OK) return r;
         }
         for (int s = 0; s < 4; ++s) {
@@ -573,47 +641,83 @@ extern "C" ncg_status ncg_ensemble_rk4_dev(ncg_ctx *ctx, ncg_body *d_bodies,
         }
         return NCG_OK;
     };
+    auto snapshot = [&]() -> ncg_status {
+        unsigned hs[2];
+        ncg_status r = read_stat(hs);
+        if (r != NCG_OK) return r;
+        snap_coll = hs[0];
+        NCG_HIP(hipMemcpyAsync(snap_b, y0, sizeof(DBody) * (size_t)ntot, hipMemcpyDeviceToDevice, st));
+        NCG_HIP(hipMemcpyAsync(snap_c, d_centrals, sizeof(double) * 8 * (size_t)nsys,
+                               hipMemcpyDeviceToDevice, st));
+        return NCG_OK;
+    };
 
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph = nullptr;
+    struct GraphGuard {
+        hipGraphExec_t &e;
+        hipGraph_t &g;
+        ~GraphGuard()
+        {
+            if (e) (void)hipGraphExecDestroy(e);
+            if (g) (void)hipGraphDestroy(g);
+        }
+    } guard{exec, graph};
+    if (collide) {
+        ncg_status r = snapshot();
+        if (r != NCG_OK) return r;
+    }
+    int32_t win0 = 0;
     for (int32_t step = 0; step < nsteps; ++step) {
         if (!use_graph || step == 0) {
             // Eager (the first step also allocates everything the graph uses).
-            ncg_status r = step_body();
+            ncg_status r = step_body(false);
             if (r != NCG_OK) return r;
-            continue;
-        }
-        if (!exec) {
-            NCG_HIP(hipStreamSynchronize(st));
-            ctx->capturing = true;
-            NCG_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-            ncg_status r = step_body();
-            hipError_t e = hipStreamEndCapture(st, &graph);
-            ctx->capturing = false;
-            if (r != NCG_OK) return r;
-            if (e != hipSuccess)
-                return ncg_fail(ctx, NCG_ERR_DEVICE, "stream capture failed: %s", hipGetErrorString(e));
-            if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
-                (void)hipGraphDestroy(graph);
-                return ncg_fail(ctx, NCG_ERR_DEVICE, "graph instantiation failed");
+        } else {
+            if (!exec) {
+                NCG_HIP(hipStreamSynchronize(st));
+                ctx->capturing = true;
+                NCG_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+                ncg_status r = step_body(false);
+                hipError_t e = hipStreamEndCapture(st, &graph);
+                ctx->capturing = false;
+                if (r != NCG_OK) return r;
+                if (e != hipSuccess)
+                    return ncg_fail(ctx, NCG_ERR_DEVICE, "stream capture failed: %s",
+                                    hipGetErrorString(e));
+                if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess)
+                    return ncg_fail(ctx, NCG_ERR_DEVICE, "graph instantiation failed");
             }
+            NCG_HIP(hipGraphLaunch(exec, st));
         }
-        NCG_HIP(hipGraphLaunch(exec, st));
+        if (!collide || ((step + 1) % ENS_CHECK != 0 && step + 1 != nsteps)) continue;
+        unsigned hs[2];
+        ncg_status r = read_stat(hs);
+        if (r != NCG_OK) return r;
+        if (hs[1] != 0u) {
+            // The window [win0, step] again, from its snapshot, checked.
+            NCG_HIP(hipMemcpyAsync(y0, snap_b, sizeof(DBody) * (size_t)ntot,
+                                   hipMemcpyDeviceToDevice, st));
+            NCG_HIP(hipMemcpyAsync(d_centrals, snap_c, sizeof(double) * 8 * (size_t)nsys,
+                                   hipMemcpyDeviceToDevice, st));
+            if ((r = write_stat(snap_coll)) != NCG_OK) return r;
+            for (int32_t q = win0; q <= step; ++q)
+                if ((r = step_body(true)) != NCG_OK) return r;
+        }
+        if ((r = snapshot()) != NCG_OK) return r;
+        win0 = step + 1;
     }
     unsigned hs[2] = {0, 0};
     NCG_HIP(hipMemcpyAsync(hs, stat, sizeof hs, hipMemcpyDeviceToHost, st));
     NCG_HIP(hipStreamSynchronize(st));
-    if (exec) {
-        (void)hipGraphExecDestroy(exec);
-        (void)hipGraphDestroy(graph);
-    }
-    if (n_collisions) *n_collisions = hs[0];
+    if (n_collisions) *n_collisions = hs[0] + host_coll;
     if (hs[1])
         return ncg_fail(ctx, NCG_ERR_CAPACITY,
                         "collision replay capacity exceeded (flags %u: 1 = more than %u candidates "
                         "in one pass, 2 = more than %d in one system, 4 = more than %d changed "
                         "bodies in one system); results are not valid",
-                        hs[1], ENS_CAND_CAP, NCG_REPLAY_RCAP, NCG_REPLAY_DCAP);
+                        hs[1], ncg_replay_cap(ctx, ENS_CAND_CAP), NCG_REPLAY_RCAP,
+                        NCG_REPLAY_DCAP);
     return NCG_OK;
 }
 

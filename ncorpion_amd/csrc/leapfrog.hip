@@ -350,15 +350,74 @@ extern "C" ncg_status ncg_mesh_leapfrog_dev(ncg_ctx *ctx, ncg_body *d_bodies, in
         ~ListsFlag() { m.want_lists = false; }
     } lists_flag{ctx->mesh};
     ctx->mesh.want_lists = prec == NCG_F64_EXACT;
+    // A mesh() pass the device replay cannot take -- a body larger than the
+    // mesh size (whose visit scans every body, collision.c:381-414), or more
+    // candidates or pending re-tests than it holds -- is undone (a snapshot
+    // of the bodies and did_collide before it) and replayed on the host with
+    // the same decision and bounce (ncg_mesh_pass with the built-in ops).
+    DBody *snap_b = dbuf<DBody>(ctx, "ml_snap_b", n);
+    int32_t *snap_dc = d_did_collide ? dbuf<int32_t>(ctx, "ml_snap_dc", n) : nullptr;
+    if (!snap_b || (d_did_collide && !snap_dc)) return NCG_ERR_NOMEM;
+    mp->host_passes = 0;
+    auto host_pass = [&](double h, double cube, ncg_mesh_stats &ms) -> ncg_status {
+        NCG_HIP(hipMemcpyAsync(b, snap_b, sizeof(DBody) * (size_t)n, hipMemcpyDeviceToDevice, st));
+        if (d_did_collide)
+            NCG_HIP(hipMemcpyAsync(d_did_collide, snap_dc, sizeof(int32_t) * (size_t)n,
+                                   hipMemcpyDeviceToDevice, st));
+        std::vector<DBody> hb((size_t)n);
+        std::vector<int32_t> hex(d_exists ? (size_t)n : 0), hdc(d_did_collide ? (size_t)n : 0);
+        NCG_HIP(hipMemcpyAsync(hb.data(), snap_b, sizeof(DBody) * (size_t)n, hipMemcpyDeviceToHost,
+                               st));
+        if (d_exists)
+            NCG_HIP(hipMemcpyAsync(hex.data(), d_exists, sizeof(int32_t) * (size_t)n,
+                                   hipMemcpyDeviceToHost, st));
+        if (d_did_collide)
+            NCG_HIP(hipMemcpyAsync(hdc.data(), snap_dc, sizeof(int32_t) * (size_t)n,
+                                   hipMemcpyDeviceToHost, st));
+        NCG_HIP(hipStreamSynchronize(st));
+        ncg_builtin_collider bc = {};
+        bc.bodies = reinterpret_cast<ncg_body *>(hb.data());
+        bc.exists = d_exists ? hex.data() : nullptr;
+        bc.did_collide = d_did_collide ? hdc.data() : nullptr;
+        bc.timestep = h;
+        bc.collision_f = collision_f;
+        bc.one_collision_only = one_collision_only;
+        ncg_replay_ops ops;
+        ncg_status r = ncg_builtin_replay_ops(&bc, &ops);
+        if (r != NCG_OK) return r;
+        r = ncg_mesh_pass(ctx, bc.bodies, n, bc.exists, h, mp->mesh_size, cube, mp->cells, 0, &ops,
+                          &ms);
+        if (r != NCG_OK) return r;
+        ms.collisions = bc.collisions;
+        NCG_HIP(hipMemcpyAsync(b, hb.data(), sizeof(DBody) * (size_t)n, hipMemcpyHostToDevice, st));
+        if (d_did_collide)
+            NCG_HIP(hipMemcpyAsync(d_did_collide, hdc.data(), sizeof(int32_t) * (size_t)n,
+                                   hipMemcpyHostToDevice, st));
+        NCG_HIP(hipStreamSynchronize(st));
+        ++mp->host_passes;
+        return NCG_OK;
+    };
     auto pass = [&](int mode, double h) -> ncg_status {
         largest3();
         NCG_CHECK_LAUNCH();
         ncg_mesh_stats ms;
         const double cube = mp->mesh_size * (double)mp->cells;
-        ncg_status r = ncg_mesh_pass_dev(ctx, d_bodies, n, d_exists, d_did_collide, h, mp->mesh_size, cube,
-                                         mp->cells, mode, collision_f, one_collision_only, &ms,
-                                         (void *)st);
+        if (mode == 0) {
+            NCG_HIP(hipMemcpyAsync(snap_b, b, sizeof(DBody) * (size_t)n, hipMemcpyDeviceToDevice, st));
+            if (d_did_collide)
+                NCG_HIP(hipMemcpyAsync(snap_dc, d_did_collide, sizeof(int32_t) * (size_t)n,
+                                       hipMemcpyDeviceToDevice, st));
+        }
+        unsigned flags = 0u;
+        ncg_status r = ncg_mesh_pass_dev_flags(ctx, d_bodies, n, d_exists, d_did_collide, h,
+                                               mp->mesh_size, cube, mp->cells, mode, collision_f,
+                                               one_collision_only, &ms, &flags, st);
         if (r != NCG_OK) return r;
+        if (flags != 0u) {
+            if (mode != 0)
+                return ncg_fail(ctx, NCG_ERR_INVALID, "neighbour pass failed (flags %u)", flags);
+            if ((r = host_pass(h, cube, ms)) != NCG_OK) return r;
+        }
         tn = ms.total_neighbours;
         ncoll += ms.collisions;
         return NCG_OK;
@@ -436,7 +495,7 @@ extern "C" ncg_status ncg_leapfrog_dev(ncg_ctx *ctx, ncg_body *d_bodies, int32_t
     const int32_t *ex = d_exists;
     const bool coll = collision_f > 0.0;
     const bool use_graph = (flags & 1) != 0;
-    const unsigned cap = 1u << 20;
+    const unsigned cap = ncg_replay_cap(ctx, 1u << 20);
     double *acc = dbuf<double>(ctx, "lf_acc", 3 * (size_t)n);
     double *cterm = dbuf<double>(ctx, "lf_cterm", 3 * (size_t)n);
     double *cm = dbuf<double>(ctx, "lf_cm", 4);
@@ -465,10 +524,103 @@ extern "C" ncg_status ncg_leapfrog_dev(ncg_ctx *ctx, ncg_body *d_bodies, int32_t
         return ncg_bf_replay_launch(ctx, b, ex, d_did_collide, one_collision_only, off, 1, cand,
                                     nullptr, cnt, cap, h, collision_f, dirty, stat, st);
     };
+    // The replay's capacity (candidates, changed bodies in LDS) is checked
+    // every LF_CHECK steps.  A window in which a pass exceeded it is run
+    // again from a snapshot of the state taken at the window's start, step
+    // by step, each collision pass checked at once: a pass that exceeds the
+    // capacity is undone (a snapshot before it) and replayed on the host with
+    // the same decision and bounce (ncg_brute_force_pass with the built-in
+    // ops), so results stay those of an unbounded replay.
+    constexpr int LF_CHECK = 64;
+    DBody *snap_b = dbuf<DBody>(ctx, "lf_snap_b", n);
+    DBody *pass_b = dbuf<DBody>(ctx, "lf_pass_b", n);
+    double *snap_c = dbuf<double>(ctx, "lf_snap_c", 8);
+    int32_t *snap_dc = d_did_collide ? dbuf<int32_t>(ctx, "lf_snap_dc", n) : nullptr;
+    int32_t *pass_dc = d_did_collide ? dbuf<int32_t>(ctx, "lf_pass_dc", n) : nullptr;
+    unsigned *hst = pbuf<unsigned>(ctx, "lf_hstat", 4);
+    if (!snap_b || !pass_b || !snap_c || (d_did_collide && (!snap_dc || !pass_dc)) || !hst)
+        return NCG_ERR_NOMEM;
+    int64_t host_coll = 0;      // collisions of passes replayed on the host
+    unsigned snap_coll = 0;     // device collision count at the snapshot
+    auto read_stat = [&](unsigned *hs) -> ncg_status {
+        NCG_HIP(hipMemcpyAsync(hst, stat, sizeof(unsigned) * 2, hipMemcpyDeviceToHost, st));
+        NCG_HIP(hipStreamSynchronize(st));
+        hs[0] = hst[0];
+        hs[1] = hst[1];
+        return NCG_OK;
+    };
+    auto write_stat = [&](unsigned coll) -> ncg_status {
+        hst[2] = coll;
+        hst[3] = 0u;
+        NCG_HIP(hipMemcpyAsync(stat, hst + 2, sizeof(unsigned) * 2, hipMemcpyHostToDevice, st));
+        NCG_HIP(hipStreamSynchronize(st));
+        return NCG_OK;
+    };
+    auto copy_state = [&](DBody *to_b, int32_t *to_dc, double *to_c, const DBody *from_b,
+                          const int32_t *from_dc, const double *from_c) -> ncg_status {
+        NCG_HIP(hipMemcpyAsync(to_b, from_b, sizeof(DBody) * (size_t)n, hipMemcpyDeviceToDevice, st));
+        if (to_dc && from_dc)
+            NCG_HIP(hipMemcpyAsync(to_dc, from_dc, sizeof(int32_t) * (size_t)n,
+                                   hipMemcpyDeviceToDevice, st));
+        if (to_c && from_c)
+            NCG_HIP(hipMemcpyAsync(to_c, from_c, sizeof(double) * 8, hipMemcpyDeviceToDevice, st));
+        return NCG_OK;
+    };
+    // A replay that ran out of room for changed bodies leaves some of their
+    // marks set.
+    auto clear_dirty = [&]() -> ncg_status {
+        hipLaunchKernelGGL(k_lf_zero_i32, dim3(cdiv(n / 4 + 2, 256)), dim3(256), 0, st,
+                           reinterpret_cast<int32_t *>(dirty), (int)(n / 4 + 1));
+        NCG_CHECK_LAUNCH();
+        return NCG_OK;
+    };
+    // A collision pass with dt h, checked: on a capacity error the pass is
+    // undone and replayed on the host.
+    auto collide_checked = [&](double h) -> ncg_status {
+        unsigned hs[2];
+        ncg_status r = read_stat(hs);
+        if (r != NCG_OK) return r;
+        const unsigned before = hs[0];
+        if ((r = copy_state(pass_b, pass_dc, nullptr, b, d_did_collide, nullptr)) != NCG_OK) return r;
+        if ((r = collide(h)) != NCG_OK) return r;
+        if ((r = read_stat(hs)) != NCG_OK) return r;
+        if (hs[1] == 0u) return NCG_OK;
+        if ((r = copy_state(b, d_did_collide, nullptr, pass_b, pass_dc, nullptr)) != NCG_OK) return r;
+        if ((r = clear_dirty()) != NCG_OK) return r;
+        if ((r = write_stat(before)) != NCG_OK) return r;
+        std::vector<DBody> hb((size_t)n);
+        std::vector<int32_t> hex(ex ? (size_t)n : 0), hdc(d_did_collide ? (size_t)n : 0);
+        NCG_HIP(hipMemcpyAsync(hb.data(), b, sizeof(DBody) * (size_t)n, hipMemcpyDeviceToHost, st));
+        if (ex)
+            NCG_HIP(hipMemcpyAsync(hex.data(), ex, sizeof(int32_t) * (size_t)n,
+                                   hipMemcpyDeviceToHost, st));
+        if (d_did_collide)
+            NCG_HIP(hipMemcpyAsync(hdc.data(), d_did_collide, sizeof(int32_t) * (size_t)n,
+                                   hipMemcpyDeviceToHost, st));
+        NCG_HIP(hipStreamSynchronize(st));
+        ncg_builtin_collider bc = {};
+        bc.bodies = reinterpret_cast<ncg_body *>(hb.data());
+        bc.exists = ex ? hex.data() : nullptr;
+        bc.did_collide = d_did_collide ? hdc.data() : nullptr;
+        bc.timestep = h;
+        bc.collision_f = collision_f;
+        bc.one_collision_only = one_collision_only;
+        ncg_replay_ops ops;
+        if ((r = ncg_builtin_replay_ops(&bc, &ops)) != NCG_OK) return r;
+        if ((r = ncg_brute_force_pass(ctx, bc.bodies, n, bc.exists, h, &ops, nullptr)) != NCG_OK)
+            return r;
+        host_coll += bc.collisions;
+        NCG_HIP(hipMemcpyAsync(b, hb.data(), sizeof(DBody) * (size_t)n, hipMemcpyHostToDevice, st));
+        if (d_did_collide)
+            NCG_HIP(hipMemcpyAsync(d_did_collide, hdc.data(), sizeof(int32_t) * (size_t)n,
+                                   hipMemcpyHostToDevice, st));
+        NCG_HIP(hipStreamSynchronize(st));
+        return NCG_OK;
+    };
     // One step after the field's preparation (the tiled forms sort first).
     ncg_central cflag = {};
     cflag.present = cp ? 1 : 0;
-    auto step = [&]() -> ncg_status {
+    auto step = [&](bool checked) -> ncg_status {
         ncg_status r;
         if (prec == NCG_F64_EXACT)
             r = ncg_grav_exact_dev(ctx, b, n, ex, d_central, G, soft, acc, cterm, cp ? cm : nullptr, st);
@@ -479,7 +631,7 @@ extern "C" ncg_status ncg_leapfrog_dev(ncg_ctx *ctx, ncg_body *d_bodies, int32_t
         hipLaunchKernelGGL(k_lf_kick, dim3(g), dim3(LT), 0, st, (int)n, ex, dt, b, acc, d_central, cm);
         NCG_CHECK_LAUNCH();
         if (coll) {
-            r = collide(dt);
+            r = checked ? collide_checked(dt) : collide(dt);
             if (r != NCG_OK) return r;
         }
         hipLaunchKernelGGL(k_lf_drift, dim3(g), dim3(LT), 0, st, (int)n, ex, dt, b, d_central);
@@ -492,7 +644,7 @@ extern "C" ncg_status ncg_leapfrog_dev(ncg_ctx *ctx, ncg_body *d_bodies, int32_t
     if (flags & 2) {
         // rk4.c:871-879: brute_force and drift with dt / 2.
         if (coll) {
-            ncg_status r = collide(0.5 * dt);
+            ncg_status r = collide_checked(0.5 * dt);
             if (r != NCG_OK) return r;
         }
         hipLaunchKernelGGL(k_lf_drift, dim3(g), dim3(LT), 0, st, (int)n, ex, 0.5 * dt, b, d_central);
@@ -501,34 +653,64 @@ extern "C" ncg_status ncg_leapfrog_dev(ncg_ctx *ctx, ncg_body *d_bodies, int32_t
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph = nullptr;
     ncg_status res = NCG_OK;
+    auto snapshot = [&]() -> ncg_status {
+        unsigned hs[2];
+        ncg_status r = read_stat(hs);
+        if (r != NCG_OK) return r;
+        snap_coll = hs[0];
+        return copy_state(snap_b, snap_dc, snap_c, b, d_did_collide, d_central);
+    };
+    if (coll) res = snapshot();
+    int32_t win0 = 0;  // first step of the current window
     for (int32_t s = 0; s < nsteps && res == NCG_OK; ++s) {
         if (prec != NCG_F64_EXACT) {
             res = ncg_tiled_order(ctx, b, n, ex, st);  // eager: the sort stays out of graphs
             if (res != NCG_OK) break;
         }
         if (!use_graph || s == 0) {
-            res = step();  // the first step also allocates what a graph uses
-            continue;
+            res = step(false);  // the first step also allocates what a graph uses
+        } else {
+            if (!exec) {
+                NCG_HIP(hipStreamSynchronize(st));
+                ctx->capturing = true;
+                NCG_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+                res = step(false);
+                hipError_t e = hipStreamEndCapture(st, &graph);
+                ctx->capturing = false;
+                if (res != NCG_OK) break;
+                if (e != hipSuccess) {
+                    res = ncg_fail(ctx, NCG_ERR_DEVICE, "stream capture failed: %s", hipGetErrorString(e));
+                    break;
+                }
+                if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+                    res = ncg_fail(ctx, NCG_ERR_DEVICE, "graph instantiation failed");
+                    break;
+                }
+            }
+            if (hipGraphLaunch(exec, st) != hipSuccess) {
+                res = ncg_fail(ctx, NCG_ERR_DEVICE, "graph launch failed");
+                break;
+            }
         }
-        if (!exec) {
-            NCG_HIP(hipStreamSynchronize(st));
-            ctx->capturing = true;
-            NCG_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-            res = step();
-            hipError_t e = hipStreamEndCapture(st, &graph);
-            ctx->capturing = false;
+        if (res != NCG_OK || !coll) continue;
+        if ((s + 1) % LF_CHECK != 0 && s + 1 != nsteps) continue;
+        unsigned hs[2];
+        if ((res = read_stat(hs)) != NCG_OK) break;
+        if (hs[1] != 0u) {
+            // The window [win0, s] again, from its snapshot, checked.
+            if ((res = copy_state(b, d_did_collide, d_central, snap_b, snap_dc, snap_c)) != NCG_OK)
+                break;
+            if ((res = clear_dirty()) != NCG_OK) break;
+            if ((res = write_stat(snap_coll)) != NCG_OK) break;
+            for (int32_t q = win0; q <= s && res == NCG_OK; ++q) {
+                if (prec != NCG_F64_EXACT && (res = ncg_tiled_order(ctx, b, n, ex, st)) != NCG_OK)
+                    break;
+                res = step(true);
+            }
             if (res != NCG_OK) break;
-            if (e != hipSuccess) {
-                res = ncg_fail(ctx, NCG_ERR_DEVICE, "stream capture failed: %s", hipGetErrorString(e));
-                break;
-            }
-            if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
-                res = ncg_fail(ctx, NCG_ERR_DEVICE, "graph instantiation failed");
-                break;
-            }
         }
-        if (hipGraphLaunch(exec, st) != hipSuccess)
-            res = ncg_fail(ctx, NCG_ERR_DEVICE, "graph launch failed");
+        if ((res = snapshot()) != NCG_OK) break;
+        win0 = s + 1;
     }
     if (res == NCG_OK && (flags & 4)) {
         // rk4.c:938-942: the backward half drift.
@@ -541,7 +723,7 @@ extern "C" ncg_status ncg_leapfrog_dev(ncg_ctx *ctx, ncg_body *d_bodies, int32_t
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     if (res != NCG_OK) return res;
-    if (n_collisions) *n_collisions = hs[0];
+    if (n_collisions) *n_collisions = hs[0] + host_coll;
     if (hs[1])
         return ncg_fail(ctx, NCG_ERR_CAPACITY,
                         "collision replay capacity exceeded (flags %u: 1 = more than %u candidates, "

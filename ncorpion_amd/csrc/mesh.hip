@@ -178,6 +178,85 @@ __device__ inline bool approach_hit(double4 pa, double4 va, double4 pb, double4 
     return t <= dt + fabs(dt) * 9.094947017729282e-13;
 }
 
+// Device passes (ncg_mesh_pass_dev) need only the total of the counts, not
+// each body's.  With at least 3 cells per axis each unordered neighbour pair
+// is met exactly once from the side of its higher cell (the 13 cells below a
+// body's own) or, within a cell, from its higher id; so the total is the sum
+// over bodies of the members of the 13 lower cells and of the own cell's
+// lower ids, which needs no search in the cells above.  The 12 cells of the
+// 4 lower stencil rows are 4 runs of the sorted order (a row's three cells
+// have consecutive keys), and cell 12 (key - 1) with the own cell's lower ids
+// (positions before the body's own, l = 13) is a fifth.  Mode 0 tests every
+// pair of the runs (the candidates of k_mcount's per-body form).
+template <int MODE>
+__global__ __launch_bounds__(MT) void k_mcount_total(const unsigned *__restrict__ skey,
+                                                     const int *__restrict__ sid,
+                                                     const double4 *__restrict__ pr,
+                                                     const double4 *__restrict__ vm,
+                                                     const unsigned *__restrict__ cs,
+                                                     const unsigned *__restrict__ ce,
+                                                     const int *__restrict__ dins, int d,
+                                                     double dt, int4 *__restrict__ cand,
+                                                     unsigned long long cap,
+                                                     unsigned long long *__restrict__ ccnt,
+                                                     unsigned long long *__restrict__ total)
+{
+    __shared__ unsigned long long sh[MT];
+    const int s = blockIdx.x * MT + threadIdx.x;
+    const int ins = *dins;
+    unsigned long long c = 0;
+    if (s < ins) {
+        const long key = skey[s];
+        int k = 0;
+        double4 pk = make_double4(0, 0, 0, 0), vk = pk;
+        if (MODE == 0) {
+            k = sid[s];
+            pk = pr[s];
+            vk = vm[s];
+        }
+        for (int r = 0; r < 5; ++r) {
+            // Cells c0, c1, c2 of the run (c2 unused for the fifth run).
+            const long k1 = r < 4 ? key + (long)(r % 3 - 1) * d + (long)(r / 3 - 1) * d * d : key;
+            const unsigned s0 = cs[k1 - 1], e0 = ce[k1 - 1], s1 = cs[k1], e1 = ce[k1];
+            unsigned a, e, s2 = 0, e2 = 0;
+            if (r < 4) {
+                s2 = cs[k1 + 1];
+                e2 = ce[k1 + 1];
+                a = e0 > s0 ? s0 : (e1 > s1 ? s1 : s2);
+                e = e2 > s2 ? e2 : (e1 > s1 ? e1 : e0);
+            } else {
+                a = e0 > s0 ? s0 : s1;  // the own cell is not empty
+                e = (unsigned)s;
+            }
+            if (a >= e) continue;
+            c += e - a;
+            if (MODE != 0) continue;
+            int np = sid[a];
+            double4 nr = pr[a], nv = vm[a];
+            for (unsigned q = a; q < e; ++q) {
+                const int p = np;
+                const double4 rq = nr, vq = nv;
+                if (q + 1 < e) {
+                    np = sid[q + 1];
+                    nr = pr[q + 1];
+                    nv = vm[q + 1];
+                }
+                const bool lower = p < k;
+                if (lower ? approach_hit(rq, vq, pk, vk, dt) : approach_hit(pk, vk, rq, vq, dt)) {
+                    // Stencil index of q's cell.
+                    const int ci = (e0 > s0 && q < e0) ? 0 : ((r == 4 || (e1 > s1 && q < e1)) ? 1 : 2);
+                    const int l = r < 4 ? 3 * r + ci : 12 + ci;
+                    const unsigned long long slot = atomicAdd(ccnt, 1ull);
+                    if (slot < cap)
+                        cand[slot] = lower ? make_int4(k, l, p, 0) : make_int4(p, 26 - l, k, 0);
+                }
+            }
+        }
+    }
+    const unsigned long long t = bsum(c, sh);
+    if (threadIdx.x == 0 && t) atomicAdd(total, t);
+}
+
 // One thread per inserted body (sorted position s): neighbour count (p < k in
 // the 27 cells) and, for mesh(), the collision candidates (k, l, p): every
 // neighbour pair is tested once.  With at least 3 cells per axis the 27
@@ -2260,14 +2339,14 @@ static ncg_status build_ord_lists(ncg_ctx *ctx, int ins, long d, const unsigned 
                                   hipStream_t st);
 static ncg_status ensure_aux(ncg_ctx *ctx);
 
-extern "C" ncg_status ncg_mesh_pass_dev(ncg_ctx *ctx, ncg_body *d_bodies, int32_t n,
-                                        const int32_t *d_exists, int32_t *d_did_collide,
-                                        double timestep, double gam, double collision_cube,
-                                        int32_t cells, int32_t mode, double collision_f,
-                                        int32_t one_collision_only, ncg_mesh_stats *stats,
-                                        void *stream)
+ncg_status ncg_mesh_pass_dev_flags(ncg_ctx *ctx, ncg_body *d_bodies, int32_t n,
+                                   const int32_t *d_exists, int32_t *d_did_collide,
+                                   double timestep, double gam, double collision_cube,
+                                   int32_t cells, int32_t mode, double collision_f,
+                                   int32_t one_collision_only, ncg_mesh_stats *stats,
+                                   unsigned *flags, hipStream_t st)
 {
-    if (!ctx) return NCG_ERR_INVALID;
+    *flags = 0u;
     NCG_REQUIRE(n >= 0 && (n == 0 || d_bodies) && (mode == 0 || mode == 1) && stats,
                 "invalid arguments");
     NCG_REQUIRE(cells >= 3 && gam > 0.0 && collision_cube > 0.0, "invalid mesh geometry");
@@ -2280,8 +2359,6 @@ extern "C" ncg_status ncg_mesh_pass_dev(ncg_ctx *ctx, ncg_body *d_bodies, int32_
     NCG_REQUIRE(nkeys < 0xffffffffLL, "collision_cube_cells %d too large for 32-bit cell keys",
                 (int)cells);
     memset(stats, 0, sizeof *stats);
-    NCG_HIP(hipSetDevice(ctx->device));
-    hipStream_t st = pick_stream(ctx, stream);
     MeshState &M = ctx->mesh;
     reset_pass(M, n);
     if (n == 0) {
@@ -2295,7 +2372,7 @@ extern "C" ncg_status ncg_mesh_pass_dev(ncg_ctx *ctx, ncg_body *d_bodies, int32_
     MeshBin B;
     ncg_status r = mesh_bin(ctx, db, n, d_exists, gam, collision_cube, d, nkeys, mode, B, st);
     if (r != NCG_OK) return r;
-    const unsigned cap = 1u << 16;
+    const unsigned cap = ncg_replay_cap(ctx, 1u << 16);
     int4 *dc = mode == 0 ? dbuf<int4>(ctx, "m_cand", cap) : nullptr;
     unsigned *stat = dbuf<unsigned>(ctx, "m_rstat", 2);
     unsigned *hs = pbuf<unsigned>(ctx, "m_hstat", 16);
@@ -2303,13 +2380,15 @@ extern "C" ncg_status ncg_mesh_pass_dev(ncg_ctx *ctx, ncg_body *d_bodies, int32_
     const unsigned g = (unsigned)cdiv(n, MT);
     {
         ProfScope ps(ctx, st, "mesh_count");
+        // The device pass keeps no per-body counts (ncg_mesh_pairs is not
+        // available after it): the total alone.
         if (mode == 0)
-            hipLaunchKernelGGL(k_mcount<0>, dim3(g), dim3(MT), 0, st, B.skey, B.sid, B.pr, B.vm, B.cs,
-                               B.ce, 0, B.cnt, (int)d, timestep, B.cbi, dc, (unsigned long long)cap,
-                               B.tot + 1, B.tot);
+            hipLaunchKernelGGL(k_mcount_total<0>, dim3(g), dim3(MT), 0, st, B.skey, B.sid, B.pr,
+                               B.vm, B.cs, B.ce, B.cnt, (int)d, timestep, dc,
+                               (unsigned long long)cap, B.tot + 1, B.tot);
         else
-            hipLaunchKernelGGL(k_mcount<1>, dim3(g), dim3(MT), 0, st, B.skey, B.sid, B.pr, B.vm, B.cs,
-                               B.ce, 0, B.cnt, (int)d, timestep, B.cbi, (int4 *)nullptr, 0ull,
+            hipLaunchKernelGGL(k_mcount_total<1>, dim3(g), dim3(MT), 0, st, B.skey, B.sid, B.pr,
+                               B.vm, B.cs, B.ce, B.cnt, (int)d, timestep, (int4 *)nullptr, 0ull,
                                B.tot + 1, B.tot);
         NCG_CHECK_LAUNCH();
     }
@@ -2372,18 +2451,38 @@ extern "C" ncg_status ncg_mesh_pass_dev(ncg_ctx *ctx, ncg_body *d_bodies, int32_
     stats->how_many_big = mode == 0 ? hc4[2] : 0;
     stats->candidates = (int64_t)ht[1];
     stats->collisions = hs[8];
-    if (hs[9] & 4u)
+    *flags = hs[9];
+    if (hs[9]) return NCG_OK;
+    M.valid = true;
+    M.pairs_known = mode == 1;
+    return NCG_OK;
+}
+
+extern "C" ncg_status ncg_mesh_pass_dev(ncg_ctx *ctx, ncg_body *d_bodies, int32_t n,
+                                        const int32_t *d_exists, int32_t *d_did_collide,
+                                        double timestep, double gam, double collision_cube,
+                                        int32_t cells, int32_t mode, double collision_f,
+                                        int32_t one_collision_only, ncg_mesh_stats *stats,
+                                        void *stream)
+{
+    if (!ctx) return NCG_ERR_INVALID;
+    NCG_HIP(hipSetDevice(ctx->device));
+    unsigned flags = 0u;
+    ncg_status r = ncg_mesh_pass_dev_flags(ctx, d_bodies, n, d_exists, d_did_collide, timestep, gam,
+                                           collision_cube, cells, mode, collision_f,
+                                           one_collision_only, stats, &flags,
+                                           pick_stream(ctx, stream));
+    if (r != NCG_OK) return r;
+    if (flags & 4u)
         return ncg_fail(ctx, NCG_ERR_INVALID,
-                        "ncg_mesh_pass_dev: %d bodies are larger than the mesh size (gam = %g); "
-                        "use ncg_mesh_pass", hc4[2], gam);
-    if (hs[9])
+                        "ncg_mesh_pass_dev: %lld bodies are larger than the mesh size (gam = %g); "
+                        "use ncg_mesh_pass", (long long)stats->how_many_big, gam);
+    if (flags)
         return ncg_fail(ctx, NCG_ERR_CAPACITY,
                         "ncg_mesh_pass_dev: replay capacity exceeded (flags %u: 1 = more than %u "
                         "candidates, 2 = more than %d pending re-tests); the state was not changed "
                         "only if flag 1 is set alone",
-                        hs[9], cap, MR_PCAP);
-    M.valid = true;
-    M.pairs_known = mode == 1;
+                        flags, ncg_replay_cap(ctx, 1u << 16), MR_PCAP);
     return NCG_OK;
 }
 
@@ -2756,9 +2855,12 @@ extern "C" ncg_status ncg_mesh_pairs(ncg_ctx *ctx, ncg_pair *out, int64_t capaci
         for (const auto &pe : at) out[pe.first] = pe.second->reg;
         return NCG_OK;
     }
-    std::vector<int> rmv((size_t)n, INT32_MAX);
+    // Removal visit of every id a registered pair can hold: the pass's slots
+    // and the ones appended during it (big visits pair with those).
+    const int32_t nall = std::max(n, M.n_end);
+    std::vector<int> rmv((size_t)nall, INT32_MAX);
     for (const auto &rv : M.removed)
-        if (rv.first < n) rmv[rv.first] = rv.second;
+        if (rv.first >= 0 && rv.first < nall) rmv[rv.first] = rv.second;
     long long o = 0;
     for (int32_t k = 0; k < n; ++k)
         for (long long q = off[k]; q < off[k] + cnt[k]; ++q) {
@@ -2772,7 +2874,8 @@ extern "C" ncg_status ncg_mesh_pairs(ncg_ctx *ctx, ncg_pair *out, int64_t capaci
                             out[o++] = {M.frag_ids[e.first + a], M.frag_ids[e.first + b]};
                 continue;
             }
-            if (rmv[gl[q].snd] < k) continue;  // removed before visit k gathered
+            const int ps = gl[q].snd;
+            if (ps >= 0 && ps < nall && rmv[ps] < k) continue;  // removed before visit k gathered
             out[o++] = gl[q];
         }
     if (o != fin)

@@ -144,6 +144,10 @@ ncg_status ncg_create(ncg_ctx **out, int device)
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->num_cus = prop.multiProcessorCount;
     load_tuning(ctx);
+    if (const char *rc = getenv("NCG_REPLAY_CAP")) {
+        const long v = atol(rc);
+        if (v > 0 && v < (1L << 30)) ctx->replay_cap = (unsigned)v;
+    }
     *out = ctx;
     return NCG_OK;
 }

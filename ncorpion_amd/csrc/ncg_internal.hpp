@@ -111,9 +111,31 @@ struct ncg_ctx {
     Rk4Graph *rk4 = nullptr;
     uint64_t buf_gen = 0;  // bumped whenever a device buffer is (re)allocated
     bool capturing = false;  // a stream capture is in progress
+    // Candidate capacity of the device collision replays (GPU-resident
+    // loops); NCG_REPLAY_CAP in the environment lowers it (tests of the
+    // overflow fallback).  0 = each loop's default.
+    unsigned replay_cap = 0;
 };
 
 void ncg_rk4_release(ncg_ctx *ctx);
+
+// Candidate capacity of a device replay whose default is `dflt`.
+static inline unsigned ncg_replay_cap(const ncg_ctx *ctx, unsigned dflt)
+{
+    return ctx->replay_cap > 0 && ctx->replay_cap < dflt ? ctx->replay_cap : dflt;
+}
+
+// ncg_mesh_pass_dev without the error mapping: *flags receives the device
+// replay's status bits (1 more candidates than its capacity, 2 more pending
+// re-tests than it holds, 4 a body larger than the mesh size); with a
+// nonzero *flags the pass's results are not valid and the bodies may have
+// been changed.
+ncg_status ncg_mesh_pass_dev_flags(ncg_ctx *ctx, ncg_body *d_bodies, int32_t n,
+                                   const int32_t *d_exists, int32_t *d_did_collide,
+                                   double timestep, double gam, double collision_cube,
+                                   int32_t cells, int32_t mode, double collision_f,
+                                   int32_t one_collision_only, ncg_mesh_stats *stats,
+                                   unsigned *flags, hipStream_t st);
 
 // Exact brute-force field with the central body in a device record (8
 // doubles {x, y, z, vx, vy, vz, M, present}); CM_acc summed in body order.

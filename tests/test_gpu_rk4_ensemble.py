@@ -236,3 +236,49 @@ def test_ensemble_collision_steps_graph_equals_eager(ctx):
     assert res[0][0] == res[1][0] > 50
     assert np.array_equal(res[0][1].view(np.uint64), res[1][1].view(np.uint64))
     assert np.array_equal(res[0][2].view(np.uint64), res[1][2].view(np.uint64))
+
+
+def test_ensemble_replay_overflow_replayed_on_host(ctx):
+    """With the device replay's candidate capacity forced down to 3
+    (NCG_REPLAY_CAP, read at context creation), the passes that overflow it
+    are undone and replayed on the host: 20 steps (more than one 16-step
+    check window) equal the unbounded device run bit for bit."""
+    import os
+    from ncorpion_amd import Context
+    torch = torch_dev()
+    rng = np.random.default_rng(13)
+    sizes = rng.integers(100, 500, 7)
+    systems = []
+    for s, nn in enumerate(sizes):
+        b = np.zeros((nn, 8))
+        r2 = np.random.default_rng(400 + s)
+        b[:, 0:3] = r2.uniform(-0.04, 0.04, (nn, 3)) + np.array([1.0, 0, 0])
+        b[:, 3:6] = r2.normal(0, 0.3, (nn, 3)) + np.array([0, 1.0, 0])
+        b[:, 6] = r2.uniform(1e-12, 1e-11, nn)
+        b[:, 7] = r2.uniform(1e-3, 2e-3, nn)
+        systems.append(b)
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)
+    allb = np.concatenate(systems)
+    cen = np.zeros((len(sizes), 8))
+    cen[:, 6] = 1.0
+    cen[:, 7] = 1.0
+    os.environ["NCG_REPLAY_CAP"] = "3"
+    try:
+        small = Context(0)
+    finally:
+        del os.environ["NCG_REPLAY_CAP"]
+    res = []
+    try:
+        for c in (ctx, small):
+            bt = torch.from_numpy(allb).cuda()
+            ct = torch.from_numpy(cen).cuda()
+            torch.cuda.synchronize()
+            nc = c.ensemble_rk4_dev(bt, off, ct, 20, 0.002, 1.0, 0.0, "fast64", graph=True,
+                                    collisions=True, collision_f=1.5)
+            c.synchronize()
+            res.append((nc, bt.cpu().numpy(), ct.cpu().numpy()))
+    finally:
+        small.close()
+    assert res[0][0] == res[1][0] > 50
+    assert np.array_equal(res[0][1].view(np.uint64), res[1][1].view(np.uint64))
+    assert np.array_equal(res[0][2].view(np.uint64), res[1][2].view(np.uint64))

@@ -278,16 +278,36 @@ def test_leapfrog_tiled_forms(ctx, prec, tol):
     assert d < tol, d
 
 
+def big_body_disk():
+    """16,384-body disk (m = 1e-8, R = 2e-3) with 24 bodies of radius
+    0.03-0.06 (same density), half of them on fast crossing orbits."""
+    n = 16384
+    b = ic.disk(n, seed=6, mass=1e-8, radius=2e-3)
+    rng = np.random.default_rng(6)
+    big = rng.choice(np.arange(n), 24, replace=False)
+    b[big, 7] = rng.uniform(0.03, 0.06, 24)
+    b[big, 6] = 1e-8 * (b[big, 7] / 2e-3) ** 3
+    b[big[:12], 3:6] += rng.normal(0.0, 0.05, (12, 3))
+    return b
+
+
 @pytest.mark.gpu
-@pytest.mark.parametrize("prec", ["exact"])
-def test_mesh_loop_reproduces_reference_run(ctx, tmp_path, prec):
-    """NcorpiON's mesh integrator (16,384 bodies, radius 1e-3, inelastic
-    bounces, 10 steps) from the reference's own starting state: after every
-    device step, the state the reference's next mesh() sees (after its
-    three_largest swap) has the same hash, and the collision counts agree."""
+@pytest.mark.parametrize("prec,cfg", [("exact", "run_mesh"), ("exact", "run_mesh_big")])
+def test_mesh_loop_reproduces_reference_run(ctx, tmp_path, prec, cfg):
+    """NcorpiON's mesh integrator (16,384 bodies, inelastic bounces, 10
+    steps) from the reference's own starting state: after every device step,
+    the state the reference's next mesh() sees (after its three_largest swap)
+    has the same hash, and the collision counts agree.  run_mesh: radius
+    1e-3; run_mesh_big: bodies larger than the mesh size, whose passes the
+    loop replays on the host."""
     torch = tt()
-    b0 = ic.disk(16384, seed=2, mass=1e-10, radius=1e-3)
-    exe = os.path.join(REF, "run_mesh", "ncorpion_trace")
+    if cfg == "run_mesh_big":
+        b0 = big_body_disk()
+        nbig_min = 1
+    else:
+        b0 = ic.disk(16384, seed=2, mass=1e-10, radius=1e-3)
+        nbig_min = 0
+    exe = os.path.join(REF, cfg, "ncorpion_trace")
     if not os.path.exists(exe):
         pytest.skip("reference builds missing (oracle/build_ref.py)")
     ic.write_init_txt(str(tmp_path / "init.txt"), b0)
@@ -308,7 +328,8 @@ def test_mesh_loop_reproduces_reference_run(ctx, tmp_path, prec):
     bodies, ex, dc, cm, ts, gam, cube, hm = read_dump_mesh(str(tmp_path / "dump.bin"))
     assert len(mesh_in) == 10 and sum(coll) > 10
     dt = 2.0 * ts  # dumped inside the half kick (timestep halved)
-    params = dict(gam=gam, gam_min=2.56 / 256, how_many_neighbours=100.0,
+    params = dict(gam=gam, gam_min=2.56 / 256,
+                  how_many_neighbours=30.0 if cfg == "run_mesh_big" else 100.0,
                   disruption_threshold=0.05, high_dumping_threshold=100.0, cells=256)
     bt = torch.from_numpy(bodies).cuda()
     ct = torch.from_numpy(np.concatenate([cm[0:7], [1.0]])).cuda()
@@ -316,6 +337,7 @@ def test_mesh_loop_reproduces_reference_run(ctx, tmp_path, prec):
     dct = torch.from_numpy(dc).cuda()
     ctx.mesh_leapfrog_dev(bt, ct, 0, dt, params, 1.0, 0.0, prec, 1.5, 0, exists_t=et,
                           did_collide_t=dct, half_kick_start=True)
+    host_passes = 0
     for s, h in enumerate(mesh_in):
         ctx.synchronize()
         st = bt.cpu().numpy().copy()
@@ -324,4 +346,42 @@ def test_mesh_loop_reproduces_reference_run(ctx, tmp_path, prec):
         assert orc.state_hash(st, e) == h, "step %d" % s
         nc = ctx.mesh_leapfrog_dev(bt, ct, 1, dt, params, 1.0, 0.0, prec, 1.5, 0, exists_t=et,
                                    did_collide_t=dct)
+        host_passes += params["host_passes"]
         assert nc == coll[s], "step %d" % s
+    if nbig_min:
+        assert host_passes > 0
+    else:
+        assert host_passes == 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("graph", [False, True])
+def test_leapfrog_replay_overflow_replayed_on_host(tmp_path, graph):
+    """With the device replay's candidate capacity forced down to 2
+    (NCG_REPLAY_CAP, read at context creation), every collision pass that
+    overflows it is undone and replayed on the host: 70 steps (more than one
+    64-step check window) of a dense cluster equal the oracle's loop bit for
+    bit."""
+    torch = tt()
+    from ncorpion_amd import Context
+    os.environ["NCG_REPLAY_CAP"] = "2"
+    try:
+        small = Context(0)
+    finally:
+        del os.environ["NCG_REPLAY_CAP"]
+    try:
+        n, dt, steps = 600, 0.004, 70
+        b = cluster(n, 77, radius=3e-3)
+        c8 = np.array([0.001, 0.0, 0.0, 0.0, 1e-3, 0.0, 1.0, 1.0])
+        ref_b, ref_c, ref_n = oracle_leapfrog(b, None, c8, 1.0, 0.01, dt, steps, 1.5, 0, True)
+        bt = torch.from_numpy(b).cuda()
+        ct = torch.from_numpy(c8).cuda()
+        torch.cuda.synchronize()
+        nc = small.leapfrog_dev(bt, ct, steps, dt, 1.0, 0.01, "exact", 1.5, 0, graph=graph,
+                                half_drift_start=True)
+        small.synchronize()
+        assert nc == ref_n and ref_n > 20
+        assert np.array_equal(bt.cpu().numpy().view(np.uint64), ref_b.view(np.uint64))
+        assert np.array_equal(ct.cpu().numpy().view(np.uint64), ref_c.view(np.uint64))
+    finally:
+        small.close()
