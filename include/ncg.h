@@ -166,6 +166,27 @@ ncg_status ncg_gravity_allpairs_dev(ncg_ctx *ctx, const ncg_body *d_bodies, int3
                                     double grav_const, double softening, ncg_precision prec,
                                     double *d_acc, double *d_cm_acc, void *stream);
 
+/* No counterpart in the reference.  ncg_gravity_allpairs_dev for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc): it calls no hipMalloc, hipFree or
+ * allocating library routine.  Its scratch (sort keys, tiles, j-split
+ * partials, CM_acc terms) is one static device array of
+ * NCG_STATIC_SCRATCH_BYTES per GPU, held from the first context on that GPU
+ * until the process ends; n above NCG_STATIC_MAX_N is refused
+ * (NCG_ERR_INVALID).  The caller passes inputs and outputs as for
+ * ncg_gravity_allpairs_dev, with the same results: NCG_F64_EXACT bit for bit;
+ * the fast forms alike up to the grouping of the j-split sums (the split
+ * count is lowered when the partials would not fit).  Every call on one GPU
+ * shares that GPU's array: calls that overlap on one GPU (from several
+ * streams or threads) race.  Nothing is carried from one call to the next. */
+#define NCG_STATIC_SCRATCH_BYTES (32 << 20)
+#define NCG_STATIC_MAX_N 32768
+ncg_status ncg_gravity_allpairs_static_dev(ncg_ctx *ctx, const ncg_body *d_bodies, int32_t n,
+                                           const int32_t *d_exists, const ncg_central *central,
+                                           double grav_const, double softening,
+                                           ncg_precision prec, double *d_acc, double *d_cm_acc,
+                                           void *stream);
+
 /* ---------------------------------------------------------------------------
  * Brute-force collision search: replaces brute_force (collision.c:425-456).
  * The GPU tests every pair i<j of existing bodies at the given state with the

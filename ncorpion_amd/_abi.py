@@ -100,6 +100,8 @@ _SIGNATURES = {
     "ncg_stream": (P, [P]),
     "ncg_gravity_allpairs": (S, [P, P, I32, P, ctypes.POINTER(Central), D, D, S, P, P]),
     "ncg_gravity_allpairs_dev": (S, [P, P, I32, P, ctypes.POINTER(Central), D, D, S, P, P, P]),
+    "ncg_gravity_allpairs_static_dev": (S, [P, P, I32, P, ctypes.POINTER(Central), D, D, S, P, P,
+                                            P]),
     "ncg_brute_force_pass": (S, [P, P, I32, P, D, ctypes.POINTER(ReplayOps), ctypes.POINTER(I64)]),
     "ncg_collide_candidates_allpairs": (S, [P, P, I32, P, D, P, I64, ctypes.POINTER(I64)]),
     "ncg_mesh_pass": (S, [P, P, I32, P, D, D, D, I32, I32, ctypes.POINTER(ReplayOps),
@@ -242,6 +244,17 @@ class Context:
         n = int(bodies_t.shape[0])
         c = make_central(central)
         self._check(self.lib.ncg_gravity_allpairs_dev(
+            self.h, bodies_t.data_ptr(), n, None if exists_t is None else exists_t.data_ptr(),
+            ctypes.byref(c), float(G), float(softening), PRECISIONS[precision], acc_t.data_ptr(),
+            None if cm_t is None else cm_t.data_ptr(), stream))
+
+    def gravity_allpairs_static_dev(self, bodies_t, acc_t, exists_t=None, central=None, G=1.0,
+                                    softening=0.0, precision="mixed", cm_t=None, stream=None):
+        """gravity_allpairs_dev through the allocation-free entry point (its
+        scratch is the library's static per-GPU array; n <= 32768)."""
+        n = int(bodies_t.shape[0])
+        c = make_central(central)
+        self._check(self.lib.ncg_gravity_allpairs_static_dev(
             self.h, bodies_t.data_ptr(), n, None if exists_t is None else exists_t.data_ptr(),
             ctypes.byref(c), float(G), float(softening), PRECISIONS[precision], acc_t.data_ptr(),
             None if cm_t is None else cm_t.data_ptr(), stream))

@@ -41,6 +41,8 @@
 //                    central term and scatters back to body order.
 #include <hipcub/hipcub.hpp>
 
+#include <mutex>
+
 #include "ncg_internal.hpp"
 
 namespace {
@@ -743,29 +745,92 @@ static TiledShape tiled_shape(ncg_ctx *ctx, int32_t n, int prec)
     return t;
 }
 
+// Device buffers of the tiled pipeline.  The order part (bounding box, keys,
+// sort) and the evaluation part are used by separate calls: the RK4 and
+// leapfrog drivers sort eagerly and capture only the evaluation in graphs.
+struct TiledWork {
+    Scal *sc = nullptr;
+    double *bpart = nullptr;
+    unsigned *key = nullptr, *key2 = nullptr;
+    int *idx = nullptr, *perm = nullptr;
+    void *sort_tmp = nullptr;
+    size_t sort_bytes = 0;
+    double4 *jd = nullptr;
+    double *jsig = nullptr;
+    float4 *ju = nullptr;
+    TileHdr *th = nullptr, *ih = nullptr;
+    double *part = nullptr;
+    double *cterm = nullptr, *p3 = nullptr;
+};
+
+static ncg_status sort_bytes(ncg_ctx *ctx, int32_t n, hipStream_t st, size_t *bytes)
+{
+    *bytes = 0;
+    NCG_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, *bytes, (unsigned *)nullptr,
+                                               (unsigned *)nullptr, (int *)nullptr, (int *)nullptr,
+                                               n, 0, 32, st));
+    return NCG_OK;
+}
+
+// Buffers of the context (grown on demand): order part, evaluation part.
+static ncg_status work_order_ctx(ncg_ctx *ctx, const TiledShape &t, int32_t n, hipStream_t st,
+                                 TiledWork &w)
+{
+    w.sc = dbuf<Scal>(ctx, "t_scal", 1);
+    w.bpart = dbuf<double>(ctx, "t_bpart", 8 * (size_t)t.nbb);
+    w.key = dbuf<unsigned>(ctx, "t_key", n);
+    w.key2 = dbuf<unsigned>(ctx, "t_key2", n);
+    w.idx = dbuf<int>(ctx, "t_idx", n);
+    w.perm = dbuf<int>(ctx, "t_perm", n);
+    if (!w.sc || !w.bpart || !w.key || !w.key2 || !w.idx || !w.perm) return NCG_ERR_NOMEM;
+    ncg_status r = sort_bytes(ctx, n, st, &w.sort_bytes);
+    if (r != NCG_OK) return r;
+    w.sort_tmp = ncg_dbuf(ctx, "t_sorttmp", w.sort_bytes);
+    return w.sort_tmp ? NCG_OK : NCG_ERR_NOMEM;
+}
+
+static ncg_status work_eval_ctx(ncg_ctx *ctx, const TiledShape &t, int32_t n, bool want_cm,
+                                double *d_cterm, TiledWork &w)
+{
+    w.sc = dbuf<Scal>(ctx, "t_scal", 1);
+    w.perm = dbuf<int>(ctx, "t_perm", n);
+    w.jd = dbuf<double4>(ctx, "t_jd", (size_t)t.ntiles * TJ + t.iblock);
+    w.jsig = dbuf<double>(ctx, "t_jsig", (size_t)t.ntiles * TJ + t.iblock);
+    w.ju = dbuf<float4>(ctx, "t_ju", (size_t)t.ntiles * TJ);
+    w.th = dbuf<TileHdr>(ctx, "t_th", t.ntiles);
+    w.ih = dbuf<TileHdr>(ctx, "t_ih", t.ngroups);
+    w.part = dbuf<double>(ctx, "t_part", (size_t)t.nsplit * 3 * n);
+    w.cterm = d_cterm ? d_cterm : (want_cm ? dbuf<double>(ctx, "cterm", 3 * (size_t)n) : nullptr);
+    w.p3 = want_cm ? dbuf<double>(ctx, "sum3", 3 * (size_t)t.nbb) : nullptr;
+    if (!w.sc || !w.perm || !w.jd || !w.jsig || !w.ju || !w.th || !w.ih || !w.part ||
+        (want_cm && (!w.cterm || !w.p3)))
+        return NCG_ERR_NOMEM;
+    return NCG_OK;
+}
+
+static ncg_status tiled_order_w(ncg_ctx *ctx, const TiledShape &t, const TiledWork &w,
+                                const DBody *d_b, int32_t n, const int32_t *d_ex, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_bbox_partial, dim3(t.nbb), dim3(NT), 0, st, d_b, n, d_ex, w.bpart);
+    hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(NT), 0, st, w.bpart, t.nbb, w.sc);
+    hipLaunchKernelGGL(k_morton, dim3(cdiv(n, NT)), dim3(NT), 0, st, d_b, n, d_ex, w.sc, w.key,
+                       w.idx);
+    NCG_CHECK_LAUNCH();
+    size_t bytes = w.sort_bytes;
+    NCG_HIP(hipcub::DeviceRadixSort::SortPairs(w.sort_tmp, bytes, w.key, w.key2, w.idx, w.perm, n,
+                                               0, 32, st));
+    return NCG_OK;
+}
+
 ncg_status ncg_tiled_order(ncg_ctx *ctx, const DBody *d_b, int32_t n, const int32_t *d_ex,
                            hipStream_t st)
 {
     if (n <= SMALL_N) return NCG_OK;  // the direct form needs no order
     const TiledShape t = tiled_shape(ctx, n, NCG_MIXED);
-    Scal *sc = dbuf<Scal>(ctx, "t_scal", 1);
-    double *bpart = dbuf<double>(ctx, "t_bpart", 8 * (size_t)t.nbb);
-    unsigned *key = dbuf<unsigned>(ctx, "t_key", n);
-    unsigned *key2 = dbuf<unsigned>(ctx, "t_key2", n);
-    int *idx = dbuf<int>(ctx, "t_idx", n);
-    int *perm = dbuf<int>(ctx, "t_perm", n);
-    if (!sc || !bpart || !key || !key2 || !idx || !perm) return NCG_ERR_NOMEM;
-    size_t tmp_bytes = 0;
-    NCG_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, key, key2, idx, perm, n, 0, 32,
-                                               st));
-    void *tmp = ncg_dbuf(ctx, "t_sorttmp", tmp_bytes);
-    if (!tmp) return NCG_ERR_NOMEM;
-    hipLaunchKernelGGL(k_bbox_partial, dim3(t.nbb), dim3(NT), 0, st, d_b, n, d_ex, bpart);
-    hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(NT), 0, st, bpart, t.nbb, sc);
-    hipLaunchKernelGGL(k_morton, dim3(cdiv(n, NT)), dim3(NT), 0, st, d_b, n, d_ex, sc, key, idx);
-    NCG_CHECK_LAUNCH();
-    NCG_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, key, key2, idx, perm, n, 0, 32, st));
-    return NCG_OK;
+    TiledWork w;
+    ncg_status r = work_order_ctx(ctx, t, n, st, w);
+    if (r != NCG_OK) return r;
+    return tiled_order_w(ctx, t, w, d_b, n, d_ex, st);
 }
 
 int ncg_small_partials(int32_t n) { return std::max(1, std::min(1024, (int)cdiv(n, NT))); }
@@ -790,42 +855,28 @@ ncg_status ncg_small_launch(ncg_ctx *ctx, const DBody *d_b, int32_t n, const int
     return NCG_OK;
 }
 
-ncg_status ncg_tiled_eval(ncg_ctx *ctx, const DBody *d_b, int32_t n, const int32_t *d_ex,
-                          const ncg_central *c, const double *d_central, double G, double soft,
-                          ncg_precision prec, double *d_acc, double *d_cterm, double *d_cm_acc,
-                          hipStream_t st)
+static ncg_status tiled_eval_w(ncg_ctx *ctx, const TiledShape &t, const TiledWork &w,
+                               const DBody *d_b, int32_t n, const int32_t *d_ex,
+                               const ncg_central *c, const double *d_central, double G,
+                               double soft, ncg_precision prec, double *d_acc, double *d_cm_acc,
+                               hipStream_t st)
 {
     const bool want_cm = d_cm_acc != nullptr;
-    if (n <= SMALL_N) {
-        double *cterm = d_cterm ? d_cterm : (want_cm ? dbuf<double>(ctx, "cterm", 3 * (size_t)n) : nullptr);
-        double *p3 = want_cm ? dbuf<double>(ctx, "sum3", 3 * (size_t)ncg_small_partials(n)) : nullptr;
-        if (want_cm && (!cterm || !p3)) return NCG_ERR_NOMEM;
-        ProfScope ps(ctx, st, prec == NCG_F64_FAST ? "pairs_fast64" : "pairs_mixed");
-        return ncg_small_launch(ctx, d_b, n, d_ex, c, d_central, G, soft, d_acc, cterm, p3,
-                                d_cm_acc, st);
-    }
-    const TiledShape t = tiled_shape(ctx, n, prec);
-    Scal *sc = dbuf<Scal>(ctx, "t_scal", 1);
-    int *perm = dbuf<int>(ctx, "t_perm", n);
-    double4 *jd = dbuf<double4>(ctx, "t_jd", (size_t)t.ntiles * TJ + t.iblock);
-    double *jsig = dbuf<double>(ctx, "t_jsig", (size_t)t.ntiles * TJ + t.iblock);
-    float4 *ju = dbuf<float4>(ctx, "t_ju", (size_t)t.ntiles * TJ);
-    TileHdr *th = dbuf<TileHdr>(ctx, "t_th", t.ntiles);
-    TileHdr *ih = dbuf<TileHdr>(ctx, "t_ih", t.ngroups);
-    double *part = dbuf<double>(ctx, "t_part", (size_t)t.nsplit * 3 * n);
-    double *cterm = d_cterm ? d_cterm : (want_cm ? dbuf<double>(ctx, "cterm", 3 * (size_t)n) : nullptr);
-    double *p3 = want_cm ? dbuf<double>(ctx, "sum3", 3 * (size_t)t.nbb) : nullptr;
-    if (!sc || !perm || !jd || !jsig || !ju || !th || !ih || !part || (want_cm && (!cterm || !p3)))
-        return NCG_ERR_NOMEM;
-    hipLaunchKernelGGL(k_gather, dim3(cdiv(n, NT)), dim3(NT), 0, st, d_b, n, perm, sc, G, soft, jd,
-                       jsig);
-    hipLaunchKernelGGL(k_tiles, dim3(t.ntiles), dim3(NT), 0, st, jd, sc, TJ, th, ju);
-    hipLaunchKernelGGL(k_tiles, dim3(t.ngroups), dim3(NT), 0, st, jd, sc, WV * t.ipl, ih,
+    hipLaunchKernelGGL(k_gather, dim3(cdiv(n, NT)), dim3(NT), 0, st, d_b, n, w.perm, w.sc, G, soft,
+                       w.jd, w.jsig);
+    hipLaunchKernelGGL(k_tiles, dim3(t.ntiles), dim3(NT), 0, st, w.jd, w.sc, TJ, w.th, w.ju);
+    hipLaunchKernelGGL(k_tiles, dim3(t.ngroups), dim3(NT), 0, st, w.jd, w.sc, WV * t.ipl, w.ih,
                        (float4 *)nullptr);
     NCG_CHECK_LAUNCH();
     const dim3 grid(t.nib, t.nsplit);
     const double theta = t.theta;
     const int src = t.src;
+    const double4 *jd = w.jd;
+    const double *jsig = w.jsig;
+    const float4 *ju = w.ju;
+    const TileHdr *th = w.th, *ih = w.ih;
+    const Scal *sc = w.sc;
+    double *part = w.part;
     {
         ProfScope ps(ctx, st, prec == NCG_F64_FAST ? "pairs_fast64" : "pairs_mixed");
         if (prec == NCG_F64_FAST) {
@@ -841,16 +892,37 @@ ncg_status ncg_tiled_eval(ncg_ctx *ctx, const DBody *d_b, int32_t n, const int32
     }
     const int cp = (c && c->present) ? 1 : 0;
     const double mu = cp ? G * c->mass : 0.0;
-    hipLaunchKernelGGL(k_finish, dim3(cdiv(n, NT)), dim3(NT), 0, st, d_b, perm, sc, part, t.nsplit,
-                       n, cp, cp ? c->x : 0.0, cp ? c->y : 0.0, cp ? c->z : 0.0, mu, d_central, G,
-                       d_acc, cterm);
+    hipLaunchKernelGGL(k_finish, dim3(cdiv(n, NT)), dim3(NT), 0, st, d_b, w.perm, w.sc, w.part,
+                       t.nsplit, n, cp, cp ? c->x : 0.0, cp ? c->y : 0.0, cp ? c->z : 0.0, mu,
+                       d_central, G, d_acc, w.cterm);
     NCG_CHECK_LAUNCH();
     if (want_cm) {
-        hipLaunchKernelGGL(k_sum3, dim3(t.nbb), dim3(NT), 0, st, cterm, n, d_ex, p3);
-        hipLaunchKernelGGL(k_sum3_final, dim3(1), dim3(NT), 0, st, p3, t.nbb, d_cm_acc);
+        hipLaunchKernelGGL(k_sum3, dim3(t.nbb), dim3(NT), 0, st, w.cterm, n, d_ex, w.p3);
+        hipLaunchKernelGGL(k_sum3_final, dim3(1), dim3(NT), 0, st, w.p3, t.nbb, d_cm_acc);
         NCG_CHECK_LAUNCH();
     }
     return NCG_OK;
+}
+
+ncg_status ncg_tiled_eval(ncg_ctx *ctx, const DBody *d_b, int32_t n, const int32_t *d_ex,
+                          const ncg_central *c, const double *d_central, double G, double soft,
+                          ncg_precision prec, double *d_acc, double *d_cterm, double *d_cm_acc,
+                          hipStream_t st)
+{
+    const bool want_cm = d_cm_acc != nullptr;
+    if (n <= SMALL_N) {
+        double *cterm = d_cterm ? d_cterm : (want_cm ? dbuf<double>(ctx, "cterm", 3 * (size_t)n) : nullptr);
+        double *p3 = want_cm ? dbuf<double>(ctx, "sum3", 3 * (size_t)ncg_small_partials(n)) : nullptr;
+        if (want_cm && (!cterm || !p3)) return NCG_ERR_NOMEM;
+        ProfScope ps(ctx, st, prec == NCG_F64_FAST ? "pairs_fast64" : "pairs_mixed");
+        return ncg_small_launch(ctx, d_b, n, d_ex, c, d_central, G, soft, d_acc, cterm, p3,
+                                d_cm_acc, st);
+    }
+    const TiledShape t = tiled_shape(ctx, n, prec);
+    TiledWork w;
+    ncg_status r = work_eval_ctx(ctx, t, n, want_cm, d_cterm, w);
+    if (r != NCG_OK) return r;
+    return tiled_eval_w(ctx, t, w, d_b, n, d_ex, c, d_central, G, soft, prec, d_acc, d_cm_acc, st);
 }
 
 ncg_status ncg_gravity_dev_impl(ncg_ctx *ctx, const DBody *d_b, int32_t n, const int32_t *d_ex,
@@ -881,6 +953,137 @@ ncg_status ncg_gravity_dev_impl(ncg_ctx *ctx, const DBody *d_b, int32_t n, const
     if (r != NCG_OK) return r;
     return ncg_tiled_eval(ctx, d_b, n, d_ex, cp ? c : nullptr, nullptr, G, soft, prec, d_acc,
                           d_cterm, cp ? d_cm_acc : nullptr, st);
+}
+
+// ---- allocation-free evaluation ----------------------------------------------
+// Scratch of ncg_gravity_allpairs_static_dev: one array per GPU, in the code
+// object's static device memory (each GPU the library runs on holds it until
+// the process ends).  Host code never passes `ncg_scratch` itself to a
+// kernel or a copy (in host code the name is a stand-in, not a device
+// address): it uses the address hipGetSymbolAddress returns for the current
+// device, looked up once per device at context creation.
+static __device__ __align__(256) unsigned char ncg_scratch[NCG_STATIC_SCRATCH_BYTES];
+
+namespace {
+constexpr int SCRATCH_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[SCRATCH_MAX_DEVICES] = {};
+}  // namespace
+
+ncg_status ncg_static_scratch_init(ncg_ctx *ctx)
+{
+    const int dev = ctx->device;
+    NCG_REQUIRE(dev >= 0 && dev < SCRATCH_MAX_DEVICES, "device ordinal %d out of range", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[dev]) return NCG_OK;
+    void *p = nullptr;
+    NCG_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(ncg_scratch)));
+    NCG_REQUIRE(p != nullptr, "no address for the static scratch array");
+    g_scratch_ptr[dev] = p;
+    return NCG_OK;
+}
+
+// Carves `w` (and the exact form's per-body terms) out of [base, base + cap);
+// lowers t.nsplit when the j-split partials would not fit.  Returns false when
+// even one split does not fit.
+static bool work_carve(unsigned char *base, size_t cap, TiledShape &t, int32_t n, bool tiled,
+                       bool want_cm, size_t sort_tmp_bytes, TiledWork &w)
+{
+    size_t off = 0;
+    auto take = [&](size_t bytes) -> void * {
+        const size_t at = (off + 255) & ~(size_t)255;
+        if (at + bytes > cap) return nullptr;
+        off = at + bytes;
+        return base + at;
+    };
+    w = TiledWork();
+    w.cterm = (double *)take(sizeof(double) * 3 * (size_t)n);
+    w.p3 = (double *)take(sizeof(double) * 3 * (size_t)std::max(t.nbb, ncg_small_partials(n)));
+    if (!w.cterm || !w.p3) return false;
+    if (!tiled) return true;
+    w.sc = (Scal *)take(sizeof(Scal));
+    w.bpart = (double *)take(sizeof(double) * 8 * (size_t)t.nbb);
+    w.key = (unsigned *)take(sizeof(unsigned) * (size_t)n);
+    w.key2 = (unsigned *)take(sizeof(unsigned) * (size_t)n);
+    w.idx = (int *)take(sizeof(int) * (size_t)n);
+    w.perm = (int *)take(sizeof(int) * (size_t)n);
+    w.sort_bytes = sort_tmp_bytes;
+    w.sort_tmp = take(sort_tmp_bytes);
+    const size_t slots = (size_t)t.ntiles * TJ + t.iblock;
+    w.jd = (double4 *)take(sizeof(double4) * slots);
+    w.jsig = (double *)take(sizeof(double) * slots);
+    w.ju = (float4 *)take(sizeof(float4) * (size_t)t.ntiles * TJ);
+    w.th = (TileHdr *)take(sizeof(TileHdr) * (size_t)t.ntiles);
+    w.ih = (TileHdr *)take(sizeof(TileHdr) * (size_t)t.ngroups);
+    if (!w.sc || !w.bpart || !w.key || !w.key2 || !w.idx || !w.perm || !w.sort_tmp || !w.jd ||
+        !w.jsig || !w.ju || !w.th || !w.ih)
+        return false;
+    const size_t at = (off + 255) & ~(size_t)255;
+    const size_t per_split = sizeof(double) * 3 * (size_t)n;
+    if (at + per_split > cap) return false;
+    t.nsplit = (int)std::min<size_t>((size_t)t.nsplit, (cap - at) / per_split);
+    w.part = (double *)take(per_split * (size_t)t.nsplit);
+    return w.part != nullptr;
+}
+
+// ---- C ABI ------------------------------------------------------------------
+extern "C" ncg_status ncg_gravity_allpairs_static_dev(ncg_ctx *ctx, const ncg_body *d_bodies,
+                                                      int32_t n, const int32_t *d_exists,
+                                                      const ncg_central *central, double G,
+                                                      double softening_parameter,
+                                                      ncg_precision prec, double *d_acc,
+                                                      double *d_cm_acc, void *stream)
+{
+    if (!ctx) return NCG_ERR_INVALID;
+    NCG_REQUIRE(n >= 0 && (n == 0 || (d_bodies && d_acc)), "invalid arguments");
+    NCG_REQUIRE(n <= NCG_STATIC_MAX_N, "n = %d exceeds the static scratch's limit of %d bodies",
+                (int)n, NCG_STATIC_MAX_N);
+    NCG_REQUIRE(prec == NCG_F64_EXACT || prec == NCG_F64_FAST || prec == NCG_MIXED,
+                "unknown precision %d", (int)prec);
+    if (n == 0) return NCG_OK;
+    NCG_HIP(hipSetDevice(ctx->device));
+    unsigned char *base = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        if (ctx->device >= 0 && ctx->device < SCRATCH_MAX_DEVICES)
+            base = static_cast<unsigned char *>(g_scratch_ptr[ctx->device]);
+    }
+    NCG_REQUIRE(base != nullptr, "static scratch not set up for device %d", ctx->device);
+    hipStream_t st = pick_stream(ctx, stream);
+    const DBody *db = reinterpret_cast<const DBody *>(d_bodies);
+    const int cp = (central && central->present) ? 1 : 0;
+    const bool want_cm = cp && d_cm_acc;
+    const bool tiled = prec != NCG_F64_EXACT && n > SMALL_N;
+    TiledShape t = tiled_shape(ctx, n, prec == NCG_F64_EXACT ? NCG_MIXED : prec);
+    size_t sbytes = 0;
+    if (tiled) {
+        ncg_status r = sort_bytes(ctx, n, st, &sbytes);
+        if (r != NCG_OK) return r;
+    }
+    TiledWork w;
+    if (!work_carve(base, NCG_STATIC_SCRATCH_BYTES, t, n, tiled, want_cm, sbytes, w))
+        return ncg_fail(ctx, NCG_ERR_CAPACITY, "n = %d does not fit the static scratch (%d bytes)",
+                        (int)n, NCG_STATIC_SCRATCH_BYTES);
+    // Every scratch word the kernels read is written first in the same call:
+    // nothing needs clearing between calls.
+    if (prec == NCG_F64_EXACT) {
+        ncg_status r = ncg_grav_exact_launch(ctx, db, n, d_exists, cp ? central : nullptr, G,
+                                             softening_parameter, d_acc, w.cterm, st);
+        if (r != NCG_OK) return r;
+        if (want_cm) return ncg_cm_seq_launch(ctx, w.cterm, n, d_exists, 1, d_cm_acc, st);
+        return NCG_OK;
+    }
+    if (!tiled)
+        return ncg_small_launch(ctx, db, n, d_exists, cp ? central : nullptr, nullptr, G,
+                                softening_parameter, d_acc, w.cterm, w.p3,
+                                want_cm ? d_cm_acc : nullptr, st);
+    // The order's launch shape (bounding-box partials) is the mixed form's,
+    // as in ncg_tiled_order.
+    const TiledShape to = tiled_shape(ctx, n, NCG_MIXED);
+    ncg_status r = tiled_order_w(ctx, to, w, db, n, d_exists, st);
+    if (r != NCG_OK) return r;
+    return tiled_eval_w(ctx, t, w, db, n, d_exists, cp ? central : nullptr, nullptr, G,
+                        softening_parameter, prec, d_acc, want_cm ? d_cm_acc : nullptr, st);
 }
 
 // ---- C ABI ------------------------------------------------------------------

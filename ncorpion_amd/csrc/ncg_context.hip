@@ -148,6 +148,11 @@ ncg_status ncg_create(ncg_ctx **out, int device)
         const long v = atol(rc);
         if (v > 0 && v < (1L << 30)) ctx->replay_cap = (unsigned)v;
     }
+    if (ncg_static_scratch_init(ctx) != NCG_OK) {
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return NCG_ERR_DEVICE;
+    }
     *out = ctx;
     return NCG_OK;
 }

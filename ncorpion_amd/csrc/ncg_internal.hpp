@@ -119,6 +119,11 @@ struct ncg_ctx {
 
 void ncg_rk4_release(ncg_ctx *ctx);
 
+// Looks up the address of the static scratch array of
+// ncg_gravity_allpairs_static_dev on the context's device (current), once
+// per device.
+ncg_status ncg_static_scratch_init(ncg_ctx *ctx);
+
 // Candidate capacity of a device replay whose default is `dflt`.
 static inline unsigned ncg_replay_cap(const ncg_ctx *ctx, unsigned dflt)
 {
