@@ -328,7 +328,7 @@ static ncg_status fwd_step_store(ncg_ctx *ctx, DBody *y, int32_t n, double *c0, 
     double *sc = dbuf<double>(ctx, "a_sc", 8);
     double *cm = dbuf<double>(ctx, "a_cm", 4);
     if (!Y || !acc || !sx || !sv || !Yc || !sc || !cm) return NCG_ERR_NOMEM;
-    ncg_status r = ncg_tiled_order(ctx, y, n, nullptr, st);
+    ncg_status r = ncg_tiled_order(ctx, y, n, nullptr, NCG_F64_FAST, st);
     if (r != NCG_OK) return r;
     ncg_central cf = {};
     cf.present = 1;  // the device central record's own flag decides (k_finish reads it)

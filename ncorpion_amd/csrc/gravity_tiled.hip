@@ -7,10 +7,13 @@
 //
 // Pipeline of one evaluation (all on one stream, no host synchronisation):
 //   1. k_bbox_*      bounding box, largest radius and count of existing bodies
-//   2. k_morton      30-bit Morton key per body over the bounding CUBE (the
-//                    same cell size on every axis, so that runs of keys are
-//                    compact in space also for a thin disk); holes sort last
-//   3. radix sort    (key, index) pairs -> spatially coherent order
+//   2. k_morton      mixed form: 30-bit Morton key per body over the bounding
+//                    CUBE (the same cell size on every axis, so that runs of
+//                    keys are compact in space also for a thin disk); FP64-fast
+//                    form (no near/far split): existence only, so that its
+//                    sums keep body order whatever the positions; holes last
+//   3. radix sort    (key, index) pairs -> spatially coherent (mixed) or body
+//                    (FP64-fast) order
 //   4. k_gather      sorted FP64 records {x, y, z, G m} and s R
 //   5. k_tiles       per tile of TJ bodies: FP64 anchor (box centre), FP32
 //                    half extent, the range of (G m)^(-1/2) over the tile and
@@ -24,12 +27,13 @@
 //                      near: FP64 separations, FP64 rsqrt refined by one Newton
 //                            step, FP64 accumulation (also every tile in
 //                            NCG_F64_FAST);
-//                      far:  scaled FP32 separations t = s_j (C + u_j - u_i)
-//                            with s_j = (G m_j)^(-1/2), C the FP64 anchor
-//                            difference and u the FP32 offsets from the
-//                            anchors; then G m_j d / r^3 = t / |t|^3, so a
-//                            term is 12 FP32 operations; FP32 tile partial
-//                            added into FP64.
+//                      far:  scaled FP32 separations t = s_j (w_i + u_j)
+//                            with s_j ~ (G m_j)^(-1/2) (the FP32 value whose
+//                            s^-2 is nearest G m_j), w_i = A_J - x_i rounded
+//                            once from FP64 per tile and u_j the offset from
+//                            the tile's anchor A_J; then G m_j d / r^3 =
+//                            t / |t|^3, so a term is 12 FP32 operations; FP32
+//                            tile partial added into FP64.
 //                    A pair is far when its box distance is at least theta
 //                    times (anchor distance + half extents), which bounds the
 //                    FP32 separation error, when the largest softening length
@@ -154,16 +158,23 @@ __global__ __launch_bounds__(NT) void k_bbox_final(const double *__restrict__ pa
 }
 
 // ---- 2. Morton keys -------------------------------------------------------
+// spatial = 0 (the FP64-fast form, which has no near/far split): key 0 for
+// existing bodies and 1 for holes, so the stable sort keeps body order and
+// the field's sums do not depend on where the bodies are.
 __global__ __launch_bounds__(NT) void k_morton(const DBody *__restrict__ b, int n,
                                                const int32_t *__restrict__ ex,
-                                               const Scal *__restrict__ sc,
+                                               const Scal *__restrict__ sc, int spatial,
                                                unsigned *__restrict__ key, int *__restrict__ idx)
 {
     const int k = blockIdx.x * NT + threadIdx.x;
     if (k >= n) return;
     idx[k] = k;
     if (ex && !ex[k]) {
-        key[k] = 0xffffffffu;
+        key[k] = spatial ? 0xffffffffu : 1u;
+        return;
+    }
+    if (!spatial) {
+        key[k] = 0u;
         return;
     }
     unsigned q[3];
@@ -277,9 +288,25 @@ __global__ __launch_bounds__(NT) void k_tiles(const double4 *__restrict__ jd,
             if (q < cnt) {
                 const double4 p = jd[s];
                 if (p.w > 0.0) {
-                    const double sj = 1.0 / sqrt(p.w);
-                    r = make_float4((float)(sj * (p.x - A[0])), (float)(sj * (p.y - A[1])),
-                                    (float)(sj * (p.z - A[2])), (float)sj);
+                    // The term is G m_j d / r^3 only as far as s^-2 = G m_j:
+                    // of the FP32 values next to (G m)^-1/2, the one whose
+                    // s^-2 is nearest G m (within 2^-24 relative, as G m
+                    // rounded to FP32 would be), and the offsets scaled by
+                    // that same s.
+                    const double sd = 1.0 / sqrt(p.w);
+                    float sf = (float)sd, best = sf;
+                    double berr = fabs(1.0 / ((double)sf * (double)sf) - p.w);
+                    const float cand[2] = {nextafterf(sf, 0.f), nextafterf(sf, 3.0e38f)};
+                    for (int c = 0; c < 2; ++c) {
+                        const double e = fabs(1.0 / ((double)cand[c] * (double)cand[c]) - p.w);
+                        if (e < berr) {
+                            berr = e;
+                            best = cand[c];
+                        }
+                    }
+                    const double sb = (double)best;
+                    r = make_float4((float)(sb * (p.x - A[0])), (float)(sb * (p.y - A[1])),
+                                    (float)(sb * (p.z - A[2])), best);
                 }
             }
             ju[s] = r;
@@ -394,7 +421,6 @@ __global__ __launch_bounds__(NT) void k_pairs(const double4 *__restrict__ jd,
 
     const TileHdr ih = ihdr[gvalid ? iblk * (NT / WV) + w : 0];
     double xi[IPL], yi[IPL], zi[IPL], si[IPL];
-    float ux[IPL], uy[IPL], uz[IPL];
     double ax[IPL], ay[IPL], az[IPL];
 #pragma unroll
     for (int q = 0; q < IPL; ++q) {
@@ -404,9 +430,6 @@ __global__ __launch_bounds__(NT) void k_pairs(const double4 *__restrict__ jd,
         yi[q] = p.y;
         zi[q] = p.z;
         si[q] = jsig[i];
-        ux[q] = (float)(p.x - ih.ax);
-        uy[q] = (float)(p.y - ih.ay);
-        uz[q] = (float)(p.z - ih.az);
         ax[q] = ay[q] = az[q] = 0.0;
     }
     // Largest softening length squared and the FP32 resolution test.
@@ -477,13 +500,13 @@ __global__ __launch_bounds__(NT) void k_pairs(const double4 *__restrict__ jd,
                 __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
                 __builtin_amdgcn_wave_barrier();
             }
-            const float cx = (float)Cx, cy = (float)Cy, cz = (float)Cz;
+            // w_i = A_J - x_i, rounded once from FP64.
             float wx[IPL], wy[IPL], wz[IPL], fx[IPL], fy[IPL], fz[IPL];
 #pragma unroll
             for (int q = 0; q < IPL; ++q) {
-                wx[q] = cx - ux[q];
-                wy[q] = cy - uy[q];
-                wz[q] = cz - uz[q];
+                wx[q] = (float)(th.ax - xi[q]);
+                wy[q] = (float)(th.ay - yi[q]);
+                wz[q] = (float)(th.az - zi[q]);
                 fx[q] = fy[q] = fz[q] = 0.f;
             }
             // Slots past cnt hold massless records (exact zeros), so whole
@@ -763,12 +786,18 @@ struct TiledWork {
     double *cterm = nullptr, *p3 = nullptr;
 };
 
+// Temporary storage of the order's sort, for both key widths (32-bit Morton
+// keys, 1-bit existence keys).
 static ncg_status sort_bytes(ncg_ctx *ctx, int32_t n, hipStream_t st, size_t *bytes)
 {
-    *bytes = 0;
-    NCG_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, *bytes, (unsigned *)nullptr,
+    size_t b32 = 0, b1 = 0;
+    NCG_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, b32, (unsigned *)nullptr,
                                                (unsigned *)nullptr, (int *)nullptr, (int *)nullptr,
                                                n, 0, 32, st));
+    NCG_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, b1, (unsigned *)nullptr,
+                                               (unsigned *)nullptr, (int *)nullptr, (int *)nullptr,
+                                               n, 0, 1, st));
+    *bytes = std::max(b32, b1);
     return NCG_OK;
 }
 
@@ -808,29 +837,34 @@ static ncg_status work_eval_ctx(ncg_ctx *ctx, const TiledShape &t, int32_t n, bo
     return NCG_OK;
 }
 
+// The order of the evaluations that follow: spatial (Morton keys over the
+// bounding cube) for the mixed form, whose near/far test needs compact tiles;
+// body order (existing bodies first) for the FP64-fast form.
 static ncg_status tiled_order_w(ncg_ctx *ctx, const TiledShape &t, const TiledWork &w,
-                                const DBody *d_b, int32_t n, const int32_t *d_ex, hipStream_t st)
+                                const DBody *d_b, int32_t n, const int32_t *d_ex,
+                                ncg_precision prec, hipStream_t st)
 {
+    const int spatial = prec == NCG_MIXED ? 1 : 0;
     hipLaunchKernelGGL(k_bbox_partial, dim3(t.nbb), dim3(NT), 0, st, d_b, n, d_ex, w.bpart);
     hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(NT), 0, st, w.bpart, t.nbb, w.sc);
-    hipLaunchKernelGGL(k_morton, dim3(cdiv(n, NT)), dim3(NT), 0, st, d_b, n, d_ex, w.sc, w.key,
-                       w.idx);
+    hipLaunchKernelGGL(k_morton, dim3(cdiv(n, NT)), dim3(NT), 0, st, d_b, n, d_ex, w.sc, spatial,
+                       w.key, w.idx);
     NCG_CHECK_LAUNCH();
     size_t bytes = w.sort_bytes;
     NCG_HIP(hipcub::DeviceRadixSort::SortPairs(w.sort_tmp, bytes, w.key, w.key2, w.idx, w.perm, n,
-                                               0, 32, st));
+                                               0, spatial ? 32 : 1, st));
     return NCG_OK;
 }
 
 ncg_status ncg_tiled_order(ncg_ctx *ctx, const DBody *d_b, int32_t n, const int32_t *d_ex,
-                           hipStream_t st)
+                           ncg_precision prec, hipStream_t st)
 {
     if (n <= SMALL_N) return NCG_OK;  // the direct form needs no order
     const TiledShape t = tiled_shape(ctx, n, NCG_MIXED);
     TiledWork w;
     ncg_status r = work_order_ctx(ctx, t, n, st, w);
     if (r != NCG_OK) return r;
-    return tiled_order_w(ctx, t, w, d_b, n, d_ex, st);
+    return tiled_order_w(ctx, t, w, d_b, n, d_ex, prec, st);
 }
 
 int ncg_small_partials(int32_t n) { return std::max(1, std::min(1024, (int)cdiv(n, NT))); }
@@ -949,7 +983,7 @@ ncg_status ncg_gravity_dev_impl(ncg_ctx *ctx, const DBody *d_b, int32_t n, const
         return NCG_OK;
     }
     NCG_REQUIRE(prec == NCG_F64_FAST || prec == NCG_MIXED, "unknown precision %d", (int)prec);
-    ncg_status r = ncg_tiled_order(ctx, d_b, n, d_ex, st);
+    ncg_status r = ncg_tiled_order(ctx, d_b, n, d_ex, prec, st);
     if (r != NCG_OK) return r;
     return ncg_tiled_eval(ctx, d_b, n, d_ex, cp ? c : nullptr, nullptr, G, soft, prec, d_acc,
                           d_cterm, cp ? d_cm_acc : nullptr, st);
@@ -1080,7 +1114,7 @@ extern "C" ncg_status ncg_gravity_allpairs_static_dev(ncg_ctx *ctx, const ncg_bo
     // The order's launch shape (bounding-box partials) is the mixed form's,
     // as in ncg_tiled_order.
     const TiledShape to = tiled_shape(ctx, n, NCG_MIXED);
-    ncg_status r = tiled_order_w(ctx, to, w, db, n, d_exists, st);
+    ncg_status r = tiled_order_w(ctx, to, w, db, n, d_exists, prec, st);
     if (r != NCG_OK) return r;
     return tiled_eval_w(ctx, t, w, db, n, d_exists, cp ? central : nullptr, nullptr, G,
                         softening_parameter, prec, d_acc, want_cm ? d_cm_acc : nullptr, st);

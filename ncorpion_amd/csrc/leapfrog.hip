@@ -664,7 +664,7 @@ extern "C" ncg_status ncg_leapfrog_dev(ncg_ctx *ctx, ncg_body *d_bodies, int32_t
     int32_t win0 = 0;  // first step of the current window
     for (int32_t s = 0; s < nsteps && res == NCG_OK; ++s) {
         if (prec != NCG_F64_EXACT) {
-            res = ncg_tiled_order(ctx, b, n, ex, st);  // eager: the sort stays out of graphs
+            res = ncg_tiled_order(ctx, b, n, ex, prec, st);  // eager: the sort stays out of graphs
             if (res != NCG_OK) break;
         }
         if (!use_graph || s == 0) {
@@ -703,7 +703,7 @@ extern "C" ncg_status ncg_leapfrog_dev(ncg_ctx *ctx, ncg_body *d_bodies, int32_t
             if ((res = clear_dirty()) != NCG_OK) break;
             if ((res = write_stat(snap_coll)) != NCG_OK) break;
             for (int32_t q = win0; q <= s && res == NCG_OK; ++q) {
-                if (prec != NCG_F64_EXACT && (res = ncg_tiled_order(ctx, b, n, ex, st)) != NCG_OK)
+                if (prec != NCG_F64_EXACT && (res = ncg_tiled_order(ctx, b, n, ex, prec, st)) != NCG_OK)
                     break;
                 res = step(true);
             }

@@ -253,10 +253,11 @@ static inline hipStream_t pick_stream(ncg_ctx *ctx, void *stream)
 static inline unsigned cdiv(long a, long b) { return (unsigned)((a + b - 1) / b); }
 
 // Internal launchers used across translation units.
-// Spatial order of the tiled forms: bounding box, Morton keys, radix sort.
+// Order of the tiled forms: bounding box, keys, radix sort -- Morton keys
+// for NCG_MIXED (spatially compact tiles), body order for NCG_F64_FAST.
 // Leaves the permutation and the count of existing bodies on the device.
 ncg_status ncg_tiled_order(ncg_ctx *ctx, const DBody *d_b, int32_t n, const int32_t *d_ex,
-                           hipStream_t s);
+                           ncg_precision prec, hipStream_t s);
 // One evaluation with the order of the last ncg_tiled_order (no allocation,
 // no host synchronisation: safe under stream capture once buffers exist).
 ncg_status ncg_tiled_eval(ncg_ctx *ctx, const DBody *d_b, int32_t n, const int32_t *d_ex,

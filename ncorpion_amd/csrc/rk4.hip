@@ -186,7 +186,7 @@ extern "C" ncg_status ncg_rk4_steps_dev(ncg_ctx *ctx, ncg_body *d_bodies, int32_
     const bool cp = present != 0.0;
     const bool use_graph = (flags & 1) != 0;
     for (int32_t step = 0; step < nsteps; ++step) {
-        ncg_status r = ncg_tiled_order(ctx, y0, n, d_exists, st);
+        ncg_status r = ncg_tiled_order(ctx, y0, n, d_exists, prec, st);
         if (r != NCG_OK) return r;
         if (!use_graph) {
             r = rk4_body(ctx, y0, n, d_exists, d_central, grav_const, softening, prec, dt, st, cp);

@@ -71,6 +71,19 @@ def test_c3_fast_forms_on_sampled_rows(ctx, c3, prec, bound):
     assert rel.max() <= bound, rel.max()
 
 
+def test_c3_mixed_every_body_against_fast64(ctx, c3):
+    """The mixed form under the tuning table's row for this size, on all
+    262,144 bodies: within 1e-6 max relative error of the FP64-fast form
+    (itself held to the oracle's rows at 1e-11 above)."""
+    b, _, _, _ = c3
+    f64, _ = ctx.gravity_allpairs(b, None, None, 1.0, C3_SOFT, "fast64")
+    mix, _ = ctx.gravity_allpairs(b, None, None, 1.0, C3_SOFT, "mixed")
+    rel = np.linalg.norm(mix - f64, axis=1) / np.linalg.norm(f64, axis=1)
+    print("c3 mixed vs fast64, all bodies: max rel %.3e, p99.9 %.3e" %
+          (rel.max(), np.quantile(rel, 0.999)))
+    assert rel.max() <= 1e-6, rel.max()
+
+
 def c2_workload():
     return ic.disk(C2_N, seed=0, mass=1e-10, radius=5e-5)
 
