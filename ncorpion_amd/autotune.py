@@ -8,8 +8,8 @@ the mixed form's max relative error on the mutual acceleration, against the
 FP64 form, stays below `max_err`.
 
 Ensembles: on 512 seeded disks of log-uniform N in [256, 8192], bodies per
-lane and persistent workgroups per CU of the ensemble pass are timed the same
-way.
+lane of the FP64-fast ensemble pass are timed the same way, and the mixed
+form's theta is chosen under the same error bound.
 
 The fastest admissible shape per (form, N) is written to the tuning table
 that libncg.so reads at context creation (ncg_tuning.txt beside it); rows of
@@ -98,9 +98,13 @@ def tune(ctx, n, reps=3, max_err=5e-7, soft=1e-6, thetas=(0.3, 0.35, 0.5), log=p
     return out
 
 
-def tune_ensemble(ctx, reps=3, log=print):
-    """Returns {form: (ipl, workgroups_per_cu, theta, src, ms)} for the 512-disk
-    ensemble; the table row applies to every ensemble (n_min 1)."""
+def tune_ensemble(ctx, reps=3, max_err=5e-7, thetas=(0.3, 0.35, 0.5), log=print):
+    """Returns {form: (ipl, 0, theta, src, ms)} for the 512-disk ensemble; the
+    table row applies to every ensemble (n_min 1).  The mixed
+    form runs one body per lane; its theta is admissible when the max
+    relative error of the mutual accelerations of every body, against the
+    FP64-fast ensemble form, stays below max_err (else the most accurate
+    theta is used)."""
     import numpy as np
     import torch
     from . import ic
@@ -112,13 +116,34 @@ def tune_ensemble(ctx, reps=3, log=print):
     cen[:, 6] = 1.0
     cen[:, 7] = 1.0
     ct = torch.from_numpy(cen).cuda()
+    c0 = torch.zeros_like(ct)  # central bodies absent: the mutual part only
     acc = torch.zeros((bt.shape[0], 3), dtype=torch.float64, device="cuda")
+    ref = torch.zeros_like(acc)
     cm = torch.zeros((len(sizes), 3), dtype=torch.float64, device="cuda")
+    ctx.set_tiling(1, 0, 0.35, 0)
+    ctx.ensemble_gravity_dev(bt, off, c0, ref, cm, 1.0, 0.0, "fast64")
+    ctx.synchronize()
+    rnorm = torch.linalg.norm(ref, dim=1)
     out = {}
     for prec in ("mixed", "fast64"):
+        cands = (0.35,)
+        if prec == "mixed":
+            errs = {}
+            for theta in thetas:
+                ctx.set_tiling(1, 0, theta, 0)
+                ctx.ensemble_gravity_dev(bt, off, c0, acc, cm, 1.0, 0.0, "mixed")
+                ctx.synchronize()
+                errs[theta] = float(torch.max(torch.linalg.norm(acc - ref, dim=1) / rnorm))
+                log("  ensemble theta=%.2f max rel err %.2e" % (theta, errs[theta]))
+            cands = tuple(t for t in thetas if errs[t] <= max_err)
+            if not cands:
+                cands = (min(errs, key=errs.get),)
+                log("  ensemble: no theta within %.1e; using the most accurate, %.2f" %
+                    (max_err, cands[0]))
         best = None
-        for ipl, wg in itertools.product((1, 2), (2, 4, 8, 12, 16)):
-            ctx.set_tiling(ipl, wg, 0.35, 0)
+        ipls = (1,) if prec == "mixed" else (1, 2)
+        for theta, ipl in itertools.product(cands, ipls):
+            ctx.set_tiling(ipl, 0, theta, 0)
             ctx.ensemble_gravity_dev(bt, off, ct, acc, cm, 1.0, 0.0, prec)
             ctx.synchronize()
             ctx.profile(True)
@@ -129,9 +154,9 @@ def tune_ensemble(ctx, reps=3, log=print):
             ms, cnt = ctx.profile_read("ens_" + prec)
             ctx.profile(False)
             ms /= max(cnt, 1)
-            log("  ensemble %s ipl=%d wg/CU=%d %.3f ms" % (prec, ipl, wg, ms))
+            log("  ensemble %s ipl=%d theta=%.2f %.3f ms" % (prec, ipl, theta, ms))
             if best is None or ms < best[4]:
-                best = (ipl, wg, 0.35, 0, ms)  # theta column unused (must be > 0)
+                best = (ipl, 0, theta, 0, ms)
         out[prec] = best
     return out
 
@@ -156,7 +181,7 @@ def write_table(rows, path):
         f.write("# Written by ncorpion_amd/autotune.py on an MI355X; one row per (form, N bucket):\n")
         f.write("# the row with the largest n_min <= N applies.\n")
         f.write("# form   n_min   ipl  nsplit  theta  src   (measured ms per evaluation at n_min)\n")
-        f.write("# (ens_* forms: nsplit is persistent workgroups per CU; theta, src unused)\n")
+        f.write("# (ens_* forms: nsplit and src unused; theta used by ens_mixed)\n")
         for prec, nmin, (ipl, ns, theta, src, ms) in rows:
             f.write("%-7s  %-7d %-4d %-7d %-6.2f %-4d  # %.3f ms\n" % (prec, nmin, ipl, ns, theta, src, ms))
 
@@ -188,7 +213,7 @@ def main(argv=None):
                     rows.append((prec, nmin, r[prec]))
     if "ensemble" in args.forms:
         tuned |= {"ens_mixed", "ens_fast64"}
-        r = tune_ensemble(ctx, log=log)
+        r = tune_ensemble(ctx, max_err=args.max_err, log=log)
         res["ensemble"] = r
         for prec in ("mixed", "fast64"):
             rows.append(("ens_" + prec, 1, r[prec]))

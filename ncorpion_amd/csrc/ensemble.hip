@@ -2,52 +2,75 @@
 //
 // Every system is a body array of its own (concatenated, offsets[S+1]) with
 // its own central body {x, y, z, vx, vy, vz, M, present}.  A force
-// evaluation is ONE persistent launch: workgroups take work units from a
-// queue (an atomic counter reset by a fill kernel, so the launch can be
-// captured in a graph); a unit is (system, block of bodies) and computes those
-// bodies' accelerations against every body of the system (brute-force, the
-// field of vector_field physics.c:120-202 plus the central term), or -- for
-// collision units -- runs the closest_approach predicate (collision.c:63-172)
-// on the block's pairs (i, j > i) and appends candidates.  Units are ordered
-// by decreasing cost so the persistent workgroups finish together.
+// evaluation is ONE launch of one workgroup per work unit: a unit is (system,
+// block of 256 bodies) and computes those bodies' accelerations against every
+// body of the system (brute-force, the field of vector_field
+// physics.c:120-202 plus the central term), or -- in a collision pass -- runs
+// the closest_approach predicate (collision.c:63-172) on the block's pairs
+// and appends candidates.  Units are ordered by decreasing cost, so the
+// dispatcher starts the large ones first and the workgroups finish together.
 //
-// Forms: NCG_F64_FAST (FP64, rsqrt + one Newton step) and NCG_MIXED (FP64
-// separations rounded to FP32, FP32 1/r^3 and pair force, FP32 partials over
-// 64-body sub-tiles added into FP64 accumulators).
+// Order (once per step, outside captured graphs): each system's bodies are
+// put along a Hilbert curve over the system's bounding cube -- one radix sort
+// of (system << 30 | key) for all systems -- and laid out in padded slots:
+// system s owns slots [poff_s, poff_s + n_s rounded up to TJ), its j tiles are
+// TJ slots and its wavefront groups 64; slots past n_s are padding.
+//
+// Forms:
+//   NCG_F64_FAST  body order, FP64 (rsqrt + one Newton step), no tiles, so
+//                 the sums do not depend on where the bodies are;
+//   NCG_MIXED     the single-system mixed form per system (tiled_dev.hpp):
+//                 for each (wavefront group, j tile) pair a uniform test picks
+//                 FP64 near terms or scaled FP32 far terms t = s_j (w_i + u_j)
+//                 with FP32 tile partials added into FP64.
+// Collision pass: the same order; a wavefront group tests the pairs (p < q
+// in slot order) of the tiles its position and velocity boxes can reach
+// within dt.  The test is conservative (below), so the candidates are those
+// of the all-pairs predicate.
 //
 // RK4 (classic, as in rk4.hip) runs every stage of all systems at once; with
 // collisions on, each step starts with a collision pass of every system and a
 // replay kernel (one wavefront per system) that decides the candidates in the
 // reference's lexicographic order with the elastic / inelastic outcome, so a
 // whole step -- pass, replay, four stages -- is one captured graph.
+#include <hipcub/hipcub.hpp>
+
 #include <algorithm>
 #include <cstring>
 #include <numeric>
 
 #include "ncg_internal.hpp"
+#include "tiled_dev.hpp"
 
 namespace {
 
 constexpr int ENT = 256;  // threads per workgroup
 constexpr unsigned ENS_CAND_CAP = 1u << 20;  // candidates of one pass, all systems
-constexpr int ETJ = 256;  // j tile (LDS)
+constexpr int ETJ = 256;  // j tile (LDS) of the FP64-fast form
 
 struct EnsUnit {
     int sys;   // system
-    int i0;    // first body (local index)
+    int i0;    // first body (local index; in slot order for tiled passes)
     int cnt;   // bodies in the unit
     int kind;  // 0 gravity, 1 collision candidates
 };
 
-__device__ inline int next_unit(unsigned *counter)
-{
-    __shared__ int u;
-    if (threadIdx.x == 0) u = (int)atomicAdd(counter, 1u);
-    __syncthreads();
-    const int r = u;
-    __syncthreads();
-    return r;
-}
+struct EnsSys {
+    int off;   // first body
+    int poff;  // first padded slot (a multiple of TJ)
+    int n;     // bodies
+    int pad;
+};
+
+// Position and velocity boxes of a tile or wavefront group (collision pass).
+struct CollHdr {
+    double px, py, pz;     // position box centre
+    double vx, vy, vz;     // velocity box centre
+    float phx, phy, phz;   // half extents, rounded up
+    float vhx, vhy, vhz;
+    float rmax;            // largest radius, rounded up
+    int count;
+};
 
 __global__ void k_zero_u32(unsigned *p, int n)
 {
@@ -55,231 +78,608 @@ __global__ void k_zero_u32(unsigned *p, int n)
     if (i < n) p[i] = 0u;
 }
 
-// One FP32 interaction of the mixed form: separation rounded from FP64,
-// softening added only when SOFT.  With SELF the body at local index js is
-// compared with the lane's own (li): a body against itself has r^2 = 1e-20
-// without softening, so its mass is zeroed there (G m y^3 would overflow for
-// G m > ~3e8 and inf * 0 poison the sum).
-template <bool SOFT, bool SELF, int IPL>
-__device__ inline void ens_mixed_term(const double4 p, const float2 g, int js, const int *li,
-                                      const double *xi, const double *yi, const double *zi,
-                                      const float *sif, float *fx, float *fy, float *fz)
+__device__ inline double wave_min(double v)
 {
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o));
+    return v;
+}
+
+__device__ inline double wave_max(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+    return v;
+}
+
+// Centre and half extent (rounded up to FP32) of [lo, hi].
+__device__ inline void box_axis(double lo, double hi, double &c, float &h)
+{
+    c = 0.5 * (lo + hi);
+    const double e = fmax(hi - c, c - lo);
+    float f = (float)e;
+    if ((double)f < e) f = nextafterf(f, 3.0e38f);
+    h = f;
+}
+
+// ---- order ------------------------------------------------------------------
+// Index along the Hilbert curve of a cell with 10-bit coordinates, after
+// J. Skilling, "Programming the Hilbert curve", AIP Conf. Proc. 707 (2004):
+// the coordinates are turned in place into the transposed index, whose bits
+// interleaved (x most significant) are the position along the curve.  Unlike
+// a Morton curve it has no jumps, so runs of keys are compact in space also
+// on a thin ring, where Morton runs straddle quadrant boundaries.
+__device__ inline unsigned hilbert10(unsigned x, unsigned y, unsigned z)
+{
+    unsigned X[3] = {x, y, z};
+    for (unsigned Q = 1u << 9; Q > 1; Q >>= 1) {
+        const unsigned P = Q - 1;
 #pragma unroll
-    for (int q = 0; q < IPL; ++q) {
-        const float gm = (SELF && js == li[q]) ? 0.f : g.x;
-        const float dx = (float)(p.x - xi[q]);
-        const float dy = (float)(p.y - yi[q]);
-        const float dz = (float)(p.z - zi[q]);
-        // 1e-20 in the chain keeps a self pair's y finite; it is below half
-        // an ulp of any r^2 above ~2e-13.
-        float r2;
-        if (SOFT) {
-            const float e = sif[q] + g.y;
-            r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, fmaf(e, e, 1e-20f))));
-        } else {
-            r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, 1e-20f)));
+        for (int i = 0; i < 3; ++i) {
+            if (X[i] & Q) {
+                X[0] ^= P;  // invert
+            } else {        // exchange
+                const unsigned t = (X[0] ^ X[i]) & P;
+                X[0] ^= t;
+                X[i] ^= t;
+            }
         }
-        const float y = __builtin_amdgcn_rsqf(r2);
-        const float s3 = (gm * y) * (y * y);
-        fx[q] = fmaf(s3, dx, fx[q]);
-        fy[q] = fmaf(s3, dy, fy[q]);
-        fz[q] = fmaf(s3, dz, fz[q]);
     }
+    X[1] ^= X[0];  // Gray encode
+    X[2] ^= X[1];
+    unsigned t = 0;
+    for (unsigned Q = 1u << 9; Q > 1; Q >>= 1)
+        if (X[2] & Q) t ^= Q - 1;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) X[i] ^= t;
+    return (expand10(X[0]) << 2) | (expand10(X[1]) << 1) | expand10(X[2]);
 }
 
-// Bodies j0 .. j1-1 (at most 64) of the staged tile (first local index jb)
-// into FP32 partials.
-template <bool SOFT, bool SELF, int IPL>
-__device__ inline void ens_mixed_run(const double4 *tp, const float2 *tf, int j0, int j1, int jb,
-                                     const int *li, const double *xi, const double *yi,
-                                     const double *zi, const float *sif, float *fx, float *fy,
-                                     float *fz)
+// One workgroup per system: bounding cube, largest radius, and the keys
+// (system << 30) | Hilbert index over the cube.
+__global__ __launch_bounds__(ENT) void k_ens_keys(const DBody *__restrict__ b,
+                                                  const EnsSys *__restrict__ sys,
+                                                  unsigned long long *__restrict__ key,
+                                                  int *__restrict__ idx,
+                                                  double *__restrict__ srmax)
 {
-    if (!SELF && j1 - j0 == 64) {
-#pragma unroll 4
-        for (int jj = j0; jj < j0 + 64; ++jj)
-            ens_mixed_term<SOFT, false, IPL>(tp[jj], tf[jj], jb + jj, li, xi, yi, zi, sif, fx, fy,
-                                             fz);
-    } else {
-        for (int jj = j0; jj < j1; ++jj)
-            ens_mixed_term<SOFT, SELF, IPL>(tp[jj], tf[jj], jb + jj, li, xi, yi, zi, sif, fx, fy,
-                                            fz);
+    __shared__ double sh[ENT];
+    const EnsSys S = sys[blockIdx.x];
+    double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300}, rm = 0.0;
+    for (int k = threadIdx.x; k < S.n; k += ENT) {
+        const DBody q = b[S.off + k];
+        const double p[3] = {q.x, q.y, q.z};
+        for (int c = 0; c < 3; ++c) {
+            mn[c] = fmin(mn[c], p[c]);
+            mx[c] = fmax(mx[c], p[c]);
+        }
+        rm = fmax(rm, q.radius);
+    }
+    double lo[3], hi[3];
+    for (int c = 0; c < 3; ++c) {
+        lo[c] = block_reduce(mn[c], sh, 0);
+        hi[c] = block_reduce(mx[c], sh, 1);
+    }
+    rm = block_reduce(rm, sh, 1);
+    if (threadIdx.x == 0) srmax[blockIdx.x] = rm;
+    const double w = fmax(hi[0] - lo[0], fmax(hi[1] - lo[1], hi[2] - lo[2]));
+    for (int k = threadIdx.x; k < S.n; k += ENT) {
+        const DBody q = b[S.off + k];
+        const double p[3] = {q.x, q.y, q.z};
+        unsigned g[3];
+        for (int c = 0; c < 3; ++c) {
+            const double base = 0.5 * (lo[c] + hi[c]) - 0.5 * w;
+            const double f = w > 0.0 ? (p[c] - base) / w : 0.0;
+            int v = (int)(f * 1023.999);
+            v = v < 0 ? 0 : (v > 1023 ? 1023 : v);
+            g[c] = (unsigned)v;
+        }
+        key[S.off + k] = ((unsigned long long)blockIdx.x << 30) | hilbert10(g[0], g[1], g[2]);
+        idx[S.off + k] = S.off + k;
     }
 }
 
-template <int MODE, int IPL>
-__global__ __launch_bounds__(ENT) void k_ens_pass(const EnsUnit *__restrict__ units, int nunits,
-                                                  unsigned *__restrict__ counter,
+// Slot -> body (-1 for padding) from the sorted body indices.
+__global__ __launch_bounds__(ENT) void k_ens_place(const EnsSys *__restrict__ sys,
+                                                   const int *__restrict__ tsys,
+                                                   const int *__restrict__ sidx, int nslots,
+                                                   int *__restrict__ pidx)
+{
+    const int s = blockIdx.x * ENT + threadIdx.x;
+    if (s >= nslots) return;
+    const EnsSys S = sys[tsys[s / TJ]];
+    const int l = s - S.poff;
+    pidx[s] = l < S.n ? sidx[S.off + l] : -1;
+}
+
+// ---- mixed form: records and tiles --------------------------------------------
+// One wavefront per j tile: the sorted FP64 records {x, y, z, G m} and s R of
+// its slots (padding: far away and massless), the tile header with its far
+// records, and the headers of its two wavefront groups.
+__global__ __launch_bounds__(ENT) void k_ens_tiles(const DBody *__restrict__ b,
+                                                   const int *__restrict__ pidx,
+                                                   const int *__restrict__ tsys,
+                                                   const EnsSys *__restrict__ sys, int ntiles,
+                                                   double G, double soft,
+                                                   double4 *__restrict__ jd,
+                                                   double *__restrict__ jsig,
+                                                   float4 *__restrict__ ju,
+                                                   TileHdr *__restrict__ thdr,
+                                                   TileHdr *__restrict__ ghdr)
+{
+    const int t = blockIdx.x * (ENT / WV) + (int)threadIdx.x / WV;
+    const int lane = threadIdx.x % WV;
+    if (t >= ntiles) return;
+    const EnsSys S = sys[tsys[t]];
+    const int cnt = min(TJ, S.n - (t * TJ - S.poff));
+    double4 p[2];
+    double mn[2][3], mx[2][3], gmin = 1e300, gmax = 0.0;
+    bool bad = false;
+    for (int r = 0; r < 2; ++r) {
+        const int s = t * TJ + r * WV + lane;
+        const int k = pidx[s];
+        double sig = 0.0;
+        p[r] = make_double4(1e30, 1e30, 1e30, 0.0);
+        for (int c = 0; c < 3; ++c) {
+            mn[r][c] = 1e300;
+            mx[r][c] = -1e300;
+        }
+        if (k >= 0) {
+            const DBody q = b[k];
+            p[r] = make_double4(q.x, q.y, q.z, G * q.mass);
+            sig = soft * q.radius;
+            const double v[3] = {q.x, q.y, q.z};
+            for (int c = 0; c < 3; ++c) mn[r][c] = mx[r][c] = v[c];
+            if (p[r].w > 0.0 && p[r].w < 1e300) {
+                gmin = fmin(gmin, p[r].w);
+                gmax = fmax(gmax, p[r].w);
+            } else if (!(p[r].w == 0.0)) {
+                bad = true;
+            }
+        }
+        jd[s] = p[r];
+        jsig[s] = sig;
+    }
+    TileHdr th, gh[2];
+    double A[3], Ag[2][3];
+    float H[3], Hg[2][3];
+    for (int c = 0; c < 3; ++c) {
+        const double l0 = wave_min(mn[0][c]), h0 = wave_max(mx[0][c]);
+        const double l1 = wave_min(mn[1][c]), h1 = wave_max(mx[1][c]);
+        box_axis(l0, h0, Ag[0][c], Hg[0][c]);
+        box_axis(l1, h1, Ag[1][c], Hg[1][c]);
+        box_axis(fmin(l0, l1), fmax(h0, h1), A[c], H[c]);
+    }
+    gmin = wave_min(gmin);
+    gmax = wave_max(gmax);
+    const bool anybad = __any(bad);
+    th.ax = A[0];
+    th.ay = A[1];
+    th.az = A[2];
+    th.hx = H[0];
+    th.hy = H[1];
+    th.hz = H[2];
+    th.count = cnt;
+    tile_s_range(gmin, gmax, anybad, th);
+    for (int r = 0; r < 2; ++r) {
+        gh[r].ax = Ag[r][0];
+        gh[r].ay = Ag[r][1];
+        gh[r].az = Ag[r][2];
+        gh[r].hx = Hg[r][0];
+        gh[r].hy = Hg[r][1];
+        gh[r].hz = Hg[r][2];
+        gh[r].count = max(0, min(WV, cnt - r * WV));
+        gh[r].smax = gh[r].smin = 0.f;
+    }
+    if (lane == 0) {
+        thdr[t] = th;
+        ghdr[2 * t] = gh[0];
+        ghdr[2 * t + 1] = gh[1];
+    }
+    for (int r = 0; r < 2; ++r) ju[t * TJ + r * WV + lane] = far_record(p[r], A);
+}
+
+// ---- mixed form: the pass -----------------------------------------------------
+// One workgroup per gravity unit (system, 256 slots); wavefront w takes the
+// unit's group w and walks every j tile of its system, near tiles
+// with FP64 terms read with wave-uniform addresses, far tiles staged in the
+// wavefront's own LDS slice.  Then the central term (physics.c:127-146), the
+// accelerations scattered to body order and the unit's CM_acc partial.
+// (At most 73 VGPRs: 7 wavefronts per SIMD.)
+__global__ __launch_bounds__(ENT, 7) void k_ens_tpass(const EnsUnit *__restrict__ units,
+                                                   const EnsSys *__restrict__ sys,
+                                                   const double *__restrict__ srmax,
+                                                   const double4 *__restrict__ jd,
+                                                   const double *__restrict__ jsig,
+                                                   const float4 *__restrict__ ju,
+                                                   const TileHdr *__restrict__ thdr,
+                                                   const TileHdr *__restrict__ ghdr,
+                                                   const int *__restrict__ pidx,
+                                                   const double *__restrict__ cen, double G,
+                                                   double soft, double theta,
+                                                   double *__restrict__ acc,
+                                                   double *__restrict__ cpart)
+{
+    __shared__ float4 lu[(ENT / WV) * TJ];
+    __shared__ double red[3][ENT];
+    const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x / WV);
+    const int lane = (int)threadIdx.x % WV;
+    float4 *tu = lu + w * TJ;
+    const int u = blockIdx.x;
+    {
+        const EnsUnit U = units[u];
+        const EnsSys S = sys[U.sys];
+        const int l0 = U.i0 + w * WV;  // the wavefront group's first local slot
+        const bool gvalid = l0 < S.n;
+        const int g0 = S.poff + l0;
+        const bool mine = l0 + lane < S.n;
+        double xi = 0.0, yi = 0.0, zi = 0.0, si = 0.0, gmi = 0.0;
+        double ax = 0.0, ay = 0.0, az = 0.0;
+        if (gvalid) {
+            const double4 p = jd[g0 + lane];
+            xi = p.x;
+            yi = p.y;
+            zi = p.z;
+            gmi = p.w;
+            si = jsig[g0 + lane];
+            const TileHdr ih = ghdr[g0 / WV];
+            const double eps = 2.0 * soft * srmax[U.sys];
+            const double eps2x = eps * eps * 67108864.0;  // 2^26
+            const double hi =
+                sqrt((double)ih.hx * ih.hx + (double)ih.hy * ih.hy + (double)ih.hz * ih.hz);
+            const int t0 = S.poff / TJ, t1 = t0 + (S.n + TJ - 1) / TJ;
+            for (int t = t0; t < t1; ++t) {
+                const TileHdr th = thdr[t];
+                const int jb = t * TJ;
+                const int cnt = th.count;
+                if (tile_near(ih, hi, th, theta, eps2x)) {
+                    if (jb < g0 + WV && jb + TJ > g0) {
+                        for (int jj = 0; jj < cnt; ++jj)
+                            near_term_self<1>(jd[jb + jj], jsig[jb + jj], jb + jj, g0 + lane, WV,
+                                              &xi, &yi, &zi, &si, &ax, &ay, &az);
+                    } else if (cnt == TJ) {
+#pragma unroll 4
+                        for (int jj = 0; jj < TJ; ++jj)
+                            near_term<1>(jd[jb + jj], jsig[jb + jj], &xi, &yi, &zi, &si, &ax, &ay,
+                                         &az);
+                    } else {
+                        for (int jj = 0; jj < cnt; ++jj)
+                            near_term<1>(jd[jb + jj], jsig[jb + jj], &xi, &yi, &zi, &si, &ax, &ay,
+                                         &az);
+                    }
+                } else {
+                    // The wavefront's own slice: ordered after its reads of
+                    // the previous far tile and before the reads below.
+                    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+                    for (int r = 0; r < TJ / WV; ++r) tu[r * WV + lane] = ju[jb + r * WV + lane];
+                    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    // w_i = A_J - x_i, rounded once from FP64.
+                    const float wx = (float)(th.ax - xi), wy = (float)(th.ay - yi),
+                                wz = (float)(th.az - zi);
+                    float fx = 0.f, fy = 0.f, fz = 0.f;
+#pragma unroll 8
+                    for (int jj = 0; jj < TJ; ++jj) far_term<1>(tu[jj], &wx, &wy, &wz, &fx, &fy, &fz);
+                    ax += (double)fx;
+                    ay += (double)fy;
+                    az += (double)fz;
+                }
+            }
+        }
+        // Central term (physics.c:127-146) and this unit's CM_acc partial.
+        const double *c = cen + 8 * (size_t)U.sys;
+        double cx = 0., cy = 0., cz = 0.;
+        if (mine) {
+            if (c[7] != 0.0) {
+                const double DX = xi - c[0], DY = yi - c[1], DZ = zi - c[2];
+                const double rk = sqrt(DX * DX + DY * DY + DZ * DZ);
+                const double r3 = rk * rk * rk;
+                const double mu = G * c[6];
+                ax += -mu * DX / r3;
+                ay += -mu * DY / r3;
+                az += -mu * DZ / r3;
+                cx = gmi * DX / r3;
+                cy = gmi * DY / r3;
+                cz = gmi * DZ / r3;
+            }
+            const int k = pidx[g0 + lane];
+            acc[3 * (size_t)k + 0] = ax;
+            acc[3 * (size_t)k + 1] = ay;
+            acc[3 * (size_t)k + 2] = az;
+        }
+        red[0][threadIdx.x] = cx;
+        red[1][threadIdx.x] = cy;
+        red[2][threadIdx.x] = cz;
+        __syncthreads();
+        for (int h = ENT / 2; h > 0; h >>= 1) {
+            if ((int)threadIdx.x < h)
+                for (int d = 0; d < 3; ++d) red[d][threadIdx.x] += red[d][threadIdx.x + h];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0)
+            for (int d = 0; d < 3; ++d) cpart[3 * (size_t)u + d] = red[d][0];
+    }
+}
+
+// ---- FP64-fast form (body order) ------------------------------------------------
+// One workgroup per gravity unit (system, 256 IPL bodies in body order), j
+// records staged in LDS ETJ at a time.
+template <int IPL>
+__global__ __launch_bounds__(ENT) void k_ens_pass(const EnsUnit *__restrict__ units,
                                                   const DBody *__restrict__ b,
                                                   const int *__restrict__ off,
                                                   const double *__restrict__ cen, double G,
-                                                  double soft, double dt,
-                                                  double *__restrict__ acc,
-                                                  double *__restrict__ cpart,
-                                                  int2 *__restrict__ cand, unsigned cand_cap,
-                                                  unsigned *__restrict__ cand_cnt)
+                                                  double soft, double *__restrict__ acc,
+                                                  double *__restrict__ cpart)
 {
     __shared__ double4 tp[ETJ];  // x, y, z, G m
     __shared__ double ts[ETJ];   // s R
-    __shared__ float2 tf[ETJ];   // G m, s R in FP32 (mixed form)
-    __shared__ double4 tv[ETJ];  // vx, vy, vz, R (collision units)
     __shared__ double red[3][ENT];
-    while (true) {
-        const int u = next_unit(counter);
-        if (u >= nunits) break;
+    const int u = blockIdx.x;
+    {
         const EnsUnit U = units[u];
         const int base = off[U.sys];
         const int ns = off[U.sys + 1] - base;
-        if (U.kind == 0) {
-            double xi[IPL], yi[IPL], zi[IPL], si[IPL], ax[IPL], ay[IPL], az[IPL];
-            int li[IPL];
+        double xi[IPL], yi[IPL], zi[IPL], si[IPL], ax[IPL], ay[IPL], az[IPL];
+        int li[IPL];
 #pragma unroll
-            for (int q = 0; q < IPL; ++q) {
-                li[q] = U.i0 + (int)threadIdx.x + q * ENT;
-                const int i = base + min(li[q], ns - 1);
-                xi[q] = b[i].x;
-                yi[q] = b[i].y;
-                zi[q] = b[i].z;
-                si[q] = soft * b[i].radius;
-                ax[q] = ay[q] = az[q] = 0.0;
-            }
-            for (int jb = 0; jb < ns; jb += ETJ) {
-                const int cnt = min(ETJ, ns - jb);
-                __syncthreads();
-                if ((int)threadIdx.x < cnt) {
-                    const DBody q = b[base + jb + threadIdx.x];
-                    tp[threadIdx.x] = make_double4(q.x, q.y, q.z, G * q.mass);
-                    ts[threadIdx.x] = soft * q.radius;
-                    tf[threadIdx.x] = make_float2((float)(G * q.mass), (float)(soft * q.radius));
-                }
-                __syncthreads();
-                // The tile holds some of the unit's own bodies: their self
-                // terms are zeroed (r^2 = 1e-200 without softening gives
-                // y^3 = 1e300, which G m > ~1.8e8 would overflow).
-                const bool diag = jb < U.i0 + ENT * IPL && jb + cnt > U.i0;
-                if (MODE == 0) {
-                    for (int jj = 0; jj < cnt; ++jj) {
-                        const double4 p = tp[jj];
-                        const double sj = ts[jj];
-#pragma unroll
-                        for (int q = 0; q < IPL; ++q) {
-                            const double dx = p.x - xi[q], dy = p.y - yi[q], dz = p.z - zi[q];
-                            const double e = si[q] + sj;
-                            const double r2 = fma(dx, dx, fma(dy, dy, fma(dz, dz, fma(e, e, 1e-200))));
-                            double y = __builtin_amdgcn_rsq(r2);
-                            y = y * fma(-0.5 * r2, y * y, 1.5);
-                            const double w = (diag && jb + jj == li[q]) ? 0.0 : p.w;
-                            const double s3 = w * y * (y * y);
-                            ax[q] = fma(s3, dx, ax[q]);
-                            ay[q] = fma(s3, dy, ay[q]);
-                            az[q] = fma(s3, dz, az[q]);
-                        }
-                    }
-                } else {
-                    float sif[IPL];
-#pragma unroll
-                    for (int q = 0; q < IPL; ++q) sif[q] = (float)si[q];
-                    for (int j0 = 0; j0 < cnt; j0 += 64) {
-                        const int j1 = min(cnt, j0 + 64);
-                        float fx[IPL], fy[IPL], fz[IPL];
-#pragma unroll
-                        for (int q = 0; q < IPL; ++q) fx[q] = fy[q] = fz[q] = 0.f;
-                        if (diag) {
-                            if (soft != 0.0)
-                                ens_mixed_run<true, true, IPL>(tp, tf, j0, j1, jb, li, xi, yi, zi,
-                                                               sif, fx, fy, fz);
-                            else
-                                ens_mixed_run<false, true, IPL>(tp, tf, j0, j1, jb, li, xi, yi, zi,
-                                                                sif, fx, fy, fz);
-                        } else if (soft != 0.0) {
-                            ens_mixed_run<true, false, IPL>(tp, tf, j0, j1, jb, li, xi, yi, zi,
-                                                            sif, fx, fy, fz);
-                        } else {
-                            ens_mixed_run<false, false, IPL>(tp, tf, j0, j1, jb, li, xi, yi, zi,
-                                                             sif, fx, fy, fz);
-                        }
-#pragma unroll
-                        for (int q = 0; q < IPL; ++q) {
-                            ax[q] += (double)fx[q];
-                            ay[q] += (double)fy[q];
-                            az[q] += (double)fz[q];
-                        }
-                    }
-                }
-            }
-            // Central term (physics.c:127-146) and this unit's CM_acc partial.
-            const double *c = cen + 8 * (size_t)U.sys;
-            const bool cp = c[7] != 0.0;
-            double cx = 0., cy = 0., cz = 0.;
-#pragma unroll
-            for (int q = 0; q < IPL; ++q) {
-                if (li[q] >= ns || li[q] >= U.i0 + U.cnt) continue;
-                const int i = base + li[q];
-                if (cp) {
-                    const double DX = xi[q] - c[0], DY = yi[q] - c[1], DZ = zi[q] - c[2];
-                    const double rk = sqrt(DX * DX + DY * DY + DZ * DZ);
-                    const double r3 = rk * rk * rk;
-                    const double mu = G * c[6];
-                    ax[q] += -mu * DX / r3;
-                    ay[q] += -mu * DY / r3;
-                    az[q] += -mu * DZ / r3;
-                    const double gm = G * b[i].mass;
-                    cx += gm * DX / r3;
-                    cy += gm * DY / r3;
-                    cz += gm * DZ / r3;
-                }
-                acc[3 * (size_t)i + 0] = ax[q];
-                acc[3 * (size_t)i + 1] = ay[q];
-                acc[3 * (size_t)i + 2] = az[q];
-            }
-            red[0][threadIdx.x] = cx;
-            red[1][threadIdx.x] = cy;
-            red[2][threadIdx.x] = cz;
+        for (int q = 0; q < IPL; ++q) {
+            li[q] = U.i0 + (int)threadIdx.x + q * ENT;
+            const int i = base + min(li[q], ns - 1);
+            xi[q] = b[i].x;
+            yi[q] = b[i].y;
+            zi[q] = b[i].z;
+            si[q] = soft * b[i].radius;
+            ax[q] = ay[q] = az[q] = 0.0;
+        }
+        for (int jb = 0; jb < ns; jb += ETJ) {
+            const int cnt = min(ETJ, ns - jb);
             __syncthreads();
-            for (int w = ENT / 2; w > 0; w >>= 1) {
-                if ((int)threadIdx.x < w)
-                    for (int d = 0; d < 3; ++d) red[d][threadIdx.x] += red[d][threadIdx.x + w];
-                __syncthreads();
+            if ((int)threadIdx.x < cnt) {
+                const DBody q = b[base + jb + threadIdx.x];
+                tp[threadIdx.x] = make_double4(q.x, q.y, q.z, G * q.mass);
+                ts[threadIdx.x] = soft * q.radius;
             }
-            if (threadIdx.x == 0)
-                for (int d = 0; d < 3; ++d) cpart[3 * (size_t)u + d] = red[d][0];
-        } else {
-            // Collision candidates among pairs (i, j > i), i in this unit.
-            const int li = U.i0 + (int)threadIdx.x;
-            const bool mine = (int)threadIdx.x < U.cnt && li < ns;
-            DBody a;
-            if (mine) a = b[base + li];
-            for (int jb = U.i0; jb < ns; jb += ETJ) {
-                const int cnt = min(ETJ, ns - jb);
-                __syncthreads();
-                if ((int)threadIdx.x < cnt) {
-                    const DBody q = b[base + jb + threadIdx.x];
-                    tp[threadIdx.x] = make_double4(q.x, q.y, q.z, 0.0);
-                    tv[threadIdx.x] = make_double4(q.vx, q.vy, q.vz, q.radius);
+            __syncthreads();
+            // The tile holds some of the unit's own bodies: their self terms
+            // are zeroed (r^2 = 1e-200 without softening gives y^3 = 1e300,
+            // which G m > ~1.8e8 would overflow).
+            const bool diag = jb < U.i0 + ENT * IPL && jb + cnt > U.i0;
+            for (int jj = 0; jj < cnt; ++jj) {
+                const double4 p = tp[jj];
+                const double sj = ts[jj];
+#pragma unroll
+                for (int q = 0; q < IPL; ++q) {
+                    const double dx = p.x - xi[q], dy = p.y - yi[q], dz = p.z - zi[q];
+                    const double e = si[q] + sj;
+                    const double r2 = fma(dx, dx, fma(dy, dy, fma(dz, dz, fma(e, e, 1e-200))));
+                    double y = __builtin_amdgcn_rsq(r2);
+                    y = y * fma(-0.5 * r2, y * y, 1.5);
+                    const double w = (diag && jb + jj == li[q]) ? 0.0 : p.w;
+                    const double s3 = w * y * (y * y);
+                    ax[q] = fma(s3, dx, ax[q]);
+                    ay[q] = fma(s3, dy, ay[q]);
+                    az[q] = fma(s3, dz, az[q]);
                 }
-                __syncthreads();
-                if (!mine) continue;
-                for (int jj = 0; jj < cnt; ++jj) {
-                    const int lj = jb + jj;
-                    if (lj <= li) continue;
-                    const double4 p = tp[jj], v = tv[jj];
-                    // closest_approach's predicate, collision.c:112-137 (this file is
-                    // compiled without FMA contraction).
-                    const double dx = a.x - p.x, dy = a.y - p.y, dz = a.z - p.z;
-                    const double dvx = a.vx - v.x, dvy = a.vy - v.y, dvz = a.vz - v.z;
-                    const double bb = dx * dvx + dy * dvy + dz * dvz;
-                    if (!(bb < 0.0)) continue;
-                    const double dr2 = dx * dx + dy * dy + dz * dz;
-                    const double dv2 = dvx * dvx + dvy * dvy + dvz * dvz;
-                    const double R = a.radius + v.w;
-                    const double disc = bb * bb + dv2 * (R * R - dr2);
-                    if (!(disc > 0.0)) continue;
-                    const double t = -(bb + sqrt(disc)) / dv2;
-                    if (t <= dt + fabs(dt) * 9.094947017729282e-13) {
-                        const unsigned slot = atomicAdd(cand_cnt, 1u);
-                        if (slot < cand_cap) cand[slot] = make_int2(base + li, base + lj);
-                    }
+            }
+        }
+        // Central term (physics.c:127-146) and this unit's CM_acc partial.
+        const double *c = cen + 8 * (size_t)U.sys;
+        const bool cp = c[7] != 0.0;
+        double cx = 0., cy = 0., cz = 0.;
+#pragma unroll
+        for (int q = 0; q < IPL; ++q) {
+            if (li[q] >= ns || li[q] >= U.i0 + U.cnt) continue;
+            const int i = base + li[q];
+            if (cp) {
+                const double DX = xi[q] - c[0], DY = yi[q] - c[1], DZ = zi[q] - c[2];
+                const double rk = sqrt(DX * DX + DY * DY + DZ * DZ);
+                const double r3 = rk * rk * rk;
+                const double mu = G * c[6];
+                ax[q] += -mu * DX / r3;
+                ay[q] += -mu * DY / r3;
+                az[q] += -mu * DZ / r3;
+                const double gm = G * b[i].mass;
+                cx += gm * DX / r3;
+                cy += gm * DY / r3;
+                cz += gm * DZ / r3;
+            }
+            acc[3 * (size_t)i + 0] = ax[q];
+            acc[3 * (size_t)i + 1] = ay[q];
+            acc[3 * (size_t)i + 2] = az[q];
+        }
+        red[0][threadIdx.x] = cx;
+        red[1][threadIdx.x] = cy;
+        red[2][threadIdx.x] = cz;
+        __syncthreads();
+        for (int h = ENT / 2; h > 0; h >>= 1) {
+            if ((int)threadIdx.x < h)
+                for (int d = 0; d < 3; ++d) red[d][threadIdx.x] += red[d][threadIdx.x + h];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0)
+            for (int d = 0; d < 3; ++d) cpart[3 * (size_t)u + d] = red[d][0];
+    }
+}
+
+// ---- collision pass -------------------------------------------------------------
+// One wavefront per tile: the sorted copy of its bodies and the position /
+// velocity boxes and largest radius of the tile and of its two groups.
+__global__ __launch_bounds__(ENT) void k_ens_ctiles(const DBody *__restrict__ b,
+                                                    const int *__restrict__ pidx,
+                                                    const int *__restrict__ tsys,
+                                                    const EnsSys *__restrict__ sys, int ntiles,
+                                                    DBody *__restrict__ jc,
+                                                    CollHdr *__restrict__ tch,
+                                                    CollHdr *__restrict__ gch)
+{
+    const int t = blockIdx.x * (ENT / WV) + (int)threadIdx.x / WV;
+    const int lane = threadIdx.x % WV;
+    if (t >= ntiles) return;
+    const EnsSys S = sys[tsys[t]];
+    const int cnt = min(TJ, S.n - (t * TJ - S.poff));
+    // Per slot r: 0-2 position, 3-5 velocity.
+    double mn[2][6], mx[2][6], rm[2];
+    for (int r = 0; r < 2; ++r) {
+        const int s = t * TJ + r * WV + lane;
+        const int k = pidx[s];
+        for (int c = 0; c < 6; ++c) {
+            mn[r][c] = 1e300;
+            mx[r][c] = -1e300;
+        }
+        rm[r] = 0.0;
+        if (k >= 0) {
+            const DBody q = b[k];
+            jc[s] = q;
+            const double v[6] = {q.x, q.y, q.z, q.vx, q.vy, q.vz};
+            for (int c = 0; c < 6; ++c) mn[r][c] = mx[r][c] = v[c];
+            rm[r] = q.radius;
+        }
+    }
+    double cg[2][6], ct[6];
+    float hg[2][6], ht[6];
+    for (int c = 0; c < 6; ++c) {
+        const double l0 = wave_min(mn[0][c]), h0 = wave_max(mx[0][c]);
+        const double l1 = wave_min(mn[1][c]), h1 = wave_max(mx[1][c]);
+        box_axis(l0, h0, cg[0][c], hg[0][c]);
+        box_axis(l1, h1, cg[1][c], hg[1][c]);
+        box_axis(fmin(l0, l1), fmax(h0, h1), ct[c], ht[c]);
+    }
+    const double r0 = wave_max(rm[0]), r1 = wave_max(rm[1]);
+    if (lane != 0) return;
+    auto up = [](double v) {
+        float f = (float)v;
+        if ((double)f < v) f = nextafterf(f, 3.0e38f);
+        return f;
+    };
+    auto put = [&](CollHdr &h, const double *cc, const float *hh, double r, int count) {
+        h.px = cc[0];
+        h.py = cc[1];
+        h.pz = cc[2];
+        h.vx = cc[3];
+        h.vy = cc[4];
+        h.vz = cc[5];
+        h.phx = hh[0];
+        h.phy = hh[1];
+        h.phz = hh[2];
+        h.vhx = hh[3];
+        h.vhy = hh[4];
+        h.vhz = hh[5];
+        h.rmax = up(r);
+        h.count = count;
+    };
+    CollHdr h;
+    put(h, ct, ht, fmax(r0, r1), cnt);
+    tch[t] = h;
+    put(h, cg[0], hg[0], r0, max(0, min(WV, cnt)));
+    gch[2 * t] = h;
+    put(h, cg[1], hg[1], r1, max(0, min(WV, cnt - WV)));
+    gch[2 * t + 1] = h;
+}
+
+// Whether some pair of group g and tile t may meet within |dt|: the pair's
+// closest_approach (below) lists it only if its first contact time
+// t_c = (|d| - R) / |dv| or less lies within dt (up to the predicate's
+// tolerance of 9.1e-13 |dt| and its rounding, a relative 4e-8 of |d|), so
+// |d| <= (R + |dv| |dt|) (1 + 1e-7).  The boxes bound |d| from below and
+// R, |dv| from above; the margin 1e-6 and the absolute term (a few ulps of
+// the coordinates, for the rounding of the box centres) keep the test
+// conservative.  NaN anywhere keeps the pair.
+__device__ inline bool may_meet(const CollHdr &g, const CollHdr &t, double adt)
+{
+    const double gx = fmax(0.0, fabs(t.px - g.px) - ((double)g.phx + t.phx));
+    const double gy = fmax(0.0, fabs(t.py - g.py) - ((double)g.phy + t.phy));
+    const double gz = fmax(0.0, fabs(t.pz - g.pz) - ((double)g.phz + t.phz));
+    const double ux = fabs(t.vx - g.vx) + ((double)g.vhx + t.vhx);
+    const double uy = fabs(t.vy - g.vy) + ((double)g.vhy + t.vhy);
+    const double uz = fabs(t.vz - g.vz) + ((double)g.vhz + t.vhz);
+    const double mag = fmax(fmax(fabs(g.px), fabs(t.px)), fmax(fmax(fabs(g.py), fabs(t.py)),
+                                                               fmax(fabs(g.pz), fabs(t.pz))));
+    const double reach = ((double)g.rmax + t.rmax + sqrt(ux * ux + uy * uy + uz * uz) * adt) *
+                             1.000001 +
+                         1e-14 * mag;
+    return !(gx * gx + gy * gy + gz * gz > reach * reach);
+}
+
+// One workgroup per collision unit (system, 256 slots); wavefront w takes
+// the unit's group w and tests its pairs (p, q > p in slot order) with every
+// tile from its own on that it may meet, staging each such tile in its own
+// LDS slice.  The predicate is
+// closest_approach's (collision.c:112-137; this file is compiled without FMA
+// contraction), symmetric in the pair bit for bit, with a tolerance of
+// 9.1e-13 |dt| on the contact time: candidates are a superset of the
+// reference's hits, decided exactly by the replay.
+__global__ __launch_bounds__(ENT) void k_ens_cpass(const EnsUnit *__restrict__ units,
+                                                   const EnsSys *__restrict__ sys,
+                                                   const DBody *__restrict__ jc,
+                                                   const int *__restrict__ pidx,
+                                                   const CollHdr *__restrict__ tch,
+                                                   const CollHdr *__restrict__ gch, double dt,
+                                                   int2 *__restrict__ cand, unsigned cand_cap,
+                                                   unsigned *__restrict__ cand_cnt)
+{
+    __shared__ double4 lp[(ENT / WV) * TJ];  // x, y, z, R
+    __shared__ double4 lv[(ENT / WV) * TJ];  // vx, vy, vz
+    __shared__ int lk[(ENT / WV) * TJ];      // body index
+    const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x / WV);
+    const int lane = (int)threadIdx.x % WV;
+    const double adt = fabs(dt);
+    double4 *tp = lp + w * TJ, *tv = lv + w * TJ;
+    int *tk = lk + w * TJ;
+    {
+        const EnsUnit U = units[blockIdx.x];
+        const EnsSys S = sys[U.sys];
+        const int l0 = U.i0 + w * WV;
+        if (l0 >= S.n) return;  // no synchronisation in this kernel
+        const int g0 = S.poff + l0;
+        const int p = g0 + lane;
+        const bool mine = l0 + lane < S.n;
+        DBody a = {};
+        int ka = -1;
+        if (mine) {
+            a = jc[p];
+            ka = pidx[p];
+        }
+        const CollHdr gh = gch[g0 / WV];
+        const int t1 = S.poff / TJ + (S.n + TJ - 1) / TJ;
+        for (int t = g0 / TJ; t < t1; ++t) {
+            const CollHdr th = tch[t];
+            if (!may_meet(gh, th, adt)) continue;
+            const int jb = t * TJ;
+            // The wavefront's own slice: ordered after its reads of the
+            // previous tile and before the reads below.
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int r = 0; r < TJ / WV; ++r) {
+                const int l = r * WV + lane;
+                if (l < th.count) {
+                    const DBody c = jc[jb + l];
+                    tp[l] = make_double4(c.x, c.y, c.z, c.radius);
+                    tv[l] = make_double4(c.vx, c.vy, c.vz, 0.0);
+                    tk[l] = pidx[jb + l];
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            for (int jj = 0; jj < th.count; ++jj) {
+                const int q = jb + jj;
+                if (!mine || q <= p) continue;
+                const double4 cp = tp[jj], cv = tv[jj];
+                const double dx = a.x - cp.x, dy = a.y - cp.y, dz = a.z - cp.z;
+                const double dvx = a.vx - cv.x, dvy = a.vy - cv.y, dvz = a.vz - cv.z;
+                const double bb = dx * dvx + dy * dvy + dz * dvz;
+                if (!(bb < 0.0)) continue;
+                const double dr2 = dx * dx + dy * dy + dz * dz;
+                const double dv2 = dvx * dvx + dvy * dvy + dvz * dvz;
+                const double R = a.radius + cp.w;
+                const double disc = bb * bb + dv2 * (R * R - dr2);
+                if (!(disc > 0.0)) continue;
+                const double tc = -(bb + sqrt(disc)) / dv2;
+                if (tc <= dt + adt * 9.094947017729282e-13) {
+                    const int kc = tk[jj];
+                    const unsigned slot = atomicAdd(cand_cnt, 1u);
+                    if (slot < cand_cap) cand[slot] = make_int2(min(ka, kc), max(ka, kc));
                 }
             }
         }
@@ -375,14 +775,18 @@ __global__ void k_ens_stage(int s, double dt, int ntot, DBody *__restrict__ y0,
     }
 }
 
+
 }  // namespace
 
 // ---- host side ------------------------------------------------------------
 
 struct EnsPlan {
     int nsys = 0, ntot = 0, ipl = 1;
-    int wg_per_cu = 8;  // persistent workgroups per CU
+    double theta = 0.35;
+    int nslots = 0, ntiles = 0, key_bits = 30;
     std::vector<int> off;
+    std::vector<EnsSys> sys;
+    std::vector<int> tsys;            // system of each tile
     std::vector<EnsUnit> grav, coll;  // unit lists (cost-ordered)
 };
 
@@ -398,7 +802,7 @@ static void make_units(EnsPlan &P, int ipl)
     }
     auto cost = [&](const EnsUnit &u) {
         const double ns = P.off[u.sys + 1] - P.off[u.sys];
-        return u.kind == 0 ? ns * u.cnt * 1.0 : 0.35 * u.cnt * (ns - u.i0 - 0.5 * u.cnt);
+        return u.kind == 0 ? ns * u.cnt * 1.0 : ns;
     };
     auto by_cost = [&](const EnsUnit &a, const EnsUnit &b) { return cost(a) > cost(b); };
     std::stable_sort(P.grav.begin(), P.grav.end(), by_cost);
@@ -421,67 +825,189 @@ static void unit_index(const EnsPlan &P, const std::vector<EnsUnit> &units, std:
     }
 }
 
+// Systems, slot layout and unit lists from the offsets; bodies per lane (the
+// FP64-fast form) and theta (the mixed form) from the tuning table.
+static void make_plan(ncg_ctx *ctx, EnsPlan &P, const int32_t *offsets, int32_t nsys,
+                      ncg_precision prec)
+{
+    P.nsys = nsys;
+    P.off.assign(offsets, offsets + nsys + 1);
+    P.ntot = P.off[nsys];
+    P.sys.resize(nsys);
+    P.tsys.clear();
+    int poff = 0;
+    for (int s = 0; s < nsys; ++s) {
+        const int n = P.off[s + 1] - P.off[s];
+        P.sys[s] = {P.off[s], poff, n, 0};
+        const int nt = (n + TJ - 1) / TJ;
+        for (int t = 0; t < nt; ++t) P.tsys.push_back(s);
+        poff += nt * TJ;
+    }
+    P.nslots = poff;
+    P.ntiles = poff / TJ;
+    P.key_bits = 30;
+    while ((1LL << (P.key_bits - 30)) < nsys) ++P.key_bits;
+    const TiledCfg tc = ncg_tiled_cfg(ctx, prec == NCG_MIXED ? 100 + NCG_MIXED : 100 + NCG_F64_FAST,
+                                      P.ntot / std::max(nsys, 1));
+    // The mixed form runs one body per lane.
+    P.ipl = prec == NCG_MIXED ? 1 : (tc.ipl == 2 ? 2 : 1);
+    P.theta = tc.theta;
+    make_units(P, P.ipl);
+}
+
 struct EnsDev {
     EnsUnit *grav = nullptr, *coll = nullptr;
     int *gfirst = nullptr, *glist = nullptr;
     int *off = nullptr;
+    EnsSys *sys = nullptr;
+    int *tsys = nullptr;
+    // Order (mixed form and collision passes).
+    unsigned long long *key = nullptr, *key2 = nullptr;
+    int *idx = nullptr, *idx2 = nullptr, *pidx = nullptr;
+    double *srmax = nullptr;
+    void *sort_tmp = nullptr;
+    size_t sort_bytes = 0;
+    // Mixed-form records and tiles.
+    double4 *jd = nullptr;
+    double *jsig = nullptr;
+    float4 *ju = nullptr;
+    TileHdr *thdr = nullptr, *ghdr = nullptr;
+    // Collision pass.
+    DBody *jc = nullptr;
+    CollHdr *tch = nullptr, *gch = nullptr;
 };
 
-template <int MODE>
-static void launch_pass(int ipl, int nblk, hipStream_t st, const EnsUnit *units, int nunits,
-                        unsigned *counter, const DBody *b, const int *off, const double *cen,
-                        double G, double soft, double dt, double *acc, double *cpart, int2 *cand,
-                        unsigned cap, unsigned *ccnt)
+// Device buffers and uploads of a plan: the unit lists always, the order when
+// `order`, the mixed form's tiles when `tiles`, the collision pass's when
+// `coll`.
+static ncg_status ens_setup(ncg_ctx *ctx, const EnsPlan &P, EnsDev &D, bool order, bool tiles,
+                            bool coll, hipStream_t st)
 {
-    if (ipl == 2)
-        hipLaunchKernelGGL((k_ens_pass<MODE, 2>), dim3(nblk), dim3(ENT), 0, st, units, nunits,
-                           counter, b, off, cen, G, soft, dt, acc, cpart, cand, cap, ccnt);
-    else
-        hipLaunchKernelGGL((k_ens_pass<MODE, 1>), dim3(nblk), dim3(ENT), 0, st, units, nunits,
-                           counter, b, off, cen, G, soft, dt, acc, cpart, cand, cap, ccnt);
-}
-
-// One pass over the unit list du[nunits] (gravity and/or collision units);
-// with gravity units, the per-system CM_acc from the index lists.
-static ncg_status ens_pass(ncg_ctx *ctx, const EnsPlan &P, const int *doff, const EnsUnit *du,
-                           int nunits, const int *dfirst, const int *dlist, const DBody *b,
-                           const double *cen, double G, double soft, ncg_precision prec,
-                           double dt, double *acc, double *cm, hipStream_t st)
-{
-    unsigned *counter = dbuf<unsigned>(ctx, "e_counter", 2);
-    double *cpart = dbuf<double>(ctx, "e_cpart", 3 * (size_t)std::max(nunits, 1));
-    const unsigned cap = ncg_replay_cap(ctx, ENS_CAND_CAP);
-    int2 *cand = dbuf<int2>(ctx, "e_cand", cap);
-    if (!counter || !cpart || !cand) return NCG_ERR_NOMEM;
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, counter, 2);
-    if (nunits == 0) return NCG_OK;
-    const int cus = ctx->num_cus > 0 ? ctx->num_cus : 256;
-    const int nblk = std::min(nunits, P.wg_per_cu * cus);
-    {
-        ProfScope ps(ctx, st, prec == NCG_MIXED ? "ens_mixed" : "ens_fast64");
-        if (prec == NCG_MIXED)
-            launch_pass<1>(P.ipl, nblk, st, du, nunits, counter, b, doff, cen, G, soft, dt, acc,
-                           cpart, cand, cap, counter + 1);
-        else
-            launch_pass<0>(P.ipl, nblk, st, du, nunits, counter, b, doff, cen, G, soft, dt, acc,
-                           cpart, cand, cap, counter + 1);
-        NCG_CHECK_LAUNCH();
+    const int nsys = P.nsys;
+    std::vector<int> gfirst, glist;
+    unit_index(P, P.grav, gfirst, glist);
+    D.grav = dbuf<EnsUnit>(ctx, "e_grav", P.grav.size());
+    D.coll = dbuf<EnsUnit>(ctx, "e_coll", P.coll.size());
+    D.gfirst = dbuf<int>(ctx, "e_gfirst", nsys + 1);
+    D.glist = dbuf<int>(ctx, "e_glist", std::max<size_t>(glist.size(), 1));
+    D.off = dbuf<int>(ctx, "e_off", nsys + 1);
+    D.sys = dbuf<EnsSys>(ctx, "e_sys", nsys);
+    D.tsys = dbuf<int>(ctx, "e_tsys", std::max(P.ntiles, 1));
+    if (!D.grav || !D.coll || !D.gfirst || !D.glist || !D.off || !D.sys || !D.tsys)
+        return NCG_ERR_NOMEM;
+    const size_t ns = std::max(P.nslots, 1), nt = std::max(P.ntiles, 1);
+    if (order) {
+        D.key = dbuf<unsigned long long>(ctx, "e_key", std::max(P.ntot, 1));
+        D.key2 = dbuf<unsigned long long>(ctx, "e_key2", std::max(P.ntot, 1));
+        D.idx = dbuf<int>(ctx, "e_idx", std::max(P.ntot, 1));
+        D.idx2 = dbuf<int>(ctx, "e_idx2", std::max(P.ntot, 1));
+        D.pidx = dbuf<int>(ctx, "e_pidx", ns);
+        D.srmax = dbuf<double>(ctx, "e_srmax", nsys);
+        NCG_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, D.sort_bytes,
+                                                   (unsigned long long *)nullptr,
+                                                   (unsigned long long *)nullptr, (int *)nullptr,
+                                                   (int *)nullptr, std::max(P.ntot, 1), 0,
+                                                   P.key_bits, st));
+        D.sort_tmp = dbuf<unsigned char>(ctx, "e_sort", D.sort_bytes);
+        if (!D.key || !D.key2 || !D.idx || !D.idx2 || !D.pidx || !D.srmax || !D.sort_tmp)
+            return NCG_ERR_NOMEM;
     }
-    if (dfirst) {
-        hipLaunchKernelGGL(k_ens_cm, dim3(cdiv(P.nsys, 128)), dim3(128), 0, st, dfirst, dlist,
-                           cpart, P.nsys, cm);
-        NCG_CHECK_LAUNCH();
+    if (tiles) {
+        D.jd = dbuf<double4>(ctx, "e_jd", ns);
+        D.jsig = dbuf<double>(ctx, "e_jsig", ns);
+        D.ju = dbuf<float4>(ctx, "e_ju", ns);
+        D.thdr = dbuf<TileHdr>(ctx, "e_thdr", nt);
+        D.ghdr = dbuf<TileHdr>(ctx, "e_ghdr", 2 * nt);
+        if (!D.jd || !D.jsig || !D.ju || !D.thdr || !D.ghdr) return NCG_ERR_NOMEM;
     }
+    if (coll) {
+        D.jc = dbuf<DBody>(ctx, "e_jc", ns);
+        D.tch = dbuf<CollHdr>(ctx, "e_tch", nt);
+        D.gch = dbuf<CollHdr>(ctx, "e_gch", 2 * nt);
+        if (!D.jc || !D.tch || !D.gch) return NCG_ERR_NOMEM;
+    }
+    NCG_HIP(hipMemcpyAsync(D.grav, P.grav.data(), sizeof(EnsUnit) * P.grav.size(), hipMemcpyHostToDevice, st));
+    NCG_HIP(hipMemcpyAsync(D.coll, P.coll.data(), sizeof(EnsUnit) * P.coll.size(), hipMemcpyHostToDevice, st));
+    NCG_HIP(hipMemcpyAsync(D.gfirst, gfirst.data(), sizeof(int) * gfirst.size(), hipMemcpyHostToDevice, st));
+    NCG_HIP(hipMemcpyAsync(D.glist, glist.data(), sizeof(int) * glist.size(), hipMemcpyHostToDevice, st));
+    NCG_HIP(hipMemcpyAsync(D.off, P.off.data(), sizeof(int) * (nsys + 1), hipMemcpyHostToDevice, st));
+    NCG_HIP(hipMemcpyAsync(D.sys, P.sys.data(), sizeof(EnsSys) * nsys, hipMemcpyHostToDevice, st));
+    if (P.ntiles)
+        NCG_HIP(hipMemcpyAsync(D.tsys, P.tsys.data(), sizeof(int) * P.ntiles, hipMemcpyHostToDevice, st));
+    // The host vectors are temporaries: the copies finish before they go.
+    NCG_HIP(hipStreamSynchronize(st));
     return NCG_OK;
 }
 
-// One evaluation of all systems.
+// The step's order from the positions in b: keys, one sort, slots.
+static ncg_status ens_order(ncg_ctx *ctx, const EnsPlan &P, const EnsDev &D, const DBody *b,
+                            hipStream_t st)
+{
+    if (P.ntot == 0) return NCG_OK;
+    size_t bytes = D.sort_bytes;
+    hipLaunchKernelGGL(k_ens_keys, dim3(P.nsys), dim3(ENT), 0, st, b, D.sys, D.key, D.idx, D.srmax);
+    NCG_CHECK_LAUNCH();
+    NCG_HIP(hipcub::DeviceRadixSort::SortPairs(D.sort_tmp, bytes, D.key, D.key2, D.idx, D.idx2,
+                                               P.ntot, 0, P.key_bits, st));
+    hipLaunchKernelGGL(k_ens_place, dim3(cdiv(P.nslots, ENT)), dim3(ENT), 0, st, D.sys, D.tsys,
+                       D.idx2, P.nslots, D.pidx);
+    NCG_CHECK_LAUNCH();
+    return NCG_OK;
+}
+
+// One evaluation of all systems (the mixed form with the order of the last
+// ens_order): accelerations and per-system CM_acc.
 static ncg_status ens_eval(ncg_ctx *ctx, const EnsPlan &P, const EnsDev &D, const DBody *b,
                            const double *cen, double G, double soft, ncg_precision prec,
                            double *acc, double *cm, hipStream_t st)
 {
-    return ens_pass(ctx, P, D.off, D.grav, (int)P.grav.size(), D.gfirst, D.glist, b, cen, G, soft,
-                    prec, 0.0, acc, cm, st);
+    const int nunits = (int)P.grav.size();
+    double *cpart = dbuf<double>(ctx, "e_cpart", 3 * (size_t)std::max(nunits, 1));
+    if (!cpart) return NCG_ERR_NOMEM;
+    if (nunits == 0) return NCG_OK;
+    if (prec == NCG_MIXED) {
+        hipLaunchKernelGGL(k_ens_tiles, dim3(cdiv(P.ntiles, ENT / WV)), dim3(ENT), 0, st, b, D.pidx,
+                           D.tsys, D.sys, P.ntiles, G, soft, D.jd, D.jsig, D.ju, D.thdr, D.ghdr);
+        NCG_CHECK_LAUNCH();
+        ProfScope ps(ctx, st, "ens_mixed");
+        hipLaunchKernelGGL(k_ens_tpass, dim3(nunits), dim3(ENT), 0, st, D.grav, D.sys, D.srmax,
+                           D.jd, D.jsig, D.ju, D.thdr, D.ghdr, D.pidx, cen, G, soft, P.theta, acc,
+                           cpart);
+        NCG_CHECK_LAUNCH();
+    } else {
+        ProfScope ps(ctx, st, "ens_fast64");
+        if (P.ipl == 2)
+            hipLaunchKernelGGL(k_ens_pass<2>, dim3(nunits), dim3(ENT), 0, st, D.grav, b, D.off,
+                               cen, G, soft, acc, cpart);
+        else
+            hipLaunchKernelGGL(k_ens_pass<1>, dim3(nunits), dim3(ENT), 0, st, D.grav, b, D.off,
+                               cen, G, soft, acc, cpart);
+        NCG_CHECK_LAUNCH();
+    }
+    hipLaunchKernelGGL(k_ens_cm, dim3(cdiv(P.nsys, 128)), dim3(128), 0, st, D.gfirst, D.glist,
+                       cpart, P.nsys, cm);
+    NCG_CHECK_LAUNCH();
+    return NCG_OK;
+}
+
+// The collision pass of all systems (order of the last ens_order): candidates
+// in cand, their count in counter[1] (counter[0] unused).
+static ncg_status ens_collide(ncg_ctx *ctx, const EnsPlan &P, const EnsDev &D, const DBody *b,
+                              double dt, int2 *cand, unsigned cap, unsigned *counter,
+                              hipStream_t st)
+{
+    const int nunits = (int)P.coll.size();
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, counter, 2);
+    NCG_CHECK_LAUNCH();
+    if (nunits == 0) return NCG_OK;
+    hipLaunchKernelGGL(k_ens_ctiles, dim3(cdiv(P.ntiles, ENT / WV)), dim3(ENT), 0, st, b, D.pidx,
+                       D.tsys, D.sys, P.ntiles, D.jc, D.tch, D.gch);
+    NCG_CHECK_LAUNCH();
+    ProfScope ps(ctx, st, "ens_collide");
+    hipLaunchKernelGGL(k_ens_cpass, dim3(nunits), dim3(ENT), 0, st, D.coll, D.sys, D.jc, D.pidx,
+                       D.tch, D.gch, dt, cand, cap, counter + 1);
+    NCG_CHECK_LAUNCH();
+    return NCG_OK;
 }
 
 extern "C" ncg_status ncg_ensemble_rk4_dev(ncg_ctx *ctx, ncg_body *d_bodies,
@@ -501,28 +1027,16 @@ extern "C" ncg_status ncg_ensemble_rk4_dev(ncg_ctx *ctx, ncg_body *d_bodies,
     hipStream_t st = pick_stream(ctx, stream);
     const bool use_graph = (flags & 1) != 0;
     const bool collide = (flags & 2) != 0;
+    const bool tiled = prec == NCG_MIXED;
     if (n_collisions) *n_collisions = 0;
 
     EnsPlan P;
-    P.nsys = nsys;
-    P.off.assign(offsets, offsets + nsys + 1);
-    P.ntot = P.off[nsys];
-    const TiledCfg tc = ncg_tiled_cfg(ctx, prec == NCG_MIXED ? 100 + NCG_MIXED : 100 + NCG_F64_FAST,
-                                      P.ntot / std::max(nsys, 1));
-    P.ipl = tc.ipl == 2 ? 2 : 1;
-    // For the ensemble forms the table's split column is workgroups per CU.
-    P.wg_per_cu = tc.nsplit > 0 ? std::min(tc.nsplit, 32) : 8;
-    make_units(P, P.ipl);
-    std::vector<int> gfirst, glist;
-    unit_index(P, P.grav, gfirst, glist);
+    make_plan(ctx, P, offsets, nsys, prec);
+    EnsDev D;
+    ncg_status rs = ens_setup(ctx, P, D, tiled || collide, tiled, collide, st);
+    if (rs != NCG_OK) return rs;
 
     const int ntot = P.ntot;
-    EnsDev D;
-    D.grav = dbuf<EnsUnit>(ctx, "e_grav", P.grav.size());
-    D.coll = dbuf<EnsUnit>(ctx, "e_coll", P.coll.size());
-    D.gfirst = dbuf<int>(ctx, "e_gfirst", nsys + 1);
-    D.glist = dbuf<int>(ctx, "e_glist", std::max<size_t>(glist.size(), 1));
-    D.off = dbuf<int>(ctx, "e_off", nsys + 1);
     DBody *Y = dbuf<DBody>(ctx, "e_Y", ntot);
     double *acc = dbuf<double>(ctx, "e_acc", 3 * (size_t)ntot);
     double *sx = dbuf<double>(ctx, "e_sx", 3 * (size_t)ntot);
@@ -532,18 +1046,14 @@ extern "C" ncg_status ncg_ensemble_rk4_dev(ncg_ctx *ctx, ncg_body *d_bodies,
     double *cm = dbuf<double>(ctx, "e_cm", 3 * (size_t)nsys);
     // Every buffer a captured step uses exists before the capture.
     unsigned *counter = dbuf<unsigned>(ctx, "e_counter", 2);
-    double *cpart = dbuf<double>(ctx, "e_cpart", 3 * std::max(P.grav.size(), P.coll.size()) + 3);
-    int2 *cand = dbuf<int2>(ctx, "e_cand", ENS_CAND_CAP);
+    double *cpart = dbuf<double>(ctx, "e_cpart", 3 * P.grav.size() + 3);
+    const unsigned cap = ncg_replay_cap(ctx, ENS_CAND_CAP);
+    int2 *cand = dbuf<int2>(ctx, "e_cand", cap);
     unsigned char *dirty = dbuf<unsigned char>(ctx, "e_dirty", (size_t)ntot + 4);
     unsigned *stat = dbuf<unsigned>(ctx, "e_stat", 2);  // collisions, error bits
-    if (!D.grav || !D.coll || !D.gfirst || !D.glist || !D.off || !Y || !acc || !sx || !sv ||
-        !Yc || !scen || !cm || !counter || !cpart || !cand || !dirty || !stat)
+    if (!Y || !acc || !sx || !sv || !Yc || !scen || !cm || !counter || !cpart || !cand || !dirty ||
+        !stat)
         return NCG_ERR_NOMEM;
-    NCG_HIP(hipMemcpyAsync(D.grav, P.grav.data(), sizeof(EnsUnit) * P.grav.size(), hipMemcpyHostToDevice, st));
-    NCG_HIP(hipMemcpyAsync(D.coll, P.coll.data(), sizeof(EnsUnit) * P.coll.size(), hipMemcpyHostToDevice, st));
-    NCG_HIP(hipMemcpyAsync(D.gfirst, gfirst.data(), sizeof(int) * gfirst.size(), hipMemcpyHostToDevice, st));
-    NCG_HIP(hipMemcpyAsync(D.glist, glist.data(), sizeof(int) * glist.size(), hipMemcpyHostToDevice, st));
-    NCG_HIP(hipMemcpyAsync(D.off, P.off.data(), sizeof(int) * (nsys + 1), hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, stat, 2);
     hipLaunchKernelGGL(k_zero_u32, dim3(cdiv(ntot / 4 + 1, 256)), dim3(256), 0, st,
                        reinterpret_cast<unsigned *>(dirty), ntot / 4 + 1);
@@ -586,12 +1096,10 @@ extern "C" ncg_status ncg_ensemble_rk4_dev(ncg_ctx *ctx, ncg_body *d_bodies,
         return NCG_OK;
     };
     auto device_pass = [&]() -> ncg_status {
-        ncg_status r = ens_pass(ctx, P, D.off, D.coll, (int)P.coll.size(), nullptr, nullptr, y0,
-                                d_centrals, G, soft, prec, dt, acc, cm, st);
+        ncg_status r = ens_collide(ctx, P, D, y0, dt, cand, cap, counter, st);
         if (r != NCG_OK) return r;
         return ncg_bf_replay_launch(ctx, y0, nullptr, nullptr, 0, D.off, nsys, cand, counter + 1,
-                                    nullptr, ncg_replay_cap(ctx, ENS_CAND_CAP), dt, collision_f,
-                                    dirty, stat, st);
+                                    nullptr, cap, dt, collision_f, dirty, stat, st);
     };
     auto checked_pass = [&]() -> ncg_status {
         unsigned hs[2];
@@ -667,8 +1175,15 @@ extern "C" ncg_status ncg_ensemble_rk4_dev(ncg_ctx *ctx, ncg_body *d_bodies,
         ncg_status r = snapshot();
         if (r != NCG_OK) return r;
     }
+    // The step's order (mixed form, collision pass) is sorted eagerly, outside
+    // the captured graph.
+    auto order = [&]() -> ncg_status {
+        return tiled || collide ? ens_order(ctx, P, D, y0, st) : NCG_OK;
+    };
     int32_t win0 = 0;
     for (int32_t step = 0; step < nsteps; ++step) {
+        ncg_status ro = order();
+        if (ro != NCG_OK) return ro;
         if (!use_graph || step == 0) {
             // Eager (the first step also allocates everything the graph uses).
             ncg_status r = step_body(false);
@@ -702,7 +1217,7 @@ extern "C" ncg_status ncg_ensemble_rk4_dev(ncg_ctx *ctx, ncg_body *d_bodies,
                                    hipMemcpyDeviceToDevice, st));
             if ((r = write_stat(snap_coll)) != NCG_OK) return r;
             for (int32_t q = win0; q <= step; ++q)
-                if ((r = step_body(true)) != NCG_OK) return r;
+                if ((r = order()) != NCG_OK || (r = step_body(true)) != NCG_OK) return r;
         }
         if ((r = snapshot()) != NCG_OK) return r;
         win0 = step + 1;
@@ -716,7 +1231,7 @@ extern "C" ncg_status ncg_ensemble_rk4_dev(ncg_ctx *ctx, ncg_body *d_bodies,
                         "collision replay capacity exceeded (flags %u: 1 = more than %u candidates "
                         "in one pass, 2 = more than %d in one system, 4 = more than %d changed "
                         "bodies in one system); results are not valid",
-                        hs[1], ncg_replay_cap(ctx, ENS_CAND_CAP), NCG_REPLAY_RCAP,
+                        hs[1], cap, NCG_REPLAY_RCAP,
                         NCG_REPLAY_DCAP);
     return NCG_OK;
 }
@@ -738,28 +1253,13 @@ extern "C" ncg_status ncg_ensemble_gravity_dev(ncg_ctx *ctx, const ncg_body *d_b
     for (int s = 0; s < nsys; ++s) NCG_REQUIRE(offsets[s + 1] >= offsets[s], "offsets not monotone");
     NCG_HIP(hipSetDevice(ctx->device));
     hipStream_t st = pick_stream(ctx, stream);
+    const bool tiled = prec == NCG_MIXED;
     EnsPlan P;
-    P.nsys = nsys;
-    P.off.assign(offsets, offsets + nsys + 1);
-    P.ntot = P.off[nsys];
-    const TiledCfg tc = ncg_tiled_cfg(ctx, prec == NCG_MIXED ? 100 + NCG_MIXED : 100 + NCG_F64_FAST,
-                                      P.ntot / std::max(nsys, 1));
-    P.ipl = tc.ipl == 2 ? 2 : 1;
-    // For the ensemble forms the table's split column is workgroups per CU.
-    P.wg_per_cu = tc.nsplit > 0 ? std::min(tc.nsplit, 32) : 8;
-    make_units(P, P.ipl);
-    std::vector<int> gfirst, glist;
-    unit_index(P, P.grav, gfirst, glist);
+    make_plan(ctx, P, offsets, nsys, prec);
     EnsDev D;
-    D.grav = dbuf<EnsUnit>(ctx, "e_grav", P.grav.size());
-    D.gfirst = dbuf<int>(ctx, "e_gfirst", nsys + 1);
-    D.glist = dbuf<int>(ctx, "e_glist", std::max<size_t>(glist.size(), 1));
-    D.off = dbuf<int>(ctx, "e_off", nsys + 1);
-    if (!D.grav || !D.gfirst || !D.glist || !D.off) return NCG_ERR_NOMEM;
-    NCG_HIP(hipMemcpyAsync(D.grav, P.grav.data(), sizeof(EnsUnit) * P.grav.size(), hipMemcpyHostToDevice, st));
-    NCG_HIP(hipMemcpyAsync(D.gfirst, gfirst.data(), sizeof(int) * gfirst.size(), hipMemcpyHostToDevice, st));
-    NCG_HIP(hipMemcpyAsync(D.glist, glist.data(), sizeof(int) * glist.size(), hipMemcpyHostToDevice, st));
-    NCG_HIP(hipMemcpyAsync(D.off, P.off.data(), sizeof(int) * (nsys + 1), hipMemcpyHostToDevice, st));
-    return ens_eval(ctx, P, D, reinterpret_cast<const DBody *>(d_bodies), d_centrals, grav_const,
-                    softening, prec, d_acc, d_cm_acc, st);
+    ncg_status r = ens_setup(ctx, P, D, tiled, tiled, false, st);
+    if (r != NCG_OK) return r;
+    const DBody *b = reinterpret_cast<const DBody *>(d_bodies);
+    if (tiled && (r = ens_order(ctx, P, D, b, st)) != NCG_OK) return r;
+    return ens_eval(ctx, P, D, b, d_centrals, grav_const, softening, prec, d_acc, d_cm_acc, st);
 }

@@ -48,24 +48,11 @@
 #include <mutex>
 
 #include "ncg_internal.hpp"
+#include "tiled_dev.hpp"
 
 namespace {
 
-constexpr int TJ = 128;     // bodies per j tile
 constexpr int NT = 256;     // threads per workgroup
-constexpr double TINY2 = 1e-200;  // floor of r^2 in the FP64 form (keeps a self pair's y finite)
-
-constexpr int WV = 64;      // lanes per wavefront
-
-struct TileHdr {
-    double ax, ay, az;  // anchor (box centre)
-    float hx, hy, hz;   // half extents, rounded up
-    int count;          // valid bodies in the tile
-    // Largest and smallest s = (G m)^(-1/2) over the tile's bodies with
-    // G m > 0 (j tiles only); smax is +inf when a mass is negative or not
-    // finite, which keeps the tile in the FP64 form.
-    float smax, smin;
-};
 
 struct Scal {           // per-evaluation scalars produced on the device
     double bmin[3], bmax[3];
@@ -73,34 +60,6 @@ struct Scal {           // per-evaluation scalars produced on the device
     int nvalid;
     int pad;
 };
-
-__device__ inline unsigned expand10(unsigned v)
-{
-    v &= 0x3ffu;
-    v = (v | (v << 16)) & 0x030000FFu;
-    v = (v | (v << 8)) & 0x0300F00Fu;
-    v = (v | (v << 4)) & 0x030C30C3u;
-    v = (v | (v << 2)) & 0x09249249u;
-    return v;
-}
-
-template <class T>
-__device__ inline T block_reduce(T v, T *sh, int op /*0 min 1 max 2 sum*/)
-{
-    const int t = threadIdx.x;
-    sh[t] = v;
-    __syncthreads();
-    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-        if (t < w) {
-            T a = sh[t], c = sh[t + w];
-            sh[t] = op == 0 ? (a < c ? a : c) : op == 1 ? (a > c ? a : c) : a + c;
-        }
-        __syncthreads();
-    }
-    T r = sh[0];
-    __syncthreads();
-    return r;
-}
 
 // ---- 1. bounding box / count ---------------------------------------------
 __global__ __launch_bounds__(NT) void k_bbox_partial(const DBody *__restrict__ b, int n,
@@ -212,11 +171,9 @@ __global__ __launch_bounds__(NT) void k_gather(const DBody *__restrict__ b, int 
 }
 
 // ---- 5. tile anchors and scaled FP32 offsets --------------------------------
-// Far-form record of body j in a tile with anchor A: {s (x - A), s G m > 0 ?
-// s : 0} with s = (G m)^(-1/2), the product formed in FP64 and rounded once.
-// A body without mass (and a padding slot) gets {1e30, 1e30, 1e30, 0}: its
-// scaled separation is 1e30 whatever the i-body, |t|^2 overflows to +inf,
-// 1/|t| = 0 and the term t / |t|^3 is exactly 0.
+// One workgroup per tile of tsize bodies: header (anchor, half extents, count,
+// range of s) and, for j tiles, the far records (far_record; slots past the
+// count get the massless record).
 __global__ __launch_bounds__(NT) void k_tiles(const double4 *__restrict__ jd,
                                               const Scal *__restrict__ sc, int tsize,
                                               TileHdr *__restrict__ hdr,
@@ -269,118 +226,18 @@ __global__ __launch_bounds__(NT) void k_tiles(const double4 *__restrict__ jd,
         th.hy = H[1];
         th.hz = H[2];
         th.count = cnt;
-        if (nbad != 0.0) {
-            th.smax = __int_as_float(0x7f800000);
-            th.smin = 0.f;
-        } else if (gh == 0.0) {  // no mass in the tile: every term is 0
-            th.smax = th.smin = 0.f;
-        } else {
-            // Rounded outwards, so the bounds hold for the FP32 values.
-            th.smax = (float)(1.0 / sqrt(gl)) * 1.0000002f;
-            th.smin = (float)(1.0 / sqrt(gh)) * 0.9999998f;
-        }
+        tile_s_range(gl, gh, nbad != 0.0, th);
         hdr[t] = th;
     }
     if (ju) {
         for (int q = threadIdx.x; q < tsize; q += NT) {
             const int s = b0 + q;
-            float4 r = make_float4(1e30f, 1e30f, 1e30f, 0.f);
-            if (q < cnt) {
-                const double4 p = jd[s];
-                if (p.w > 0.0) {
-                    // The term is G m_j d / r^3 only as far as s^-2 = G m_j:
-                    // of the FP32 values next to (G m)^-1/2, the one whose
-                    // s^-2 is nearest G m (within 2^-24 relative, as G m
-                    // rounded to FP32 would be), and the offsets scaled by
-                    // that same s.
-                    const double sd = 1.0 / sqrt(p.w);
-                    float sf = (float)sd, best = sf;
-                    double berr = fabs(1.0 / ((double)sf * (double)sf) - p.w);
-                    const float cand[2] = {nextafterf(sf, 0.f), nextafterf(sf, 3.0e38f)};
-                    for (int c = 0; c < 2; ++c) {
-                        const double e = fabs(1.0 / ((double)cand[c] * (double)cand[c]) - p.w);
-                        if (e < berr) {
-                            berr = e;
-                            best = cand[c];
-                        }
-                    }
-                    const double sb = (double)best;
-                    r = make_float4((float)(sb * (p.x - A[0])), (float)(sb * (p.y - A[1])),
-                                    (float)(sb * (p.z - A[2])), best);
-                }
-            }
-            ju[s] = r;
+            ju[s] = q < cnt ? far_record(jd[s], A) : make_float4(1e30f, 1e30f, 1e30f, 0.f);
         }
     }
 }
 
 // ---- 6. pair kernel -------------------------------------------------------
-// One interaction of the FP64 form: a += G m_j (x_j - x_i) / (r^2 + eps^2)^(3/2),
-// hardware 1/sqrt refined by one Newton step (v_rsq_f64 is good to ~2^-29;
-// one step brings it to FP64 rounding).  TINY2 enters the FMA chain with
-// eps^2, so that a body against itself (r = 0, no softening) has a finite
-// y = 1e100 (its mass is zeroed by near_term_self, so it adds exactly 0);
-// above r^2 ~ 1e-184 it is below half an ulp and leaves r^2 as it is (one
-// FMA instead of a multiply and a floor).
-template <int IPL>
-__device__ inline void near_term(const double4 p, double sj, const double *xi, const double *yi,
-                                 const double *zi, const double *si, double *ax, double *ay,
-                                 double *az)
-{
-#pragma unroll
-    for (int q = 0; q < IPL; ++q) {
-        const double dx = p.x - xi[q], dy = p.y - yi[q], dz = p.z - zi[q];
-        const double e = si[q] + sj;
-        const double r2 = fma(dx, dx, fma(dy, dy, fma(dz, dz, fma(e, e, TINY2))));
-        double y = __builtin_amdgcn_rsq(r2);
-        const double h = 0.5 * r2;
-        y = y * fma(-h, y * y, 1.5);
-        const double sc3 = p.w * y * (y * y);
-        ax[q] = fma(sc3, dx, ax[q]);
-        ay[q] = fma(sc3, dy, ay[q]);
-        az[q] = fma(sc3, dz, az[q]);
-    }
-}
-
-// near_term for a j slot that may be one of the lane's own bodies (i slot q
-// is is0 + q stride): that pair's mass is zeroed.  Without softening its
-// r^2 is TINY2 and y^3 = 1e300, so G m y^3 would overflow to inf for
-// G m > ~1.8e8 and inf * 0 poison the sum; with the mass zeroed the term is
-// 0 * 1e300 * 0 = 0 whatever G m is.  Used only where the j tile overlaps
-// the lane's i slots.
-template <int IPL>
-__device__ inline void near_term_self(const double4 p, double sj, int js, int is0, int stride,
-                                      const double *xi, const double *yi, const double *zi,
-                                      const double *si, double *ax, double *ay, double *az)
-{
-#pragma unroll
-    for (int q = 0; q < IPL; ++q) {
-        const double4 pq = make_double4(p.x, p.y, p.z, js == is0 + q * stride ? 0.0 : p.w);
-        near_term<1>(pq, sj, xi + q, yi + q, zi + q, si + q, ax + q, ay + q, az + q);
-    }
-}
-
-// One interaction of the FP32 far form from the scaled record u = {s u_j, s}
-// (k_tiles) and the lane's w_i = C - u_i: t = s w_i + s u_j, y = 1/|t|, and
-// the term G m_j d / r^3 = y^3 t.
-template <int IPL>
-__device__ inline void far_term(const float4 u, const float *wx, const float *wy, const float *wz,
-                                float *fx, float *fy, float *fz)
-{
-#pragma unroll
-    for (int q = 0; q < IPL; ++q) {
-        const float tx = fmaf(u.w, wx[q], u.x);
-        const float ty = fmaf(u.w, wy[q], u.y);
-        const float tz = fmaf(u.w, wz[q], u.z);
-        const float r2 = fmaf(tz, tz, fmaf(ty, ty, tx * tx));
-        const float y = __builtin_amdgcn_rsqf(r2);
-        const float y3 = (y * y) * y;
-        fx[q] = fmaf(y3, tx, fx[q]);
-        fy[q] = fmaf(y3, ty, fy[q]);
-        fz[q] = fmaf(y3, tz, fz[q]);
-    }
-}
-
 // MODE 0: FP64-fast on every tile; MODE 1: mixed (near FP64, far FP32).
 // SRC 0: the form picked per wavefront; near tiles read with wave-uniform
 //        addresses (scalar loads), far tiles staged by the wavefront in its
@@ -442,21 +299,8 @@ __global__ __launch_bounds__(NT) void k_pairs(const double4 *__restrict__ jd,
         const int jb = t * TJ;
         const int cnt = th.count;
         bool near = true;
-        const double Cx = th.ax - ih.ax, Cy = th.ay - ih.ay, Cz = th.az - ih.az;
         if (MODE == 1) {
-            const double gx = fmax(0.0, fabs(Cx) - ((double)ih.hx + th.hx));
-            const double gy = fmax(0.0, fabs(Cy) - ((double)ih.hy + th.hy));
-            const double gz = fmax(0.0, fabs(Cz) - ((double)ih.hz + th.hz));
-            const double dmin2 = gx * gx + gy * gy + gz * gz;
-            const double hj = sqrt((double)th.hx * th.hx + (double)th.hy * th.hy +
-                                   (double)th.hz * th.hz);
-            const double size = sqrt(Cx * Cx + Cy * Cy + Cz * Cz) + hi + hj;
-            const double smax = th.smax, smin = th.smin;
-            // Scaled distances |t|^2 = s^2 r^2 of every pair in [1e-24, 1e24].
-            const bool range = size * size * (smax * smax) <= 1e24 &&
-                               (smin == 0.0 || dmin2 * (smin * smin) >= 1e-24);
-            near = (dmin2 < theta * theta * size * size) || (dmin2 <= eps2x) || !range ||
-                   !gvalid;
+            near = tile_near(ih, hi, th, theta, eps2x) || !gvalid;
             if (LDS) near = __syncthreads_or(near && gvalid);
         }
         if (near) {

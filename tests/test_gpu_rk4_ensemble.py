@@ -101,14 +101,14 @@ def test_ensemble_gravity_matches_per_system_oracle(ctx, prec):
             # oracle's own running sum, which adds each small mutual term to
             # an accumulator of the central term's size.
             mut, _ = orc.vector_field_brute(sysb, None, None, 1.0, 0.01)
-            tol = 1e-12 if prec == "fast64" else 5e-6
+            tol = 1e-12 if prec == "fast64" else 1e-6
             bound = (tol * np.linalg.norm(mut, axis=1)
                      + (4e-16 * math.sqrt(len(sysb)) + 1e-15) * np.linalg.norm(ref, axis=1))
             err = np.linalg.norm(got - ref, axis=1)
             assert np.all(err <= bound), (err / bound).max()
 
 
-@pytest.mark.parametrize("prec,tol", [("fast64", 1e-12), ("mixed", 5e-6)])
+@pytest.mark.parametrize("prec,tol", [("fast64", 1e-12), ("mixed", 1e-6)])
 def test_ensemble_no_softening_with_large_gm(ctx, prec, tol):
     # Without softening a body's pair with itself has r = 0; with G m up to
     # 1e3 times the FP32 form's overflow point it must still add nothing.
@@ -282,3 +282,60 @@ def test_ensemble_replay_overflow_replayed_on_host(ctx):
     assert res[0][0] == res[1][0] > 50
     assert np.array_equal(res[0][1].view(np.uint64), res[1][1].view(np.uint64))
     assert np.array_equal(res[0][2].view(np.uint64), res[1][2].view(np.uint64))
+
+
+def _collision_run(ctx, systems, dt, steps, f=1.5):
+    torch = torch_dev()
+    off = np.concatenate([[0], np.cumsum([len(x) for x in systems])]).astype(np.int32)
+    cen = np.zeros((len(systems), 8))
+    cen[:, 6] = 1.0
+    cen[:, 7] = 1.0
+    bt = torch.from_numpy(np.concatenate(systems)).cuda()
+    ct = torch.from_numpy(cen).cuda()
+    torch.cuda.synchronize()
+    nc = ctx.ensemble_rk4_dev(bt, off, ct, steps, dt, 1.0, 0.0, "fast64", graph=True,
+                              collisions=True, collision_f=f)
+    ctx.synchronize()
+    out = bt.cpu().numpy()
+    total = 0
+    for s, sysb in enumerate(systems):
+        ref, rc = sysb.copy(), np.array([0, 0, 0, 0, 0, 0, 1.0])
+        for _ in range(steps):
+            ref, _, pairs = orc.brute_force(ref, dt, 1, f, 0)
+            total += len(pairs)
+            ref, rc = orc.rk4_step(ref, rc, 1.0, 0.0, dt)
+        got = out[off[s]:off[s + 1]]
+        assert np.abs(got[:, :6] - ref[:, :6]).max() < 1e-9, s
+    return nc, total
+
+
+def test_ensemble_collisions_on_spread_disks_match_oracle(ctx):
+    """Disks of 300-4000 bodies (the benchmark's kind, with R = 1e-3 for 42
+    collisions): the collision pass tests only the tiles a group's position
+    and velocity boxes can reach within dt, and the collisions equal the
+    oracle's all-pairs brute_force over 3 steps."""
+    rng = np.random.default_rng(21)
+    sizes = np.rint(np.exp(rng.uniform(np.log(300), np.log(4000), 16))).astype(int)
+    systems = [ic.disk(int(nn), seed=500 + s, mass=1e-8, radius=1e-3) for s, nn in enumerate(sizes)]
+    nc, total = _collision_run(ctx, systems, 2 * math.pi / 256, 3)
+    assert nc == total and total > 20, (nc, total)
+
+
+def test_ensemble_collisions_of_clumps_crossing_within_dt(ctx):
+    """Two clumps 0.5 apart moving through each other at relative speed 20:
+    their tiles are far apart in position but meet within dt = 0.04, which
+    only the velocity boxes of the collision pass's reach test can tell."""
+    systems = []
+    for s in range(3):
+        r = np.random.default_rng(600 + s)
+        n = 300
+        b = np.zeros((2 * n, 8))
+        b[:n, 0:3] = r.uniform(-0.05, 0.05, (n, 3)) + np.array([0.75, 0.0, 0.0])
+        b[n:, 0:3] = r.uniform(-0.05, 0.05, (n, 3)) + np.array([1.25, 0.0, 0.0])
+        b[:n, 3:6] = r.normal(0, 0.01, (n, 3)) + np.array([10.0, 1.0, 0.0])
+        b[n:, 3:6] = r.normal(0, 0.01, (n, 3)) + np.array([-10.0, 1.0, 0.0])
+        b[:, 6] = 1e-12
+        b[:, 7] = 6e-4
+        systems.append(b)
+    nc, total = _collision_run(ctx, systems, 0.04, 1)
+    assert nc == total and total > 100, (nc, total)
