@@ -638,7 +638,7 @@ def precision_run(ctx, torch, b, soft, dt, steps=20):
 ADJ_SOFT = 1.0  # softening length R_i + R_j: smooth field without collisions
 
 
-def adjoint_run(ctx, torch, n=16384, steps=200, K=20):
+def adjoint_run(ctx, torch, n=16384, steps=200, K=20, fd_dirs=4):
     """Collision-free disk of 16384 bodies (m = 1e-8, R = 2e-4, softening
     length R_i + R_j), 200 FP64 RK4 steps dt = 2 pi/256; loss = mean squared
     distance of the final positions to a seeded target (sigma 1e-3); adjoint
@@ -685,12 +685,38 @@ def adjoint_run(ctx, torch, n=16384, steps=200, K=20):
     t_all, _ = timed(lambda: ctx.rk4_adjoint_dev(bt, c0, gfin, g0, gm, gc, steps, dt, K, 1.0,
                                                  ADJ_SOFT))
     same = bool(torch.equal(g_ck, g0))
+    # Directional derivatives of the loss along seeded directions in
+    # (x0, v0, m) against Richardson-extrapolated central differences,
+    # (4 D(h) - D(2h)) / 3 with h = 2e-8 (the recipe of the 64-direction
+    # acceptance test in tests/test_gpu_adjoint.py).
+    def loss(bb0):
+        bb, cc = bb0.clone(), c0.clone()
+        torch.cuda.synchronize()
+        ctx.rk4_steps_dev(bb, cc, steps, dt, 1.0, ADJ_SOFT, "fast64", graph=True)
+        ctx.synchronize()
+        return float(((bb[:, 0:3] - target) ** 2).sum() / n)
+
+    h, fd_err = 2e-8, []
+    for q in range(fd_dirs):
+        r = np.random.default_rng(100 + q)
+        d = np.zeros((n, 8))
+        d[:, 0:6] = r.normal(size=(n, 6))
+        d[:, 6] = r.normal(size=n) * b[:, 6]
+        d = torch.from_numpy(d).cuda()
+        ad = float((g0 * d[:, 0:6]).sum() + (gm * d[:, 6]).sum())
+        d1 = (loss(bt + h * d) - loss(bt - h * d)) / (2 * h)
+        d2 = (loss(bt + 2 * h * d) - loss(bt - 2 * h * d)) / (4 * h)
+        fd_err.append(abs((4.0 * d1 - d2) / 3.0 - ad) / abs(ad))
     return {"n": n, "steps": steps, "checkpoint_every": K, "forward_s": t_fwd,
             "forward_with_checkpoints_s": t_fck, "backward_s": t_bwd,
             "adjoint_over_forward": t_bwd / t_fwd,
             "forward_and_backward_over_forward": (t_fck + t_bwd) / t_fwd,
             "all_in_one_s": t_all, "all_in_one_over_forward": t_all / t_fwd,
             "gradients_identical_both_ways": same,
+            "fd_check": {"directions": fd_dirs, "h": h,
+                         "max_rel_err_richardson": max(fd_err) if fd_err else None,
+                         "reference": "Richardson-extrapolated central differences of the "
+                                      "loss along seeded directions in (x0, v0, m)"},
             "note": "backward = segment recomputation from the checkpoints + 4 VJPs per step"}
 
 

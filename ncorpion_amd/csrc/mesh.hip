@@ -345,8 +345,10 @@ __global__ __launch_bounds__(MT) void k_mcount(const unsigned *__restrict__ skey
 // (elastic or inelastic, collision.c:351-366 with physics.c:545-605) and no
 // body larger than the mesh size.  Then a body changes only during its own
 // visit or as the partner p < k of a later visit, so every body still ahead
-// of the current visit holds its start-of-pass state.  The replay walks the
-// visits with something to decide, in ascending id:
+// of the current visit holds its start-of-pass state, and since a bounce
+// removes no body, every binned body (only existing ones are binned) exists
+// throughout the pass: the replay reads no existence flags.  The replay walks
+// the visits with something to decide, in ascending id:
 //   - events of visit k: candidate pairs (k, p) of the pass and pending
 //     re-tests, in traversal order (cell l = 26..0, ascending p), each tested
 //     exactly with the current states; the first hit is resolved;
@@ -357,11 +359,15 @@ __global__ __launch_bounds__(MT) void k_mcount(const unsigned *__restrict__ skey
 //     hit become pending re-tests of visit q.  A visitor's own state is its
 //     start-of-pass state (it is still ahead), and a re-test is exact when it
 //     runs, so the pending list only has to be a superset.
-// One block of MR_T threads.  stat[0] += collisions; stat[1] |= 1 (more candidates than
-// cap), 2 (more than MR_PCAP pending re-tests), 4 (a body larger than the
-// mesh size).
+// One block of MR_T threads; a scan round takes MR_R entries per thread, whose
+// loads are issued together (a scan is bound by the latency of its rounds).
+// stat[0] += collisions; stat[1] |= 1 (more candidates than cap), 2 (more
+// than MR_PCAP pending re-tests), 4 (a body larger than the mesh size).
 constexpr int MR_PCAP = 8192;
-constexpr int MR_T = 512;  // threads of k_mesh_replay: scans take MR_T entries at a time
+constexpr int MR_T = 512;  // threads of k_mesh_replay
+constexpr int MR_R = 2;    // entries per thread in a scan round
+constexpr int MR_W = MR_R * MR_T / 64;  // hit masks of a round
+static_assert(MR_R == 2, "the round votes below test h[0] and h[1]");
 
 __global__ void k_lf_fill_u32(unsigned *__restrict__ p, unsigned n)
 {
@@ -387,6 +393,17 @@ __global__ __launch_bounds__(MT) void k_mcand_keys(const int4 *__restrict__ cand
     vals[i] = i;
 }
 
+// The candidates in visiting order, gathered once for the replay's serial walk.
+__global__ __launch_bounds__(MT) void k_mcand_gather(const int4 *__restrict__ cand,
+                                                     const unsigned *__restrict__ order,
+                                                     const unsigned long long *__restrict__ ccnt,
+                                                     unsigned cap, int4 *__restrict__ sorted)
+{
+    const unsigned i = blockIdx.x * MT + threadIdx.x;
+    const unsigned long long m = *ccnt;
+    if (i < cap && i < m) sorted[i] = cand[order[i]];
+}
+
 __device__ inline bool mr_before(int4 a, int4 b)  // (q, 26 - l, d) order
 {
     if (a.x != b.x) return a.x < b.x;
@@ -395,23 +412,25 @@ __device__ inline bool mr_before(int4 a, int4 b)  // (q, 26 - l, d) order
 }
 
 __global__ __launch_bounds__(MR_T) void k_mesh_replay(
-    DBody *__restrict__ b, const int32_t *__restrict__ ex, int32_t *__restrict__ dcf, int one_coll,
+    DBody *__restrict__ b, int32_t *__restrict__ dcf, int one_coll,
     double f, double dt, const unsigned *__restrict__ key, const int *__restrict__ sid,
     const unsigned *__restrict__ cs, const unsigned *__restrict__ ce,
     const double4 *__restrict__ pr, const double4 *__restrict__ vm, int d,
-    const int4 *__restrict__ cand, const unsigned *__restrict__ order,
-    const unsigned long long *__restrict__ ccnt, unsigned cap, const int *__restrict__ cnt,
-    unsigned *__restrict__ stat)
+    const int4 *__restrict__ csorted, const unsigned long long *__restrict__ ccnt, unsigned cap,
+    const int *__restrict__ cnt, unsigned *__restrict__ stat)
 {
     __shared__ int4 P[MR_PCAP];
     __shared__ int s_np, s_pi, s_ci, s_k, s_hl, s_hp, s_hit;
-    __shared__ int s_q[MR_T], s_l[MR_T];
-    __shared__ unsigned long long s_m[MR_T / 64];  // per-wavefront hit masks of a scan round
+    // Entry r MR_T + lane of a scan round: its visitor and stencil cell; hit
+    // mask w covers entries 64 w .. 64 w + 63 of the round.
+    __shared__ int s_q[MR_R * MR_T], s_l[MR_R * MR_T];
+    __shared__ unsigned long long s_m[MR_W];
     // 27 position ranges [ra[i], ra[i] + rn[i]) walked as one index space,
     // pre[i] = positions before range i, rl[i] = its stencil cell: set 0 for
     // the later-visitor scans, set 1 for the rest of a visit (which calls
     // those scans).
-    __shared__ unsigned s_ra[2][27], s_rn[2][27], s_pre[2][28];
+    __shared__ unsigned s_ra[2][27], s_rn[2][27];
+    __shared__ __align__(16) unsigned s_pre[2][28];
     __shared__ int s_rl[2][27];
     __shared__ double s_ap[6], s_tc;
     const int lane = threadIdx.x, wv = lane >> 6;
@@ -432,29 +451,51 @@ __global__ __launch_bounds__(MR_T) void k_mesh_replay(
     }
     __syncthreads();
     unsigned hits = 0;
-    auto cand_at = [&](int i) { return cand[order[i]]; };
-    // Prefix of the 27 ranges of set r written by lanes 0..26 (every lane calls it).
+    auto cand_at = [&](int i) { return csorted[i]; };
+    // Prefix of the 27 ranges of set r (every lane calls it): a scan across
+    // the first wavefront's lanes.
     auto flatten = [&](int r) {
         __syncthreads();
-        if (lane == 0) {
-            unsigned t = 0;
-            for (int i = 0; i < 27; ++i) {
-                s_pre[r][i] = t;
-                t += s_rn[r][i];
+        if (lane < 64) {
+            const unsigned v = lane < 27 ? s_rn[r][lane] : 0u;
+            unsigned incl = v;
+            for (int o = 1; o < 32; o <<= 1) {
+                const unsigned t = __shfl_up(incl, o);
+                if (lane >= o) incl += t;
             }
-            s_pre[r][27] = t;
+            if (lane < 27) s_pre[r][lane] = incl - v;
+            if (lane == 26) s_pre[r][27] = incl;
         }
         __syncthreads();
     };
-    // Range of flat index x in set r.
+    // Range of flat index x < s_pre[r][27] in set r: the last i with
+    // s_pre[r][i] <= x (a non-empty range), counted over all 28 prefixes.
     auto range_of = [&](int r, unsigned x) {
-        int lo = 0, hi = 27;
-        while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (s_pre[r][mid] <= x) lo = mid;
-            else hi = mid;
+        const uint4 *pp = reinterpret_cast<const uint4 *>(s_pre[r]);
+        int c = -1;
+#pragma unroll
+        for (int j = 0; j < 7; ++j) {
+            const uint4 v = pp[j];
+            c += (v.x <= x) + (v.y <= x) + (v.z <= x) + (v.w <= x);
         }
-        return lo;
+        return c;
+    };
+    // Ranges of set 1 (the rest of visit kk from cell l0 down), loaded by
+    // the second wavefront: called before an eager scan, whose set-0 loads
+    // then run beside these.
+    auto rest_ranges = [&](unsigned kk, int l0) {
+        if (lane >= 64 && lane < 64 + 27) {
+            const int l = l0 - (lane - 64);  // range `lane - 64` is cell l0 - (lane - 64)
+            unsigned lo = 0, e = 0;
+            if (l >= 0) {
+                const unsigned c = (unsigned)((long)kk + stencil_offset(l, d));
+                lo = cs[c];
+                e = ce[c];
+            }
+            s_ra[1][lane - 64] = lo;
+            s_rn[1][lane - 64] = e - lo;
+            s_rl[1][lane - 64] = l;
+        }
     };
     // Later visitors of body dd (just changed during visit kc) that may hit
     // it: the members above kc of its 27 cells, MR_T at a time across cells.
@@ -475,22 +516,32 @@ __global__ __launch_bounds__(MR_T) void k_mesh_replay(
         }
         flatten(0);
         const unsigned total = s_pre[0][27];
-        for (unsigned base = 0; base < total; base += MR_T) {
-            const unsigned x = base + lane;
-            bool h = false;
-            if (x < total) {
-                const int i = range_of(0, x);
-                const unsigned u = s_ra[0][i] + (x - s_pre[0][i]);
-                const int q = sid[u];
-                s_q[lane] = q;
-                s_l[lane] = s_rl[0][i];
-                if (q > kc && (!ex || ex[q])) h = approach_hit(pd, vd, pr[u], vm[u], dt);
+        for (unsigned base = 0; base < total; base += MR_R * MR_T) {
+            bool h[MR_R];
+#pragma unroll
+            for (int r = 0; r < MR_R; ++r) {
+                const unsigned x = base + r * MR_T + lane;
+                h[r] = false;
+                if (x < total) {
+                    const int i = range_of(0, x);
+                    const unsigned u = s_ra[0][i] + (x - s_pre[0][i]);
+                    const int q = sid[u];
+                    const double4 pu = pr[u], vu = vm[u];  // loaded beside sid[u]
+                    s_q[r * MR_T + lane] = q;
+                    s_l[r * MR_T + lane] = s_rl[0][i];
+                    if (q > kc) h[r] = approach_hit(pd, vd, pu, vu, dt);
+                }
             }
-            const unsigned long long wm = __ballot(h);
-            if ((lane & 63) == 0) s_m[wv] = wm;
+            // Most rounds hit nothing: one barrier decides.
+            if (!__syncthreads_or(h[0] || h[1])) continue;
+#pragma unroll
+            for (int r = 0; r < MR_R; ++r) {
+                const unsigned long long wm = __ballot(h[r]);
+                if ((lane & 63) == 0) s_m[r * (MR_T / 64) + wv] = wm;
+            }
             __syncthreads();
             if (lane == 0) {
-                for (int w = 0; w < MR_T / 64; ++w) {
+                for (int w = 0; w < MR_W; ++w) {
                   unsigned long long mask = s_m[w];
                   while (mask) {
                     const int bit = w * 64 + __ffsll((long long)mask) - 1;
@@ -551,7 +602,7 @@ __global__ __launch_bounds__(MR_T) void k_mesh_replay(
                     if (ev.y == prev.y && ev.z == prev.z) continue;
                     prev = ev;
                     double ap[6], tc;
-                    if (bf_hit(b, ex, dcf, ev.z, k, dt, ap, &tc)) {
+                    if (bf_hit(b, nullptr, dcf, ev.z, k, dt, ap, &tc)) {
                         for (int q = 0; q < 6; ++q) s_ap[q] = ap[q];
                         s_tc = tc;
                         s_hl = ev.y;
@@ -572,56 +623,61 @@ __global__ __launch_bounds__(MR_T) void k_mesh_replay(
         const int k = s_k;
         if (k == INT_MAX) break;
         if (!s_hit) continue;
-        eager(s_hp, k);
         // The rest of visit k in traversal order -- cells s_hl down to 0,
-        // ids below k, after s_hp in cell s_hl -- MR_T pairs at a time across
-        // cells; the first hit wins and the walk resumes after it.
-        const unsigned kk = key[k];
-        const int l0 = s_hl, p0 = s_hp;
+        // ids below k, after s_hp in cell s_hl -- MR_R MR_T pairs at a time
+        // across cells; the first hit wins and the walk resumes after it.
         // Whole cells, walked in traversal order; the members outside the
         // rest of the visit (ids at or above k, and up to p0 in cell l0) are
         // skipped in the scan.
-        if (lane < 27) {
-            const int l = l0 - lane;  // range `lane` is cell l0 - lane
-            unsigned lo = 0, e = 0;
-            if (l >= 0) {
-                const unsigned c = (unsigned)((long)kk + stencil_offset(l, d));
-                lo = cs[c];
-                e = ce[c];
-            }
-            s_ra[1][lane] = lo;
-            s_rn[1][lane] = e - lo;
-            s_rl[1][lane] = l;
-        }
+        const int l0 = s_hl, p0 = s_hp;
+        rest_ranges(key[k], l0);
+        eager(s_hp, k);
         flatten(1);
         const unsigned total = s_pre[1][27];
         unsigned base = 0;
         while (base < total) {
-            const unsigned x = base + lane;
-            int p = -1;
-            double ap[6], tc;
-            bool h = false;
-            if (x < total) {
-                const int i = range_of(1, x);
-                p = sid[s_ra[1][i] + (x - s_pre[1][i])];
-                if (p < k && (s_rl[1][i] != l0 || p > p0))
-                    h = bf_hit(b, ex, dcf, p, k, dt, ap, &tc);
+            int pv[MR_R];
+            double ap[MR_R][6], tc[MR_R];
+            bool h[MR_R];
+#pragma unroll
+            for (int r = 0; r < MR_R; ++r) {
+                const unsigned x = base + r * MR_T + lane;
+                pv[r] = -1;
+                h[r] = false;
+                if (x < total) {
+                    const int i = range_of(1, x);
+                    const int p = sid[s_ra[1][i] + (x - s_pre[1][i])];
+                    pv[r] = p;
+                    if (p < k && (s_rl[1][i] != l0 || p > p0))
+                        h[r] = bf_hit(b, nullptr, dcf, p, k, dt, ap[r], &tc[r]);
+                }
             }
-            const unsigned long long hm = __ballot(h);
-            __syncthreads();  // the masks of the previous round have been read
-            if ((lane & 63) == 0) s_m[wv] = hm;
-            __syncthreads();
-            int first = -1;
-            for (int w = 0; w < MR_T / 64 && first < 0; ++w)
-                if (s_m[w]) first = w * 64 + __ffsll((long long)s_m[w]) - 1;
-            if (first < 0) {
-                base += MR_T;
+            // Most rounds hit nothing: one barrier decides (and orders the
+            // reads of the previous round's masks before the writes below).
+            if (!__syncthreads_or(h[0] || h[1])) {
+                base += MR_R * MR_T;
                 continue;
             }
-            if (lane == first) {
-                for (int q = 0; q < 6; ++q) s_ap[q] = ap[q];
-                s_tc = tc;
-                s_hp = p;
+#pragma unroll
+            for (int r = 0; r < MR_R; ++r) {
+                const unsigned long long hm = __ballot(h[r]);
+                if ((lane & 63) == 0) s_m[r * (MR_T / 64) + wv] = hm;
+            }
+            __syncthreads();
+            int first = -1;  // first hit of the round in traversal order
+            for (int w = 0; w < MR_W && first < 0; ++w)
+                if (s_m[w]) first = w * 64 + __ffsll((long long)s_m[w]) - 1;
+            if (first < 0) {
+                base += MR_R * MR_T;
+                continue;
+            }
+#pragma unroll
+            for (int r = 0; r < MR_R; ++r) {
+                if (r * MR_T + lane == first) {
+                    for (int q = 0; q < 6; ++q) s_ap[q] = ap[r][q];
+                    s_tc = tc[r];
+                    s_hp = pv[r];
+                }
             }
             __syncthreads();
             if (lane == 0) resolve(s_hp, k);
@@ -2424,9 +2480,13 @@ ncg_status ncg_mesh_pass_dev_flags(ncg_ctx *ctx, ncg_body *d_bodies, int32_t n,
                            vin);
         NCG_CHECK_LAUNCH();
         NCG_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tb, kin, kout, vin, vout, (int)cap, 0, 64, st));
-        hipLaunchKernelGGL(k_mesh_replay, dim3(1), dim3(MR_T), 0, st, db, d_exists, d_did_collide,
+        int4 *csorted = dbuf<int4>(ctx, "m_csorted", cap);
+        if (!csorted) return NCG_ERR_NOMEM;
+        hipLaunchKernelGGL(k_mcand_gather, dim3(cdiv(cap, MT)), dim3(MT), 0, st, dc, vout, B.tot + 1,
+                           cap, csorted);
+        hipLaunchKernelGGL(k_mesh_replay, dim3(1), dim3(MR_T), 0, st, db, d_did_collide,
                            (int)one_collision_only, collision_f, timestep, B.key, B.sid, B.cs, B.ce,
-                           B.pr, B.vm, (int)d, dc, vout, B.tot + 1, cap, B.cnt, stat);
+                           B.pr, B.vm, (int)d, csorted, B.tot + 1, cap, B.cnt, stat);
         NCG_CHECK_LAUNCH();
     }
     // Counters: inserted, small, big, big-list length; neighbours,
