@@ -163,6 +163,9 @@ __global__ __launch_bounds__(MT) void k_copy_keys(const unsigned *__restrict__ s
 
 // closest_approach's predicate (collision.c:112-137) with a 2^-40 relative
 // slack on the time test; the host decides every candidate exactly.
+// Symmetric in the pair bit for bit: swapping a and b negates every
+// difference exactly, which leaves the products, sums, R and t unchanged (this
+// file is compiled without FMA contraction), so one order serves both ids.
 __device__ inline bool approach_hit(double4 pa, double4 va, double4 pb, double4 vb, double dt)
 {
     const double dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
@@ -242,7 +245,7 @@ __global__ __launch_bounds__(MT) void k_mcount_total(const unsigned *__restrict_
                     nv = vm[q + 1];
                 }
                 const bool lower = p < k;
-                if (lower ? approach_hit(rq, vq, pk, vk, dt) : approach_hit(pk, vk, rq, vq, dt)) {
+                if (approach_hit(pk, vk, rq, vq, dt)) {
                     // Stencil index of q's cell.
                     const int ci = (e0 > s0 && q < e0) ? 0 : ((r == 4 || (e1 > s1 && q < e1)) ? 1 : 2);
                     const int l = r < 4 ? 3 * r + ci : 12 + ci;
@@ -252,6 +255,107 @@ __global__ __launch_bounds__(MT) void k_mcount_total(const unsigned *__restrict_
                 }
             }
         }
+    }
+    const unsigned long long t = bsum(c, sh);
+    if (threadIdx.x == 0 && t) atomicAdd(total, t);
+}
+
+// The candidate pass of k_mcount_total (mode 0) with each stencil row staged
+// once per workgroup: the 256 consecutive bodies of a workgroup lie in a few
+// neighbouring cells, so the union of their runs of one row (cells i - 1 ..
+// i + 3 for cells i .. i + 2) is a short stretch of the sorted order; it is
+// copied to LDS (records, ids) and the lanes walk their own runs there
+// instead of loading every neighbour record from memory.  A workgroup whose
+// union is longer than MC_CAP (its bodies straddle distant cells) walks
+// memory as before.  Same tests, same candidates, same total.
+constexpr int MC_CAP = 512;
+
+__global__ __launch_bounds__(MT) void k_mcand_staged(const unsigned *__restrict__ skey,
+                                                     const int *__restrict__ sid,
+                                                     const double4 *__restrict__ pr,
+                                                     const double4 *__restrict__ vm,
+                                                     const unsigned *__restrict__ cs,
+                                                     const unsigned *__restrict__ ce,
+                                                     const int *__restrict__ dins, int d,
+                                                     double dt, int4 *__restrict__ cand,
+                                                     unsigned long long cap,
+                                                     unsigned long long *__restrict__ ccnt,
+                                                     unsigned long long *__restrict__ total)
+{
+    __shared__ double4 sp[MC_CAP], sv[MC_CAP];
+    __shared__ int sq[MC_CAP];
+    __shared__ unsigned s_lo, s_hi;
+    __shared__ unsigned long long sh[MT];
+    const int s = blockIdx.x * MT + threadIdx.x;
+    const int ins = *dins;
+    const bool act = s < ins;
+    unsigned long long c = 0;
+    long key = 0;
+    int k = 0;
+    double4 pk = make_double4(0, 0, 0, 0), vk = pk;
+    if (act) {
+        key = skey[s];
+        k = sid[s];
+        pk = pr[s];
+        vk = vm[s];
+    }
+    for (int r = 0; r < 5; ++r) {
+        unsigned a = 0, e = 0, s0 = 0, e0 = 0, s1 = 0, e1 = 0;
+        if (act) {
+            // Cells c0, c1, c2 of the run (c2 unused for the fifth run).
+            const long k1 = r < 4 ? key + (long)(r % 3 - 1) * d + (long)(r / 3 - 1) * d * d : key;
+            s0 = cs[k1 - 1];
+            e0 = ce[k1 - 1];
+            s1 = cs[k1];
+            e1 = ce[k1];
+            if (r < 4) {
+                const unsigned s2 = cs[k1 + 1], e2 = ce[k1 + 1];
+                a = e0 > s0 ? s0 : (e1 > s1 ? s1 : s2);
+                e = e2 > s2 ? e2 : (e1 > s1 ? e1 : e0);
+            } else {
+                a = e0 > s0 ? s0 : s1;  // the own cell is not empty
+                e = (unsigned)s;
+            }
+            if (a < e) c += e - a;
+            else a = e = 0;
+        }
+        if (threadIdx.x == 0) {
+            s_lo = 0xffffffffu;
+            s_hi = 0u;
+        }
+        __syncthreads();
+        if (a < e) {
+            atomicMin(&s_lo, a);
+            atomicMax(&s_hi, e);
+        }
+        __syncthreads();
+        const unsigned lo = s_lo, hi = s_hi;
+        const bool staged = hi > lo && hi - lo <= (unsigned)MC_CAP;
+        if (staged) {
+            for (unsigned q = lo + threadIdx.x; q < hi; q += MT) {
+                sp[q - lo] = pr[q];
+                sv[q - lo] = vm[q];
+                sq[q - lo] = sid[q];
+            }
+            __syncthreads();
+        }
+        auto test = [&](unsigned q, int p, double4 rq, double4 vq) {
+            const bool lower = p < k;
+            if (approach_hit(pk, vk, rq, vq, dt)) {
+                // Stencil index of q's cell.
+                const int ci = (e0 > s0 && q < e0) ? 0 : ((r == 4 || (e1 > s1 && q < e1)) ? 1 : 2);
+                const int l = r < 4 ? 3 * r + ci : 12 + ci;
+                const unsigned long long slot = atomicAdd(ccnt, 1ull);
+                if (slot < cap)
+                    cand[slot] = lower ? make_int4(k, l, p, 0) : make_int4(p, 26 - l, k, 0);
+            }
+        };
+        if (staged) {
+            for (unsigned q = a; q < e; ++q) test(q, sq[q - lo], sp[q - lo], sv[q - lo]);
+        } else {
+            for (unsigned q = a; q < e; ++q) test(q, sid[q], pr[q], vm[q]);
+        }
+        __syncthreads();  // the row's reads are done before the next row is staged
     }
     const unsigned long long t = bsum(c, sh);
     if (threadIdx.x == 0 && t) atomicAdd(total, t);
@@ -310,7 +414,7 @@ __global__ __launch_bounds__(MT) void k_mcount(const unsigned *__restrict__ skey
                     }
                     const bool lower = p < k;
                     c += lower;
-                    if (lower ? approach_hit(rq, vq, pk, vk, dt) : approach_hit(pk, vk, rq, vq, dt)) {
+                    if (approach_hit(pk, vk, rq, vq, dt)) {
                         const unsigned long long slot = atomicAdd(ccnt, 1ull);
                         if (slot < cap)
                             cand[slot] = lower ? make_int4(k, l, p, 0) : make_int4(p, 26 - l, k, 0);
@@ -2439,7 +2543,7 @@ ncg_status ncg_mesh_pass_dev_flags(ncg_ctx *ctx, ncg_body *d_bodies, int32_t n,
         // The device pass keeps no per-body counts (ncg_mesh_pairs is not
         // available after it): the total alone.
         if (mode == 0)
-            hipLaunchKernelGGL(k_mcount_total<0>, dim3(g), dim3(MT), 0, st, B.skey, B.sid, B.pr,
+            hipLaunchKernelGGL(k_mcand_staged, dim3(g), dim3(MT), 0, st, B.skey, B.sid, B.pr,
                                B.vm, B.cs, B.ce, B.cnt, (int)d, timestep, dc,
                                (unsigned long long)cap, B.tot + 1, B.tot);
         else
