@@ -601,6 +601,41 @@ __global__ __launch_bounds__(MR_T) void k_mesh_replay(
             s_rl[1][lane - 64] = l;
         }
     };
+    // The round's later visitors that may hit changed body dd (h[r] for
+    // entry r MR_T + lane, whose visitor and cell are in s_q, s_l) become
+    // pending re-tests.  Every lane calls it; most rounds hit nothing, and
+    // one barrier vote decides that.
+    auto record = [&](const bool *h, int dd) {
+        if (!__syncthreads_or(h[0] || h[1])) return;
+#pragma unroll
+        for (int r = 0; r < MR_R; ++r) {
+            const unsigned long long wm = __ballot(h[r]);
+            if ((lane & 63) == 0) s_m[r * (MR_T / 64) + wv] = wm;
+        }
+        __syncthreads();
+        if (lane == 0) {
+            for (int w = 0; w < MR_W; ++w) {
+              unsigned long long mask = s_m[w];
+              while (mask) {
+                const int bit = w * 64 + __ffsll((long long)mask) - 1;
+                mask &= mask - 1;
+                // In visit q, dd sits in cell 26 - l (the stencil is symmetric).
+                const int4 ev = make_int4(s_q[bit], 26 - s_l[bit], dd, 0);
+                if (s_np >= MR_PCAP) {
+                    atomicOr(stat + 1, 2u);
+                    continue;
+                }
+                int at = s_np++;
+                while (at > s_pi && mr_before(ev, P[at - 1])) {
+                    P[at] = P[at - 1];
+                    --at;
+                }
+                P[at] = ev;
+              }
+            }
+        }
+        __syncthreads();
+    };
     // Later visitors of body dd (just changed during visit kc) that may hit
     // it: the members above kc of its 27 cells, MR_T at a time across cells.
     auto eager = [&](int dd, int kc) {
@@ -636,36 +671,7 @@ __global__ __launch_bounds__(MR_T) void k_mesh_replay(
                     if (q > kc) h[r] = approach_hit(pd, vd, pu, vu, dt);
                 }
             }
-            // Most rounds hit nothing: one barrier decides.
-            if (!__syncthreads_or(h[0] || h[1])) continue;
-#pragma unroll
-            for (int r = 0; r < MR_R; ++r) {
-                const unsigned long long wm = __ballot(h[r]);
-                if ((lane & 63) == 0) s_m[r * (MR_T / 64) + wv] = wm;
-            }
-            __syncthreads();
-            if (lane == 0) {
-                for (int w = 0; w < MR_W; ++w) {
-                  unsigned long long mask = s_m[w];
-                  while (mask) {
-                    const int bit = w * 64 + __ffsll((long long)mask) - 1;
-                    mask &= mask - 1;
-                    // In visit q, dd sits in cell 26 - l (the stencil is symmetric).
-                    const int4 ev = make_int4(s_q[bit], 26 - s_l[bit], dd, 0);
-                    if (s_np >= MR_PCAP) {
-                        atomicOr(stat + 1, 2u);
-                        continue;
-                    }
-                    int at = s_np++;
-                    while (at > s_pi && mr_before(ev, P[at - 1])) {
-                        P[at] = P[at - 1];
-                        --at;
-                    }
-                    P[at] = ev;
-                  }
-                }
-            }
-            __syncthreads();
+            record(h, dd);
         }
     };
     auto resolve = [&](int p, int k) {  // lane 0
@@ -733,32 +739,51 @@ __global__ __launch_bounds__(MR_T) void k_mesh_replay(
         // Whole cells, walked in traversal order; the members outside the
         // rest of the visit (ids at or above k, and up to p0 in cell l0) are
         // skipped in the scan.
+        // One walk over all 27 cells of k (l = 26 .. 0) does two things: the
+        // rest of visit k (ids below k in cells l0 .. 0, after p0 in cell l0)
+        // and, while k keeps the state of its first hit, k's own
+        // later-visitor scan (ids above k).  A further hit changes k: the
+        // scan's pending re-tests so far stay (a superset does no harm) and
+        // k's scan runs again at the end with its final state.
         const int l0 = s_hl, p0 = s_hp;
-        rest_ranges(key[k], l0);
+        rest_ranges(key[k], 26);
         eager(s_hp, k);
         flatten(1);
         const unsigned total = s_pre[1][27];
+        const DBody K = b[k];
+        const double4 pkk = make_double4(K.x, K.y, K.z, K.radius);
+        const double4 vkk = make_double4(K.vx, K.vy, K.vz, K.mass);
+        bool stale = false;  // k changed after its first hit (the same in every lane)
         unsigned base = 0;
         while (base < total) {
             int pv[MR_R];
             double ap[MR_R][6], tc[MR_R];
-            bool h[MR_R];
+            bool h[MR_R], g[MR_R];
 #pragma unroll
             for (int r = 0; r < MR_R; ++r) {
                 const unsigned x = base + r * MR_T + lane;
                 pv[r] = -1;
                 h[r] = false;
+                g[r] = false;
                 if (x < total) {
                     const int i = range_of(1, x);
-                    const int p = sid[s_ra[1][i] + (x - s_pre[1][i])];
+                    const unsigned u = s_ra[1][i] + (x - s_pre[1][i]);
+                    const int p = sid[u];
+                    const double4 pu = pr[u], vu = vm[u];  // loaded beside sid[u]
+                    const int l = s_rl[1][i];
                     pv[r] = p;
-                    if (p < k && (s_rl[1][i] != l0 || p > p0))
+                    s_q[r * MR_T + lane] = p;
+                    s_l[r * MR_T + lane] = l;
+                    if (p < k && (l < l0 || (l == l0 && p > p0)))
                         h[r] = bf_hit(b, nullptr, dcf, p, k, dt, ap[r], &tc[r]);
+                    else if (!stale && p > k)
+                        g[r] = approach_hit(pkk, vkk, pu, vu, dt);
                 }
             }
             // Most rounds hit nothing: one barrier decides (and orders the
             // reads of the previous round's masks before the writes below).
             if (!__syncthreads_or(h[0] || h[1])) {
+                if (!stale) record(g, k);
                 base += MR_R * MR_T;
                 continue;
             }
@@ -788,9 +813,10 @@ __global__ __launch_bounds__(MR_T) void k_mesh_replay(
             __threadfence_block();
             __syncthreads();
             eager(s_hp, k);
+            stale = true;
             base = base + first + 1;
         }
-        eager(k, k);
+        if (stale) eager(k, k);
     }
     if (lane == 0 && hits) atomicAdd(stat, hits);
 }
