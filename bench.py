@@ -535,11 +535,17 @@ def dropin_run():
                  if ("call", k) in t}
         kern = {k[1]: v for k, v in t.items() if k[0] == "kernel"}
         hot = sum(v["seconds"] for v in calls.values())
+        first = sum(v.get("first", 0.0) for v in calls.values())
         dev = sum(v["ms"] for v in kern.values()) * 1e-3
         out["c2"] = {"n": MESH_N, "steps": 3, "precision": "exact",
                      "seconds_per_step": hot / 3, "device_kernel_seconds_per_step": dev / 3,
                      "host_and_transfer_seconds_per_step": (hot - dev) / 3,
-                     "calls": {k: {"calls": int(v["calls"]), "seconds": v["seconds"]}
+                     # Each function's first call also creates the GPU context
+                     # and loads its kernels; without them, about two steps.
+                     "first_calls_seconds": first,
+                     "seconds_per_step_without_first_calls": (hot - first) / 2,
+                     "calls": {k: {"calls": int(v["calls"]), "seconds": v["seconds"],
+                                   "first": v.get("first")}
                                for k, v in calls.items()},
                      "kernels_ms": {k: v["ms"] for k, v in kern.items()},
                      "process_wall_seconds": wall}

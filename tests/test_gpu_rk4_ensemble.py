@@ -339,3 +339,58 @@ def test_ensemble_collisions_of_clumps_crossing_within_dt(ctx):
         systems.append(b)
     nc, total = _collision_run(ctx, systems, 0.04, 1)
     assert nc == total and total > 100, (nc, total)
+
+
+@pytest.mark.parametrize("prec", ["fast64", "mixed"])
+def test_ensemble_with_empty_and_tiny_systems(ctx, prec):
+    """Systems of 0, 1 and 2 bodies between ordinary ones (the padded slot
+    layout of the mixed form and the collision pass give an empty system no
+    slot): gravity equals the per-system oracle, and RK4 steps with
+    collisions leave every body finite and match the single-system runs of
+    the nonempty systems."""
+    torch = torch_dev()
+    sizes = [0, 300, 0, 1, 2, 0, 129, 1000, 0]
+    systems = [small_disk(int(nn), 700 + s) if nn else np.zeros((0, 8))
+               for s, nn in enumerate(sizes)]
+    allb = np.concatenate(systems)
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)
+    S = len(sizes)
+    cen = np.zeros((S, 8))
+    cen[:, 6] = 1.0
+    cen[:, 7] = 1.0
+    bt = torch.from_numpy(allb).cuda()
+    ct = torch.from_numpy(cen).cuda()
+    at = torch.zeros((len(allb), 3), dtype=torch.float64, device="cuda")
+    cm = torch.zeros((S, 3), dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    ctx.ensemble_gravity_dev(bt, off, ct, at, cm, 1.0, 0.01, prec)
+    ctx.synchronize()
+    acc, cma = at.cpu().numpy(), cm.cpu().numpy()
+    for s, sysb in enumerate(systems):
+        if len(sysb) == 0:
+            assert np.all(cma[s] == 0.0)
+            continue
+        ref, rcm = orc.vector_field_brute(sysb, None, (0, 0, 0, 1.0), 1.0, 0.01)
+        got = acc[off[s]:off[s + 1]]
+        mut, _ = orc.vector_field_brute(sysb, None, None, 1.0, 0.01)
+        tol = 1e-12 if prec == "fast64" else 1e-6
+        bound = (tol * np.linalg.norm(mut, axis=1)
+                 + (4e-16 * math.sqrt(len(sysb)) + 1e-15) * np.linalg.norm(ref, axis=1))
+        assert np.all(np.linalg.norm(got - ref, axis=1) <= bound), s
+        assert np.allclose(cma[s], rcm, rtol=1e-12, atol=0)
+    dt = 2 * math.pi / 256
+    ctx.ensemble_rk4_dev(bt, off, ct, 3, dt, 1.0, 0.0, prec, graph=True, collisions=True,
+                         collision_f=1.5)
+    ctx.synchronize()
+    out = bt.cpu().numpy()
+    assert np.all(np.isfinite(out))
+    if prec == "fast64":
+        for s, sysb in enumerate(systems):
+            if len(sysb) == 0:
+                continue
+            ref, rc = sysb.copy(), np.array([0, 0, 0, 0, 0, 0, 1.0])
+            for _ in range(3):
+                ref, _, _ = orc.brute_force(ref, dt, 1, 1.5, 0)
+                ref, rc = orc.rk4_step(ref, rc, 1.0, 0.0, dt)
+            moved = np.abs(ref[:, :6] - sysb[:, :6]).max()
+            assert np.abs(out[off[s]:off[s + 1], :6] - ref[:, :6]).max() < 1e-11 * max(moved, 1e-300), s
