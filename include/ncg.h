@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define NCG_ABI_VERSION 3
+#define NCG_ABI_VERSION 4
 
 typedef struct ncg_ctx ncg_ctx;
 
@@ -132,6 +132,8 @@ typedef struct {
     int64_t collisions;
     int64_t candidates;     /* GPU candidate pairs handed to the host */
     int64_t host_tests;     /* pairs the host re-tested after a collision */
+    int64_t replay_conflicts; /* device mesh() passes: 1 when clusters of the parallel
+                                 replay met and the pass was replayed by one workgroup */
 } ncg_mesh_stats;
 
 int ncg_abi_version(void);
@@ -272,6 +274,7 @@ typedef struct {
     double r_high_dumping;         /* high_dumping_threshold: checked, no body may go this far */
     int32_t cells;                 /* collision_cube_cells */
     int32_t host_passes;           /* out: mesh() passes of the call replayed on the host */
+    int32_t conflict_passes;       /* out: device mesh() passes whose replay clusters met */
 } ncg_mesh_loop_params;
 
 /* GPU-resident mesh loop: NcorpiON's integration_mesh (SBAB1, rk4.c:662-837)

@@ -44,7 +44,7 @@ class MeshLoopParams(ctypes.Structure):
     _fields_ = [("mesh_size", ctypes.c_double), ("mesh_size_min", ctypes.c_double),
                 ("target_neighbours", ctypes.c_double), ("r_disruption", ctypes.c_double),
                 ("r_high_dumping", ctypes.c_double), ("cells", ctypes.c_int32),
-                ("host_passes", ctypes.c_int32)]
+                ("host_passes", ctypes.c_int32), ("conflict_passes", ctypes.c_int32)]
 
 
 class BuiltinCollider(ctypes.Structure):
@@ -80,7 +80,8 @@ class BuiltinOps:
 
 class MeshStats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int64) for n in ("total_neighbours", "how_many_small", "how_many_big",
-                                             "collisions", "candidates", "host_tests")]
+                                             "collisions", "candidates", "host_tests",
+                                             "replay_conflicts")]
 
 
 _lib = None
@@ -340,7 +341,7 @@ class Context:
         mp = MeshLoopParams(float(params["gam"]), float(params["gam_min"]),
                             float(params["how_many_neighbours"]),
                             float(params["disruption_threshold"]),
-                            float(params["high_dumping_threshold"]), int(params["cells"]), 0)
+                            float(params["high_dumping_threshold"]), int(params["cells"]), 0, 0)
         nc = I64()
         flags = (2 if half_kick_start else 0) | (4 if half_kick_end else 0)
         self._check(self.lib.ncg_mesh_leapfrog_dev(
@@ -351,6 +352,7 @@ class Context:
             ctypes.byref(nc), stream))
         params["gam"] = mp.mesh_size
         params["host_passes"] = int(mp.host_passes)
+        params["conflict_passes"] = int(mp.conflict_passes)
         return int(nc.value)
 
     def mesh_gravity_dev(self, bodies_t, acc_t, central_t=None, cm_t=None, exists_t=None, G=1.0,

@@ -359,6 +359,7 @@ extern "C" ncg_status ncg_mesh_leapfrog_dev(ncg_ctx *ctx, ncg_body *d_bodies, in
     int32_t *snap_dc = d_did_collide ? dbuf<int32_t>(ctx, "ml_snap_dc", n) : nullptr;
     if (!snap_b || (d_did_collide && !snap_dc)) return NCG_ERR_NOMEM;
     mp->host_passes = 0;
+    mp->conflict_passes = 0;
     auto host_pass = [&](double h, double cube, ncg_mesh_stats &ms) -> ncg_status {
         NCG_HIP(hipMemcpyAsync(b, snap_b, sizeof(DBody) * (size_t)n, hipMemcpyDeviceToDevice, st));
         if (d_did_collide)
@@ -413,6 +414,7 @@ extern "C" ncg_status ncg_mesh_leapfrog_dev(ncg_ctx *ctx, ncg_body *d_bodies, in
                                                mp->mesh_size, cube, mp->cells, mode, collision_f,
                                                one_collision_only, &ms, &flags, st);
         if (r != NCG_OK) return r;
+        mp->conflict_passes += (int32_t)ms.replay_conflicts;
         if (flags != 0u) {
             if (mode != 0)
                 return ncg_fail(ctx, NCG_ERR_INVALID, "neighbour pass failed (flags %u)", flags);

@@ -508,6 +508,126 @@ __global__ __launch_bounds__(MT) void k_mcand_gather(const int4 *__restrict__ ca
     if (i < cap && i < m) sorted[i] = cand[order[i]];
 }
 
+// ---- clusters of the replay --------------------------------------------------
+// Visits far apart in the mesh do not interact: a visit reads and changes
+// only bodies of its 27 cells, and a changed body's later visitors are in
+// its own 27 cells.  The candidates are grouped into clusters whose visits'
+// cells are within `dist` cells (4 by default) of another visit of the cluster
+// (k_mcluster), and each cluster is replayed by its own workgroup in id
+// order.  Every workgroup claims the cells it reads or changes in a hash
+// table; when two clusters claim one cell the clustered results are thrown
+// away (the bodies restored from a snapshot) and the pass is replayed by one
+// workgroup over all candidates, so the result is always the one-workgroup
+// replay's.
+constexpr int MCL_MAXM = 1024;     // candidates clustered (more: one cluster)
+constexpr int MCL_TAB = 1 << 16;   // claim table entries (cell << 32 | cluster + 1)
+
+__device__ inline bool claim_cell(unsigned long long *__restrict__ tab, unsigned c, int cl)
+{
+    const unsigned long long mine = ((unsigned long long)c << 32) | (unsigned)(cl + 1);
+    unsigned h = (c * 2654435761u) >> 16;
+    for (int probe = 0; probe < MCL_TAB; ++probe, h = (h + 1) & (MCL_TAB - 1)) {
+        unsigned long long cur = tab[h];
+        if (cur == 0ull) {
+            cur = atomicCAS(&tab[h], 0ull, mine);
+            if (cur == 0ull) return true;
+        }
+        if ((unsigned)(cur >> 32) == c) return cur == mine;
+    }
+    return false;  // table full: treated as a conflict
+}
+
+__global__ __launch_bounds__(MT) void k_fill_u64(unsigned long long *__restrict__ p, int n)
+{
+    const int i = blockIdx.x * MT + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+// One workgroup of MCL_T threads: clusters of the m <= MCL_MAXM candidates
+// (connected components of their visits' cells under Chebyshev distance
+// <= dist, by hooking roots with atomicMin and path halving), then the
+// candidates regrouped by cluster (each cluster's in their visiting order),
+// coff[0 .. ncl] their offsets.  More candidates: one cluster of all.
+constexpr int MCL_T = 1024;
+
+__global__ __launch_bounds__(MCL_T) void k_mcluster(const int4 *__restrict__ cs_in,
+                                                    const unsigned long long *__restrict__ ccnt,
+                                                    unsigned cap,
+                                                    const unsigned *__restrict__ key, int d,
+                                                    int dist, int4 *__restrict__ cs_out,
+                                                    int *__restrict__ coff,
+                                                    int *__restrict__ ncl)
+{
+    __shared__ int lab[MCL_MAXM], cx[MCL_MAXM], cy[MCL_MAXM], cz[MCL_MAXM];
+    __shared__ int s_changed;
+    const int t = threadIdx.x;
+    const unsigned long long total = *ccnt;
+    const int m = (int)(total < cap ? total : cap);
+    if (m > MCL_MAXM || m <= 1) {
+        for (int i = t; i < m; i += MCL_T) cs_out[i] = cs_in[i];
+        if (t == 0) {
+            *ncl = m > 0 ? 1 : 0;
+            coff[0] = 0;
+            coff[1] = m;
+        }
+        return;
+    }
+    for (int i = t; i < m; i += MCL_T) {
+        const unsigned c = key[cs_in[i].x];
+        cx[i] = (int)(c % (unsigned)d);
+        cy[i] = (int)((c / (unsigned)d) % (unsigned)d);
+        cz[i] = (int)(c / ((unsigned)d * (unsigned)d));
+        lab[i] = i;
+    }
+    __syncthreads();
+    auto root = [&](int x) {
+        while (true) {
+            const int p = lab[x];
+            if (p == x) return x;
+            const int g = lab[p];
+            if (g != p) atomicMin(&lab[x], g);  // path halving
+            x = p;
+        }
+    };
+    while (true) {
+        if (t == 0) s_changed = 0;
+        __syncthreads();
+        for (int i = t; i < m; i += MCL_T) {
+            for (int j = i + 1; j < m; ++j) {
+                if (abs(cx[i] - cx[j]) > dist || abs(cy[i] - cy[j]) > dist ||
+                    abs(cz[i] - cz[j]) > dist)
+                    continue;
+                const int ri = root(i), rj = root(j);
+                if (ri != rj) {
+                    atomicMin(&lab[ri > rj ? ri : rj], ri < rj ? ri : rj);
+                    s_changed = 1;
+                }
+            }
+        }
+        __syncthreads();
+        const int ch = s_changed;
+        __syncthreads();
+        if (!ch) break;
+    }
+    for (int i = t; i < m; i += MCL_T) lab[i] = root(i);
+    __syncthreads();
+    if (t == 0) {
+        // Cluster ids in order of their first candidate; stable regrouping.
+        int nc = 0;
+        for (int i = 0; i < m; ++i)
+            if (lab[i] == i) cx[i] = nc++;  // cx reused: cluster id of root i
+        for (int c = 0; c <= nc; ++c) coff[c] = 0;
+        for (int i = 0; i < m; ++i) coff[cx[lab[i]] + 1]++;
+        for (int c = 0; c < nc; ++c) coff[c + 1] += coff[c];
+        for (int i = 0; i < m; ++i) cy[i] = 0;  // cy reused: fill per cluster
+        for (int i = 0; i < m; ++i) {
+            const int c = cx[lab[i]];
+            cs_out[coff[c] + cy[c]++] = cs_in[i];
+        }
+        *ncl = nc;
+    }
+}
+
 __device__ inline bool mr_before(int4 a, int4 b)  // (q, 26 - l, d) order
 {
     if (a.x != b.x) return a.x < b.x;
@@ -521,10 +641,12 @@ __global__ __launch_bounds__(MR_T) void k_mesh_replay(
     const unsigned *__restrict__ cs, const unsigned *__restrict__ ce,
     const double4 *__restrict__ pr, const double4 *__restrict__ vm, int d,
     const int4 *__restrict__ csorted, const unsigned long long *__restrict__ ccnt, unsigned cap,
-    const int *__restrict__ cnt, unsigned *__restrict__ stat)
+    const int *__restrict__ cnt, unsigned *__restrict__ stat, const int *__restrict__ coff,
+    const int *__restrict__ ncl, unsigned long long *__restrict__ ctab,
+    unsigned *__restrict__ conflict)
 {
     __shared__ int4 P[MR_PCAP];
-    __shared__ int s_np, s_pi, s_ci, s_k, s_hl, s_hp, s_hit;
+    __shared__ int s_np, s_pi, s_ci, s_k, s_hl, s_hp, s_hit, s_stop;
     // Entry r MR_T + lane of a scan round: its visitor and stencil cell; hit
     // mask w covers entries 64 w .. 64 w + 63 of the round.
     __shared__ int s_q[MR_R * MR_T], s_l[MR_R * MR_T];
@@ -547,7 +669,18 @@ __global__ __launch_bounds__(MR_T) void k_mesh_replay(
         if (lane == 0) atomicOr(stat + 1, 1u);
         return;
     }
-    const int m = (int)total;
+    // coff: this workgroup replays cluster blockIdx.x (candidates
+    // coff[b] .. coff[b + 1] of csorted), claiming the cells it touches when
+    // there is more than one cluster; else the whole list.
+    int c0 = 0, m = (int)total;
+    bool claims = false;
+    if (coff) {
+        const int nc = *ncl;
+        if ((int)blockIdx.x >= nc) return;
+        c0 = coff[blockIdx.x];
+        m = coff[blockIdx.x + 1] - c0;
+        claims = nc > 1;
+    }
     if (lane == 0) {
         s_np = 0;
         s_pi = 0;
@@ -555,7 +688,14 @@ __global__ __launch_bounds__(MR_T) void k_mesh_replay(
     }
     __syncthreads();
     unsigned hits = 0;
-    auto cand_at = [&](int i) { return csorted[i]; };
+    auto cand_at = [&](int i) { return csorted[c0 + i]; };
+    // The 27 cells around cell key kd, claimed for this cluster (every lane
+    // calls it).
+    auto claim = [&](unsigned kd) {
+        if (claims && lane < 27 && kd != NOKEY &&
+            !claim_cell(ctab, (unsigned)((long)kd + stencil_offset(lane, d)), (int)blockIdx.x))
+            atomicOr(conflict, 1u);
+    };
     // Prefix of the 27 ranges of set r (every lane calls it): a scan across
     // the first wavefront's lanes.
     auto flatten = [&](int r) {
@@ -641,6 +781,7 @@ __global__ __launch_bounds__(MR_T) void k_mesh_replay(
     auto eager = [&](int dd, int kc) {
         const unsigned kd = key[dd];
         if (kd == NOKEY) return;
+        claim(kd);
         const DBody D = b[dd];
         const double4 pd = make_double4(D.x, D.y, D.z, D.radius);
         const double4 vd = make_double4(D.vx, D.vy, D.vz, D.mass);
@@ -684,6 +825,9 @@ __global__ __launch_bounds__(MR_T) void k_mesh_replay(
     };
     while (true) {
         if (lane == 0) {
+            // Another cluster met this one: the results are thrown away.
+            s_stop = claims && __hip_atomic_load(conflict, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT) != 0u;
             if (s_pi >= MR_PCAP / 2) {  // drop the consumed re-tests
                 for (int q = s_pi; q < s_np; ++q) P[q - s_pi] = P[q];
                 s_np -= s_pi;
@@ -731,7 +875,8 @@ __global__ __launch_bounds__(MR_T) void k_mesh_replay(
         __threadfence_block();
         __syncthreads();
         const int k = s_k;
-        if (k == INT_MAX) break;
+        if (k == INT_MAX || s_stop) break;
+        claim(key[k]);  // the cells visit k read (its events and, after a hit, the rest)
         if (!s_hit) continue;
         // The rest of visit k in traversal order -- cells s_hl down to 0,
         // ids below k, after s_hp in cell s_hl -- MR_R MR_T pairs at a time
@@ -2614,10 +2759,50 @@ ncg_status ncg_mesh_pass_dev_flags(ncg_ctx *ctx, ncg_body *d_bodies, int32_t n,
         if (!csorted) return NCG_ERR_NOMEM;
         hipLaunchKernelGGL(k_mcand_gather, dim3(cdiv(cap, MT)), dim3(MT), 0, st, dc, vout, B.tot + 1,
                            cap, csorted);
-        hipLaunchKernelGGL(k_mesh_replay, dim3(1), dim3(MR_T), 0, st, db, d_did_collide,
-                           (int)one_collision_only, collision_f, timestep, B.key, B.sid, B.cs, B.ce,
-                           B.pr, B.vm, (int)d, csorted, B.tot + 1, cap, B.cnt, stat);
+        // Clusters replayed in parallel, from a snapshot of the bodies and
+        // did_collide that a conflict between clusters restores.
+        int4 *cscl = dbuf<int4>(ctx, "m_cscl", cap);
+        int *coff = dbuf<int>(ctx, "m_coff", MCL_MAXM + 2);
+        int *ncl = dbuf<int>(ctx, "m_ncl", 1);
+        unsigned long long *ctab = dbuf<unsigned long long>(ctx, "m_ctab", MCL_TAB);
+        unsigned *confl = dbuf<unsigned>(ctx, "m_confl", 1);
+        DBody *snapb = dbuf<DBody>(ctx, "m_snapb", n);
+        int32_t *snapdc = d_did_collide ? dbuf<int32_t>(ctx, "m_snapdc", n) : nullptr;
+        if (!cscl || !coff || !ncl || !ctab || !confl || !snapb || (d_did_collide && !snapdc))
+            return NCG_ERR_NOMEM;
+        hipLaunchKernelGGL(k_mcluster, dim3(1), dim3(MCL_T), 0, st, csorted, B.tot + 1, cap, B.key,
+                           (int)d, ctx->replay_cluster_dist, cscl, coff, ncl);
+        hipLaunchKernelGGL(k_fill_u64, dim3(cdiv(MCL_TAB, MT)), dim3(MT), 0, st, ctab, MCL_TAB);
+        hipLaunchKernelGGL(k_lf_fill_u32, dim3(1), dim3(64), 0, st, confl, 1u);
         NCG_CHECK_LAUNCH();
+        NCG_HIP(hipMemcpyAsync(snapb, db, sizeof(DBody) * (size_t)n, hipMemcpyDeviceToDevice, st));
+        if (d_did_collide)
+            NCG_HIP(hipMemcpyAsync(snapdc, d_did_collide, sizeof(int32_t) * (size_t)n,
+                                   hipMemcpyDeviceToDevice, st));
+        hipLaunchKernelGGL(k_mesh_replay, dim3(MCL_MAXM), dim3(MR_T), 0, st, db, d_did_collide,
+                           (int)one_collision_only, collision_f, timestep, B.key, B.sid, B.cs, B.ce,
+                           B.pr, B.vm, (int)d, cscl, B.tot + 1, cap, B.cnt, stat, coff, ncl, ctab,
+                           confl);
+        NCG_CHECK_LAUNCH();
+        unsigned hconf = 0u;
+        NCG_HIP(hipMemcpyAsync(hs + 10, confl, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+        NCG_HIP(hipStreamSynchronize(st));
+        hconf = hs[10];
+        if (hconf) {
+            // Two clusters met: the one-workgroup replay from the snapshot.
+            NCG_HIP(hipMemcpyAsync(db, snapb, sizeof(DBody) * (size_t)n, hipMemcpyDeviceToDevice, st));
+            if (d_did_collide)
+                NCG_HIP(hipMemcpyAsync(d_did_collide, snapdc, sizeof(int32_t) * (size_t)n,
+                                       hipMemcpyDeviceToDevice, st));
+            hipLaunchKernelGGL(k_lf_fill_u32, dim3(1), dim3(64), 0, st, stat, 2u);
+            hipLaunchKernelGGL(k_mesh_replay, dim3(1), dim3(MR_T), 0, st, db, d_did_collide,
+                               (int)one_collision_only, collision_f, timestep, B.key, B.sid, B.cs,
+                               B.ce, B.pr, B.vm, (int)d, csorted, B.tot + 1, cap, B.cnt, stat,
+                               (const int *)nullptr, (const int *)nullptr,
+                               (unsigned long long *)nullptr, (unsigned *)nullptr);
+            NCG_CHECK_LAUNCH();
+        }
+        stats->replay_conflicts = hconf ? 1 : 0;
     }
     // Counters: inserted, small, big, big-list length; neighbours,
     // candidates; collisions, error bits.

@@ -148,6 +148,10 @@ ncg_status ncg_create(ncg_ctx **out, int device)
         const long v = atol(rc);
         if (v > 0 && v < (1L << 30)) ctx->replay_cap = (unsigned)v;
     }
+    if (const char *cd = getenv("NCG_REPLAY_CLUSTER_DIST")) {
+        const long v = atol(cd);
+        if (v >= 0 && v <= 64) ctx->replay_cluster_dist = (int)v;
+    }
     if (ncg_static_scratch_init(ctx) != NCG_OK) {
         (void)hipStreamDestroy(ctx->stream);
         delete ctx;

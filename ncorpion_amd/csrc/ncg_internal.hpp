@@ -115,6 +115,9 @@ struct ncg_ctx {
     // loops); NCG_REPLAY_CAP in the environment lowers it (tests of the
     // overflow fallback).  0 = each loop's default.
     unsigned replay_cap = 0;
+    // Cells between visits of one cluster of the parallel mesh replay
+    // (NCG_REPLAY_CLUSTER_DIST; a test knob: 0 makes clusters meet).
+    int replay_cluster_dist = 4;
 };
 
 void ncg_rk4_release(ncg_ctx *ctx);

@@ -26,7 +26,7 @@ def test_library_exports_every_symbol():
     missing = [n for n in declared() if not hasattr(lib, n)]
     assert not missing, missing
     lib.ncg_abi_version.restype = ctypes.c_int
-    assert lib.ncg_abi_version() == 3
+    assert lib.ncg_abi_version() == 4
 
 
 def test_python_binding_covers_header():

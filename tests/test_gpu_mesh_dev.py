@@ -67,6 +67,43 @@ def test_mesh_pass_dev_matches_host_replay(ctx, seed, cells, gam, rmax, f, one, 
     assert np.array_equal(ddc, hdc)
 
 
+def test_mesh_pass_dev_cluster_conflicts_fall_back_to_one_workgroup():
+    """The device replay runs clusters of visits in parallel workgroups and,
+    when two clusters claim one cell, restores the bodies and replays the
+    pass in one workgroup.  With the cluster distance forced to 0
+    (NCG_REPLAY_CLUSTER_DIST, read at context creation) every cell is a
+    cluster, so dense passes meet: the results still equal the host replay
+    bit for bit, and the fallback ran."""
+    import os
+    from ncorpion_amd import Context
+    os.environ["NCG_REPLAY_CLUSTER_DIST"] = "0"
+    try:
+        c0 = Context(0)
+    finally:
+        del os.environ["NCG_REPLAY_CLUSTER_DIST"]
+    conflicts = 0
+    try:
+        for seed, cells, gam, rmax, f, one, holes in [
+                (41, 32, 0.004, 3.5e-3, 1.5, 0, True), (42, 64, 0.0025, 2.2e-3, 1.25, 1, False),
+                (43, 16, 0.01, 6e-3, 2.0, 1, False)]:
+            n, dt = 6000, 0.01
+            b = blob(n, seed, rmax=rmax)
+            cube = cells * gam
+            ex = None
+            if holes:
+                ex = (np.random.default_rng(seed).uniform(size=n) > 0.05).astype(np.int32)
+            hb, hdc, hst, hcoll = host_pass(c0, b, ex, dt, gam, cube, cells, f, one)
+            db, ddc, dst = dev_pass(c0, b, ex, dt, gam, cube, cells, f, one)
+            conflicts += dst["replay_conflicts"]
+            assert dst["collisions"] == hcoll > 10
+            live = np.ones(n, bool) if ex is None else ex != 0
+            assert np.array_equal(bits(db[live]), bits(hb[live]))
+            assert np.array_equal(ddc, hdc)
+    finally:
+        c0.close()
+    assert conflicts > 0
+
+
 def test_mesh_pass_dev_against_oracle_and_field(ctx):
     """The device pass against the oracle's mesh() directly, then the field
     on the device after it equals the host entry point's (bit for bit)."""
