@@ -252,9 +252,13 @@ ncg_status ncg_mesh_pass(ncg_ctx *ctx, const ncg_body *bodies, int32_t n,
  * mesh()'s visiting order on the device with the library's own outcome, the
  * bounce of collision (physics.c:545-605) with restitution collision_f (2:
  * elastic), and needs every body to be at most the mesh size gam (bodies
- * larger than gam: NCG_ERR_INVALID, use ncg_mesh_pass).  Enqueued on
- * `stream` (NULL: the context's stream); returns after the counters are
- * read back (one synchronisation).  stats->host_tests is 0.  The binning is
+ * larger than gam: NCG_ERR_INVALID, use ncg_mesh_pass).  The replay runs
+ * clusters of candidate visits (cells within 4 of each other) in parallel
+ * workgroups; when two clusters touch one cell the pass is replayed by one
+ * workgroup from a snapshot (stats->replay_conflicts = 1), so the result is
+ * always that of the visiting order.  Enqueued on `stream` (NULL: the
+ * context's stream); returns after the counters are read back (one
+ * synchronisation, two after a conflict).  stats->host_tests is 0.  The binning is
  * kept for ncg_mesh_gravity / ncg_mesh_gravity_dev; ncg_mesh_pairs is not
  * available after it. */
 ncg_status ncg_mesh_pass_dev(ncg_ctx *ctx, ncg_body *d_bodies, int32_t n, const int32_t *d_exists,
