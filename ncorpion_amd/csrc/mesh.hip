@@ -611,20 +611,27 @@ __global__ __launch_bounds__(MCL_T) void k_mcluster(const int4 *__restrict__ cs_
     }
     for (int i = t; i < m; i += MCL_T) lab[i] = root(i);
     __syncthreads();
+    __shared__ int s_off[MCL_MAXM + 1];
     if (t == 0) {
-        // Cluster ids in order of their first candidate; stable regrouping.
+        // Cluster ids in order of their first candidate, and their offsets.
         int nc = 0;
         for (int i = 0; i < m; ++i)
             if (lab[i] == i) cx[i] = nc++;  // cx reused: cluster id of root i
-        for (int c = 0; c <= nc; ++c) coff[c] = 0;
-        for (int i = 0; i < m; ++i) coff[cx[lab[i]] + 1]++;
-        for (int c = 0; c < nc; ++c) coff[c + 1] += coff[c];
-        for (int i = 0; i < m; ++i) cy[i] = 0;  // cy reused: fill per cluster
-        for (int i = 0; i < m; ++i) {
-            const int c = cx[lab[i]];
-            cs_out[coff[c] + cy[c]++] = cs_in[i];
-        }
+        for (int c = 0; c <= nc; ++c) s_off[c] = 0;
+        for (int i = 0; i < m; ++i) s_off[cx[lab[i]] + 1]++;
+        for (int c = 0; c < nc; ++c) s_off[c + 1] += s_off[c];
         *ncl = nc;
+        cy[0] = nc;  // cy reused: the count, for the other threads
+    }
+    __syncthreads();
+    const int nc = cy[0];
+    for (int c = t; c <= nc; c += MCL_T) coff[c] = s_off[c];
+    // Stable regrouping: candidate i goes after the earlier ones of its cluster.
+    for (int i = t; i < m; i += MCL_T) {
+        const int r = lab[i];
+        int pos = s_off[cx[r]];
+        for (int j = 0; j < i; ++j) pos += lab[j] == r;
+        cs_out[pos] = cs_in[i];
     }
 }
 
