@@ -315,7 +315,8 @@ ncg_status ncg_mesh_leapfrog_dev(ncg_ctx *ctx, ncg_body *d_bodies, int32_t n, in
  * in the pass's pair-list order and bodies 0..2 and CM_acc in body order (the
  * first exact call after a pass builds id-ordered neighbourhood lists, 108
  * bytes per binned body, reused by later calls of the pass).  NCG_F64_FAST
- * and NCG_MIXED sum per neighbour cell.  n: the slots of the pass plus the
+ * sums per neighbour cell; NCG_MIXED is the same FP64 form here (every
+ * neighbour term is a near one, where FP32 terms lose ~1e-5 to cancellation).  n: the slots of the pass plus the
  * bodies it appended; pairs of absent bodies are skipped (physics.c:351). */
 ncg_status ncg_mesh_gravity(ncg_ctx *ctx, const ncg_body *bodies, int32_t n,
                             const int32_t *exists, const ncg_central *central, double grav_const,

@@ -258,6 +258,47 @@ __global__ void k_adj_begin(double h, int n, const double *__restrict__ Lx,
     }
 }
 
+// Final cotangent (x, v interleaved per body, and the central body's) into
+// the adjoint's separate x and v arrays; thread n takes the central one.
+__global__ void k_adj_split(const double *__restrict__ g, const double *__restrict__ gc, int n,
+                            double *__restrict__ Lx, double *__restrict__ Lv,
+                            double *__restrict__ cL)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        for (int d = 0; d < 3; ++d) {
+            Lx[3 * (size_t)i + d] = g[6 * (size_t)i + d];
+            Lv[3 * (size_t)i + d] = g[6 * (size_t)i + 3 + d];
+        }
+    } else if (i == n) {
+        for (int d = 0; d < 6; ++d) cL[d] = gc ? gc[d] : 0.0;
+    }
+}
+
+// The adjoint at t = 0 back to the interleaved layout, the central body's
+// (x, v) and g_M; thread n takes the central one.
+__global__ void k_adj_merge(const double *__restrict__ Lx, const double *__restrict__ Lv,
+                            const double *__restrict__ cL, const double *__restrict__ gcm, int n,
+                            double *__restrict__ g0, double *__restrict__ gc0)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        for (int d = 0; d < 3; ++d) {
+            g0[6 * (size_t)i + d] = Lx[3 * (size_t)i + d];
+            g0[6 * (size_t)i + 3 + d] = Lv[3 * (size_t)i + d];
+        }
+    } else if (i == n) {
+        for (int d = 0; d < 6; ++d) gc0[d] = cL[d];
+        gc0[6] = gcm[3];
+    }
+}
+
+__global__ void k_adj_zero(double *__restrict__ a, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = 0.0;
+}
+
 __global__ void k_add(double *__restrict__ a, const double *__restrict__ b, int n)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -370,25 +411,11 @@ static ncg_status adjoint_backward(ncg_ctx *ctx, const DBody *ck, const double *
         return NCG_ERR_NOMEM;
     double *cL = cst, *cK = cst + 6, *cA = cst + 12, *gcm = cst + 18, *lamc = cst + 22;
     // Final cotangent: d_gfinal is (x, v) per body, 6n, interleaved x,y,z,vx,vy,vz.
-    std::vector<double> hg(6 * (size_t)n);
-    NCG_HIP(hipMemcpyAsync(hg.data(), d_gfinal, sizeof(double) * 6 * n, hipMemcpyDeviceToHost, st));
-    double hc[6] = {0, 0, 0, 0, 0, 0};
-    if (d_gcentral_final) NCG_HIP(hipMemcpyAsync(hc, d_gcentral_final, sizeof(double) * 6, hipMemcpyDeviceToHost, st));
-    NCG_HIP(hipStreamSynchronize(st));
-    {
-        std::vector<double> hx(3 * (size_t)n), hv(3 * (size_t)n);
-        for (int32_t i = 0; i < n; ++i)
-            for (int d = 0; d < 3; ++d) {
-                hx[3 * (size_t)i + d] = hg[6 * (size_t)i + d];
-                hv[3 * (size_t)i + d] = hg[6 * (size_t)i + 3 + d];
-            }
-        NCG_HIP(hipMemcpyAsync(Lx, hx.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice, st));
-        NCG_HIP(hipMemcpyAsync(Lv, hv.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice, st));
-        NCG_HIP(hipMemcpyAsync(cL, hc, sizeof(double) * 6, hipMemcpyHostToDevice, st));
-        NCG_HIP(hipStreamSynchronize(st));
-    }
-    NCG_HIP(hipMemsetAsync(d_gm, 0, sizeof(double) * n, st));
-    NCG_HIP(hipMemsetAsync(gcm, 0, sizeof(double) * 4, st));
+    hipLaunchKernelGGL(k_adj_split, dim3(cdiv(n + 1, 256)), dim3(256), 0, st, d_gfinal,
+                       d_gcentral_final, (int)n, Lx, Lv, cL);
+    hipLaunchKernelGGL(k_adj_zero, dim3(cdiv(n, 256) + 1), dim3(256), 0, st, d_gm, (int)n);
+    hipLaunchKernelGGL(k_adj_zero, dim3(1), dim3(64), 0, st, gcm, 4);
+    NCG_CHECK_LAUNCH();
     for (int seg = (nsteps - 1) / K; seg >= 0 && nsteps > 0; --seg) {
         const int s0 = seg * K, s1 = std::min(nsteps, s0 + K);
         NCG_HIP(hipMemcpyAsync(y, ck + (size_t)seg * n, sizeof(DBody) * n, hipMemcpyDeviceToDevice, st));
@@ -421,21 +448,9 @@ static ncg_status adjoint_backward(ncg_ctx *ctx, const DBody *ck, const double *
         }
     }
     // Outputs: g0 interleaved (x, v) per body; central (x, v) and g_M.
-    std::vector<double> hx(3 * (size_t)n), hv(3 * (size_t)n), hcl(6), hgm(4);
-    NCG_HIP(hipMemcpyAsync(hx.data(), Lx, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, st));
-    NCG_HIP(hipMemcpyAsync(hv.data(), Lv, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, st));
-    NCG_HIP(hipMemcpyAsync(hcl.data(), cL, sizeof(double) * 6, hipMemcpyDeviceToHost, st));
-    NCG_HIP(hipMemcpyAsync(hgm.data(), gcm, sizeof(double) * 4, hipMemcpyDeviceToHost, st));
-    NCG_HIP(hipStreamSynchronize(st));
-    for (int32_t i = 0; i < n; ++i)
-        for (int d = 0; d < 3; ++d) {
-            hg[6 * (size_t)i + d] = hx[3 * (size_t)i + d];
-            hg[6 * (size_t)i + 3 + d] = hv[3 * (size_t)i + d];
-        }
-    NCG_HIP(hipMemcpyAsync(d_g0, hg.data(), sizeof(double) * 6 * n, hipMemcpyHostToDevice, st));
-    double gc7[7] = {hcl[0], hcl[1], hcl[2], hcl[3], hcl[4], hcl[5], hgm[3]};
-    NCG_HIP(hipMemcpyAsync(d_gcentral0, gc7, sizeof(double) * 7, hipMemcpyHostToDevice, st));
-    NCG_HIP(hipStreamSynchronize(st));
+    hipLaunchKernelGGL(k_adj_merge, dim3(cdiv(n + 1, 256)), dim3(256), 0, st, Lx, Lv, cL, gcm,
+                       (int)n, d_g0, d_gcentral0);
+    NCG_CHECK_LAUNCH();
     return NCG_OK;
 }
 

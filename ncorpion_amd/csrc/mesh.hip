@@ -1091,10 +1091,9 @@ __global__ __launch_bounds__(MT) void k_gweights(const double4 *__restrict__ pm,
 // order to rounding).  The three cells (i-1, i, i+1) of a stencil row have
 // consecutive keys, so their members are one contiguous run of the sorted
 // order: the walk is 9 runs.  The lane's own record enters with weight 0.
-// FORM 1: FP64 with hardware 1/sqrt and a Newton step; 2: separations in
-// FP64, 1/r^3 and forces in FP32, summed in FP32 per stencil row and added
-// to the FP64 total.  SOFT: softening lengths s (R_k + R_p) (else 0).
-template <int FORM, bool SOFT>
+// FP64 with hardware 1/sqrt and a Newton step.  SOFT: softening lengths
+// s (R_k + R_p) (else 0).
+template <bool SOFT>
 __global__ __launch_bounds__(MT) void k_mgrav(const DBody *__restrict__ b, int n,
                                               const int32_t *__restrict__ ex, int ins,
                                               const unsigned *__restrict__ cs,
@@ -1131,26 +1130,7 @@ __global__ __launch_bounds__(MT) void k_mgrav(const DBody *__restrict__ b, int n
         // Empty cells hold [0, 0).
         const unsigned a = e0 > s0 ? s0 : (e1 > s1 ? s1 : s2);
         const unsigned e = e2 > s2 ? e2 : (e1 > s1 ? e1 : e0);
-        if (FORM == 2) {
-            float fx = 0.f, fy = 0.f, fz = 0.f;
-#pragma unroll 2
-            for (unsigned u = a; u < e; ++u) {
-                const double4 p = pw[u];
-                const float w = u == (unsigned)t ? 0.f : (float)p.w;
-                const float dx = (float)(p.x - me.x), dy = (float)(p.y - me.y), dz = (float)(p.z - me.z);
-                const float ee = SOFT ? (float)(s * (Rk + rr[u])) : 0.f;
-                // The floor keeps y^3 finite for the lane's own record (weight 0).
-                const float r2 = fmaxf(fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, ee * ee))), 1e-20f);
-                const float y = __builtin_amdgcn_rsqf(r2);
-                const float f = w * (y * (y * y));
-                fx = fmaf(f, dx, fx);
-                fy = fmaf(f, dy, fy);
-                fz = fmaf(f, dz, fz);
-            }
-            ax += (double)fx;
-            ay += (double)fy;
-            az += (double)fz;
-        } else {
+        {
 #pragma unroll 2
             for (unsigned u = a; u < e; ++u) {
                 const double4 p = pw[u];
@@ -3041,17 +3021,14 @@ static ncg_status mesh_gravity_core(ncg_ctx *ctx, const DBody *db, int32_t n, co
             hipLaunchKernelGGL(k_gweights, gs, dim3(MT), 0, st, (const double4 *)pm, (const int *)sx,
                                ins, G, pw);
         const bool soft = s != 0.0;
-        if (prec == NCG_F64_FAST && soft)
-            hipLaunchKernelGGL((k_mgrav<1, true>), gs, dim3(MT), 0, st, db, (int)n, dex, ins, cs, ce,
-                               pw, rr, sx, sid, skey, (int)d, dcen, G, s, dacc, dct);
-        else if (prec == NCG_F64_FAST)
-            hipLaunchKernelGGL((k_mgrav<1, false>), gs, dim3(MT), 0, st, db, (int)n, dex, ins, cs, ce,
-                               pw, rr, sx, sid, skey, (int)d, dcen, G, s, dacc, dct);
-        else if (soft)
-            hipLaunchKernelGGL((k_mgrav<2, true>), gs, dim3(MT), 0, st, db, (int)n, dex, ins, cs, ce,
+        // NCG_MIXED runs this FP64 form too: every neighbour term is a near
+        // one, and FP32 terms lost ~1e-5 to the cancellation of the sums (2.3e-5
+        // at 2^20 bodies) for 7 % less time.
+        if (soft)
+            hipLaunchKernelGGL((k_mgrav<true>), gs, dim3(MT), 0, st, db, (int)n, dex, ins, cs, ce,
                                pw, rr, sx, sid, skey, (int)d, dcen, G, s, dacc, dct);
         else
-            hipLaunchKernelGGL((k_mgrav<2, false>), gs, dim3(MT), 0, st, db, (int)n, dex, ins, cs, ce,
+            hipLaunchKernelGGL((k_mgrav<false>), gs, dim3(MT), 0, st, db, (int)n, dex, ins, cs, ce,
                                pw, rr, sx, sid, skey, (int)d, dcen, G, s, dacc, dct);
         hipLaunchKernelGGL(k_mgrav_rest, dim3(cdiv(n, MT)), dim3(MT), 0, st, db, (int)n, dex, key,
                            (int)nkey, dcen,

@@ -156,7 +156,9 @@ def test_c2_mesh_pass_collisions_equal_reference(ctx):
 
 def test_c2_mesh_loop_mixed_field_close_to_fp64(ctx):
     """Three steps of the GPU-resident mesh integrator (2^20 bodies,
-    collisions) with the mixed field against the FP64-fast one."""
+    collisions) with the mixed field against the FP64-fast one: on the mesh
+    path the mixed form is the FP64-fast one (every neighbour term is a
+    near one), so the runs agree bit for bit."""
     import torch
     b = c2_workload()
     res = {}
@@ -174,11 +176,5 @@ def test_c2_mesh_loop_mixed_field_close_to_fp64(ctx):
         ctx.synchronize()
         res[prec] = (bt.cpu().numpy(), nc)
     (bf, nf), (bm, nm) = res["fast64"], res["mixed"]
-    assert nf > 100
-    moved = np.abs(bf[:, :3] - b[:, :3]).max()
-    err = np.abs(bm[:, :3] - bf[:, :3]).max()
-    # A few collisions may differ between the forms (they decide on
-    # positions that differ by rounding); everything else agrees closely.
-    close = np.all(np.abs(bm[:, :6] - bf[:, :6]) <= 1e-9 * max(1.0, moved), axis=1)
-    assert close.mean() > 0.999, (close.mean(), err)
-    assert abs(nm - nf) <= max(3, nf // 50)
+    assert nf > 100 and nm == nf
+    assert np.array_equal(bm.view(np.uint64), bf.view(np.uint64))

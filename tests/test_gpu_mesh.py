@@ -78,17 +78,18 @@ def test_mesh_gravity_matches_reference(ctx):
     refn, _ = orc.vector_field_mesh(g["bodies"], g["pairs"], g["exists"], None, float(g["G"]),
                                     float(g["soft"]))
     assert np.array_equal(bits(accn[ex]), bits(refn[ex]))
-    # Mixed form (FP32 forces, per-cell FP32 sums into FP64) on the mutual part:
-    # every neighbour term is a near one, so the FP32 rounding of each term
-    # (~1e-7) is amplified by the cancellation of the neighbour sum.
+    # Mixed form on the mesh path: every neighbour term is a near one, where
+    # FP32 terms lose ~1e-5 to the cancellation of the neighbour sums, so it
+    # runs the FP64-fast form (same bits) and meets the 1e-6 bar by far.
     accm, _ = ctx.mesh_gravity(g["bodies"], g["exists"], None, float(g["G"]), float(g["soft"]),
                                "mixed")
+    accf, _ = ctx.mesh_gravity(g["bodies"], g["exists"], None, float(g["G"]), float(g["soft"]),
+                               "fast64")
+    assert np.array_equal(bits(accm[ex]), bits(accf[ex]))
     dm = np.linalg.norm(accm[ex] - refn[ex], axis=1)
     rn = np.linalg.norm(refn[ex], axis=1)
-    print("mesh mixed: max rel %.3e normwise %.3e" % ((dm / rn).max(),
-                                                      dm.max() / np.sqrt(np.mean(rn * rn))))
-    assert dm.max() / np.sqrt(np.mean(rn * rn)) < 1e-4
-    assert (dm / rn).max() < 1e-4
+    print("mesh mixed: max rel %.3e" % (dm / rn).max())
+    assert (dm / rn).max() < 1e-11
 
 
 def blob(n, seed, half=0.07, rmin=5e-4, rmax=2e-3, vs=0.3):
