@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define NCG_ABI_VERSION 4
+#define NCG_ABI_VERSION 5
 
 typedef struct ncg_ctx ncg_ctx;
 
@@ -387,16 +387,22 @@ ncg_status ncg_leapfrog_dev(ncg_ctx *ctx, ncg_body *d_bodies, int32_t n, const i
  * d_bodies (system s = records offsets[s] .. offsets[s+1]-1; offsets is a HOST
  * array of nsys+1 ints, offsets[0] = 0), each with its own central body
  * (d_centrals: 8 doubles per system, layout of ncg_rk4_steps_dev).  Every
- * force evaluation of all systems is one persistent launch over a queue of
- * (system, body block) work units ordered by cost.
+ * force evaluation of all systems is one launch.  NCG_F64_FAST sums every
+ * body's terms in body order; NCG_MIXED puts each system's bodies in Hilbert
+ * order, cuts them into 128-body tiles, and one workgroup takes one
+ * (system, body group) unit against every tile of its system: FP64 terms
+ * for near tiles, the scaled FP32 far term with FP64 tile anchors for far
+ * ones.
  *
  * ncg_ensemble_rk4_dev advances every system by nsteps RK4 steps of dt.
  * flags bit 0: replay each step from a hipGraph; bit 1: collisions -- each
- * step starts with a brute-force collision pass per system (candidates from
- * the same persistent launch as the stage-1 evaluation), replayed on the
- * host in the reference's lexicographic order (collision.c:425-456) with the
- * library's own restatement of closest_approach and of the elastic /
- * inelastic outcome with parameter collision_f (physics.c:545-605).
+ * step starts with a brute-force collision pass per system (tile pairs that
+ * cannot meet within dt skipped), replayed on the device in the reference's
+ * lexicographic order (collision.c:425-456) with the library's own
+ * restatement of closest_approach and of the elastic / inelastic outcome
+ * with parameter collision_f (physics.c:545-605); a 16-step window whose
+ * replay overflowed its buffers runs again step by step, such passes
+ * replayed on the host.
  * *n_collisions (may be NULL) receives the collisions resolved.
  * ncg_ensemble_gravity_dev evaluates every system once (d_acc: 3 per body,
  * d_cm_acc: 3 per system).  prec: NCG_F64_FAST or NCG_MIXED.
@@ -466,6 +472,111 @@ ncg_status ncg_rk4_adjoint_ckpt_dev(ncg_ctx *ctx, const ncg_body *d_ckpt, int32_
                                     double *d_gcentral0, void *stream);
 
 ncg_status ncg_builtin_replay_ops(ncg_builtin_collider *state, ncg_replay_ops *ops);
+
+/* ---------------------------------------------------------------------------
+ * Collision outcome models of collision_treatment (physics.c:1043-1091):
+ * the bounce (types 0 and 1: collision, physics.c:545-605), the instant
+ * merger (type 2: merger, physics.c:628-707) and the fragmentation model
+ * (type 3: fragmentation, physics.c:710-1040 -- a merger below the ejected
+ * mass threshold, a super-catastrophic disruption, a partial fragmentation
+ * into one tail body, or a full one into n_tilde tail fragments).  The
+ * fields are the reference's compile-time parameters (parameters.h). */
+typedef enum {
+    NCG_OUTCOME_BOUNCE = 0,
+    NCG_OUTCOME_MERGER = 1,
+    NCG_OUTCOME_FRAGMENTATION = 2
+} ncg_outcome_kind;
+
+#define NCG_MAX_TILDE 63
+
+typedef struct {
+    int32_t kind;               /* ncg_outcome_kind */
+    int32_t n_tilde;            /* N_tilde (parameters.h:297), 1 .. NCG_MAX_TILDE */
+    int32_t pq[4];              /* pq_min_max (parameters.h:307):
+                                   (pq[1] - pq[0] + 1) (pq[3] - pq[2] + 1) = n_tilde */
+    double collision_f;         /* bounce: 2 elastic, else collision_parameter */
+    double mu, nu, c1, k;       /* mu_parameter, nu_parameter, C1_parameter, k_parameter
+                                   (parameters.h:298-301) */
+    double merge_threshold;     /* merging_threshold (parameters.h:302) */
+    double frag_threshold;      /* fragment_threshold (parameters.h:303) */
+    double grav_const;          /* G (the escape speed, physics.c:754) */
+} ncg_outcome_model;
+
+/* Built-in ops with an outcome model, on host arrays.  Bodies are removed
+ * and created with the reference's slot rules (lose_moonlet,
+ * structure.c:1077-1087; get_free_index, structure.c:849-875): a removed
+ * slot is pushed on free_indexes, a created body takes the last freed slot
+ * (brute force) or the slot after largest_id (mesh mode, physics.c:994-996).
+ * The decision is that of ncg_builtin_collider (c.exists is pointed at
+ * `exists`).  Fill the public fields, then pass the ops to
+ * ncg_brute_force_pass or ncg_mesh_pass. */
+typedef struct {
+    ncg_builtin_collider c;      /* bodies (n_max records), did_collide, timestep,
+                                    one_collision_only; collision_f is model.collision_f */
+    ncg_outcome_model model;
+    int32_t *exists;             /* n_max entries, updated by the outcomes */
+    int32_t *free_indexes;       /* n_max entries: the stack of free slots */
+    double *central;             /* 8 doubles {x, y, z, vx, vy, vz, M, present} or NULL: a
+                                    disruption's discarded mass is added to M when present
+                                    (physics.c:826-829) */
+    int32_t n_max;               /* slots of bodies / exists / did_collide (N_max) */
+    int32_t largest_id;          /* in/out */
+    int32_t how_many_free;       /* in/out */
+    int32_t mesh;                /* mesh mode (fragments appended, slots 0..2 never reused) */
+    int32_t need_to_reduce_com;  /* out: set by a super-catastrophic disruption (physics.c:815) */
+    int32_t error;               /* out: 1 N_max exceeded, 2 an ejection speed below the escape
+                                    speed (where the reference aborts; the pass's state is then
+                                    not valid) */
+    int64_t mergers, super_catastrophic, partial, full;  /* out: outcome counts */
+    int32_t created_[NCG_MAX_TILDE + 1];
+    int32_t n_created_;
+    int32_t pad_;
+} ncg_outcome_state;
+
+ncg_status ncg_builtin_outcome_ops(ncg_outcome_state *state, ncg_replay_ops *ops);
+
+/* GPU-resident brute-force leapfrog (ncg_leapfrog_dev) with an outcome
+ * model and end_of_timestep's removals (rk4.c:155-221, 348-353, 395-397):
+ * after each step, bodies closer to the central body than r_disruption are
+ * merged into it (central_mass_bool: d_central[7] != 0) and bodies farther
+ * than r_high_dumping are removed; the centre of mass is reduced when a
+ * super-catastrophic disruption or a removal asked for it (reduce_to_com);
+ * did_collide is cleared (one_collision_only or the fragmentation model);
+ * and the array is compacted (tidy_up, structure.c:881-911) when fewer than
+ * 9/10 of largest_id slots hold bodies.  Steps without such events run on the
+ * device as in ncg_leapfrog_dev (64-step windows, graph replay with flags
+ * bit 0); a window in which a collision pass under the merger or
+ * fragmentation model resolved a pair, or a body crossed a threshold, runs
+ * again from its snapshot step by step, and each such pass or step end is
+ * replayed on the host with the reference's arithmetic (the outcome's `pow`
+ * is the host's).  d_bodies, d_exists and d_did_collide (may be NULL unless
+ * one_collision_only) hold n_max slots; the call works on slots
+ * 0 .. largest_id and updates largest_id, how_many_free and free_indexes.
+ * flags: as ncg_leapfrog_dev.  No counterpart as one call in NcorpiON. */
+typedef struct {
+    ncg_outcome_model model;
+    double r_disruption;         /* disruption_threshold; <= 0: none */
+    double r_high_dumping;       /* high_dumping_threshold; <= 0: none */
+    int32_t reduce_to_com;       /* reduce_to_COM_bool */
+    int32_t n_max;               /* slots of the device arrays and of free_indexes (N_max) */
+    int32_t largest_id;          /* in/out */
+    int32_t how_many_free;       /* in/out */
+    int32_t *free_indexes;       /* host array of n_max, in/out */
+    int32_t need_to_reduce_com;  /* in/out (need_to_reduce_COM_bool) */
+    int32_t host_passes;         /* out: collision passes replayed on the host */
+    int32_t host_step_ends;      /* out: step ends (removals, reduction, tidy_up) on the host */
+    int32_t removed;             /* out: bodies removed by the thresholds */
+    int32_t tidy_ups;            /* out */
+    int32_t pad_;
+    int64_t mergers, super_catastrophic, partial, full;  /* out: outcome counts of the call */
+} ncg_leapfrog_model_params;
+
+ncg_status ncg_leapfrog_model_dev(ncg_ctx *ctx, ncg_body *d_bodies, int32_t *d_exists,
+                                  int32_t *d_did_collide, double *d_central, double grav_const,
+                                  double softening, ncg_precision prec, double dt,
+                                  int32_t nsteps, int32_t one_collision_only, int32_t flags,
+                                  ncg_leapfrog_model_params *params, int64_t *n_collisions,
+                                  void *stream);
 
 /* ---------------------------------------------------------------------------
  * Total energy (FP64) of bodies d_bodies (n, d_exists may be NULL) and the

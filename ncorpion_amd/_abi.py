@@ -78,6 +78,46 @@ class BuiltinOps:
         return int(self.state.collisions)
 
 
+OUTCOMES = {"bounce": 0, "merger": 1, "fragmentation": 2}
+MAX_TILDE = 63
+
+
+class OutcomeModel(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("n_tilde", ctypes.c_int32), ("pq", ctypes.c_int32 * 4),
+                ("collision_f", ctypes.c_double), ("mu", ctypes.c_double), ("nu", ctypes.c_double),
+                ("c1", ctypes.c_double), ("k", ctypes.c_double),
+                ("merge_threshold", ctypes.c_double), ("frag_threshold", ctypes.c_double),
+                ("grav_const", ctypes.c_double)]
+
+
+def outcome_model(kind, collision_f=1.5, G=1.0, n_tilde=15, pq=(-1, 3, -1, 1), mu=0.55, nu=0.4,
+                  c1=1.5, k=0.2, merging_threshold=0.0002, fragment_threshold=1.5e-8):
+    """An OutcomeModel; the defaults are NcorpiON's parameters.h values."""
+    m = OutcomeModel()
+    m.kind = OUTCOMES[kind]
+    m.n_tilde = int(n_tilde)
+    for q in range(4):
+        m.pq[q] = int(pq[q])
+    m.collision_f = float(collision_f)
+    m.mu, m.nu, m.c1, m.k = float(mu), float(nu), float(c1), float(k)
+    m.merge_threshold = float(merging_threshold)
+    m.frag_threshold = float(fragment_threshold)
+    m.grav_const = float(G)
+    return m
+
+
+class LeapfrogModelParams(ctypes.Structure):
+    _fields_ = [("model", OutcomeModel), ("r_disruption", ctypes.c_double),
+                ("r_high_dumping", ctypes.c_double), ("reduce_to_com", ctypes.c_int32),
+                ("n_max", ctypes.c_int32), ("largest_id", ctypes.c_int32),
+                ("how_many_free", ctypes.c_int32), ("free_indexes", ctypes.c_void_p),
+                ("need_to_reduce_com", ctypes.c_int32), ("host_passes", ctypes.c_int32),
+                ("host_step_ends", ctypes.c_int32), ("removed", ctypes.c_int32),
+                ("tidy_ups", ctypes.c_int32), ("pad_", ctypes.c_int32),
+                ("mergers", ctypes.c_int64), ("super_catastrophic", ctypes.c_int64),
+                ("partial", ctypes.c_int64), ("full", ctypes.c_int64)]
+
+
 class MeshStats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int64) for n in ("total_neighbours", "how_many_small", "how_many_big",
                                              "collisions", "candidates", "host_tests",
@@ -124,9 +164,12 @@ _SIGNATURES = {
     "ncg_energy_dev": (S, [P, P, I32, P, P, D, D, P, P]),
     "ncg_leapfrog_dev": (S, [P, P, I32, P, P, P, D, D, S, D, I32, D, I32, I32,
                              ctypes.POINTER(I64), P]),
+    "ncg_leapfrog_model_dev": (S, [P, P, P, P, P, D, D, S, D, I32, I32, I32,
+                                   ctypes.POINTER(LeapfrogModelParams), ctypes.POINTER(I64), P]),
     "ncg_rk4_forward_ckpt_dev": (S, [P, P, I32, P, D, D, D, I32, I32, P, P, P]),
     "ncg_rk4_adjoint_ckpt_dev": (S, [P, P, I32, P, D, D, D, I32, I32, P, P, P, P, P, P]),
     "ncg_builtin_replay_ops": (S, [ctypes.POINTER(BuiltinCollider), ctypes.POINTER(ReplayOps)]),
+    "ncg_builtin_outcome_ops": (S, [P, ctypes.POINTER(ReplayOps)]),
     "ncg_format_fixed": (S, [P, P, I64, I32, P, I64, ctypes.POINTER(I64)]),
     "ncg_cart2ell": (S, [P, P, I32, P, P, D, D, P, P]),
     "ncg_profile_enable": (S, [P, S]),
@@ -314,6 +357,45 @@ class Context:
             None if did_collide_t is None else did_collide_t.data_ptr(), central_t.data_ptr(),
             float(G), float(softening), PRECISIONS[precision], float(dt), int(nsteps),
             float(collision_f), int(one_collision_only), flags, ctypes.byref(nc), stream))
+        return int(nc.value)
+
+    def leapfrog_model_dev(self, bodies_t, central_t, exists_t, nsteps, dt, state, model,
+                           G=1.0, softening=0.0, precision="exact", one_collision_only=0,
+                           did_collide_t=None, r_disruption=0.0, r_high_dumping=0.0,
+                           reduce_to_com=1, graph=False, half_drift_start=False,
+                           half_drift_end=False, stream=None):
+        """NcorpiON's brute-force leapfrog with an outcome model and
+        end_of_timestep's removals (see ncg_leapfrog_model_dev).  bodies_t,
+        exists_t, did_collide_t: device tensors of n_max slots.  state: dict
+        with largest_id, how_many_free, free_indexes (int32 numpy array of
+        n_max) and need_to_reduce_com, updated in place; the call's counters
+        are added under their names.  Returns the collisions resolved."""
+        n_max = int(bodies_t.shape[0])
+        free = state["free_indexes"]
+        assert free.dtype == np.int32 and free.shape[0] == n_max and free.flags["C_CONTIGUOUS"]
+        mp = LeapfrogModelParams()
+        mp.model = model
+        mp.r_disruption = float(r_disruption)
+        mp.r_high_dumping = float(r_high_dumping)
+        mp.reduce_to_com = int(reduce_to_com)
+        mp.n_max = n_max
+        mp.largest_id = int(state["largest_id"])
+        mp.how_many_free = int(state["how_many_free"])
+        mp.free_indexes = free.ctypes.data
+        mp.need_to_reduce_com = int(state.get("need_to_reduce_com", 0))
+        nc = I64()
+        flags = (1 if graph else 0) | (2 if half_drift_start else 0) | (4 if half_drift_end else 0)
+        self._check(self.lib.ncg_leapfrog_model_dev(
+            self.h, bodies_t.data_ptr(), exists_t.data_ptr(),
+            None if did_collide_t is None else did_collide_t.data_ptr(), central_t.data_ptr(),
+            float(G), float(softening), PRECISIONS[precision], float(dt), int(nsteps),
+            int(one_collision_only), flags, ctypes.byref(mp), ctypes.byref(nc), stream))
+        state["largest_id"] = int(mp.largest_id)
+        state["how_many_free"] = int(mp.how_many_free)
+        state["need_to_reduce_com"] = int(mp.need_to_reduce_com)
+        for f in ("host_passes", "host_step_ends", "removed", "tidy_ups", "mergers",
+                  "super_catastrophic", "partial", "full"):
+            state[f] = state.get(f, 0) + int(getattr(mp, f))
         return int(nc.value)
 
     def mesh_pass_dev(self, bodies_t, dt, gam, cube, cells, mode, collision_f=1.5,

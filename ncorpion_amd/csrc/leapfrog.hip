@@ -477,46 +477,90 @@ extern "C" ncg_status ncg_mesh_leapfrog_dev(ncg_ctx *ctx, ncg_body *d_bodies, in
     return NCG_OK;
 }
 
-extern "C" ncg_status ncg_leapfrog_dev(ncg_ctx *ctx, ncg_body *d_bodies, int32_t n,
-                                       const int32_t *d_exists, int32_t *d_did_collide,
-                                       double *d_central, double grav_const, double softening,
-                                       ncg_precision prec, double dt, int32_t nsteps,
-                                       double collision_f, int32_t one_collision_only,
-                                       int32_t flags, int64_t *n_collisions, void *stream)
+namespace {
+
+// end_of_timestep's removal tests (rk4.c:158-179): sets *flag when an existing
+// body j < n is closer to the central body than r_dis (central body present)
+// or farther than r_hd from it (from the origin without one).
+__global__ __launch_bounds__(LT) void k_lf_thresholds(int n, const int32_t *__restrict__ ex,
+                                                      const DBody *__restrict__ b,
+                                                      const double *__restrict__ cen,
+                                                      double r_dis, double r_hd,
+                                                      unsigned *__restrict__ flag)
 {
-    if (!ctx) return NCG_ERR_INVALID;
-    NCG_REQUIRE(n >= 1 && d_bodies && d_central && nsteps >= 0, "invalid arguments");
-    NCG_REQUIRE(prec == NCG_F64_EXACT || prec == NCG_F64_FAST || prec == NCG_MIXED,
-                "unknown precision %d", (int)prec);
-    NCG_REQUIRE(!(collision_f > 0.0 && one_collision_only && !d_did_collide),
-                "one_collision_only needs a did_collide array");
+    const int i = blockIdx.x * LT + threadIdx.x;
+    if (i >= n || (ex && !ex[i])) return;
+    const bool cm = cen[7] != 0.0;
+    const double DX = b[i].x - (cm ? cen[0] : 0.);
+    const double DY = b[i].y - (cm ? cen[1] : 0.);
+    const double DZ = b[i].z - (cm ? cen[2] : 0.);
+    const double d2 = DX * DX + DY * DY + DZ * DZ;
+    if ((d2 < r_dis * r_dis && cm) || d2 > r_hd * r_hd) atomicOr(flag, 1u);
+}
+
+// Host-side state of a run with an outcome model (slots and flags of the
+// reference's globals), kept in window snapshots.
+struct ModelHost {
+    int32_t largest_id = 0, how_many_free = 0, need = 0, live = 0;
+    std::vector<int32_t> free;
+    int32_t host_passes = 0, host_step_ends = 0, removed = 0, tidy_ups = 0;
+    int64_t mergers = 0, super_catastrophic = 0, partial = 0, full = 0, coll = 0;
+};
+
+// The brute-force loop; mp == nullptr: ncg_leapfrog_dev (bounce with
+// collision_f, fixed slots); else ncg_leapfrog_model_dev.
+ncg_status leapfrog_run(ncg_ctx *ctx, ncg_body *d_bodies, int32_t n_in, int32_t *d_exists_w,
+                        const int32_t *d_exists, int32_t *d_did_collide, double *d_central,
+                        double grav_const, double softening, ncg_precision prec, double dt,
+                        int32_t nsteps, double collision_f, int32_t one_collision_only,
+                        int32_t flags, ncg_leapfrog_model_params *mp, int64_t *n_collisions,
+                        void *stream)
+{
     NCG_HIP(hipSetDevice(ctx->device));
     hipStream_t st = pick_stream(ctx, stream);
     if (n_collisions) *n_collisions = 0;
     DBody *b = reinterpret_cast<DBody *>(d_bodies);
     const int32_t *ex = d_exists;
-    const bool coll = collision_f > 0.0;
+    const bool model_host = mp && mp->model.kind != NCG_OUTCOME_BOUNCE;
+    const bool frag = mp && mp->model.kind == NCG_OUTCOME_FRAGMENTATION;
+    // The device replay applies the bounce; under the other models any pass
+    // it resolves a pair in is replayed on the host, so its f only needs to
+    // be a valid restitution.
+    const double f = model_host ? 2.0 : (mp ? mp->model.collision_f : collision_f);
+    const bool coll = model_host || f > 0.0;
+    const bool thr = mp && (mp->r_disruption > 0.0 || mp->r_high_dumping > 0.0);
+    const double r_dis = mp && mp->r_disruption > 0.0 ? mp->r_disruption : 0.0;
+    const double r_hd = mp && mp->r_high_dumping > 0.0 ? mp->r_high_dumping : INFINITY;
+    const bool clear_dc = coll && d_did_collide && (one_collision_only || frag);
+    const int32_t ncap = mp ? mp->n_max : n_in;
+    int32_t n = n_in;
     const bool use_graph = (flags & 1) != 0;
     const unsigned cap = ncg_replay_cap(ctx, 1u << 20);
-    double *acc = dbuf<double>(ctx, "lf_acc", 3 * (size_t)n);
-    double *cterm = dbuf<double>(ctx, "lf_cterm", 3 * (size_t)n);
+    double *acc = dbuf<double>(ctx, "lf_acc", 3 * (size_t)ncap);
+    double *cterm = dbuf<double>(ctx, "lf_cterm", 3 * (size_t)ncap);
     double *cm = dbuf<double>(ctx, "lf_cm", 4);
     int2 *cand = dbuf<int2>(ctx, "lf_cand", cap);
     unsigned long long *cnt = dbuf<unsigned long long>(ctx, "lf_cnt", 1);
-    unsigned char *dirty = dbuf<unsigned char>(ctx, "lf_dirty", (size_t)n + 4);
-    unsigned *stat = dbuf<unsigned>(ctx, "lf_stat", 2);
+    unsigned char *dirty = dbuf<unsigned char>(ctx, "lf_dirty", (size_t)ncap + 4);
+    unsigned *stat = dbuf<unsigned>(ctx, "lf_stat", 4);
     int *off = dbuf<int>(ctx, "lf_off", 2);
     if (!acc || !cterm || !cm || !cand || !cnt || !dirty || !stat || !off) return NCG_ERR_NOMEM;
     double present = 0.0;
     NCG_HIP(hipMemcpyAsync(&present, d_central + 7, sizeof(double), hipMemcpyDeviceToHost, st));
     NCG_HIP(hipStreamSynchronize(st));
     const bool cp = present != 0.0;
-    hipLaunchKernelGGL(k_lf_off, dim3(1), dim3(1), 0, st, off, (int)n);
-    hipLaunchKernelGGL(k_lf_zero_i32, dim3(cdiv(n / 4 + 2, 256)), dim3(256), 0, st,
-                       reinterpret_cast<int32_t *>(dirty), (int)(n / 4 + 1));
-    hipLaunchKernelGGL(k_lf_zero_i32, dim3(1), dim3(64), 0, st, reinterpret_cast<int32_t *>(stat), 2);
+    hipLaunchKernelGGL(k_lf_zero_i32, dim3(cdiv(ncap / 4 + 2, 256)), dim3(256), 0, st,
+                       reinterpret_cast<int32_t *>(dirty), (int)(ncap / 4 + 1));
+    hipLaunchKernelGGL(k_lf_zero_i32, dim3(1), dim3(64), 0, st, reinterpret_cast<int32_t *>(stat), 3);
     NCG_CHECK_LAUNCH();
-    const unsigned g = (unsigned)cdiv(n + 1, LT);
+    unsigned g = 0;
+    auto set_n = [&](int32_t nn) {
+        n = nn;
+        g = (unsigned)cdiv(n + 1, LT);
+        hipLaunchKernelGGL(k_lf_off, dim3(1), dim3(1), 0, st, off, (int)n);
+    };
+    set_n(n);
+    NCG_CHECK_LAUNCH();
     const double G = grav_const, soft = softening;
 
     auto collide = [&](double h) -> ncg_status {
@@ -524,45 +568,67 @@ extern "C" ncg_status ncg_leapfrog_dev(ncg_ctx *ctx, ncg_body *d_bodies, int32_t
         ncg_status r = ncg_candidates_launch(ctx, b, n, ex, h, cand, cap, cnt, st);
         if (r != NCG_OK) return r;
         return ncg_bf_replay_launch(ctx, b, ex, d_did_collide, one_collision_only, off, 1, cand,
-                                    nullptr, cnt, cap, h, collision_f, dirty, stat, st);
+                                    nullptr, cnt, cap, h, f, dirty, stat, st);
     };
     // The replay's capacity (candidates, changed bodies in LDS) is checked
-    // every LF_CHECK steps.  A window in which a pass exceeded it is run
+    // every LF_CHECK steps.  A window in which a pass exceeded it -- or, with
+    // an outcome model, in which a pass resolved a pair under the merger or
+    // fragmentation model or a body crossed a removal threshold -- is run
     // again from a snapshot of the state taken at the window's start, step
-    // by step, each collision pass checked at once: a pass that exceeds the
-    // capacity is undone (a snapshot before it) and replayed on the host with
-    // the same decision and bounce (ncg_brute_force_pass with the built-in
-    // ops), so results stay those of an unbounded replay.
+    // by step, each collision pass and step end checked at once: such a pass
+    // is undone (a snapshot before it) and replayed on the host with the
+    // same decision and the outcome (ncg_brute_force_pass with the built-in
+    // ops), such a step end runs end_of_timestep's host half, so results are
+    // those of the reference's loop.
     constexpr int LF_CHECK = 64;
-    DBody *snap_b = dbuf<DBody>(ctx, "lf_snap_b", n);
-    DBody *pass_b = dbuf<DBody>(ctx, "lf_pass_b", n);
+    DBody *snap_b = dbuf<DBody>(ctx, "lf_snap_b", ncap);
+    DBody *pass_b = dbuf<DBody>(ctx, "lf_pass_b", ncap);
     double *snap_c = dbuf<double>(ctx, "lf_snap_c", 8);
-    int32_t *snap_dc = d_did_collide ? dbuf<int32_t>(ctx, "lf_snap_dc", n) : nullptr;
-    int32_t *pass_dc = d_did_collide ? dbuf<int32_t>(ctx, "lf_pass_dc", n) : nullptr;
-    unsigned *hst = pbuf<unsigned>(ctx, "lf_hstat", 4);
-    if (!snap_b || !pass_b || !snap_c || (d_did_collide && (!snap_dc || !pass_dc)) || !hst)
+    int32_t *snap_dc = d_did_collide ? dbuf<int32_t>(ctx, "lf_snap_dc", ncap) : nullptr;
+    int32_t *pass_dc = d_did_collide ? dbuf<int32_t>(ctx, "lf_pass_dc", ncap) : nullptr;
+    int32_t *snap_ex = mp ? dbuf<int32_t>(ctx, "lf_snap_ex", ncap) : nullptr;
+    unsigned *hst = pbuf<unsigned>(ctx, "lf_hstat", 8);
+    if (!snap_b || !pass_b || !snap_c || (d_did_collide && (!snap_dc || !pass_dc)) ||
+        (mp && !snap_ex) || !hst)
         return NCG_ERR_NOMEM;
-    int64_t host_coll = 0;      // collisions of passes replayed on the host
+    int64_t host_coll = 0;      // collisions of passes replayed on the host (bounce only)
     unsigned snap_coll = 0;     // device collision count at the snapshot
+    ModelHost mh, snap_mh;      // model runs: host state now and at the snapshot
+    std::vector<DBody> hb;
+    std::vector<int32_t> hex, hdc;
+    double hcen[8];
+    if (mp) {
+        mh.largest_id = mp->largest_id;
+        mh.how_many_free = mp->how_many_free;
+        mh.need = mp->need_to_reduce_com;
+        mh.free.assign(mp->free_indexes, mp->free_indexes + ncap);
+        hex.assign((size_t)ncap, 0);
+        NCG_HIP(hipMemcpyAsync(hex.data(), d_exists_w, sizeof(int32_t) * (size_t)n,
+                               hipMemcpyDeviceToHost, st));
+        NCG_HIP(hipStreamSynchronize(st));
+        for (int32_t j = 0; j < n; ++j) mh.live += hex[j] != 0;
+    }
     auto read_stat = [&](unsigned *hs) -> ncg_status {
-        NCG_HIP(hipMemcpyAsync(hst, stat, sizeof(unsigned) * 2, hipMemcpyDeviceToHost, st));
+        NCG_HIP(hipMemcpyAsync(hst, stat, sizeof(unsigned) * 3, hipMemcpyDeviceToHost, st));
         NCG_HIP(hipStreamSynchronize(st));
         hs[0] = hst[0];
         hs[1] = hst[1];
+        hs[2] = hst[2];
         return NCG_OK;
     };
-    auto write_stat = [&](unsigned coll) -> ncg_status {
-        hst[2] = coll;
-        hst[3] = 0u;
-        NCG_HIP(hipMemcpyAsync(stat, hst + 2, sizeof(unsigned) * 2, hipMemcpyHostToDevice, st));
+    auto write_stat = [&](unsigned coll_count) -> ncg_status {
+        hst[4] = coll_count;
+        hst[5] = 0u;
+        hst[6] = 0u;
+        NCG_HIP(hipMemcpyAsync(stat, hst + 4, sizeof(unsigned) * 3, hipMemcpyHostToDevice, st));
         NCG_HIP(hipStreamSynchronize(st));
         return NCG_OK;
     };
     auto copy_state = [&](DBody *to_b, int32_t *to_dc, double *to_c, const DBody *from_b,
-                          const int32_t *from_dc, const double *from_c) -> ncg_status {
-        NCG_HIP(hipMemcpyAsync(to_b, from_b, sizeof(DBody) * (size_t)n, hipMemcpyDeviceToDevice, st));
+                          const int32_t *from_dc, const double *from_c, int32_t slots) -> ncg_status {
+        NCG_HIP(hipMemcpyAsync(to_b, from_b, sizeof(DBody) * (size_t)slots, hipMemcpyDeviceToDevice, st));
         if (to_dc && from_dc)
-            NCG_HIP(hipMemcpyAsync(to_dc, from_dc, sizeof(int32_t) * (size_t)n,
+            NCG_HIP(hipMemcpyAsync(to_dc, from_dc, sizeof(int32_t) * (size_t)slots,
                                    hipMemcpyDeviceToDevice, st));
         if (to_c && from_c)
             NCG_HIP(hipMemcpyAsync(to_c, from_c, sizeof(double) * 8, hipMemcpyDeviceToDevice, st));
@@ -571,27 +637,19 @@ extern "C" ncg_status ncg_leapfrog_dev(ncg_ctx *ctx, ncg_body *d_bodies, int32_t
     // A replay that ran out of room for changed bodies leaves some of their
     // marks set.
     auto clear_dirty = [&]() -> ncg_status {
-        hipLaunchKernelGGL(k_lf_zero_i32, dim3(cdiv(n / 4 + 2, 256)), dim3(256), 0, st,
-                           reinterpret_cast<int32_t *>(dirty), (int)(n / 4 + 1));
+        hipLaunchKernelGGL(k_lf_zero_i32, dim3(cdiv(ncap / 4 + 2, 256)), dim3(256), 0, st,
+                           reinterpret_cast<int32_t *>(dirty), (int)(ncap / 4 + 1));
         NCG_CHECK_LAUNCH();
         return NCG_OK;
     };
-    // A collision pass with dt h, checked: on a capacity error the pass is
-    // undone and replayed on the host.
-    auto collide_checked = [&](double h) -> ncg_status {
-        unsigned hs[2];
-        ncg_status r = read_stat(hs);
-        if (r != NCG_OK) return r;
-        const unsigned before = hs[0];
-        if ((r = copy_state(pass_b, pass_dc, nullptr, b, d_did_collide, nullptr)) != NCG_OK) return r;
-        if ((r = collide(h)) != NCG_OK) return r;
-        if ((r = read_stat(hs)) != NCG_OK) return r;
-        if (hs[1] == 0u) return NCG_OK;
-        if ((r = copy_state(b, d_did_collide, nullptr, pass_b, pass_dc, nullptr)) != NCG_OK) return r;
-        if ((r = clear_dirty()) != NCG_OK) return r;
-        if ((r = write_stat(before)) != NCG_OK) return r;
-        std::vector<DBody> hb((size_t)n);
-        std::vector<int32_t> hex(ex ? (size_t)n : 0), hdc(d_did_collide ? (size_t)n : 0);
+    // Host copies of the state (slots 0 .. n - 1; exists and did_collide
+    // zero beyond), and back (`slots` slots of exists, which covers the slots
+    // a compaction emptied).
+    auto download = [&]() -> ncg_status {
+        hb.assign((size_t)ncap, DBody{});
+        if (!mp) hex.assign(ex ? (size_t)n : 0, 0);
+        else std::fill(hex.begin(), hex.end(), 0);
+        hdc.assign(d_did_collide ? (size_t)ncap : 0, 0);
         NCG_HIP(hipMemcpyAsync(hb.data(), b, sizeof(DBody) * (size_t)n, hipMemcpyDeviceToHost, st));
         if (ex)
             NCG_HIP(hipMemcpyAsync(hex.data(), ex, sizeof(int32_t) * (size_t)n,
@@ -599,24 +657,136 @@ extern "C" ncg_status ncg_leapfrog_dev(ncg_ctx *ctx, ncg_body *d_bodies, int32_t
         if (d_did_collide)
             NCG_HIP(hipMemcpyAsync(hdc.data(), d_did_collide, sizeof(int32_t) * (size_t)n,
                                    hipMemcpyDeviceToHost, st));
+        NCG_HIP(hipMemcpyAsync(hcen, d_central, sizeof hcen, hipMemcpyDeviceToHost, st));
         NCG_HIP(hipStreamSynchronize(st));
-        ncg_builtin_collider bc = {};
-        bc.bodies = reinterpret_cast<ncg_body *>(hb.data());
-        bc.exists = ex ? hex.data() : nullptr;
-        bc.did_collide = d_did_collide ? hdc.data() : nullptr;
-        bc.timestep = h;
-        bc.collision_f = collision_f;
-        bc.one_collision_only = one_collision_only;
-        ncg_replay_ops ops;
-        if ((r = ncg_builtin_replay_ops(&bc, &ops)) != NCG_OK) return r;
-        if ((r = ncg_brute_force_pass(ctx, bc.bodies, n, bc.exists, h, &ops, nullptr)) != NCG_OK)
-            return r;
-        host_coll += bc.collisions;
-        NCG_HIP(hipMemcpyAsync(b, hb.data(), sizeof(DBody) * (size_t)n, hipMemcpyHostToDevice, st));
-        if (d_did_collide)
-            NCG_HIP(hipMemcpyAsync(d_did_collide, hdc.data(), sizeof(int32_t) * (size_t)n,
+        return NCG_OK;
+    };
+    auto upload = [&](int32_t slots) -> ncg_status {
+        NCG_HIP(hipMemcpyAsync(b, hb.data(), sizeof(DBody) * (size_t)slots, hipMemcpyHostToDevice, st));
+        if (mp)
+            NCG_HIP(hipMemcpyAsync(d_exists_w, hex.data(), sizeof(int32_t) * (size_t)slots,
                                    hipMemcpyHostToDevice, st));
+        if (d_did_collide)
+            NCG_HIP(hipMemcpyAsync(d_did_collide, hdc.data(), sizeof(int32_t) * (size_t)slots,
+                                   hipMemcpyHostToDevice, st));
+        NCG_HIP(hipMemcpyAsync(d_central, hcen, sizeof hcen, hipMemcpyHostToDevice, st));
         NCG_HIP(hipStreamSynchronize(st));
+        return NCG_OK;
+    };
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+    bool need_eager = true;  // the next unchecked step runs eagerly (and allocates)
+    auto drop_graph = [&]() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+        need_eager = true;
+    };
+    // A collision pass with dt h, checked: on a capacity error (or, under the
+    // merger and fragmentation models, a resolved pair) the pass is undone
+    // and replayed on the host.
+    auto collide_checked = [&](double h) -> ncg_status {
+        unsigned hs[3];
+        ncg_status r = read_stat(hs);
+        if (r != NCG_OK) return r;
+        const unsigned before = hs[0];
+        if ((r = copy_state(pass_b, pass_dc, nullptr, b, d_did_collide, nullptr, n)) != NCG_OK) return r;
+        if ((r = collide(h)) != NCG_OK) return r;
+        if ((r = read_stat(hs)) != NCG_OK) return r;
+        if (hs[1] == 0u && !(model_host && hs[0] != before)) return NCG_OK;
+        if ((r = copy_state(b, d_did_collide, nullptr, pass_b, pass_dc, nullptr, n)) != NCG_OK) return r;
+        if ((r = clear_dirty()) != NCG_OK) return r;
+        if ((r = write_stat(before)) != NCG_OK) return r;
+        if ((r = download()) != NCG_OK) return r;
+        ncg_replay_ops ops;
+        if (!mp) {
+            ncg_builtin_collider bc = {};
+            bc.bodies = reinterpret_cast<ncg_body *>(hb.data());
+            bc.exists = ex ? hex.data() : nullptr;
+            bc.did_collide = d_did_collide ? hdc.data() : nullptr;
+            bc.timestep = h;
+            bc.collision_f = f;
+            bc.one_collision_only = one_collision_only;
+            if ((r = ncg_builtin_replay_ops(&bc, &ops)) != NCG_OK) return r;
+            if ((r = ncg_brute_force_pass(ctx, bc.bodies, n, bc.exists, h, &ops, nullptr)) != NCG_OK)
+                return r;
+            host_coll += bc.collisions;
+            return upload(n);
+        }
+        ncg_outcome_state os = {};
+        os.c.bodies = reinterpret_cast<ncg_body *>(hb.data());
+        os.c.did_collide = d_did_collide ? hdc.data() : nullptr;
+        os.c.timestep = h;
+        os.c.one_collision_only = one_collision_only;
+        os.model = mp->model;
+        os.exists = hex.data();
+        os.free_indexes = mh.free.data();
+        os.central = hcen;
+        os.n_max = ncap;
+        os.largest_id = mh.largest_id;
+        os.how_many_free = mh.how_many_free;
+        os.need_to_reduce_com = mh.need;
+        if ((r = ncg_builtin_outcome_ops(&os, &ops)) != NCG_OK)
+            return ncg_fail(ctx, r, "invalid outcome model");
+        if ((r = ncg_brute_force_pass(ctx, os.c.bodies, n, os.exists, h, &ops, nullptr)) != NCG_OK)
+            return r;
+        if (os.error)
+            return ncg_fail(ctx, NCG_ERR_INVALID,
+                            os.error == 1 ? "a fragmentation needs more than n_max = %d slots"
+                                          : "an ejection speed fell below the escape speed "
+                                            "(n_max %d); NcorpiON aborts here",
+                            (int)ncap);
+        mh.largest_id = os.largest_id;
+        mh.how_many_free = os.how_many_free;
+        mh.need = os.need_to_reduce_com;
+        mh.coll += os.c.collisions;
+        mh.mergers += os.mergers;
+        mh.super_catastrophic += os.super_catastrophic;
+        mh.partial += os.partial;
+        mh.full += os.full;
+        ++mh.host_passes;
+        const int32_t nn = mh.largest_id + 1;
+        mh.live = 0;
+        for (int32_t j = 0; j < nn; ++j) mh.live += hex[j] != 0;
+        if ((r = upload(std::max(n, nn))) != NCG_OK) return r;
+        if (nn != n) {
+            set_n(nn);
+            drop_graph();
+        }
+        return NCG_OK;
+    };
+    // end_of_timestep's host half (removals, reduction of the centre of
+    // mass, tidy_up) when a body crossed a threshold or the host state asks
+    // for it.
+    auto tidy_due = [&]() { return mh.live < 9 * mh.largest_id / 10; };
+    auto end_checked = [&]() -> ncg_status {
+        if (!mp) return NCG_OK;
+        bool due = (mh.need && mp->reduce_to_com) || tidy_due();
+        ncg_status r;
+        if (thr) {
+            unsigned hs[3];
+            if ((r = read_stat(hs)) != NCG_OK) return r;
+            due = due || hs[2] != 0u;
+        }
+        if (!due) return NCG_OK;
+        if ((r = download()) != NCG_OK) return r;
+        const int32_t n_before = n;
+        int32_t lid = mh.largest_id, hmf = mh.how_many_free;
+        mh.live = ncg_host_step_end(reinterpret_cast<ncg_body *>(hb.data()), hex.data(),
+                                    mh.free.data(), lid, hmf, hcen, r_dis, r_hd,
+                                    mp->reduce_to_com, mh.need, mh.removed, mh.tidy_ups);
+        mh.largest_id = lid;
+        mh.how_many_free = hmf;
+        ++mh.host_step_ends;
+        if ((r = upload(n_before)) != NCG_OK) return r;
+        hst[7] = 0u;
+        NCG_HIP(hipMemcpyAsync(stat + 2, hst + 7, sizeof(unsigned), hipMemcpyHostToDevice, st));
+        NCG_HIP(hipStreamSynchronize(st));
+        if (lid + 1 != n) {
+            set_n(lid + 1);
+            drop_graph();
+        }
         return NCG_OK;
     };
     // One step after the field's preparation (the tiled forms sort first).
@@ -637,10 +807,13 @@ extern "C" ncg_status ncg_leapfrog_dev(ncg_ctx *ctx, ncg_body *d_bodies, int32_t
             if (r != NCG_OK) return r;
         }
         hipLaunchKernelGGL(k_lf_drift, dim3(g), dim3(LT), 0, st, (int)n, ex, dt, b, d_central);
-        if (coll && one_collision_only)
+        if (thr)
+            hipLaunchKernelGGL(k_lf_thresholds, dim3(g), dim3(LT), 0, st, (int)n, ex, b, d_central,
+                               r_dis, r_hd, stat + 2);
+        if (clear_dc)
             hipLaunchKernelGGL(k_lf_zero_i32, dim3(cdiv(n, 256)), dim3(256), 0, st, d_did_collide, (int)n);
         NCG_CHECK_LAUNCH();
-        return NCG_OK;
+        return checked ? end_checked() : NCG_OK;
     };
 
     if (flags & 2) {
@@ -652,25 +825,37 @@ extern "C" ncg_status ncg_leapfrog_dev(ncg_ctx *ctx, ncg_body *d_bodies, int32_t
         hipLaunchKernelGGL(k_lf_drift, dim3(g), dim3(LT), 0, st, (int)n, ex, 0.5 * dt, b, d_central);
         NCG_CHECK_LAUNCH();
     }
-    hipGraphExec_t exec = nullptr;
-    hipGraph_t graph = nullptr;
     ncg_status res = NCG_OK;
+    int32_t snap_n = n;
+    const bool windows = coll || thr;
     auto snapshot = [&]() -> ncg_status {
-        unsigned hs[2];
+        unsigned hs[3];
         ncg_status r = read_stat(hs);
         if (r != NCG_OK) return r;
         snap_coll = hs[0];
-        return copy_state(snap_b, snap_dc, snap_c, b, d_did_collide, d_central);
+        snap_mh = mh;
+        snap_n = n;
+        if (snap_ex)
+            NCG_HIP(hipMemcpyAsync(snap_ex, d_exists_w, sizeof(int32_t) * (size_t)ncap,
+                                   hipMemcpyDeviceToDevice, st));
+        return copy_state(snap_b, snap_dc, snap_c, b, d_did_collide, d_central, mp ? ncap : n);
     };
-    if (coll) res = snapshot();
+    if (windows) res = snapshot();
     int32_t win0 = 0;  // first step of the current window
     for (int32_t s = 0; s < nsteps && res == NCG_OK; ++s) {
+        // A step end the host state already asks for (only possible at the
+        // call's start: afterwards such states arise in checked steps, which
+        // settle them) runs checked.
+        const bool host_due = mp && ((mh.need && mp->reduce_to_com) || tidy_due());
         if (prec != NCG_F64_EXACT) {
             res = ncg_tiled_order(ctx, b, n, ex, prec, st);  // eager: the sort stays out of graphs
             if (res != NCG_OK) break;
         }
-        if (!use_graph || s == 0) {
+        if (host_due) {
+            res = step(true);
+        } else if (!use_graph || need_eager) {
             res = step(false);  // the first step also allocates what a graph uses
+            need_eager = false;
         } else {
             if (!exec) {
                 NCG_HIP(hipStreamSynchronize(st));
@@ -694,13 +879,22 @@ extern "C" ncg_status ncg_leapfrog_dev(ncg_ctx *ctx, ncg_body *d_bodies, int32_t
                 break;
             }
         }
-        if (res != NCG_OK || !coll) continue;
+        if (res != NCG_OK || !windows) continue;
         if ((s + 1) % LF_CHECK != 0 && s + 1 != nsteps) continue;
-        unsigned hs[2];
+        unsigned hs[3];
         if ((res = read_stat(hs)) != NCG_OK) break;
-        if (hs[1] != 0u) {
+        if (hs[1] != 0u || (model_host && hs[0] != snap_coll) || (thr && hs[2] != 0u)) {
             // The window [win0, s] again, from its snapshot, checked.
-            if ((res = copy_state(b, d_did_collide, d_central, snap_b, snap_dc, snap_c)) != NCG_OK)
+            mh = snap_mh;
+            if (n != snap_n) {
+                set_n(snap_n);
+                drop_graph();
+            }
+            if (snap_ex)
+                NCG_HIP(hipMemcpyAsync(d_exists_w, snap_ex, sizeof(int32_t) * (size_t)ncap,
+                                       hipMemcpyDeviceToDevice, st));
+            if ((res = copy_state(b, d_did_collide, d_central, snap_b, snap_dc, snap_c,
+                                  mp ? ncap : n)) != NCG_OK)
                 break;
             if ((res = clear_dirty()) != NCG_OK) break;
             if ((res = write_stat(snap_coll)) != NCG_OK) break;
@@ -722,10 +916,23 @@ extern "C" ncg_status ncg_leapfrog_dev(ncg_ctx *ctx, ncg_body *d_bodies, int32_t
     unsigned hs[2] = {0, 0};
     (void)hipMemcpyAsync(hs, stat, sizeof hs, hipMemcpyDeviceToHost, st);
     (void)hipStreamSynchronize(st);
-    if (exec) (void)hipGraphExecDestroy(exec);
-    if (graph) (void)hipGraphDestroy(graph);
+    drop_graph();
     if (res != NCG_OK) return res;
-    if (n_collisions) *n_collisions = hs[0] + host_coll;
+    if (n_collisions) *n_collisions = hs[0] + host_coll + mh.coll;
+    if (mp) {
+        mp->largest_id = mh.largest_id;
+        mp->how_many_free = mh.how_many_free;
+        std::copy(mh.free.begin(), mh.free.end(), mp->free_indexes);
+        mp->need_to_reduce_com = mh.need;
+        mp->host_passes = mh.host_passes;
+        mp->host_step_ends = mh.host_step_ends;
+        mp->removed = mh.removed;
+        mp->tidy_ups = mh.tidy_ups;
+        mp->mergers = mh.mergers;
+        mp->super_catastrophic = mh.super_catastrophic;
+        mp->partial = mh.partial;
+        mp->full = mh.full;
+    }
     if (hs[1])
         return ncg_fail(ctx, NCG_ERR_CAPACITY,
                         "collision replay capacity exceeded (flags %u: 1 = more than %u candidates, "
@@ -733,4 +940,61 @@ extern "C" ncg_status ncg_leapfrog_dev(ncg_ctx *ctx, ncg_body *d_bodies, int32_t
                         "results are not valid",
                         hs[1], cap, NCG_REPLAY_RCAP, NCG_REPLAY_DCAP);
     return NCG_OK;
+}
+
+}  // namespace
+
+extern "C" ncg_status ncg_leapfrog_dev(ncg_ctx *ctx, ncg_body *d_bodies, int32_t n,
+                                       const int32_t *d_exists, int32_t *d_did_collide,
+                                       double *d_central, double grav_const, double softening,
+                                       ncg_precision prec, double dt, int32_t nsteps,
+                                       double collision_f, int32_t one_collision_only,
+                                       int32_t flags, int64_t *n_collisions, void *stream)
+{
+    if (!ctx) return NCG_ERR_INVALID;
+    NCG_REQUIRE(n >= 1 && d_bodies && d_central && nsteps >= 0, "invalid arguments");
+    NCG_REQUIRE(prec == NCG_F64_EXACT || prec == NCG_F64_FAST || prec == NCG_MIXED,
+                "unknown precision %d", (int)prec);
+    NCG_REQUIRE(!(collision_f > 0.0 && one_collision_only && !d_did_collide),
+                "one_collision_only needs a did_collide array");
+    return leapfrog_run(ctx, d_bodies, n, nullptr, d_exists, d_did_collide, d_central, grav_const,
+                        softening, prec, dt, nsteps, collision_f, one_collision_only, flags,
+                        nullptr, n_collisions, stream);
+}
+
+extern "C" ncg_status ncg_leapfrog_model_dev(ncg_ctx *ctx, ncg_body *d_bodies, int32_t *d_exists,
+                                             int32_t *d_did_collide, double *d_central,
+                                             double grav_const, double softening,
+                                             ncg_precision prec, double dt, int32_t nsteps,
+                                             int32_t one_collision_only, int32_t flags,
+                                             ncg_leapfrog_model_params *params,
+                                             int64_t *n_collisions, void *stream)
+{
+    if (!ctx) return NCG_ERR_INVALID;
+    NCG_REQUIRE(params && d_bodies && d_exists && d_central && nsteps >= 0, "invalid arguments");
+    const ncg_leapfrog_model_params &P = *params;
+    NCG_REQUIRE(P.n_max >= 1 && P.largest_id >= 0 && P.largest_id < P.n_max &&
+                    P.how_many_free >= 0 && P.how_many_free <= P.n_max && P.free_indexes,
+                "invalid slots (n_max %d, largest_id %d, how_many_free %d)", (int)P.n_max,
+                (int)P.largest_id, (int)P.how_many_free);
+    NCG_REQUIRE(prec == NCG_F64_EXACT || prec == NCG_F64_FAST || prec == NCG_MIXED,
+                "unknown precision %d", (int)prec);
+    const ncg_outcome_model &M = P.model;
+    NCG_REQUIRE(M.kind >= NCG_OUTCOME_BOUNCE && M.kind <= NCG_OUTCOME_FRAGMENTATION,
+                "unknown outcome model %d", (int)M.kind);
+    NCG_REQUIRE(M.kind != NCG_OUTCOME_FRAGMENTATION ||
+                    (M.n_tilde >= 1 && M.n_tilde <= NCG_MAX_TILDE && M.pq[1] >= M.pq[0] &&
+                     M.pq[3] >= M.pq[2] &&
+                     (int64_t)(M.pq[1] - M.pq[0] + 1) * (M.pq[3] - M.pq[2] + 1) == M.n_tilde),
+                "invalid fragmentation lattice (n_tilde %d, pq %d %d %d %d)", (int)M.n_tilde,
+                (int)M.pq[0], (int)M.pq[1], (int)M.pq[2], (int)M.pq[3]);
+    const bool coll = M.kind != NCG_OUTCOME_BOUNCE || M.collision_f > 0.0;
+    NCG_REQUIRE(!(coll && one_collision_only && !d_did_collide),
+                "one_collision_only needs a did_collide array");
+    for (int32_t q = 0; q < P.how_many_free; ++q)
+        NCG_REQUIRE(P.free_indexes[q] >= 0 && P.free_indexes[q] < P.n_max,
+                    "free slot %d out of range", (int)P.free_indexes[q]);
+    return leapfrog_run(ctx, d_bodies, P.largest_id + 1, d_exists, d_exists, d_did_collide,
+                        d_central, grav_const, softening, prec, dt, nsteps, 0.0, one_collision_only,
+                        flags, params, n_collisions, stream);
 }

@@ -162,6 +162,15 @@ ncg_status ncg_candidates_launch(ncg_ctx *ctx, const DBody *d_b, int32_t n, cons
 // the count in d_count32 or d_count64; d_ex, d_dc may be null.  d_stat[0] +=
 // collisions, d_stat[1] |= capacity error bits.  d_dirty: one zeroed byte
 // per body, left zeroed.
+// end_of_timestep's host half (rk4.c:155-221, 395-397): removals by the
+// disruption / high-dumping distances (r_hd = +inf: none), the reduction of
+// the centre of mass, tidy_up; returns the bodies left (how_many_moonlets).
+// central: 8 doubles {x, y, z, vx, vy, vz, M, present}.
+int32_t ncg_host_step_end(ncg_body *X, int32_t *exists, int32_t *free_indexes, int32_t &largest_id,
+                          int32_t &how_many_free, double *central, double r_dis, double r_hd,
+                          int reduce_to_com, int32_t &need_to_reduce, int32_t &removed,
+                          int32_t &tidied);
+
 constexpr int NCG_REPLAY_RCAP = 1024;  // candidates of one system per pass
 constexpr int NCG_REPLAY_DCAP = 256;   // bodies changed in one system per pass
 ncg_status ncg_bf_replay_launch(ncg_ctx *ctx, DBody *d_b, const int32_t *d_ex, int32_t *d_dc,

@@ -152,6 +152,13 @@ CONFIGS = {
         "inelastic_collision_bool": "0", "fragmentation_bool": "1", "N_max": "4000"})),
     "run_c1_merge": ("run", _run_cfg("brute", 2000, 40, DT1, {
         "inelastic_collision_bool": "0", "instant_merger_bool": "1"})),
+    # The fragmentation model with end_of_timestep's removals: bodies closer
+    # to the central body than 0.93 merge into it, bodies beyond 1.07 are
+    # dropped (the centre of mass is then reduced), and the array is
+    # compacted once a tenth of its slots are empty (tidy_up).
+    "run_c1_rm": ("run", _run_cfg("brute", 2000, 40, DT1, {
+        "inelastic_collision_bool": "0", "fragmentation_bool": "1", "N_max": "4000",
+        "disruption_threshold": "0.93", "high_dumping_threshold": "1.07"})),
     # Mesh, 16384 bodies, cells of side 0.01 (256 per axis), 10 steps of 2 pi/512.
     # The initial mesh size comes from sma_min/sma_max/inclination_max
     # (structure.c:517), so those are set to the disk's range.

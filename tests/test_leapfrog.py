@@ -385,3 +385,86 @@ def test_leapfrog_replay_overflow_replayed_on_host(tmp_path, graph):
         assert np.array_equal(ct.cpu().numpy().view(np.uint64), ref_c.view(np.uint64))
     finally:
         small.close()
+
+
+# Reference runs with the collision models that remove and create bodies,
+# and with end_of_timestep's removals: (config, outcome model, N_max,
+# disruption_threshold, high_dumping_threshold), as built by
+# oracle/build_ref.py.  Initial conditions as in test_gpu_dropin.py.
+MODEL_RUNS = [("run_c1_frag", "fragmentation", 4000, 0.05, 100.0),
+              ("run_c1_merge", "merger", 2016, 0.05, 100.0),
+              ("run_c1_rm", "fragmentation", 4000, 0.93, 1.07)]
+
+
+def _model_setup(ctx, tmp_path, cfg, kind, n_max):
+    torch = tt()
+    from ncorpion_amd import _abi
+    b0 = ic.disk(2000, seed=1, mass=1e-8, radius=4e-3)
+    (bodies, ex, dc, cm, ts), vf, ncoll = traced_reference(cfg, b0, tmp_path)
+    n0 = bodies.shape[0]
+    B = np.zeros((n_max, 8))
+    B[:n0] = bodies
+    E = np.zeros(n_max, np.int32)
+    E[:n0] = ex
+    DC = np.zeros(n_max, np.int32)
+    DC[:n0] = dc
+    t = dict(b=torch.from_numpy(B).cuda(), e=torch.from_numpy(E).cuda(),
+             dc=torch.from_numpy(DC).cuda(),
+             c=torch.from_numpy(np.concatenate([cm[0:7], [1.0]])).cuda())
+    state = dict(largest_id=n0 - 1, how_many_free=0, free_indexes=np.zeros(n_max, np.int32),
+                 need_to_reduce_com=0)
+    return t, state, _abi.outcome_model(kind, G=1.0), 2.0 * ts, vf, ncoll
+
+
+def _model_hash(t, state):
+    n = state["largest_id"] + 1
+    return orc.state_hash(t["b"].cpu().numpy()[:n], t["e"].cpu().numpy()[:n])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("cfg,kind,n_max,rdis,rhd", MODEL_RUNS)
+def test_leapfrog_model_reproduces_reference_run(ctx, tmp_path, cfg, kind, n_max, rdis, rhd):
+    """The GPU-resident loop with the reference's outcome model (mergers,
+    super-catastrophic, partial and full fragmentation; bodies removed and
+    created at free or appended slots) and end_of_timestep's removals,
+    centre-of-mass reduction and compaction: one step per call from the
+    reference's own state before its half drift, the state hash (every slot
+    up to largest_id) equal to the reference's at each vector_field call."""
+    t, state, model, dt, vf, ncoll = _model_setup(ctx, tmp_path, cfg, kind, n_max)
+    kw = dict(G=1.0, softening=0.0, precision="exact", did_collide_t=t["dc"],
+              r_disruption=rdis, r_high_dumping=rhd)
+    total = ctx.leapfrog_model_dev(t["b"], t["c"], t["e"], 0, dt, state, model,
+                                   half_drift_start=True, **kw)
+    for s, h in enumerate(vf):
+        ctx.synchronize()
+        assert _model_hash(t, state) == h, "step %d" % s
+        total += ctx.leapfrog_model_dev(t["b"], t["c"], t["e"], 1, dt, state, model, **kw)
+    print("%s: %d collisions (%d treatment calls), mergers %d, super-catastrophic %d, "
+          "partial %d, full %d, removed %d, tidy_ups %d, largest_id %d" % (
+              cfg, total, ncoll, state["mergers"], state["super_catastrophic"],
+              state["partial"], state["full"], state["removed"], state["tidy_ups"],
+              state["largest_id"]))
+    # Treatment calls of pairs with a body removed earlier in the pass do
+    # nothing (physics.c:1057-1059) and are not counted here.
+    assert 100 < total <= ncoll
+    if kind == "fragmentation":
+        assert state["full"] > 0 and state["partial"] > 0
+    if cfg == "run_c1_rm":
+        assert state["removed"] > 100 and state["tidy_ups"] > 0 and state["super_catastrophic"] > 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("cfg,kind,n_max,rdis,rhd", MODEL_RUNS)
+def test_leapfrog_model_one_call_matches_reference(ctx, tmp_path, cfg, kind, n_max, rdis, rhd):
+    """The same runs in one call of 39 steps replayed from a captured graph:
+    windows whose passes resolved pairs or whose bodies crossed a threshold
+    run again step by step with those passes and step ends on the host; the
+    state then equals the reference's before its 40th step."""
+    t, state, model, dt, vf, ncoll = _model_setup(ctx, tmp_path, cfg, kind, n_max)
+    ctx.leapfrog_model_dev(t["b"], t["c"], t["e"], len(vf) - 1, dt, state, model, G=1.0,
+                           softening=0.0, precision="exact", did_collide_t=t["dc"],
+                           r_disruption=rdis, r_high_dumping=rhd, graph=True,
+                           half_drift_start=True)
+    ctx.synchronize()
+    assert _model_hash(t, state) == vf[-1]
+    assert state["host_passes"] > 10
