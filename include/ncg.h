@@ -522,7 +522,8 @@ typedef struct {
     int32_t n_max;               /* slots of bodies / exists / did_collide (N_max) */
     int32_t largest_id;          /* in/out */
     int32_t how_many_free;       /* in/out */
-    int32_t mesh;                /* mesh mode (fragments appended, slots 0..2 never reused) */
+    int32_t mesh;                /* mesh mode: fragments appended (get_free_index(1)); the
+                                    list of free slots is then never read and not kept */
     int32_t need_to_reduce_com;  /* out: set by a super-catastrophic disruption (physics.c:815) */
     int32_t error;               /* out: 1 N_max exceeded, 2 an ejection speed below the escape
                                     speed (where the reference aborts; the pass's state is then
@@ -577,6 +578,26 @@ ncg_status ncg_leapfrog_model_dev(ncg_ctx *ctx, ncg_body *d_bodies, int32_t *d_e
                                   int32_t nsteps, int32_t one_collision_only, int32_t flags,
                                   ncg_leapfrog_model_params *params, int64_t *n_collisions,
                                   void *stream);
+
+/* GPU-resident mesh loop (ncg_mesh_leapfrog_dev) with an outcome model and
+ * end_of_timestep's removals, as ncg_leapfrog_model_dev: a mesh() pass in
+ * which the device replay resolved a pair under the merger or fragmentation
+ * model is undone and replayed on the host (ncg_mesh_pass with the model's
+ * ops in mesh mode: fragments appended after largest_id with their pairs
+ * registered, physics.c:994-1010), and a step end with a body across a
+ * removal distance, a pending centre-of-mass reduction or due compaction
+ * runs on the host, before the mesh-size update, which counts the bodies
+ * left.  The loop synchronises once per step anyway (the census sets the
+ * next mesh size), so no windows are needed.  The removal distances are
+ * params' (mesh's r_disruption / r_high_dumping are not read); in mesh mode
+ * free_indexes is neither read nor written (fragments are appended). */
+ncg_status ncg_mesh_leapfrog_model_dev(ncg_ctx *ctx, ncg_body *d_bodies, int32_t *d_exists,
+                                       int32_t *d_did_collide, double *d_central,
+                                       double grav_const, double softening, ncg_precision prec,
+                                       double dt, int32_t nsteps, ncg_mesh_loop_params *mesh,
+                                       ncg_leapfrog_model_params *params,
+                                       int32_t one_collision_only, int32_t flags,
+                                       int64_t *n_collisions, void *stream);
 
 /* ---------------------------------------------------------------------------
  * Total energy (FP64) of bodies d_bodies (n, d_exists may be NULL) and the

@@ -166,6 +166,10 @@ _SIGNATURES = {
                              ctypes.POINTER(I64), P]),
     "ncg_leapfrog_model_dev": (S, [P, P, P, P, P, D, D, S, D, I32, I32, I32,
                                    ctypes.POINTER(LeapfrogModelParams), ctypes.POINTER(I64), P]),
+    "ncg_mesh_leapfrog_model_dev": (S, [P, P, P, P, P, D, D, S, D, I32,
+                                        ctypes.POINTER(MeshLoopParams),
+                                        ctypes.POINTER(LeapfrogModelParams), I32, I32,
+                                        ctypes.POINTER(I64), P]),
     "ncg_rk4_forward_ckpt_dev": (S, [P, P, I32, P, D, D, D, I32, I32, P, P, P]),
     "ncg_rk4_adjoint_ckpt_dev": (S, [P, P, I32, P, D, D, D, I32, I32, P, P, P, P, P, P]),
     "ncg_builtin_replay_ops": (S, [ctypes.POINTER(BuiltinCollider), ctypes.POINTER(ReplayOps)]),
@@ -371,6 +375,19 @@ class Context:
         n_max) and need_to_reduce_com, updated in place; the call's counters
         are added under their names.  Returns the collisions resolved."""
         n_max = int(bodies_t.shape[0])
+        mp = self._model_params(n_max, state, model, r_disruption, r_high_dumping, reduce_to_com)
+        nc = I64()
+        flags = (1 if graph else 0) | (2 if half_drift_start else 0) | (4 if half_drift_end else 0)
+        self._check(self.lib.ncg_leapfrog_model_dev(
+            self.h, bodies_t.data_ptr(), exists_t.data_ptr(),
+            None if did_collide_t is None else did_collide_t.data_ptr(), central_t.data_ptr(),
+            float(G), float(softening), PRECISIONS[precision], float(dt), int(nsteps),
+            int(one_collision_only), flags, ctypes.byref(mp), ctypes.byref(nc), stream))
+        self._model_state_back(mp, state)
+        return int(nc.value)
+
+    @staticmethod
+    def _model_params(n_max, state, model, r_disruption, r_high_dumping, reduce_to_com):
         free = state["free_indexes"]
         assert free.dtype == np.int32 and free.shape[0] == n_max and free.flags["C_CONTIGUOUS"]
         mp = LeapfrogModelParams()
@@ -383,19 +400,43 @@ class Context:
         mp.how_many_free = int(state["how_many_free"])
         mp.free_indexes = free.ctypes.data
         mp.need_to_reduce_com = int(state.get("need_to_reduce_com", 0))
-        nc = I64()
-        flags = (1 if graph else 0) | (2 if half_drift_start else 0) | (4 if half_drift_end else 0)
-        self._check(self.lib.ncg_leapfrog_model_dev(
-            self.h, bodies_t.data_ptr(), exists_t.data_ptr(),
-            None if did_collide_t is None else did_collide_t.data_ptr(), central_t.data_ptr(),
-            float(G), float(softening), PRECISIONS[precision], float(dt), int(nsteps),
-            int(one_collision_only), flags, ctypes.byref(mp), ctypes.byref(nc), stream))
+        return mp
+
+    @staticmethod
+    def _model_state_back(mp, state):
         state["largest_id"] = int(mp.largest_id)
         state["how_many_free"] = int(mp.how_many_free)
         state["need_to_reduce_com"] = int(mp.need_to_reduce_com)
         for f in ("host_passes", "host_step_ends", "removed", "tidy_ups", "mergers",
                   "super_catastrophic", "partial", "full"):
             state[f] = state.get(f, 0) + int(getattr(mp, f))
+
+    def mesh_leapfrog_model_dev(self, bodies_t, central_t, exists_t, nsteps, dt, params, state,
+                                model, G=1.0, softening=0.0, precision="exact",
+                                one_collision_only=0, did_collide_t=None, r_disruption=0.0,
+                                r_high_dumping=0.0, reduce_to_com=1, half_kick_start=False,
+                                half_kick_end=False, stream=None):
+        """NcorpiON's mesh integrator with an outcome model and end_of_timestep's
+        removals (see ncg_mesh_leapfrog_model_dev).  params: as
+        mesh_leapfrog_dev (gam updated in place); state, model: as
+        leapfrog_model_dev.  Returns the collisions resolved."""
+        n_max = int(bodies_t.shape[0])
+        mesh = MeshLoopParams(float(params["gam"]), float(params["gam_min"]),
+                              float(params["how_many_neighbours"]), 0.0, 0.0,
+                              int(params["cells"]), 0, 0)
+        mp = self._model_params(n_max, state, model, r_disruption, r_high_dumping, reduce_to_com)
+        nc = I64()
+        flags = (2 if half_kick_start else 0) | (4 if half_kick_end else 0)
+        self._check(self.lib.ncg_mesh_leapfrog_model_dev(
+            self.h, bodies_t.data_ptr(), exists_t.data_ptr(),
+            None if did_collide_t is None else did_collide_t.data_ptr(), central_t.data_ptr(),
+            float(G), float(softening), PRECISIONS[precision], float(dt), int(nsteps),
+            ctypes.byref(mesh), ctypes.byref(mp), int(one_collision_only), flags,
+            ctypes.byref(nc), stream))
+        params["gam"] = mesh.mesh_size
+        params["host_passes"] = int(mesh.host_passes)
+        params["conflict_passes"] = int(mesh.conflict_passes)
+        self._model_state_back(mp, state)
         return int(nc.value)
 
     def mesh_pass_dev(self, bodies_t, dt, gam, cube, cells, mode, collision_f=1.5,

@@ -86,13 +86,15 @@ void builtin_resolve(void *u, int32_t a, int32_t bb)
 // ---- outcome models -----------------------------------------------------------
 using Outcome = ncg_outcome_state;
 
-// lose_moonlet (structure.c:1077-1087).
+// lose_moonlet (structure.c:1077-1087).  In mesh mode the list of free
+// slots is never read (fragments are appended, get_free_index(1)), so it is
+// not kept.
 void lose(Outcome *s, int a)
 {
+    s->exists[a] = 0;
+    if (s->mesh) return;
     s->free_indexes[s->how_many_free] = a;
     ++s->how_many_free;
-    s->exists[a] = 0;
-    if (a <= 2 && s->mesh) --s->how_many_free;
 }
 
 // get_free_index (structure.c:849-875); -1 (error 1) where the reference aborts.
@@ -464,9 +466,10 @@ extern "C" ncg_status ncg_builtin_outcome_ops(ncg_outcome_state *state, ncg_repl
 int32_t ncg_host_step_end(ncg_body *X, int32_t *exists, int32_t *free_indexes, int32_t &largest_id,
                           int32_t &how_many_free, double *central, double r_dis, double r_hd,
                           int reduce_to_com, int32_t &need_to_reduce, int32_t &removed,
-                          int32_t &tidied)
+                          int32_t &tidied, int mesh)
 {
     Outcome s = {};
+    s.mesh = mesh;
     s.exists = exists;
     s.free_indexes = free_indexes;
     s.largest_id = largest_id;

@@ -280,44 +280,83 @@ __global__ void k_lf_zero_i32_one(int *__restrict__ p) { *p = 0; }
 
 }  // namespace
 
-extern "C" ncg_status ncg_mesh_leapfrog_dev(ncg_ctx *ctx, ncg_body *d_bodies, int32_t n,
-                                            int32_t *d_exists, int32_t *d_did_collide,
-                                            double *d_central, double grav_const,
-                                            double softening, ncg_precision prec, double dt,
-                                            int32_t nsteps, ncg_mesh_loop_params *mp,
-                                            double collision_f, int32_t one_collision_only,
-                                            int32_t flags, int64_t *n_collisions, void *stream)
+namespace {
+
+// Slots, outcome model and did_collide of a model run.
+ncg_status check_model_params(ncg_ctx *ctx, const ncg_leapfrog_model_params &P,
+                              int32_t one_collision_only, const int32_t *d_did_collide)
 {
-    if (!ctx) return NCG_ERR_INVALID;
-    NCG_REQUIRE(n >= 3 && d_bodies && d_central && mp && nsteps >= 0, "invalid arguments");
-    NCG_REQUIRE(prec == NCG_F64_EXACT || prec == NCG_F64_FAST || prec == NCG_MIXED,
-                "unknown precision %d", (int)prec);
-    NCG_REQUIRE(mp->mesh_size > 0.0 && mp->mesh_size_min > 0.0 && mp->cells >= 3 &&
-                    mp->target_neighbours > 0.0,
-                "invalid mesh parameters");
-    NCG_REQUIRE(!(collision_f > 0.0 && one_collision_only && !d_did_collide),
+    NCG_REQUIRE(P.n_max >= 1 && P.largest_id >= 0 && P.largest_id < P.n_max &&
+                    P.how_many_free >= 0 && P.how_many_free <= P.n_max && P.free_indexes,
+                "invalid slots (n_max %d, largest_id %d, how_many_free %d)", (int)P.n_max,
+                (int)P.largest_id, (int)P.how_many_free);
+    const ncg_outcome_model &M = P.model;
+    NCG_REQUIRE(M.kind >= NCG_OUTCOME_BOUNCE && M.kind <= NCG_OUTCOME_FRAGMENTATION,
+                "unknown outcome model %d", (int)M.kind);
+    NCG_REQUIRE(M.kind != NCG_OUTCOME_FRAGMENTATION ||
+                    (M.n_tilde >= 1 && M.n_tilde <= NCG_MAX_TILDE && M.pq[1] >= M.pq[0] &&
+                     M.pq[3] >= M.pq[2] &&
+                     (int64_t)(M.pq[1] - M.pq[0] + 1) * (M.pq[3] - M.pq[2] + 1) == M.n_tilde),
+                "invalid fragmentation lattice (n_tilde %d, pq %d %d %d %d)", (int)M.n_tilde,
+                (int)M.pq[0], (int)M.pq[1], (int)M.pq[2], (int)M.pq[3]);
+    const bool coll = M.kind != NCG_OUTCOME_BOUNCE || M.collision_f > 0.0;
+    NCG_REQUIRE(!(coll && one_collision_only && !d_did_collide),
                 "one_collision_only needs a did_collide array");
+    for (int32_t q = 0; q < P.how_many_free; ++q)
+        NCG_REQUIRE(P.free_indexes[q] >= 0 && P.free_indexes[q] < P.n_max,
+                    "free slot %d out of range", (int)P.free_indexes[q]);
+    return NCG_OK;
+}
+
+// NcorpiON's mesh integrator; lp == nullptr: ncg_mesh_leapfrog_dev (bounce
+// with collision_f, fixed slots, no removals); else ncg_mesh_leapfrog_model_dev.
+ncg_status mesh_loop_run(ncg_ctx *ctx, ncg_body *d_bodies, int32_t n_in, int32_t *d_exists,
+                         int32_t *d_did_collide, double *d_central, double grav_const,
+                         double softening, ncg_precision prec, double dt, int32_t nsteps,
+                         ncg_mesh_loop_params *mp, double collision_f, int32_t one_collision_only,
+                         int32_t flags, ncg_leapfrog_model_params *lp, int64_t *n_collisions,
+                         void *stream)
+{
     NCG_HIP(hipSetDevice(ctx->device));
     hipStream_t st = pick_stream(ctx, stream);
     if (n_collisions) *n_collisions = 0;
     DBody *b = reinterpret_cast<DBody *>(d_bodies);
-    const bool coll = collision_f > 0.0;
-    double *acc = dbuf<double>(ctx, "ml_acc", 3 * (size_t)n);
+    const bool model_host = lp && lp->model.kind != NCG_OUTCOME_BOUNCE;
+    const bool frag = lp && lp->model.kind == NCG_OUTCOME_FRAGMENTATION;
+    // The device replay applies the bounce; under the other models a pass in
+    // which it resolved a pair is replayed on the host.
+    const double f = model_host ? 2.0 : (lp ? lp->model.collision_f : collision_f);
+    const bool coll = model_host || f > 0.0;
+    const int32_t ncap = lp ? lp->n_max : n_in;
+    int32_t n = n_in;
+    // Removal distances: the model's, else the mesh parameters' (checked only).
+    const double r_dis = lp ? (lp->r_disruption > 0.0 ? lp->r_disruption : 0.0) : mp->r_disruption;
+    const double r_hd = lp ? (lp->r_high_dumping > 0.0 ? lp->r_high_dumping : INFINITY)
+                           : mp->r_high_dumping;
+    double *acc = dbuf<double>(ctx, "ml_acc", 3 * (size_t)ncap);
     double *cm = dbuf<double>(ctx, "ml_cm", 4);
     unsigned *err = dbuf<unsigned>(ctx, "ml_err", 2);
     int *count = dbuf<int>(ctx, "ml_count", 1);
     int *hpin = pbuf<int>(ctx, "ml_hpin", 4);
     if (!acc || !cm || !err || !count || !hpin) return NCG_ERR_NOMEM;
-    const unsigned g = (unsigned)cdiv(n + 1, LT);
+    unsigned g = (unsigned)cdiv(n + 1, LT);
     const double G = grav_const, soft = softening;
     int64_t ncoll = 0;
+    // Host-side slot state of a model run.
+    int32_t largest_id = lp ? lp->largest_id : n - 1;
+    int32_t how_many_free = lp ? lp->how_many_free : 0;
+    int32_t need = lp ? lp->need_to_reduce_com : 0;
+    std::vector<int32_t> free_list;
+    if (lp) free_list.assign(lp->free_indexes, lp->free_indexes + ncap);
+    int32_t host_step_ends = 0, removed = 0, tidy_ups = 0, host_passes = 0;
+    int64_t mergers = 0, super_catastrophic = 0, partial = 0, full = 0;
     // Status of the device-side checks (top-3 and census) read with the
-    // counters of each step.
-    auto census = [&](int &how_many) -> ncg_status {
+    // counters of each step.  *crossed: a body crossed a removal distance
+    // (an error without a model).
+    auto census = [&](int &how_many, bool *crossed) -> ncg_status {
         hipLaunchKernelGGL(k_lf_zero_i32_one, dim3(1), dim3(1), 0, st, count);
         hipLaunchKernelGGL(k_lf_census, dim3(std::min<long>(1024, cdiv(n, LT))), dim3(LT), 0, st, b,
-                           (int)n, d_exists,
-                           d_central, mp->r_disruption, mp->r_high_dumping, count, err);
+                           (int)n, d_exists, d_central, r_dis, r_hd, count, err);
         NCG_CHECK_LAUNCH();
         NCG_HIP(hipMemcpyAsync(hpin, count, sizeof(int), hipMemcpyDeviceToHost, st));
         NCG_HIP(hipMemcpyAsync(hpin + 1, err, sizeof(unsigned), hipMemcpyDeviceToHost, st));
@@ -326,16 +365,22 @@ extern "C" ncg_status ncg_mesh_leapfrog_dev(ncg_ctx *ctx, ncg_body *d_bodies, in
         if (hpin[1] & 1u)
             return ncg_fail(ctx, NCG_ERR_INVALID, "fewer than three bodies: three_largest_moonlets "
                                                   "is undefined");
-        if (hpin[1] & 2u)
-            return ncg_fail(ctx, NCG_ERR_INVALID,
-                            "a body crossed disruption_threshold or high_dumping_threshold: "
-                            "end_of_timestep would remove it, which this loop does not do");
+        if (hpin[1] & 2u) {
+            if (!lp)
+                return ncg_fail(ctx, NCG_ERR_INVALID,
+                                "a body crossed disruption_threshold or high_dumping_threshold: "
+                                "end_of_timestep would remove it, which this loop does not do");
+            hipLaunchKernelGGL(k_lf_zero_i32, dim3(1), dim3(64), 0, st, reinterpret_cast<int32_t *>(err), 2);
+            NCG_CHECK_LAUNCH();
+        }
+        if (crossed) *crossed = (hpin[1] & 2u) != 0;
         return NCG_OK;
     };
-    const int nparts = (int)std::min<long>(T3_PARTS, cdiv(n, T3B));
-    Top3 *parts = dbuf<Top3>(ctx, "ml_top3", (size_t)nparts);
+    const int nparts_max = (int)std::min<long>(T3_PARTS, cdiv(ncap, T3B));
+    Top3 *parts = dbuf<Top3>(ctx, "ml_top3", (size_t)nparts_max);
     if (!parts) return NCG_ERR_NOMEM;
     auto largest3 = [&]() {
+        const int nparts = (int)std::min<long>(T3_PARTS, cdiv(n, T3B));
         hipLaunchKernelGGL(k_top3_part, dim3(nparts), dim3(T3B), 0, st, (const DBody *)b, (int)n,
                            (const int32_t *)d_exists, parts);
         hipLaunchKernelGGL(k_top3_first, dim3(1), dim3(T3), 0, st, b, (int)n, d_exists,
@@ -351,51 +396,112 @@ extern "C" ncg_status ncg_mesh_leapfrog_dev(ncg_ctx *ctx, ncg_body *d_bodies, in
     } lists_flag{ctx->mesh};
     ctx->mesh.want_lists = prec == NCG_F64_EXACT;
     // A mesh() pass the device replay cannot take -- a body larger than the
-    // mesh size (whose visit scans every body, collision.c:381-414), or more
-    // candidates or pending re-tests than it holds -- is undone (a snapshot
-    // of the bodies and did_collide before it) and replayed on the host with
-    // the same decision and bounce (ncg_mesh_pass with the built-in ops).
-    DBody *snap_b = dbuf<DBody>(ctx, "ml_snap_b", n);
-    int32_t *snap_dc = d_did_collide ? dbuf<int32_t>(ctx, "ml_snap_dc", n) : nullptr;
+    // mesh size (whose visit scans every body, collision.c:381-414), more
+    // candidates or pending re-tests than it holds, or, under the merger and
+    // fragmentation models, any resolved pair -- is undone (a snapshot of the
+    // bodies and did_collide before it) and replayed on the host with the
+    // same decision and outcome (ncg_mesh_pass with the built-in ops).
+    DBody *snap_b = dbuf<DBody>(ctx, "ml_snap_b", ncap);
+    int32_t *snap_dc = d_did_collide ? dbuf<int32_t>(ctx, "ml_snap_dc", ncap) : nullptr;
     if (!snap_b || (d_did_collide && !snap_dc)) return NCG_ERR_NOMEM;
     mp->host_passes = 0;
     mp->conflict_passes = 0;
+    std::vector<DBody> hb;
+    std::vector<int32_t> hex, hdc;
+    double hcen[8];
+    // Host copies of the state (slots 0 .. n - 1; zero beyond) and back.
+    auto download = [&](const DBody *from_b, const int32_t *from_dc) -> ncg_status {
+        hb.assign((size_t)ncap, DBody{});
+        hex.assign(d_exists ? (size_t)ncap : 0, 0);
+        hdc.assign(d_did_collide ? (size_t)ncap : 0, 0);
+        NCG_HIP(hipMemcpyAsync(hb.data(), from_b, sizeof(DBody) * (size_t)n, hipMemcpyDeviceToHost, st));
+        if (d_exists)
+            NCG_HIP(hipMemcpyAsync(hex.data(), d_exists, sizeof(int32_t) * (size_t)n,
+                                   hipMemcpyDeviceToHost, st));
+        if (d_did_collide)
+            NCG_HIP(hipMemcpyAsync(hdc.data(), from_dc, sizeof(int32_t) * (size_t)n,
+                                   hipMemcpyDeviceToHost, st));
+        NCG_HIP(hipMemcpyAsync(hcen, d_central, sizeof hcen, hipMemcpyDeviceToHost, st));
+        NCG_HIP(hipStreamSynchronize(st));
+        return NCG_OK;
+    };
+    auto upload = [&](int32_t slots) -> ncg_status {
+        NCG_HIP(hipMemcpyAsync(b, hb.data(), sizeof(DBody) * (size_t)slots, hipMemcpyHostToDevice, st));
+        if (d_exists && lp)
+            NCG_HIP(hipMemcpyAsync(d_exists, hex.data(), sizeof(int32_t) * (size_t)slots,
+                                   hipMemcpyHostToDevice, st));
+        if (d_did_collide)
+            NCG_HIP(hipMemcpyAsync(d_did_collide, hdc.data(), sizeof(int32_t) * (size_t)slots,
+                                   hipMemcpyHostToDevice, st));
+        if (lp) NCG_HIP(hipMemcpyAsync(d_central, hcen, sizeof hcen, hipMemcpyHostToDevice, st));
+        NCG_HIP(hipStreamSynchronize(st));
+        return NCG_OK;
+    };
+    auto set_n = [&](int32_t nn) {
+        n = nn;
+        g = (unsigned)cdiv(n + 1, LT);
+    };
     auto host_pass = [&](double h, double cube, ncg_mesh_stats &ms) -> ncg_status {
         NCG_HIP(hipMemcpyAsync(b, snap_b, sizeof(DBody) * (size_t)n, hipMemcpyDeviceToDevice, st));
         if (d_did_collide)
             NCG_HIP(hipMemcpyAsync(d_did_collide, snap_dc, sizeof(int32_t) * (size_t)n,
                                    hipMemcpyDeviceToDevice, st));
-        std::vector<DBody> hb((size_t)n);
-        std::vector<int32_t> hex(d_exists ? (size_t)n : 0), hdc(d_did_collide ? (size_t)n : 0);
-        NCG_HIP(hipMemcpyAsync(hb.data(), snap_b, sizeof(DBody) * (size_t)n, hipMemcpyDeviceToHost,
-                               st));
-        if (d_exists)
-            NCG_HIP(hipMemcpyAsync(hex.data(), d_exists, sizeof(int32_t) * (size_t)n,
-                                   hipMemcpyDeviceToHost, st));
-        if (d_did_collide)
-            NCG_HIP(hipMemcpyAsync(hdc.data(), snap_dc, sizeof(int32_t) * (size_t)n,
-                                   hipMemcpyDeviceToHost, st));
-        NCG_HIP(hipStreamSynchronize(st));
-        ncg_builtin_collider bc = {};
-        bc.bodies = reinterpret_cast<ncg_body *>(hb.data());
-        bc.exists = d_exists ? hex.data() : nullptr;
-        bc.did_collide = d_did_collide ? hdc.data() : nullptr;
-        bc.timestep = h;
-        bc.collision_f = collision_f;
-        bc.one_collision_only = one_collision_only;
+        ncg_status r = download(snap_b, snap_dc);
+        if (r != NCG_OK) return r;
         ncg_replay_ops ops;
-        ncg_status r = ncg_builtin_replay_ops(&bc, &ops);
+        if (!lp) {
+            ncg_builtin_collider bc = {};
+            bc.bodies = reinterpret_cast<ncg_body *>(hb.data());
+            bc.exists = d_exists ? hex.data() : nullptr;
+            bc.did_collide = d_did_collide ? hdc.data() : nullptr;
+            bc.timestep = h;
+            bc.collision_f = f;
+            bc.one_collision_only = one_collision_only;
+            if ((r = ncg_builtin_replay_ops(&bc, &ops)) != NCG_OK) return r;
+            r = ncg_mesh_pass(ctx, bc.bodies, n, bc.exists, h, mp->mesh_size, cube, mp->cells, 0,
+                              &ops, &ms);
+            if (r != NCG_OK) return r;
+            ms.collisions = bc.collisions;
+            ++mp->host_passes;
+            return upload(n);
+        }
+        ncg_outcome_state os = {};
+        os.c.bodies = reinterpret_cast<ncg_body *>(hb.data());
+        os.c.did_collide = d_did_collide ? hdc.data() : nullptr;
+        os.c.timestep = h;
+        os.c.one_collision_only = one_collision_only;
+        os.model = lp->model;
+        os.exists = hex.data();
+        os.free_indexes = free_list.data();
+        os.central = hcen;
+        os.n_max = ncap;
+        os.largest_id = largest_id;
+        os.how_many_free = how_many_free;
+        os.mesh = 1;
+        os.need_to_reduce_com = need;
+        if ((r = ncg_builtin_outcome_ops(&os, &ops)) != NCG_OK)
+            return ncg_fail(ctx, r, "invalid outcome model");
+        r = ncg_mesh_pass(ctx, os.c.bodies, n, os.exists, h, mp->mesh_size, cube, mp->cells, 0,
+                          &ops, &ms);
         if (r != NCG_OK) return r;
-        r = ncg_mesh_pass(ctx, bc.bodies, n, bc.exists, h, mp->mesh_size, cube, mp->cells, 0, &ops,
-                          &ms);
-        if (r != NCG_OK) return r;
-        ms.collisions = bc.collisions;
-        NCG_HIP(hipMemcpyAsync(b, hb.data(), sizeof(DBody) * (size_t)n, hipMemcpyHostToDevice, st));
-        if (d_did_collide)
-            NCG_HIP(hipMemcpyAsync(d_did_collide, hdc.data(), sizeof(int32_t) * (size_t)n,
-                                   hipMemcpyHostToDevice, st));
-        NCG_HIP(hipStreamSynchronize(st));
+        if (os.error)
+            return ncg_fail(ctx, NCG_ERR_INVALID,
+                            os.error == 1 ? "a fragmentation needs more than n_max = %d slots"
+                                          : "an ejection speed fell below the escape speed "
+                                            "(n_max %d); NcorpiON aborts here",
+                            (int)ncap);
+        ms.collisions = os.c.collisions;
+        largest_id = os.largest_id;
+        how_many_free = os.how_many_free;
+        need = os.need_to_reduce_com;
+        mergers += os.mergers;
+        super_catastrophic += os.super_catastrophic;
+        partial += os.partial;
+        full += os.full;
+        ++host_passes;
         ++mp->host_passes;
+        if ((r = upload(largest_id + 1)) != NCG_OK) return r;
+        set_n(largest_id + 1);
         return NCG_OK;
     };
     auto pass = [&](int mode, double h) -> ncg_status {
@@ -409,15 +515,15 @@ extern "C" ncg_status ncg_mesh_leapfrog_dev(ncg_ctx *ctx, ncg_body *d_bodies, in
                 NCG_HIP(hipMemcpyAsync(snap_dc, d_did_collide, sizeof(int32_t) * (size_t)n,
                                        hipMemcpyDeviceToDevice, st));
         }
-        unsigned flags = 0u;
+        unsigned pflags = 0u;
         ncg_status r = ncg_mesh_pass_dev_flags(ctx, d_bodies, n, d_exists, d_did_collide, h,
-                                               mp->mesh_size, cube, mp->cells, mode, collision_f,
-                                               one_collision_only, &ms, &flags, st);
+                                               mp->mesh_size, cube, mp->cells, mode, f,
+                                               one_collision_only, &ms, &pflags, st);
         if (r != NCG_OK) return r;
         mp->conflict_passes += (int32_t)ms.replay_conflicts;
-        if (flags != 0u) {
+        if (pflags != 0u || (model_host && mode == 0 && ms.collisions > 0)) {
             if (mode != 0)
-                return ncg_fail(ctx, NCG_ERR_INVALID, "neighbour pass failed (flags %u)", flags);
+                return ncg_fail(ctx, NCG_ERR_INVALID, "neighbour pass failed (flags %u)", pflags);
             if ((r = host_pass(h, cube, ms)) != NCG_OK) return r;
         }
         tn = ms.total_neighbours;
@@ -440,16 +546,34 @@ extern "C" ncg_status ncg_mesh_leapfrog_dev(ncg_ctx *ctx, ncg_body *d_bodies, in
         if (gam < mp->mesh_size_min) gam = mp->mesh_size_min;
         mp->mesh_size = gam;
     };
+    // end_of_timestep's host half (removals, reduction of the centre of
+    // mass, tidy_up) when a body crossed a distance or the slot state asks.
+    auto step_end = [&](int &how_many, bool crossed) -> ncg_status {
+        if (!lp) return NCG_OK;
+        if (!crossed && !(need && lp->reduce_to_com) && !(how_many < 9 * largest_id / 10))
+            return NCG_OK;
+        ncg_status r = download(b, d_did_collide);
+        if (r != NCG_OK) return r;
+        const int32_t n_before = n;
+        how_many = ncg_host_step_end(reinterpret_cast<ncg_body *>(hb.data()), hex.data(),
+                                     free_list.data(), largest_id, how_many_free, hcen, r_dis,
+                                     r_hd, lp->reduce_to_com, need, removed, tidy_ups, 1);
+        ++host_step_ends;
+        if ((r = upload(n_before)) != NCG_OK) return r;
+        set_n(largest_id + 1);
+        return NCG_OK;
+    };
     hipLaunchKernelGGL(k_lf_zero_i32, dim3(1), dim3(64), 0, st, reinterpret_cast<int32_t *>(err), 2);
     NCG_CHECK_LAUNCH();
     int how_many = 0;
-    ncg_status r = census(how_many);
+    bool crossed = false;
+    ncg_status r = census(how_many, nullptr);
     if (r != NCG_OK) return r;
     if (flags & 2) {
         // Half kick before the loop (rk4.c:697-717).
         if ((r = pass(1, 0.5 * dt)) != NCG_OK) return r;
         if ((r = kick(0.5 * dt)) != NCG_OK) return r;
-        if ((r = census(how_many)) != NCG_OK) return r;
+        if ((r = census(how_many, nullptr)) != NCG_OK) return r;
         update_gam(how_many);
     }
     for (int32_t s = 0; s < nsteps; ++s) {
@@ -459,11 +583,13 @@ extern "C" ncg_status ncg_mesh_leapfrog_dev(ncg_ctx *ctx, ncg_body *d_bodies, in
         NCG_CHECK_LAUNCH();
         if (!coll && (r = pass(1, dt)) != NCG_OK) return r;
         if ((r = kick(dt)) != NCG_OK) return r;
-        // end_of_timestep (rk4.c:130-393): did_collide, census, mesh size.
-        if (coll && one_collision_only)
+        // end_of_timestep (rk4.c:130-393): removals, centre of mass,
+        // did_collide, census, mesh size, tidy_up.
+        if (coll && d_did_collide && (one_collision_only || frag))
             hipLaunchKernelGGL(k_lf_zero_i32, dim3(cdiv(n, 256)), dim3(256), 0, st, d_did_collide,
                                (int)n);
-        if ((r = census(how_many)) != NCG_OK) return r;
+        if ((r = census(how_many, &crossed)) != NCG_OK) return r;
+        if ((r = step_end(how_many, crossed)) != NCG_OK) return r;
         update_gam(how_many);
     }
     if (flags & 4) {
@@ -474,7 +600,70 @@ extern "C" ncg_status ncg_mesh_leapfrog_dev(ncg_ctx *ctx, ncg_body *d_bodies, in
     }
     NCG_HIP(hipStreamSynchronize(st));
     if (n_collisions) *n_collisions = ncoll;
+    if (lp) {
+        lp->largest_id = largest_id;
+        lp->how_many_free = how_many_free;
+        std::copy(free_list.begin(), free_list.end(), lp->free_indexes);
+        lp->need_to_reduce_com = need;
+        lp->host_passes = host_passes;
+        lp->host_step_ends = host_step_ends;
+        lp->removed = removed;
+        lp->tidy_ups = tidy_ups;
+        lp->mergers = mergers;
+        lp->super_catastrophic = super_catastrophic;
+        lp->partial = partial;
+        lp->full = full;
+    }
     return NCG_OK;
+}
+
+}  // namespace
+
+extern "C" ncg_status ncg_mesh_leapfrog_dev(ncg_ctx *ctx, ncg_body *d_bodies, int32_t n,
+                                            int32_t *d_exists, int32_t *d_did_collide,
+                                            double *d_central, double grav_const,
+                                            double softening, ncg_precision prec, double dt,
+                                            int32_t nsteps, ncg_mesh_loop_params *mp,
+                                            double collision_f, int32_t one_collision_only,
+                                            int32_t flags, int64_t *n_collisions, void *stream)
+{
+    if (!ctx) return NCG_ERR_INVALID;
+    NCG_REQUIRE(n >= 3 && d_bodies && d_central && mp && nsteps >= 0, "invalid arguments");
+    NCG_REQUIRE(prec == NCG_F64_EXACT || prec == NCG_F64_FAST || prec == NCG_MIXED,
+                "unknown precision %d", (int)prec);
+    NCG_REQUIRE(mp->mesh_size > 0.0 && mp->mesh_size_min > 0.0 && mp->cells >= 3 &&
+                    mp->target_neighbours > 0.0,
+                "invalid mesh parameters");
+    NCG_REQUIRE(!(collision_f > 0.0 && one_collision_only && !d_did_collide),
+                "one_collision_only needs a did_collide array");
+    return mesh_loop_run(ctx, d_bodies, n, d_exists, d_did_collide, d_central, grav_const,
+                         softening, prec, dt, nsteps, mp, collision_f, one_collision_only, flags,
+                         nullptr, n_collisions, stream);
+}
+
+extern "C" ncg_status ncg_mesh_leapfrog_model_dev(ncg_ctx *ctx, ncg_body *d_bodies,
+                                                  int32_t *d_exists, int32_t *d_did_collide,
+                                                  double *d_central, double grav_const,
+                                                  double softening, ncg_precision prec, double dt,
+                                                  int32_t nsteps, ncg_mesh_loop_params *mp,
+                                                  ncg_leapfrog_model_params *params,
+                                                  int32_t one_collision_only, int32_t flags,
+                                                  int64_t *n_collisions, void *stream)
+{
+    if (!ctx) return NCG_ERR_INVALID;
+    NCG_REQUIRE(d_bodies && d_exists && d_central && mp && params && nsteps >= 0,
+                "invalid arguments");
+    NCG_REQUIRE(prec == NCG_F64_EXACT || prec == NCG_F64_FAST || prec == NCG_MIXED,
+                "unknown precision %d", (int)prec);
+    NCG_REQUIRE(mp->mesh_size > 0.0 && mp->mesh_size_min > 0.0 && mp->cells >= 3 &&
+                    mp->target_neighbours > 0.0,
+                "invalid mesh parameters");
+    ncg_status r = check_model_params(ctx, *params, one_collision_only, d_did_collide);
+    if (r != NCG_OK) return r;
+    NCG_REQUIRE(params->largest_id >= 2, "fewer than three slots");
+    return mesh_loop_run(ctx, d_bodies, params->largest_id + 1, d_exists, d_did_collide, d_central,
+                         grav_const, softening, prec, dt, nsteps, mp, 0.0, one_collision_only,
+                         flags, params, n_collisions, stream);
 }
 
 namespace {
@@ -775,7 +964,7 @@ ncg_status leapfrog_run(ncg_ctx *ctx, ncg_body *d_bodies, int32_t n_in, int32_t 
         int32_t lid = mh.largest_id, hmf = mh.how_many_free;
         mh.live = ncg_host_step_end(reinterpret_cast<ncg_body *>(hb.data()), hex.data(),
                                     mh.free.data(), lid, hmf, hcen, r_dis, r_hd,
-                                    mp->reduce_to_com, mh.need, mh.removed, mh.tidy_ups);
+                                    mp->reduce_to_com, mh.need, mh.removed, mh.tidy_ups, 0);
         mh.largest_id = lid;
         mh.how_many_free = hmf;
         ++mh.host_step_ends;
@@ -972,28 +1161,11 @@ extern "C" ncg_status ncg_leapfrog_model_dev(ncg_ctx *ctx, ncg_body *d_bodies, i
 {
     if (!ctx) return NCG_ERR_INVALID;
     NCG_REQUIRE(params && d_bodies && d_exists && d_central && nsteps >= 0, "invalid arguments");
-    const ncg_leapfrog_model_params &P = *params;
-    NCG_REQUIRE(P.n_max >= 1 && P.largest_id >= 0 && P.largest_id < P.n_max &&
-                    P.how_many_free >= 0 && P.how_many_free <= P.n_max && P.free_indexes,
-                "invalid slots (n_max %d, largest_id %d, how_many_free %d)", (int)P.n_max,
-                (int)P.largest_id, (int)P.how_many_free);
     NCG_REQUIRE(prec == NCG_F64_EXACT || prec == NCG_F64_FAST || prec == NCG_MIXED,
                 "unknown precision %d", (int)prec);
-    const ncg_outcome_model &M = P.model;
-    NCG_REQUIRE(M.kind >= NCG_OUTCOME_BOUNCE && M.kind <= NCG_OUTCOME_FRAGMENTATION,
-                "unknown outcome model %d", (int)M.kind);
-    NCG_REQUIRE(M.kind != NCG_OUTCOME_FRAGMENTATION ||
-                    (M.n_tilde >= 1 && M.n_tilde <= NCG_MAX_TILDE && M.pq[1] >= M.pq[0] &&
-                     M.pq[3] >= M.pq[2] &&
-                     (int64_t)(M.pq[1] - M.pq[0] + 1) * (M.pq[3] - M.pq[2] + 1) == M.n_tilde),
-                "invalid fragmentation lattice (n_tilde %d, pq %d %d %d %d)", (int)M.n_tilde,
-                (int)M.pq[0], (int)M.pq[1], (int)M.pq[2], (int)M.pq[3]);
-    const bool coll = M.kind != NCG_OUTCOME_BOUNCE || M.collision_f > 0.0;
-    NCG_REQUIRE(!(coll && one_collision_only && !d_did_collide),
-                "one_collision_only needs a did_collide array");
-    for (int32_t q = 0; q < P.how_many_free; ++q)
-        NCG_REQUIRE(P.free_indexes[q] >= 0 && P.free_indexes[q] < P.n_max,
-                    "free slot %d out of range", (int)P.free_indexes[q]);
+    ncg_status r = check_model_params(ctx, *params, one_collision_only, d_did_collide);
+    if (r != NCG_OK) return r;
+    const ncg_leapfrog_model_params &P = *params;
     return leapfrog_run(ctx, d_bodies, P.largest_id + 1, d_exists, d_exists, d_did_collide,
                         d_central, grav_const, softening, prec, dt, nsteps, 0.0, one_collision_only,
                         flags, params, n_collisions, stream);

@@ -165,11 +165,12 @@ ncg_status ncg_candidates_launch(ncg_ctx *ctx, const DBody *d_b, int32_t n, cons
 // end_of_timestep's host half (rk4.c:155-221, 395-397): removals by the
 // disruption / high-dumping distances (r_hd = +inf: none), the reduction of
 // the centre of mass, tidy_up; returns the bodies left (how_many_moonlets).
-// central: 8 doubles {x, y, z, vx, vy, vz, M, present}.
+// central: 8 doubles {x, y, z, vx, vy, vz, M, present}; mesh: the slot
+// rules of mesh mode (no list of free slots).
 int32_t ncg_host_step_end(ncg_body *X, int32_t *exists, int32_t *free_indexes, int32_t &largest_id,
                           int32_t &how_many_free, double *central, double r_dis, double r_hd,
                           int reduce_to_com, int32_t &need_to_reduce, int32_t &removed,
-                          int32_t &tidied);
+                          int32_t &tidied, int mesh);
 
 constexpr int NCG_REPLAY_RCAP = 1024;  // candidates of one system per pass
 constexpr int NCG_REPLAY_DCAP = 256;   // bodies changed in one system per pass

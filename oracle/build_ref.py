@@ -183,6 +183,14 @@ CONFIGS = {
         "how_many_neighbours": "30.0", "switch_to_brute_force": "110",
         "sma_min": "0.9", "sma_max": "1.1", "inclination_max": "0.01",
         "inelastic_collision_bool": "0", "fragmentation_bool": "1", "N_max": "40000"})),
+    # The fragmentation model with end_of_timestep's removals on the mesh
+    # path (bodies inside 0.93 or beyond 1.07 removed, then a compaction).
+    "run_mesh_rm": ("run", _run_cfg("mesh", 16384, 10, DT2, {
+        "collision_cube_cells": "256", "collision_cube_min": "2.56",
+        "how_many_neighbours": "100.0", "switch_to_brute_force": "110",
+        "sma_min": "0.9", "sma_max": "1.1", "inclination_max": "0.01",
+        "inelastic_collision_bool": "0", "fragmentation_bool": "1", "N_max": "40000",
+        "disruption_threshold": "0.93", "high_dumping_threshold": "1.07"})),
     # Inelastic bounces with bodies larger than the mesh size (the
     # GPU-resident mesh loop replays such passes on the host).
     "run_mesh_big": ("run", _run_cfg("mesh", 16384, 10, DT2, {

@@ -468,3 +468,83 @@ def test_leapfrog_model_one_call_matches_reference(ctx, tmp_path, cfg, kind, n_m
     ctx.synchronize()
     assert _model_hash(t, state) == vf[-1]
     assert state["host_passes"] > 10
+
+
+# Mesh runs with the collision models and removals: (config, mass, radius,
+# N_max, disruption_threshold, high_dumping_threshold); 16,384-body disks
+# (seed 2) as in test_gpu_dropin.py.
+MESH_MODEL_RUNS = [("run_mesh_frag", "fragmentation", 1e-8, 2e-3, 40000, 0.05, 100.0),
+                   ("run_mesh_merge", "merger", 1e-10, 1e-3, 16400, 0.05, 100.0),
+                   ("run_mesh_rm", "fragmentation", 1e-8, 2e-3, 40000, 0.93, 1.07)]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("cfg,kind,mass,radius,n_max,rdis,rhd", MESH_MODEL_RUNS)
+def test_mesh_loop_model_reproduces_reference_run(ctx, tmp_path, cfg, kind, mass, radius, n_max,
+                                                  rdis, rhd):
+    """NcorpiON's mesh integrator with its default collision model (mergers,
+    partial, full and super-catastrophic fragmentation: fragments appended
+    with their pairs) or instant mergers, and with end_of_timestep's
+    removals: from the reference's own starting state, the state its next
+    mesh() sees (every slot up to largest_id, after the three_largest swap)
+    has the same hash after every device step."""
+    torch = tt()
+    from ncorpion_amd import _abi
+    b0 = ic.disk(16384, seed=2, mass=mass, radius=radius)
+    exe = os.path.join(REF, cfg, "ncorpion_trace")
+    if not os.path.exists(exe):
+        pytest.skip("reference builds missing (oracle/build_ref.py)")
+    ic.write_init_txt(str(tmp_path / "init.txt"), b0)
+    env = dict(os.environ, NCG_TRACE=str(tmp_path / "trace.txt"),
+               NCG_TRACE_DUMP=str(tmp_path / "dump.bin"))
+    r = subprocess.run([exe], cwd=str(tmp_path), env=env, stdout=subprocess.PIPE,
+                       stderr=subprocess.STDOUT, text=True, timeout=600)
+    assert r.returncode == 0, r.stdout[-2000:]
+    lines = open(tmp_path / "trace.txt").read().splitlines()
+    mesh_in, coll = [], []
+    for l in lines:
+        f = l.split()
+        if f[0] == "MESH" and f[2] == "in":
+            mesh_in.append(int(f[3], 16))
+            coll.append(0)
+        elif f[0] == "C" and coll:
+            coll[-1] += 1
+    bodies, ex, dc, cm, ts, gam, cube, hm = read_dump_mesh(str(tmp_path / "dump.bin"))
+    assert len(mesh_in) == 10
+    n0 = bodies.shape[0]
+    B = np.zeros((n_max, 8))
+    B[:n0] = bodies
+    E = np.zeros(n_max, np.int32)
+    E[:n0] = ex
+    DC = np.zeros(n_max, np.int32)
+    DC[:n0] = dc
+    bt, et, dct = (torch.from_numpy(a).cuda() for a in (B, E, DC))
+    ct = torch.from_numpy(np.concatenate([cm[0:7], [1.0]])).cuda()
+    dt = 2.0 * ts  # dumped inside the half kick (timestep halved)
+    params = dict(gam=gam, gam_min=2.56 / 256, how_many_neighbours=100.0, cells=256)
+    state = dict(largest_id=n0 - 1, how_many_free=0, free_indexes=np.zeros(n_max, np.int32),
+                 need_to_reduce_com=0)
+    model = _abi.outcome_model(kind, G=1.0)
+    kw = dict(G=1.0, softening=0.0, precision="exact", did_collide_t=dct, r_disruption=rdis,
+              r_high_dumping=rhd)
+    ctx.mesh_leapfrog_model_dev(bt, ct, et, 0, dt, params, state, model, half_kick_start=True,
+                                **kw)
+    total = 0
+    for s, h in enumerate(mesh_in):
+        ctx.synchronize()
+        n = state["largest_id"] + 1
+        st = bt.cpu().numpy()[:n].copy()
+        e = et.cpu().numpy()[:n].copy()
+        three_largest_first(st, e)
+        assert orc.state_hash(st, e) == h, "step %d" % s
+        nc = ctx.mesh_leapfrog_model_dev(bt, ct, et, 1, dt, params, state, model, **kw)
+        assert nc <= coll[s], "step %d" % s
+        total += nc
+    print("%s: %d collisions (%d treatment calls), mergers %d, super-catastrophic %d, "
+          "partial %d, full %d, removed %d, tidy_ups %d, largest_id %d, host passes %d" % (
+              cfg, total, sum(coll), state["mergers"], state["super_catastrophic"],
+              state["partial"], state["full"], state["removed"], state["tidy_ups"],
+              state["largest_id"], state["host_passes"]))
+    assert total > 100
+    if cfg == "run_mesh_rm":
+        assert state["removed"] > 100 and state["tidy_ups"] > 0
