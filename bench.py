@@ -223,6 +223,7 @@ def mesh_run(ctx, torch, passes=3):
     ref = None
     for prec in ("exact", "fast64", "mixed"):
         acc, _ = ctx.mesh_gravity(b, ex, None, 1.0, 0.0, prec)
+        ctx.profile_read("mesh_gravity", reset=True)  # the first call of a form is not timed
         acc, _ = ctx.mesh_gravity(b, ex, None, 1.0, 0.0, prec)
         ms, k = ctx.profile_read("mesh_gravity", reset=True)
         ms /= k
@@ -238,6 +239,9 @@ def mesh_run(ctx, torch, passes=3):
             r = np.linalg.norm(ref, axis=1)
             g["max_rel_err_vs_exact"] = float((d / r).max())
             g["normwise_err_vs_exact"] = float(d.max() / np.sqrt(np.mean(r * r)))
+        if prec == "mixed":
+            g["form"] = ("the FP64-fast kernel: every neighbour term is a near one, where FP32 "
+                         "terms lose ~1e-5 to cancellation")
         grav[prec] = g
     out["gravity"] = grav
     ctx.profile(False)
