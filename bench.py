@@ -730,6 +730,11 @@ def adjoint_run(ctx, torch, n=16384, steps=200, K=20, fd_dirs=4):
             "note": "backward = segment recomputation from the checkpoints + 4 VJPs per step"}
 
 
+
+def progress(section):
+    """One line on stderr per finished section (the JSON line stays alone on stdout)."""
+    print("bench.py: %s done" % section, file=sys.stderr, flush=True)
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -869,23 +874,33 @@ def main():
                            "normwise_max_err": float(d.max() / torch.sqrt(torch.mean(r * r))),
                            "reference": "FP64-fast form (mutual acceleration only)"}
 
+    progress("headline")
     if not args.no_collisions:
         out["collision_filter"] = collision_filter_run(ctx, b, dt)
+        progress("collision_filter")
     if not args.no_precision:
         out["precision_study"] = precision_run(ctx, torch, b, soft, dt)
+        progress("precision_study")
     if not args.no_leapfrog:
         out["leapfrog_c1"] = leapfrog_run(ctx, torch)
+        progress("leapfrog_c1")
     if not args.no_mesh:
         out["mesh_module"] = mesh_run(ctx, torch)
+        progress("mesh_module")
         out["mesh_loop"] = mesh_loop_run(ctx, torch)
+        progress("mesh_loop")
     if not args.no_output:
         out["output"] = output_run(ctx)
+        progress("output")
     if not args.no_dropin:
         out["dropin"] = dropin_run()
+        progress("dropin")
     if not args.no_ensemble:
         out["ensemble"] = ensemble_run(ctx, torch)
+        progress("ensemble")
     if not args.no_adjoint:
         out["adjoint"] = adjoint_run(ctx, torch)
+        progress("adjoint")
     if args.copy_calibration_mib > 0:
         # Plain elementwise copy (reads and writes the stated bytes once):
         # the reference point for the HBM byte counters.
