@@ -156,6 +156,11 @@ CONFIGS = {
     # to the central body than 0.93 merge into it, bodies beyond 1.07 are
     # dropped (the centre of mass is then reduced), and the array is
     # compacted once a tenth of its slots are empty (tidy_up).
+    # The fragmentation model with one collision per body and step
+    # (did_collide set by every outcome, cleared by end_of_timestep).
+    "run_c1_frag1": ("run", _run_cfg("brute", 2000, 40, DT1, {
+        "inelastic_collision_bool": "0", "fragmentation_bool": "1", "N_max": "4000",
+        "one_collision_only_bool": "1"})),
     "run_c1_rm": ("run", _run_cfg("brute", 2000, 40, DT1, {
         "inelastic_collision_bool": "0", "fragmentation_bool": "1", "N_max": "4000",
         "disruption_threshold": "0.93", "high_dumping_threshold": "1.07"})),

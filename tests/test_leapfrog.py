@@ -393,7 +393,9 @@ def test_leapfrog_replay_overflow_replayed_on_host(tmp_path, graph):
 # oracle/build_ref.py.  Initial conditions as in test_gpu_dropin.py.
 MODEL_RUNS = [("run_c1_frag", "fragmentation", 4000, 0.05, 100.0),
               ("run_c1_merge", "merger", 2016, 0.05, 100.0),
-              ("run_c1_rm", "fragmentation", 4000, 0.93, 1.07)]
+              ("run_c1_rm", "fragmentation", 4000, 0.93, 1.07),
+              ("run_c1_frag1", "fragmentation", 4000, 0.05, 100.0)]
+ONE_COLLISION = {"run_c1_frag1": 1}
 
 
 def _model_setup(ctx, tmp_path, cfg, kind, n_max):
@@ -432,7 +434,8 @@ def test_leapfrog_model_reproduces_reference_run(ctx, tmp_path, cfg, kind, n_max
     up to largest_id) equal to the reference's at each vector_field call."""
     t, state, model, dt, vf, ncoll = _model_setup(ctx, tmp_path, cfg, kind, n_max)
     kw = dict(G=1.0, softening=0.0, precision="exact", did_collide_t=t["dc"],
-              r_disruption=rdis, r_high_dumping=rhd)
+              r_disruption=rdis, r_high_dumping=rhd,
+              one_collision_only=ONE_COLLISION.get(cfg, 0))
     total = ctx.leapfrog_model_dev(t["b"], t["c"], t["e"], 0, dt, state, model,
                                    half_drift_start=True, **kw)
     for s, h in enumerate(vf):
@@ -464,7 +467,7 @@ def test_leapfrog_model_one_call_matches_reference(ctx, tmp_path, cfg, kind, n_m
     ctx.leapfrog_model_dev(t["b"], t["c"], t["e"], len(vf) - 1, dt, state, model, G=1.0,
                            softening=0.0, precision="exact", did_collide_t=t["dc"],
                            r_disruption=rdis, r_high_dumping=rhd, graph=True,
-                           half_drift_start=True)
+                           half_drift_start=True, one_collision_only=ONE_COLLISION.get(cfg, 0))
     ctx.synchronize()
     assert _model_hash(t, state) == vf[-1]
     assert state["host_passes"] > 10
@@ -548,3 +551,37 @@ def test_mesh_loop_model_reproduces_reference_run(ctx, tmp_path, cfg, kind, mass
     assert total > 100
     if cfg == "run_mesh_rm":
         assert state["removed"] > 100 and state["tidy_ups"] > 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["fast64", "mixed"])
+def test_leapfrog_model_tiled_field_one_call_equals_steps(ctx, tmp_path, prec):
+    """With a tiled field (whose spatial order is rebuilt every step over the
+    slots in use) and the fragmentation model with removals, one 39-step
+    call replayed from graphs ends in the same state, bit for bit, as 39
+    calls of one step: windows, re-runs and recaptures after the slot count
+    changes do not alter the run."""
+    runs = []
+    for one_call in (True, False):
+        d = tmp_path / ("one_call" if one_call else "steps")
+        d.mkdir()
+        t, state, model, dt, vf, ncoll = _model_setup(ctx, d, "run_c1_rm", "fragmentation", 4000)
+        kw = dict(G=1.0, softening=0.0, precision=prec, did_collide_t=t["dc"],
+                  r_disruption=0.93, r_high_dumping=1.07)
+        if one_call:
+            nc = ctx.leapfrog_model_dev(t["b"], t["c"], t["e"], 39, dt, state, model, graph=True,
+                                        half_drift_start=True, **kw)
+        else:
+            nc = ctx.leapfrog_model_dev(t["b"], t["c"], t["e"], 0, dt, state, model,
+                                        half_drift_start=True, **kw)
+            for _ in range(39):
+                nc += ctx.leapfrog_model_dev(t["b"], t["c"], t["e"], 1, dt, state, model, **kw)
+        ctx.synchronize()
+        n = state["largest_id"] + 1
+        runs.append((nc, n, t["b"].cpu().numpy()[:n].copy(), t["e"].cpu().numpy()[:n].copy(),
+                     t["c"].cpu().numpy().copy(), state["removed"], state["full"]))
+    (n1, s1, b1, e1, c1, r1, f1), (n2, s2, b2, e2, c2, r2, f2) = runs
+    assert n1 == n2 > 100 and s1 == s2 and r1 == r2 > 100 and f1 == f2 > 0
+    assert np.array_equal(e1, e2)
+    assert np.array_equal(b1[e1 != 0].view(np.uint64), b2[e2 != 0].view(np.uint64))
+    assert np.array_equal(c1.view(np.uint64), c2.view(np.uint64))
