@@ -160,6 +160,19 @@ ncg_status ncg_gravity_allpairs(ncg_ctx *ctx, const ncg_body *bodies, int32_t n,
                                 double grav_const, double softening, ncg_precision prec,
                                 double *acc, double *cm_acc);
 
+/* ncg_gravity_allpairs for a vector_field with terms beyond the central
+ * point mass (the J2 bulge and the point-mass perturber, physics.c:148-172,
+ * and the inner fluid disk's mass in mu, physics.c:79): start (3n) holds,
+ * for every existing body, the value vector_field has accumulated before the
+ * mutual terms (the central, J2 and perturber terms, in the reference's
+ * order), and each body's mutual terms are added to it in the reference's
+ * order (NCG_F64_EXACT: bit for bit; the other forms add it after their
+ * sums).  The caller also forms CM_acc, which those terms feed. */
+ncg_status ncg_gravity_allpairs_from(ncg_ctx *ctx, const ncg_body *bodies, int32_t n,
+                                     const int32_t *exists, const double *start,
+                                     double grav_const, double softening, ncg_precision prec,
+                                     double *acc);
+
 /* Same on device buffers: bodies (n records), exists (n or NULL), acc (3n),
  * cm_acc (3 doubles or NULL; in the reference's order for NCG_F64_EXACT, as
  * above). */
@@ -322,6 +335,14 @@ ncg_status ncg_mesh_gravity(ncg_ctx *ctx, const ncg_body *bodies, int32_t n,
                             const int32_t *exists, const ncg_central *central, double grav_const,
                             double softening, ncg_precision prec, double *acc,
                             double *cm_acc);
+
+/* ncg_mesh_gravity with the caller's start values, as
+ * ncg_gravity_allpairs_from: body k's sum starts from start[3k .. 3k + 2]
+ * instead of the central term, then takes bodies 0..2 and the pass's pairs
+ * in the reference's order. */
+ncg_status ncg_mesh_gravity_from(ncg_ctx *ctx, const ncg_body *bodies, int32_t n,
+                                 const int32_t *exists, const double *start, double grav_const,
+                                 double softening, ncg_precision prec, double *acc);
 
 /* ncg_mesh_gravity on device arrays, enqueued on `stream` (NULL: the
  * context's stream) without synchronisation: d_bodies (n records), d_exists

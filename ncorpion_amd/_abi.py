@@ -149,6 +149,8 @@ _SIGNATURES = {
                           ctypes.POINTER(MeshStats)]),
     "ncg_mesh_gravity": (S, [P, P, I32, P, ctypes.POINTER(Central), D, D, S, P, P]),
     "ncg_mesh_gravity_dev": (S, [P, P, I32, P, P, D, D, S, P, P, P]),
+    "ncg_gravity_allpairs_from": (S, [P, P, I32, P, P, D, D, S, P]),
+    "ncg_mesh_gravity_from": (S, [P, P, I32, P, P, D, D, S, P]),
     "ncg_mesh_leapfrog_dev": (S, [P, P, I32, P, P, P, D, D, S, D, I32,
                                   ctypes.POINTER(MeshLoopParams), D, I32, I32,
                                   ctypes.POINTER(I64), P]),
@@ -285,6 +287,20 @@ class Context:
                                                   float(G), float(softening),
                                                   PRECISIONS[precision], _ptr(out), _ptr(cm)))
         return out, cm
+
+    def gravity_allpairs_from(self, bodies, start, exists=None, G=1.0, softening=0.0,
+                              precision="exact"):
+        """Mutual accelerations added to the caller's start values (3 per body;
+        see ncg_gravity_allpairs_from)."""
+        b = as_bodies(bodies)
+        n = b.shape[0]
+        e = as_exists(exists, n)
+        st = np.ascontiguousarray(start, dtype=np.float64).reshape(n, 3)
+        out = np.zeros((n, 3))
+        self._check(self.lib.ncg_gravity_allpairs_from(self.h, _ptr(b), n, _ptr(e), _ptr(st),
+                                                       float(G), float(softening),
+                                                       PRECISIONS[precision], _ptr(out)))
+        return out
 
     def gravity_allpairs_dev(self, bodies_t, acc_t, exists_t=None, central=None, G=1.0,
                              softening=0.0, precision="mixed", cm_t=None, stream=None):
@@ -682,6 +698,19 @@ class Context:
                                               float(G), float(softening), PRECISIONS[precision],
                                               _ptr(out), _ptr(cm)))
         return out, cm
+
+    def mesh_gravity_from(self, bodies, start, exists=None, G=1.0, softening=0.0,
+                          precision="exact"):
+        """ncg_mesh_gravity_from: the last pass's field added to start values."""
+        b = as_bodies(bodies)
+        n = b.shape[0]
+        e = as_exists(exists, n)
+        st = np.ascontiguousarray(start, dtype=np.float64).reshape(n, 3)
+        out = np.zeros((n, 3))
+        self._check(self.lib.ncg_mesh_gravity_from(self.h, _ptr(b), n, _ptr(e), _ptr(st),
+                                                   float(G), float(softening),
+                                                   PRECISIONS[precision], _ptr(out)))
+        return out
 
     # -- output path -------------------------------------------------------------
     def format_fixed(self, values, decimals=14):

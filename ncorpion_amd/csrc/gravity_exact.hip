@@ -48,6 +48,7 @@ __global__ __launch_bounds__(256) void k_grav_exact(const DBody *__restrict__ b,
                                                     double cx, double cy, double cz, double mu,
                                                     const double *__restrict__ dcen,
                                                     double G, double s,
+                                                    const double *__restrict__ init,
                                                     double *__restrict__ acc,
                                                     double *__restrict__ cterm)
 {
@@ -70,7 +71,11 @@ __global__ __launch_bounds__(256) void k_grav_exact(const DBody *__restrict__ b,
         Z = b[k].z;
         Rk = b[k].radius;
         const double mk = b[k].mass;
-        if (cp) {
+        if (init) {  // the caller's start value (physics.c:121-172 beyond the point mass)
+            aX = init[3 * (size_t)k];
+            aY = init[3 * (size_t)k + 1];
+            aZ = init[3 * (size_t)k + 2];
+        } else if (cp) {
             // physics.c:121-146
             const double DX = X - cx, DY = Y - cy, DZ = Z - cz;
             const double rk = sqrt(DX * DX + DY * DY + DZ * DZ);
@@ -219,6 +224,7 @@ __global__ __launch_bounds__(256) void k_grav_exact_w(const DBody *__restrict__ 
                                                       double cx, double cy, double cz, double mu,
                                                       const double *__restrict__ dcen,
                                                       double G, double s,
+                                                      const double *__restrict__ init,
                                                       double *__restrict__ acc,
                                                       double *__restrict__ cterm)
 {
@@ -236,7 +242,9 @@ __global__ __launch_bounds__(256) void k_grav_exact_w(const DBody *__restrict__ 
     const DBody me = b[k];
     const double X = me.x, Y = me.y, Z = me.z, Rk = me.radius;
     double a = 0.;  // component `lane` (lanes 0..2)
-    if (cp) {
+    if (init) {
+        if (lane < 3) a = init[3 * (size_t)k + lane];
+    } else if (cp) {
         // physics.c:121-146
         const double DX = X - cx, DY = Y - cy, DZ = Z - cz;
         const double rk = sqrt(DX * DX + DY * DY + DZ * DZ);
@@ -301,18 +309,18 @@ __global__ __launch_bounds__(256) void k_grav_exact_w(const DBody *__restrict__ 
 // reaction terms when central->present.
 ncg_status ncg_grav_exact_launch(ncg_ctx *ctx, const DBody *d_b, int32_t n, const int32_t *d_ex,
                                  const ncg_central *c, double G, double soft, double *d_acc,
-                                 double *d_cterm, hipStream_t st)
+                                 double *d_cterm, hipStream_t st, const double *d_init)
 {
     const int cp = (c && c->present) ? 1 : 0;
     const double mu = cp ? G * c->mass : 0.0;  // physics.c:131
     if (n <= EXW_MAX)
         hipLaunchKernelGGL(k_grav_exact_w, dim3(cdiv(n, 4)), dim3(256), 0, st, d_b, (int)n, d_ex, cp,
                            cp ? c->x : 0.0, cp ? c->y : 0.0, cp ? c->z : 0.0, mu,
-                           (const double *)nullptr, G, soft, d_acc, d_cterm);
+                           (const double *)nullptr, G, soft, d_init, d_acc, d_cterm);
     else
         hipLaunchKernelGGL(k_grav_exact, dim3(cdiv(n, 256)), dim3(256), 0, st, d_b, (int)n, d_ex, cp,
                            cp ? c->x : 0.0, cp ? c->y : 0.0, cp ? c->z : 0.0, mu,
-                           (const double *)nullptr, G, soft, d_acc, d_cterm);
+                           (const double *)nullptr, G, soft, d_init, d_acc, d_cterm);
     NCG_CHECK_LAUNCH();
     return NCG_OK;
 }
@@ -327,10 +335,12 @@ ncg_status ncg_grav_exact_dev(ncg_ctx *ctx, const DBody *d_b, int32_t n, const i
         ProfScope ps(ctx, st, "exact");
         if (n <= EXW_MAX)
             hipLaunchKernelGGL(k_grav_exact_w, dim3(cdiv(n, 4)), dim3(256), 0, st, d_b, (int)n, d_ex,
-                               0, 0.0, 0.0, 0.0, 0.0, d_cen, G, soft, d_acc, d_cterm);
+                               0, 0.0, 0.0, 0.0, 0.0, d_cen, G, soft, (const double *)nullptr, d_acc,
+                               d_cterm);
         else
             hipLaunchKernelGGL(k_grav_exact, dim3(cdiv(n, 256)), dim3(256), 0, st, d_b, (int)n, d_ex,
-                               0, 0.0, 0.0, 0.0, 0.0, d_cen, G, soft, d_acc, d_cterm);
+                               0, 0.0, 0.0, 0.0, 0.0, d_cen, G, soft, (const double *)nullptr, d_acc,
+                               d_cterm);
         NCG_CHECK_LAUNCH();
     }
     if (d_cm_acc) {

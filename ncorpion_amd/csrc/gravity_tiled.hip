@@ -803,9 +803,20 @@ ncg_status ncg_tiled_eval(ncg_ctx *ctx, const DBody *d_b, int32_t n, const int32
     return tiled_eval_w(ctx, t, w, d_b, n, d_ex, c, d_central, G, soft, prec, d_acc, d_cm_acc, st);
 }
 
+// acc += start value for every existing body (the fast forms with start values).
+__global__ __launch_bounds__(NT) void k_add_start3(int n, const int32_t *__restrict__ ex,
+                                                   const double *__restrict__ init,
+                                                   double *__restrict__ acc)
+{
+    const int k = blockIdx.x * NT + threadIdx.x;
+    if (k >= n || (ex && !ex[k])) return;
+    for (int c = 0; c < 3; ++c) acc[3 * (size_t)k + c] += init[3 * (size_t)k + c];
+}
+
 ncg_status ncg_gravity_dev_impl(ncg_ctx *ctx, const DBody *d_b, int32_t n, const int32_t *d_ex,
                                 const ncg_central *c, double G, double soft, ncg_precision prec,
-                                double *d_acc, double *d_cterm, double *d_cm_acc, hipStream_t st)
+                                double *d_acc, double *d_cterm, double *d_cm_acc, hipStream_t st,
+                                const double *d_init)
 {
     NCG_REQUIRE(n >= 0, "n must be >= 0");
     if (n == 0) return NCG_OK;
@@ -819,7 +830,7 @@ ncg_status ncg_gravity_dev_impl(ncg_ctx *ctx, const DBody *d_b, int32_t n, const
         ncg_status r;
         {
             ProfScope ps(ctx, st, "exact");
-            r = ncg_grav_exact_launch(ctx, d_b, n, d_ex, c, G, soft, d_acc, cterm, st);
+            r = ncg_grav_exact_launch(ctx, d_b, n, d_ex, c, G, soft, d_acc, cterm, st, d_init);
         }
         if (r != NCG_OK) return r;
         // CM_acc in the reference's order: the terms added body by body.
@@ -829,8 +840,12 @@ ncg_status ncg_gravity_dev_impl(ncg_ctx *ctx, const DBody *d_b, int32_t n, const
     NCG_REQUIRE(prec == NCG_F64_FAST || prec == NCG_MIXED, "unknown precision %d", (int)prec);
     ncg_status r = ncg_tiled_order(ctx, d_b, n, d_ex, prec, st);
     if (r != NCG_OK) return r;
-    return ncg_tiled_eval(ctx, d_b, n, d_ex, cp ? c : nullptr, nullptr, G, soft, prec, d_acc,
-                          d_cterm, cp ? d_cm_acc : nullptr, st);
+    r = ncg_tiled_eval(ctx, d_b, n, d_ex, cp ? c : nullptr, nullptr, G, soft, prec, d_acc, d_cterm,
+                       cp ? d_cm_acc : nullptr, st);
+    if (r != NCG_OK || !d_init) return r;
+    hipLaunchKernelGGL(k_add_start3, dim3(cdiv(n, NT)), dim3(NT), 0, st, (int)n, d_ex, d_init, d_acc);
+    NCG_CHECK_LAUNCH();
+    return NCG_OK;
 }
 
 // ---- allocation-free evaluation ----------------------------------------------
@@ -946,7 +961,7 @@ extern "C" ncg_status ncg_gravity_allpairs_static_dev(ncg_ctx *ctx, const ncg_bo
     // nothing needs clearing between calls.
     if (prec == NCG_F64_EXACT) {
         ncg_status r = ncg_grav_exact_launch(ctx, db, n, d_exists, cp ? central : nullptr, G,
-                                             softening_parameter, d_acc, w.cterm, st);
+                                             softening_parameter, d_acc, w.cterm, st, nullptr);
         if (r != NCG_OK) return r;
         if (want_cm) return ncg_cm_seq_launch(ctx, w.cterm, n, d_exists, 1, d_cm_acc, st);
         return NCG_OK;
@@ -979,10 +994,11 @@ extern "C" ncg_status ncg_gravity_allpairs_dev(ncg_ctx *ctx, const ncg_body *d_b
                                 pick_stream(ctx, stream));
 }
 
-extern "C" ncg_status ncg_gravity_allpairs(ncg_ctx *ctx, const ncg_body *bodies, int32_t n,
-                                           const int32_t *exists, const ncg_central *central,
-                                           double G, double softening_parameter,
-                                           ncg_precision prec, double *acc, double *cm_acc)
+// ncg_gravity_allpairs and ncg_gravity_allpairs_from (start: 3n start values, or null).
+static ncg_status gravity_host(ncg_ctx *ctx, const ncg_body *bodies, int32_t n,
+                               const int32_t *exists, const ncg_central *central,
+                               const double *start, double G, double softening_parameter,
+                               ncg_precision prec, double *acc, double *cm_acc)
 {
     if (!ctx) return NCG_ERR_INVALID;
     NCG_REQUIRE(n >= 0 && (n == 0 || (bodies && acc)), "invalid arguments");
@@ -998,13 +1014,18 @@ extern "C" ncg_status ncg_gravity_allpairs(ncg_ctx *ctx, const ncg_body *bodies,
     double *dacc = dbuf<double>(ctx, "h_acc", 3 * (size_t)n);
     double *dct = cp ? dbuf<double>(ctx, "h_cterm", 3 * (size_t)n) : nullptr;
     double *dcm = dbuf<double>(ctx, "h_cm", 4);
-    if (!db || (exists && !dex) || !dacc || (cp && !dct) || !dcm) return NCG_ERR_NOMEM;
+    double *dinit = start ? dbuf<double>(ctx, "h_start", 3 * (size_t)n) : nullptr;
+    if (!db || (exists && !dex) || !dacc || (cp && !dct) || !dcm || (start && !dinit))
+        return NCG_ERR_NOMEM;
     NCG_HIP(hipMemcpyAsync(db, bodies, sizeof(DBody) * (size_t)n, hipMemcpyHostToDevice, st));
+    if (start)
+        NCG_HIP(hipMemcpyAsync(dinit, start, sizeof(double) * 3 * (size_t)n, hipMemcpyHostToDevice,
+                               st));
     if (exists)
         NCG_HIP(hipMemcpyAsync(dex, exists, sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, st));
     const bool exact = prec == NCG_F64_EXACT;
     ncg_status r = ncg_gravity_dev_impl(ctx, db, n, dex, central, G, softening_parameter, prec,
-                                        dacc, dct, (cp && !exact) ? dcm : nullptr, st);
+                                        dacc, dct, (cp && !exact) ? dcm : nullptr, st, dinit);
     if (r != NCG_OK) return r;
     // Holes keep the caller's values: copy back per existing body.
     double *hacc = hbuf<double>(ctx, "acc", 3 * (size_t)n);
@@ -1043,4 +1064,24 @@ extern "C" ncg_status ncg_gravity_allpairs(ncg_ctx *ctx, const ncg_body *bodies,
         }
     }
     return NCG_OK;
+}
+
+extern "C" ncg_status ncg_gravity_allpairs(ncg_ctx *ctx, const ncg_body *bodies, int32_t n,
+                                           const int32_t *exists, const ncg_central *central,
+                                           double G, double softening_parameter,
+                                           ncg_precision prec, double *acc, double *cm_acc)
+{
+    return gravity_host(ctx, bodies, n, exists, central, nullptr, G, softening_parameter, prec, acc,
+                        cm_acc);
+}
+
+extern "C" ncg_status ncg_gravity_allpairs_from(ncg_ctx *ctx, const ncg_body *bodies, int32_t n,
+                                                const int32_t *exists, const double *start,
+                                                double G, double softening_parameter,
+                                                ncg_precision prec, double *acc)
+{
+    if (!ctx) return NCG_ERR_INVALID;
+    NCG_REQUIRE(n == 0 || start, "invalid arguments");
+    return gravity_host(ctx, bodies, n, exists, nullptr, start, G, softening_parameter, prec, acc,
+                        nullptr);
 }

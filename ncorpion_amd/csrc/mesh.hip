@@ -1040,6 +1040,26 @@ __device__ inline void central_term(double x, double y, double z, double m, doub
     ct[2] = G * m * DZ / r3;
 }
 
+// A caller's start value of body k (vector_field's value before its mutual
+// terms: the central, J2 and perturber terms, physics.c:121-172), in place of
+// the central term.
+__device__ inline void start_value(const double *init, int k, double &ax, double &ay, double &az)
+{
+    ax = init[3 * (size_t)k];
+    ay = init[3 * (size_t)k + 1];
+    az = init[3 * (size_t)k + 2];
+}
+
+// acc += start value for every existing body (the fast forms).
+__global__ __launch_bounds__(MT) void k_add_start(int n, const int32_t *__restrict__ ex,
+                                                  const double *__restrict__ init,
+                                                  double *__restrict__ acc)
+{
+    const int k = blockIdx.x * MT + threadIdx.x;
+    if (k >= n || (ex && !ex[k])) return;
+    for (int c = 0; c < 3; ++c) acc[3 * (size_t)k + c] += init[3 * (size_t)k + c];
+}
+
 // Bodies the sorted kernel does not cover: k <= 2 (central term only; their
 // mutual part is added by k_big_final) and k > 2 not binned, appended ones
 // (k >= nkey) included (central term and bodies 0..2).
@@ -1447,7 +1467,8 @@ __global__ __launch_bounds__(MT) void k_mord(const DBody *__restrict__ b, int n,
                                              const unsigned *__restrict__ L,
                                              const unsigned *__restrict__ lw,
                                              const unsigned *__restrict__ le,
-                                             double *__restrict__ acc)
+                                             double *__restrict__ acc,
+                                             const double *__restrict__ init)
 {
     const int cp = cen && cen[7] != 0.0;
     const double cx = cp ? cen[0] : 0.0, cy = cp ? cen[1] : 0.0, cz = cp ? cen[2] : 0.0;
@@ -1458,7 +1479,8 @@ __global__ __launch_bounds__(MT) void k_mord(const DBody *__restrict__ b, int n,
     const double4 me = pm[t];
     const double Rk = rr[t];
     double ax = 0., ay = 0., az = 0., ct[3];
-    if (cp) central_term(me.x, me.y, me.z, me.w, cx, cy, cz, mu, G, ax, ay, az, ct);
+    if (init) start_value(init, k, ax, ay, az);
+    else if (cp) central_term(me.x, me.y, me.z, me.w, cx, cy, cz, mu, G, ax, ay, az, ct);
     for (int big = 0; big < 3 && big < n; ++big) {
         if (ex && !ex[big]) continue;
         const DBody g = b[big];
@@ -1561,7 +1583,8 @@ __global__ __launch_bounds__(MT) void k_mrest_ord(const DBody *__restrict__ b, i
                                                   double G, double s, const int *__restrict__ bvk,
                                                   const int *__restrict__ bvoff,
                                                   const int *__restrict__ bvo, int nbv,
-                                                  double *__restrict__ acc)
+                                                  double *__restrict__ acc,
+                                                  const double *__restrict__ init)
 {
     const int cp = cen && cen[7] != 0.0;
     const double cx = cp ? cen[0] : 0.0, cy = cp ? cen[1] : 0.0, cz = cp ? cen[2] : 0.0;
@@ -1570,7 +1593,8 @@ __global__ __launch_bounds__(MT) void k_mrest_ord(const DBody *__restrict__ b, i
     if (k >= n || (ex && !ex[k]) || (k < nkey && key[k] != NOKEY)) return;
     const DBody q = b[k];
     double ax = 0., ay = 0., az = 0., ct[3];
-    if (cp) central_term(q.x, q.y, q.z, q.mass, cx, cy, cz, mu, G, ax, ay, az, ct);
+    if (init) start_value(init, k, ax, ay, az);
+    else if (cp) central_term(q.x, q.y, q.z, q.mass, cx, cy, cz, mu, G, ax, ay, az, ct);
     for (int big = 0; big < 3; ++big) {
         if (ex && !ex[big]) continue;
         const DBody g = b[big];
@@ -1599,7 +1623,8 @@ __global__ __launch_bounds__(MT) void k_mexplicit(const DBody *__restrict__ b,
                                                   const double *__restrict__ cen, double G, double s,
                                                   const int *__restrict__ tg, int ntg,
                                                   const int *__restrict__ pl,
-                                                  double *__restrict__ acc)
+                                                  double *__restrict__ acc,
+                                                  const double *__restrict__ init)
 {
     const int cp = cen && cen[7] != 0.0;
     const double cx = cp ? cen[0] : 0.0, cy = cp ? cen[1] : 0.0, cz = cp ? cen[2] : 0.0;
@@ -1610,7 +1635,8 @@ __global__ __launch_bounds__(MT) void k_mexplicit(const DBody *__restrict__ b,
     if (k <= 2 || (ex && !ex[k])) return;
     const DBody q = b[k];
     double ax = 0., ay = 0., az = 0., ct[3];
-    if (cp) central_term(q.x, q.y, q.z, q.mass, cx, cy, cz, mu, G, ax, ay, az, ct);
+    if (init) start_value(init, k, ax, ay, az);
+    else if (cp) central_term(q.x, q.y, q.z, q.mass, cx, cy, cz, mu, G, ax, ay, az, ct);
     for (int big = 0; big < 3; ++big) {
         if (ex && !ex[big]) continue;
         const DBody g = b[big];
@@ -1709,7 +1735,8 @@ static_assert((SQ_CH / 2) % (2 * SQ_G) == 0, "whole ping-pong rounds per chunk")
 __global__ __launch_bounds__(256) void k_mseq(const DBody *__restrict__ b, int n,
                                               const int32_t *__restrict__ ex, const double *__restrict__ cen, double G, double s,
                                               const double2 *__restrict__ T, int nch,
-                                              double *__restrict__ acc, double *__restrict__ cm)
+                                              double *__restrict__ acc, double *__restrict__ cm,
+                                              const double *__restrict__ init)
 {
     const int cp = cen && cen[7] != 0.0;
     const double cx = cp ? cen[0] : 0.0, cy = cp ? cen[1] : 0.0, cz = cp ? cen[2] : 0.0;
@@ -1730,7 +1757,9 @@ __global__ __launch_bounds__(256) void k_mseq(const DBody *__restrict__ b, int n
         double sum = 0.0;
         if (lane < 9) {
             const int i = lane / 3, c = lane % 3;
-            if (cp && bex[i]) {
+            if (init && bex[i]) {
+                sum = init[3 * i + c];
+            } else if (cp && bex[i]) {
                 double a[3], ct[3];
                 central_term(g[i].x, g[i].y, g[i].z, g[i].mass, cx, cy, cz, mu, G, a[0], a[1], a[2],
                              ct);
@@ -2894,9 +2923,12 @@ static ncg_status ensure_aux(ncg_ctx *ctx)
 // Mesh field of the bodies db (device), central body dcen (8-double record
 // {x, y, z, vx, vy, vz, M, present} or null), over the binning of the last
 // pass, into dacc (3 per slot) and dcm (CM_acc).
+// dinit (3n, or null): start values in place of the central term (dcen is
+// then null).
 static ncg_status mesh_gravity_core(ncg_ctx *ctx, const DBody *db, int32_t n, const int32_t *dex,
                                     const double *dcen, double G, double s, ncg_precision prec,
-                                    double *dacc, double *dcm, hipStream_t st)
+                                    double *dacc, double *dcm, hipStream_t st,
+                                    const double *dinit = nullptr)
 {
     MeshState &M = ctx->mesh;
     const long d = M.cells;
@@ -2992,21 +3024,21 @@ static ncg_status mesh_gravity_core(ncg_ctx *ctx, const DBody *db, int32_t n, co
         NCG_HIP(hipEventRecord(ctx->ev_fork, st));
         NCG_HIP(hipStreamWaitEvent(ctx->aux, ctx->ev_fork, 0));
         hipLaunchKernelGGL(k_mseq, dim3(1), dim3(256), 0, st, db, (int)n, dex, dcen, G, s,
-                           reinterpret_cast<const double2 *>(T), nch, dacc, dcm);
+                           reinterpret_cast<const double2 *>(T), nch, dacc, dcm, dinit);
         NCG_CHECK_LAUNCH();
         hipStream_t ax = ctx->aux;
         if (ins > 0)
             hipLaunchKernelGGL(k_mord, dim3(cdiv(ins, MT)), dim3(MT), 0, ax, db, (int)n, dex, ins,
                                cs, ce, pm, rr, sx, sid, skey, (int)d, dcen, G, s,
-                               bvk, bvoff, bvo, nbv, L, lw, le, dacc);
+                               bvk, bvoff, bvo, nbv, L, lw, le, dacc, dinit);
         if (n > 3)
             hipLaunchKernelGGL(k_mrest_ord, dim3(cdiv(n - 3, MT)), dim3(MT), 0, ax, db, (int)n, dex,
-                               key, (int)nkey, dcen, G, s, bvk, bvoff, bvo, nbv, dacc);
+                               key, (int)nkey, dcen, G, s, bvk, bvoff, bvo, nbv, dacc, dinit);
         NCG_CHECK_LAUNCH();
         // Survivors and fragments of full fragmentations: host-listed order.
         if (ntg) {
             hipLaunchKernelGGL(k_mexplicit, dim3(cdiv(ntg, MT)), dim3(MT), 0, ax, db, dex, dcen, G, s,
-                               (const int *)tg, ntg, (const int *)pl, dacc);
+                               (const int *)tg, ntg, (const int *)pl, dacc, dinit);
             NCG_CHECK_LAUNCH();
         }
         NCG_HIP(hipEventRecord(ctx->ev_join, ax));
@@ -3042,14 +3074,20 @@ static ncg_status mesh_gravity_core(ncg_ctx *ctx, const DBody *db, int32_t n, co
             hipLaunchKernelGGL(k_extra_pairs, dim3(1), dim3(1), 0, st, db, dex, dextra,
                                (int)M.extra_pairs.size(), G, s, dacc);
         NCG_CHECK_LAUNCH();
+        // The fast forms add the start values after the mutual sums.
+        if (dinit) {
+            hipLaunchKernelGGL(k_add_start, dim3(cdiv(n, MT)), dim3(MT), 0, st, n, dex, dinit, dacc);
+            NCG_CHECK_LAUNCH();
+        }
     }
     return NCG_OK;
 }
 
-extern "C" ncg_status ncg_mesh_gravity(ncg_ctx *ctx, const ncg_body *bodies, int32_t n,
-                                       const int32_t *exists, const ncg_central *central,
-                                       double grav_const, double softening, ncg_precision prec,
-                                       double *acc, double *cm_acc)
+// ncg_mesh_gravity and ncg_mesh_gravity_from (start: 3n start values, or null).
+static ncg_status mesh_gravity_host(ncg_ctx *ctx, const ncg_body *bodies, int32_t n,
+                                    const int32_t *exists, const ncg_central *central,
+                                    const double *start, double grav_const, double softening,
+                                    ncg_precision prec, double *acc, double *cm_acc)
 {
     if (!ctx) return NCG_ERR_INVALID;
     NCG_REQUIRE(n >= 0 && (n == 0 || (bodies && acc)), "invalid arguments");
@@ -3070,8 +3108,13 @@ extern "C" ncg_status ncg_mesh_gravity(ncg_ctx *ctx, const ncg_body *bodies, int
     double *dcm = dbuf<double>(ctx, "g_cm", 4);
     double *dcen = dbuf<double>(ctx, "g_cen", 8);
     double *hcen = pbuf<double>(ctx, "g_hcen", 8);
-    if (!db || (exists && !dex) || !dacc || !dcm || !dcen || !hcen) return NCG_ERR_NOMEM;
+    double *dinit = start ? dbuf<double>(ctx, "g_start", 3 * (size_t)n) : nullptr;
+    if (!db || (exists && !dex) || !dacc || !dcm || !dcen || !hcen || (start && !dinit))
+        return NCG_ERR_NOMEM;
     NCG_HIP(hipMemcpyAsync(db, bodies, sizeof(DBody) * (size_t)n, hipMemcpyHostToDevice, st));
+    if (start)
+        NCG_HIP(hipMemcpyAsync(dinit, start, sizeof(double) * 3 * (size_t)n, hipMemcpyHostToDevice,
+                               st));
     if (exists)
         NCG_HIP(hipMemcpyAsync(dex, exists, sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, st));
     if (cp) {
@@ -3082,7 +3125,7 @@ extern "C" ncg_status ncg_mesh_gravity(ncg_ctx *ctx, const ncg_body *bodies, int
         NCG_HIP(hipMemcpyAsync(dcen, hcen, sizeof(double) * 8, hipMemcpyHostToDevice, st));
     }
     ncg_status r = mesh_gravity_core(ctx, db, n, dex, cp ? dcen : nullptr, grav_const, softening,
-                                     prec, dacc, dcm, st);
+                                     prec, dacc, dcm, st, dinit);
     if (r != NCG_OK) return r;
     double *hacc = hbuf<double>(ctx, "macc", 3 * (size_t)n);
     double hcm[4] = {0, 0, 0, 0};
@@ -3101,6 +3144,26 @@ extern "C" ncg_status ncg_mesh_gravity(ncg_ctx *ctx, const ncg_body *bodies, int
         cm_acc[2] = hcm[2];
     }
     return NCG_OK;
+}
+
+extern "C" ncg_status ncg_mesh_gravity(ncg_ctx *ctx, const ncg_body *bodies, int32_t n,
+                                       const int32_t *exists, const ncg_central *central,
+                                       double grav_const, double softening, ncg_precision prec,
+                                       double *acc, double *cm_acc)
+{
+    return mesh_gravity_host(ctx, bodies, n, exists, central, nullptr, grav_const, softening, prec,
+                             acc, cm_acc);
+}
+
+extern "C" ncg_status ncg_mesh_gravity_from(ncg_ctx *ctx, const ncg_body *bodies, int32_t n,
+                                            const int32_t *exists, const double *start,
+                                            double grav_const, double softening,
+                                            ncg_precision prec, double *acc)
+{
+    if (!ctx) return NCG_ERR_INVALID;
+    NCG_REQUIRE(n == 0 || start, "invalid arguments");
+    return mesh_gravity_host(ctx, bodies, n, exists, nullptr, start, grav_const, softening, prec,
+                             acc, nullptr);
 }
 
 extern "C" ncg_status ncg_mesh_gravity_dev(ncg_ctx *ctx, const ncg_body *d_bodies, int32_t n,

@@ -285,7 +285,8 @@ ncg_status ncg_rk4_stage_launch(ncg_ctx *ctx, int s, double dt, int32_t n, const
 ncg_status ncg_gravity_dev_impl(ncg_ctx *ctx, const DBody *d_bodies, int32_t n,
                                 const int32_t *d_exists, const ncg_central *central, double G,
                                 double soft, ncg_precision prec, double *d_acc,
-                                double *d_cterm, double *d_cm_acc, hipStream_t s);
+                                double *d_cterm, double *d_cm_acc, hipStream_t s,
+                                const double *d_init = nullptr);
 // Direct form of the fast gravity forms for 1..4096 bodies (k_small): one
 // wavefront per body; with d_cm_acc, CM_acc as a fixed-shape tree sum of the
 // terms d_cterm (3n) through d_p3 (3 * ncg_small_partials(n) doubles).
@@ -297,8 +298,10 @@ ncg_status ncg_small_launch(ncg_ctx *ctx, const DBody *d_b, int32_t n, const int
 // Exact form (gravity_exact.hip): the field with per-body CM_acc terms d_cterm
 // (3n, written when the central body is present), and CM_acc summed from
 // them in body order.
+// d_init (3n, or null): each body's sum starts from it instead of the
+// central term.
 ncg_status ncg_grav_exact_launch(ncg_ctx *ctx, const DBody *d_b, int32_t n, const int32_t *d_ex,
                                  const ncg_central *c, double G, double soft, double *d_acc,
-                                 double *d_cterm, hipStream_t st);
+                                 double *d_cterm, hipStream_t st, const double *d_init);
 ncg_status ncg_cm_seq_launch(ncg_ctx *ctx, const double *d_cterm, int32_t n, const int32_t *d_ex,
                              int cp, double *d_cm_acc, hipStream_t st);

@@ -11,7 +11,11 @@
  * What each replaced function does here:
  *   vector_field         physics.c:63-441 -> ncg_gravity_allpairs (brute force,
  *                        or N < switch_to_brute_force) / ncg_mesh_gravity (mesh),
- *                        writes X[k].vx/vy/vz and CM_acc
+ *                        writes X[k].vx/vy/vz and CM_acc; with the J2 bulge or
+ *                        the point-mass perturber, the terms before the mutual
+ *                        ones are formed here (start_terms) and the library adds
+ *                        the mutual terms to them (ncg_*_from); the inner fluid
+ *                        disk's mass enters the central mass
  *   brute_force          collision.c:425-456 -> ncg_brute_force_pass, with the
  *                        reference's closest_approach and collision_treatment as
  *                        the host decision and outcome (every outcome: bounces,
@@ -37,6 +41,7 @@
  * Errors follow the reference convention: message on stderr, then abort().
  */
 #include <stdio.h>
+#include <math.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -44,13 +49,14 @@
 #include "structure.h"
 #include "physics.h"
 #include "collision.h"
+#include "spring.h"
 #include "ncg.h"
 
 #if !(brute_force_bool || mesh_bool)
 #error "ncg drop-in: set brute_force_bool or mesh_bool in parameters.h (tree modules are not accelerated)"
 #endif
-#if viscoelastic_bool || J2_bool || inner_fluid_disk_bool
-#error "ncg drop-in: viscoelastic, J2 and inner-fluid-disk physics are not supported on the GPU path"
+#if viscoelastic_bool
+#error "ncg drop-in: the viscoelastic body is not supported on the GPU path"
 #endif
 
 static ncg_ctx *g_ctx = NULL;
@@ -90,10 +96,6 @@ static void ncg_release(void)
 static ncg_ctx *ctx(void)
 {
     if (g_ctx) return g_ctx;
-    if (pert_mass > 0.) {
-        fprintf(stderr, "Error : ncg drop-in: the point-mass perturber is not supported on the GPU path.\n");
-        abort();
-    }
     if (!mutual_bool) {
         fprintf(stderr, "Error : ncg drop-in: mutual_bool 0 is not supported on the GPU path.\n");
         abort();
@@ -126,7 +128,9 @@ static ncg_central central(void)
         c.x = CM.x;
         c.y = CM.y;
         c.z = CM.z;
-        c.mass = CM.mass;  /* physics.c:79 without the inner fluid disk */
+        /* physics.c:79, the inner fluid disk's mass included */
+        c.mass = inner_fluid_disk_bool ? CM.mass + fluid_disk_Sigma*M_PI*(Rout*Rout - R_unit*R_unit)
+                                       : CM.mass;
     }
     return c;
 }
@@ -149,6 +153,67 @@ static double *acc_buffer(int n)
     return g_acc;
 }
 
+/* Start values with the J2 bulge or the perturber (3 doubles per slot). */
+static double *g_start = NULL;
+static size_t g_start_cap = 0;
+
+/* vector_field's terms before the mutual ones (physics.c:77-80, 110-172):
+ * the central point mass, the J2 bulge and the point-mass perturber, with
+ * the reference's expressions in its order, and CM_acc from the same loop.
+ * Compiled like the reference (no FMA contraction), so the values are its. */
+static void start_terms(const struct moonlet *X, int n, double *st, double *cm)
+{
+    typ XX = central_mass_bool ? CM.x : 0., YY = central_mass_bool ? CM.y : 0.,
+        ZZ = central_mass_bool ? CM.z : 0.;
+    typ M = 0., mu, x, y, z, mk, DX, DY, DZ, rk, r2, r3, r5, K, aX, aY, aZ;
+    typ X_sun = 0., Y_sun = 0., Z_sun = 0., r_sun, KKK, aX_sun = 0., aY_sun = 0., aZ_sun = 0.;
+    typ Xp, Yp, Zp, Rp, KK;
+    int k;
+    if (central_mass_bool)
+        M = inner_fluid_disk_bool ? CM.mass + fluid_disk_Sigma*M_PI*(Rout*Rout - R_unit*R_unit)
+                                  : CM.mass;
+    cm[0] = 0.;  cm[1] = 0.;  cm[2] = 0.;
+    if (pert_mass > 0.) {
+        need_to_reduce_COM_bool = 1;
+        get_pert_coordinates(t_init + time_elapsed + 0.5*timestep, &X_sun, &Y_sun, &Z_sun);
+        r_sun  = sqrt(X_sun*X_sun + Y_sun*Y_sun + Z_sun*Z_sun);
+        KKK    = G*pert_mass/(r_sun*r_sun*r_sun);
+        aX_sun = KKK*X_sun;  aY_sun = KKK*Y_sun;  aZ_sun = KKK*Z_sun;
+    }
+    for (k = 0; k < n; k++) {
+        if (!exists[k]) continue;
+        x  = X[k].x;  y = X[k].y;  z = X[k].z;  mk = X[k].mass;
+        DX = x - XX;  DY = y - YY;  DZ = z - ZZ;
+        rk = sqrt(DX*DX + DY*DY + DZ*DZ);
+        r3 = rk*rk*rk;
+        mu = G*M;
+        if (central_mass_bool) {
+            aX =  -mu*DX/r3;  aY = -mu*DY/r3;  aZ = -mu*DZ/r3;
+            cm[0] += G*mk*DX/r3;  cm[1] += G*mk*DY/r3;  cm[2] += G*mk*DZ/r3;
+        } else {
+            aX = 0.;  aY = 0.;  aZ = 0.;
+        }
+        if (J2_bool) {
+            r2 = rk*rk;
+            r5 = r3*r2;
+            K  = mu*R_unit*R_unit*J2/r5;
+            aX += K*(-1.5*DX + 7.5/r2*DZ*DZ*DX);
+            aY += K*(-1.5*DY + 7.5/r2*DZ*DZ*DY);
+            aZ += K*(-4.5*DZ + 7.5/r2*DZ*DZ*DZ);
+            cm[0] += K*( 1.5*DX - 7.5/r2*DZ*DZ*DX)*mk/M;
+            cm[1] += K*( 1.5*DY - 7.5/r2*DZ*DZ*DY)*mk/M;
+            cm[2] += K*( 4.5*DZ - 7.5/r2*DZ*DZ*DZ)*mk/M;
+        }
+        if (pert_mass > 0.) {
+            Xp = X_sun - x;  Yp = Y_sun - y;  Zp = Z_sun - z;
+            Rp = sqrt(Xp*Xp + Yp*Yp + Zp*Zp);
+            KK = G*pert_mass/(Rp*Rp*Rp);
+            aX += KK*Xp - aX_sun;  aY += KK*Yp - aY_sun;  aZ += KK*Zp - aZ_sun;
+        }
+        st[3 * k] = aX;  st[3 * k + 1] = aY;  st[3 * k + 2] = aZ;
+    }
+}
+
 void ncg_dropin_vector_field(struct moonlet *X)
 {
     ncg_ctx *c = ctx();
@@ -158,14 +223,37 @@ void ncg_dropin_vector_field(struct moonlet *X)
     double cm[3] = {0., 0., 0.};
     ncg_central cen = central();
     ncg_status st;
+    if (J2_bool || pert_mass > 0.) {
+        /* The library adds the mutual terms to the reference's own start
+         * values; CM_acc comes from the same host loop. */
+        if ((size_t)n * 3 > g_start_cap) {
+            free(g_start);
+            g_start_cap = (size_t)n * 3 + 1024;
+            g_start = (double *)malloc(sizeof(double) * g_start_cap);
+            if (!g_start) {
+                fprintf(stderr, "Error : ncg drop-in: cannot allocate the start values.\n");
+                abort();
+            }
+        }
+        start_terms(X, n, g_start, cm);
 #if mesh_bool
-    if (!force_naive_bool)
-        st = ncg_mesh_gravity(c, (const ncg_body *)X, n, exists, &cen, G, softening_parameter,
-                              g_prec, acc, cm);
-    else
+        if (!force_naive_bool)
+            st = ncg_mesh_gravity_from(c, (const ncg_body *)X, n, exists, g_start, G,
+                                       softening_parameter, g_prec, acc);
+        else
 #endif
-        st = ncg_gravity_allpairs(c, (const ncg_body *)X, n, exists, &cen, G, softening_parameter,
+            st = ncg_gravity_allpairs_from(c, (const ncg_body *)X, n, exists, g_start, G,
+                                           softening_parameter, g_prec, acc);
+    } else {
+#if mesh_bool
+        if (!force_naive_bool)
+            st = ncg_mesh_gravity(c, (const ncg_body *)X, n, exists, &cen, G, softening_parameter,
                                   g_prec, acc, cm);
+        else
+#endif
+            st = ncg_gravity_allpairs(c, (const ncg_body *)X, n, exists, &cen, G,
+                                      softening_parameter, g_prec, acc, cm);
+    }
     if (st != NCG_OK) ncg_die("vector_field", st);
     for (int k = 0; k < n; k++) {
         if (!exists[k]) continue;

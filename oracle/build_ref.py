@@ -152,6 +152,21 @@ CONFIGS = {
         "inelastic_collision_bool": "0", "fragmentation_bool": "1", "N_max": "4000"})),
     "run_c1_merge": ("run", _run_cfg("brute", 2000, 40, DT1, {
         "inelastic_collision_bool": "0", "instant_merger_bool": "1"})),
+    # vector_field's terms beyond the central point mass: the J2 bulge of the
+    # central body (R_unit 0.05), a point-mass perturber (mass 0.01 on a
+    # circular orbit of radius 5), and an inner fluid disk whose mass enters
+    # mu (physics.c:79, 110-172).
+    "run_c1_j2": ("run", _run_cfg("brute", 2000, 40, DT1, {
+        "J2_bool": "1", "J2_value": "0.001"})),
+    "run_c1_pert": ("run", _run_cfg("brute", 2000, 40, DT1, {
+        "pert_mass": "0.01", "pert_sma": "5."})),
+    "run_c1_disk": ("run", _run_cfg("brute", 2000, 40, DT1, {
+        "inner_fluid_disk_bool": "1", "inner_mass": "0.001"})),
+    "run_mesh_j2": ("run", _run_cfg("mesh", 16384, 10, DT2, {
+        "collision_cube_cells": "256", "collision_cube_min": "2.56",
+        "how_many_neighbours": "100.0", "switch_to_brute_force": "110",
+        "sma_min": "0.9", "sma_max": "1.1", "inclination_max": "0.01",
+        "J2_bool": "1", "J2_value": "0.001", "pert_mass": "0.01", "pert_sma": "5."})),
     # The fragmentation model with end_of_timestep's removals: bodies closer
     # to the central body than 0.93 merge into it, bodies beyond 1.07 are
     # dropped (the centre of mass is then reduced), and the array is

@@ -57,6 +57,28 @@ def test_dense_run_identical_with_collisions(tmp_path):
     assert ncoll > 20
 
 
+@pytest.mark.parametrize("cfg", ["run_c1_j2", "run_c1_pert", "run_c1_disk"])
+def test_central_terms_runs_identical(tmp_path, cfg):
+    # vector_field's terms beyond the central point mass (physics.c:79,
+    # 110-172): the J2 bulge (J2 = 1e-3, R_unit 0.05), a point-mass perturber
+    # (mass 0.01 on a circular orbit of radius 5; vector_field also asks for
+    # the centre-of-mass reduction every step) and an inner fluid disk (mass
+    # 1e-3 in mu).  2000 bodies, 40 steps with collisions: every traced call
+    # identical, accelerations and CM_acc included.
+    b = ic.disk(2000, seed=1, mass=1e-8, radius=4e-3)
+    tr = compare(cfg, b, tmp_path, 40)
+    assert sum(1 for l in tr if l.startswith("C ")) > 100
+
+
+def test_mesh_run_with_j2_and_perturber_identical(tmp_path):
+    # The mesh module with the J2 bulge and the perturber: the library adds
+    # bodies 0..2 and the pass's pairs to the reference's start values.
+    b = ic.disk(16384, seed=2, mass=1e-10, radius=1e-3)
+    tr = compare("run_mesh_j2", b, tmp_path, 10)
+    assert sum(1 for l in tr if l.startswith("C ")) > 10
+    assert any(l.startswith("MESH") for l in tr)
+
+
 @pytest.mark.parametrize("cfg", ["run_c1_frag", "run_c1_merge"])
 def test_fragmentation_and_merger_runs_identical(tmp_path, cfg):
     # The reference's default collision model (mergers, partial and full

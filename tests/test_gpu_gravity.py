@@ -173,3 +173,40 @@ def test_empty_and_single(ctx):
     for p in ("exact", "fast64", "mixed"):
         acc, _ = ctx.gravity_allpairs(b, None, (0, 0, 0, 2.0), 1.0, 0.0, p)
         assert acc[0, 0] == -2.0 and acc[0, 1] == 0.0
+
+
+def central_start(b, ex, cen, G):
+    """The central term of vector_field (physics.c:121-136) in numpy, which
+    rounds each operation as the reference does: ((-mu) DX) / r^3."""
+    cx, cy, cz, M = cen
+    mu = G * M
+    DX, DY, DZ = b[:, 0] - cx, b[:, 1] - cy, b[:, 2] - cz
+    rk = np.sqrt(DX * DX + DY * DY + DZ * DZ)
+    r3 = rk * rk * rk
+    st = np.stack([-mu * DX / r3, -mu * DY / r3, -mu * DZ / r3], axis=1)
+    st[~ex] = 0.0
+    return st
+
+
+@pytest.mark.parametrize("n,prec", [(1000, "exact"), (40000, "exact"), (1000, "fast64"),
+                                    (8192, "fast64"), (8192, "mixed")])
+def test_start_values_equal_central_term(ctx, n, prec):
+    """ncg_gravity_allpairs_from with the central term as the start values
+    equals ncg_gravity_allpairs with the central body: bit for bit in the
+    exact form (both paths of the exact kernel, n below and above 32,768),
+    within the form's tolerance in the others (which add the start values
+    after their sums)."""
+    rng = np.random.default_rng(n)
+    b = ic.disk(n, seed=n, mass=1e-8, radius=2e-4)
+    ex = np.ones(n, np.int32)
+    ex[rng.choice(n, n // 50, replace=False)] = 0
+    cen = (1e-3, -2e-3, 0.0, 1.0)
+    G, soft = 1.0, 0.25
+    ref, _ = ctx.gravity_allpairs(b, ex, cen, G, soft, prec)
+    got = ctx.gravity_allpairs_from(b, central_start(b, ex != 0, cen, G), ex, G, soft, prec)
+    live = ex != 0
+    if prec == "exact":
+        assert np.array_equal(bits(got[live]), bits(ref[live]))
+    else:
+        tol = 1e-12 if prec == "fast64" else 1e-6
+        assert rel_err(got[live], ref[live])[0] < tol

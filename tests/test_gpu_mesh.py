@@ -425,3 +425,31 @@ def test_mesh_pass_with_big_bodies_and_all_outcomes(ctx):
     print(seen)
     assert all(v > 0 for v in seen.values()), seen
 
+
+
+def test_mesh_start_values_equal_central_term(ctx):
+    """ncg_mesh_gravity_from with the central term as start values equals
+    ncg_mesh_gravity with the central body: bit for bit in the exact form,
+    to 1e-12 in the FP64-fast one."""
+    gn = load("gnm_n1500")
+    g = load("vf_mesh_n1500")
+    b = np.array(gn["bodies"])
+    ctx.mesh_pass(b, gn["exists"], 0.0, float(gn["gam"]), float(gn["cube"]), int(gn["cells"]), 1)
+    c8 = g["central"]
+    cen = (c8[0], c8[1], c8[2], c8[6])
+    ex = g["exists"].astype(bool)
+    G = float(g["G"])
+    bb = np.array(g["bodies"])
+    mu = G * cen[3]
+    DX, DY, DZ = bb[:, 0] - cen[0], bb[:, 1] - cen[1], bb[:, 2] - cen[2]
+    rk = np.sqrt(DX * DX + DY * DY + DZ * DZ)
+    r3 = rk * rk * rk
+    st = np.stack([-mu * DX / r3, -mu * DY / r3, -mu * DZ / r3], axis=1)
+    st[~ex] = 0.0
+    for prec in ("exact", "fast64"):
+        ref, _ = ctx.mesh_gravity(g["bodies"], g["exists"], cen, G, float(g["soft"]), prec)
+        got = ctx.mesh_gravity_from(g["bodies"], st, g["exists"], G, float(g["soft"]), prec)
+        if prec == "exact":
+            assert np.array_equal(bits(got[ex]), bits(ref[ex]))
+        else:
+            assert np.abs(got[ex] - ref[ex]).max() / np.abs(ref[ex]).max() < 1e-12
