@@ -283,7 +283,9 @@ __global__ __launch_bounds__(ENT) void k_ens_tiles(const DBody *__restrict__ b,
 // with FP64 terms read with wave-uniform addresses, far tiles staged in the
 // wavefront's own LDS slice.  Then the central term (physics.c:127-146), the
 // accelerations scattered to body order and the unit's CM_acc partial.
-// (At most 73 VGPRs: 7 wavefronts per SIMD.)
+// (At most 73 VGPRs: 7 wavefronts per SIMD.)  SOFT false: no softening (the
+// near terms skip s_i + s_j, same bits).
+template <bool SOFT>
 __global__ __launch_bounds__(ENT, 7) void k_ens_tpass(const EnsUnit *__restrict__ units,
                                                    const EnsSys *__restrict__ sys,
                                                    const double *__restrict__ srmax,
@@ -333,17 +335,18 @@ __global__ __launch_bounds__(ENT, 7) void k_ens_tpass(const EnsUnit *__restrict_
                 if (tile_near(ih, hi, th, theta, eps2x)) {
                     if (jb < g0 + WV && jb + TJ > g0) {
                         for (int jj = 0; jj < cnt; ++jj)
-                            near_term_self<1>(jd[jb + jj], jsig[jb + jj], jb + jj, g0 + lane, WV,
-                                              &xi, &yi, &zi, &si, &ax, &ay, &az);
+                            near_term_self<1, SOFT>(jd[jb + jj], SOFT ? jsig[jb + jj] : 0.0,
+                                                    jb + jj, g0 + lane, WV, &xi, &yi, &zi, &si,
+                                                    &ax, &ay, &az);
                     } else if (cnt == TJ) {
 #pragma unroll 4
                         for (int jj = 0; jj < TJ; ++jj)
-                            near_term<1>(jd[jb + jj], jsig[jb + jj], &xi, &yi, &zi, &si, &ax, &ay,
-                                         &az);
+                            near_term<1, SOFT>(jd[jb + jj], SOFT ? jsig[jb + jj] : 0.0, &xi, &yi,
+                                               &zi, &si, &ax, &ay, &az);
                     } else {
                         for (int jj = 0; jj < cnt; ++jj)
-                            near_term<1>(jd[jb + jj], jsig[jb + jj], &xi, &yi, &zi, &si, &ax, &ay,
-                                         &az);
+                            near_term<1, SOFT>(jd[jb + jj], SOFT ? jsig[jb + jj] : 0.0, &xi, &yi,
+                                               &zi, &si, &ax, &ay, &az);
                     }
                 } else {
                     // The wavefront's own slice: ordered after its reads of
@@ -970,9 +973,14 @@ static ncg_status ens_eval(ncg_ctx *ctx, const EnsPlan &P, const EnsDev &D, cons
                            D.tsys, D.sys, P.ntiles, G, soft, D.jd, D.jsig, D.ju, D.thdr, D.ghdr);
         NCG_CHECK_LAUNCH();
         ProfScope ps(ctx, st, "ens_mixed");
-        hipLaunchKernelGGL(k_ens_tpass, dim3(nunits), dim3(ENT), 0, st, D.grav, D.sys, D.srmax,
-                           D.jd, D.jsig, D.ju, D.thdr, D.ghdr, D.pidx, cen, G, soft, P.theta, acc,
-                           cpart);
+        if (soft != 0.0)
+            hipLaunchKernelGGL(k_ens_tpass<true>, dim3(nunits), dim3(ENT), 0, st, D.grav, D.sys,
+                               D.srmax, D.jd, D.jsig, D.ju, D.thdr, D.ghdr, D.pidx, cen, G, soft,
+                               P.theta, acc, cpart);
+        else
+            hipLaunchKernelGGL(k_ens_tpass<false>, dim3(nunits), dim3(ENT), 0, st, D.grav, D.sys,
+                               D.srmax, D.jd, D.jsig, D.ju, D.thdr, D.ghdr, D.pidx, cen, G, soft,
+                               P.theta, acc, cpart);
         NCG_CHECK_LAUNCH();
     } else {
         ProfScope ps(ctx, st, "ens_fast64");

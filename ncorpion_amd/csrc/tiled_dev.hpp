@@ -57,7 +57,9 @@ __device__ inline T block_reduce(T v, T *sh, int op /*0 min 1 max 2 sum*/)
 // y = 1e100 (its mass is zeroed by near_term_self, so it adds exactly 0);
 // above r^2 ~ 1e-184 it is below half an ulp and leaves r^2 as it is (one
 // FMA instead of a multiply and a floor).
-template <int IPL>
+// SOFT false: no softening (s_i = s_j = 0), two instructions fewer and the
+// same bits (fma(0, 0, TINY2) is TINY2).
+template <int IPL, bool SOFT = true>
 __device__ inline void near_term(const double4 p, double sj, const double *xi, const double *yi,
                                  const double *zi, const double *si, double *ax, double *ay,
                                  double *az)
@@ -65,8 +67,12 @@ __device__ inline void near_term(const double4 p, double sj, const double *xi, c
 #pragma unroll
     for (int q = 0; q < IPL; ++q) {
         const double dx = p.x - xi[q], dy = p.y - yi[q], dz = p.z - zi[q];
-        const double e = si[q] + sj;
-        const double r2 = fma(dx, dx, fma(dy, dy, fma(dz, dz, fma(e, e, TINY2))));
+        double seed = TINY2;
+        if (SOFT) {
+            const double e = si[q] + sj;
+            seed = fma(e, e, TINY2);
+        }
+        const double r2 = fma(dx, dx, fma(dy, dy, fma(dz, dz, seed)));
         double y = __builtin_amdgcn_rsq(r2);
         const double h = 0.5 * r2;
         y = y * fma(-h, y * y, 1.5);
@@ -83,7 +89,7 @@ __device__ inline void near_term(const double4 p, double sj, const double *xi, c
 // G m > ~1.8e8 and inf * 0 poison the sum; with the mass zeroed the term is
 // 0 * 1e300 * 0 = 0 whatever G m is.  Used only where the j tile overlaps
 // the lane's i slots.
-template <int IPL>
+template <int IPL, bool SOFT = true>
 __device__ inline void near_term_self(const double4 p, double sj, int js, int is0, int stride,
                                       const double *xi, const double *yi, const double *zi,
                                       const double *si, double *ax, double *ay, double *az)
@@ -91,7 +97,7 @@ __device__ inline void near_term_self(const double4 p, double sj, int js, int is
 #pragma unroll
     for (int q = 0; q < IPL; ++q) {
         const double4 pq = make_double4(p.x, p.y, p.z, js == is0 + q * stride ? 0.0 : p.w);
-        near_term<1>(pq, sj, xi + q, yi + q, zi + q, si + q, ax + q, ay + q, az + q);
+        near_term<1, SOFT>(pq, sj, xi + q, yi + q, zi + q, si + q, ax + q, ay + q, az + q);
     }
 }
 
