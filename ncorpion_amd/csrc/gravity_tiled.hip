@@ -19,11 +19,14 @@
 //                    half extent, the range of (G m)^(-1/2) over the tile and
 //                    the scaled far-form records (below); the anchor and
 //                    extent also per wavefront group of 64*IPL i-bodies
+//   5b. k_near_mask  mixed form: the near/far decision (below) of every
+//                    (wavefront group, tile) pair, one bit each, made once
+//                    instead of once per j split by vector instructions
 //   6. k_pairs<MODE> one workgroup per (i-block, j-split); every lane owns IPL
 //                    bodies, every wavefront a contiguous group of 64*IPL of
 //                    them, and walks the j tiles of its split.  Tile data is
 //                    read with wave-uniform addresses (scalar loads).  For each
-//                    (wavefront group, tile) pair a uniform test picks the form:
+//                    (wavefront group, tile) pair a uniform bit picks the form:
 //                      near: FP64 separations, FP64 rsqrt refined by one Newton
 //                            step, FP64 accumulation (also every tile in
 //                            NCG_F64_FAST);
@@ -237,6 +240,33 @@ __global__ __launch_bounds__(NT) void k_tiles(const double4 *__restrict__ jd,
     }
 }
 
+// ---- 5b. near/far decisions of the mixed form ------------------------------
+// One lane per (wavefront group g, j tile t): the tile_near decision of the
+// pair kernel, made once for all of its j splits, as one bit of
+// mask[g nw + t / 64].  The pair kernel then tests a bit with scalar
+// instructions instead of spending ~60 vector instructions (two FP64 square
+// roots) per group and tile.
+__global__ __launch_bounds__(WV) void k_near_mask(const TileHdr *__restrict__ thdr,
+                                                  const TileHdr *__restrict__ ihdr,
+                                                  const Scal *__restrict__ sc, int gw, int nw,
+                                                  double theta, double soft,
+                                                  unsigned long long *__restrict__ mask)
+{
+    const int nv = sc->nvalid;
+    const int g = blockIdx.y;
+    const int t = blockIdx.x * WV + (int)threadIdx.x;
+    bool near = false;
+    if (g * gw < nv && t * TJ < nv) {
+        const TileHdr ih = ihdr[g];
+        const double eps = 2.0 * soft * sc->rmax;
+        const double eps2x = eps * eps * 67108864.0;  // 2^26
+        const double hi = sqrt((double)ih.hx * ih.hx + (double)ih.hy * ih.hy + (double)ih.hz * ih.hz);
+        near = tile_near(ih, hi, thdr[t], theta, eps2x);
+    }
+    const unsigned long long bits = __ballot(near);
+    if (threadIdx.x == 0) mask[(size_t)g * nw + blockIdx.x] = bits;
+}
+
 // ---- 6. pair kernel -------------------------------------------------------
 // MODE 0: FP64-fast on every tile; MODE 1: mixed (near FP64, far FP32).
 // SRC 0: the form picked per wavefront; near tiles read with wave-uniform
@@ -255,7 +285,8 @@ __global__ __launch_bounds__(NT) void k_pairs(const double4 *__restrict__ jd,
                                               const TileHdr *__restrict__ ihdr,
                                               const Scal *__restrict__ sc, int nsplit,
                                               double theta, double soft, int nslots,
-                                              double *__restrict__ part)
+                                              double *__restrict__ part,
+                                              const unsigned long long *__restrict__ mask, int nw)
 {
     constexpr bool LDS = SRC == 1;
     constexpr int GW = WV * IPL;  // bodies of one wavefront group
@@ -276,7 +307,9 @@ __global__ __launch_bounds__(NT) void k_pairs(const double4 *__restrict__ jd,
     const bool gvalid = g0 < nv;
     if (!LDS && !gvalid) return;
 
-    const TileHdr ih = ihdr[gvalid ? iblk * (NT / WV) + w : 0];
+    const int gidx = gvalid ? iblk * (NT / WV) + w : 0;  // the group's decision row
+    (void)ihdr;  // the group headers enter through k_near_mask
+    const unsigned long long *mrow = MODE == 1 ? mask + (size_t)gidx * nw : nullptr;
     double xi[IPL], yi[IPL], zi[IPL], si[IPL];
     double ax[IPL], ay[IPL], az[IPL];
 #pragma unroll
@@ -289,18 +322,14 @@ __global__ __launch_bounds__(NT) void k_pairs(const double4 *__restrict__ jd,
         si[q] = jsig[i];
         ax[q] = ay[q] = az[q] = 0.0;
     }
-    // Largest softening length squared and the FP32 resolution test.
-    const double eps = 2.0 * soft * sc->rmax;
-    const double eps2x = eps * eps * 67108864.0;  // 2^26
-    const double hi = sqrt((double)ih.hx * ih.hx + (double)ih.hy * ih.hy + (double)ih.hz * ih.hz);
-
     for (int t = t0; t < t1; ++t) {
         const TileHdr th = thdr[t];
         const int jb = t * TJ;
         const int cnt = th.count;
         bool near = true;
         if (MODE == 1) {
-            near = tile_near(ih, hi, th, theta, eps2x) || !gvalid;
+            // The decision of k_near_mask for this group and tile.
+            near = !gvalid || ((mrow[t >> 6] >> (t & 63)) & 1ull) != 0;
             if (LDS) near = __syncthreads_or(near && gvalid);
         }
         if (near) {
@@ -573,14 +602,14 @@ template <int MODE, int IPL>
 static void launch_pairs(int src, dim3 grid, hipStream_t st, const double4 *jd,
                          const double *jsig, const float4 *ju, const TileHdr *th,
                          const TileHdr *ih, const Scal *sc, int nsplit, double theta, double soft,
-                         int n, double *part)
+                         int n, double *part, const unsigned long long *mask, int nw)
 {
     if (src == 1)
         hipLaunchKernelGGL((k_pairs<MODE, IPL, 1>), grid, dim3(NT), 0, st, jd, jsig, ju, th, ih, sc,
-                           nsplit, theta, soft, n, part);
+                           nsplit, theta, soft, n, part, mask, nw);
     else
         hipLaunchKernelGGL((k_pairs<MODE, IPL, 0>), grid, dim3(NT), 0, st, jd, jsig, ju, th, ih, sc,
-                           nsplit, theta, soft, n, part);
+                           nsplit, theta, soft, n, part, mask, nw);
 }
 
 // Grid shape of the tiled forms for n slots (same in order and eval).
@@ -628,7 +657,11 @@ struct TiledWork {
     TileHdr *th = nullptr, *ih = nullptr;
     double *part = nullptr;
     double *cterm = nullptr, *p3 = nullptr;
+    unsigned long long *mask = nullptr;  // near/far decisions (mixed form)
 };
+
+// 64-bit words of one group's row of decisions.
+static int mask_words(const TiledShape &t) { return (int)cdiv(t.ntiles, 64); }
 
 // Temporary storage of the order's sort, for both key widths (32-bit Morton
 // keys, 1-bit existence keys).
@@ -675,8 +708,9 @@ static ncg_status work_eval_ctx(ncg_ctx *ctx, const TiledShape &t, int32_t n, bo
     w.part = dbuf<double>(ctx, "t_part", (size_t)t.nsplit * 3 * n);
     w.cterm = d_cterm ? d_cterm : (want_cm ? dbuf<double>(ctx, "cterm", 3 * (size_t)n) : nullptr);
     w.p3 = want_cm ? dbuf<double>(ctx, "sum3", 3 * (size_t)t.nbb) : nullptr;
+    w.mask = dbuf<unsigned long long>(ctx, "t_mask", (size_t)t.ngroups * mask_words(t));
     if (!w.sc || !w.perm || !w.jd || !w.jsig || !w.ju || !w.th || !w.ih || !w.part ||
-        (want_cm && (!w.cterm || !w.p3)))
+        (want_cm && (!w.cterm || !w.p3)) || !w.mask)
         return NCG_ERR_NOMEM;
     return NCG_OK;
 }
@@ -755,16 +789,23 @@ static ncg_status tiled_eval_w(ncg_ctx *ctx, const TiledShape &t, const TiledWor
     const TileHdr *th = w.th, *ih = w.ih;
     const Scal *sc = w.sc;
     double *part = w.part;
+    const int nw = mask_words(t);
+    const unsigned long long *mask = w.mask;
+    if (prec != NCG_F64_FAST) {
+        hipLaunchKernelGGL(k_near_mask, dim3(nw, t.ngroups), dim3(WV), 0, st, th, ih, sc,
+                           WV * t.ipl, nw, theta, soft, w.mask);
+        NCG_CHECK_LAUNCH();
+    }
     {
         ProfScope ps(ctx, st, prec == NCG_F64_FAST ? "pairs_fast64" : "pairs_mixed");
         if (prec == NCG_F64_FAST) {
-            if (t.ipl == 1) launch_pairs<0, 1>(src, grid, st, jd, jsig, ju, th, ih, sc, t.nsplit, theta, soft, n, part);
-            else if (t.ipl == 2) launch_pairs<0, 2>(src, grid, st, jd, jsig, ju, th, ih, sc, t.nsplit, theta, soft, n, part);
-            else launch_pairs<0, 4>(src, grid, st, jd, jsig, ju, th, ih, sc, t.nsplit, theta, soft, n, part);
+            if (t.ipl == 1) launch_pairs<0, 1>(src, grid, st, jd, jsig, ju, th, ih, sc, t.nsplit, theta, soft, n, part, mask, nw);
+            else if (t.ipl == 2) launch_pairs<0, 2>(src, grid, st, jd, jsig, ju, th, ih, sc, t.nsplit, theta, soft, n, part, mask, nw);
+            else launch_pairs<0, 4>(src, grid, st, jd, jsig, ju, th, ih, sc, t.nsplit, theta, soft, n, part, mask, nw);
         } else {
-            if (t.ipl == 1) launch_pairs<1, 1>(src, grid, st, jd, jsig, ju, th, ih, sc, t.nsplit, theta, soft, n, part);
-            else if (t.ipl == 2) launch_pairs<1, 2>(src, grid, st, jd, jsig, ju, th, ih, sc, t.nsplit, theta, soft, n, part);
-            else launch_pairs<1, 4>(src, grid, st, jd, jsig, ju, th, ih, sc, t.nsplit, theta, soft, n, part);
+            if (t.ipl == 1) launch_pairs<1, 1>(src, grid, st, jd, jsig, ju, th, ih, sc, t.nsplit, theta, soft, n, part, mask, nw);
+            else if (t.ipl == 2) launch_pairs<1, 2>(src, grid, st, jd, jsig, ju, th, ih, sc, t.nsplit, theta, soft, n, part, mask, nw);
+            else launch_pairs<1, 4>(src, grid, st, jd, jsig, ju, th, ih, sc, t.nsplit, theta, soft, n, part, mask, nw);
         }
         NCG_CHECK_LAUNCH();
     }
@@ -908,8 +949,10 @@ static bool work_carve(unsigned char *base, size_t cap, TiledShape &t, int32_t n
     w.ju = (float4 *)take(sizeof(float4) * (size_t)t.ntiles * TJ);
     w.th = (TileHdr *)take(sizeof(TileHdr) * (size_t)t.ntiles);
     w.ih = (TileHdr *)take(sizeof(TileHdr) * (size_t)t.ngroups);
+    w.mask = (unsigned long long *)take(sizeof(unsigned long long) * (size_t)t.ngroups *
+                                        mask_words(t));
     if (!w.sc || !w.bpart || !w.key || !w.key2 || !w.idx || !w.perm || !w.sort_tmp || !w.jd ||
-        !w.jsig || !w.ju || !w.th || !w.ih)
+        !w.jsig || !w.ju || !w.th || !w.ih || !w.mask)
         return false;
     const size_t at = (off + 255) & ~(size_t)255;
     const size_t per_split = sizeof(double) * 3 * (size_t)n;
