@@ -20,9 +20,10 @@
 //   NCG_F64_FAST  body order, FP64 (rsqrt + one Newton step), no tiles, so
 //                 the sums do not depend on where the bodies are;
 //   NCG_MIXED     the single-system mixed form per system (tiled_dev.hpp):
-//                 for each (wavefront group, j tile) pair a uniform test picks
-//                 FP64 near terms or scaled FP32 far terms t = s_j (w_i + u_j)
-//                 with FP32 tile partials added into FP64.
+//                 for each (wavefront group, j tile) pair a uniform test
+//                 (precomputed by k_ens_mask, one bit per pair) picks FP64 near
+//                 terms or scaled FP32 far terms t = s_j (w_i + u_j) with FP32
+//                 tile partials added into FP64.
 // Collision pass: the same order; a wavefront group tests the pairs (p < q
 // in slot order) of the tiles its position and velocity boxes can reach
 // within dt.  The test is conservative (below), so the candidates are those
@@ -277,6 +278,40 @@ __global__ __launch_bounds__(ENT) void k_ens_tiles(const DBody *__restrict__ b,
     for (int r = 0; r < 2; ++r) ju[t * TJ + r * WV + lane] = far_record(p[r], A);
 }
 
+// ---- mixed form: near/far decisions ---------------------------------------------
+// The tile_near decision of every (wavefront group, j tile of its system)
+// pair, one bit each: group g0 / WV's row of `nw` words, bit t - t0 for the
+// system's tile t.  Same unit and wavefront layout as k_ens_tpass, which then
+// tests a bit with scalar instructions instead of ~65 vector instructions
+// per group and tile.
+__global__ __launch_bounds__(ENT) void k_ens_mask(const EnsUnit *__restrict__ units,
+                                                  const EnsSys *__restrict__ sys,
+                                                  const double *__restrict__ srmax,
+                                                  const TileHdr *__restrict__ thdr,
+                                                  const TileHdr *__restrict__ ghdr, double soft,
+                                                  double theta, int nw,
+                                                  unsigned long long *__restrict__ mask)
+{
+    const int w = (int)threadIdx.x / WV, lane = (int)threadIdx.x % WV;
+    const EnsUnit U = units[blockIdx.x];
+    const EnsSys S = sys[U.sys];
+    const int l0 = U.i0 + w * WV;
+    if (l0 >= S.n) return;  // the whole wavefront
+    const int g0 = S.poff + l0;
+    const TileHdr ih = ghdr[g0 / WV];
+    const double eps = 2.0 * soft * srmax[U.sys];
+    const double eps2x = eps * eps * 67108864.0;  // 2^26
+    const double hi = sqrt((double)ih.hx * ih.hx + (double)ih.hy * ih.hy + (double)ih.hz * ih.hz);
+    const int t0 = S.poff / TJ, nt = (S.n + TJ - 1) / TJ;
+    unsigned long long *row = mask + (size_t)(g0 / WV) * nw;
+    for (int word = 0; word < nw; ++word) {
+        const int lt = word * WV + lane;
+        const bool near = lt < nt && tile_near(ih, hi, thdr[t0 + lt], theta, eps2x);
+        const unsigned long long bits = __ballot(near);
+        if (lane == 0) row[word] = bits;
+    }
+}
+
 // ---- mixed form: the pass -----------------------------------------------------
 // One workgroup per gravity unit (system, 256 slots); wavefront w takes the
 // unit's group w and walks every j tile of its system, near tiles
@@ -296,8 +331,9 @@ __global__ __launch_bounds__(ENT, 7) void k_ens_tpass(const EnsUnit *__restrict_
                                                    const TileHdr *__restrict__ ghdr,
                                                    const int *__restrict__ pidx,
                                                    const double *__restrict__ cen, double G,
-                                                   double soft, double theta,
-                                                   double *__restrict__ acc,
+                                                   double soft,
+                                                   const unsigned long long *__restrict__ mask,
+                                                   int nw, double *__restrict__ acc,
                                                    double *__restrict__ cpart)
 {
     __shared__ float4 lu[(ENT / WV) * TJ];
@@ -322,17 +358,14 @@ __global__ __launch_bounds__(ENT, 7) void k_ens_tpass(const EnsUnit *__restrict_
             zi = p.z;
             gmi = p.w;
             si = jsig[g0 + lane];
-            const TileHdr ih = ghdr[g0 / WV];
-            const double eps = 2.0 * soft * srmax[U.sys];
-            const double eps2x = eps * eps * 67108864.0;  // 2^26
-            const double hi =
-                sqrt((double)ih.hx * ih.hx + (double)ih.hy * ih.hy + (double)ih.hz * ih.hz);
+            // The group's decisions (k_ens_mask).
+            const unsigned long long *mrow = mask + (size_t)(g0 / WV) * nw;
             const int t0 = S.poff / TJ, t1 = t0 + (S.n + TJ - 1) / TJ;
             for (int t = t0; t < t1; ++t) {
                 const TileHdr th = thdr[t];
                 const int jb = t * TJ;
                 const int cnt = th.count;
-                if (tile_near(ih, hi, th, theta, eps2x)) {
+                if ((mrow[(t - t0) >> 6] >> ((t - t0) & 63)) & 1ull) {
                     if (jb < g0 + WV && jb + TJ > g0) {
                         for (int jj = 0; jj < cnt; ++jj)
                             near_term_self<1, SOFT>(jd[jb + jj], SOFT ? jsig[jb + jj] : 0.0,
@@ -787,6 +820,7 @@ struct EnsPlan {
     int nsys = 0, ntot = 0, ipl = 1;
     double theta = 0.35;
     int nslots = 0, ntiles = 0, key_bits = 30;
+    int maxt = 0;                     // tiles of the largest system
     std::vector<int> off;
     std::vector<EnsSys> sys;
     std::vector<int> tsys;            // system of each tile
@@ -845,6 +879,7 @@ static void make_plan(ncg_ctx *ctx, EnsPlan &P, const int32_t *offsets, int32_t 
         const int nt = (n + TJ - 1) / TJ;
         for (int t = 0; t < nt; ++t) P.tsys.push_back(s);
         poff += nt * TJ;
+        P.maxt = std::max(P.maxt, nt);
     }
     P.nslots = poff;
     P.ntiles = poff / TJ;
@@ -875,6 +910,8 @@ struct EnsDev {
     double *jsig = nullptr;
     float4 *ju = nullptr;
     TileHdr *thdr = nullptr, *ghdr = nullptr;
+    unsigned long long *mask = nullptr;  // near/far decisions, 2 nt rows of nw words
+    int nw = 1;
     // Collision pass.
     DBody *jc = nullptr;
     CollHdr *tch = nullptr, *gch = nullptr;
@@ -921,7 +958,9 @@ static ncg_status ens_setup(ncg_ctx *ctx, const EnsPlan &P, EnsDev &D, bool orde
         D.ju = dbuf<float4>(ctx, "e_ju", ns);
         D.thdr = dbuf<TileHdr>(ctx, "e_thdr", nt);
         D.ghdr = dbuf<TileHdr>(ctx, "e_ghdr", 2 * nt);
-        if (!D.jd || !D.jsig || !D.ju || !D.thdr || !D.ghdr) return NCG_ERR_NOMEM;
+        D.nw = std::max(1, (int)cdiv(P.maxt, WV));
+        D.mask = dbuf<unsigned long long>(ctx, "e_mask", 2 * (size_t)nt * D.nw);
+        if (!D.jd || !D.jsig || !D.ju || !D.thdr || !D.ghdr || !D.mask) return NCG_ERR_NOMEM;
     }
     if (coll) {
         D.jc = dbuf<DBody>(ctx, "e_jc", ns);
@@ -972,15 +1011,18 @@ static ncg_status ens_eval(ncg_ctx *ctx, const EnsPlan &P, const EnsDev &D, cons
         hipLaunchKernelGGL(k_ens_tiles, dim3(cdiv(P.ntiles, ENT / WV)), dim3(ENT), 0, st, b, D.pidx,
                            D.tsys, D.sys, P.ntiles, G, soft, D.jd, D.jsig, D.ju, D.thdr, D.ghdr);
         NCG_CHECK_LAUNCH();
+        hipLaunchKernelGGL(k_ens_mask, dim3(nunits), dim3(ENT), 0, st, D.grav, D.sys, D.srmax,
+                           D.thdr, D.ghdr, soft, P.theta, D.nw, D.mask);
+        NCG_CHECK_LAUNCH();
         ProfScope ps(ctx, st, "ens_mixed");
         if (soft != 0.0)
             hipLaunchKernelGGL(k_ens_tpass<true>, dim3(nunits), dim3(ENT), 0, st, D.grav, D.sys,
                                D.srmax, D.jd, D.jsig, D.ju, D.thdr, D.ghdr, D.pidx, cen, G, soft,
-                               P.theta, acc, cpart);
+                               D.mask, D.nw, acc, cpart);
         else
             hipLaunchKernelGGL(k_ens_tpass<false>, dim3(nunits), dim3(ENT), 0, st, D.grav, D.sys,
                                D.srmax, D.jd, D.jsig, D.ju, D.thdr, D.ghdr, D.pidx, cen, G, soft,
-                               P.theta, acc, cpart);
+                               D.mask, D.nw, acc, cpart);
         NCG_CHECK_LAUNCH();
     } else {
         ProfScope ps(ctx, st, "ens_fast64");
