@@ -253,8 +253,8 @@ __global__ __launch_bounds__(WV) void k_near_mask(const TileHdr *__restrict__ th
                                                   unsigned long long *__restrict__ mask)
 {
     const int nv = sc->nvalid;
-    const int g = blockIdx.y;
-    const int t = blockIdx.x * WV + (int)threadIdx.x;
+    const int g = blockIdx.x;  // groups on x: more than 65,535 above 4M bodies
+    const int t = blockIdx.y * WV + (int)threadIdx.x;
     bool near = false;
     if (g * gw < nv && t * TJ < nv) {
         const TileHdr ih = ihdr[g];
@@ -264,7 +264,7 @@ __global__ __launch_bounds__(WV) void k_near_mask(const TileHdr *__restrict__ th
         near = tile_near(ih, hi, thdr[t], theta, eps2x);
     }
     const unsigned long long bits = __ballot(near);
-    if (threadIdx.x == 0) mask[(size_t)g * nw + blockIdx.x] = bits;
+    if (threadIdx.x == 0) mask[(size_t)g * nw + blockIdx.y] = bits;
 }
 
 // ---- 6. pair kernel -------------------------------------------------------
@@ -792,7 +792,7 @@ static ncg_status tiled_eval_w(ncg_ctx *ctx, const TiledShape &t, const TiledWor
     const int nw = mask_words(t);
     const unsigned long long *mask = w.mask;
     if (prec != NCG_F64_FAST) {
-        hipLaunchKernelGGL(k_near_mask, dim3(nw, t.ngroups), dim3(WV), 0, st, th, ih, sc,
+        hipLaunchKernelGGL(k_near_mask, dim3(t.ngroups, nw), dim3(WV), 0, st, th, ih, sc,
                            WV * t.ipl, nw, theta, soft, w.mask);
         NCG_CHECK_LAUNCH();
     }
