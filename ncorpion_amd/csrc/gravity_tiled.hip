@@ -417,6 +417,8 @@ __global__ __launch_bounds__(NT) void k_pairs(const double4 *__restrict__ jd,
 // as k_finish adds it.
 constexpr int SMALL_N = 4096;
 
+// SOFT false: no softening, the near terms skip s_k + s_j (same bits).
+template <bool SOFT>
 __global__ __launch_bounds__(NT) void k_small(const DBody *__restrict__ b, int n,
                                               const int32_t *__restrict__ ex, int cp, double cx,
                                               double cy, double cz, double mu,
@@ -455,12 +457,14 @@ __global__ __launch_bounds__(NT) void k_small(const DBody *__restrict__ b, int n
             // (near_term_self).
 #pragma unroll
             for (int u = 0; u < NT / 64; ++u)
-                near_term_self<1>(sp[lane + 64 * u], ss[lane + 64 * u], base + lane + 64 * u, k, 0,
-                                  &X, &Y, &Z, &Sk, &ax, &ay, &az);
+                near_term_self<1, SOFT>(sp[lane + 64 * u], ss[lane + 64 * u],
+                                        base + lane + 64 * u, k, 0, &X, &Y, &Z, &Sk, &ax, &ay,
+                                        &az);
         } else {
 #pragma unroll
             for (int u = 0; u < NT / 64; ++u)
-                near_term<1>(sp[lane + 64 * u], ss[lane + 64 * u], &X, &Y, &Z, &Sk, &ax, &ay, &az);
+                near_term<1, SOFT>(sp[lane + 64 * u], ss[lane + 64 * u], &X, &Y, &Z, &Sk, &ax, &ay,
+                                   &az);
         }
     }
     for (int off = 32; off > 0; off >>= 1) {
@@ -754,9 +758,15 @@ ncg_status ncg_small_launch(ncg_ctx *ctx, const DBody *d_b, int32_t n, const int
 {
     NCG_REQUIRE(n >= 1 && n <= SMALL_N, "the direct form takes 1..%d bodies", SMALL_N);
     const int cp = (c && c->present) ? 1 : 0;
-    hipLaunchKernelGGL(k_small, dim3(std::max(1, (int)cdiv(n, NT / 64))), dim3(NT), 0, st, d_b, n,
-                       d_ex, cp, cp ? c->x : 0.0, cp ? c->y : 0.0, cp ? c->z : 0.0,
-                       cp ? G * c->mass : 0.0, d_central, G, soft, d_acc, d_cterm);
+    const dim3 grid(std::max(1, (int)cdiv(n, NT / 64)));
+    if (soft != 0.0)
+        hipLaunchKernelGGL(k_small<true>, grid, dim3(NT), 0, st, d_b, n, d_ex, cp,
+                           cp ? c->x : 0.0, cp ? c->y : 0.0, cp ? c->z : 0.0,
+                           cp ? G * c->mass : 0.0, d_central, G, soft, d_acc, d_cterm);
+    else
+        hipLaunchKernelGGL(k_small<false>, grid, dim3(NT), 0, st, d_b, n, d_ex, cp,
+                           cp ? c->x : 0.0, cp ? c->y : 0.0, cp ? c->z : 0.0,
+                           cp ? G * c->mass : 0.0, d_central, G, soft, d_acc, d_cterm);
     NCG_CHECK_LAUNCH();
     if (d_cm_acc) {
         const int nb = ncg_small_partials(n);
