@@ -162,6 +162,15 @@ CONFIGS = {
         "pert_mass": "0.01", "pert_sma": "5."})),
     "run_c1_disk": ("run", _run_cfg("brute", 2000, 40, DT1, {
         "inner_fluid_disk_bool": "1", "inner_mass": "0.001"})),
+    # J2 with the fragmentation model on the mesh path: the survivors and
+    # fragments of full fragmentations (explicit term lists) start from the
+    # glue's values too.
+    "run_mesh_j2frag": ("run", _run_cfg("mesh", 16384, 10, DT2, {
+        "collision_cube_cells": "256", "collision_cube_min": "2.56",
+        "how_many_neighbours": "100.0", "switch_to_brute_force": "110",
+        "sma_min": "0.9", "sma_max": "1.1", "inclination_max": "0.01",
+        "J2_bool": "1", "J2_value": "0.001", "inelastic_collision_bool": "0",
+        "fragmentation_bool": "1", "N_max": "40000"})),
     "run_mesh_j2": ("run", _run_cfg("mesh", 16384, 10, DT2, {
         "collision_cube_cells": "256", "collision_cube_min": "2.56",
         "how_many_neighbours": "100.0", "switch_to_brute_force": "110",

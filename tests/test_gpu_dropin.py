@@ -70,6 +70,17 @@ def test_central_terms_runs_identical(tmp_path, cfg):
     assert sum(1 for l in tr if l.startswith("C ")) > 100
 
 
+def test_mesh_run_with_j2_and_fragmentation_identical(tmp_path):
+    # J2 with the fragmentation model on the mesh path: fragments appended
+    # with their pairs, the survivors' and fragments' explicit term lists
+    # starting from the glue's start values.
+    b = ic.disk(16384, seed=2, mass=1e-8, radius=2e-3)
+    tr = compare("run_mesh_j2frag", b, tmp_path, 10)
+    coll = [l.split() for l in tr if l.startswith("C ")]
+    assert len(coll) > 100
+    assert any(int(c[2]) >= 16384 or int(c[3]) >= 16384 for c in coll)  # fragments collide
+
+
 def test_mesh_run_with_j2_and_perturber_identical(tmp_path):
     # The mesh module with the J2 bulge and the perturber: the library adds
     # bodies 0..2 and the pass's pairs to the reference's start values.
