@@ -15,8 +15,11 @@ The fastest admissible shape per (form, N) is written to the tuning table
 that libncg.so reads at context creation (ncg_tuning.txt beside it); rows of
 forms that were not tuned are kept.
 
-    python -m ncorpion_amd.autotune --n 4096 16384 65536 262144 \
+    python -m ncorpion_amd.autotune --n 8192 16384 65536 262144 \
         --out ncorpion_amd/ncg_tuning.txt
+
+Sizes up to 4096 are skipped (the direct form serves them); the first tuned
+size's row applies from n_min 1.
     python -m ncorpion_amd.autotune --forms ensemble
 """
 import argparse
@@ -27,6 +30,9 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_TABLE = os.path.join(HERE, "ncg_tuning.txt")
+# Up to this many bodies the fast forms run the direct kernel (k_small), which
+# has no tiled launch shape: every candidate would time the same.
+DIRECT_MAX_N = 4096
 
 
 def _workload(n, seed=0):
@@ -188,7 +194,7 @@ def write_table(rows, path):
 
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
-    ap.add_argument("--n", type=int, nargs="+", default=[4096, 16384, 65536, 262144])
+    ap.add_argument("--n", type=int, nargs="+", default=[8192, 16384, 65536, 262144])
     ap.add_argument("--out", default=DEFAULT_TABLE)
     ap.add_argument("--max-err", type=float, default=5e-7)
     ap.add_argument("--json", default=None)
@@ -202,7 +208,12 @@ def main(argv=None):
     rows, res = [], {}
     if "single" in args.forms:
         tuned |= {"mixed", "fast64"}
-        ns = sorted(args.n)
+        skipped = sorted(n for n in args.n if n <= DIRECT_MAX_N)
+        if skipped:
+            log("skipping n = %s: the direct form serves n <= %d" % (skipped, DIRECT_MAX_N))
+        ns = sorted(n for n in args.n if n > DIRECT_MAX_N)
+        if not ns:
+            ap.error("no size above %d to tune" % DIRECT_MAX_N)
         for i, n in enumerate(ns):
             r = tune(ctx, n, max_err=args.max_err, log=log)
             res[n] = r
