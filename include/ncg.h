@@ -309,10 +309,12 @@ typedef struct {
  * bounce (ncg_mesh_pass with the built-in ops): the results are the same,
  * the pass slower; params->host_passes counts them.  No body may cross the
  * disruption or high-dumping thresholds (end_of_timestep's removals are not
- * applied: NCG_ERR_INVALID).  One synchronisation per mesh pass (its
- * counters set the next mesh size), two with NCG_F64_EXACT (the binned count
- * sizes the field's neighbourhood lists, built on a second stream during the
- * pass).  d_exists may be written (a body moved into an empty slot 0..2).
+ * applied: NCG_ERR_INVALID, reported with the next pass's counters or at the
+ * end of the call; ncg_mesh_leapfrog_model_dev applies them).  One
+ * synchronisation per mesh pass (its counters set the next mesh size; the
+ * census of the bodies, whose count cannot change here, is read with them),
+ * two with NCG_F64_EXACT (the binned count sizes the field's neighbourhood
+ * lists, built on a second stream during the pass).  d_exists may be written (a body moved into an empty slot 0..2).
  * *n_collisions (may be NULL): collisions resolved. */
 ncg_status ncg_mesh_leapfrog_dev(ncg_ctx *ctx, ncg_body *d_bodies, int32_t n, int32_t *d_exists,
                                  int32_t *d_did_collide, double *d_central, double grav_const,

@@ -504,6 +504,31 @@ ncg_status mesh_loop_run(ncg_ctx *ctx, ncg_body *d_bodies, int32_t n_in, int32_t
         set_n(largest_id + 1);
         return NCG_OK;
     };
+    // Without an outcome model no body is removed or created, so the count
+    // of a step's census is the first one's: the census only checks the
+    // removal distances, and its flags are read at the next pass's
+    // synchronisation instead of one of their own.
+    bool census_pending = false;
+    auto census_deferred = [&]() -> ncg_status {
+        hipLaunchKernelGGL(k_lf_census, dim3(std::min<long>(1024, cdiv(n, LT))), dim3(LT), 0, st, b,
+                           (int)n, d_exists, d_central, r_dis, r_hd, count, err);
+        NCG_CHECK_LAUNCH();
+        NCG_HIP(hipMemcpyAsync(hpin + 2, err, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+        census_pending = true;
+        return NCG_OK;
+    };
+    auto census_read = [&]() -> ncg_status {  // after a synchronisation of st
+        if (!census_pending) return NCG_OK;
+        census_pending = false;
+        if (hpin[2] & 1u)
+            return ncg_fail(ctx, NCG_ERR_INVALID, "fewer than three bodies: three_largest_moonlets "
+                                                  "is undefined");
+        if (hpin[2] & 2u)
+            return ncg_fail(ctx, NCG_ERR_INVALID,
+                            "a body crossed disruption_threshold or high_dumping_threshold: "
+                            "end_of_timestep would remove it, which this loop does not do");
+        return NCG_OK;
+    };
     auto pass = [&](int mode, double h) -> ncg_status {
         largest3();
         NCG_CHECK_LAUNCH();
@@ -520,6 +545,7 @@ ncg_status mesh_loop_run(ncg_ctx *ctx, ncg_body *d_bodies, int32_t n_in, int32_t
                                                mp->mesh_size, cube, mp->cells, mode, f,
                                                one_collision_only, &ms, &pflags, st);
         if (r != NCG_OK) return r;
+        if ((r = census_read()) != NCG_OK) return r;  // the pass synchronised st
         mp->conflict_passes += (int32_t)ms.replay_conflicts;
         if (pflags != 0u || (model_host && mode == 0 && ms.collisions > 0)) {
             if (mode != 0)
@@ -588,8 +614,12 @@ ncg_status mesh_loop_run(ncg_ctx *ctx, ncg_body *d_bodies, int32_t n_in, int32_t
         if (coll && d_did_collide && (one_collision_only || frag))
             hipLaunchKernelGGL(k_lf_zero_i32, dim3(cdiv(n, 256)), dim3(256), 0, st, d_did_collide,
                                (int)n);
-        if ((r = census(how_many, &crossed)) != NCG_OK) return r;
-        if ((r = step_end(how_many, crossed)) != NCG_OK) return r;
+        if (lp) {
+            if ((r = census(how_many, &crossed)) != NCG_OK) return r;
+            if ((r = step_end(how_many, crossed)) != NCG_OK) return r;
+        } else if ((r = census_deferred()) != NCG_OK) {
+            return r;
+        }
         update_gam(how_many);
     }
     if (flags & 4) {
@@ -599,6 +629,7 @@ ncg_status mesh_loop_run(ncg_ctx *ctx, ncg_body *d_bodies, int32_t n_in, int32_t
         if ((r = kick(-0.5 * dt)) != NCG_OK) return r;
     }
     NCG_HIP(hipStreamSynchronize(st));
+    if ((r = census_read()) != NCG_OK) return r;
     if (n_collisions) *n_collisions = ncoll;
     if (lp) {
         lp->largest_id = largest_id;

@@ -585,3 +585,34 @@ def test_leapfrog_model_tiled_field_one_call_equals_steps(ctx, tmp_path, prec):
     assert np.array_equal(e1, e2)
     assert np.array_equal(b1[e1 != 0].view(np.uint64), b2[e2 != 0].view(np.uint64))
     assert np.array_equal(c1.view(np.uint64), c2.view(np.uint64))
+
+
+@pytest.mark.gpu
+def test_mesh_loop_refuses_a_threshold_crossing(ctx):
+    """Without an outcome model the mesh loop applies no removal: a body that
+    crosses high_dumping_threshold during the run makes the call fail (the
+    check is read with the next pass's counters or at the end), while the
+    same run without the crossing succeeds."""
+    torch = tt()
+    from ncorpion_amd import NcgError
+    b = ic.disk(4000, seed=3, mass=1e-10, radius=1e-4)
+    c8 = np.array([0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 1.0, 1.0])
+    for crossing in (False, True):
+        bb = b.copy()
+        if crossing:
+            bb[10, 0:3] = [1.199, 0.0, 0.0]
+            bb[10, 3:6] = [2.0, 0.0, 0.0]
+        bt = torch.from_numpy(bb).cuda()
+        ct = torch.from_numpy(c8).cuda()
+        et = torch.ones(bb.shape[0], dtype=torch.int32, device="cuda")
+        dct = torch.zeros(bb.shape[0], dtype=torch.int32, device="cuda")
+        params = dict(gam=0.05, gam_min=0.01, how_many_neighbours=30.0,
+                      disruption_threshold=0.05, high_dumping_threshold=1.2, cells=64)
+        run = lambda: ctx.mesh_leapfrog_dev(bt, ct, 3, 0.01, params, 1.0, 0.0, "fast64", 1.5, 0,  # noqa: E731
+                                            exists_t=et, did_collide_t=dct)
+        if crossing:
+            with pytest.raises(NcgError, match="high_dumping_threshold"):
+                run()
+        else:
+            run()
+        ctx.synchronize()
